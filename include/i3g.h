@@ -1,0 +1,192 @@
+/*
+ * i3g — MI355X-native hot path for I3ELVIS-style staggered-grid
+ * marker-in-cell geodynamics.  C ABI: plain pointers and sizes only.
+ *
+ * Conventions (shared with the reference, head3mg.c):
+ *   node index n = m3*nx*ny + m1*ny + m2      (y fastest, then x, then z)
+ *   y is vertical (depth, positive down); gravity g = (GX, GY, GZ).
+ *   unknown index within a level u = n*4 + c, c = 0:P 1:vx 2:vy 3:vz.
+ *   level l occupies nodes [mgp[l], mgp[l]+mgn[l]) of every per-node array.
+ *   Staggered positions:
+ *     vx(m1,m2,m3) at (x[m1],    ymid[m2], zmid[m3])   m2<ny-1, m3<nz-1
+ *     vy(m1,m2,m3) at (xmid[m1], y[m2],    zmid[m3])   m1<nx-1, m3<nz-1
+ *     vz(m1,m2,m3) at (xmid[m1], ymid[m2], z[m3])      m1<nx-1, m2<ny-1
+ *     P (m1,m2,m3) centre of cell [m1-1,m1]x[m2-1,m2]x[m3-1,m3], m1,m2,m3>=1
+ *
+ * Every function returns I3G_OK (0) or a negative I3G_E* code and never
+ * exits the process.  Host pointers are borrowed for the duration of a call.
+ * One context per host thread; a context owns one HIP stream.
+ */
+#ifndef I3G_H
+#define I3G_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define I3G_OK 0
+#define I3G_EINVAL -1     /* bad argument / shape */
+#define I3G_ENOMEM -2     /* device or host allocation failed */
+#define I3G_EDEVICE -3    /* HIP runtime error */
+#define I3G_ENOCONV -4    /* solver did not reach the tolerance */
+#define I3G_EBC -5        /* unsupported boundary-condition table */
+#define I3G_ECAP -6       /* size above a documented capacity */
+
+#define I3G_MAXLEV 10
+
+/* ------------------------------------------------------------------ */
+/* Host-side setup (no device needed)                                  */
+/* ------------------------------------------------------------------ */
+
+/* Multigrid hierarchy; replaces the reference's gridcheck()
+ * (load3mg.c:1162-1283): mgx[l] = (mgx[l-1]-5)/2+5, coarse lines keep the
+ * three outer lines of each side at full resolution. */
+typedef struct {
+	int nlev;                      /* multinum + 1 */
+	long mgx[I3G_MAXLEV], mgy[I3G_MAXLEV], mgz[I3G_MAXLEV];
+	long mgp[I3G_MAXLEV], mgn[I3G_MAXLEV], mgs[I3G_MAXLEV];
+	long ntot;                     /* sum of mgn */
+} i3g_hier;
+
+int i3g_hierarchy(long nx, long ny, long nz, int multinum, i3g_hier *h);
+/* coarse grid lines of level l (mggx/mggy/mggz in the reference) */
+int i3g_level_lines(const i3g_hier *h, int l, const double *gx, const double *gy,
+		    const double *gz, double *lx, double *ly, double *lz);
+
+/* One boundary rule of the reference's input deck (in3mg.c:236-407):
+ * unknown `var` (0:P 1:vx 2:vy 3:vz 4:T) over the inclusive box
+ * [m10,m11]x[m20,m21]x[m30,m31] of level-0 indices obeys
+ *   value = cnst + koef * value(shifted by dm1,dm2,dm3)      (koef used
+ *   only when a shift is nonzero).  Box ends may be given relative to the
+ * last line: set the *_rel flag and the index is (n-1)+offset. */
+typedef struct {
+	int var;
+	long m10, m11, m20, m21, m30, m31;
+	int rel10, rel11, rel20, rel21, rel30, rel31;
+	double cnst, koef;
+	int dm1, dm2, dm3;
+} i3g_bc_rule;
+
+/* Free-slip box with fixed top/bottom temperature: the rule set of the
+ * reference deck (init.t3c:115-168) with all symmetry coefficients 1.0. */
+int i3g_bc_rules_freeslip(double t_top, double t_bottom, i3g_bc_rule *rules,
+			  int cap);
+
+/* Expands rules over every level exactly as in3mg.c:315-407.
+ * bondm: length 10*nodenum0 (reference layout: 4 per node over all levels,
+ * temperature at 8*nodenum0 + node); bondv: [cap][2]; bondn: [cap].
+ * Returns the number of records (bondnum) or a negative error. */
+long i3g_bc_expand(const i3g_bc_rule *rules, int nrules, const i3g_hier *h,
+		   int *bondm, float *bondv, int *bondn, long cap);
+
+/* Seeded synthetic configurations (the benchmark configurations).
+ *   which 0: 100 km cube, sphere r=20 km (eta 1e23 / rho 3200) in
+ *            eta 1e21 / rho 3300, T linear 273 K -> 1600 K
+ *   which 1: 500 km cube, log10 eta = 21 + sum of 32 seeded Gaussian blobs
+ *            (sigma 20-60 km, amplitude +-2), rho = 3300 - 50*blob
+ *   which 2: 1000 x 500(depth) x 1000 km subduction: lithosphere,
+ *            asthenosphere, 45-degree slab, weak interface channel
+ * Fills node fields (nu, ro, tk) and stress-point viscosities (nxx at
+ * cell centres indexed by the max-corner node, nxy/nxz/nyz at edges) by
+ * evaluating the analytic fields at each position. Arrays are level-0 size.
+ * params out: [0]=nubeg [1]=nuend. */
+int i3g_gen_config(int which, long nx, long ny, long nz, unsigned long long seed,
+		   double *gx, double *gy, double *gz, float *nu, float *ro,
+		   float *tk, float *nxx, float *nxy, float *nxz, float *nyz,
+		   double *params);
+
+/* ------------------------------------------------------------------ */
+/* Device context and the Stokes solve                                 */
+/* ------------------------------------------------------------------ */
+
+typedef struct i3g_ctx i3g_ctx;
+
+typedef struct {
+	long nx, ny, nz;               /* xnumx, ynumy, znumz */
+	const double *gx, *gy, *gz;    /* level-0 grid lines */
+	int multinum;                  /* coarse levels (reference multinum) */
+	const int *bondm;              /* reference layout, 10*nodenum0 */
+	const float *bondv;            /* [nrec][2] */
+	const int *bondn;              /* [nrec] partner+1 (global sol0 index) */
+	long nrec;                     /* bondnum + 1 */
+} i3g_setup;
+
+/* Creates a context on the current HIP device; replaces the static state
+ * of head3mg.c:33-287 for the Stokes/heat/marker path. */
+int i3g_create(const i3g_setup *s, i3g_ctx **out);
+int i3g_destroy(i3g_ctx *ctx);
+
+/* Viscosity/density inputs of the Stokes operator (the arrays vpiterate
+ * reads, move3mg.c:1585). nu/ro at level-0 nodes; nxx at cell centres;
+ * nxy/nxz/nyz at edges (index of the edge's own node, as in sxycalc etc.).
+ * phys: [0]=GXKOEF [1]=GYKOEF [2]=GZKOEF [3]=nubeg [4]=nuend [5]=viscmod.
+ * Coarse-level viscosities are restricted as ronurestrict (move3mg.c:2642). */
+int i3g_set_coefficients(i3g_ctx *ctx, const float *nu, const float *nxx,
+			 const float *nxy, const float *nxz, const float *nyz,
+			 const float *ro, const double *phys);
+
+/* FP64 residual r = b - A x of the level-0 system, matrix-free, for
+ * x = (P, vx, vy, vz) (host arrays, nodenum each).  Replaces evaluating the
+ * assembled rows of x/y/zstokserr, continerr and bonderr
+ * (move3mg.c:718-1579).  r may be NULL.  norms (may be NULL) receives
+ * [0] ||r||_w over non-BC rows, [1] ||b||_w over the same rows, where
+ * continuity rows are weighted by eta_cell / mean cell size. */
+int i3g_stokes_residual(i3g_ctx *ctx, const double *P, const double *vx,
+			const double *vy, const double *vz, double *rP,
+			double *rvx, double *rvy, double *rvz, double *norms);
+
+typedef struct {
+	double rtol;          /* relative weighted residual target (1e-8) */
+	int precision;        /* 64: plain FP64 multigrid; 32: FP32 V-cycles
+				 inside FP64 defect correction */
+	int max_outer;        /* outer iteration cap */
+	int krylov;           /* Krylov restart length (0: stationary cycles) */
+	int nu_pre[I3G_MAXLEV], nu_post[I3G_MAXLEV]; /* sweeps per level */
+	double p0koef, p1koef, v0koef, v1koef;       /* mode.t3c:3032-3036 */
+	double inner_rtol;    /* mixed precision: inner reduction per outer */
+	int warm_start;       /* 0: start from given fields, 1: zero */
+	long pxinit, pyinit, pzinit; double pinit;   /* pressure gauge */
+	int verbose;
+} i3g_stokes_params;
+
+typedef struct {
+	int converged;
+	int outer_iters;       /* outer (FP64) iterations */
+	int vcycles;           /* V-cycles applied */
+	double rel_res;        /* final weighted relative residual */
+	double res0;           /* initial relative residual */
+	double seconds;        /* device time of the solve */
+	double hist[256];      /* relative residual after each outer iteration */
+	int nhist;
+} i3g_stats;
+
+void i3g_stokes_default_params(i3g_stokes_params *p);
+
+/* Solve to p->rtol from (vx,vy,vz,pr) (used as the initial guess unless
+ * warm_start==1) and write the converged fields back; replaces vpiterate
+ * (move3mg.c:1585-2636).  Fields are FP64 on the host in this entry point.
+ * Pressure is shifted so that P(pxinit,pyinit,pzinit) = pinit as
+ * move3mg.c:2248-2269 does. */
+int i3g_stokes_solve(i3g_ctx *ctx, double *P, double *vx, double *vy,
+		     double *vz, const i3g_stokes_params *p, i3g_stats *st);
+
+/* Device-resident variant used for timing: the solution stays in the
+ * context (initial guess: last solution or zero). */
+int i3g_stokes_solve_resident(i3g_ctx *ctx, const i3g_stokes_params *p,
+			      i3g_stats *st);
+int i3g_stokes_get_solution(i3g_ctx *ctx, double *P, double *vx, double *vy,
+			    double *vz);
+int i3g_stokes_set_solution(i3g_ctx *ctx, const double *P, const double *vx,
+			    const double *vy, const double *vz);
+
+/* Per-kernel timing of the last solve (HIP events): names and average
+ * milliseconds; returns the count. */
+int i3g_kernel_stats(i3g_ctx *ctx, char *names, int name_len, double *avg_ms,
+		     long *calls, double *bytes, int cap);
+
+const char *i3g_error_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,554 @@
+/*
+ * i3g oracle — TEST INFRASTRUCTURE ONLY (see i3g_oracle.h).
+ * Plain-C FP64 restatement of the reference Stokes operator, transfers and
+ * norms.  Each function cites the reference lines it follows.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "i3g_oracle.h"
+
+#define NODE(L, m1, m2, m3) ((m3) * (L)->nx * (L)->ny + (m1) * (L)->ny + (m2))
+
+static double clampd(double v, double lo, double hi)
+{
+	if (v > hi) v = hi;
+	if (v < lo) v = lo;
+	return v;
+}
+
+static double avg8(const float *nu, const long *v, int viscmod)
+{
+	int k;
+	double s = 0;
+	if (viscmod == 1) {
+		for (k = 0; k < 8; k++) s += log(nu[v[k]]);
+		return exp(s / 8.0);
+	}
+	if (viscmod == 2) {
+		for (k = 0; k < 8; k++) s += 1.0 / nu[v[k]];
+		return 1.0 / (s / 8.0);
+	}
+	{
+		/* the reference adds the eight float values in float */
+		float fs = nu[v[0]];
+		for (k = 1; k < 8; k++) fs = fs + nu[v[k]];
+		s = fs;
+	}
+	return s / 8.0;
+}
+
+static double avg2(float a, float b, int viscmod)
+{
+	if (viscmod == 1) return exp((log(a) + log(b)) / 2.0);
+	if (viscmod == 2) return 1.0 / ((1.0 / a + 1.0 / b) / 2.0);
+	return (double)(float)(a + b) / 2.0;
+}
+
+/* move3mg.c:36-48 (cells), 402-414 (xy), 517-529 (xz), 632-644 (yz) */
+void or_eta(const orl_t *L, const float *nu, const float *nxx, const float *nxy,
+	    const float *nxz, const float *nyz, int viscmod, double nubeg,
+	    double nuend, double *etaC, double *etaXY, double *etaXZ,
+	    double *etaYZ)
+{
+	long nx = L->nx, ny = L->ny, nz = L->nz, m1, m2, m3;
+	int lev0 = L->level == 0;
+	for (m3 = 0; m3 < nz; m3++)
+	for (m1 = 0; m1 < nx; m1++)
+	for (m2 = 0; m2 < ny; m2++) {
+		long n = NODE(L, m1, m2, m3);
+		double e;
+		etaC[n] = etaXY[n] = etaXZ[n] = etaYZ[n] = 0;
+		if (m1 >= 1 && m2 >= 1 && m3 >= 1) {
+			if (lev0 && m1 > 3 && m2 > 3 && m3 > 3 && m1 < nx - 3 &&
+			    m2 < ny - 3 && m3 < nz - 3) {
+				e = nxx[n];
+			} else {
+				long v[8];
+				v[0] = NODE(L, m1 - 1, m2 - 1, m3 - 1); v[1] = v[0] + 1;
+				v[2] = v[0] + ny; v[3] = v[2] + 1;
+				v[4] = v[0] + nx * ny; v[5] = v[4] + 1;
+				v[6] = v[4] + ny; v[7] = v[6] + 1;
+				e = avg8(nu, v, viscmod);
+			}
+			etaC[n] = clampd(e, nubeg, nuend);
+		}
+		if (m3 < nz - 1) {
+			if (lev0 && m1 > 3 && m2 > 3 && m3 > 2 && m1 < nx - 4 &&
+			    m2 < ny - 4 && m3 < nz - 4)
+				e = nxy[n];
+			else
+				e = avg2(nu[n], nu[n + nx * ny], viscmod);
+			etaXY[n] = clampd(e, nubeg, nuend);
+		}
+		if (m2 < ny - 1) {
+			if (lev0 && m1 > 3 && m2 > 2 && m3 > 3 && m1 < nx - 4 &&
+			    m2 < ny - 4 && m3 < nz - 4)
+				e = nxz[n];
+			else
+				e = avg2(nu[n], nu[n + 1], viscmod);
+			etaXZ[n] = clampd(e, nubeg, nuend);
+		}
+		if (m1 < nx - 1) {
+			if (lev0 && m1 > 2 && m2 > 3 && m3 > 3 && m1 < nx - 4 &&
+			    m2 < ny - 4 && m3 < nz - 4)
+				e = nyz[n];
+			else
+				e = avg2(nu[n], nu[n + ny], viscmod);
+			etaYZ[n] = clampd(e, nubeg, nuend);
+		}
+	}
+}
+
+static int exists(const orl_t *L, long m1, long m2, long m3, int c)
+{
+	switch (c) {
+	case 0: return m1 && m2 && m3;
+	case 1: return m2 < L->ny - 1 && m3 < L->nz - 1;
+	case 2: return m1 < L->nx - 1 && m3 < L->nz - 1;
+	default: return m1 < L->nx - 1 && m2 < L->ny - 1;
+	}
+}
+
+/* move3mg.c:751, :943, :1142 — RHS = -g_i * mean of 4 node densities */
+void or_rhs(const orl_t *L, const float *ro, const double *g, const int *bondm,
+	    const float *bondv, double *b)
+{
+	long nx = L->nx, ny = L->ny, nz = L->nz, m1, m2, m3, sx = ny, sz = nx * ny;
+	int c;
+	for (m3 = 0; m3 < nz; m3++)
+	for (m1 = 0; m1 < nx; m1++)
+	for (m2 = 0; m2 < ny; m2++) {
+		long n = NODE(L, m1, m2, m3);
+		for (c = 0; c < 4; c++) {
+			long u = n * 4 + c;
+			int rec = bondm[(L->mgp + n) * 4 + c];
+			b[u] = 0;
+			if (!exists(L, m1, m2, m3, c)) continue;
+			if (rec) { b[u] = bondv[2 * rec]; continue; }
+			if (c == 1)
+				b[u] = -g[0] * (ro[n] + ro[n + 1] + ro[n + sz] + ro[n + 1 + sz]) / 4.0;
+			else if (c == 2)
+				b[u] = -g[1] * (ro[n] + ro[n + sx] + ro[n + sz] + ro[n + sx + sz]) / 4.0;
+			else if (c == 3)
+				b[u] = -g[2] * (ro[n] + ro[n + 1] + ro[n + sx] + ro[n + sx + 1]) / 4.0;
+		}
+	}
+}
+
+/* Stresses in terms of x (interleaved) — value forms of sxxcalc..syzcalc
+ * (move3mg.c:3-710); eta from or_eta. */
+typedef struct {
+	const orl_t *L;
+	const double *eC, *eXY, *eXZ, *eYZ, *x;
+} ctx_t;
+
+#define XV(C, m1, m2, m3, c) ((C)->x[NODE((C)->L, m1, m2, m3) * 4 + (c)])
+
+static double s_xx(const ctx_t *C, long a, long b, long c)
+{
+	const orl_t *L = C->L;
+	return 2.0 * C->eC[NODE(L, a, b, c)] *
+	       (XV(C, a, b - 1, c - 1, 1) - XV(C, a - 1, b - 1, c - 1, 1)) /
+	       (L->gx[a] - L->gx[a - 1]);
+}
+static double s_yy(const ctx_t *C, long a, long b, long c)
+{
+	const orl_t *L = C->L;
+	return 2.0 * C->eC[NODE(L, a, b, c)] *
+	       (XV(C, a - 1, b, c - 1, 2) - XV(C, a - 1, b - 1, c - 1, 2)) /
+	       (L->gy[b] - L->gy[b - 1]);
+}
+static double s_zz(const ctx_t *C, long a, long b, long c)
+{
+	const orl_t *L = C->L;
+	return 2.0 * C->eC[NODE(L, a, b, c)] *
+	       (XV(C, a - 1, b - 1, c, 3) - XV(C, a - 1, b - 1, c - 1, 3)) /
+	       (L->gz[c] - L->gz[c - 1]);
+}
+static double s_xy(const ctx_t *C, long a, long b, long c)
+{
+	const orl_t *L = C->L;
+	return 2.0 * C->eXY[NODE(L, a, b, c)] *
+	       ((XV(C, a, b, c, 1) - XV(C, a, b - 1, c, 1)) / (L->gy[b + 1] - L->gy[b - 1]) +
+		(XV(C, a, b, c, 2) - XV(C, a - 1, b, c, 2)) / (L->gx[a + 1] - L->gx[a - 1]));
+}
+static double s_xz(const ctx_t *C, long a, long b, long c)
+{
+	const orl_t *L = C->L;
+	return 2.0 * C->eXZ[NODE(L, a, b, c)] *
+	       ((XV(C, a, b, c, 1) - XV(C, a, b, c - 1, 1)) / (L->gz[c + 1] - L->gz[c - 1]) +
+		(XV(C, a, b, c, 3) - XV(C, a - 1, b, c, 3)) / (L->gx[a + 1] - L->gx[a - 1]));
+}
+static double s_yz(const ctx_t *C, long a, long b, long c)
+{
+	const orl_t *L = C->L;
+	return 2.0 * C->eYZ[NODE(L, a, b, c)] *
+	       ((XV(C, a, b, c, 2) - XV(C, a, b, c - 1, 2)) / (L->gz[c + 1] - L->gz[c - 1]) +
+		(XV(C, a, b, c, 3) - XV(C, a, b - 1, c, 3)) / (L->gy[b + 1] - L->gy[b - 1]));
+}
+
+/* A x for a stencil row (x/y/zstokserr move3mg.c:718-1301, continerr
+ * :1310-1401) and the row's diagonal coefficient. */
+static double row_ax(const ctx_t *C, long i, long j, long k, int c, double *diag)
+{
+	const orl_t *L = C->L;
+	const double *gx = L->gx, *gy = L->gy, *gz = L->gz;
+	double ax = 0, xkf, ykf, zkf, d = 0;
+	long nyx = L->ny, nxy = L->nx * L->ny;
+	if (c == 1) {
+		xkf = 2.0 / (gx[i + 1] - gx[i - 1]);
+		ykf = 1.0 / (gy[j + 1] - gy[j]);
+		zkf = 1.0 / (gz[k + 1] - gz[k]);
+		ax = xkf * (s_xx(C, i + 1, j + 1, k + 1) - s_xx(C, i, j + 1, k + 1)) +
+		     ykf * (s_xy(C, i, j + 1, k) - s_xy(C, i, j, k)) +
+		     zkf * (s_xz(C, i, j, k + 1) - s_xz(C, i, j, k)) -
+		     xkf * (XV(C, i + 1, j + 1, k + 1, 0) - XV(C, i, j + 1, k + 1, 0));
+		d = -xkf * (2.0 * C->eC[NODE(L, i + 1, j + 1, k + 1)] / (gx[i + 1] - gx[i]) +
+			    2.0 * C->eC[NODE(L, i, j + 1, k + 1)] / (gx[i] - gx[i - 1])) -
+		    ykf * (2.0 * C->eXY[NODE(L, i, j + 1, k)] / (gy[j + 2] - gy[j]) +
+			   2.0 * C->eXY[NODE(L, i, j, k)] / (gy[j + 1] - gy[j - 1])) -
+		    zkf * (2.0 * C->eXZ[NODE(L, i, j, k + 1)] / (gz[k + 2] - gz[k]) +
+			   2.0 * C->eXZ[NODE(L, i, j, k)] / (gz[k + 1] - gz[k - 1]));
+	} else if (c == 2) {
+		xkf = 1.0 / (gx[i + 1] - gx[i]);
+		ykf = 2.0 / (gy[j + 1] - gy[j - 1]);
+		zkf = 1.0 / (gz[k + 1] - gz[k]);
+		ax = ykf * (s_yy(C, i + 1, j + 1, k + 1) - s_yy(C, i + 1, j, k + 1)) +
+		     xkf * (s_xy(C, i + 1, j, k) - s_xy(C, i, j, k)) +
+		     zkf * (s_yz(C, i, j, k + 1) - s_yz(C, i, j, k)) -
+		     ykf * (XV(C, i + 1, j + 1, k + 1, 0) - XV(C, i + 1, j, k + 1, 0));
+		d = -ykf * (2.0 * C->eC[NODE(L, i + 1, j + 1, k + 1)] / (gy[j + 1] - gy[j]) +
+			    2.0 * C->eC[NODE(L, i + 1, j, k + 1)] / (gy[j] - gy[j - 1])) -
+		    xkf * (2.0 * C->eXY[NODE(L, i + 1, j, k)] / (gx[i + 2] - gx[i]) +
+			   2.0 * C->eXY[NODE(L, i, j, k)] / (gx[i + 1] - gx[i - 1])) -
+		    zkf * (2.0 * C->eYZ[NODE(L, i, j, k + 1)] / (gz[k + 2] - gz[k]) +
+			   2.0 * C->eYZ[NODE(L, i, j, k)] / (gz[k + 1] - gz[k - 1]));
+	} else if (c == 3) {
+		xkf = 1.0 / (gx[i + 1] - gx[i]);
+		ykf = 1.0 / (gy[j + 1] - gy[j]);
+		zkf = 2.0 / (gz[k + 1] - gz[k - 1]);
+		ax = zkf * (s_zz(C, i + 1, j + 1, k + 1) - s_zz(C, i + 1, j + 1, k)) +
+		     xkf * (s_xz(C, i + 1, j, k) - s_xz(C, i, j, k)) +
+		     ykf * (s_yz(C, i, j + 1, k) - s_yz(C, i, j, k)) -
+		     zkf * (XV(C, i + 1, j + 1, k + 1, 0) - XV(C, i + 1, j + 1, k, 0));
+		d = -zkf * (2.0 * C->eC[NODE(L, i + 1, j + 1, k + 1)] / (gz[k + 1] - gz[k]) +
+			    2.0 * C->eC[NODE(L, i + 1, j + 1, k)] / (gz[k] - gz[k - 1])) -
+		    xkf * (2.0 * C->eXZ[NODE(L, i + 1, j, k)] / (gx[i + 2] - gx[i]) +
+			   2.0 * C->eXZ[NODE(L, i, j, k)] / (gx[i + 1] - gx[i - 1])) -
+		    ykf * (2.0 * C->eYZ[NODE(L, i, j + 1, k)] / (gy[j + 2] - gy[j]) +
+			   2.0 * C->eYZ[NODE(L, i, j, k)] / (gy[j + 1] - gy[j - 1]));
+	} else {
+		ax = (XV(C, i, j - 1, k - 1, 1) - XV(C, i - 1, j - 1, k - 1, 1)) / (gx[i] - gx[i - 1]) +
+		     (XV(C, i - 1, j, k - 1, 2) - XV(C, i - 1, j - 1, k - 1, 2)) / (gy[j] - gy[j - 1]) +
+		     (XV(C, i - 1, j - 1, k, 3) - XV(C, i - 1, j - 1, k - 1, 3)) / (gz[k] - gz[k - 1]);
+		d = 0;
+	}
+	(void)nyx; (void)nxy;
+	if (diag) *diag = d;
+	return ax;
+}
+
+/* bonderr (move3mg.c:1532-1579): A x = x_i - k x_partner, b = const. */
+static double bc_ax(const orl_t *L, const int *bondn, const float *bondv, int rec,
+		    long u, const double *x)
+{
+	double ax = x[u];
+	if (bondn[rec]) ax -= bondv[2 * rec + 1] * x[bondn[rec] - 1 - L->mgp * 4];
+	return ax;
+}
+
+void or_residual(const orl_t *L, const double *etaC, const double *etaXY,
+		 const double *etaXZ, const double *etaYZ, const int *bondm,
+		 const float *bondv, const int *bondn, const double *b,
+		 const double *x, double *r)
+{
+	ctx_t C = {L, etaC, etaXY, etaXZ, etaYZ, x};
+	long m1, m2, m3;
+	int c;
+	for (m3 = 0; m3 < L->nz; m3++)
+	for (m1 = 0; m1 < L->nx; m1++)
+	for (m2 = 0; m2 < L->ny; m2++) {
+		long n = NODE(L, m1, m2, m3);
+		for (c = 0; c < 4; c++) {
+			long u = n * 4 + c;
+			int rec = bondm[(L->mgp + n) * 4 + c];
+			r[u] = 0;
+			if (!exists(L, m1, m2, m3, c)) continue;
+			if (rec) r[u] = b[u] - bc_ax(L, bondn, bondv, rec, u, x);
+			else r[u] = b[u] - row_ax(&C, m1, m2, m3, c, NULL);
+		}
+	}
+}
+
+void or_wnorms(const orl_t *L, const double *etaC, const int *bondm,
+	       const double *b, const double *r, double *out)
+{
+	long m1, m2, m3;
+	int c;
+	double sr = 0, sb = 0;
+	for (m3 = 0; m3 < L->nz; m3++)
+	for (m1 = 0; m1 < L->nx; m1++)
+	for (m2 = 0; m2 < L->ny; m2++) {
+		long n = NODE(L, m1, m2, m3);
+		for (c = 0; c < 4; c++) {
+			long u = n * 4 + c;
+			double w = 1.0;
+			if (!exists(L, m1, m2, m3, c)) continue;
+			if (bondm[(L->mgp + n) * 4 + c]) continue;
+			if (c == 0) {
+				double h = ((L->gx[m1] - L->gx[m1 - 1]) + (L->gy[m2] - L->gy[m2 - 1]) +
+					    (L->gz[m3] - L->gz[m3 - 1])) / 3.0;
+				w = etaC[n] / h;
+			}
+			sr += (r[u] * w) * (r[u] * w);
+			sb += (b[u] * w) * (b[u] * w);
+		}
+	}
+	out[0] = sqrt(sr);
+	out[1] = sqrt(sb);
+}
+
+/* move3mg.c:1691-1764: sigma/eps at cells (m1,m2,m3>0) and edges. */
+void or_stress(const orl_t *L, const double *etaC, const double *etaXY,
+	       const double *etaXZ, const double *etaYZ, const float *vx,
+	       const float *vy, const float *vz, float *eps6, float *sig6)
+{
+	long nx = L->nx, ny = L->ny, nz = L->nz, N = nx * ny * nz, m1, m2, m3;
+	const double *gx = L->gx, *gy = L->gy, *gz = L->gz;
+	for (m3 = 0; m3 < nz; m3++)
+	for (m1 = 0; m1 < nx; m1++)
+	for (m2 = 0; m2 < ny; m2++) {
+		long n = NODE(L, m1, m2, m3);
+		double e;
+		if (m1 > 0 && m2 > 0 && m3 > 0) {
+			long v0 = NODE(L, m1 - 1, m2 - 1, m3 - 1);
+			e = (vx[v0 + ny] - vx[v0]) * (1.0 / (gx[m1] - gx[m1 - 1]));
+			eps6[0 * N + n] = (float)e; sig6[0 * N + n] = (float)(2.0 * etaC[n] * e);
+			e = (vy[v0 + 1] - vy[v0]) * (1.0 / (gy[m2] - gy[m2 - 1]));
+			eps6[1 * N + n] = (float)e; sig6[1 * N + n] = (float)(2.0 * etaC[n] * e);
+			e = (vz[v0 + nx * ny] - vz[v0]) * (1.0 / (gz[m3] - gz[m3 - 1]));
+			eps6[2 * N + n] = (float)e; sig6[2 * N + n] = (float)(2.0 * etaC[n] * e);
+		}
+		if (m1 > 0 && m1 < nx - 1 && m2 > 0 && m2 < ny - 1 && m3 < nz - 1) {
+			e = (vx[n] - vx[n - 1]) * (1.0 / (gy[m2 + 1] - gy[m2 - 1])) +
+			    (vy[n] - vy[n - ny]) * (1.0 / (gx[m1 + 1] - gx[m1 - 1]));
+			eps6[3 * N + n] = (float)e; sig6[3 * N + n] = (float)(2.0 * etaXY[n] * e);
+		}
+		if (m1 > 0 && m1 < nx - 1 && m2 < ny - 1 && m3 > 0 && m3 < nz - 1) {
+			e = (vx[n] - vx[n - nx * ny]) * (1.0 / (gz[m3 + 1] - gz[m3 - 1])) +
+			    (vz[n] - vz[n - ny]) * (1.0 / (gx[m1 + 1] - gx[m1 - 1]));
+			eps6[4 * N + n] = (float)e; sig6[4 * N + n] = (float)(2.0 * etaXZ[n] * e);
+		}
+		if (m1 < nx - 1 && m2 > 0 && m2 < ny - 1 && m3 > 0 && m3 < nz - 1) {
+			e = (vy[n] - vy[n - nx * ny]) * (1.0 / (gz[m3 + 1] - gz[m3 - 1])) +
+			    (vz[n] - vz[n - 1]) * (1.0 / (gy[m2 + 1] - gy[m2 - 1]));
+			eps6[5 * N + n] = (float)e; sig6[5 * N + n] = (float)(2.0 * etaYZ[n] * e);
+		}
+	}
+}
+
+/* ------------------------------------------------------------------ */
+/* Transfers                                                          */
+/* ------------------------------------------------------------------ */
+
+/* Base coarse index of a fine index along one axis (the m10min/m20min/
+ * m30min blocks of move3mg.c:2779-2789, 3201-3214): kind 0 = node line
+ * (clamp n-2), 1 = mid-point (clamp n-3, shift down if the fine mid lies
+ * below the coarse mid), 2 = cell centre (clamp n-2, shift if below). */
+static long base_idx(long m, long nf, long nc, const double *gf, const double *gc,
+		     int kind)
+{
+	long r = (m - 2) / 2 + 2;
+	if (m < 3) r = m;
+	if (m > nf - 4) r = nc - (nf - m);
+	if (kind == 0) {
+		if (r > nc - 2) r = nc - 2;
+	} else if (kind == 1) {
+		if (r > nc - 3) r = nc - 3;
+		if ((gf[m] + gf[m + 1]) / 2.0 < (gc[r] + gc[r + 1]) / 2.0 && r > 0) r -= 1;
+	} else {
+		if (r > nc - 2) r = nc - 2;
+		if ((gf[m] + gf[m - 1]) / 2.0 < (gc[r] + gc[r - 1]) / 2.0 && r > 1) r -= 1;
+	}
+	return r;
+}
+
+/* one-dimensional weight for coarse index cc (cc == base or base+1) */
+static double w1d(long m, long base, long cc, const double *gf, const double *gc,
+		  int kind)
+{
+	double d, dd;
+	if (kind == 0) {
+		d = gc[base + 1] - gc[base];
+		dd = (gc[cc] - gf[m]) / d;
+	} else if (kind == 1) {
+		d = (gc[base + 2] - gc[base]) / 2.0;
+		dd = (gc[cc] + gc[cc + 1] - gf[m] - gf[m + 1]) / 2.0 / d;
+	} else {
+		d = (gc[base + 1] - gc[base - 1]) / 2.0;
+		dd = (gc[cc] + gc[cc - 1] - gf[m] - gf[m - 1]) / 2.0 / d;
+	}
+	if (cc == base) dd = -dd;
+	return 1.0 - dd;
+}
+
+/* move3mg.c:2642-2741 */
+void or_ronurestrict(const orl_t *F, const orl_t *C, const float *nu_f,
+		     float *nu_c, int viscmod)
+{
+	long Nc = C->nx * C->ny * C->nz, m1, m2, m3, n;
+	double *sw = calloc((size_t)Nc, sizeof(double));
+	for (n = 0; n < Nc; n++) nu_c[n] = 0;
+	for (m3 = 0; m3 < F->nz; m3++)
+	for (m1 = 0; m1 < F->nx; m1++)
+	for (m2 = 0; m2 < F->ny; m2++) {
+		long m4 = NODE(F, m1, m2, m3), a, b, c;
+		long b1 = base_idx(m1, F->nx, C->nx, F->gx, C->gx, 0);
+		long b2 = base_idx(m2, F->ny, C->ny, F->gy, C->gy, 0);
+		long b3 = base_idx(m3, F->nz, C->nz, F->gz, C->gz, 0);
+		double dx = C->gx[b1 + 1] - C->gx[b1];
+		double dy = C->gy[b2 + 1] - C->gy[b2];
+		double dz = C->gz[b3 + 1] - C->gz[b3];
+		for (c = b3; c <= b3 + 1; c++) {
+			double ddz = 1.0 - fabs((C->gz[c] - F->gz[m3]) / dz);
+			for (a = b1; a <= b1 + 1; a++) {
+				double ddx = 1.0 - fabs((C->gx[a] - F->gx[m1]) / dx);
+				for (b = b2; b <= b2 + 1; b++) {
+					double ddy = 1.0 - fabs((C->gy[b] - F->gy[m2]) / dy);
+					long m40 = NODE(C, a, b, c);
+					double swt = ddx * ddy * ddz, v;
+					if (viscmod == 1) v = log(nu_f[m4]) * swt;
+					else if (viscmod == 2) v = 1.0 / nu_f[m4] * swt;
+					else v = nu_f[m4] * swt;
+					nu_c[m40] = (float)(nu_c[m40] + v);
+					sw[m40] += swt;
+				}
+			}
+		}
+	}
+	for (n = 0; n < Nc; n++) {
+		nu_c[n] = (float)(nu_c[n] / sw[n]);
+		if (viscmod == 1) nu_c[n] = (float)exp(nu_c[n]);
+		if (viscmod == 2) nu_c[n] = (float)(1.0 / nu_c[n]);
+	}
+	free(sw);
+}
+
+/* per-component axis kinds: [comp][axis] 0 node,1 mid,2 centre */
+static const int KIND[4][3] = {{2, 2, 2}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
+
+static void bases(const orl_t *F, const orl_t *C, int comp, long m1, long m2,
+		  long m3, long *b)
+{
+	b[0] = base_idx(m1, F->nx, C->nx, F->gx, C->gx, KIND[comp][0]);
+	b[1] = base_idx(m2, F->ny, C->ny, F->gy, C->gy, KIND[comp][1]);
+	b[2] = base_idx(m3, F->nz, C->nz, F->gz, C->gz, KIND[comp][2]);
+}
+
+/* move3mg.c:3165-3520 */
+void or_vprestrict(const orl_t *F, const orl_t *C, const int *bondm,
+		   const double *r_f, const float *nd_f, const float *nd_c,
+		   double *b_c)
+{
+	long Nc = C->nx * C->ny * C->nz, m1, m2, m3, u;
+	double *sw = calloc((size_t)(4 * Nc), sizeof(double));
+	int comp;
+	for (u = 0; u < 4 * Nc; u++) b_c[u] = 0;
+	for (m3 = 0; m3 < F->nz; m3++)
+	for (m1 = 0; m1 < F->nx; m1++)
+	for (m2 = 0; m2 < F->ny; m2++) {
+		long m4 = NODE(F, m1, m2, m3);
+		for (comp = 1; comp <= 4; comp++) {
+			int cc = comp & 3; /* order vx, vy, vz, P */
+			long b[3], a, bb, c;
+			if (!exists(F, m1, m2, m3, cc)) continue;
+			if (bondm[(F->mgp + m4) * 4 + cc]) continue;
+			bases(F, C, cc, m1, m2, m3, b);
+			for (c = b[2]; c <= b[2] + 1; c++) {
+				double ddz = w1d(m3, b[2], c, F->gz, C->gz, KIND[cc][2]);
+				for (a = b[0]; a <= b[0] + 1; a++) {
+					double ddx = w1d(m1, b[0], a, F->gx, C->gx, KIND[cc][0]);
+					for (bb = b[1]; bb <= b[1] + 1; bb++) {
+						double ddy = w1d(m2, b[1], bb, F->gy, C->gy, KIND[cc][1]);
+						long m40 = NODE(C, a, bb, c);
+						double swt;
+						if (bondm[(C->mgp + m40) * 4 + cc]) continue;
+						swt = ddx * ddy * ddz;
+						if (cc == 0)
+							b_c[m40 * 4] += r_f[m4 * 4] * nd_f[m4] / nd_c[m40] * swt;
+						else
+							b_c[m40 * 4 + cc] += r_f[m4 * 4 + cc] * swt;
+						sw[m40 * 4 + cc] += swt;
+					}
+				}
+			}
+		}
+	}
+	for (u = 0; u < 4 * Nc; u++)
+		if (sw[u] != 0) b_c[u] /= sw[u];
+	free(sw);
+}
+
+/* move3mg.c:2748-3063 */
+void or_vpprolong(const orl_t *F, const orl_t *C, const double *x_c,
+		  double *x_f, double v1koef, double p1koef)
+{
+	long m1, m2, m3;
+	int comp;
+	for (m3 = 0; m3 < F->nz; m3++)
+	for (m1 = 0; m1 < F->nx; m1++)
+	for (m2 = 0; m2 < F->ny; m2++) {
+		long m4 = NODE(F, m1, m2, m3);
+		for (comp = 1; comp <= 4; comp++) {
+			int cc = comp & 3;
+			long b[3], a, bb, c;
+			double kf = cc ? v1koef : p1koef;
+			if (!exists(F, m1, m2, m3, cc)) continue;
+			bases(F, C, cc, m1, m2, m3, b);
+			for (c = b[2]; c <= b[2] + 1; c++) {
+				double ddz = w1d(m3, b[2], c, F->gz, C->gz, KIND[cc][2]);
+				for (a = b[0]; a <= b[0] + 1; a++) {
+					double ddx = w1d(m1, b[0], a, F->gx, C->gx, KIND[cc][0]);
+					for (bb = b[1]; bb <= b[1] + 1; bb++) {
+						double ddy = w1d(m2, b[1], bb, F->gy, C->gy, KIND[cc][1]);
+						double swt = ddx * ddy * ddz;
+						x_f[m4 * 4 + cc] += x_c[NODE(C, a, bb, c) * 4 + cc] * swt * kf;
+					}
+				}
+			}
+		}
+	}
+}
+
+/* gausmat4 update rules (move3mg.c:3087-3131), lexicographic order. */
+void or_gs_sweep(const orl_t *L, const double *etaC, const double *etaXY,
+		 const double *etaXZ, const double *etaYZ, const int *bondm,
+		 const float *bondv, const int *bondn, const double *b, double *x,
+		 double p0koef, double v0koef)
+{
+	ctx_t C = {L, etaC, etaXY, etaXZ, etaYZ, x};
+	long m1, m2, m3;
+	int c;
+	for (m3 = 0; m3 < L->nz; m3++)
+	for (m1 = 0; m1 < L->nx; m1++)
+	for (m2 = 0; m2 < L->ny; m2++) {
+		long n = NODE(L, m1, m2, m3);
+		for (c = 0; c < 4; c++) {
+			long u = n * 4 + c;
+			int rec = bondm[(L->mgp + n) * 4 + c];
+			double ax, d, res;
+			if (!exists(L, m1, m2, m3, c)) continue;
+			if (rec) {
+				res = bc_ax(L, bondn, bondv, rec, u, x) - b[u];
+				x[u] -= res;
+				continue;
+			}
+			ax = row_ax(&C, m1, m2, m3, c, &d);
+			res = ax - b[u];
+			if (c == 0) x[u] -= res * etaC[n] * p0koef;
+			else x[u] -= res / d * v0koef;
+		}
+	}
+}

@@ -1,0 +1,78 @@
+/*
+ * i3g oracle — TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, FP64, single-threaded restatement of the reference algorithm
+ * for the Stokes hot path.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's CPU-baseline leg may load it, and only as the checker.
+ * Vectors use the reference's sol0 layout: unknown (node n, comp c) at
+ * n*4+c with c = 0:P 1:vx 2:vy 3:vz; a level's node n is level-local.
+ * Boundary tables are the reference's global layout (bondm over all levels,
+ * bondn partner+1 as a global sol0 index); `mgp` is the level's node offset.
+ */
+#ifndef I3G_ORACLE_H
+#define I3G_ORACLE_H
+
+typedef struct {
+	long nx, ny, nz;   /* level dims */
+	long mgp;          /* node offset of the level in the global tables */
+	int level;         /* 0 = finest */
+	const double *gx, *gy, *gz; /* level lines */
+} orl_t;
+
+/* Effective viscosity at the stress points (move3mg.c:36-48 sxx/syy/szz
+ * and continerr, :402-414 sxy, :517-529 sxz, :632-644 syz). nxx..nyz are
+ * used only on level 0 (may be NULL otherwise). Output arrays have N
+ * entries; positions where the stress does not exist are set to 0. */
+void or_eta(const orl_t *L, const float *nu, const float *nxx, const float *nxy,
+	    const float *nxz, const float *nyz, int viscmod, double nubeg,
+	    double nuend, double *etaC, double *etaXY, double *etaXZ,
+	    double *etaYZ);
+
+/* Momentum right-hand side on level 0 from node densities (move3mg.c:751
+ * and the y/z analogues); continuity and BC entries get 0/const. Output
+ * 4N interleaved; BC rows get their constant (bonderr ui1[0]). */
+void or_rhs(const orl_t *L, const float *ro, const double *g, const int *bondm,
+	    const float *bondv, double *b);
+
+/* r = b - A x for every existing unknown of the level (rows of
+ * x/y/zstokserr, continerr and bonderr with ynerr=0, move3mg.c:718-1579);
+ * missing unknowns get r = 0. */
+void or_residual(const orl_t *L, const double *etaC, const double *etaXY,
+		 const double *etaXZ, const double *etaYZ, const int *bondm,
+		 const float *bondv, const int *bondn, const double *b,
+		 const double *x, double *r);
+
+/* Weighted norms: out[0] = ||r||, out[1] = ||b|| over non-BC rows with
+ * continuity rows scaled by etaC / mean cell size. */
+void or_wnorms(const orl_t *L, const double *etaC, const int *bondm,
+	       const double *b, const double *r, double *out);
+
+/* Value-mode strain rates and stresses (move3mg.c:1691-1764 loop) from FP32
+ * fields; eps/sig arrays are N each (xx,yy,zz,xy,xz,yz). */
+void or_stress(const orl_t *L, const double *etaC, const double *etaXY,
+	       const double *etaXZ, const double *etaYZ, const float *vx,
+	       const float *vy, const float *vz, float *eps6, float *sig6);
+
+/* Coarse viscosity restriction, scatter form (move3mg.c:2642-2741). */
+void or_ronurestrict(const orl_t *F, const orl_t *C, const float *nu_f,
+		     float *nu_c, int viscmod);
+
+/* Residual restriction, scatter form (move3mg.c:3165-3520).  r_f: fine 4Nf
+ * interleaved; nd_f/nd_c: cell viscosities (float, as stored by continerr);
+ * b_c: coarse 4Nc interleaved output. */
+void or_vprestrict(const orl_t *F, const orl_t *C, const int *bondm,
+		   const double *r_f, const float *nd_f, const float *nd_c,
+		   double *b_c);
+
+/* Correction prolongation (move3mg.c:2748-3063): x_f += P x_c. */
+void or_vpprolong(const orl_t *F, const orl_t *C, const double *x_c,
+		  double *x_f, double v1koef, double p1koef);
+
+/* One reference-style Gauss-Seidel relaxation sweep (gausmat4 update rules,
+ * move3mg.c:3087-3131) in lexicographic node order, matrix-free. */
+void or_gs_sweep(const orl_t *L, const double *etaC, const double *etaXY,
+		 const double *etaXZ, const double *etaYZ, const int *bondm,
+		 const float *bondv, const int *bondn, const double *b, double *x,
+		 double p0koef, double v0koef);
+
+#endif
