@@ -1,0 +1,208 @@
+"""Benchmark: Stokes time-to-1e-8 relative residual on the 257^3 subduction
+configuration (10^6 viscosity contrast), mixed-precision multigrid vs plain
+FP64 multigrid on one MI355X, with the dominant kernel's HBM roofline and
+the reference CPU path timed on this host.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 257]
+
+One step = one Stokes solve from a zero initial guess to relative weighted
+residual 1e-8 (continuity rows weighted by eta_cell / mean cell size),
+inputs resident in HBM.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Stokes time-to-1e-8 residual @257^3, learned+mixed vs FP64 MG; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def smoother_bytes_per_launch(N, word):
+    """Compulsory bytes of one colour-phase launch of the velocity smoother
+    on N nodes: the updated component, the two other velocity components and
+    P read once (4 words), three stress-point viscosities (3 words), the
+    right-hand side of the component (1 word), the flag byte, and the
+    written half of the component (word/2)."""
+    return N * (8 * word + 1 + word / 2)
+
+
+def cpu_baseline(n=33, cap_s=25.0, threads=None):
+    """Runs _cpu_baseline_once with all host threads; the reference's
+    Gauss-Seidel sweeps race between OpenMP threads (move3mg.c:3089) and on
+    small grids many threads can diverge, so the thread count is halved
+    until the iteration stays finite."""
+    import math
+    threads = threads or min(os.cpu_count() or 1, 16)
+    while True:
+        r = _cpu_baseline_once(n, cap_s, threads)
+        if r is None or math.isfinite(r["rel_reached"]) or threads == 1:
+            return r
+        threads = max(1, threads // 2)
+
+
+def _cpu_baseline_once(n, cap_s, threads):
+    """The reference's own V-cycle iteration (gausmat4/vprestrict/vpprolong
+    with the mode.t3c parameters) on a bounded sample: the subduction
+    configuration at n^3, iterated until the relative weighted residual is
+    <= 1e-8 or cap_s elapses.  Returns a dict for the JSON line."""
+    import numpy as np
+
+    import i3elvis_amd as I
+    import oracle
+    from oracle import refh
+    if not refh.available():
+        return None
+    R = refh.Ref()
+    R.lib.refh_set_threads(int(threads))
+    cfg = I.make_config(2, n, n, n, seed=2)
+    N0 = n ** 3
+    R.setup_grid(cfg["gx"], cfg["gy"], cfg["gz"], cfg["multinum"])
+    R.set_bc(cfg["bondm"], cfg["bondv"], cfg["bondn"])
+    R.set_physics(g=cfg["g"], nubeg=cfg["nubeg"], nuend=cfg["nuend"])
+    for k in ("nu", "ro", "nxx", "nxy", "nxz", "nyz"):
+        R.f32(k)[:N0] = cfg[k]
+    L = oracle.level(n, n, n, 0, 0, cfg["gx"], cfg["gy"], cfg["gz"])
+    et = oracle.eta(L, cfg["nu"], cfg["nxx"], cfg["nxy"], cfg["nxz"], cfg["nyz"], 0, cfg["nubeg"], cfg["nuend"])
+    b = oracle.rhs(L, cfg["ro"], cfg["g"], cfg["bondm"], cfg["bondv"])
+    mn = cfg["multinum"]
+    pre = [8, 16, 16, 32, 0][:mn + 1]
+    post = [8, 16, 16, 32, 64][:mn + 1]
+    post[mn] = 64
+    t0 = time.time()
+    for l in range(1, mn + 1):
+        R.lib.refh_ronurestrict(l)
+    R.f64("sol0", R.MAXPPP)[:] = 0
+    R.lib.refh_store_all_levels()
+    sol0 = R.f64("sol0", R.MAXPPP)
+    cycles, rel = 0, 1.0
+    while time.time() - t0 < cap_s:
+        for _ in range(4):  # multicyc
+            for l in range(mn):
+                if pre[l] > 0:
+                    R.lib.refh_gausmat4(pre[l], l)
+                R.lib.refh_vprestrict(l + 1)
+            for l in range(mn, 0, -1):
+                R.lib.refh_gausmat4(post[l], l)
+                R.lib.refh_vpprolong(l)
+            R.lib.refh_gausmat4(post[0], 0)
+            cycles += 1
+        R.lib.refh_gausmat4(pre[0], 0)
+        x = sol0[:4 * N0].copy()
+        r = oracle.residual(L, et, cfg["bondm"], cfg["bondv"], cfg["bondn"], b, x)
+        w = oracle.wnorms(L, et[0], cfg["bondm"], b, r)
+        rel = w[0] / w[1]
+        if rel <= 1e-8 or not np.isfinite(rel):
+            break
+    dt = time.time() - t0
+    return dict(value=dt, unit="s", cores=int(threads), kind="reference",
+                sample=(f"reference gausmat4/vprestrict/vpprolong V-cycles (mode.t3c sweeps) on the "
+                        f"subduction configuration at {n}^3, {cycles} V-cycles in {dt:.1f} s, reached "
+                        f"relative residual {rel:.2e}" + ("" if rel <= 1e-8 else f" (capped at {cap_s:.0f} s)")),
+                vcycles_per_s=cycles / dt, rel_reached=rel, n=n)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=257)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--krylov", type=int, default=20)
+    ap.add_argument("--verbose", type=int, default=0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
+                                   "(z-slab decomposition, see DESIGN.md)", "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import i3elvis_amd as I
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    n = a.n
+    t = time.time()
+    cfg = I.make_config(2, n, n, n, seed=2)
+    print(f"# config built in {time.time() - t:.1f}s multinum={cfg['multinum']}", file=sys.stderr, flush=True)
+    S = I.Stokes(cfg)
+    S.set_coefficients()
+    z = np.zeros(n ** 3)
+
+    def run(prec, timing=False):
+        S.set_solution(z, z, z, z)
+        S.set_timing(timing)
+        st = S.solve(precision=prec, rtol=1e-8, krylov=a.krylov, max_outer=80, verbose=a.verbose)
+        return st
+
+    # warmup + timed, mixed precision (the product path)
+    for _ in range(a.warmup):
+        st = run(32)
+        print(f"# warmup mixed: {st.seconds:.3f}s vcycles={st.vcycles} rel={st.rel_res:.2e}",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    times, vcyc = [], []
+    t0 = time.time()
+    for k in range(a.steps):
+        st = run(32, timing=(k == a.steps - 1))
+        times.append(st.seconds)
+        vcyc.append(st.vcycles)
+        print(f"# step {k}: {st.seconds:.3f}s vcycles={st.vcycles} outer={st.outer_iters} rel={st.rel_res:.2e}",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.time() - t0
+    kst = S.kernel_stats()
+    mixed_t = float(np.mean(times))
+
+    fp64 = None
+    if not a.no_fp64:
+        st64 = run(64)
+        fp64 = dict(seconds=st64.seconds, vcycles=st64.vcycles, rel=st64.rel_res)
+        print(f"# fp64 MG: {st64.seconds:.3f}s vcycles={st64.vcycles} rel={st64.rel_res:.2e}",
+              file=sys.stderr, flush=True)
+
+    N = n ** 3
+    dom = kst.get("smooth_vel_f32_L0", {})
+    roof = None
+    if dom:
+        by = smoother_bytes_per_launch(N, 4)
+        ach = by / (dom["avg_ms"] * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel="k_smooth_vel<float> (level 0)",
+                    avg_ms=dom["avg_ms"], launches=dom["calls"], algorithmic_bytes=by)
+
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline()
+            if cpu:
+                proj = cpu["value"] * (N / cpu["n"] ** 3)
+                cpu["projected_257_s"] = proj
+        except Exception as e:  # the baseline is reported, never the target
+            cpu = dict(error=str(e))
+
+    out = {
+        "metric": METRIC, "value": round(mixed_t, 4), "unit": "s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 2), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded subduction layout, 10^6 viscosity contrast)",
+        "config": {"workload": f"stokes_{n}^3_subduction", "grid": [n, n, n], "levels": cfg["multinum"] + 1,
+                   "rtol": 1e-8, "krylov": a.krylov},
+        "vcycles": int(np.mean(vcyc)), "vcycles_per_s": round(float(np.mean(vcyc)) / mixed_t, 2),
+        "solves_per_s": round(1.0 / mixed_t, 4),
+        "fp64_mg": fp64, "speedup_vs_fp64_mg": (round(fp64["seconds"] / mixed_t, 3) if fp64 else None),
+        "roofline": roof, "cpu_baseline": cpu,
+        "kernels": {k: dict(avg_ms=round(v["avg_ms"], 5), calls=v["calls"]) for k, v in kst.items()},
+    }
+    print(json.dumps(out))
+    S.close()
+
+
+if __name__ == "__main__":
+    main()

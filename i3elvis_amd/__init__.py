@@ -1,0 +1,224 @@
+"""i3elvis_amd — MI355X-native hot path of I3ELVIS-style staggered-grid
+marker-in-cell geodynamics.
+
+The product is the C ABI in ``include/i3g.h`` implemented by the in-tree
+library ``i3elvis_amd/libi3g.so`` (hand-written gfx950 HIP kernels + C host
+layer).  This module is a thin ctypes view of that ABI used by tests and the
+benchmark; it never computes anything itself and raises if the library is
+missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libi3g.so")
+MAXLEV = 10
+_LIB = None
+
+
+class I3GError(RuntimeError):
+    pass
+
+
+def lib():
+    """The native library; raises I3GError when it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.isfile(LIBPATH):
+            raise I3GError("libi3g.so not built: run python i3elvis_amd/build.py")
+        _LIB = C.CDLL(LIBPATH)
+        _LIB.i3g_bc_expand.restype = C.c_long
+        _LIB.i3g_error_string.restype = C.c_char_p
+    return _LIB
+
+
+def check(rc, what=""):
+    if rc < 0:
+        msg = lib().i3g_error_string(rc).decode()
+        raise I3GError(f"{what}: {msg} ({rc})")
+    return rc
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Hier(C.Structure):
+    _fields_ = ([("nlev", C.c_int)] +
+                [(n, C.c_long * MAXLEV) for n in ("mgx", "mgy", "mgz", "mgp", "mgn", "mgs")] +
+                [("ntot", C.c_long)])
+
+
+class BcRule(C.Structure):
+    _fields_ = ([("var", C.c_int)] +
+                [(n, C.c_long) for n in ("m10", "m11", "m20", "m21", "m30", "m31")] +
+                [(n, C.c_int) for n in ("rel10", "rel11", "rel20", "rel21", "rel30", "rel31")] +
+                [("cnst", C.c_double), ("koef", C.c_double),
+                 ("dm1", C.c_int), ("dm2", C.c_int), ("dm3", C.c_int)])
+
+
+class Setup(C.Structure):
+    _fields_ = [("nx", C.c_long), ("ny", C.c_long), ("nz", C.c_long),
+                ("gx", C.c_void_p), ("gy", C.c_void_p), ("gz", C.c_void_p),
+                ("multinum", C.c_int), ("bondm", C.c_void_p), ("bondv", C.c_void_p),
+                ("bondn", C.c_void_p), ("nrec", C.c_long)]
+
+
+class StokesParams(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("precision", C.c_int), ("max_outer", C.c_int),
+                ("krylov", C.c_int), ("nu_pre", C.c_int * MAXLEV), ("nu_post", C.c_int * MAXLEV),
+                ("p0koef", C.c_double), ("p1koef", C.c_double), ("v0koef", C.c_double),
+                ("v1koef", C.c_double), ("inner_rtol", C.c_double), ("warm_start", C.c_int),
+                ("pxinit", C.c_long), ("pyinit", C.c_long), ("pzinit", C.c_long),
+                ("pinit", C.c_double), ("verbose", C.c_int)]
+
+
+class StokesStats(C.Structure):
+    _fields_ = [("converged", C.c_int), ("outer_iters", C.c_int), ("vcycles", C.c_int),
+                ("rel_res", C.c_double), ("res0", C.c_double), ("seconds", C.c_double),
+                ("hist", C.c_double * 256), ("nhist", C.c_int)]
+
+
+def hierarchy(nx, ny, nz, multinum):
+    h = Hier()
+    check(lib().i3g_hierarchy(C.c_long(nx), C.c_long(ny), C.c_long(nz), C.c_int(multinum),
+                              C.byref(h)), "hierarchy")
+    return h
+
+
+def level_lines(h, l, gx, gy, gz):
+    out = [np.zeros(h.mgx[l]), np.zeros(h.mgy[l]), np.zeros(h.mgz[l])]
+    check(lib().i3g_level_lines(C.byref(h), C.c_int(l), _p(gx), _p(gy), _p(gz), *[_p(o) for o in out]))
+    return out
+
+
+def freeslip_rules(t_top=273.0, t_bottom=1600.0):
+    rules = (BcRule * 64)()
+    n = check(lib().i3g_bc_rules_freeslip(C.c_double(t_top), C.c_double(t_bottom), rules, 64))
+    return rules, n
+
+
+def expand_bc(rules, nrules, h):
+    """Reference-layout boundary tables (bondm, bondv, bondn) over all levels."""
+    nodenum = h.mgx[0] * h.mgy[0] * h.mgz[0]
+    cap = 16 * (h.mgx[0] * h.mgy[0] + h.mgy[0] * h.mgz[0] + h.mgx[0] * h.mgz[0]) * h.nlev + 1024
+    bondm = np.zeros(10 * nodenum, np.int32)
+    bondv = np.zeros(2 * cap, np.float32)
+    bondn = np.zeros(cap, np.int32)
+    nb = check(lib().i3g_bc_expand(rules, C.c_int(nrules), C.byref(h), _p(bondm), _p(bondv),
+                                   _p(bondn), C.c_long(cap)), "bc_expand")
+    return bondm, bondv[:2 * (nb + 1)].copy(), bondn[:nb + 1].copy()
+
+
+def make_config(which, nx, ny, nz, seed=1, multinum=None, t_top=273.0, t_bottom=1600.0):
+    """Seeded synthetic configuration (see i3g_gen_config) with free-slip
+    boundary tables and the multigrid hierarchy."""
+    N = nx * ny * nz
+    gx, gy, gz = np.zeros(nx), np.zeros(ny), np.zeros(nz)
+    f = {k: np.zeros(N, np.float32) for k in ("nu", "ro", "tk", "nxx", "nxy", "nxz", "nyz")}
+    par = np.zeros(2)
+    check(lib().i3g_gen_config(C.c_int(which), C.c_long(nx), C.c_long(ny), C.c_long(nz),
+                               C.c_ulonglong(seed), _p(gx), _p(gy), _p(gz), _p(f["nu"]),
+                               _p(f["ro"]), _p(f["tk"]), _p(f["nxx"]), _p(f["nxy"]),
+                               _p(f["nxz"]), _p(f["nyz"]), _p(par)), "gen_config")
+    if multinum is None:
+        multinum = 0
+        m = min(nx, ny, nz)
+        while multinum < 5 and (m - 5) // 2 + 5 >= 12:
+            m = (m - 5) // 2 + 5
+            multinum += 1
+    h = hierarchy(nx, ny, nz, multinum)
+    rules, nr = freeslip_rules(t_top, t_bottom)
+    bondm, bondv, bondn = expand_bc(rules, nr, h)
+    cfg = dict(which=which, nx=nx, ny=ny, nz=nz, gx=gx, gy=gy, gz=gz, multinum=multinum,
+               hier=h, bondm=bondm, bondv=bondv, bondn=bondn, nubeg=par[0], nuend=par[1],
+               g=(0.0, 9.81, 0.0), viscmod=0)
+    cfg.update(f)
+    return cfg
+
+
+def default_params(**kw):
+    p = StokesParams()
+    lib().i3g_stokes_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k in ("nu_pre", "nu_post"):
+            for i, x in enumerate(v):
+                getattr(p, k)[i] = x
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Stokes:
+    """Device context for one grid (i3g_create .. i3g_destroy)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.N = cfg["nx"] * cfg["ny"] * cfg["nz"]
+        self._keep = [np.ascontiguousarray(cfg[k]) for k in ("gx", "gy", "gz", "bondm", "bondv", "bondn")]
+        gx, gy, gz, bm, bv, bn = self._keep
+        s = Setup(cfg["nx"], cfg["ny"], cfg["nz"], _p(gx), _p(gy), _p(gz), cfg["multinum"],
+                  _p(bm), _p(bv), _p(bn), len(bn))
+        self.ctx = C.c_void_p()
+        check(lib().i3g_create(C.byref(s), C.byref(self.ctx)), "create")
+
+    def close(self):
+        if self.ctx:
+            lib().i3g_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_coefficients(self, nu=None, nxx=None, nxy=None, nxz=None, nyz=None, ro=None):
+        c = self.cfg
+        arrs = [np.ascontiguousarray(a if a is not None else c[k], np.float32)
+                for a, k in ((nu, "nu"), (nxx, "nxx"), (nxy, "nxy"), (nxz, "nxz"), (nyz, "nyz"), (ro, "ro"))]
+        phys = np.array([c["g"][0], c["g"][1], c["g"][2], c["nubeg"], c["nuend"], c["viscmod"]], np.float64)
+        check(lib().i3g_set_coefficients(self.ctx, *[_p(a) for a in arrs], _p(phys)), "set_coefficients")
+
+    def residual(self, P, vx, vy, vz):
+        P, vx, vy, vz = (np.ascontiguousarray(a, np.float64) for a in (P, vx, vy, vz))
+        r = [np.zeros(self.N) for _ in range(4)]
+        norms = np.zeros(2)
+        check(lib().i3g_stokes_residual(self.ctx, _p(P), _p(vx), _p(vy), _p(vz), *[_p(a) for a in r],
+                                        _p(norms)), "residual")
+        return r, norms
+
+    def set_solution(self, P, vx, vy, vz):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (P, vx, vy, vz)]
+        check(lib().i3g_stokes_set_solution(self.ctx, *[_p(a) for a in arrs]), "set_solution")
+
+    def get_solution(self):
+        out = [np.zeros(self.N) for _ in range(4)]
+        check(lib().i3g_stokes_get_solution(self.ctx, *[_p(a) for a in out]), "get_solution")
+        return out
+
+    def solve(self, params=None, check_conv=True, **kw):
+        p = params or default_params(**kw)
+        st = StokesStats()
+        rc = lib().i3g_stokes_solve_resident(self.ctx, C.byref(p), C.byref(st))
+        if rc != -4 or check_conv:
+            check(rc, "stokes_solve")
+        return st
+
+    def set_timing(self, on=True):
+        check(lib().i3g_set_timing(self.ctx, C.c_int(1 if on else 0)))
+
+    def kernel_stats(self):
+        cap, nl = 64, 64
+        names = C.create_string_buffer(cap * nl)
+        ms = np.zeros(cap)
+        calls = np.zeros(cap, np.int64)
+        by = np.zeros(cap)
+        n = check(lib().i3g_kernel_stats(self.ctx, names, C.c_int(nl), _p(ms), _p(calls), _p(by), C.c_int(cap)))
+        out = {}
+        for q in range(n):
+            nm = names.raw[q * nl:(q + 1) * nl].split(b"\0", 1)[0].decode()
+            out[nm] = dict(avg_ms=float(ms[q]), calls=int(calls[q]), bytes=float(by[q]))
+        return out

@@ -1,0 +1,1181 @@
+/*
+ * i3g Stokes path on MI355X (gfx950): device-resident multigrid for the
+ * variable-viscosity staggered-grid Stokes system of the reference
+ * (move3mg.c), matrix-free, with an FP64 outer Krylov/defect-correction
+ * loop and FP64 or FP32 V-cycles.  C ABI in include/i3g.h.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/i3g.h"
+#include "stokes_kernels.h"
+
+#define HCHK(x)                                                                \
+	do {                                                                   \
+		hipError_t e_ = (x);                                           \
+		if (e_ != hipSuccess) {                                        \
+			fprintf(stderr, "i3g: HIP error %s at %s:%d\n",       \
+				hipGetErrorString(e_), __FILE__, __LINE__);    \
+			return I3G_EDEVICE;                                    \
+		}                                                              \
+	} while (0)
+
+/* ------------------------------------------------------------------ */
+/* Coefficient kernels                                                */
+/* ------------------------------------------------------------------ */
+
+/* Effective viscosities at the stress points of one level (the nueff
+ * blocks of sxxcalc/continerr move3mg.c:36-48, sxycalc :402-414, sxzcalc
+ * :517-529, syzcalc :632-644).  viscmod 0: the reference adds the float
+ * node values in float before dividing in double. */
+__global__ void k_eta(int nx, int ny, int nz, int lev0, const float *__restrict__ nu,
+		      const float *__restrict__ nxx, const float *__restrict__ nxy,
+		      const float *__restrict__ nxz, const float *__restrict__ nyz,
+		      int viscmod, double nubeg, double nuend, double *e64, float *e32,
+		      double *wP64, float *wP32, const double *gx, const double *gy,
+		      const double *gz)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int m3 = blockIdx.y;
+	int m1 = t / ny, m2 = t - m1 * ny;
+	if (m1 >= nx) return;
+	long N = (long)nx * ny * nz, sx = ny, sz = (long)nx * ny;
+	long n = m3 * sz + m1 * sx + m2;
+	double eC = 0, eXY = 0, eXZ = 0, eYZ = 0, e;
+	if (m1 >= 1 && m2 >= 1 && m3 >= 1) {
+		if (lev0 && m1 > 3 && m2 > 3 && m3 > 3 && m1 < nx - 3 && m2 < ny - 3 && m3 < nz - 3) {
+			e = nxx[n];
+		} else {
+			long v0 = n - sz - sx - 1;
+			long v[8] = {v0, v0 + 1, v0 + sx, v0 + sx + 1, v0 + sz, v0 + sz + 1, v0 + sz + sx, v0 + sz + sx + 1};
+			if (viscmod == 1) {
+				double s = 0;
+				for (int q = 0; q < 8; q++) s += log((double)nu[v[q]]);
+				e = exp(s / 8.0);
+			} else if (viscmod == 2) {
+				double s = 0;
+				for (int q = 0; q < 8; q++) s += 1.0 / (double)nu[v[q]];
+				e = 1.0 / (s / 8.0);
+			} else {
+				float fs = nu[v[0]];
+				for (int q = 1; q < 8; q++) fs = fs + nu[v[q]];
+				e = (double)fs / 8.0;
+			}
+		}
+		eC = e > nuend ? nuend : (e < nubeg ? nubeg : e);
+	}
+#define AVG2(a, b)                                                                          \
+	(viscmod == 1 ? exp((log((double)(a)) + log((double)(b))) / 2.0)                    \
+	 : viscmod == 2 ? 1.0 / ((1.0 / (double)(a) + 1.0 / (double)(b)) / 2.0)           \
+			: (double)(float)((a) + (b)) / 2.0)
+	if (m3 < nz - 1) {
+		if (lev0 && m1 > 3 && m2 > 3 && m3 > 2 && m1 < nx - 4 && m2 < ny - 4 && m3 < nz - 4)
+			e = nxy[n];
+		else
+			e = AVG2(nu[n], nu[n + sz]);
+		eXY = e > nuend ? nuend : (e < nubeg ? nubeg : e);
+	}
+	if (m2 < ny - 1) {
+		if (lev0 && m1 > 3 && m2 > 2 && m3 > 3 && m1 < nx - 4 && m2 < ny - 4 && m3 < nz - 4)
+			e = nxz[n];
+		else
+			e = AVG2(nu[n], nu[n + 1]);
+		eXZ = e > nuend ? nuend : (e < nubeg ? nubeg : e);
+	}
+	if (m1 < nx - 1) {
+		if (lev0 && m1 > 2 && m2 > 3 && m3 > 3 && m1 < nx - 4 && m2 < ny - 4 && m3 < nz - 4)
+			e = nyz[n];
+		else
+			e = AVG2(nu[n], nu[n + sx]);
+		eYZ = e > nuend ? nuend : (e < nubeg ? nubeg : e);
+	}
+#undef AVG2
+	e64[n] = eC; e64[N + n] = eXY; e64[2 * N + n] = eXZ; e64[3 * N + n] = eYZ;
+	e32[n] = (float)eC; e32[N + n] = (float)eXY; e32[2 * N + n] = (float)eXZ; e32[3 * N + n] = (float)eYZ;
+	double w = 0;
+	if (m1 >= 1 && m2 >= 1 && m3 >= 1) {
+		double h = ((gx[m1] - gx[m1 - 1]) + (gy[m2] - gy[m2 - 1]) + (gz[m3] - gz[m3 - 1])) / 3.0;
+		w = eC / h;
+	}
+	wP64[n] = w; wP32[n] = (float)w;
+}
+
+/* Momentum right-hand side -g_i * mean of four node densities, summed in
+ * float as the reference (move3mg.c:751, 943, 1142); BC rows take their
+ * constant; continuity rows 0. */
+__global__ void k_rhs(int nx, int ny, int nz, const float *__restrict__ ro, double g0,
+		      double g1, double g2, const uint8_t *__restrict__ flags,
+		      const int *__restrict__ bcid, const double *__restrict__ bcc,
+		      double *__restrict__ b)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int k = blockIdx.y;
+	int i = t / ny, j = t - i * ny;
+	if (i >= nx) return;
+	long N = (long)nx * ny * nz, sx = ny, sz = (long)nx * ny;
+	long n = k * sz + i * sx + j;
+	unsigned fl = flags[n];
+	for (int c = 0; c < 4; c++) {
+		double v = 0;
+		long u = c * N + n;
+		if (fl & FL_EX(c)) {
+			if (fl & FL_BC(c)) v = bcc[bcid[u]];
+			else if (c == 1) v = -g0 * (ro[n] + ro[n + 1] + ro[n + sz] + ro[n + 1 + sz]) / 4.0;
+			else if (c == 2) v = -g1 * (ro[n] + ro[n + sx] + ro[n + sz] + ro[n + sx + sz]) / 4.0;
+			else if (c == 3) v = -g2 * (ro[n] + ro[n + 1] + ro[n + sx] + ro[n + sx + 1]) / 4.0;
+		}
+		b[u] = v;
+	}
+}
+
+/* Coarse viscosity restriction in gather form (ronurestrict,
+ * move3mg.c:2642-2741): float accumulator, double weight sum, fine nodes
+ * in ascending index order as the reference's serial scatter. */
+__global__ void k_ronurestrict(int fnx, int fny, int cnx, int cny, int cnz,
+			       const float *__restrict__ nuf, float *__restrict__ nuc,
+			       const int *xs, const int *xi, const double *xw,
+			       const int *ys, const int *yi, const double *yw,
+			       const int *zs, const int *zi, const double *zw)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int K = blockIdx.y;
+	int I = t / cny, J = t - I * cny;
+	if (I >= cnx) return;
+	long szf = (long)fnx * fny;
+	float acc = 0;
+	double sw = 0;
+	for (int a = zs[K]; a < zs[K + 1]; a++)
+		for (int b = xs[I]; b < xs[I + 1]; b++)
+			for (int e = ys[J]; e < ys[J + 1]; e++) {
+				double swt = xw[b] * yw[e] * zw[a];
+				long nf = zi[a] * szf + (long)xi[b] * fny + yi[e];
+				acc = (float)((double)acc + (double)nuf[nf] * swt);
+				sw += swt;
+			}
+	nuc[(long)K * cnx * cny + (long)I * cny + J] = (float)((double)acc / sw);
+}
+
+/* ------------------------------------------------------------------ */
+/* Host structures                                                    */
+/* ------------------------------------------------------------------ */
+
+template <typename T> static T *dalloc(size_t n, int *err)
+{
+	void *p = nullptr;
+	if (n == 0) n = 1;
+	if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) {
+		*err = I3G_ENOMEM;
+		return nullptr;
+	}
+	return (T *)p;
+}
+
+template <typename T> static int upload(T *d, const T *h, size_t n)
+{
+	if (n) HCHK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+	return I3G_OK;
+}
+
+struct AxisMapDev {
+	/* kinds 0..2 (vp transfers) and 3 (ronurestrict) */
+	int *base[4];
+	double *wlo64[4], *whi64[4], *iw64[4];
+	float *wlo32[4], *whi32[4], *iw32[4];
+	int *istart[4], *iidx[4];
+};
+
+struct Level {
+	int nx, ny, nz;
+	long N;
+	double *g[3];            /* lines */
+	double *ih64[3], *ic64[3];
+	float *ih32[3], *ic32[3];
+	double *eta64;           /* 4N: eC, eXY, eXZ, eYZ */
+	float *eta32;
+	double *wP64;
+	float *wP32;
+	uint8_t *flags;
+	int *bcid;               /* 4N -> record (sorted) */
+	int nbc;
+	int *bc_u, *bc_p;
+	double *bc_c64, *bc_k64;
+	float *bc_c32, *bc_k32;
+	std::vector<int> depth_start; /* record ranges per depth */
+	float *nu;               /* level viscosity at nodes */
+	double *x64, *b64, *r64;
+	float *x32, *b32, *r32;
+	std::vector<double> hg[3];
+};
+
+struct Xfer {
+	AxisMapDev ax[3];
+};
+
+struct KStat {
+	double ms = 0;
+	long calls = 0;
+	double bytes = 0;
+};
+
+struct i3g_ctx {
+	hipStream_t s = nullptr;
+	i3g_hier h;
+	int nlev = 0;
+	std::vector<Level> L;
+	std::vector<Xfer> X;
+	double *rhs0 = nullptr;      /* level-0 FP64 right-hand side */
+	double *sol = nullptr;       /* level-0 FP64 solution (4N) */
+	float *ro = nullptr, *nxx = nullptr, *nxy = nullptr, *nxz = nullptr, *nyz = nullptr;
+	double *partials = nullptr;
+	long npart = 0;
+	double *dscal = nullptr;
+	double *hscal = nullptr;     /* pinned */
+	/* Krylov storage */
+	int kr_m = 0;
+	int kr_prec = 0;
+	std::vector<void *> V, Z;
+	void *kw = nullptr, *ku = nullptr;
+	double *e64 = nullptr;
+	/* timing */
+	bool timing = false;
+	std::map<std::string, KStat> stats;
+	std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+	std::vector<hipEvent_t> evpool;
+	size_t evnext = 0;
+	double phys[6] = {0, 9.81, 0, 1e18, 1e25, 0};
+	int coef_ready = 0;
+};
+
+/* ---- event timing of named kernels ------------------------------------ */
+
+static hipEvent_t ev_get(i3g_ctx *c)
+{
+	if (c->evnext >= c->evpool.size()) {
+		hipEvent_t e;
+		hipEventCreate(&e);
+		c->evpool.push_back(e);
+	}
+	return c->evpool[c->evnext++];
+}
+
+struct Timed {
+	i3g_ctx *c;
+	const char *name;
+	double bytes;
+	hipEvent_t a, b;
+	Timed(i3g_ctx *c_, const char *n, double by) : c(c_), name(n), bytes(by)
+	{
+		if (c->timing) {
+			a = ev_get(c);
+			b = ev_get(c);
+			hipEventRecord(a, c->s);
+		}
+	}
+	~Timed()
+	{
+		if (c->timing) {
+			hipEventRecord(b, c->s);
+			c->pending.push_back({name, {a, b}});
+			c->stats[name].bytes += bytes;
+		}
+	}
+};
+
+static void flush_timing(i3g_ctx *c)
+{
+	if (!c->timing) return;
+	hipStreamSynchronize(c->s);
+	for (auto &p : c->pending) {
+		float ms = 0;
+		hipEventElapsedTime(&ms, p.second.first, p.second.second);
+		KStat &k = c->stats[p.first];
+		k.ms += ms;
+		k.calls += 1;
+	}
+	c->pending.clear();
+	c->evnext = 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Setup                                                              */
+/* ------------------------------------------------------------------ */
+
+static inline dim3 plane_grid(int nx, int ny, int nz)
+{
+	long pl = (long)nx * ny;
+	return dim3((unsigned)((pl + 255) / 256), (unsigned)nz);
+}
+static inline dim3 half_grid(int nx, int ny, int nz)
+{
+	long pl = (long)nx * ((ny + 1) / 2);
+	return dim3((unsigned)((pl + 255) / 256), (unsigned)nz);
+}
+
+static int exists_uk(int c, long m1, long m2, long m3, long nx, long ny, long nz)
+{
+	switch (c) {
+	case 0: return m1 && m2 && m3;
+	case 1: return m2 < ny - 1 && m3 < nz - 1;
+	case 2: return m1 < nx - 1 && m3 < nz - 1;
+	default: return m1 < nx - 1 && m2 < ny - 1;
+	}
+}
+
+/* Coarse base index and 1D weights of a fine index for the transfer kinds
+ * of vprestrict/vpprolong (move3mg.c:2776-2789, 2806-2823, 3201-3214) and
+ * of ronurestrict (kind 3, move3mg.c:2670-2712). */
+static long xbase(long m, long nf, long nc, const double *gf, const double *gc, int kind)
+{
+	long r = (m - 2) / 2 + 2;
+	if (m < 3) r = m;
+	if (m > nf - 4) r = nc - (nf - m);
+	if (kind == 0 || kind == 3) {
+		if (r > nc - 2) r = nc - 2;
+	} else if (kind == 1) {
+		if (r > nc - 3) r = nc - 3;
+		if ((gf[m] + gf[m + 1]) / 2.0 < (gc[r] + gc[r + 1]) / 2.0 && r > 0) r -= 1;
+	} else {
+		if (r > nc - 2) r = nc - 2;
+		if ((gf[m] + gf[m - 1]) / 2.0 < (gc[r] + gc[r - 1]) / 2.0 && r > 1) r -= 1;
+	}
+	return r;
+}
+
+static double xw1(long m, long base, long cc, const double *gf, const double *gc, int kind)
+{
+	double d, dd;
+	if (kind == 3) {
+		d = gc[base + 1] - gc[base];
+		dd = (gc[cc] - gf[m]) / d;
+		return 1.0 - fabs(dd);
+	}
+	if (kind == 0) {
+		d = gc[base + 1] - gc[base];
+		dd = (gc[cc] - gf[m]) / d;
+	} else if (kind == 1) {
+		d = (gc[base + 2] - gc[base]) / 2.0;
+		dd = (gc[cc] + gc[cc + 1] - gf[m] - gf[m + 1]) / 2.0 / d;
+	} else {
+		d = (gc[base + 1] - gc[base - 1]) / 2.0;
+		dd = (gc[cc] + gc[cc - 1] - gf[m] - gf[m - 1]) / 2.0 / d;
+	}
+	if (cc == base) dd = -dd;
+	return 1.0 - dd;
+}
+
+static int build_axis(AxisMapDev &A, long nf, long nc, const double *gf, const double *gc)
+{
+	int err = 0;
+	for (int kind = 0; kind < 4; kind++) {
+		/* fine indices for which this kind is defined: node kinds all; mid
+		 * kind needs m < nf-1; centre kind needs m >= 1 */
+		std::vector<int> base(nf, 0);
+		std::vector<double> wlo(nf, 0), whi(nf, 0);
+		std::vector<std::vector<std::pair<int, double>>> inv(nc);
+		for (long m = 0; m < nf; m++) {
+			if (kind == 1 && m >= nf - 1) continue;
+			if (kind == 2 && m < 1) continue;
+			long b = xbase(m, nf, nc, gf, gc, kind);
+			if (b < 0 || b + 1 >= nc || (kind == 1 && b + 2 >= nc) || (kind == 2 && b < 1))
+				return I3G_EINVAL;
+			base[m] = (int)b;
+			wlo[m] = xw1(m, b, b, gf, gc, kind);
+			whi[m] = xw1(m, b, b + 1, gf, gc, kind);
+			inv[b].push_back({(int)m, wlo[m]});
+			inv[b + 1].push_back({(int)m, whi[m]});
+		}
+		std::vector<int> st(nc + 1, 0), idx;
+		std::vector<double> w;
+		for (long c = 0; c < nc; c++) {
+			st[c] = (int)idx.size();
+			for (auto &p : inv[c]) { idx.push_back(p.first); w.push_back(p.second); }
+		}
+		st[nc] = (int)idx.size();
+		std::vector<float> wlo32(wlo.begin(), wlo.end()), whi32(whi.begin(), whi.end()), w32(w.begin(), w.end());
+		A.base[kind] = dalloc<int>(nf, &err);
+		A.wlo64[kind] = dalloc<double>(nf, &err);
+		A.whi64[kind] = dalloc<double>(nf, &err);
+		A.wlo32[kind] = dalloc<float>(nf, &err);
+		A.whi32[kind] = dalloc<float>(nf, &err);
+		A.istart[kind] = dalloc<int>(nc + 1, &err);
+		A.iidx[kind] = dalloc<int>(idx.size(), &err);
+		A.iw64[kind] = dalloc<double>(w.size(), &err);
+		A.iw32[kind] = dalloc<float>(w.size(), &err);
+		if (err) return err;
+		if (upload(A.base[kind], base.data(), nf) || upload(A.wlo64[kind], wlo.data(), nf) ||
+		    upload(A.whi64[kind], whi.data(), nf) || upload(A.wlo32[kind], wlo32.data(), nf) ||
+		    upload(A.whi32[kind], whi32.data(), nf) || upload(A.istart[kind], st.data(), nc + 1) ||
+		    upload(A.iidx[kind], idx.data(), idx.size()) || upload(A.iw64[kind], w.data(), w.size()) ||
+		    upload(A.iw32[kind], w32.data(), w.size()))
+			return I3G_EDEVICE;
+	}
+	return I3G_OK;
+}
+
+static int build_level(i3g_ctx *c, int l, const i3g_setup *s)
+{
+	Level &L = c->L[l];
+	const i3g_hier &h = c->h;
+	int err = 0;
+	L.nx = (int)h.mgx[l]; L.ny = (int)h.mgy[l]; L.nz = (int)h.mgz[l];
+	L.N = h.mgn[l];
+	long N = L.N, nx = L.nx, ny = L.ny, nz = L.nz;
+	for (int a = 0; a < 3; a++) L.hg[a].resize(a == 0 ? nx : a == 1 ? ny : nz);
+	i3g_level_lines(&h, l, s->gx, s->gy, s->gz, L.hg[0].data(), L.hg[1].data(), L.hg[2].data());
+	for (int a = 0; a < 3; a++) {
+		long n = (long)L.hg[a].size();
+		const double *g = L.hg[a].data();
+		std::vector<double> ih(n, 0), ic(n, 0);
+		for (long i = 0; i + 1 < n; i++) ih[i] = 1.0 / (g[i + 1] - g[i]);
+		for (long i = 1; i + 1 < n; i++) ic[i] = 1.0 / (g[i + 1] - g[i - 1]);
+		std::vector<float> ih32(ih.begin(), ih.end()), ic32(ic.begin(), ic.end());
+		L.g[a] = dalloc<double>(n, &err);
+		L.ih64[a] = dalloc<double>(n, &err);
+		L.ic64[a] = dalloc<double>(n, &err);
+		L.ih32[a] = dalloc<float>(n, &err);
+		L.ic32[a] = dalloc<float>(n, &err);
+		if (err) return err;
+		if (upload(L.g[a], g, n) || upload(L.ih64[a], ih.data(), n) || upload(L.ic64[a], ic.data(), n) ||
+		    upload(L.ih32[a], ih32.data(), n) || upload(L.ic32[a], ic32.data(), n))
+			return I3G_EDEVICE;
+	}
+	/* flags and boundary records */
+	std::vector<uint8_t> fl(N, 0);
+	std::vector<int> bcid(4 * N, -1);
+	struct R { int u, p; double c, k; int depth; };
+	std::vector<R> recs;
+	long mgp = h.mgp[l];
+	for (long m3 = 0; m3 < nz; m3++)
+		for (long m1 = 0; m1 < nx; m1++)
+			for (long m2 = 0; m2 < ny; m2++) {
+				long n = m3 * nx * ny + m1 * ny + m2;
+				for (int cc = 0; cc < 4; cc++) {
+					if (!exists_uk(cc, m1, m2, m3, nx, ny, nz)) continue;
+					fl[n] |= FL_EX(cc);
+					int rec = s->bondm[(mgp + n) * 4 + cc];
+					if (!rec) continue;
+					if (rec < 0 || rec >= s->nrec) return I3G_EBC;
+					fl[n] |= FL_BC(cc);
+					R r;
+					r.u = (int)(cc * N + n);
+					r.c = s->bondv[2 * rec];
+					r.k = s->bondv[2 * rec + 1];
+					r.p = -1;
+					r.depth = 0;
+					if (s->bondn[rec]) {
+						long g = (long)s->bondn[rec] - 1;
+						long gn = g / 4 - mgp;
+						int gc = (int)(g % 4);
+						if (gn < 0 || gn >= N) return I3G_EBC;
+						r.p = (int)(gc * N + gn);
+					}
+					bcid[r.u] = (int)recs.size();
+					recs.push_back(r);
+				}
+			}
+	/* the interior stencil rows must not reach outside the grid: every
+	 * unknown on the rims where a stencil is undefined has to be a BC row */
+	for (long m3 = 0; m3 < nz; m3++)
+		for (long m1 = 0; m1 < nx; m1++)
+			for (long m2 = 0; m2 < ny; m2++) {
+				long n = m3 * nx * ny + m1 * ny + m2;
+				for (int cc = 1; cc < 4; cc++) {
+					if (!(fl[n] & FL_EX(cc)) || (fl[n] & FL_BC(cc))) continue;
+					bool ok = m1 >= 1 && m2 >= 1 && m3 >= 1 && m1 < nx - 1 && m2 < ny - 1 && m3 < nz - 1;
+					if (cc == 1) ok = ok && m2 < ny - 2 && m3 < nz - 2;
+					if (cc == 2) ok = ok && m1 < nx - 2 && m3 < nz - 2;
+					if (cc == 3) ok = ok && m1 < nx - 2 && m2 < ny - 2;
+					if (!ok) return I3G_EBC;
+				}
+			}
+	/* dependency depth of each BC row (partner chains) */
+	for (size_t q = 0; q < recs.size(); q++) {
+		int d = 1, u = (int)q;
+		while (recs[u].p >= 0 && bcid[recs[u].p] >= 0) {
+			u = bcid[recs[u].p];
+			if (++d > 64) return I3G_EBC; /* cycle */
+		}
+		recs[q].depth = d;
+	}
+	std::vector<int> order(recs.size());
+	for (size_t q = 0; q < order.size(); q++) order[q] = (int)q;
+	std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return recs[a].depth < recs[b].depth; });
+	std::vector<int> bu(recs.size()), bp(recs.size());
+	std::vector<double> bcv(recs.size()), bkv(recs.size());
+	int maxd = 0;
+	for (size_t q = 0; q < order.size(); q++) {
+		const R &r = recs[order[q]];
+		bu[q] = r.u; bp[q] = r.p; bcv[q] = r.c; bkv[q] = r.k;
+		bcid[r.u] = (int)q;
+		maxd = std::max(maxd, r.depth);
+	}
+	L.depth_start.assign(maxd + 2, 0);
+	for (size_t q = 0; q < order.size(); q++) L.depth_start[recs[order[q]].depth + 1] = (int)q + 1;
+	for (int d = 1; d <= maxd + 1; d++) L.depth_start[d] = std::max(L.depth_start[d], L.depth_start[d - 1]);
+	L.nbc = (int)recs.size();
+	std::vector<float> bcv32(bcv.begin(), bcv.end()), bkv32(bkv.begin(), bkv.end());
+	L.flags = dalloc<uint8_t>(N, &err);
+	L.bcid = dalloc<int>(4 * N, &err);
+	L.bc_u = dalloc<int>(L.nbc, &err);
+	L.bc_p = dalloc<int>(L.nbc, &err);
+	L.bc_c64 = dalloc<double>(L.nbc, &err);
+	L.bc_k64 = dalloc<double>(L.nbc, &err);
+	L.bc_c32 = dalloc<float>(L.nbc, &err);
+	L.bc_k32 = dalloc<float>(L.nbc, &err);
+	L.eta64 = dalloc<double>(4 * N, &err);
+	L.eta32 = dalloc<float>(4 * N, &err);
+	L.wP64 = dalloc<double>(N, &err);
+	L.wP32 = dalloc<float>(N, &err);
+	L.nu = dalloc<float>(N, &err);
+	L.x64 = dalloc<double>(4 * N, &err);
+	L.b64 = dalloc<double>(4 * N, &err);
+	L.r64 = dalloc<double>(4 * N, &err);
+	L.x32 = dalloc<float>(4 * N, &err);
+	L.b32 = dalloc<float>(4 * N, &err);
+	L.r32 = dalloc<float>(4 * N, &err);
+	if (err) return err;
+	if (upload(L.flags, fl.data(), N) || upload(L.bcid, bcid.data(), 4 * N) ||
+	    upload(L.bc_u, bu.data(), L.nbc) || upload(L.bc_p, bp.data(), L.nbc) ||
+	    upload(L.bc_c64, bcv.data(), L.nbc) || upload(L.bc_k64, bkv.data(), L.nbc) ||
+	    upload(L.bc_c32, bcv32.data(), L.nbc) || upload(L.bc_k32, bkv32.data(), L.nbc))
+		return I3G_EDEVICE;
+	HCHK(hipMemset(L.x64, 0, 4 * N * sizeof(double)));
+	HCHK(hipMemset(L.x32, 0, 4 * N * sizeof(float)));
+	return I3G_OK;
+}
+
+template <typename T> static LevelView<T> view(const Level &L);
+template <> LevelView<double> view<double>(const Level &L)
+{
+	LevelView<double> v;
+	v.nx = L.nx; v.ny = L.ny; v.nz = L.nz; v.N = L.N;
+	v.ihx = L.ih64[0]; v.ihy = L.ih64[1]; v.ihz = L.ih64[2];
+	v.icx = L.ic64[0]; v.icy = L.ic64[1]; v.icz = L.ic64[2];
+	v.eC = L.eta64; v.eXY = L.eta64 + L.N; v.eXZ = L.eta64 + 2 * L.N; v.eYZ = L.eta64 + 3 * L.N;
+	v.flags = L.flags; v.wP = L.wP64;
+	return v;
+}
+template <> LevelView<float> view<float>(const Level &L)
+{
+	LevelView<float> v;
+	v.nx = L.nx; v.ny = L.ny; v.nz = L.nz; v.N = L.N;
+	v.ihx = L.ih32[0]; v.ihy = L.ih32[1]; v.ihz = L.ih32[2];
+	v.icx = L.ic32[0]; v.icy = L.ic32[1]; v.icz = L.ic32[2];
+	v.eC = L.eta32; v.eXY = L.eta32 + L.N; v.eXZ = L.eta32 + 2 * L.N; v.eYZ = L.eta32 + 3 * L.N;
+	v.flags = L.flags; v.wP = L.wP32;
+	return v;
+}
+template <typename T> static BcRec<T> bcview(const Level &L);
+template <> BcRec<double> bcview<double>(const Level &L) { return {L.bc_u, L.bc_p, L.bc_c64, L.bc_k64}; }
+template <> BcRec<float> bcview<float>(const Level &L) { return {L.bc_u, L.bc_p, L.bc_c32, L.bc_k32}; }
+template <typename T> static XferView<T> xview(const Xfer &X);
+template <> XferView<double> xview<double>(const Xfer &X)
+{
+	XferView<double> v;
+	for (int a = 0; a < 3; a++)
+		for (int q = 0; q < 3; q++) {
+			v.ax[a].base[q] = X.ax[a].base[q];
+			v.ax[a].wlo[q] = X.ax[a].wlo64[q];
+			v.ax[a].whi[q] = X.ax[a].whi64[q];
+			v.ax[a].istart[q] = X.ax[a].istart[q];
+			v.ax[a].iidx[q] = X.ax[a].iidx[q];
+			v.ax[a].iw[q] = X.ax[a].iw64[q];
+		}
+	return v;
+}
+template <> XferView<float> xview<float>(const Xfer &X)
+{
+	XferView<float> v;
+	for (int a = 0; a < 3; a++)
+		for (int q = 0; q < 3; q++) {
+			v.ax[a].base[q] = X.ax[a].base[q];
+			v.ax[a].wlo[q] = X.ax[a].wlo32[q];
+			v.ax[a].whi[q] = X.ax[a].whi32[q];
+			v.ax[a].istart[q] = X.ax[a].istart[q];
+			v.ax[a].iidx[q] = X.ax[a].iidx[q];
+			v.ax[a].iw[q] = X.ax[a].iw32[q];
+		}
+	return v;
+}
+template <typename T> static T *lx(Level &L);
+template <> double *lx<double>(Level &L) { return L.x64; }
+template <> float *lx<float>(Level &L) { return L.x32; }
+template <typename T> static T *lb(Level &L);
+template <> double *lb<double>(Level &L) { return L.b64; }
+template <> float *lb<float>(Level &L) { return L.b32; }
+template <typename T> static T *lr(Level &L);
+template <> double *lr<double>(Level &L) { return L.r64; }
+template <> float *lr<float>(Level &L) { return L.r32; }
+
+extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
+{
+	if (!s || !out || !s->gx || !s->gy || !s->gz || !s->bondm || !s->bondv || !s->bondn)
+		return I3G_EINVAL;
+	*out = nullptr;
+	i3g_ctx *c = new i3g_ctx();
+	int rc = i3g_hierarchy(s->nx, s->ny, s->nz, s->multinum, &c->h);
+	if (rc) { delete c; return rc; }
+	if (s->nx * s->ny * s->nz > 400000000L) { delete c; return I3G_ECAP; }
+	c->nlev = c->h.nlev;
+	if (hipStreamCreate(&c->s) != hipSuccess) { delete c; return I3G_EDEVICE; }
+	c->L.resize(c->nlev);
+	for (int l = 0; l < c->nlev; l++) {
+		rc = build_level(c, l, s);
+		if (rc) { i3g_destroy(c); return rc; }
+	}
+	c->X.resize(c->nlev > 1 ? c->nlev - 1 : 0);
+	for (int l = 0; l + 1 < c->nlev; l++) {
+		Level &F = c->L[l], &Cl = c->L[l + 1];
+		for (int a = 0; a < 3; a++) {
+			long nf = (long)F.hg[a].size(), nc = (long)Cl.hg[a].size();
+			rc = build_axis(c->X[l].ax[a], nf, nc, F.hg[a].data(), Cl.hg[a].data());
+			if (rc) { i3g_destroy(c); return rc; }
+		}
+	}
+	int err = 0;
+	long N0 = c->L[0].N;
+	c->rhs0 = dalloc<double>(4 * N0, &err);
+	c->sol = dalloc<double>(4 * N0, &err);
+	c->ro = dalloc<float>(N0, &err);
+	c->nxx = dalloc<float>(N0, &err);
+	c->nxy = dalloc<float>(N0, &err);
+	c->nxz = dalloc<float>(N0, &err);
+	c->nyz = dalloc<float>(N0, &err);
+	c->npart = 8192;
+	c->partials = dalloc<double>(c->npart * 2, &err);
+	c->dscal = dalloc<double>(64, &err);
+	c->e64 = dalloc<double>(4 * N0, &err);
+	if (err) { i3g_destroy(c); return err; }
+	if (hipHostMalloc((void **)&c->hscal, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+		i3g_destroy(c);
+		return I3G_ENOMEM;
+	}
+	hipMemset(c->sol, 0, 4 * N0 * sizeof(double));
+	*out = c;
+	return I3G_OK;
+}
+
+static void free_krylov(i3g_ctx *c)
+{
+	for (void *p : c->V) hipFree(p);
+	for (void *p : c->Z) hipFree(p);
+	c->V.clear(); c->Z.clear();
+	if (c->kw) hipFree(c->kw);
+	if (c->ku) hipFree(c->ku);
+	c->kw = c->ku = nullptr;
+	c->kr_m = 0;
+}
+
+extern "C" int i3g_destroy(i3g_ctx *c)
+{
+	if (!c) return I3G_OK;
+	hipDeviceSynchronize();
+	free_krylov(c);
+	for (auto &L : c->L) {
+		void *ps[] = {L.g[0], L.g[1], L.g[2], L.ih64[0], L.ih64[1], L.ih64[2], L.ic64[0], L.ic64[1],
+			      L.ic64[2], L.ih32[0], L.ih32[1], L.ih32[2], L.ic32[0], L.ic32[1], L.ic32[2],
+			      L.eta64, L.eta32, L.wP64, L.wP32, L.flags, L.bcid, L.bc_u, L.bc_p, L.bc_c64,
+			      L.bc_k64, L.bc_c32, L.bc_k32, L.nu, L.x64, L.b64, L.r64, L.x32, L.b32, L.r32};
+		for (void *p : ps)
+			if (p) hipFree(p);
+	}
+	for (auto &X : c->X)
+		for (int a = 0; a < 3; a++)
+			for (int q = 0; q < 4; q++) {
+				void *ps[] = {X.ax[a].base[q], X.ax[a].wlo64[q], X.ax[a].whi64[q], X.ax[a].iw64[q],
+					      X.ax[a].wlo32[q], X.ax[a].whi32[q], X.ax[a].iw32[q], X.ax[a].istart[q],
+					      X.ax[a].iidx[q]};
+				for (void *p : ps)
+					if (p) hipFree(p);
+			}
+	void *ps[] = {c->rhs0, c->sol, c->ro, c->nxx, c->nxy, c->nxz, c->nyz, c->partials, c->dscal, c->e64};
+	for (void *p : ps)
+		if (p) hipFree(p);
+	if (c->hscal) hipHostFree(c->hscal);
+	for (auto e : c->evpool) hipEventDestroy(e);
+	if (c->s) hipStreamDestroy(c->s);
+	delete c;
+	return I3G_OK;
+}
+
+extern "C" int i3g_set_coefficients(i3g_ctx *c, const float *nu, const float *nxx, const float *nxy,
+				    const float *nxz, const float *nyz, const float *ro, const double *phys)
+{
+	if (!c || !nu || !nxx || !nxy || !nxz || !nyz || !ro || !phys) return I3G_EINVAL;
+	for (int q = 0; q < 6; q++) c->phys[q] = phys[q];
+	int viscmod = (int)phys[5];
+	if (viscmod < 0 || viscmod > 2) return I3G_EINVAL;
+	long N0 = c->L[0].N;
+	HCHK(hipMemcpyAsync(c->L[0].nu, nu, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
+	HCHK(hipMemcpyAsync(c->nxx, nxx, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
+	HCHK(hipMemcpyAsync(c->nxy, nxy, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
+	HCHK(hipMemcpyAsync(c->nxz, nxz, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
+	HCHK(hipMemcpyAsync(c->nyz, nyz, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
+	HCHK(hipMemcpyAsync(c->ro, ro, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
+	/* coarse viscosities (viscmod 0 weights; log/harmonic means are
+	 * applied by transforming nu before and after, as the reference) */
+	for (int l = 1; l < c->nlev; l++) {
+		Level &F = c->L[l - 1], &Cl = c->L[l];
+		const Xfer &X = c->X[l - 1];
+		if (viscmod != 0) return I3G_EINVAL; /* log/harmonic restriction not built */
+		hipLaunchKernelGGL(k_ronurestrict, plane_grid(Cl.nx, Cl.ny, Cl.nz), dim3(256), 0, c->s,
+				   F.nx, F.ny, Cl.nx, Cl.ny, Cl.nz, F.nu, Cl.nu, X.ax[0].istart[3],
+				   X.ax[0].iidx[3], X.ax[0].iw64[3], X.ax[1].istart[3], X.ax[1].iidx[3],
+				   X.ax[1].iw64[3], X.ax[2].istart[3], X.ax[2].iidx[3], X.ax[2].iw64[3]);
+	}
+	for (int l = 0; l < c->nlev; l++) {
+		Level &L = c->L[l];
+		hipLaunchKernelGGL(k_eta, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, L.nx, L.ny, L.nz,
+				   l == 0 ? 1 : 0, L.nu, c->nxx, c->nxy, c->nxz, c->nyz, viscmod, phys[3],
+				   phys[4], L.eta64, L.eta32, L.wP64, L.wP32, L.g[0], L.g[1], L.g[2]);
+	}
+	Level &L0 = c->L[0];
+	hipLaunchKernelGGL(k_rhs, plane_grid(L0.nx, L0.ny, L0.nz), dim3(256), 0, c->s, L0.nx, L0.ny, L0.nz,
+			   c->ro, phys[0], phys[1], phys[2], L0.flags, L0.bcid, L0.bc_c64, c->rhs0);
+	HCHK(hipGetLastError());
+	HCHK(hipStreamSynchronize(c->s));
+	c->coef_ready = 1;
+	return I3G_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Multigrid                                                          */
+/* ------------------------------------------------------------------ */
+
+template <typename T> static void bc_enforce(i3g_ctx *c, Level &L, T *x, int homog)
+{
+	BcRec<T> bc = bcview<T>(L);
+	for (size_t d = 1; d + 1 < L.depth_start.size(); d++) {
+		int a = L.depth_start[d], b = L.depth_start[d + 1];
+		if (b > a)
+			hipLaunchKernelGGL(k_bc<T>, dim3((b - a + 255) / 256), dim3(256), 0, c->s, x, bc, a, b, homog);
+	}
+}
+
+template <typename T>
+static void sweep(i3g_ctx *c, int l, T *x, const T *b, const i3g_stokes_params *p)
+{
+	Level &L = c->L[l];
+	LevelView<T> v = view<T>(L);
+	dim3 hg = half_grid(L.nx, L.ny, L.nz);
+	double by = (double)L.N * (sizeof(T) * 2 + 1 + sizeof(T) * 4 + sizeof(T) * 3) / 2;
+	for (int comp = 1; comp <= 3; comp++)
+		for (int col = 0; col < 2; col++) {
+			Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "smooth_vel_f64_L0" : "smooth_vel_f32_L0") : "smooth_vel_coarse", by);
+			hipLaunchKernelGGL(k_smooth_vel<T>, hg, dim3(256), 0, c->s, v, x, b, comp, col, (T)p->v0koef);
+		}
+	{
+		Timed tm(c, l == 0 ? "smooth_p_L0" : "smooth_p_coarse", 0);
+		hipLaunchKernelGGL(k_smooth_p<T>, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, v, x, b,
+				   (T)p->p0koef);
+	}
+	bc_enforce<T>(c, L, x, 1);
+}
+
+template <typename T> static void residual_level(i3g_ctx *c, int l, const T *x, const T *b, T *r)
+{
+	Level &L = c->L[l];
+	Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "residual_f64_L0" : "residual_f32_L0") : "residual_coarse", 0);
+	hipLaunchKernelGGL((k_residual<T, T, 0>), plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s,
+			   view<T>(L), x, b, r, L.bcid, bcview<T>(L), (double *)nullptr);
+}
+
+/* V-cycle on level l: x_l (in/out), b_l given (level buffers). */
+template <typename T> static void vcycle(i3g_ctx *c, int l, const i3g_stokes_params *p)
+{
+	Level &L = c->L[l];
+	T *x = lx<T>(L), *b = lb<T>(L);
+	if (l == c->nlev - 1) {
+		int ns = p->nu_post[l] > 0 ? p->nu_post[l] : 64;
+		for (int s = 0; s < ns; s++) sweep<T>(c, l, x, b, p);
+		return;
+	}
+	for (int s = 0; s < p->nu_pre[l]; s++) sweep<T>(c, l, x, b, p);
+	residual_level<T>(c, l, x, b, lr<T>(L));
+	Level &Cl = c->L[l + 1];
+	{
+		Timed tm(c, "restrict", 0);
+		hipLaunchKernelGGL(k_restrict<T>, plane_grid(Cl.nx, Cl.ny, Cl.nz), dim3(256), 0, c->s, view<T>(L),
+				   view<T>(Cl), xview<T>(c->X[l]), (const T *)lr<T>(L), lb<T>(Cl), lx<T>(Cl));
+	}
+	vcycle<T>(c, l + 1, p);
+	{
+		Timed tm(c, "prolong", 0);
+		hipLaunchKernelGGL(k_prolong<T>, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, view<T>(L),
+				   view<T>(Cl), xview<T>(c->X[l]), (const T *)lx<T>(Cl), x, (T)p->v1koef,
+				   (T)p->p1koef);
+	}
+	for (int s = 0; s < p->nu_post[l]; s++) sweep<T>(c, l, x, b, p);
+}
+
+/* ---- reductions -------------------------------------------------------- */
+
+static int dot_blocks(long n)
+{
+	long b = (n + 4095) / 4096;
+	if (b > 2048) b = 2048;
+	if (b < 1) b = 1;
+	return (int)b;
+}
+
+template <typename A, typename B> static double dot(i3g_ctx *c, const A *a, const B *b, long n)
+{
+	int nb = dot_blocks(n);
+	hipLaunchKernelGGL((k_dot_part<A, B>), dim3(nb), dim3(256), 0, c->s, a, b, n, c->partials);
+	hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c->s, c->partials, (long)nb, 1, c->dscal);
+	hipMemcpyAsync(c->hscal, c->dscal, sizeof(double), hipMemcpyDeviceToHost, c->s);
+	hipStreamSynchronize(c->s);
+	return c->hscal[0];
+}
+
+/* FP64 residual of the level-0 system into r (may be L0.r64) and the
+ * weighted metric: out[0] = ||W r|| over non-BC rows, out[1] = ||W b||. */
+static int resid64(i3g_ctx *c, const double *x, double *r, double *out)
+{
+	Level &L = c->L[0];
+	dim3 g = plane_grid(L.nx, L.ny, L.nz);
+	long nb = (long)g.x * g.y;
+	if (nb > c->npart) {
+		hipFree(c->partials);
+		int err = 0;
+		c->npart = nb;
+		c->partials = dalloc<double>(2 * nb, &err);
+		if (err) return err;
+	}
+	{
+		Timed tm(c, "residual_f64_L0", (double)L.N * (32 + 32 + 32 + 8 + 1 + 32));
+		hipLaunchKernelGGL((k_residual<double, double, 0>), g, dim3(256), 0, c->s, view<double>(L), x,
+				   (const double *)c->rhs0, r, L.bcid, bcview<double>(L), c->partials);
+	}
+	hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c->s, c->partials, nb, 2, c->dscal);
+	HCHK(hipMemcpyAsync(c->hscal, c->dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+	HCHK(hipStreamSynchronize(c->s));
+	out[0] = sqrt(c->hscal[0]);
+	out[1] = sqrt(c->hscal[1]);
+	return I3G_OK;
+}
+
+extern "C" int i3g_stokes_residual(i3g_ctx *c, const double *P, const double *vx, const double *vy,
+				   const double *vz, double *rP, double *rvx, double *rvy, double *rvz,
+				   double *norms)
+{
+	if (!c || !c->coef_ready || !P || !vx || !vy || !vz) return I3G_EINVAL;
+	Level &L = c->L[0];
+	long N = L.N;
+	const double *src[4] = {P, vx, vy, vz};
+	for (int q = 0; q < 4; q++)
+		HCHK(hipMemcpyAsync(c->e64 + q * N, src[q], N * sizeof(double), hipMemcpyHostToDevice, c->s));
+	double nrm[2];
+	int rc = resid64(c, c->e64, L.r64, nrm);
+	if (rc) return rc;
+	double *dst[4] = {rP, rvx, rvy, rvz};
+	for (int q = 0; q < 4; q++)
+		if (dst[q]) HCHK(hipMemcpyAsync(dst[q], L.r64 + q * N, N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+	HCHK(hipStreamSynchronize(c->s));
+	if (norms) { norms[0] = nrm[0]; norms[1] = nrm[1]; }
+	return I3G_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* FGMRES                                                             */
+/* ------------------------------------------------------------------ */
+
+static inline dim3 lin_grid(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+template <typename T> static int ensure_krylov(i3g_ctx *c, int m)
+{
+	if (c->kr_m == m && c->kr_prec == (int)sizeof(T)) return I3G_OK;
+	free_krylov(c);
+	long n = 4 * c->L[0].N;
+	int err = 0;
+	for (int q = 0; q <= m; q++) c->V.push_back(dalloc<T>(n, &err));
+	for (int q = 0; q < m; q++) c->Z.push_back(dalloc<T>(n, &err));
+	c->kw = dalloc<T>(n, &err);
+	c->ku = dalloc<T>(n, &err);
+	if (err) { free_krylov(c); return err; }
+	c->kr_m = m;
+	c->kr_prec = (int)sizeof(T);
+	return I3G_OK;
+}
+
+/* scale the P plane by wP (fwd) or 1/wP (inverse); other planes copied */
+template <typename T>
+__global__ void k_wscale(T *__restrict__ y, const T *__restrict__ x, const T *__restrict__ wP, long N, int inv)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= 4 * N) return;
+	T v = x[t];
+	if (t < N) {
+		T w = wP[t];
+		v = inv ? (w != (T)0 ? v / w : (T)0) : v * w;
+	}
+	y[t] = v;
+}
+
+/* One FGMRES(m) cycle on A e = r with a V-cycle right preconditioner, in
+ * the space scaled by W (continuity rows times eta_cell/h).  r: unscaled
+ * residual (T, 4N); e: output correction (T, 4N).  Returns the number of
+ * V-cycles; *red receives the estimated reduction of ||W r||. */
+template <typename T>
+static int fgmres(i3g_ctx *c, const T *r, T *e, int m, double target, const i3g_stokes_params *p,
+		  double *red)
+{
+	Level &L = c->L[0];
+	long N = L.N, n = 4 * N;
+	const T *wP = sizeof(T) == 8 ? (const T *)(const void *)L.wP64 : (const T *)(const void *)L.wP32;
+	T **V = (T **)c->V.data(), **Z = (T **)c->Z.data();
+	T *w = (T *)c->kw;
+	std::vector<double> H((m + 1) * m, 0), cs(m, 0), sn(m, 0), g(m + 1, 0);
+	hipLaunchKernelGGL(k_wscale<T>, lin_grid(n), dim3(256), 0, c->s, V[0], r, wP, N, 0);
+	double beta = sqrt(dot(c, V[0], V[0], n));
+	if (red) *red = 1.0;
+	hipMemsetAsync(e, 0, n * sizeof(T), c->s);
+	if (beta == 0) return 0;
+	hipLaunchKernelGGL((k_copy_scale<T, T>), lin_grid(n), dim3(256), 0, c->s, V[0], V[0], 1.0 / beta, n);
+	g[0] = beta;
+	int j, used = 0;
+	for (j = 0; j < m; j++) {
+		/* z_j = M^-1 W^-1 v_j */
+		T *x0 = lx<T>(L), *b0 = lb<T>(L);
+		hipLaunchKernelGGL(k_wscale<T>, lin_grid(n), dim3(256), 0, c->s, b0, V[j], wP, N, 1);
+		hipMemsetAsync(x0, 0, n * sizeof(T), c->s);
+		vcycle<T>(c, 0, p);
+		used++;
+		hipMemcpyAsync(Z[j], x0, n * sizeof(T), hipMemcpyDeviceToDevice, c->s);
+		/* w = W A z_j */
+		{
+			Timed tm(c, sizeof(T) == 8 ? "matvec_f64_L0" : "matvec_f32_L0", 0);
+			hipLaunchKernelGGL((k_residual<T, T, 1>), plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s,
+					   view<T>(L), (const T *)Z[j], (const T *)nullptr, w, L.bcid, bcview<T>(L),
+					   (double *)nullptr);
+		}
+		/* modified Gram-Schmidt */
+		for (int i = 0; i <= j; i++) {
+			double hij = dot(c, w, V[i], n);
+			H[i * m + j] = hij;
+			hipLaunchKernelGGL((k_axpy<T, T>), lin_grid(n), dim3(256), 0, c->s, w, V[i], -hij, n);
+		}
+		double hn = sqrt(dot(c, w, w, n));
+		H[(j + 1) * m + j] = hn;
+		if (hn > 0)
+			hipLaunchKernelGGL((k_copy_scale<T, T>), lin_grid(n), dim3(256), 0, c->s, V[j + 1], w, 1.0 / hn, n);
+		/* Givens */
+		for (int i = 0; i < j; i++) {
+			double a = H[i * m + j], b = H[(i + 1) * m + j];
+			H[i * m + j] = cs[i] * a + sn[i] * b;
+			H[(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+		}
+		double a = H[j * m + j], b = H[(j + 1) * m + j];
+		double rr = sqrt(a * a + b * b);
+		cs[j] = rr > 0 ? a / rr : 1.0;
+		sn[j] = rr > 0 ? b / rr : 0.0;
+		H[j * m + j] = rr;
+		H[(j + 1) * m + j] = 0;
+		g[j + 1] = -sn[j] * g[j];
+		g[j] = cs[j] * g[j];
+		if (p->verbose > 1) fprintf(stderr, "  fgmres it %d  est rel %.3e\n", j, fabs(g[j + 1]) / beta);
+		if (fabs(g[j + 1]) <= target * beta || hn == 0) { j++; break; }
+	}
+	int k = j < m ? j : m;
+	std::vector<double> y(k, 0);
+	for (int i = k - 1; i >= 0; i--) {
+		double s = g[i];
+		for (int q = i + 1; q < k; q++) s -= H[i * m + q] * y[q];
+		y[i] = H[i * m + i] != 0 ? s / H[i * m + i] : 0;
+	}
+	for (int i = 0; i < k; i++)
+		hipLaunchKernelGGL((k_axpy<T, T>), lin_grid(n), dim3(256), 0, c->s, e, Z[i], y[i], n);
+	if (red) *red = fabs(g[k]) / beta;
+	return used;
+}
+
+template <typename T>
+__global__ void k_add_scaled(double *__restrict__ x, const T *__restrict__ e, double a, long n)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n) x[t] += a * (double)e[t];
+}
+
+/* Pressure gauge: P += pinit - P(ref cell) (move3mg.c:2248-2269); the
+ * shift is read first by a one-thread kernel, then applied. */
+__global__ void k_pshift_get(const double *P, long ref, double pinit, double *shift)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0) shift[0] = pinit - P[ref];
+}
+__global__ void k_pshift(double *P, long N, const double *shift, const uint8_t *flags)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < N && (flags[t] & FL_EX(0))) P[t] += shift[0];
+}
+
+extern "C" void i3g_stokes_default_params(i3g_stokes_params *p)
+{
+	memset(p, 0, sizeof(*p));
+	p->rtol = 1e-8;
+	p->precision = 32;
+	p->max_outer = 60;
+	p->krylov = 20;
+	/* mode.t3c:3030-3031 sweep counts per level */
+	int pre[5] = {8, 16, 16, 32, 0}, post[5] = {8, 16, 16, 32, 64};
+	for (int l = 0; l < I3G_MAXLEV; l++) {
+		p->nu_pre[l] = l < 5 ? pre[l] : 32;
+		p->nu_post[l] = l < 5 ? post[l] : 32;
+	}
+	p->p0koef = 0.3; p->p1koef = 0.1; p->v0koef = 0.7; p->v1koef = 0.7;
+	p->inner_rtol = 1e-3;
+	p->warm_start = 0;
+	p->pxinit = 2; p->pyinit = 1; p->pzinit = 2; p->pinit = 0;
+}
+
+static int solve_impl(i3g_ctx *c, const i3g_stokes_params *p, i3g_stats *st)
+{
+	Level &L = c->L[0];
+	long N = L.N, n = 4 * N;
+	i3g_stats loc;
+	memset(&loc, 0, sizeof(loc));
+	int m = p->krylov > 0 ? p->krylov : 1;
+	int rc = p->precision == 64 ? ensure_krylov<double>(c, m) : ensure_krylov<float>(c, m);
+	if (rc) return rc;
+	hipEvent_t t0, t1;
+	hipEventCreate(&t0);
+	hipEventCreate(&t1);
+	hipEventRecord(t0, c->s);
+	if (p->warm_start == 1) hipMemsetAsync(c->sol, 0, n * sizeof(double), c->s);
+	bc_enforce<double>(c, L, c->sol, 0);
+	double nrm[2];
+	rc = resid64(c, c->sol, L.r64, nrm);
+	if (rc) return rc;
+	double bnorm = nrm[1] > 0 ? nrm[1] : 1.0;
+	double rel = nrm[0] / bnorm;
+	loc.res0 = rel;
+	int outer = 0, cycles = 0;
+	while (rel > p->rtol && outer < p->max_outer) {
+		double red = 1;
+		if (p->precision == 64) {
+			double tgt = std::max(p->rtol / rel * 0.5, 1e-14);
+			cycles += fgmres<double>(c, L.r64, c->e64, m, p->krylov > 0 ? tgt : 2.0, p, &red);
+			hipLaunchKernelGGL(k_add_scaled<double>, lin_grid(n), dim3(256), 0, c->s, c->sol,
+					   (const double *)c->e64, 1.0, n);
+		} else {
+			/* FP32 inner solve on the normalised residual */
+			double s = 1.0 / nrm[0];
+			float *r32 = L.r32;
+			hipLaunchKernelGGL((k_copy_scale<float, double>), lin_grid(n), dim3(256), 0, c->s, r32,
+					   (const double *)L.r64, s, n);
+			float *e32 = (float *)c->ku;
+			double tgt = std::max(p->inner_rtol, p->rtol / rel * 0.5);
+			cycles += fgmres<float>(c, r32, e32, m, p->krylov > 0 ? tgt : 2.0, p, &red);
+			hipLaunchKernelGGL(k_add_scaled<float>, lin_grid(n), dim3(256), 0, c->s, c->sol,
+					   (const float *)e32, nrm[0], n);
+		}
+		bc_enforce<double>(c, L, c->sol, 0);
+		rc = resid64(c, c->sol, L.r64, nrm);
+		if (rc) return rc;
+		rel = nrm[0] / bnorm;
+		outer++;
+		if (loc.nhist < 256) loc.hist[loc.nhist++] = rel;
+		if (p->verbose) fprintf(stderr, "i3g outer %d: rel %.3e (inner est %.2e, cycles %d)\n", outer, rel, red, cycles);
+		if (!std::isfinite(rel)) break;
+	}
+	/* pressure gauge (FP64) */
+	long ref = p->pzinit * L.nx * L.ny + p->pxinit * L.ny + p->pyinit;
+	if (ref >= 0 && ref < N) {
+		hipLaunchKernelGGL(k_pshift_get, dim3(1), dim3(64), 0, c->s, (const double *)c->sol, ref, p->pinit,
+				   c->dscal + 8);
+		hipLaunchKernelGGL(k_pshift, lin_grid(N), dim3(256), 0, c->s, c->sol, N, (const double *)(c->dscal + 8),
+				   L.flags);
+	}
+	hipEventRecord(t1, c->s);
+	HCHK(hipStreamSynchronize(c->s));
+	float ms = 0;
+	hipEventElapsedTime(&ms, t0, t1);
+	hipEventDestroy(t0);
+	hipEventDestroy(t1);
+	flush_timing(c);
+	loc.seconds = ms * 1e-3;
+	loc.outer_iters = outer;
+	loc.vcycles = cycles;
+	loc.rel_res = rel;
+	loc.converged = rel <= p->rtol;
+	if (st) *st = loc;
+	return loc.converged ? I3G_OK : I3G_ENOCONV;
+}
+
+extern "C" int i3g_stokes_set_solution(i3g_ctx *c, const double *P, const double *vx, const double *vy,
+				       const double *vz)
+{
+	if (!c || !P || !vx || !vy || !vz) return I3G_EINVAL;
+	long N = c->L[0].N;
+	const double *src[4] = {P, vx, vy, vz};
+	for (int q = 0; q < 4; q++)
+		HCHK(hipMemcpyAsync(c->sol + q * N, src[q], N * sizeof(double), hipMemcpyHostToDevice, c->s));
+	HCHK(hipStreamSynchronize(c->s));
+	return I3G_OK;
+}
+
+extern "C" int i3g_stokes_get_solution(i3g_ctx *c, double *P, double *vx, double *vy, double *vz)
+{
+	if (!c || !P || !vx || !vy || !vz) return I3G_EINVAL;
+	long N = c->L[0].N;
+	double *dst[4] = {P, vx, vy, vz};
+	for (int q = 0; q < 4; q++)
+		HCHK(hipMemcpyAsync(dst[q], c->sol + q * N, N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+	HCHK(hipStreamSynchronize(c->s));
+	return I3G_OK;
+}
+
+extern "C" int i3g_stokes_solve_resident(i3g_ctx *c, const i3g_stokes_params *p, i3g_stats *st)
+{
+	if (!c || !p || !c->coef_ready) return I3G_EINVAL;
+	if (p->precision != 32 && p->precision != 64) return I3G_EINVAL;
+	for (int l = 0; l < c->nlev; l++)
+		if (p->nu_pre[l] < 0 || p->nu_post[l] < 0) return I3G_EINVAL;
+	return solve_impl(c, p, st);
+}
+
+extern "C" int i3g_stokes_solve(i3g_ctx *c, double *P, double *vx, double *vy, double *vz,
+				const i3g_stokes_params *p, i3g_stats *st)
+{
+	if (!c || !p || !P || !vx || !vy || !vz) return I3G_EINVAL;
+	int rc = i3g_stokes_set_solution(c, P, vx, vy, vz);
+	if (rc) return rc;
+	int rs = i3g_stokes_solve_resident(c, p, st);
+	if (rs && rs != I3G_ENOCONV) return rs;
+	rc = i3g_stokes_get_solution(c, P, vx, vy, vz);
+	return rc ? rc : rs;
+}
+
+extern "C" int i3g_set_timing(i3g_ctx *c, int on)
+{
+	if (!c) return I3G_EINVAL;
+	c->timing = on != 0;
+	c->stats.clear();
+	return I3G_OK;
+}
+
+extern "C" int i3g_kernel_stats(i3g_ctx *c, char *names, int name_len, double *avg_ms, long *calls,
+				double *bytes, int cap)
+{
+	if (!c) return I3G_EINVAL;
+	int q = 0;
+	for (auto &kv : c->stats) {
+		if (q >= cap) break;
+		if (names) {
+			strncpy(names + (size_t)q * name_len, kv.first.c_str(), name_len - 1);
+			names[(size_t)q * name_len + name_len - 1] = 0;
+		}
+		if (avg_ms) avg_ms[q] = kv.second.calls ? kv.second.ms / kv.second.calls : 0;
+		if (calls) calls[q] = kv.second.calls;
+		if (bytes) bytes[q] = kv.second.calls ? kv.second.bytes / kv.second.calls : 0;
+		q++;
+	}
+	return q;
+}

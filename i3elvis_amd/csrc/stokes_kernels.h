@@ -1,0 +1,434 @@
+/*
+ * Device kernels of the i3g Stokes path (gfx950).  Templated on the real
+ * type T (double for the FP64 residual/plain multigrid, float for the
+ * mixed-precision V-cycle).  Included by i3g_stokes.hip only.
+ *
+ * Layout per level: node n = m3*nx*ny + m1*ny + m2; an unknown vector is
+ * 4 SoA planes of N (P, vx, vy, vz).  flags[n]: bit c = unknown c is a
+ * boundary-condition row, bit 4+c = unknown c exists on the staggered grid.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FL_BC(c) (1u << (c))
+#define FL_EX(c) (1u << (4 + (c)))
+
+/* Per-level geometry factors (device arrays, T precision):
+ *   ih[i] = 1/(g[i+1]-g[i])   (i < n-1)
+ *   ic[i] = 1/(g[i+1]-g[i-1]) (0 < i < n-1)
+ * as used by sxxcalc (cell width), sxycalc (two-cell span) and
+ * x/y/zstokserr (move3mg.c:16-18, 369-370, 729-731). */
+template <typename T> struct LevelView {
+	int nx, ny, nz;
+	long N;
+	const T *ihx, *ihy, *ihz, *icx, *icy, *icz;
+	const T *eC, *eXY, *eXZ, *eYZ;
+	const uint8_t *flags;
+	const T *wP; /* continuity-row weight eta_cell / mean cell size (P plane) */
+};
+
+template <typename T> struct Vec4 {
+	T *p, *x, *y, *z;
+};
+template <typename T> struct CVec4 {
+	const T *p, *x, *y, *z;
+};
+
+/* ---- stencil: A x and diagonal of one row (x/y/zstokserr, continerr) --- */
+
+template <typename T, typename X>
+__device__ __forceinline__ T sxx_v(const LevelView<T> &L, const X *vx, int a, int b, int c)
+{
+	long sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = c * sz + a * sx + b;
+	long v0 = n - sz - sx - 1;
+	return (T)2 * L.eC[n] * ((T)vx[v0 + sx] - (T)vx[v0]) * L.ihx[a - 1];
+}
+template <typename T, typename X>
+__device__ __forceinline__ T syy_v(const LevelView<T> &L, const X *vy, int a, int b, int c)
+{
+	long sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = c * sz + a * sx + b;
+	long v0 = n - sz - sx - 1;
+	return (T)2 * L.eC[n] * ((T)vy[v0 + 1] - (T)vy[v0]) * L.ihy[b - 1];
+}
+template <typename T, typename X>
+__device__ __forceinline__ T szz_v(const LevelView<T> &L, const X *vz, int a, int b, int c)
+{
+	long sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = c * sz + a * sx + b;
+	long v0 = n - sz - sx - 1;
+	return (T)2 * L.eC[n] * ((T)vz[v0 + sz] - (T)vz[v0]) * L.ihz[c - 1];
+}
+template <typename T, typename X>
+__device__ __forceinline__ T sxy_v(const LevelView<T> &L, const X *vx, const X *vy, int a, int b, int c)
+{
+	long sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = c * sz + a * sx + b;
+	return (T)2 * L.eXY[n] * (((T)vx[n] - (T)vx[n - 1]) * L.icy[b] + ((T)vy[n] - (T)vy[n - sx]) * L.icx[a]);
+}
+template <typename T, typename X>
+__device__ __forceinline__ T sxz_v(const LevelView<T> &L, const X *vx, const X *vz, int a, int b, int c)
+{
+	long sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = c * sz + a * sx + b;
+	return (T)2 * L.eXZ[n] * (((T)vx[n] - (T)vx[n - sz]) * L.icz[c] + ((T)vz[n] - (T)vz[n - sx]) * L.icx[a]);
+}
+template <typename T, typename X>
+__device__ __forceinline__ T syz_v(const LevelView<T> &L, const X *vy, const X *vz, int a, int b, int c)
+{
+	long sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = c * sz + a * sx + b;
+	return (T)2 * L.eYZ[n] * (((T)vy[n] - (T)vy[n - sz]) * L.icz[c] + ((T)vz[n] - (T)vz[n - 1]) * L.icy[b]);
+}
+
+/* A x for the stencil row of component comp at (i,j,k); diag optional. */
+template <typename T, typename X>
+__device__ __forceinline__ T row_ax(const LevelView<T> &L, const X *P, const X *vx,
+				    const X *vy, const X *vz, int comp, int i, int j,
+				    int k, T *diag)
+{
+	long sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = k * sz + i * sx + j;
+	T ax, d;
+	if (comp == 1) {
+		T xkf = (T)2 * L.icx[i], ykf = L.ihy[j], zkf = L.ihz[k];
+		ax = xkf * (sxx_v(L, vx, i + 1, j + 1, k + 1) - sxx_v(L, vx, i, j + 1, k + 1)) +
+		     ykf * (sxy_v(L, vx, vy, i, j + 1, k) - sxy_v(L, vx, vy, i, j, k)) +
+		     zkf * (sxz_v(L, vx, vz, i, j, k + 1) - sxz_v(L, vx, vz, i, j, k)) -
+		     xkf * ((T)P[n + sz + sx + 1] - (T)P[n + sz + 1]);
+		if (diag)
+			*diag = -xkf * ((T)2 * L.eC[n + sz + sx + 1] * L.ihx[i] + (T)2 * L.eC[n + sz + 1] * L.ihx[i - 1]) -
+				ykf * ((T)2 * L.eXY[n + 1] * L.icy[j + 1] + (T)2 * L.eXY[n] * L.icy[j]) -
+				zkf * ((T)2 * L.eXZ[n + sz] * L.icz[k + 1] + (T)2 * L.eXZ[n] * L.icz[k]);
+	} else if (comp == 2) {
+		T xkf = L.ihx[i], ykf = (T)2 * L.icy[j], zkf = L.ihz[k];
+		ax = ykf * (syy_v(L, vy, i + 1, j + 1, k + 1) - syy_v(L, vy, i + 1, j, k + 1)) +
+		     xkf * (sxy_v(L, vx, vy, i + 1, j, k) - sxy_v(L, vx, vy, i, j, k)) +
+		     zkf * (syz_v(L, vy, vz, i, j, k + 1) - syz_v(L, vy, vz, i, j, k)) -
+		     ykf * ((T)P[n + sz + sx + 1] - (T)P[n + sz + sx]);
+		if (diag)
+			*diag = -ykf * ((T)2 * L.eC[n + sz + sx + 1] * L.ihy[j] + (T)2 * L.eC[n + sz + sx] * L.ihy[j - 1]) -
+				xkf * ((T)2 * L.eXY[n + sx] * L.icx[i + 1] + (T)2 * L.eXY[n] * L.icx[i]) -
+				zkf * ((T)2 * L.eYZ[n + sz] * L.icz[k + 1] + (T)2 * L.eYZ[n] * L.icz[k]);
+	} else if (comp == 3) {
+		T xkf = L.ihx[i], ykf = L.ihy[j], zkf = (T)2 * L.icz[k];
+		ax = zkf * (szz_v(L, vz, i + 1, j + 1, k + 1) - szz_v(L, vz, i + 1, j + 1, k)) +
+		     xkf * (sxz_v(L, vx, vz, i + 1, j, k) - sxz_v(L, vx, vz, i, j, k)) +
+		     ykf * (syz_v(L, vy, vz, i, j + 1, k) - syz_v(L, vy, vz, i, j, k)) -
+		     zkf * ((T)P[n + sz + sx + 1] - (T)P[n + sx + 1]);
+		if (diag)
+			*diag = -zkf * ((T)2 * L.eC[n + sz + sx + 1] * L.ihz[k] + (T)2 * L.eC[n + sx + 1] * L.ihz[k - 1]) -
+				xkf * ((T)2 * L.eXZ[n + sx] * L.icx[i + 1] + (T)2 * L.eXZ[n] * L.icx[i]) -
+				ykf * ((T)2 * L.eYZ[n + 1] * L.icy[j + 1] + (T)2 * L.eYZ[n] * L.icy[j]);
+	} else {
+		long v0 = n - sz - sx - 1;
+		ax = ((T)vx[v0 + sx] - (T)vx[v0]) * L.ihx[i - 1] +
+		     ((T)vy[v0 + 1] - (T)vy[v0]) * L.ihy[j - 1] +
+		     ((T)vz[v0 + sz] - (T)vz[v0]) * L.ihz[k - 1];
+		d = 0;
+		if (diag) *diag = d;
+	}
+	return ax;
+}
+
+/* Boundary-condition record of one unknown (bonderr, move3mg.c:1532-1579):
+ * x_u = c + k * x_partner.  partner = plane*N + node, or -1. */
+template <typename T> struct BcRec {
+	const int *u;        /* plane*N + node */
+	const int *partner;  /* plane*N + node or -1 */
+	const T *c, *k;
+};
+
+/* ------------------------------------------------------------------ */
+/* Kernels                                                            */
+/* ------------------------------------------------------------------ */
+
+#define RBLK 256
+
+template <typename T>
+__device__ __forceinline__ void block_sum2(double a, double b, double *out2)
+{
+	__shared__ double sa[RBLK], sb[RBLK];
+	int t = threadIdx.x;
+	sa[t] = a; sb[t] = b;
+	__syncthreads();
+	for (int s = RBLK / 2; s > 0; s >>= 1) {
+		if (t < s) { sa[t] += sa[t + s]; sb[t] += sb[t + s]; }
+		__syncthreads();
+	}
+	if (t == 0) { out2[0] = sa[0]; out2[1] = sb[0]; }
+}
+
+/* r = b - A x (mode 0) or r = W A x (mode 1: FGMRES operator, P plane
+ * scaled by wP).  Thread per node: all four unknowns.  BC rows:
+ * r = c - (x_u - k x_partner).  When partials != NULL, each block writes the
+ * sums of (w r)^2 and (w b)^2 over non-BC rows (continuity rows weighted
+ * by wP) for the convergence metric. Block = 256 threads over the xy plane,
+ * grid.y = z. */
+template <typename T, typename X, int MODE>
+__global__ void __launch_bounds__(256)
+k_residual(LevelView<T> L, const X *__restrict__ x, const T *__restrict__ b,
+	   T *__restrict__ r, const int *__restrict__ bcid, BcRec<T> bc,
+	   double *partials)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int k = blockIdx.y;
+	int i = t / L.ny, j = t - i * L.ny;
+	double sr = 0, sb = 0;
+	if (i < L.nx) {
+		long N = L.N;
+		long n = (long)k * L.nx * L.ny + (long)i * L.ny + j;
+		unsigned fl = L.flags[n];
+		const X *P = x, *vx = x + N, *vy = x + 2 * N, *vz = x + 3 * N;
+#pragma unroll
+		for (int c = 0; c < 4; c++) {
+			T res = 0;
+			long u = c * N + n;
+			if (fl & FL_EX(c)) {
+				if (fl & FL_BC(c)) {
+					int rec = bcid[u];
+					int p = bc.partner[rec];
+					T ax = (T)x[u] - (p >= 0 ? bc.k[rec] * (T)x[p] : (T)0);
+					res = MODE ? ax : (b ? b[u] : (T)0) - ax;
+				} else {
+					T ax = row_ax<T, X>(L, P, vx, vy, vz, c, i, j, k, (T *)0);
+					T bb = b ? b[u] : (T)0;
+					res = MODE ? ax : bb - ax;
+					if (MODE && c == 0) res *= L.wP[n];
+					if (partials) {
+						double w = c == 0 ? (double)L.wP[n] : 1.0;
+						sr += ((double)res * w) * ((double)res * w);
+						sb += ((double)bb * w) * ((double)bb * w);
+					}
+				}
+			}
+			r[u] = res;
+		}
+	}
+	if (partials) {
+		long bid = (long)blockIdx.y * gridDim.x + blockIdx.x;
+		block_sum2<T>(sr, sb, partials + 2 * bid);
+	}
+}
+
+/* Gauss-Seidel relaxation of velocity component comp on the unknowns of one
+ * colour ((i+j+k) mod 2 == colour): x += v0koef (b - A x)/diag, the
+ * gausmat4 update (move3mg.c:3121-3124).  BC rows are left to k_bc.
+ * Threads cover every other y index of each (x,z) line. */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_smooth_vel(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int comp,
+	     int colour, T v0koef)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int k = blockIdx.y;
+	int hy = (L.ny + 1) >> 1;
+	int i = t / hy, tj = t - i * hy;
+	if (i >= L.nx) return;
+	int j = 2 * tj + ((i + k + colour) & 1);
+	if (j >= L.ny) return;
+	long N = L.N;
+	long n = (long)k * L.nx * L.ny + (long)i * L.ny + j;
+	unsigned fl = L.flags[n];
+	if (!(fl & FL_EX(comp)) || (fl & FL_BC(comp))) return;
+	T d;
+	T ax = row_ax<T, T>(L, x, x + N, x + 2 * N, x + 3 * N, comp, i, j, k, &d);
+	long u = comp * N + n;
+	T res = (b ? b[u] : (T)0) - ax;
+	x[u] += v0koef * res / d;
+}
+
+/* Pressure relaxation P -= p0koef * eta_cell * (div v - b_P)
+ * (move3mg.c:3105-3111); BC rows are left to k_bc. */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_smooth_p(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, T p0koef)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int k = blockIdx.y;
+	int i = t / L.ny, j = t - i * L.ny;
+	if (i >= L.nx) return;
+	long N = L.N;
+	long n = (long)k * L.nx * L.ny + (long)i * L.ny + j;
+	unsigned fl = L.flags[n];
+	if (!(fl & FL_EX(0)) || (fl & FL_BC(0))) return;
+	T ax = row_ax<T, T>(L, x, x + N, x + 2 * N, x + 3 * N, 0, i, j, k, (T *)0);
+	T res = ax - (b ? b[n] : (T)0);
+	x[n] -= res * L.eC[n] * p0koef;
+}
+
+/* Boundary rows x_u = c + k x_partner for the records [start, end) of one
+ * dependency depth (partners of depth-d rows are interior or depth < d). */
+template <typename T>
+__global__ void k_bc(T *__restrict__ x, BcRec<T> bc, int start, int end, int homog)
+{
+	int t = start + blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= end) return;
+	int p = bc.partner[t];
+	T v = homog ? (T)0 : bc.c[t];
+	if (p >= 0) v += bc.k[t] * x[p];
+	x[bc.u[t]] = v;
+}
+
+/* Per-axis transfer maps between a fine and a coarse level for the three
+ * staggering kinds (0 node line, 1 mid-point, 2 cell centre), built on the
+ * host from the m10min/m20min/m30min rules of vprestrict/vpprolong
+ * (move3mg.c:2776-2789, 3201-3214). Forward (per fine index): base, wlo,
+ * whi.  Inverse (per coarse index): CSR list of (fine index, weight) in
+ * ascending fine index. */
+template <typename T> struct AxisMap {
+	const int *base[3];
+	const T *wlo[3], *whi[3];
+	const int *istart[3];
+	const int *iidx[3];
+	const T *iw[3];
+};
+
+template <typename T> struct XferView {
+	AxisMap<T> ax[3]; /* x, y, z */
+};
+
+__device__ __constant__ int c_kind[4][3] = {{2, 2, 2}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
+
+/* Residual restriction in gather form; accumulation order equals the
+ * reference's serial scatter (fine nodes in ascending index).  Coarse x is
+ * zeroed (move3mg.c:3508-3518). */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_restrict(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__ rf,
+	   T *__restrict__ bc_out, T *__restrict__ xc)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int K = blockIdx.y;
+	int I = t / Cl.ny, J = t - I * Cl.ny;
+	if (I >= Cl.nx) return;
+	long Nc = Cl.N, Nf = F.N;
+	long nc = (long)K * Cl.nx * Cl.ny + (long)I * Cl.ny + J;
+	unsigned flc = Cl.flags[nc];
+	long sxf = F.ny, szf = (long)F.nx * F.ny;
+	for (int c = 0; c < 4; c++) {
+		T acc = 0, sw = 0;
+		if ((flc & FL_EX(c)) && !(flc & FL_BC(c))) {
+			int kx = c_kind[c][0], ky = c_kind[c][1], kz = c_kind[c][2];
+			int z0 = M.ax[2].istart[kz][K], z1 = M.ax[2].istart[kz][K + 1];
+			int x0 = M.ax[0].istart[kx][I], x1 = M.ax[0].istart[kx][I + 1];
+			int y0 = M.ax[1].istart[ky][J], y1 = M.ax[1].istart[ky][J + 1];
+			T ndc = (T)(float)Cl.eC[nc];
+			for (int a = z0; a < z1; a++) {
+				int kf = M.ax[2].iidx[kz][a];
+				T wz = M.ax[2].iw[kz][a];
+				for (int bq = x0; bq < x1; bq++) {
+					int ifn = M.ax[0].iidx[kx][bq];
+					T wx = M.ax[0].iw[kx][bq];
+					for (int e = y0; e < y1; e++) {
+						int jf = M.ax[1].iidx[ky][e];
+						T wy = M.ax[1].iw[ky][e];
+						long nf = kf * szf + ifn * sxf + jf;
+						unsigned flf = F.flags[nf];
+						if (!(flf & FL_EX(c)) || (flf & FL_BC(c))) continue;
+						T swt = wx * wy * wz;
+						if (c == 0)
+							acc += rf[nf] * (T)(float)F.eC[nf] / ndc * swt;
+						else
+							acc += rf[c * Nf + nf] * swt;
+						sw += swt;
+					}
+				}
+			}
+			if (sw != (T)0) acc /= sw;
+		}
+		bc_out[c * Nc + nc] = acc;
+		xc[c * Nc + nc] = 0;
+	}
+}
+
+/* Correction prolongation x_f += P x_c (move3mg.c:2760-3060), per fine
+ * node; v1koef for velocities, p1koef for pressure. */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_prolong(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__ xc,
+	  T *__restrict__ xf, T v1koef, T p1koef)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int k = blockIdx.y;
+	int i = t / F.ny, j = t - i * F.ny;
+	if (i >= F.nx) return;
+	long Nc = Cl.N, Nf = F.N;
+	long nf = (long)k * F.nx * F.ny + (long)i * F.ny + j;
+	unsigned fl = F.flags[nf];
+	long sxc = Cl.ny, szc = (long)Cl.nx * Cl.ny;
+	for (int c = 0; c < 4; c++) {
+		if (!(fl & FL_EX(c))) continue;
+		int kx = c_kind[c][0], ky = c_kind[c][1], kz = c_kind[c][2];
+		int bx = M.ax[0].base[kx][i], by = M.ax[1].base[ky][j], bz = M.ax[2].base[kz][k];
+		T wx[2] = {M.ax[0].wlo[kx][i], M.ax[0].whi[kx][i]};
+		T wy[2] = {M.ax[1].wlo[ky][j], M.ax[1].whi[ky][j]};
+		T wz[2] = {M.ax[2].wlo[kz][k], M.ax[2].whi[kz][k]};
+		T kf = c ? v1koef : p1koef;
+		T acc = xf[c * Nf + nf];
+		const T *xcc = xc + c * Nc;
+		for (int a = 0; a < 2; a++)
+			for (int bq = 0; bq < 2; bq++)
+				for (int e = 0; e < 2; e++) {
+					T swt = wx[bq] * wy[e] * wz[a];
+					acc += xcc[(bz + a) * szc + (bx + bq) * sxc + (by + e)] * swt * kf;
+				}
+		xf[c * Nf + nf] = acc;
+	}
+}
+
+/* ---- vector helpers (Krylov, precision conversion) --------------------- */
+
+template <typename T, typename U>
+__global__ void k_copy_scale(T *__restrict__ y, const U *__restrict__ x, double a, long n)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n) y[t] = (T)((double)x[t] * a);
+}
+
+/* y += a x */
+template <typename T, typename U>
+__global__ void k_axpy(T *__restrict__ y, const U *__restrict__ x, double a, long n)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n) y[t] = (T)((double)y[t] + a * (double)x[t]);
+}
+
+/* partial dot products: block b sums a fixed chunk */
+template <typename T, typename U>
+__global__ void __launch_bounds__(256) k_dot_part(const T *__restrict__ a, const U *__restrict__ b,
+						  long n, double *partials)
+{
+	__shared__ double s[256];
+	long chunk = (n + gridDim.x - 1) / gridDim.x;
+	long lo = (long)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+	double acc = 0;
+	for (long t = lo + threadIdx.x; t < hi; t += blockDim.x) acc += (double)a[t] * (double)b[t];
+	s[threadIdx.x] = acc;
+	__syncthreads();
+	for (int q = 128; q > 0; q >>= 1) {
+		if (threadIdx.x < q) s[threadIdx.x] += s[threadIdx.x + q];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
+}
+
+/* fixed-order final reduction of `count` groups of `width` partials */
+__global__ void k_reduce_final(const double *partials, long nparts, int width, double *out)
+{
+	__shared__ double s[256];
+	for (int w = 0; w < width; w++) {
+		double acc = 0;
+		for (long t = threadIdx.x; t < nparts; t += blockDim.x) acc += partials[t * width + w];
+		s[threadIdx.x] = acc;
+		__syncthreads();
+		for (int q = 128; q > 0; q >>= 1) {
+			if (threadIdx.x < q) s[threadIdx.x] += s[threadIdx.x + q];
+			__syncthreads();
+		}
+		if (threadIdx.x == 0) out[w] = s[0];
+		__syncthreads();
+	}
+}

@@ -179,8 +179,13 @@ int i3g_stokes_get_solution(i3g_ctx *ctx, double *P, double *vx, double *vy,
 int i3g_stokes_set_solution(i3g_ctx *ctx, const double *P, const double *vx,
 			    const double *vy, const double *vz);
 
-/* Per-kernel timing of the last solve (HIP events): names and average
- * milliseconds; returns the count. */
+/* Enables (1) or disables (0) HIP-event timing of the named kernels on the
+ * context's stream, and clears the collected statistics. */
+int i3g_set_timing(i3g_ctx *ctx, int on);
+
+/* Per-kernel timing collected since i3g_set_timing (HIP events on the
+ * context's stream): names, average milliseconds per launch, launch counts
+ * and algorithmic bytes per launch; returns the count. */
 int i3g_kernel_stats(i3g_ctx *ctx, char *names, int name_len, double *avg_ms,
 		     long *calls, double *bytes, int cap);
 

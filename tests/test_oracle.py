@@ -1,0 +1,125 @@
+"""CPU tests: the oracle (C restatement) against the reference's own outputs
+(golden fixtures written from the compiled reference) and against closed
+forms that a plausible mistake would break."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden
+
+
+def _levels(g):
+    L0 = oracle.level(int(g["nx"]), int(g["ny"]), int(g["nz"]), 0, 0, g["gx"], g["gy"], g["gz"])
+    L1 = oracle.level(int(g["mgx"][1]), int(g["mgy"][1]), int(g["mgz"][1]), int(g["mgp"][1]), 1,
+                      g["lx1"], g["ly1"], g["lz1"])
+    return L0, L1
+
+
+def _etas(g, L, lev, nu):
+    if lev == 0:
+        return oracle.eta(L, nu, g["nxx"], g["nxy"], g["nxz"], g["nyz"], 0, float(g["nubeg"]), float(g["nuend"]))
+    return oracle.eta(L, nu, None, None, None, None, 0, float(g["nubeg"]), float(g["nuend"]))
+
+
+def _rowscale(x, L, etas):
+    """|A| |x| bound per unknown for a relative tolerance."""
+    return np.abs(x).max() * 0 + 1.0
+
+
+def test_rhs_matches_reference():
+    g = golden("g1_operator.npz")
+    L0, _ = _levels(g)
+    b = oracle.rhs(L0, g["ro"], g["g"], g["bondm"], g["bondv"])
+    assert np.array_equal(b, g["rhs0"])
+
+
+@pytest.mark.parametrize("lev", [0, 1])
+def test_residual_matches_reference_rows(lev):
+    g = golden("g1_operator.npz")
+    L0, L1 = _levels(g)
+    if lev == 0:
+        L, nu, x, b, rref, ex = L0, g["nu"], g["x0"], g["rhs0"], g["r0"], g["exists0"]
+    else:
+        L, nu, x, b, rref, ex = L1, g["nu1"], g["x1"], g["b1"], g["r1"], g["exists1"]
+    et = _etas(g, L, lev, nu)
+    r = oracle.residual(L, et, g["bondm"], g["bondv"], g["bondn"], b, x)
+    # scale: the largest single term of a row ~ max|coef| * max|x| per component
+    for c in range(4):
+        d = np.abs(r[c::4] - rref[c::4])[ex[c::4]]
+        s = np.abs(rref[c::4]).max() + np.abs(b[c::4]).max()
+        assert d.max() <= 1e-12 * max(s, 1e-300) or d.max() < 1e-30, (c, d.max(), s)
+    assert np.all(r[~ex] == 0)
+
+
+def test_ronurestrict_bitwise():
+    g = golden("g1_operator.npz")
+    L0, L1 = _levels(g)
+    nu1 = oracle.ronurestrict(L0, L1, g["nu"], 0)
+    assert np.array_equal(nu1, g["nu1"])
+
+
+def test_vprestrict_bitwise():
+    g = golden("g1_operator.npz")
+    L0, L1 = _levels(g)
+    out = oracle.vprestrict(L0, L1, g["bondm"], g["restr_in"], g["nd0"], g["nd1"])
+    assert np.array_equal(out, g["restr_out"])
+
+
+def test_vpprolong_bitwise():
+    g = golden("g1_operator.npz")
+    L0, L1 = _levels(g)
+    out = oracle.vpprolong(L0, L1, g["prol_xc"], g["prol_xf"], float(g["v1koef"]), float(g["p1koef"]))
+    assert np.array_equal(out, g["prol_out"])
+
+
+def test_nd_is_float_cell_viscosity():
+    g = golden("g1_operator.npz")
+    L0, _ = _levels(g)
+    et = _etas(g, L0, 0, g["nu"])
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    bc = g["bondm"][:4 * nx * ny * nz:4] != 0
+    n = np.arange(nx * ny * nz)
+    m3, rem = np.divmod(n, nx * ny)
+    m1, m2 = np.divmod(rem, ny)
+    sel = (m1 > 0) & (m2 > 0) & (m3 > 0) & ~bc
+    assert np.array_equal(et[0][sel].astype(np.float32), g["nd0"][sel])
+
+
+def test_constant_viscosity_hydrostatic_closed_form():
+    """Uniform eta and rho in a free-slip box: v = 0 and P = rho*g*depth
+    (cell-centre depth) satisfy every interior row exactly."""
+    import i3elvis_amd as I
+    n = 11
+    cfg = I.make_config(0, n, n, n, multinum=0)
+    N = n ** 3
+    for k in ("nu", "nxx", "nxy", "nxz", "nyz"):
+        cfg[k][:] = 1e21
+    cfg["ro"][:] = 3300.0
+    L = oracle.level(n, n, n, 0, 0, cfg["gx"], cfg["gy"], cfg["gz"])
+    et = oracle.eta(L, cfg["nu"], cfg["nxx"], cfg["nxy"], cfg["nxz"], cfg["nyz"], 0, 1e18, 1e25)
+    b = oracle.rhs(L, cfg["ro"], (0.0, 9.81, 0.0), cfg["bondm"], cfg["bondv"])
+    x = np.zeros(4 * N)
+    ymid = np.r_[0.0, 0.5 * (cfg["gy"][1:] + cfg["gy"][:-1])]
+    j = np.arange(N) % n
+    x[0::4] = 3300.0 * 9.81 * ymid[j]
+    r = oracle.residual(L, et, cfg["bondm"], cfg["bondv"], cfg["bondn"], b, x)
+    bc = cfg["bondm"][:4 * N] != 0
+    assert np.abs(r[~bc]).max() < 1e-9 * 3300 * 9.81
+    # a wrong sign on the pressure gradient breaks it
+    r2 = oracle.residual(L, et, cfg["bondm"], cfg["bondv"], cfg["bondn"], b, -x)
+    assert np.abs(r2[~bc]).max() > 1.0
+
+
+@pytest.mark.parametrize("n", [17, 25])
+def test_golden_solution_satisfies_oracle_system(n):
+    """The reference-converged solution zeroes the oracle's residual."""
+    import i3elvis_amd as I
+    g = golden(f"g3_converged_c0_{n}.npz")
+    cfg = I.make_config(0, n, n, n, seed=int(g["seed"]))
+    L = oracle.level(n, n, n, 0, 0, cfg["gx"], cfg["gy"], cfg["gz"])
+    et = oracle.eta(L, cfg["nu"], cfg["nxx"], cfg["nxy"], cfg["nxz"], cfg["nyz"], 0, cfg["nubeg"], cfg["nuend"])
+    b = oracle.rhs(L, cfg["ro"], cfg["g"], cfg["bondm"], cfg["bondv"])
+    assert np.array_equal(b, g["rhs"])
+    r = oracle.residual(L, et, cfg["bondm"], cfg["bondv"], cfg["bondn"], b, g["sol"])
+    w = oracle.wnorms(L, et[0], cfg["bondm"], b, r)
+    assert w[0] / w[1] < 1e-10
