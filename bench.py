@@ -114,8 +114,10 @@ def main():
     ap.add_argument("--n", type=int, default=257)
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--krylov", type=int, default=20)
+    ap.add_argument("--krylov", type=int, default=200)
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--profile-vcycles", type=int, default=0,
+                    help="profiling run: one mixed-precision solve capped at this many V-cycles, no JSON metric")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
@@ -135,11 +137,18 @@ def main():
     S.set_coefficients()
     z = np.zeros(n ** 3)
 
-    def run(prec, timing=False):
+    def run(prec, timing=False, cap=2000):
         S.set_solution(z, z, z, z)
         S.set_timing(timing)
-        st = S.solve(precision=prec, rtol=1e-8, krylov=a.krylov, max_outer=80, verbose=a.verbose)
+        st = S.solve(check_conv=cap >= 2000, precision=prec, rtol=1e-8, krylov=a.krylov, max_outer=20,
+                     max_vcycles=cap, verbose=a.verbose)
         return st
+
+    if a.profile_vcycles:
+        st = run(32, timing=False, cap=a.profile_vcycles)
+        print(f"# profile run: {st.vcycles} V-cycles in {st.seconds:.3f}s rel={st.rel_res:.2e}", file=sys.stderr)
+        S.close()
+        return
 
     # warmup + timed, mixed precision (the product path)
     for _ in range(a.warmup):

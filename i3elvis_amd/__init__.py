@@ -68,7 +68,7 @@ class Setup(C.Structure):
 
 class StokesParams(C.Structure):
     _fields_ = [("rtol", C.c_double), ("precision", C.c_int), ("max_outer", C.c_int),
-                ("krylov", C.c_int), ("nu_pre", C.c_int * MAXLEV), ("nu_post", C.c_int * MAXLEV),
+                ("krylov", C.c_int), ("max_vcycles", C.c_int), ("nu_pre", C.c_int * MAXLEV), ("nu_post", C.c_int * MAXLEV), ("nu_coarse", C.c_int),
                 ("p0koef", C.c_double), ("p1koef", C.c_double), ("v0koef", C.c_double),
                 ("v1koef", C.c_double), ("inner_rtol", C.c_double), ("warm_start", C.c_int),
                 ("pxinit", C.c_long), ("pyinit", C.c_long), ("pzinit", C.c_long),

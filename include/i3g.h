@@ -139,9 +139,11 @@ typedef struct {
 	double rtol;          /* relative weighted residual target (1e-8) */
 	int precision;        /* 64: plain FP64 multigrid; 32: FP32 V-cycles
 				 inside FP64 defect correction */
-	int max_outer;        /* outer iteration cap */
-	int krylov;           /* Krylov restart length (0: stationary cycles) */
+	int max_outer;        /* outer (FP64 refinement) iteration cap */
+	int krylov;           /* GCR direction storage (directions kept) */
+	int max_vcycles;      /* V-cycle cap per outer iteration */
 	int nu_pre[I3G_MAXLEV], nu_post[I3G_MAXLEV]; /* sweeps per level */
+	int nu_coarse;        /* sweeps on the coarsest level */
 	double p0koef, p1koef, v0koef, v1koef;       /* mode.t3c:3032-3036 */
 	double inner_rtol;    /* mixed precision: inner reduction per outer */
 	int warm_start;       /* 0: start from given fields, 1: zero */
