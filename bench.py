@@ -114,8 +114,10 @@ def main():
     ap.add_argument("--n", type=int, default=257)
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--krylov", type=int, default=200)
+    ap.add_argument("--krylov", type=int, default=400)
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="copy 1 GiB three times with a plain device copy (HBM counter calibration)")
     ap.add_argument("--profile-vcycles", type=int, default=0,
                     help="profiling run: one mixed-precision solve capped at this many V-cycles, no JSON metric")
     a = ap.parse_args()
@@ -129,6 +131,14 @@ def main():
 
     import i3elvis_amd as I
     assert torch.cuda.is_available(), "bench needs a GPU"
+    if a.copy_calibration:
+        x = torch.ones(2 ** 28, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print("# copied 1 GiB three times", file=sys.stderr)
+        return
     n = a.n
     t = time.time()
     cfg = I.make_config(2, n, n, n, seed=2)

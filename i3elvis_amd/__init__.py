@@ -68,7 +68,7 @@ class Setup(C.Structure):
 
 class StokesParams(C.Structure):
     _fields_ = [("rtol", C.c_double), ("precision", C.c_int), ("max_outer", C.c_int),
-                ("krylov", C.c_int), ("max_vcycles", C.c_int), ("nu_pre", C.c_int * MAXLEV), ("nu_post", C.c_int * MAXLEV), ("nu_coarse", C.c_int),
+                ("krylov", C.c_int), ("max_vcycles", C.c_int), ("nu_pre", C.c_int * MAXLEV), ("nu_post", C.c_int * MAXLEV), ("nu_coarse", C.c_int), ("inner_m", C.c_int), ("inner_tol", C.c_double),
                 ("p0koef", C.c_double), ("p1koef", C.c_double), ("v0koef", C.c_double),
                 ("v1koef", C.c_double), ("inner_rtol", C.c_double), ("warm_start", C.c_int),
                 ("pxinit", C.c_long), ("pyinit", C.c_long), ("pzinit", C.c_long),
@@ -222,3 +222,107 @@ class Stokes:
             nm = names.raw[q * nl:(q + 1) * nl].split(b"\0", 1)[0].decode()
             out[nm] = dict(avg_ms=float(ms[q]), calls=int(calls[q]), bytes=float(by[q]))
         return out
+
+
+# ---- markers ----------------------------------------------------------------
+
+ROCK_FIELDS = ("nu0", "dh", "dv", "n0", "n1", "ro", "bro", "aro", "cp", "kt", "kf", "kp", "ht")
+
+
+class Rock(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ROCK_FIELDS]
+
+
+class MkParams(C.Structure):
+    _fields_ = [("nubeg", C.c_double), ("nuend", C.c_double), ("nukoef", C.c_double), ("densimod", C.c_int),
+                ("timestep", C.c_double), ("vdeep", C.c_double), ("nudeep", C.c_double), ("viscmod", C.c_int)]
+
+
+def rocks_array(rocks):
+    arr = (Rock * len(rocks))()
+    for i, r in enumerate(rocks):
+        vals = r if not isinstance(r, dict) else [r.get(f, 0.0) for f in ROCK_FIELDS]
+        for f, v in zip(ROCK_FIELDS, vals):
+            setattr(arr[i], f, float(v))
+    return arr
+
+
+class Markers:
+    """Device marker set (i3g_markers_create .. destroy)."""
+
+    def __init__(self, gx, gy, gz, capacity):
+        self._g = [np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz)]
+        self.nx, self.ny, self.nz = (len(a) for a in self._g)
+        self.N = self.nx * self.ny * self.nz
+        self.h = C.c_void_p()
+        check(lib().i3g_markers_create(C.c_long(self.nx), C.c_long(self.ny), C.c_long(self.nz),
+                                       *[_p(a) for a in self._g], C.c_long(capacity), C.byref(self.h)),
+              "markers_create")
+        lib().i3g_markers_count.restype = C.c_long
+
+    def close(self):
+        if self.h:
+            lib().i3g_markers_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n(self):
+        return lib().i3g_markers_count(self.h)
+
+    def upload(self, x, y, z, tk, eta, mtype, ids=None):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (x, y, z, tk, eta)]
+        t = np.ascontiguousarray(mtype, np.int8)
+        i = None if ids is None else np.ascontiguousarray(ids, np.int32)
+        check(lib().i3g_markers_upload(self.h, C.c_long(len(arrs[0])), *[_p(a) for a in arrs], _p(t), _p(i)),
+              "markers_upload")
+
+    def download(self):
+        n = self.n
+        out = [np.zeros(n, np.float32) for _ in range(5)]
+        t = np.zeros(n, np.int8)
+        ids = np.zeros(n, np.int32)
+        check(lib().i3g_markers_download(self.h, *[_p(a) for a in out], _p(t), _p(ids)), "markers_download")
+        return dict(x=out[0], y=out[1], z=out[2], tk=out[3], eta=out[4], type=t, id=ids)
+
+    def cell_index(self):
+        n = self.n
+        out = [np.zeros(n, np.int32) for _ in range(4)]
+        check(lib().i3g_markers_cell_index(self.h, *[_p(a) for a in out]), "markers_cell_index")
+        return out
+
+    def sort(self):
+        order = np.zeros(self.n, np.int64)
+        check(lib().i3g_markers_sort(self.h, _p(order)), "markers_sort")
+        return order
+
+    def to_nodes(self, rocks, pr, fld, nubeg, nuend, nukoef=0.0, densimod=1, timestep=1.0, vdeep=1e30,
+                 nudeep=1e30):
+        fld = np.array(fld, np.float32, copy=True)
+        pr = np.ascontiguousarray(pr, np.float64)
+        prm = MkParams(nubeg, nuend, nukoef, densimod, timestep, vdeep, nudeep, 0)
+        ra = rocks_array(rocks)
+        check(lib().i3g_markers_to_nodes(self.h, ra, C.c_int(len(rocks)), C.byref(prm), _p(pr),
+                                         *[_p(fld[q]) for q in range(11)]), "markers_to_nodes")
+        return fld
+
+    def set_tbc(self, bondm, bondv, bondn):
+        """Temperature boundary rows from the reference-layout tables (bondm
+        over all levels or just level 0's nine blocks); None switches off."""
+        if bondm is None:
+            check(lib().i3g_markers_set_tbc(self.h, None, None, None, C.c_long(0)), "markers_set_tbc")
+            return
+        bt = np.ascontiguousarray(np.asarray(bondm)[8 * self.N:9 * self.N], np.int32)
+        bv = np.ascontiguousarray(bondv, np.float32)
+        bn = np.ascontiguousarray(bondn, np.int32)
+        self._tbc = (bt, bv, bn)
+        check(lib().i3g_markers_set_tbc(self.h, _p(bt), _p(bv), _p(bn), C.c_long(len(bn))), "markers_set_tbc")
+
+    def advect_rk4(self, dt, vx, vy, vz):
+        v = [np.ascontiguousarray(a, np.float32) for a in (vx, vy, vz)]
+        check(lib().i3g_markers_advect_rk4(self.h, C.c_double(dt), *[_p(a) for a in v]), "markers_rk4")

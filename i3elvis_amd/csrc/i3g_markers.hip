@@ -1,0 +1,825 @@
+/*
+ * i3g markers on MI355X: cell search, deterministic cell sort, marker ->
+ * node transfer (gather form of the reference's ronurecalc scatter) and RK4
+ * advection.  C ABI in include/i3g.h.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/i3g.h"
+
+/* The reference evaluates every product and sum with its own rounding (no
+ * fused multiply-add on its x86-64 build); keep that for bit parity. */
+#pragma clang fp contract(off)
+
+#define MCHK(x)                                                                \
+	do {                                                                   \
+		hipError_t e_ = (x);                                           \
+		if (e_ != hipSuccess) {                                        \
+			fprintf(stderr, "i3g markers: HIP error %s at %s:%d\n", \
+				hipGetErrorString(e_), __FILE__, __LINE__);    \
+			return I3G_EDEVICE;                                    \
+		}                                                              \
+	} while (0)
+
+#define MAXROCK 64
+
+struct i3g_markers {
+	hipStream_t s = nullptr;
+	int nx = 0, ny = 0, nz = 0;
+	long N = 0;              /* nodes */
+	long cap = 0, n = 0;     /* marker capacity / count */
+	double *gx = nullptr, *gy = nullptr, *gz = nullptr;
+	/* SoA marker state (current order) and a second buffer for permuting */
+	float *x = nullptr, *y = nullptr, *z = nullptr, *tk = nullptr, *eta = nullptr;
+	signed char *type = nullptr;
+	int *id = nullptr;
+	float *x2 = nullptr, *y2 = nullptr, *z2 = nullptr, *tk2 = nullptr, *eta2 = nullptr;
+	signed char *type2 = nullptr;
+	int *id2 = nullptr;
+	int *key = nullptr, *c1 = nullptr, *c2 = nullptr, *c3 = nullptr;
+	int *perm = nullptr;     /* sorted position -> previous position */
+	/* cell bins over nodes: start[N+1], count/cursor[N] */
+	int *cnt = nullptr, *start = nullptr, *cur = nullptr;
+	int *bsum = nullptr;     /* scan block sums */
+	int sorted = 0;
+	/* per-marker properties for the node transfer */
+	double *pnu = nullptr, *pro = nullptr, *pcp = nullptr, *pkt = nullptr, *pht = nullptr, *pbb = nullptr;
+	double *ptk = nullptr;
+	uint8_t *pin = nullptr;
+	/* node-field staging */
+	double *pr = nullptr;
+	float *fld = nullptr;    /* 11 x N float in/out fields */
+	float *vel = nullptr;    /* 3 x N velocities */
+	i3g_rock *rocks = nullptr;
+	int *bad = nullptr;
+	/* temperature boundary rows of level 0 (tkrecalc): per node partner
+	 * (-2 none, -1 constant only), constant and coefficient */
+	int *tbc_p = nullptr;
+	float *tbc_c = nullptr, *tbc_k = nullptr, *tk1 = nullptr;
+	int has_tbc = 0;
+};
+
+/* ------------------------------------------------------------------ */
+/* Cell search                                                        */
+/* ------------------------------------------------------------------ */
+
+/* largest i with g[i] <= v (binary search as m1serch, mark3mg.c:1018-1037),
+ * clamped to n-2 */
+__device__ __forceinline__ int serch(const double *g, int n, double v)
+{
+	int lo = 0, hi = n - 1, mid;
+	do {
+		mid = (lo + hi) / 2;
+		if (g[mid] > v) hi = mid; else lo = mid;
+	} while (hi - lo > 1);
+	if (lo > n - 2) lo = n - 2;
+	return lo;
+}
+
+__global__ void k_cell_index(long n, const float *__restrict__ x, const float *__restrict__ y,
+			     const float *__restrict__ z, const double *gx, const double *gy, const double *gz,
+			     int nx, int ny, int nz, int *key, int *c1, int *c2, int *c3, int *cnt)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	double a = x[t], b = y[t], c = z[t];
+	if (a < 0) a = 0;
+	if (a > xs) a = xs;
+	if (b < 0) b = 0;
+	if (b > ys) b = ys;
+	if (c < 0) c = 0;
+	if (c > zs) c = zs;
+	int m1 = serch(gx, nx, a), m2 = serch(gy, ny, b), m3 = serch(gz, nz, c);
+	int k = m3 * nx * ny + m1 * ny + m2;
+	key[t] = k;
+	if (c1) { c1[t] = m1; c2[t] = m2; c3[t] = m3; }
+	if (cnt) atomicAdd(&cnt[k], 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* Counting sort by cell, then by id within a cell                    */
+/* ------------------------------------------------------------------ */
+
+#define SCAN_B 1024
+
+/* exclusive scan of cnt[0..N) into start[0..N], three passes */
+__global__ void k_scan_local(const int *cnt, int *start, long N, int *bsum)
+{
+	__shared__ int s[SCAN_B];
+	long base = (long)blockIdx.x * SCAN_B;
+	int t = threadIdx.x;
+	int v = base + t < N ? cnt[base + t] : 0;
+	s[t] = v;
+	__syncthreads();
+	for (int off = 1; off < SCAN_B; off <<= 1) {
+		int a = t >= off ? s[t - off] : 0;
+		__syncthreads();
+		s[t] += a;
+		__syncthreads();
+	}
+	if (base + t < N) start[base + t] = s[t] - v;
+	if (t == SCAN_B - 1) bsum[blockIdx.x] = s[t];
+}
+
+__global__ void k_scan_bsum(int *bsum, long nb)
+{
+	/* one block, sequential over chunks of 1024 */
+	__shared__ int s[SCAN_B];
+	__shared__ int carry;
+	if (threadIdx.x == 0) carry = 0;
+	__syncthreads();
+	for (long base = 0; base < nb; base += SCAN_B) {
+		int t = threadIdx.x;
+		int v = base + t < nb ? bsum[base + t] : 0;
+		s[t] = v;
+		__syncthreads();
+		for (int off = 1; off < SCAN_B; off <<= 1) {
+			int a = t >= off ? s[t - off] : 0;
+			__syncthreads();
+			s[t] += a;
+			__syncthreads();
+		}
+		if (base + t < nb) bsum[base + t] = s[t] - v + carry;
+		__syncthreads();
+		if (t == SCAN_B - 1) carry += s[t];
+		__syncthreads();
+	}
+}
+
+__global__ void k_scan_add(int *start, long N, const int *bsum, long total_slot)
+{
+	long i = (long)blockIdx.x * SCAN_B + threadIdx.x;
+	if (i < N) start[i] += bsum[blockIdx.x];
+	(void)total_slot;
+}
+
+__global__ void k_fill(long n, const int *key, const int *start, int *cur, int *perm)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	int k = key[t];
+	int slot = atomicAdd(&cur[k], 1);
+	perm[start[k] + slot] = (int)t;
+}
+
+/* within each cell, order the marker positions by marker id (insertion
+ * sort; cells hold a few tens of markers) */
+__global__ void k_cellsort(long N, const int *start, const int *cnt, int *perm, const int *id)
+{
+	long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= N) return;
+	int s = start[c], e = s + cnt[c];
+	for (int i = s + 1; i < e; i++) {
+		int v = perm[i], vid = id[v];
+		int j = i - 1;
+		while (j >= s && id[perm[j]] > vid) { perm[j + 1] = perm[j]; j--; }
+		perm[j + 1] = v;
+	}
+}
+
+template <typename T> __global__ void k_gather(long n, const int *perm, const T *src, T *dst)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n) dst[t] = src[perm[t]];
+}
+
+/* ------------------------------------------------------------------ */
+/* Marker properties (ronurecalc per-marker part)                     */
+/* ------------------------------------------------------------------ */
+
+/* P at a point by prcalc1 (mark3mg.c:2917-3001): trilinear in cell
+ * centres, cell indices from the (already clamped) search. */
+__device__ double pr_interp(const double *pr, const double *gx, const double *gy, const double *gz, int nx,
+			    int ny, int nz, double x, double y, double z, int m1, int m2, int m3)
+{
+	int a = m1, b = m2, c = m3;
+	if (x > (gx[a] + gx[a + 1]) / 2.0) a += 1;
+	if (a < 1) a = 1;
+	if (a > nx - 2) a = nx - 2;
+	if (y > (gy[b] + gy[b + 1]) / 2.0) b += 1;
+	if (b < 1) b = 1;
+	if (b > ny - 2) b = ny - 2;
+	if (z > (gz[c] + gz[c + 1]) / 2.0) c += 1;
+	if (c < 1) c = 1;
+	if (c > nz - 2) c = nz - 2;
+	double dx = (gx[a + 1] - gx[a - 1]) / 2.0, dy = (gy[b + 1] - gy[b - 1]) / 2.0, dz = (gz[c + 1] - gz[c - 1]) / 2.0;
+	double p = 0;
+	for (int m30 = c; m30 <= c + 1; m30++) {
+		double ddz = ((gz[m30 - 1] + gz[m30]) / 2.0 - z) / dz;
+		if (m30 == c) ddz = -ddz;
+		ddz = 1.0 - ddz;
+		for (int m10 = a; m10 <= a + 1; m10++) {
+			double ddx = ((gx[m10 - 1] + gx[m10]) / 2.0 - x) / dx;
+			if (m10 == a) ddx = -ddx;
+			ddx = 1.0 - ddx;
+			for (int m20 = b; m20 <= b + 1; m20++) {
+				double ddy = ((gy[m20 - 1] + gy[m20]) / 2.0 - y) / dy;
+				if (m20 == b) ddy = -ddy;
+				ddy = 1.0 - ddy;
+				long m40 = (long)m30 * nx * ny + (long)m10 * ny + m20;
+				p += pr[m40] * (ddx * ddy * ddz);
+			}
+		}
+	}
+	return p;
+}
+
+/* viscalc branches A/B with the final limits (mark3mg.c:1266-1299,
+ * 1626-1681); brittle and power-law terms are zero for these rocks. */
+__device__ double visc_ab(const i3g_rock &r, double mtk, double mpb, double nubeg, double nuend)
+{
+	double rt = 8.314 * mtk, nunewt = 0;
+	if (r.nu0 != 0) {
+		if (r.dh == 0 && r.dv == 0) nunewt = 1.0 / r.nu0;
+		else {
+			double e1 = (r.dh + r.dv * mpb) / rt;
+			if (e1 > 150.0) e1 = 150.0;
+			if (r.dh < 0 && r.dv > 0) {
+				e1 = r.dh * (mtk - r.dv);
+				if (e1 < -150.0) e1 = -150.0;
+			}
+			if (r.dh < 0 && r.dv < 0) {
+				e1 = (-r.dh) * (1.0 - (mtk - (-r.dv)) / (-r.dv)) / 8.314 / (-r.dv);
+				if (e1 > 150.0) e1 = 150.0;
+			}
+			nunewt = 1.0 / (r.nu0 * exp(e1));
+		}
+	}
+	double nueff = nunewt;
+	if (nueff <= 0) nueff = 1.0 / r.n1;
+	nueff = 1.0 / nueff;
+	if (nueff < r.n0) nueff = r.n0;
+	if (nueff > r.n1) nueff = r.n1;
+	if (nueff < nubeg) nueff = nubeg;
+	if (nueff > nuend) nueff = nuend;
+	return nueff;
+}
+
+__global__ void k_marker_props(long n, const float *__restrict__ x, const float *__restrict__ y,
+			       const float *__restrict__ z, const float *__restrict__ tk, float *eta,
+			       const signed char *__restrict__ type, const int *c1, const int *c2, const int *c3,
+			       const double *pr, const double *gx, const double *gy, const double *gz, int nx, int ny,
+			       int nz, const i3g_rock *rocks, int nrock, i3g_mk_params prm, double *pnu, double *pro,
+			       double *pcp, double *pkt, double *pht, double *pbb, double *ptk, uint8_t *pin, int *bad)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	float fx = x[t], fy = y[t], fz = z[t];
+	int mt = type[t];
+	/* marker participates (mark3mg.c:460-461) */
+	int in = fx > 0 && fy > 0 && fz > 0 && fx < xs && fy < ys && fz < zs && mt < 50 && tk[t] > 0 && mt < nrock;
+	pin[t] = (uint8_t)in;
+	if (!in) return;
+	/* types with air/water, eclogite or melt branches in the reference
+	 * (mark3mg.c:463-474, 530-641, 660-686) are not handled here */
+	if (mt < 2 || mt == 7 || mt == 8 || (mt >= 16 && mt <= 18) || mt > 20) {
+		atomicOr(bad, 1);
+		pin[t] = 0;
+		return;
+	}
+	const i3g_rock r = rocks[mt];
+	double mpb = pr_interp(pr, gx, gy, gz, nx, ny, nz, (double)fx, (double)fy, (double)fz, c1[t], c2[t], c3[t]) * 1e-5;
+	double mtk = (double)tk[t];
+	double mnu = eta[t];
+	if (mnu <= 0 || prm.timestep <= 0) {
+		mnu = visc_ab(r, mtk, mpb, prm.nubeg, prm.nuend);
+		eta[t] = (float)mnu;
+	}
+	/* dencalc1 (mark3mg.c:3006-3024) */
+	double mro, mbb = 0;
+	if (prm.densimod == 0) mro = r.ro;
+	else {
+		mro = r.ro * (1.0 - r.bro * (mtk - 298.15)) * (1.0 + r.aro * (mpb - 1.0) * 1e-3);
+		mbb = r.bro / (1.0 - r.bro * (mtk - 298.15));
+	}
+	double mcp = r.cp;
+	double mkt = (r.kt + r.kf / (mtk + 77.0)) * exp(r.kp * mpb);
+	double mht = r.ht;
+	/* viscosity caps of mark3mg.c:790-793 */
+	if ((double)fy > prm.vdeep && mnu > prm.nudeep) mnu = prm.nudeep;
+	if (fy < 5e+4 && fx > xs - 1e+4 && fz > 1.5e+5 && fz < 8.7e+5 && mnu > 1e+19) mnu = 1e+19;
+	pnu[t] = mnu; pro[t] = mro; pcp[t] = mcp; pkt[t] = mkt; pht[t] = mht; pbb[t] = mbb; ptk[t] = mtk;
+}
+
+/* ------------------------------------------------------------------ */
+/* Marker -> node gather                                              */
+/* ------------------------------------------------------------------ */
+
+/* Per node: merge the markers of the (up to) 8 cells around it by id and
+ * accumulate the reference's weights (mark3mg.c:818-933) in that order;
+ * then normalise (mark3mg.c:938-1002).  fld planes: 0 nu, 1 ro, 2 cp,
+ * 3 kt, 4 ht, 5 et, 6 tk, 7 nxx, 8 nxy, 9 nxz, 10 nyz. */
+__global__ void __launch_bounds__(256)
+k_marker_gather(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz,
+		const int *start, const int *cnt, const int *__restrict__ id, const float *__restrict__ x,
+		const float *__restrict__ y, const float *__restrict__ z, const uint8_t *pin, const double *pnu,
+		const double *pro, const double *pcp, const double *pkt, const double *pht, const double *pbb,
+		const double *ptk, double nukoef, double nubeg, double nuend, float *fld)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int m3 = blockIdx.y;
+	int m1 = t / ny, m2 = t - m1 * ny;
+	if (m1 >= nx) return;
+	long N = (long)nx * ny * nz;
+	long nd = (long)m3 * nx * ny + (long)m1 * ny + m2;
+	int ps[8], pe[8], ca[8], cb[8], cc[8];
+	int ncell = 0;
+	for (int c = m3 - 1; c <= m3; c++)
+		for (int a = m1 - 1; a <= m1; a++)
+			for (int b = m2 - 1; b <= m2; b++) {
+				if (a < 0 || b < 0 || c < 0 || a > nx - 2 || b > ny - 2 || c > nz - 2) continue;
+				long k = (long)c * nx * ny + (long)a * ny + b;
+				ps[ncell] = start[k];
+				pe[ncell] = start[k] + cnt[k];
+				ca[ncell] = a; cb[ncell] = b; cc[ncell] = c;
+				ncell++;
+			}
+	float anu = 0, aro = 0, acp = 0, akt = 0, aht = 0, aet = 0, atk = 0, axx = 0, axy = 0, axz = 0, ayz = 0;
+	double w1 = 0, w0 = 0, wxx = 0, wxy = 0, wxz = 0, wyz = 0;
+	while (true) {
+		int best = -1, bid = 0x7fffffff;
+		for (int q = 0; q < ncell; q++)
+			if (ps[q] < pe[q]) {
+				int v = id[ps[q]];
+				if (v < bid) { bid = v; best = q; }
+			}
+		if (best < 0) break;
+		int mk = ps[best]++;
+		if (!pin[mk]) continue;
+		int a = ca[best], b = cb[best], c = cc[best];
+		double dx = gx[a + 1] - gx[a], dy = gy[b + 1] - gy[b], dz = gz[c + 1] - gz[c];
+		double ddz = 1.0 - fabs((gz[m3] - (double)z[mk]) / dz);
+		double ddx = 1.0 - fabs((gx[m1] - (double)x[mk]) / dx);
+		double ddy = 1.0 - fabs((gy[m2] - (double)y[mk]) / dy);
+		double swt = ddx * ddy * ddz;
+		double mnu = pnu[mk];
+		if (ddx > nukoef && ddy > nukoef && ddz > nukoef) {
+			anu = (float)((double)anu + mnu * swt);
+			w1 += swt;
+		}
+		if (m1 > a && m2 > b && m3 > c) {
+			double s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (0.5 - fabs(ddy - 0.5)) * (0.5 - fabs(ddz - 0.5));
+			axx = (float)((double)axx + mnu * s1);
+			wxx += s1;
+		}
+		if (m3 == c && ddx > 0.5 && ddy > 0.5) {
+			double s1 = 8.0 * (ddx - 0.5) * (ddy - 0.5) * (0.5 - fabs(ddz - 0.5));
+			axy = (float)((double)axy + mnu * s1);
+			wxy += s1;
+		}
+		if (m2 == b && ddx > 0.5 && ddz > 0.5) {
+			double s1 = 8.0 * (ddx - 0.5) * (0.5 - fabs(ddy - 0.5)) * (ddz - 0.5);
+			axz = (float)((double)axz + mnu * s1);
+			wxz += s1;
+		}
+		if (m1 == a && ddy > 0.5 && ddz > 0.5) {
+			double s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (ddy - 0.5) * (ddz - 0.5);
+			ayz = (float)((double)ayz + mnu * s1);
+			wyz += s1;
+		}
+		if (swt > 0) {
+			double mro = pro[mk];
+			aro = (float)((double)aro + mro * swt);
+			acp = (float)((double)acp + pcp[mk] * mro * swt);
+			akt = (float)((double)akt + pkt[mk] * swt);
+			aht = (float)((double)aht + pht[mk] * swt);
+			aet = (float)((double)aet + pbb[mk] * swt);
+			atk = (float)((double)atk + ptk[mk] * swt);
+			w0 += swt;
+		}
+	}
+	float *f = fld;
+	if (w1 > 0.01) f[nd] = (float)((double)anu / w1);
+	if (wxx > 0.01) f[7 * N + nd] = (float)((double)axx / wxx);
+	if (wxy > 0.01) f[8 * N + nd] = (float)((double)axy / wxy);
+	if (wxz > 0.01) f[9 * N + nd] = (float)((double)axz / wxz);
+	if (wyz > 0.01) f[10 * N + nd] = (float)((double)ayz / wyz);
+	if (w0 > 0.01) {
+		float rn = (float)((double)aro / w0);
+		f[N + nd] = rn;
+		f[2 * N + nd] = (float)((double)acp / w0 / (double)rn);
+		f[3 * N + nd] = (float)((double)akt / w0);
+		f[4 * N + nd] = (float)((double)aht / w0);
+		f[5 * N + nd] = (float)((double)aet / w0);
+		f[6 * N + nd] = (float)((double)atk / w0);
+	}
+	float nv = f[nd];
+	if (nv < nubeg) nv = (float)nubeg;
+	if (nv > nuend) nv = (float)nuend;
+	f[nd] = nv;
+}
+
+/* tk = c (+ k * tk1[partner]) on temperature boundary nodes, in float as
+ * tkrecalc (heat3mg.c:849-899); tk1 is the tk plane before the update */
+__global__ void k_tkbc(long N, const int *__restrict__ p, const float *__restrict__ c, const float *__restrict__ k,
+		       const float *__restrict__ tk1, float *__restrict__ tk)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= N) return;
+	int q = p[t];
+	if (q == -2) return;
+	float v = c[t];
+	if (q >= 0) v = v + k[t] * tk1[q];
+	tk[t] = v;
+}
+
+/* ------------------------------------------------------------------ */
+/* RK4 advection                                                      */
+/* ------------------------------------------------------------------ */
+
+/* staggered trilinear velocity at a point, vxyzcalc1 (mark3mg.c:2347-2530) */
+__device__ void vxyz(const float *vx, const float *vy, const float *vz, const double *gx, const double *gy,
+		     const double *gz, int nx, int ny, int nz, double x, double y, double z, double *v)
+{
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	if (x < 0) x = 0;
+	if (x > xs) x = xs;
+	if (y < 0) y = 0;
+	if (y > ys) y = ys;
+	if (z < 0) z = 0;
+	if (z > zs) z = zs;
+	int m1 = serch(gx, nx, x), m2 = serch(gy, ny, y), m3 = serch(gz, nz, z);
+	long sx = ny, sz = (long)nx * ny;
+	for (int comp = 0; comp < 3; comp++) {
+		int a = m1, b = m2, c = m3;
+		/* node-aligned axis: comp; the other two use cell-mid lines */
+		if (comp == 0) {
+			if (a > nx - 2) a = nx - 2;
+		} else {
+			if (x < (gx[a] + gx[a + 1]) / 2.0) a -= 1;
+			if (a < 0) a = 0;
+			if (a > nx - 3) a = nx - 3;
+		}
+		if (comp == 1) {
+			if (b > ny - 2) b = ny - 2;
+		} else {
+			if (y < (gy[b] + gy[b + 1]) / 2.0) b -= 1;
+			if (b < 0) b = 0;
+			if (b > ny - 3) b = ny - 3;
+		}
+		if (comp == 2) {
+			if (c > nz - 2) c = nz - 2;
+		} else {
+			if (z < (gz[c] + gz[c + 1]) / 2.0) c -= 1;
+			if (c < 0) c = 0;
+			if (c > nz - 3) c = nz - 3;
+		}
+		double dx = comp == 0 ? gx[a + 1] - gx[a] : (gx[a + 2] - gx[a]) / 2.0;
+		double dy = comp == 1 ? gy[b + 1] - gy[b] : (gy[b + 2] - gy[b]) / 2.0;
+		double dz = comp == 2 ? gz[c + 1] - gz[c] : (gz[c + 2] - gz[c]) / 2.0;
+		const float *f = comp == 0 ? vx : comp == 1 ? vy : vz;
+		double acc = 0;
+		for (int m30 = c; m30 <= c + 1; m30++) {
+			double ddz = comp == 2 ? (gz[m30] - z) / dz : ((gz[m30] + gz[m30 + 1]) / 2.0 - z) / dz;
+			if (m30 == c) ddz = -ddz;
+			ddz = 1.0 - ddz;
+			for (int m10 = a; m10 <= a + 1; m10++) {
+				double ddx = comp == 0 ? (gx[m10] - x) / dx : ((gx[m10] + gx[m10 + 1]) / 2.0 - x) / dx;
+				if (m10 == a) ddx = -ddx;
+				ddx = 1.0 - ddx;
+				for (int m20 = b; m20 <= b + 1; m20++) {
+					double ddy = comp == 1 ? (gy[m20] - y) / dy : ((gy[m20] + gy[m20 + 1]) / 2.0 - y) / dy;
+					if (m20 == b) ddy = -ddy;
+					ddy = 1.0 - ddy;
+					acc += f[m30 * sz + m10 * sx + m20] * (ddx * ddy * ddz);
+				}
+			}
+		}
+		v[comp] = acc;
+	}
+}
+
+__global__ void k_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, const signed char *type,
+		      const float *vx, const float *vy, const float *vz, const double *gx, const double *gy,
+		      const double *gz, int nx, int ny, int nz, double dt)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	float fx = x[t], fy = y[t], fz = z[t];
+	int mt = type[t];
+	/* moved markers (mark3mg.c:61): inside the grid, or outgrid != 1 with
+	 * markt < 50 or >= 100 (outgrid 0 here) */
+	if (!((fx >= 0 && fy >= 0 && fz >= 0 && (double)fx <= xs && (double)fy <= ys && (double)fz <= zs) ||
+	      (mt < 50 || mt >= 100)))
+		return;
+	double v1[3], v2[3], v3[3], v4[3];
+	double px = fx, py = fy, pz = fz;
+	vxyz(vx, vy, vz, gx, gy, gz, nx, ny, nz, px, py, pz, v1);
+	vxyz(vx, vy, vz, gx, gy, gz, nx, ny, nz, px + v1[0] * dt / 2.0, py + v1[1] * dt / 2.0, pz + v1[2] * dt / 2.0, v2);
+	vxyz(vx, vy, vz, gx, gy, gz, nx, ny, nz, px + v2[0] * dt / 2.0, py + v2[1] * dt / 2.0, pz + v2[2] * dt / 2.0, v3);
+	vxyz(vx, vy, vz, gx, gy, gz, nx, ny, nz, px + v3[0] * dt, py + v3[1] * dt, pz + v3[2] * dt, v4);
+	double v[3];
+	for (int q = 0; q < 3; q++) v[q] = (v1[q] + 2.0 * v2[q] + 2.0 * v3[q] + v4[q]) / 6.0;
+	if (mt < 100) {
+		fx = (float)((double)fx + (dt * v[0] - 0.0));
+		fy = (float)((double)fy + dt * v[1]);
+		fz = (float)((double)fz + dt * v[2]);
+		x[t] = fx; y[t] = fy; z[t] = fz;
+		if (fx < 0 || fy < 0 || fz < 0 || (double)fx > xs || (double)fy > ys || (double)fz > zs) {
+			tk[t] = 0;
+			eta[t] = 0;
+		}
+	}
+}
+
+/* ------------------------------------------------------------------ */
+/* C ABI                                                              */
+/* ------------------------------------------------------------------ */
+
+template <typename T> static T *mk_alloc(size_t n, int *err)
+{
+	void *p = nullptr;
+	if (n == 0) n = 1;
+	if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) { *err = I3G_ENOMEM; return nullptr; }
+	return (T *)p;
+}
+
+static inline dim3 g1(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+extern "C" int i3g_markers_create(long nx, long ny, long nz, const double *gx, const double *gy, const double *gz,
+				  long capacity, i3g_markers **out)
+{
+	if (!out || !gx || !gy || !gz || nx < 3 || ny < 3 || nz < 3 || capacity < 1 || capacity > 2000000000L)
+		return I3G_EINVAL;
+	i3g_markers *m = new i3g_markers();
+	m->nx = (int)nx; m->ny = (int)ny; m->nz = (int)nz;
+	m->N = nx * ny * nz;
+	m->cap = capacity;
+	int err = 0;
+	if (hipStreamCreate(&m->s) != hipSuccess) { delete m; return I3G_EDEVICE; }
+	m->gx = mk_alloc<double>(nx, &err); m->gy = mk_alloc<double>(ny, &err); m->gz = mk_alloc<double>(nz, &err);
+	long c = capacity;
+	m->x = mk_alloc<float>(c, &err); m->y = mk_alloc<float>(c, &err); m->z = mk_alloc<float>(c, &err);
+	m->tk = mk_alloc<float>(c, &err); m->eta = mk_alloc<float>(c, &err);
+	m->type = mk_alloc<signed char>(c, &err); m->id = mk_alloc<int>(c, &err);
+	m->x2 = mk_alloc<float>(c, &err); m->y2 = mk_alloc<float>(c, &err); m->z2 = mk_alloc<float>(c, &err);
+	m->tk2 = mk_alloc<float>(c, &err); m->eta2 = mk_alloc<float>(c, &err);
+	m->type2 = mk_alloc<signed char>(c, &err); m->id2 = mk_alloc<int>(c, &err);
+	m->key = mk_alloc<int>(c, &err); m->c1 = mk_alloc<int>(c, &err); m->c2 = mk_alloc<int>(c, &err);
+	m->c3 = mk_alloc<int>(c, &err); m->perm = mk_alloc<int>(c, &err);
+	m->cnt = mk_alloc<int>(m->N, &err); m->start = mk_alloc<int>(m->N + 1, &err); m->cur = mk_alloc<int>(m->N, &err);
+	m->bsum = mk_alloc<int>(m->N / SCAN_B + 2, &err);
+	m->rocks = mk_alloc<i3g_rock>(MAXROCK, &err);
+	m->bad = mk_alloc<int>(1, &err);
+	if (err) { i3g_markers_destroy(m); return err; }
+	if (hipMemcpy(m->gx, gx, nx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(m->gy, gy, ny * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(m->gz, gz, nz * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+		i3g_markers_destroy(m);
+		return I3G_EDEVICE;
+	}
+	*out = m;
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_destroy(i3g_markers *m)
+{
+	if (!m) return I3G_OK;
+	hipDeviceSynchronize();
+	void *ps[] = {m->gx, m->gy, m->gz, m->x, m->y, m->z, m->tk, m->eta, m->type, m->id, m->x2, m->y2, m->z2,
+		      m->tk2, m->eta2, m->type2, m->id2, m->key, m->c1, m->c2, m->c3, m->perm, m->cnt, m->start,
+		      m->cur, m->bsum, m->pnu, m->pro, m->pcp, m->pkt, m->pht, m->pbb, m->ptk, m->pin, m->pr, m->fld,
+		      m->vel, m->rocks, m->bad, m->tbc_p, m->tbc_c, m->tbc_k, m->tk1};
+	for (void *p : ps)
+		if (p) hipFree(p);
+	if (m->s) hipStreamDestroy(m->s);
+	delete m;
+	return I3G_OK;
+}
+
+extern "C" long i3g_markers_count(const i3g_markers *m) { return m ? m->n : I3G_EINVAL; }
+
+extern "C" int i3g_markers_upload(i3g_markers *m, long n, const float *x, const float *y, const float *z,
+				  const float *tk, const float *eta, const signed char *type, const int *id)
+{
+	if (!m || n < 0 || n > m->cap || !x || !y || !z || !tk || !eta || !type) return I3G_EINVAL;
+	m->n = n;
+	m->sorted = 0;
+	MCHK(hipMemcpyAsync(m->x, x, n * sizeof(float), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->y, y, n * sizeof(float), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->z, z, n * sizeof(float), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->tk, tk, n * sizeof(float), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->eta, eta, n * sizeof(float), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->type, type, n * sizeof(signed char), hipMemcpyHostToDevice, m->s));
+	if (id) {
+		MCHK(hipMemcpyAsync(m->id, id, n * sizeof(int), hipMemcpyHostToDevice, m->s));
+	} else {
+		std::vector<int> ids(n);
+		for (long i = 0; i < n; i++) ids[i] = (int)i;
+		MCHK(hipMemcpyAsync(m->id, ids.data(), n * sizeof(int), hipMemcpyHostToDevice, m->s));
+		MCHK(hipStreamSynchronize(m->s));
+	}
+	MCHK(hipStreamSynchronize(m->s));
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_download(i3g_markers *m, float *x, float *y, float *z, float *tk, float *eta,
+				    signed char *type, int *id)
+{
+	if (!m) return I3G_EINVAL;
+	long n = m->n;
+	if (x) MCHK(hipMemcpyAsync(x, m->x, n * sizeof(float), hipMemcpyDeviceToHost, m->s));
+	if (y) MCHK(hipMemcpyAsync(y, m->y, n * sizeof(float), hipMemcpyDeviceToHost, m->s));
+	if (z) MCHK(hipMemcpyAsync(z, m->z, n * sizeof(float), hipMemcpyDeviceToHost, m->s));
+	if (tk) MCHK(hipMemcpyAsync(tk, m->tk, n * sizeof(float), hipMemcpyDeviceToHost, m->s));
+	if (eta) MCHK(hipMemcpyAsync(eta, m->eta, n * sizeof(float), hipMemcpyDeviceToHost, m->s));
+	if (type) MCHK(hipMemcpyAsync(type, m->type, n * sizeof(signed char), hipMemcpyDeviceToHost, m->s));
+	if (id) MCHK(hipMemcpyAsync(id, m->id, n * sizeof(int), hipMemcpyDeviceToHost, m->s));
+	MCHK(hipStreamSynchronize(m->s));
+	return I3G_OK;
+}
+
+static int cell_index(i3g_markers *m, int with_count)
+{
+	if (with_count) MCHK(hipMemsetAsync(m->cnt, 0, m->N * sizeof(int), m->s));
+	if (m->n)
+		hipLaunchKernelGGL(k_cell_index, g1(m->n), dim3(256), 0, m->s, m->n, (const float *)m->x,
+				   (const float *)m->y, (const float *)m->z, (const double *)m->gx,
+				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, m->key, m->c1,
+				   m->c2, m->c3, with_count ? m->cnt : (int *)nullptr);
+	MCHK(hipGetLastError());
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_cell_index(i3g_markers *m, int *m1, int *m2, int *m3, int *key)
+{
+	if (!m) return I3G_EINVAL;
+	int rc = cell_index(m, 0);
+	if (rc) return rc;
+	long n = m->n;
+	if (m1) MCHK(hipMemcpyAsync(m1, m->c1, n * sizeof(int), hipMemcpyDeviceToHost, m->s));
+	if (m2) MCHK(hipMemcpyAsync(m2, m->c2, n * sizeof(int), hipMemcpyDeviceToHost, m->s));
+	if (m3) MCHK(hipMemcpyAsync(m3, m->c3, n * sizeof(int), hipMemcpyDeviceToHost, m->s));
+	if (key) MCHK(hipMemcpyAsync(key, m->key, n * sizeof(int), hipMemcpyDeviceToHost, m->s));
+	MCHK(hipStreamSynchronize(m->s));
+	return I3G_OK;
+}
+
+template <typename T> static void permute(i3g_markers *m, T *&a, T *&b)
+{
+	hipLaunchKernelGGL(k_gather<T>, g1(m->n), dim3(256), 0, m->s, m->n, (const int *)m->perm, (const T *)a, b);
+	T *t = a;
+	a = b;
+	b = t;
+}
+
+extern "C" int i3g_markers_sort(i3g_markers *m, long *order)
+{
+	if (!m) return I3G_EINVAL;
+	long n = m->n, N = m->N;
+	int rc = cell_index(m, 1);
+	if (rc) return rc;
+	long nb = (N + SCAN_B - 1) / SCAN_B;
+	hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(SCAN_B), 0, m->s, (const int *)m->cnt, m->start, N, m->bsum);
+	hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(SCAN_B), 0, m->s, m->bsum, nb);
+	hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SCAN_B), 0, m->s, m->start, N, (const int *)m->bsum, 0L);
+	MCHK(hipMemsetAsync(m->cur, 0, N * sizeof(int), m->s));
+	if (n) {
+		hipLaunchKernelGGL(k_fill, g1(n), dim3(256), 0, m->s, n, (const int *)m->key, (const int *)m->start, m->cur,
+				   m->perm);
+		hipLaunchKernelGGL(k_cellsort, g1(N), dim3(256), 0, m->s, N, (const int *)m->start, (const int *)m->cnt,
+				   m->perm, (const int *)m->id);
+		permute(m, m->x, m->x2);
+		permute(m, m->y, m->y2);
+		permute(m, m->z, m->z2);
+		permute(m, m->tk, m->tk2);
+		permute(m, m->eta, m->eta2);
+		permute(m, m->type, m->type2);
+		permute(m, m->id, m->id2);
+		/* cell indices in the new order */
+		rc = cell_index(m, 0);
+		if (rc) return rc;
+	}
+	MCHK(hipGetLastError());
+	if (order && n) {
+		std::vector<int> p(n);
+		MCHK(hipMemcpyAsync(p.data(), m->perm, n * sizeof(int), hipMemcpyDeviceToHost, m->s));
+		MCHK(hipStreamSynchronize(m->s));
+		for (long i = 0; i < n; i++) order[i] = p[i];
+	}
+	MCHK(hipStreamSynchronize(m->s));
+	m->sorted = 1;
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int nrock, const i3g_mk_params *prm,
+				    const double *pr, float *nu, float *ro, float *cp, float *kt, float *ht, float *et,
+				    float *tk, float *nxx, float *nxy, float *nxz, float *nyz)
+{
+	if (!m || !rocks || !prm || !pr || nrock < 1 || nrock > MAXROCK) return I3G_EINVAL;
+	if (prm->viscmod != 0 || prm->densimod < 0 || prm->densimod > 1) return I3G_EINVAL;
+	if (!m->sorted) return I3G_EINVAL;
+	float *host[11] = {nu, ro, cp, kt, ht, et, tk, nxx, nxy, nxz, nyz};
+	for (int q = 0; q < 11; q++)
+		if (!host[q]) return I3G_EINVAL;
+	int err = 0;
+	long n = m->n, N = m->N;
+	if (!m->pnu) {
+		long c = m->cap;
+		m->pnu = mk_alloc<double>(c, &err); m->pro = mk_alloc<double>(c, &err); m->pcp = mk_alloc<double>(c, &err);
+		m->pkt = mk_alloc<double>(c, &err); m->pht = mk_alloc<double>(c, &err); m->pbb = mk_alloc<double>(c, &err);
+		m->ptk = mk_alloc<double>(c, &err); m->pin = mk_alloc<uint8_t>(c, &err);
+		m->pr = mk_alloc<double>(N, &err); m->fld = mk_alloc<float>(11 * N, &err);
+		if (err) return err;
+	}
+	MCHK(hipMemcpyAsync(m->rocks, rocks, nrock * sizeof(i3g_rock), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemsetAsync(m->bad, 0, sizeof(int), m->s));
+	MCHK(hipMemcpyAsync(m->pr, pr, N * sizeof(double), hipMemcpyHostToDevice, m->s));
+	for (int q = 0; q < 11; q++)
+		MCHK(hipMemcpyAsync(m->fld + q * N, host[q], N * sizeof(float), hipMemcpyHostToDevice, m->s));
+	if (n)
+		hipLaunchKernelGGL(k_marker_props, g1(n), dim3(256), 0, m->s, n, (const float *)m->x, (const float *)m->y,
+				   (const float *)m->z, (const float *)m->tk, m->eta, (const signed char *)m->type,
+				   (const int *)m->c1, (const int *)m->c2, (const int *)m->c3, (const double *)m->pr,
+				   (const double *)m->gx, (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz,
+				   (const i3g_rock *)m->rocks, nrock, *prm, m->pnu, m->pro, m->pcp, m->pkt, m->pht, m->pbb,
+				   m->ptk, m->pin, m->bad);
+	dim3 gg((unsigned)(((long)m->nx * m->ny + 255) / 256), (unsigned)m->nz);
+	hipLaunchKernelGGL(k_marker_gather, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz, (const double *)m->gx,
+			   (const double *)m->gy, (const double *)m->gz, (const int *)m->start, (const int *)m->cnt,
+			   (const int *)m->id, (const float *)m->x, (const float *)m->y, (const float *)m->z,
+			   (const uint8_t *)m->pin, (const double *)m->pnu, (const double *)m->pro, (const double *)m->pcp,
+			   (const double *)m->pkt, (const double *)m->pht, (const double *)m->pbb, (const double *)m->ptk,
+			   prm->nukoef, prm->nubeg, prm->nuend, m->fld);
+	if (m->has_tbc) {
+		MCHK(hipMemcpyAsync(m->tk1, m->fld + 6 * N, N * sizeof(float), hipMemcpyDeviceToDevice, m->s));
+		hipLaunchKernelGGL(k_tkbc, g1(N), dim3(256), 0, m->s, N, (const int *)m->tbc_p, (const float *)m->tbc_c,
+				   (const float *)m->tbc_k, (const float *)m->tk1, m->fld + 6 * N);
+	}
+	MCHK(hipGetLastError());
+	int bad = 0;
+	MCHK(hipMemcpyAsync(&bad, m->bad, sizeof(int), hipMemcpyDeviceToHost, m->s));
+	MCHK(hipStreamSynchronize(m->s));
+	if (bad) return I3G_EINVAL;
+	for (int q = 0; q < 11; q++)
+		MCHK(hipMemcpyAsync(host[q], m->fld + q * N, N * sizeof(float), hipMemcpyDeviceToHost, m->s));
+	MCHK(hipStreamSynchronize(m->s));
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_set_tbc(i3g_markers *m, const int *bondm_t, const float *bondv, const int *bondn,
+				   long nrec)
+{
+	if (!m) return I3G_EINVAL;
+	if (!bondm_t) {
+		m->has_tbc = 0;
+		return I3G_OK;
+	}
+	if (!bondv || !bondn || nrec < 1) return I3G_EINVAL;
+	long N = m->N;
+	std::vector<int> p(N, -2);
+	std::vector<float> c(N, 0.f), k(N, 0.f);
+	for (long i = 0; i < N; i++) {
+		long r = bondm_t[i];
+		if (r <= 0) continue;
+		if (r >= nrec) return I3G_EBC;
+		long q = (long)bondn[r] - 1;
+		if (q < -1 || q >= N) return I3G_EBC;
+		p[i] = (int)q;
+		c[i] = bondv[2 * r];
+		k[i] = bondv[2 * r + 1];
+	}
+	int err = 0;
+	if (!m->tbc_p) {
+		m->tbc_p = mk_alloc<int>(N, &err); m->tbc_c = mk_alloc<float>(N, &err);
+		m->tbc_k = mk_alloc<float>(N, &err); m->tk1 = mk_alloc<float>(N, &err);
+		if (err) return err;
+	}
+	MCHK(hipMemcpyAsync(m->tbc_p, p.data(), N * sizeof(int), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->tbc_c, c.data(), N * sizeof(float), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->tbc_k, k.data(), N * sizeof(float), hipMemcpyHostToDevice, m->s));
+	MCHK(hipStreamSynchronize(m->s));
+	m->has_tbc = 1;
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_advect_rk4(i3g_markers *m, double dt, const float *vx, const float *vy, const float *vz)
+{
+	if (!m || !vx || !vy || !vz) return I3G_EINVAL;
+	long N = m->N, n = m->n;
+	int err = 0;
+	if (!m->vel) {
+		m->vel = mk_alloc<float>(3 * N, &err);
+		if (err) return err;
+	}
+	MCHK(hipMemcpyAsync(m->vel, vx, N * sizeof(float), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->vel + N, vy, N * sizeof(float), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->vel + 2 * N, vz, N * sizeof(float), hipMemcpyHostToDevice, m->s));
+	if (n)
+		hipLaunchKernelGGL(k_rk4, g1(n), dim3(256), 0, m->s, n, m->x, m->y, m->z, m->tk, m->eta,
+				   (const signed char *)m->type, (const float *)m->vel, (const float *)(m->vel + N),
+				   (const float *)(m->vel + 2 * N), (const double *)m->gx, (const double *)m->gy,
+				   (const double *)m->gz, m->nx, m->ny, m->nz, dt);
+	MCHK(hipGetLastError());
+	MCHK(hipStreamSynchronize(m->s));
+	m->sorted = 0;
+	return I3G_OK;
+}

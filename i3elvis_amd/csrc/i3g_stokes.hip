@@ -248,6 +248,10 @@ struct i3g_ctx {
 	void **dP = nullptr, **dQ = nullptr;
 	void *kr = nullptr, *ke = nullptr;
 	double *dh = nullptr;
+	/* inner FGMRES storage */
+	int in_m = 0, in_prec = 0;
+	std::vector<void *> iV, iZ;
+	void **diV = nullptr, **diZ = nullptr;
 	double *e64 = nullptr;
 	/* timing */
 	bool timing = false;
@@ -687,6 +691,13 @@ static void free_krylov(i3g_ctx *c)
 	c->dh = nullptr;
 	c->kr_m = 0;
 	c->kr_k = 0;
+	for (void *q : c->iV) hipFree(q);
+	for (void *q : c->iZ) hipFree(q);
+	c->iV.clear(); c->iZ.clear();
+	if (c->diV) hipFree(c->diV);
+	if (c->diZ) hipFree(c->diZ);
+	c->diV = c->diZ = nullptr;
+	c->in_m = 0;
 }
 
 extern "C" int i3g_destroy(i3g_ctx *c)
@@ -840,6 +851,7 @@ template <typename T> static void coarse_cycle_graph(i3g_ctx *c, const i3g_stoke
 		}
 		c->cg_key[q] = key;
 	}
+	Timed tm(c, "coarse_levels_graph", 0);
 	hipGraphLaunch(c->cg_exec[q], c->s);
 }
 
@@ -1083,6 +1095,113 @@ struct GcrHost {
 	std::vector<double> alpha;  /* accumulated coefficients of q_j */
 };
 
+template <typename T> static int ensure_inner(i3g_ctx *c, int m)
+{
+	if (c->in_m == m && c->in_prec == (int)sizeof(T)) return I3G_OK;
+	for (void *q : c->iV) hipFree(q);
+	for (void *q : c->iZ) hipFree(q);
+	c->iV.clear(); c->iZ.clear();
+	if (c->diV) hipFree(c->diV);
+	if (c->diZ) hipFree(c->diZ);
+	c->diV = c->diZ = nullptr;
+	c->in_m = 0;
+	long n = 4 * c->L[0].N;
+	int err = 0;
+	for (int q = 0; q <= m; q++) c->iV.push_back(dalloc<T>(n, &err));
+	for (int q = 0; q < m; q++) c->iZ.push_back(dalloc<T>(n, &err));
+	c->diV = dalloc<void *>(m + 1, &err);
+	c->diZ = dalloc<void *>(m, &err);
+	if (err) return err;
+	if (upload(c->diV, c->iV.data(), m + 1) || upload(c->diZ, c->iZ.data(), m)) return I3G_EDEVICE;
+	c->in_m = m;
+	c->in_prec = (int)sizeof(T);
+	return I3G_OK;
+}
+
+/* Inner FGMRES(m) in precision T on (W A) z = r with the V-cycle as right
+ * preconditioner: at most m V-cycles, stopping early at a relative
+ * reduction tol.  The operator product is evaluated in T (an FP32 inner
+ * solve only has to reduce its residual by about tol, well above FP32
+ * rounding).  out (T) receives z in the unscaled correction space.
+ * Returns the V-cycles applied. */
+template <typename T, typename R>
+static int inner_fgmres(i3g_ctx *c, const R *r, T *out, int m, double tol, const i3g_stokes_params *p)
+{
+	Level &L = c->L[0];
+	long N = L.N, n = 4 * N;
+	dim3 lg = lin_grid(n);
+	if (ensure_inner<T>(c, m)) return 0;
+	T **V = (T **)c->iV.data(), **Z = (T **)c->iZ.data();
+	T *const *dV = (T *const *)c->diV;
+	double *dh = c->dh + 3 * c->kr_m + 4; /* 12 scratch doubles */
+	std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), g(m + 1, 0.0);
+	hipLaunchKernelGGL((k_copy_scale<T, R>), lg, dim3(256), 0, c->s, V[0], r, 1.0, n);
+	double beta = sqrt(dot(c, V[0], V[0], n));
+	hipMemsetAsync(out, 0, n * sizeof(T), c->s);
+	if (beta == 0) return 0;
+	hipLaunchKernelGGL((k_copy_scale<T, T>), lg, dim3(256), 0, c->s, V[0], (const T *)V[0], 1.0 / beta, n);
+	g[0] = beta;
+	int j, used = 0;
+	for (j = 0; j < m; j++) {
+		T *x0 = lx<T>(L), *b0 = lb<T>(L);
+		hipLaunchKernelGGL((k_wscale<T, T>), lg, dim3(256), 0, c->s, b0, (const T *)V[j], (const double *)L.wP64,
+				   L.flags, N, 1, 1.0);
+		hipMemsetAsync(x0, 0, n * sizeof(T), c->s);
+		vcycle<T>(c, 0, p);
+		used++;
+		hipMemcpyAsync(Z[j], x0, n * sizeof(T), hipMemcpyDeviceToDevice, c->s);
+		T *w = V[j + 1];
+		{
+			Timed tm(c, sizeof(T) == 8 ? "matvec_f64_L0" : "matvec_f32_L0", 0);
+			hipLaunchKernelGGL((k_residual<T, T, 1>), plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, view<T>(L),
+					   (const T *)Z[j], (const T *)nullptr, w, L.bcid, bcview<T>(L), (double *)nullptr);
+		}
+		/* classical Gram-Schmidt, twice, coefficients to the host */
+		double *h1 = c->dh, *h2 = c->dh + c->kr_m; /* kr_m >= m + 1 */
+		for (int pass = 0; pass < 2; pass++) {
+			double *hh = pass ? h2 : h1;
+			multidot_dev<T, T>(c, (const T *const *)dV, j + 1, (const T *)w, n, hh);
+			hipLaunchKernelGGL((k_multiaxpy<T, T>), lg, dim3(256), 0, c->s, w, (const T *const *)dV, (const double *)hh,
+					   j + 1, n, -1.0);
+		}
+		multidot_dev<T, T>(c, (const T *const *)(dV + j + 1), 1, (const T *)w, n, dh);
+		hipMemcpyAsync(c->hscal + 16, h1, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, c->s);
+		hipMemcpyAsync(c->hscal + 16 + m + 1, h2, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, c->s);
+		hipMemcpyAsync(c->hscal + 8, dh, sizeof(double), hipMemcpyDeviceToHost, c->s);
+		hipLaunchKernelGGL(k_scale_rsqrt<T>, lg, dim3(256), 0, c->s, w, (const double *)dh, n);
+		hipStreamSynchronize(c->s);
+		for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = c->hscal[16 + i] + c->hscal[16 + m + 1 + i];
+		double hn = sqrt(c->hscal[8]);
+		H[(size_t)(j + 1) * m + j] = hn;
+		for (int i = 0; i < j; i++) {
+			double a = H[(size_t)i * m + j], b = H[(size_t)(i + 1) * m + j];
+			H[(size_t)i * m + j] = cs[i] * a + sn[i] * b;
+			H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+		}
+		double a = H[(size_t)j * m + j], b = H[(size_t)(j + 1) * m + j];
+		double rr = sqrt(a * a + b * b);
+		cs[j] = rr > 0 ? a / rr : 1.0;
+		sn[j] = rr > 0 ? b / rr : 0.0;
+		H[(size_t)j * m + j] = rr;
+		H[(size_t)(j + 1) * m + j] = 0;
+		g[j + 1] = -sn[j] * g[j];
+		g[j] = cs[j] * g[j];
+		if (fabs(g[j + 1]) <= tol * beta || hn == 0 || !std::isfinite(hn)) { j++; break; }
+	}
+	int k = j < m ? j : m;
+	std::vector<double> y(k, 0.0);
+	for (int i = k - 1; i >= 0; i--) {
+		double s = g[i];
+		for (int q = i + 1; q < k; q++) s -= H[(size_t)i * m + q] * y[q];
+		y[i] = H[(size_t)i * m + i] != 0 ? s / H[(size_t)i * m + i] : 0;
+	}
+	hipMemcpyAsync(c->dh, y.data(), k * sizeof(double), hipMemcpyHostToDevice, c->s);
+	hipLaunchKernelGGL((k_multiaxpy<T, T>), lg, dim3(256), 0, c->s, out, (const T *const *)c->diZ,
+			   (const double *)c->dh, k, n, 1.0);
+	hipStreamSynchronize(c->s);
+	return used;
+}
+
 template <typename T, typename R> static void gcr_flush(i3g_ctx *c, GcrHost &G, int k, R *e)
 {
 	int m = c->kr_m;
@@ -1134,13 +1253,18 @@ static int gcr(i3g_ctx *c, GcrHost &G, R *r, R *e, double target, int max_new, c
 			k = 0;
 		}
 		T *pk = (T *)c->P[k], *qk = (T *)c->Q[k];
-		/* p_k = M W^-1 r */
-		T *x0 = lx<T>(L), *b0 = lb<T>(L);
-		hipLaunchKernelGGL((k_wscale<T, R>), lg, dim3(256), 0, c->s, b0, (const R *)r, (const double *)L.wP64, L.flags, N, 1, 1.0);
-		hipMemsetAsync(x0, 0, n * sizeof(T), c->s);
-		vcycle<T>(c, 0, p);
-		used++;
-		hipMemcpyAsync(pk, x0, n * sizeof(T), hipMemcpyDeviceToDevice, c->s);
+		if (p->inner_m > 0) {
+			/* p_k = inner FGMRES(m) approximation of (W A)^-1 r */
+			used += inner_fgmres<T, R>(c, r, pk, p->inner_m, p->inner_tol, p);
+		} else {
+			/* p_k = M W^-1 r */
+			T *x0 = lx<T>(L), *b0 = lb<T>(L);
+			hipLaunchKernelGGL((k_wscale<T, R>), lg, dim3(256), 0, c->s, b0, (const R *)r, (const double *)L.wP64, L.flags, N, 1, 1.0);
+			hipMemsetAsync(x0, 0, n * sizeof(T), c->s);
+			vcycle<T>(c, 0, p);
+			used++;
+			hipMemcpyAsync(pk, x0, n * sizeof(T), hipMemcpyDeviceToDevice, c->s);
+		}
 		/* q_k = W A p_k in FP64 */
 		{
 			Timed tm(c, "matvec_f64_L0", 0);
@@ -1220,7 +1344,7 @@ extern "C" void i3g_stokes_default_params(i3g_stokes_params *p)
 	p->rtol = 1e-8;
 	p->precision = 32;
 	p->max_outer = 60;
-	p->krylov = 200;
+	p->krylov = 400;
 	p->max_vcycles = 2000;
 	/* Sweep counts: the reference's 8/16/16/32 per level and 64 on the
 	 * coarsest (mode.t3c:3030-3031), with the finest level doubled; the
@@ -1235,6 +1359,8 @@ extern "C" void i3g_stokes_default_params(i3g_stokes_params *p)
 		p->nu_post[l] = l < 4 ? post[l] : 32;
 	}
 	p->nu_coarse = 64;
+	p->inner_m = 10;
+	p->inner_tol = 0.1;
 	p->p0koef = 0.6; p->p1koef = 1.0; p->v0koef = 0.85; p->v1koef = 0.7;
 	p->inner_rtol = 1e-6;
 	p->warm_start = 0;
@@ -1251,6 +1377,7 @@ static int solve_impl(i3g_ctx *c, const i3g_stokes_params *p, i3g_stats *st)
 	int rc = p->precision == 64 ? ensure_krylov<double>(c, m) : ensure_krylov<float>(c, m);
 	if (rc) return rc;
 	m = c->kr_m;
+	if (p->inner_m + 1 > m) return I3G_EINVAL;
 	hipEvent_t t0, t1;
 	hipEventCreate(&t0);
 	hipEventCreate(&t1);

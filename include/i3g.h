@@ -144,6 +144,8 @@ typedef struct {
 	int max_vcycles;      /* V-cycle cap per outer iteration */
 	int nu_pre[I3G_MAXLEV], nu_post[I3G_MAXLEV]; /* sweeps per level */
 	int nu_coarse;        /* sweeps on the coarsest level */
+	int inner_m;          /* inner FGMRES length per outer direction (0: one V-cycle) */
+	double inner_tol;     /* inner FGMRES relative reduction */
 	double p0koef, p1koef, v0koef, v1koef;       /* mode.t3c:3032-3036 */
 	double inner_rtol;    /* mixed precision: inner reduction per outer */
 	int warm_start;       /* 0: start from given fields, 1: zero */
@@ -190,6 +192,85 @@ int i3g_set_timing(i3g_ctx *ctx, int on);
  * and algorithmic bytes per launch; returns the count. */
 int i3g_kernel_stats(i3g_ctx *ctx, char *names, int name_len, double *avg_ms,
 		     long *calls, double *bytes, int cap);
+
+/* ------------------------------------------------------------------ */
+/* Markers                                                            */
+/* ------------------------------------------------------------------ */
+
+/* Device-resident marker set on one level-0 grid (the reference's
+ * markx/marky/markz/markk/markv/markt arrays, head3mg.c:97-105), kept
+ * sorted by cell.  Marker ids give the reference's marker order. */
+typedef struct i3g_markers i3g_markers;
+
+/* Rock properties of one marker type (the per-type tables of the reference
+ * deck, init.t3c ROCKS_DESCRIPTIONS; mark3mg.c:1114-1299, 3006-3024):
+ * Newtonian rheology branch A (E = V = 0) or B (Arrhenius), density
+ * ro*(1-bro*(T-298.15))*(1+aro*(P_bar-1)*1e-3), conductivity
+ * (kt + kf/(T+77))*exp(kp*P_bar).  Brittle/power-law terms are zero. */
+typedef struct {
+	double nu0, dh, dv;        /* marknu, markdh (J), markdv (J/bar) */
+	double n0, n1;             /* markn0, markn1 viscosity limits */
+	double ro, bro, aro;       /* markro, markbb, markaa */
+	double cp, kt, kf, kp, ht; /* markcp, markkt, markkf, markkp, markht */
+} i3g_rock;
+
+/* Parameters of the marker -> node transfer (mode.t3c values used by
+ * ronurecalc): nubeg, nuend, nukoef, densimod (0|1), timestep (>0: the
+ * stored marker viscosity is reused), vdeep/nudeep viscosity cap below
+ * vdeep, viscmod (0 only). */
+typedef struct {
+	double nubeg, nuend, nukoef;
+	int densimod;
+	double timestep;
+	double vdeep, nudeep;
+	int viscmod;
+} i3g_mk_params;
+
+int i3g_markers_create(long nx, long ny, long nz, const double *gx, const double *gy,
+		       const double *gz, long capacity, i3g_markers **out);
+int i3g_markers_destroy(i3g_markers *m);
+/* id may be NULL (ids 0..n-1); type is the rock type (markt) */
+int i3g_markers_upload(i3g_markers *m, long n, const float *x, const float *y, const float *z,
+		       const float *tk, const float *eta, const signed char *type, const int *id);
+int i3g_markers_download(i3g_markers *m, float *x, float *y, float *z, float *tk, float *eta,
+			 signed char *type, int *id);
+long i3g_markers_count(const i3g_markers *m);
+
+/* Cell of every marker by the reference's binary searches m1serch/m2serch/
+ * m3serch (mark3mg.c:1021-1087) after clamping the coordinate to the
+ * domain (prcalc1, mark3mg.c:2930-2932); key = m3*nx*ny + m1*ny + m2
+ * (mark3mg.c:810-812).  Outputs (host, may be NULL) in resident order. */
+int i3g_markers_cell_index(i3g_markers *m, int *m1, int *m2, int *m3, int *key);
+
+/* Stable sort of the resident markers by (cell key, id): counting sort
+ * plus an in-cell ordering by id; deterministic.  order (host, may be
+ * NULL) receives the previous position of each marker. */
+int i3g_markers_sort(i3g_markers *m, long *order);
+
+/* Marker -> node transfer; replaces ronurecalc (mark3mg.c:404-1015) for
+ * marker types without melt/phase-change/water branches.  pr: cell
+ * pressures (level-0 node layout, host).  Node arrays are in/out (a node
+ * with too little marker weight keeps its value, as the reference).
+ * Markers must be sorted.  Deterministic: every node sums its markers in
+ * ascending id, the order of the reference's serial loop. */
+int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int nrock, const i3g_mk_params *prm,
+			 const double *pr, float *nu, float *ro, float *cp, float *kt, float *ht,
+			 float *et, float *tk, float *nxx, float *nxy, float *nxz, float *nyz);
+
+/* Temperature boundary rows applied at the end of every marker -> node
+ * transfer, as ronurecalc calls tkrecalc (mark3mg.c:1007, heat3mg.c:849-899):
+ * bondm_t is the level-0 T block of the reference's bondm (bondm +
+ * 8*nodenum, nodenum entries, record number or 0), bondv/bondn its record
+ * tables (bondv1[r][0..1] at bondv[2r..2r+1], bondn1[r] = partner node + 1
+ * or 0), nrec records.  bondm_t NULL switches the update off. */
+int i3g_markers_set_tbc(i3g_markers *m, const int *bondm_t, const float *bondv, const int *bondn,
+			long nrec);
+
+/* RK4 advection of the resident markers by dt in the staggered velocity
+ * field (vxyzcalc1 + movemark, mark3mg.c:215-284, 2347-2530); markers that
+ * leave the grid get tk = eta = 0 (mark3mg.c:280-284). */
+int i3g_markers_advect_rk4(i3g_markers *m, double dt, const float *vx, const float *vy,
+			   const float *vz);
 
 const char *i3g_error_string(int code);
 

@@ -121,3 +121,68 @@ def gs_sweep(L, etas, bondm, bondv, bondn, b, x, p0koef, v0koef):
     lib().or_gs_sweep(C.byref(L), *[_p(e) for e in etas], _p(bondm), _p(bondv),
                       _p(bondn), _p(b), _p(x), C.c_double(p0koef), C.c_double(v0koef))
     return x
+
+
+# ---- markers ----------------------------------------------------------------
+
+ROCK_FIELDS = ("nu0", "dh", "dv", "n0", "n1", "ro", "bro", "aro", "cp", "kt", "kf", "kp", "ht")
+
+
+class Rock(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ROCK_FIELDS]
+
+
+def rocks_array(rocks):
+    """rocks: list of dicts with ROCK_FIELDS"""
+    arr = (Rock * len(rocks))()
+    for i, r in enumerate(rocks):
+        for f in ROCK_FIELDS:
+            setattr(arr[i], f, float(r.get(f, 0.0)))
+    return arr
+
+
+def mk_cells(x, y, z, gx, gy, gz):
+    n = len(x)
+    out = np.zeros(3 * n, np.int32)
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
+    lib().or_mk_cells(C.c_long(n), _p(x), _p(y), _p(z), _p(gx), C.c_long(len(gx)), _p(gy),
+                      C.c_long(len(gy)), _p(gz), C.c_long(len(gz)), _p(out))
+    return out.reshape(n, 3)
+
+
+def ronurecalc(x, y, z, tk, eta, mtype, gx, gy, gz, pr, rocks, fld, nubeg, nuend, nukoef=0.0,
+               densimod=1, timestep=1.0, vdeep=1e30, nudeep=1e30):
+    """Returns (fld (11,N) float32, eta float32) after the reference's
+    marker -> node transfer; inputs are not modified."""
+    fld = np.array(fld, np.float32, copy=True)
+    eta = np.array(eta, np.float32, copy=True)
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
+    ra = rocks_array(rocks)
+    lib().or_ronurecalc(C.c_long(len(x)), _p(x), _p(y), _p(z), _p(tk), _p(eta), _p(mtype), _p(gx),
+                        C.c_long(len(gx)), _p(gy), C.c_long(len(gy)), _p(gz), C.c_long(len(gz)), _p(pr), ra,
+                        C.c_double(nubeg), C.c_double(nuend), C.c_double(nukoef), C.c_int(densimod),
+                        C.c_double(timestep), C.c_double(vdeep), C.c_double(nudeep), _p(fld))
+    return fld, eta
+
+
+def vxyz(vx, vy, vz, gx, gy, gz, x, y, z):
+    v = np.zeros(3)
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
+    lib().or_vxyz(_p(vx), _p(vy), _p(vz), _p(gx), C.c_long(len(gx)), _p(gy), C.c_long(len(gy)), _p(gz),
+                  C.c_long(len(gz)), C.c_double(x), C.c_double(y), C.c_double(z), _p(v))
+    return v
+
+
+def rk4(x, y, z, tk, eta, mtype, vx, vy, vz, gx, gy, gz, dt):
+    x, y, z, tk, eta = (np.array(a, np.float32, copy=True) for a in (x, y, z, tk, eta))
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
+    lib().or_rk4(C.c_long(len(x)), _p(x), _p(y), _p(z), _p(tk), _p(eta), _p(mtype), _p(vx), _p(vy), _p(vz),
+                 _p(gx), C.c_long(len(gx)), _p(gy), C.c_long(len(gy)), _p(gz), C.c_long(len(gz)),
+                 C.c_double(dt))
+    return x, y, z, tk, eta
+
+
+def tkrecalc(tk, bondm, bondv, bondn):
+    tk = np.array(tk, np.float32, copy=True)
+    lib().or_tkrecalc(C.c_long(len(tk)), _p(bondm), _p(bondv), _p(bondn), _p(tk))
+    return tk

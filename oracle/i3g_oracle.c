@@ -552,3 +552,276 @@ void or_gs_sweep(const orl_t *L, const double *etaC, const double *etaXY,
 		}
 	}
 }
+
+/* ------------------------------------------------------------------ */
+/* Markers                                                            */
+/* ------------------------------------------------------------------ */
+
+static long serch(const double *g, long n, double v)
+{
+	long lo = 0, hi = n - 1, mid;
+	do {
+		mid = (lo + hi) / 2;
+		if (g[mid] > v) hi = mid; else lo = mid;
+	} while (hi - lo > 1);
+	if (lo > n - 2) lo = n - 2;
+	return lo;
+}
+
+static double clampx(double v, double hi)
+{
+	if (v < 0) v = 0;
+	if (v > hi) v = hi;
+	return v;
+}
+
+void or_mk_cells(long n, const float *x, const float *y, const float *z, const double *gx, long nx,
+		 const double *gy, long ny, const double *gz, long nz, int *out)
+{
+	long i;
+	for (i = 0; i < n; i++) {
+		out[3 * i + 0] = (int)serch(gx, nx, clampx(x[i], gx[nx - 1]));
+		out[3 * i + 1] = (int)serch(gy, ny, clampx(y[i], gy[ny - 1]));
+		out[3 * i + 2] = (int)serch(gz, nz, clampx(z[i], gz[nz - 1]));
+	}
+}
+
+/* prcalc1 (mark3mg.c:2917-3001) */
+static double or_prcalc(const double *pr, const double *gx, long nx, const double *gy, long ny, const double *gz,
+			long nz, double x, double y, double z, long *cell)
+{
+	long m1, m2, m3, a, b, c, m10, m20, m30;
+	double dx, dy, dz, p = 0;
+	x = clampx(x, gx[nx - 1]); y = clampx(y, gy[ny - 1]); z = clampx(z, gz[nz - 1]);
+	cell[0] = m1 = serch(gx, nx, x);
+	cell[1] = m2 = serch(gy, ny, y);
+	cell[2] = m3 = serch(gz, nz, z);
+	a = m1; if (x > (gx[a] + gx[a + 1]) / 2.0) a += 1; if (a < 1) a = 1; if (a > nx - 2) a = nx - 2;
+	b = m2; if (y > (gy[b] + gy[b + 1]) / 2.0) b += 1; if (b < 1) b = 1; if (b > ny - 2) b = ny - 2;
+	c = m3; if (z > (gz[c] + gz[c + 1]) / 2.0) c += 1; if (c < 1) c = 1; if (c > nz - 2) c = nz - 2;
+	dx = (gx[a + 1] - gx[a - 1]) / 2.0; dy = (gy[b + 1] - gy[b - 1]) / 2.0; dz = (gz[c + 1] - gz[c - 1]) / 2.0;
+	for (m30 = c; m30 <= c + 1; m30++) {
+		double ddz = ((gz[m30 - 1] + gz[m30]) / 2.0 - z) / dz;
+		if (m30 == c) ddz = -ddz;
+		ddz = 1.0 - ddz;
+		for (m10 = a; m10 <= a + 1; m10++) {
+			double ddx = ((gx[m10 - 1] + gx[m10]) / 2.0 - x) / dx;
+			if (m10 == a) ddx = -ddx;
+			ddx = 1.0 - ddx;
+			for (m20 = b; m20 <= b + 1; m20++) {
+				double ddy = ((gy[m20 - 1] + gy[m20]) / 2.0 - y) / dy;
+				if (m20 == b) ddy = -ddy;
+				ddy = 1.0 - ddy;
+				p += pr[m30 * nx * ny + m10 * ny + m20] * (ddx * ddy * ddz);
+			}
+		}
+	}
+	return p;
+}
+
+/* viscalc branches A/B and limits (mark3mg.c:1266-1299, 1626-1681) */
+static double or_visc(const or_rock_t *r, double mtk, double mpb, double nubeg, double nuend)
+{
+	double rt = 8.314 * mtk, nunewt = 0, nueff;
+	if (r->nu0) {
+		if (r->dh == 0 && r->dv == 0) nunewt = 1.0 / r->nu0;
+		else {
+			double e1 = (r->dh + r->dv * mpb) / rt;
+			if (e1 > 150.0) e1 = 150.0;
+			if (r->dh < 0 && r->dv > 0) { e1 = r->dh * (mtk - r->dv); if (e1 < -150.0) e1 = -150.0; }
+			if (r->dh < 0 && r->dv < 0) {
+				e1 = (-r->dh) * (1.0 - (mtk - (-r->dv)) / (-r->dv)) / 8.314 / (-r->dv);
+				if (e1 > 150.0) e1 = 150.0;
+			}
+			nunewt = 1.0 / (r->nu0 * exp(e1));
+		}
+	}
+	nueff = nunewt;
+	if (nueff <= 0) nueff = 1.0 / r->n1;
+	nueff = 1.0 / nueff;
+	if (nueff < r->n0) nueff = r->n0;
+	if (nueff > r->n1) nueff = r->n1;
+	if (nueff < nubeg) nueff = nubeg;
+	if (nueff > nuend) nueff = nuend;
+	return nueff;
+}
+
+void or_ronurecalc(long n, const float *x, const float *y, const float *z, const float *tk, float *eta,
+		   const signed char *type, const double *gx, long nx, const double *gy, long ny, const double *gz,
+		   long nz, const double *pr, const or_rock_t *rocks, double nubeg, double nuend, double nukoef,
+		   int densimod, double timestep, double vdeep, double nudeep, float *fld)
+{
+	long N = nx * ny * nz, i, q;
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	float *acc = calloc((size_t)(11 * N), sizeof(float));
+	double *w = calloc((size_t)(6 * N), sizeof(double)); /* w1 w0 wxx wxy wxz wyz */
+	for (i = 0; i < n; i++) {
+		long cell[3], m1, m2, m3, m10, m20, m30;
+		double mpb, mtk, mnu, mro, mbb = 0, mcp, mkt, mht, dx, dy, dz;
+		const or_rock_t *r;
+		int mt = type[i];
+		if (!(x[i] > 0 && y[i] > 0 && z[i] > 0 && x[i] < xs && y[i] < ys && z[i] < zs && mt < 50 && tk[i] > 0))
+			continue;
+		r = &rocks[mt];
+		mpb = or_prcalc(pr, gx, nx, gy, ny, gz, nz, x[i], y[i], z[i], cell) * 1e-5;
+		mtk = tk[i];
+		mnu = eta[i];
+		if (mnu <= 0 || timestep <= 0) {
+			mnu = or_visc(r, mtk, mpb, nubeg, nuend);
+			eta[i] = (float)mnu;
+		}
+		if (densimod == 0) mro = r->ro;
+		else {
+			mro = r->ro * (1.0 - r->bro * (mtk - 298.15)) * (1.0 + r->aro * (mpb - 1.0) * 1e-3);
+			mbb = r->bro / (1.0 - r->bro * (mtk - 298.15));
+		}
+		mcp = r->cp;
+		mkt = (r->kt + r->kf / (mtk + 77.0)) * exp(r->kp * mpb);
+		mht = r->ht;
+		if (y[i] > vdeep && mnu > nudeep) mnu = nudeep;
+		if (y[i] < 5e+4 && x[i] > xs - 1e+4 && z[i] > 1.5e+5 && z[i] < 8.7e+5 && mnu > 1e+19) mnu = 1e+19;
+		m10 = cell[0]; m20 = cell[1]; m30 = cell[2];
+		dx = gx[m10 + 1] - gx[m10]; dy = gy[m20 + 1] - gy[m20]; dz = gz[m30 + 1] - gz[m30];
+		for (m3 = m30; m3 <= m30 + 1; m3++) {
+			double ddz = 1.0 - fabs((gz[m3] - z[i]) / dz);
+			for (m1 = m10; m1 <= m10 + 1; m1++) {
+				double ddx = 1.0 - fabs((gx[m1] - x[i]) / dx);
+				for (m2 = m20; m2 <= m20 + 1; m2++) {
+					double ddy = 1.0 - fabs((gy[m2] - y[i]) / dy);
+					double swt = ddx * ddy * ddz, s1;
+					long m4 = m3 * nx * ny + m1 * ny + m2;
+					if (ddx > nukoef && ddy > nukoef && ddz > nukoef) {
+						acc[m4] = (float)(acc[m4] + mnu * swt);
+						w[m4] += swt;
+					}
+					if (m1 > m10 && m2 > m20 && m3 > m30) {
+						s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (0.5 - fabs(ddy - 0.5)) * (0.5 - fabs(ddz - 0.5));
+						acc[7 * N + m4] = (float)(acc[7 * N + m4] + mnu * s1);
+						w[2 * N + m4] += s1;
+					}
+					if (m3 == m30 && ddx > 0.5 && ddy > 0.5) {
+						s1 = 8.0 * (ddx - 0.5) * (ddy - 0.5) * (0.5 - fabs(ddz - 0.5));
+						acc[8 * N + m4] = (float)(acc[8 * N + m4] + mnu * s1);
+						w[3 * N + m4] += s1;
+					}
+					if (m2 == m20 && ddx > 0.5 && ddz > 0.5) {
+						s1 = 8.0 * (ddx - 0.5) * (0.5 - fabs(ddy - 0.5)) * (ddz - 0.5);
+						acc[9 * N + m4] = (float)(acc[9 * N + m4] + mnu * s1);
+						w[4 * N + m4] += s1;
+					}
+					if (m1 == m10 && ddy > 0.5 && ddz > 0.5) {
+						s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (ddy - 0.5) * (ddz - 0.5);
+						acc[10 * N + m4] = (float)(acc[10 * N + m4] + mnu * s1);
+						w[5 * N + m4] += s1;
+					}
+					if (swt > 0) {
+						acc[1 * N + m4] = (float)(acc[1 * N + m4] + mro * swt);
+						acc[2 * N + m4] = (float)(acc[2 * N + m4] + mcp * mro * swt);
+						acc[3 * N + m4] = (float)(acc[3 * N + m4] + mkt * swt);
+						acc[4 * N + m4] = (float)(acc[4 * N + m4] + mht * swt);
+						acc[5 * N + m4] = (float)(acc[5 * N + m4] + mbb * swt);
+						acc[6 * N + m4] = (float)(acc[6 * N + m4] + mtk * swt);
+						w[N + m4] += swt;
+					}
+				}
+			}
+		}
+	}
+	for (i = 0; i < N; i++) {
+		if (w[i] > 0.01) fld[i] = (float)(acc[i] / w[i]);
+		for (q = 0; q < 4; q++)
+			if (w[(2 + q) * N + i] > 0.01) fld[(7 + q) * N + i] = (float)(acc[(7 + q) * N + i] / w[(2 + q) * N + i]);
+		if (w[N + i] > 0.01) {
+			fld[N + i] = (float)(acc[N + i] / w[N + i]);
+			fld[2 * N + i] = (float)(acc[2 * N + i] / w[N + i] / fld[N + i]);
+			for (q = 3; q <= 6; q++) fld[q * N + i] = (float)(acc[q * N + i] / w[N + i]);
+		}
+		if (fld[i] < nubeg) fld[i] = (float)nubeg;
+		if (fld[i] > nuend) fld[i] = (float)nuend;
+	}
+	free(acc);
+	free(w);
+}
+
+void or_vxyz(const float *vx, const float *vy, const float *vz, const double *gx, long nx, const double *gy,
+	     long ny, const double *gz, long nz, double x, double y, double z, double *v)
+{
+	long m1, m2, m3, comp;
+	x = clampx(x, gx[nx - 1]); y = clampx(y, gy[ny - 1]); z = clampx(z, gz[nz - 1]);
+	m1 = serch(gx, nx, x); m2 = serch(gy, ny, y); m3 = serch(gz, nz, z);
+	for (comp = 0; comp < 3; comp++) {
+		long a = m1, b = m2, c = m3, m10, m20, m30;
+		double dx, dy, dz, acc = 0;
+		const float *f = comp == 0 ? vx : comp == 1 ? vy : vz;
+		if (comp == 0) { if (a > nx - 2) a = nx - 2; }
+		else { if (x < (gx[a] + gx[a + 1]) / 2.0) a -= 1; if (a < 0) a = 0; if (a > nx - 3) a = nx - 3; }
+		if (comp == 1) { if (b > ny - 2) b = ny - 2; }
+		else { if (y < (gy[b] + gy[b + 1]) / 2.0) b -= 1; if (b < 0) b = 0; if (b > ny - 3) b = ny - 3; }
+		if (comp == 2) { if (c > nz - 2) c = nz - 2; }
+		else { if (z < (gz[c] + gz[c + 1]) / 2.0) c -= 1; if (c < 0) c = 0; if (c > nz - 3) c = nz - 3; }
+		dx = comp == 0 ? gx[a + 1] - gx[a] : (gx[a + 2] - gx[a]) / 2.0;
+		dy = comp == 1 ? gy[b + 1] - gy[b] : (gy[b + 2] - gy[b]) / 2.0;
+		dz = comp == 2 ? gz[c + 1] - gz[c] : (gz[c + 2] - gz[c]) / 2.0;
+		for (m30 = c; m30 <= c + 1; m30++) {
+			double ddz = comp == 2 ? (gz[m30] - z) / dz : ((gz[m30] + gz[m30 + 1]) / 2.0 - z) / dz;
+			if (m30 == c) ddz = -ddz;
+			ddz = 1.0 - ddz;
+			for (m10 = a; m10 <= a + 1; m10++) {
+				double ddx = comp == 0 ? (gx[m10] - x) / dx : ((gx[m10] + gx[m10 + 1]) / 2.0 - x) / dx;
+				if (m10 == a) ddx = -ddx;
+				ddx = 1.0 - ddx;
+				for (m20 = b; m20 <= b + 1; m20++) {
+					double ddy = comp == 1 ? (gy[m20] - y) / dy : ((gy[m20] + gy[m20 + 1]) / 2.0 - y) / dy;
+					if (m20 == b) ddy = -ddy;
+					ddy = 1.0 - ddy;
+					acc += f[m30 * nx * ny + m10 * ny + m20] * (ddx * ddy * ddz);
+				}
+			}
+		}
+		v[comp] = acc;
+	}
+}
+
+void or_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, const signed char *type,
+	    const float *vx, const float *vy, const float *vz, const double *gx, long nx, const double *gy,
+	    long ny, const double *gz, long nz, double dt)
+{
+	long i;
+	int q;
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	for (i = 0; i < n; i++) {
+		double v1[3], v2[3], v3[3], v4[3], v[3], px = x[i], py = y[i], pz = z[i];
+		int mt = type[i];
+		if (!((x[i] >= 0 && y[i] >= 0 && z[i] >= 0 && x[i] <= xs && y[i] <= ys && z[i] <= zs) || mt < 50 || mt >= 100))
+			continue;
+		or_vxyz(vx, vy, vz, gx, nx, gy, ny, gz, nz, px, py, pz, v1);
+		or_vxyz(vx, vy, vz, gx, nx, gy, ny, gz, nz, px + v1[0] * dt / 2.0, py + v1[1] * dt / 2.0, pz + v1[2] * dt / 2.0, v2);
+		or_vxyz(vx, vy, vz, gx, nx, gy, ny, gz, nz, px + v2[0] * dt / 2.0, py + v2[1] * dt / 2.0, pz + v2[2] * dt / 2.0, v3);
+		or_vxyz(vx, vy, vz, gx, nx, gy, ny, gz, nz, px + v3[0] * dt, py + v3[1] * dt, pz + v3[2] * dt, v4);
+		for (q = 0; q < 3; q++) v[q] = (v1[q] + 2.0 * v2[q] + 2.0 * v3[q] + v4[q]) / 6.0;
+		if (mt < 100) {
+			x[i] += dt * v[0] - 0.0;
+			y[i] += dt * v[1];
+			z[i] += dt * v[2];
+			if (x[i] < 0 || y[i] < 0 || z[i] < 0 || x[i] > xs || y[i] > ys || z[i] > zs) {
+				tk[i] = 0;
+				eta[i] = 0;
+			}
+		}
+	}
+}
+
+void or_tkrecalc(long nodenum, const int *bondm, const float *bondv, const int *bondn, float *tk)
+{
+	long m;
+	float *tk1 = malloc(sizeof(float) * (size_t)nodenum);
+	memcpy(tk1, tk, sizeof(float) * (size_t)nodenum);
+	for (m = 0; m < nodenum; m++) {
+		int rec = bondm[8 * nodenum + m];
+		if (rec > 0) {
+			tk[m] = bondv[2 * rec];
+			if (bondn[rec]) tk[m] += bondv[2 * rec + 1] * tk1[bondn[rec] - 1];
+		}
+	}
+	free(tk1);
+}

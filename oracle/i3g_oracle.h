@@ -75,4 +75,38 @@ void or_gs_sweep(const orl_t *L, const double *etaC, const double *etaXY,
 		 const float *bondv, const int *bondn, const double *b, double *x,
 		 double p0koef, double v0koef);
 
+/* ---- markers (test infrastructure) ---------------------------------- */
+
+typedef struct {
+	double nu0, dh, dv, n0, n1, ro, bro, aro, cp, kt, kf, kp, ht;
+} or_rock_t;
+
+/* m1serch/m2serch/m3serch after clamping to the domain (mark3mg.c:1021-1087,
+ * 2930-2936); out[3*i..] = (m1, m2, m3) */
+void or_mk_cells(long n, const float *x, const float *y, const float *z, const double *gx, long nx,
+		 const double *gy, long ny, const double *gz, long nz, int *out);
+
+/* ronurecalc (mark3mg.c:404-1015) for Newtonian rock types without air/
+ * water, eclogite or melt branches: serial marker loop, float accumulators,
+ * double weights, normalisation.  fld: 11 planes of N (nu ro cp kt ht et tk
+ * nxx nxy nxz nyz), in/out.  eta (markv) in/out. */
+void or_ronurecalc(long n, const float *x, const float *y, const float *z, const float *tk, float *eta,
+		   const signed char *type, const double *gx, long nx, const double *gy, long ny, const double *gz,
+		   long nz, const double *pr, const or_rock_t *rocks, double nubeg, double nuend, double nukoef,
+		   int densimod, double timestep, double vdeep, double nudeep, float *fld);
+
+/* vxyzcalc1 (mark3mg.c:2347-2530): v[3] at a point */
+void or_vxyz(const float *vx, const float *vy, const float *vz, const double *gx, long nx, const double *gy,
+	     long ny, const double *gz, long nz, double x, double y, double z, double *v);
+
+/* movemark RK4 position update (mark3mg.c:215-284) for markers of type <
+ * 50 inside the grid; out-of-grid markers get tk = eta = 0. */
+void or_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, const signed char *type,
+	    const float *vx, const float *vy, const float *vz, const double *gx, long nx, const double *gy,
+	    long ny, const double *gz, long nz, double dt);
+
+/* tkrecalc (heat3mg.c:846-899): temperature boundary rows on level 0 from
+ * a copy of tk (reference bondm layout, T records at 8*nodenum). */
+void or_tkrecalc(long nodenum, const int *bondm, const float *bondv, const int *bondn, float *tk);
+
 #endif

@@ -124,9 +124,125 @@ def g3(R, n, which=0, seed=1):
     print(f"g3_converged_c{which}_{n}.npz written; max |b - A x| = {np.abs(r).max():.3e}")
 
 
+ROCKS = [  # types 0,1 unused (air/water branches), 2..6 used
+    dict(nu0=1e21, n0=1e18, n1=1e25, ro=3000, cp=1000, kt=2.0),
+    dict(nu0=1e21, n0=1e18, n1=1e25, ro=1000, cp=1000, kt=2.0),
+    dict(nu0=1e21, dh=0, dv=0, n0=1e18, n1=1e25, ro=3300, bro=3e-5, aro=1e-3, cp=1000, kt=3.0, kf=0, kp=0, ht=1e-9),
+    dict(nu0=1e23, dh=0, dv=0, n0=1e18, n1=1e25, ro=3200, bro=2e-5, aro=1e-3, cp=1100, kt=0.64, kf=807.0, kp=4e-6, ht=2e-6),
+    dict(nu0=2e16, dh=3e5, dv=4.0, n0=1e19, n1=1e24, ro=3350, bro=3e-5, aro=1e-3, cp=1000, kt=0.73, kf=1293.0, kp=4e-6, ht=2e-8),
+    dict(nu0=1e10, dh=3e5, dv=0, n0=1e18, n1=1e25, ro=3250, bro=3e-5, aro=1e-3, cp=1000, kt=1.0, kf=500.0, kp=0, ht=0),
+    dict(nu0=0, dh=0, dv=0, n0=1e20, n1=5e22, ro=2900, bro=3e-5, aro=1e-3, cp=1200, kt=2.5, kf=0, kp=0, ht=1e-7),
+]
+
+
+def g5(R):
+    """Marker -> node (ronurecalc), cell search and vxyzcalc1 through the
+    reference, on a small non-uniform grid."""
+    import ctypes as C
+    rng = np.random.default_rng(77)
+    nx, ny, nz = 11, 9, 10
+    gx = np.cumsum(np.r_[0, rng.uniform(0.7, 1.3, nx - 1)]) * 5e3
+    gy = np.cumsum(np.r_[0, rng.uniform(0.7, 1.3, ny - 1)]) * 5e3
+    gz = np.cumsum(np.r_[0, rng.uniform(0.7, 1.3, nz - 1)]) * 5e3
+    N = nx * ny * nz
+    h = I.hierarchy(nx, ny, nz, 0)
+    rules, nr = I.freeslip_rules(273.0, 1600.0)
+    bondm, bondv, bondn = I.expand_bc(rules, nr, h)
+    R.setup_grid(gx, gy, gz, 0)
+    R.set_bc(bondm, bondv, bondn)
+    # markers: 2x2x2 per cell with jitter, generation order = cell order,
+    # plus a few outside the domain and on cell lines
+    xs, ys, zs = [], [], []
+    for k in range(nz - 1):
+        for i in range(nx - 1):
+            for j in range(ny - 1):
+                for a in range(2):
+                    for b in range(2):
+                        for c in range(2):
+                            xs.append(gx[i] + (b + 0.5 + rng.uniform(-0.45, 0.45)) * (gx[i + 1] - gx[i]) / 2)
+                            ys.append(gy[j] + (c + 0.5 + rng.uniform(-0.45, 0.45)) * (gy[j + 1] - gy[j]) / 2)
+                            zs.append(gz[k] + (a + 0.5 + rng.uniform(-0.45, 0.45)) * (gz[k + 1] - gz[k]) / 2)
+    extra = 64
+    xs += list(rng.uniform(-2e3, gx[-1] + 2e3, extra))
+    ys += list(rng.uniform(-2e3, gy[-1] + 2e3, extra))
+    zs += list(rng.uniform(-2e3, gz[-1] + 2e3, extra))
+    x = np.array(xs, np.float32); y = np.array(ys, np.float32); z = np.array(zs, np.float32)
+    x[:5] = np.float32(gx[3]); y[5:10] = np.float32(gy[4])  # exactly on grid lines
+    n = len(x)
+    mtype = rng.integers(2, 7, n).astype(np.int8)
+    tk = rng.uniform(600, 1700, n).astype(np.float32)
+    tk[::97] = 0  # excluded markers (markk <= 0)
+    eta = np.where(rng.uniform(size=n) < 0.5, 0.0, 10 ** rng.uniform(19, 23, n)).astype(np.float32)
+    pr = rng.uniform(1e7, 1e9, N)
+    # reference globals
+    R.set("marknum", n, C.c_long)
+    R.arr("markx", C.c_float, R.MAXMRK)[:n] = x
+    R.arr("marky", C.c_float, R.MAXMRK)[:n] = y
+    R.arr("markz", C.c_float, R.MAXMRK)[:n] = z
+    R.arr("markk", C.c_float, R.MAXMRK)[:n] = tk
+    R.arr("markv", C.c_float, R.MAXMRK)[:n] = eta
+    R.arr("markt", C.c_byte, R.MAXMRK)[:n] = mtype
+    R.arr("markrr", C.c_float, R.MAXMRK)[:n] = 0
+    R.arr("marke", C.c_float, R.MAXMRK)[:n] = 0
+    names = dict(nu0="marknu", dh="markdh", dv="markdv", n0="markn0", n1="markn1", ro="markro", bro="markbb",
+                 aro="markaa", cp="markcp", kt="markkt", kf="markkf", kp="markkp", ht="markht")
+    for f, gname in names.items():
+        a = R.arr(gname, C.c_double, R.MAXTMR)
+        a[:] = 0
+        for t, r in enumerate(ROCKS):
+            a[t] = r.get(f, 0.0)
+    for gname in ("markss", "markll", "marka0", "marka1", "markb0", "markb1", "marke0", "marke1"):
+        R.arr(gname, C.c_double, R.MAXTMR)[:] = 0
+    R.arr("markmm", C.c_double, R.MAXTMR)[:] = 1.0
+    R.set("densimod", 1, C.c_int)
+    for k, v in dict(eroslev=-1e30, sedilev=1e30, waterlev=1e30, vdeep=gy[-1] * 0.8, nudeep=1e21, nukoef=0.0,
+                     nubeg=1e18, nuend=1e25, timestep=1e10).items():
+        R.set(k, v)
+    R.set("viscmod", 0, C.c_int)
+    R.f64("pr")[:N] = pr
+    init = np.zeros((11, N), np.float32)
+    init[0] = 5e20; init[1] = 3111; init[2] = 999; init[3] = 2.2; init[4] = 1e-9; init[5] = 1e-5
+    init[6] = 1000; init[7:] = 7e20
+    gnames = ["nu", "ro", "cp", "kt", "ht", "et", "tk", "nxx", "nxy", "nxz", "nyz"]
+    for q, gname in enumerate(gnames):
+        R.f32(gname)[:N] = init[q]
+    R.lib.refh_ronurecalc()
+    out = np.stack([R.f32(gname)[:N].copy() for gname in gnames])
+    eta_out = R.arr("markv", C.c_float, R.MAXMRK)[:n].copy()
+    # cell search at random points (clamped as prcalc1 does)
+    pts = np.stack([rng.uniform(-3e3, gx[-1] + 3e3, 400), rng.uniform(-3e3, gy[-1] + 3e3, 400),
+                    rng.uniform(-3e3, gz[-1] + 3e3, 400)], 1).astype(np.float32)
+    pts[:20, 0] = gx[rng.integers(0, nx, 20)]
+    cells = np.array([[R.lib.refh_m1serch(min(max(float(p[0]), 0.0), gx[-1])),
+                       R.lib.refh_m2serch(min(max(float(p[1]), 0.0), gy[-1])),
+                       R.lib.refh_m3serch(min(max(float(p[2]), 0.0), gz[-1]))] for p in pts], np.int32)
+    # staggered velocity interpolation
+    vxf = rng.standard_normal(N).astype(np.float32) * 1e-9
+    vyf = rng.standard_normal(N).astype(np.float32) * 1e-9
+    vzf = rng.standard_normal(N).astype(np.float32) * 1e-9
+    R.f32("vx")[:N] = vxf; R.f32("vy")[:N] = vyf; R.f32("vz")[:N] = vzf
+    vout = np.zeros((len(pts), 3))
+    buf = (C.c_double * 3)()
+    for q, p in enumerate(pts):
+        R.lib.refh_vxyzcalc1(C.c_double(float(p[0])), C.c_double(float(p[1])), C.c_double(float(p[2])), buf)
+        vout[q] = list(buf)
+    np.savez_compressed(os.path.join(HERE, "g5_markers.npz"), gx=gx, gy=gy, gz=gz, bondm=bondm, bondv=bondv,
+                        bondn=bondn, x=x, y=y, z=z, mtype=mtype, tk=tk, eta=eta, pr=pr, init=init, out=out,
+                        eta_out=eta_out, pts=pts, cells=cells, vx=vxf, vy=vyf, vz=vzf, vout=vout,
+                        rocks=np.array([[r.get(f, 0.0) for f in ("nu0", "dh", "dv", "n0", "n1", "ro", "bro", "aro",
+                                                                    "cp", "kt", "kf", "kp", "ht")] for r in ROCKS]),
+                        nubeg=1e18, nuend=1e25, vdeep=gy[-1] * 0.8, nudeep=1e21, timestep=1e10)
+    print("g5_markers.npz written")
+
+
 if __name__ == "__main__":
     R = Ref()
     R.lib.refh_set_threads(1)
-    g1(R)
-    g3(R, 17)
-    g3(R, 25)
+    which = sys.argv[1:] or ["g1", "g3", "g5"]
+    if "g1" in which:
+        g1(R)
+    if "g3" in which:
+        g3(R, 17)
+        g3(R, 25)
+    if "g5" in which:
+        g5(R)

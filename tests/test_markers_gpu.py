@@ -1,0 +1,122 @@
+"""GPU parity tests of the marker path, through the C ABI.
+
+Cell indices and the sort order must be bit-exact (integer work).  The
+marker -> node fields follow the reference's serial accumulation order, so
+they are expected to agree to the last float bit except where the device's
+exp differs from libm by an ulp (Arrhenius viscosities): tolerance 2e-7
+relative, with the bit-exact fraction reported.
+"""
+import numpy as np
+import pytest
+
+import i3elvis_amd as I
+import oracle
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _markers(g):
+    M = I.Markers(g["gx"], g["gy"], g["gz"], len(g["x"]) + 16)
+    M.upload(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"])
+    return M
+
+
+def test_cell_index_bitexact(gpu):
+    g = golden("g5_markers.npz")
+    print("case: g5 cell index", len(g["x"]), flush=True)
+    M = _markers(g)
+    m1, m2, m3, key = M.cell_index()
+    c = oracle.mk_cells(g["x"], g["y"], g["z"], g["gx"], g["gy"], g["gz"])
+    assert np.array_equal(np.stack([m1, m2, m3], 1), c)
+    nx, ny = len(g["gx"]), len(g["gy"])
+    assert np.array_equal(key, c[:, 2] * nx * ny + c[:, 0] * ny + c[:, 1])
+    M.close()
+
+
+def test_sort_is_stable_by_cell_then_id(gpu):
+    g = golden("g5_markers.npz")
+    rng = np.random.default_rng(9)
+    perm = rng.permutation(len(g["x"]))
+    M = I.Markers(g["gx"], g["gy"], g["gz"], len(g["x"]) + 16)
+    ids = np.arange(len(g["x"]), dtype=np.int32)
+    M.upload(g["x"][perm], g["y"][perm], g["z"][perm], g["tk"][perm], g["eta"][perm], g["mtype"][perm], ids[perm])
+    key0 = M.cell_index()[3]
+    order = M.sort()
+    d = M.download()
+    expect = np.lexsort((ids[perm], key0))
+    assert np.array_equal(order, expect)
+    assert np.array_equal(d["id"], ids[perm][expect])
+    assert np.array_equal(d["x"], g["x"][perm][expect])
+    # sorting again is the identity
+    assert np.array_equal(M.sort(), np.arange(len(order)))
+    M.close()
+
+
+def test_to_nodes_matches_reference(gpu):
+    g = golden("g5_markers.npz")
+    M = _markers(g)
+    M.sort()
+    M.set_tbc(g["bondm"], g["bondv"], g["bondn"])
+    rocks = [list(r) for r in g["rocks"]]
+    fld = M.to_nodes(rocks, g["pr"], g["init"], float(g["nubeg"]), float(g["nuend"]), 0.0, 1,
+                     float(g["timestep"]), float(g["vdeep"]), float(g["nudeep"]))
+    ref = g["out"]
+    exact = 0
+    for q in range(11):
+        a, b = fld[q], ref[q]
+        rel = np.abs(a.astype(np.float64) - b) / np.maximum(np.abs(b), 1e-30)
+        exact += int(np.sum(a == b))
+        assert rel.max() <= 2e-7, (q, rel.max())
+    print("bit-exact node values:", exact, "of", ref.size, flush=True)
+    d = M.download()
+    eta_ref = g["eta_out"][d["id"]]
+    assert np.allclose(d["eta"], eta_ref, rtol=2e-7, atol=0)
+    M.close()
+
+
+def test_rk4_matches_oracle(gpu):
+    g = golden("g5_markers.npz")
+    rng = np.random.default_rng(11)
+    N = len(g["pr"])
+    vx, vy, vz = (rng.standard_normal(N).astype(np.float32) * 1e-9 for _ in range(3))
+    M = _markers(g)
+    dt = 3e11
+    M.advect_rk4(dt, vx, vy, vz)
+    d = M.download()
+    ox, oy, oz, otk, oeta = oracle.rk4(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"], vx, vy, vz,
+                                       g["gx"], g["gy"], g["gz"], dt)
+    for a, b in ((d["x"], ox), (d["y"], oy), (d["z"], oz)):
+        ulp = np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+        assert ulp.max() <= 1, ulp.max()
+    assert np.array_equal(d["tk"] == 0, otk == 0)
+    M.close()
+
+
+def test_large_marker_set_deterministic(gpu):
+    """129^3 grid, 8 markers per cell: two runs give identical node fields;
+    constant properties map to constants where weights are complete."""
+    n = 129
+    cfg = I.make_config(1, n, n, n, seed=3)
+    gx, gy, gz = cfg["gx"], cfg["gy"], cfg["gz"]
+    rng = np.random.default_rng(1)
+    cells = np.stack(np.meshgrid(np.arange(n - 1), np.arange(n - 1), np.arange(n - 1), indexing="ij"), -1).reshape(-1, 3)
+    reps = 8
+    c = np.repeat(cells, reps, 0)
+    x = (gx[c[:, 0]] + rng.uniform(0.01, 0.99, len(c)) * (gx[1] - gx[0])).astype(np.float32)
+    y = (gy[c[:, 1]] + rng.uniform(0.01, 0.99, len(c)) * (gy[1] - gy[0])).astype(np.float32)
+    z = (gz[c[:, 2]] + rng.uniform(0.01, 0.99, len(c)) * (gz[1] - gz[0])).astype(np.float32)
+    m = len(x)
+    print("case: large marker set", m, flush=True)
+    M = I.Markers(gx, gy, gz, m)
+    M.upload(x, y, z, np.full(m, 1000, np.float32), np.full(m, 1e21, np.float32), np.full(m, 2, np.int8))
+    M.sort()
+    rocks = [[1e21, 0, 0, 1e18, 1e25, 3300, 0, 0, 1000, 3, 0, 0, 0]] * 3
+    init = np.zeros((11, n ** 3), np.float32)
+    pr = np.zeros(n ** 3)
+    f1 = M.to_nodes(rocks, pr, init, 1e18, 1e25, densimod=0, timestep=1.0)
+    f2 = M.to_nodes(rocks, pr, init, 1e18, 1e25, densimod=0, timestep=1.0)
+    assert np.array_equal(f1, f2)
+    assert np.allclose(f1[0], 1e21, rtol=1e-6)
+    assert np.allclose(f1[1], 3300, rtol=1e-6)
+    M.close()

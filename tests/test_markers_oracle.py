@@ -1,0 +1,63 @@
+"""CPU tests: marker oracle functions against the reference's own outputs
+(g5 fixture written from the compiled reference) and closed forms."""
+import numpy as np
+
+import oracle
+from conftest import golden
+
+
+def _rocks(g):
+    return [dict(zip(oracle.ROCK_FIELDS, r)) for r in g["rocks"]]
+
+
+def test_cell_search_bitexact():
+    g = golden("g5_markers.npz")
+    p = g["pts"]
+    c = oracle.mk_cells(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), g["gx"], g["gy"], g["gz"])
+    assert np.array_equal(c, g["cells"])
+
+
+def test_cell_search_closed_form():
+    """largest i with g[i] <= clamp(x), clamped to n-2 (searchsorted form)"""
+    rng = np.random.default_rng(3)
+    g = np.cumsum(np.r_[0, rng.uniform(0.5, 1.5, 40)])
+    x = np.r_[rng.uniform(-2, g[-1] + 2, 5000), g].astype(np.float32)
+    c = oracle.mk_cells(x, x, x, g, g, g)[:, 0]
+    xc = np.clip(x.astype(np.float64), 0, g[-1])
+    ref = np.clip(np.searchsorted(g, xc, side="right") - 1, 0, len(g) - 2)
+    assert np.array_equal(c, ref)
+
+
+def test_ronurecalc_matches_reference():
+    g = golden("g5_markers.npz")
+    fld, eta = oracle.ronurecalc(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"], g["gx"], g["gy"], g["gz"],
+                                 g["pr"], _rocks(g), g["init"], float(g["nubeg"]), float(g["nuend"]), 0.0, 1,
+                                 float(g["timestep"]), float(g["vdeep"]), float(g["nudeep"]))
+    fld[6] = oracle.tkrecalc(fld[6], g["bondm"], g["bondv"], g["bondn"])
+    assert np.array_equal(eta, g["eta_out"])
+    for q in range(11):
+        assert np.array_equal(fld[q], g["out"][q]), q
+
+
+def test_vxyz_matches_reference():
+    g = golden("g5_markers.npz")
+    for p, vref in zip(g["pts"], g["vout"]):
+        v = oracle.vxyz(g["vx"], g["vy"], g["vz"], g["gx"], g["gy"], g["gz"], float(p[0]), float(p[1]), float(p[2]))
+        assert np.array_equal(v, vref)
+
+
+def test_rk4_uniform_flow_translates():
+    """constant velocity: every RK4 stage is the same, x += dt*v exactly in
+    double before the float store"""
+    gx = np.linspace(0, 1e5, 12)
+    N = 12 ** 3
+    vx = np.full(N, 1e-9, np.float32)
+    vy = np.full(N, -2e-9, np.float32)
+    vz = np.full(N, 0.5e-9, np.float32)
+    rng = np.random.default_rng(5)
+    x, y, z = (rng.uniform(2e4, 8e4, 100).astype(np.float32) for _ in range(3))
+    dt = 1e12
+    out = oracle.rk4(x, y, z, np.ones(100, np.float32), np.ones(100, np.float32), np.full(100, 3, np.int8),
+                     vx, vy, vz, gx, gx, gx, dt)
+    assert np.array_equal(out[0], (x.astype(np.float64) + dt * np.float64(np.float32(1e-9))).astype(np.float32))
+    assert np.array_equal(out[1], (y.astype(np.float64) + dt * np.float64(np.float32(-2e-9))).astype(np.float32))
