@@ -186,3 +186,71 @@ def tkrecalc(tk, bondm, bondv, bondn):
     tk = np.array(tk, np.float32, copy=True)
     lib().or_tkrecalc(C.c_long(len(tk)), _p(bondm), _p(bondv), _p(bondn), _p(tk))
     return tk
+
+
+# ---- heat -------------------------------------------------------------------
+
+class HeatSys(C.Structure):
+    _fields_ = [("nx", C.c_long), ("ny", C.c_long), ("nz", C.c_long),
+                ("gx", C.c_void_p), ("gy", C.c_void_p), ("gz", C.c_void_p),
+                ("ro", C.c_void_p), ("cp", C.c_void_p), ("kt", C.c_void_p), ("ht", C.c_void_p),
+                ("tk0", C.c_void_p), ("dt", C.c_double),
+                ("bondm_t", C.c_void_p), ("bondv", C.c_void_p), ("bondn", C.c_void_p),
+                ("timesum", C.c_double), ("timebeg", C.c_double), ("timeend", C.c_double),
+                ("tempend", C.c_double)]
+
+
+def heat_sys(gx, gy, gz, ro, cp, kt, ht, tk0, dt, bondm, bondv, bondn, timesum=0.0, timebeg=1e30,
+             timeend=1e30, tempend=0.0):
+    """Level-0 heat system descriptor; bondm may be the full reference table
+    (its T block at 8*nodenum is taken) or the T block itself."""
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
+    N = len(gx) * len(gy) * len(gz)
+    bondm = np.asarray(bondm)
+    bt = np.ascontiguousarray(bondm[8 * N:9 * N] if len(bondm) > N else bondm, np.int32)
+    keep = [gx, gy, gz] + [np.ascontiguousarray(a, np.float32) for a in (ro, cp, kt, ht, tk0)] + [
+        bt, np.ascontiguousarray(bondv, np.float32), np.ascontiguousarray(bondn, np.int32)]
+    H = HeatSys(len(gx), len(gy), len(gz), *[_p(a) for a in keep[:3]], *[_p(a) for a in keep[3:8]], dt,
+                *[_p(a) for a in keep[8:]], timesum, timebeg, timeend, tempend)
+    H._keep = keep
+    H.N = N
+    return H
+
+
+def heat_matrix(H):
+    """(scipy CSR matrix, rhs) of the level-0 heat system."""
+    import scipy.sparse as sp
+    N = H.N
+    cnt = np.zeros(N, np.int32)
+    cols = np.zeros(7 * N, np.int32)
+    vals = np.zeros(7 * N)
+    rhs = np.zeros(N)
+    lib().or_heat_rows(C.byref(H), _p(cnt), _p(cols), _p(vals), _p(rhs))
+    mask = np.arange(7)[None, :] < cnt[:, None]
+    rows = np.repeat(np.arange(N), 7).reshape(N, 7)
+    A = sp.csr_matrix((vals.reshape(N, 7)[mask], (rows[mask], cols.reshape(N, 7)[mask])), shape=(N, N))
+    return A, rhs
+
+
+def heat_maxrate(H):
+    lib().or_heat_maxrate.restype = C.c_double
+    return lib().or_heat_maxrate(C.byref(H))
+
+
+def heat_norms(H, x):
+    out = np.zeros(2)
+    x = np.ascontiguousarray(x, np.float64)
+    lib().or_heat_norms(C.byref(H), _p(x), _p(out))
+    return out
+
+
+def mk_tk_update(x, y, z, mtype, markk, gx, gy, gz, tk, tk2, kt, ro, cp, heatdif, dt, zdeep=1e30, tdeep=0.0):
+    markk = np.array(markk, np.float32, copy=True)
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (x, y, z)]
+    nodes = [np.ascontiguousarray(a, np.float32) for a in (tk, tk2, kt, ro, cp)]
+    t = np.ascontiguousarray(mtype, np.int8)
+    lib().or_mk_tk_update(C.c_long(len(x)), *[_p(a) for a in arrs], _p(t), _p(markk), _p(gx), C.c_long(len(gx)),
+                          _p(gy), C.c_long(len(gy)), _p(gz), C.c_long(len(gz)), *[_p(a) for a in nodes],
+                          C.c_double(heatdif), C.c_double(dt), C.c_double(zdeep), C.c_double(tdeep))
+    return markk

@@ -825,3 +825,252 @@ void or_tkrecalc(long nodenum, const int *bondm, const float *bondv, const int *
 	}
 	free(tk1);
 }
+
+/* ---- heat (test infrastructure) ----------------------------------------- */
+
+/* Row of level-0 node (m1,m2,m3) of the implicit heat system, in the order
+ * the reference stores it: heaterr(ynerr=1) heat3mg.c:160-225 for interior
+ * nodes (diagonal first, then x-, x+, y-, y+, z-, z+), tbonderr(ynerr=0)
+ * heat3mg.c:818-841 for boundary nodes (1 at the node, -koef at the
+ * partner), with the bottom-line plume override of titerate
+ * heat3mg.c:444-455.  Returns the entry count. */
+static int heat_row(const or_heat_t *H, long m1, long m2, long m3, int *col, double *val, double *rhs)
+{
+	long nx = H->nx, ny = H->ny, nxy = nx * ny;
+	long v3 = m3 * nxy + m1 * ny + m2;
+	int rec = H->bondm_t[v3];
+	if (rec > 0) {
+		double c = H->bondv[2 * rec];
+		int n = 1;
+		col[0] = (int)v3;
+		val[0] = 1.0;
+		if (H->bondn[rec]) {
+			col[1] = H->bondn[rec] - 1;
+			val[1] = -(double)H->bondv[2 * rec + 1];
+			n = 2;
+		}
+		if (H->timesum > H->timebeg && m2 == ny - 1) {
+			if (H->timesum >= H->timeend) c = H->tempend;
+			else c = c + (H->tempend - c) / (H->timeend - H->timebeg) * (H->timesum - H->timebeg);
+		}
+		*rhs = c;
+		return n;
+	}
+	const double *gx = H->gx, *gy = H->gy, *gz = H->gz;
+	const float *kt = H->kt;
+	double xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]);
+	double ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+	double zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+	long v[7] = {v3 - nxy, v3 - ny, v3 - 1, v3, v3 + 1, v3 + ny, v3 + nxy};
+	double rocp = (float)(H->ro[v3] * H->cp[v3]);
+	double iv;
+	*rhs = (double)H->ht[v3] + H->tk0[v3] / H->dt * rocp;
+	val[0] = rocp / H->dt;
+	col[0] = (int)v3;
+	iv = (float)(kt[v[1]] + kt[v[3]]) / 2.0 * xkf / (gx[m1] - gx[m1 - 1]);
+	val[1] = -iv; col[1] = (int)v[1]; val[0] += iv;
+	iv = (float)(kt[v[3]] + kt[v[5]]) / 2.0 * xkf / (gx[m1 + 1] - gx[m1]);
+	val[0] += iv; val[2] = -iv; col[2] = (int)v[5];
+	iv = (float)(kt[v[2]] + kt[v[3]]) / 2.0 * ykf / (gy[m2] - gy[m2 - 1]);
+	val[3] = -iv; col[3] = (int)v[2]; val[0] += iv;
+	iv = (float)(kt[v[3]] + kt[v[4]]) / 2.0 * ykf / (gy[m2 + 1] - gy[m2]);
+	val[0] += iv; val[4] = -iv; col[4] = (int)v[4];
+	iv = (float)(kt[v[0]] + kt[v[3]]) / 2.0 * zkf / (gz[m3] - gz[m3 - 1]);
+	val[5] = -iv; col[5] = (int)v[0]; val[0] += iv;
+	iv = (float)(kt[v[3]] + kt[v[6]]) / 2.0 * zkf / (gz[m3 + 1] - gz[m3]);
+	val[0] += iv; val[6] = -iv; col[6] = (int)v[6];
+	return 7;
+}
+
+void or_heat_rows(const or_heat_t *H, int *cnt, int *cols, double *vals, double *rhs)
+{
+	long m1, m2, m3;
+	for (m3 = 0; m3 < H->nz; m3++)
+		for (m1 = 0; m1 < H->nx; m1++)
+			for (m2 = 0; m2 < H->ny; m2++) {
+				long v = m3 * H->nx * H->ny + m1 * H->ny + m2;
+				cnt[v] = heat_row(H, m1, m2, m3, cols + 7 * v, vals + 7 * v, rhs + v);
+			}
+}
+
+/* Step limit of titerate (heat3mg.c:327-368): largest |H/(ro cp)| over the
+ * non-boundary nodes (heaterr ynerr=0 without adiabatic/shear terms). */
+double or_heat_maxrate(const or_heat_t *H)
+{
+	long v, N = H->nx * H->ny * H->nz;
+	double mx = 0;
+	for (v = 0; v < N; v++)
+		if (H->bondm_t[v] == 0) {
+			double rocp = (float)(H->ro[v] * H->cp[v]);
+			double r = fabs((double)H->ht[v] / rocp);
+			if (r > mx) mx = r;
+		}
+	return mx;
+}
+
+/* Convergence norms of titerate (heat3mg.c:512-560): heatsum = RMS over
+ * non-boundary nodes of heaterr(ynerr=2)*dt with tk stored as float,
+ * bondsum = RMS of the boundary-row errors (tbonderr ynerr=1 on the
+ * double iterate).  out[0] = heatsum, out[1] = bondsum. */
+void or_heat_norms(const or_heat_t *H, const double *x, double *out)
+{
+	long nx = H->nx, ny = H->ny, nz = H->nz, nxy = nx * ny, N = nxy * nz;
+	long m1, m2, m3;
+	double hs = 0, hn = 0, bs = 0, bn = 0;
+	float *tk = malloc(sizeof(float) * (size_t)N);
+	for (m1 = 0; m1 < N; m1++) tk[m1] = (float)x[m1];
+	const double *gx = H->gx, *gy = H->gy, *gz = H->gz;
+	const float *kt = H->kt;
+	for (m3 = 0; m3 < nz; m3++)
+		for (m1 = 0; m1 < nx; m1++)
+			for (m2 = 0; m2 < ny; m2++) {
+				long v3 = m3 * nxy + m1 * ny + m2;
+				int rec = H->bondm_t[v3];
+				if (rec > 0) {
+					double e = x[v3] - H->bondv[2 * rec];
+					if (H->bondn[rec]) e -= H->bondv[2 * rec + 1] * x[H->bondn[rec] - 1];
+					bs += e * e;
+					bn += 1;
+					continue;
+				}
+				long v[7] = {v3 - nxy, v3 - ny, v3 - 1, v3, v3 + 1, v3 + ny, v3 + nxy};
+				double xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]);
+				double ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+				double zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+				double rocp = (float)(H->ro[v3] * H->cp[v3]);
+				double e = (double)(tk[v3] - H->tk0[v3]) / H->dt, iv;
+				iv = (float)(kt[v[1]] + kt[v[3]]) / 2.0 * xkf / (gx[m1] - gx[m1 - 1]) / rocp;
+				e += (double)(tk[v3] - tk[v[1]]) * iv;
+				iv = (float)(kt[v[3]] + kt[v[5]]) / 2.0 * xkf / (gx[m1 + 1] - gx[m1]) / rocp;
+				e -= (double)(tk[v[5]] - tk[v3]) * iv;
+				iv = (float)(kt[v[2]] + kt[v[3]]) / 2.0 * ykf / (gy[m2] - gy[m2 - 1]) / rocp;
+				e += (double)(tk[v3] - tk[v[2]]) * iv;
+				iv = (float)(kt[v[3]] + kt[v[4]]) / 2.0 * ykf / (gy[m2 + 1] - gy[m2]) / rocp;
+				e -= (double)(tk[v[4]] - tk[v3]) * iv;
+				iv = (float)(kt[v[0]] + kt[v[3]]) / 2.0 * zkf / (gz[m3] - gz[m3 - 1]) / rocp;
+				e += (double)(tk[v3] - tk[v[0]]) * iv;
+				iv = (float)(kt[v[3]] + kt[v[6]]) / 2.0 * zkf / (gz[m3 + 1] - gz[m3]) / rocp;
+				e -= (double)(tk[v[6]] - tk[v3]) * iv;
+				e -= (double)H->ht[v3] / rocp; /* tk1 = H (no adiabatic/shear terms) */
+				e *= H->dt;
+				hs += e * e;
+				hn += 1;
+			}
+	free(tk);
+	out[0] = sqrt(hs / hn);
+	out[1] = bn > 0 ? sqrt(bs / bn) : 0;
+}
+
+/* tkcalc1 (mark3mg.c:2844-2910): trilinear weights of the 8 nodes of the
+ * cell holding (x,y,z); returns the cell and spacings. */
+static void tk_weights(const double *gx, long nx, const double *gy, long ny, const double *gz, long nz, double x,
+		       double y, double z, long *cell, double *d, long *wn, double *wi)
+{
+	long m10 = serch(gx, nx, x), m20 = serch(gy, ny, y), m30 = serch(gz, nz, z);
+	long a, b, c;
+	int q = 0;
+	cell[0] = m10; cell[1] = m20; cell[2] = m30;
+	d[0] = gx[m10 + 1] - gx[m10];
+	d[1] = gy[m20 + 1] - gy[m20];
+	d[2] = gz[m30 + 1] - gz[m30];
+	for (c = m30; c <= m30 + 1; c++) {
+		double ddz = 1.0 - fabs((gz[c] - z) / d[2]);
+		for (a = m10; a <= m10 + 1; a++) {
+			double ddx = 1.0 - fabs((gx[a] - x) / d[0]);
+			for (b = m20; b <= m20 + 1; b++) {
+				double ddy = 1.0 - fabs((gy[b] - y) / d[1]);
+				wi[q] = ddx * ddy * ddz;
+				wn[q] = c * nx * ny + a * ny + b;
+				q++;
+			}
+		}
+	}
+}
+
+/* Marker temperature update after the heat solve (titerate,
+ * heat3mg.c:592-782): markers of type 50..99 inside the grid take the
+ * interpolated new temperature; for markers of type < 50 strictly inside:
+ * new markers (markk <= 0) take the interpolated start temperature (deep
+ * floor tdeep below zdeep); with heatdif > 0 the subgrid diffusion relaxes
+ * the marker towards the grid, its change is averaged back onto the nodes
+ * in marker order and removed again (antidiffusion); finally the marker
+ * takes the interpolated temperature change.  The marker loops run over
+ * n markers (the reference visits one index past marknum, which the caller
+ * keeps outside the grid). */
+void or_mk_tk_update(long n, const float *x, const float *y, const float *z, const signed char *type, float *markk,
+		     const double *gx, long nx, const double *gy, long ny, const double *gz, long nz, const float *tk,
+		     const float *tk2, const float *kt, const float *ro, const float *cp, double heatdif, double dt,
+		     double zdeep, double tdeep)
+{
+	long N = nx * ny * nz, i;
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	double *s0 = calloc((size_t)N, sizeof(double)), *s1 = calloc((size_t)N, sizeof(double));
+	/* fluid markers (50 <= type < 100) take the new grid temperature */
+	for (i = 0; i < n; i++) {
+		if (!(type[i] >= 50 && type[i] < 100 && x[i] > 0 && y[i] > 0 && z[i] > 0 && (double)x[i] < xs &&
+		      (double)y[i] < ys && (double)z[i] < zs))
+			continue;
+		long cell[3], wn[8];
+		double d[3], wi[8], e2 = 0;
+		int q;
+		tk_weights(gx, nx, gy, ny, gz, nz, x[i], y[i], z[i], cell, d, wn, wi);
+		for (q = 0; q < 8; q++) e2 += wi[q] * tk[wn[q]];
+		markk[i] = (float)e2;
+	}
+	for (i = 0; i < n; i++) {
+		if (!(type[i] < 50 && x[i] > 0 && y[i] > 0 && z[i] > 0 && (double)x[i] < xs && (double)y[i] < ys &&
+		      (double)z[i] < zs))
+			continue;
+		long cell[3], wn[8];
+		double d[3], wi[8], e2 = 0, e3 = 0;
+		int q;
+		tk_weights(gx, nx, gy, ny, gz, nz, x[i], y[i], z[i], cell, d, wn, wi);
+		for (q = 0; q < 8; q++) {
+			e2 += wi[q] * tk[wn[q]];
+			e3 += wi[q] * tk2[wn[q]];
+		}
+		if (markk[i] <= 0) {
+			markk[i] = (float)e3;
+			if (y[i] > zdeep && markk[i] < tdeep) markk[i] = (float)tdeep;
+		}
+		if (heatdif) {
+			double mkt = 0, rocp = 0, mtk = tk[wn[0]], dtk, dtk1;
+			for (q = 0; q < 8; q++) {
+				mkt += kt[wn[q]] * wi[q];
+				rocp += ro[wn[q]] * cp[wn[q]] * wi[q];
+				if (tk[wn[q]] > mtk) mtk = tk[wn[q]];
+			}
+			if ((cell[0] <= 0 || cell[0] >= nx - 2 || cell[1] <= 0 || cell[1] >= ny - 2 || cell[2] <= 0 ||
+			     cell[2] >= nz - 2) &&
+			    (markk[i] + e2 - e3) > mtk)
+				markk[i] = (float)(mtk - e2 + e3);
+			dtk = e3 - (double)markk[i];
+			dtk1 = -heatdif * mkt / rocp * 2.0 * (1.0 / d[0] / d[0] + 1.0 / d[1] / d[1] + 1.0 / d[2] / d[2]) * dt;
+			if (dtk1 < -150.0) dtk1 = -150.0;
+			dtk1 = dtk * (1.0 - exp(dtk1));
+			markk[i] += (float)dtk1;
+			for (q = 0; q < 8; q++) {
+				s1[wn[q]] += dtk1 * wi[q];
+				s0[wn[q]] += wi[q];
+			}
+		}
+		markk[i] += (float)(e2 - e3);
+	}
+	if (heatdif) {
+		for (i = 0; i < N; i++)
+			if (s0[i]) s1[i] /= s0[i];
+		for (i = 0; i < n; i++) {
+			if (!(type[i] < 50 && x[i] > 0 && y[i] > 0 && z[i] > 0 && (double)x[i] < xs && (double)y[i] < ys &&
+			      (double)z[i] < zs))
+				continue;
+			long cell[3], wn[8];
+			double d[3], wi[8], a = 0;
+			int q;
+			tk_weights(gx, nx, gy, ny, gz, nz, x[i], y[i], z[i], cell, d, wn, wi);
+			for (q = 0; q < 8; q++) a -= s1[wn[q]] * wi[q];
+			markk[i] += (float)a;
+		}
+	}
+	free(s0);
+	free(s1);
+}

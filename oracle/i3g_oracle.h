@@ -109,4 +109,31 @@ void or_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, const s
  * a copy of tk (reference bondm layout, T records at 8*nodenum). */
 void or_tkrecalc(long nodenum, const int *bondm, const float *bondv, const int *bondn, float *tk);
 
+/* ---- heat (test infrastructure) -------------------------------------- */
+
+typedef struct {
+	long nx, ny, nz;
+	const double *gx, *gy, *gz;
+	const float *ro, *cp, *kt, *ht; /* node fields */
+	const float *tk0;               /* temperature at the start of the sub-step */
+	double dt;                      /* sub-step */
+	const int *bondm_t;             /* level-0 T block of bondm (nodenum entries) */
+	const float *bondv;             /* bondv1[r][0..1] at [2r..2r+1] */
+	const int *bondn;               /* bondn1[r] = partner node + 1 or 0 */
+	double timesum, timebeg, timeend, tempend; /* bottom-line plume override */
+} or_heat_t;
+
+/* Rows of the level-0 heat system (7 slots per node: cnt entries of
+ * cols/vals at 7*node) and right-hand side, heat3mg.c:160-225, 818-841. */
+void or_heat_rows(const or_heat_t *H, int *cnt, int *cols, double *vals, double *rhs);
+/* max |H/(ro cp)| over non-boundary nodes (heat3mg.c:327-368) */
+double or_heat_maxrate(const or_heat_t *H);
+/* out[0] heatsum, out[1] bondsum of titerate (heat3mg.c:512-560) */
+void or_heat_norms(const or_heat_t *H, const double *x, double *out);
+/* marker temperature update after the heat solve (heat3mg.c:592-782) */
+void or_mk_tk_update(long n, const float *x, const float *y, const float *z, const signed char *type, float *markk,
+		     const double *gx, long nx, const double *gy, long ny, const double *gz, long nz, const float *tk,
+		     const float *tk2, const float *kt, const float *ro, const float *cp, double heatdif, double dt,
+		     double zdeep, double tdeep);
+
 #endif

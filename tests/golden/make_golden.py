@@ -235,10 +235,71 @@ def g5(R):
     print("g5_markers.npz written")
 
 
+def g6(R, n=17):
+    """Implicit heat step with sub-steps (titerate, heat3mg.c:287-588) and
+    the marker temperature update with subgrid diffusion (heat3mg.c:592-782)
+    through the reference, configuration-0 geometry."""
+    import ctypes as C
+    import oracle
+    rng = np.random.default_rng(606)
+    cfg = I.make_config(0, n, n, n, seed=1)
+    gx, gy, gz = cfg["gx"], cfg["gy"], cfg["gz"]
+    N = n ** 3
+    R.setup_grid(gx, gy, gz, 0)
+    R.set_bc(cfg["bondm"], cfg["bondv"], cfg["bondn"])
+    ro = cfg["ro"].astype(np.float32)
+    cp = rng.uniform(900, 1300, N).astype(np.float32)
+    kt = rng.uniform(1.5, 4.5, N).astype(np.float32)
+    ht = rng.uniform(0, 2e-6, N).astype(np.float32)
+    tk = (cfg["tk"] + rng.uniform(-50, 50, N)).astype(np.float32)
+    dt = 1e13
+    H = oracle.heat_sys(gx, gy, gz, ro, cp, kt, ht, tk, dt, cfg["bondm"], cfg["bondv"], cfg["bondn"])
+    maxtkstep = oracle.heat_maxrate(H) * dt / 2.5  # three sub-steps
+    # markers: 2x2x2 per cell with jitter, a few outside the grid or of type >= 50
+    cells = np.stack(np.meshgrid(np.arange(n - 1), np.arange(n - 1), np.arange(n - 1), indexing="ij"), -1).reshape(-1, 3)
+    c = np.repeat(cells, 8, 0)
+    def jit(g, k):
+        return g[k] + rng.uniform(0.02, 0.98, len(k)) * (g[k + 1] - g[k])
+    x = jit(gx, c[:, 0]).astype(np.float32)
+    y = jit(gy, c[:, 1]).astype(np.float32)
+    z = jit(gz, c[:, 2]).astype(np.float32)
+    m = len(x)
+    mtype = rng.integers(2, 7, m).astype(np.int8)
+    mtype[::101] = 60
+    x[::211] = -1.0
+    markk = rng.uniform(300, 1600, m).astype(np.float32)
+    markk[::53] = 0.0
+    zdeep, tdeep = gy[-1] * 0.7, 1500.0
+    for k, v in dict(HEATMIN=1e-2, t0koef=1.0, t1koef=1.0, heatdif=1.0, timestep=dt, maxtkstep=maxtkstep,
+                     timesum=0.0, timebeg=1e30, timeend=2e30, tempend=0.0, zdeep=zdeep, tdeep=tdeep).items():
+        R.set(k, v)
+    for k, v in dict(multinumt=0, multicyct=1, frictyn=0, adiabyn=0).items():
+        R.set(k, v, C.c_int)
+    R.i32("multittt", 100)[0] = 2000  # the double iterate converges before the first check
+    R.set("marknum", m, C.c_long)
+    R.arr("markx", C.c_float, R.MAXMRK)[:m + 1] = np.r_[x, -1.0]
+    R.arr("marky", C.c_float, R.MAXMRK)[:m + 1] = np.r_[y, -1.0]
+    R.arr("markz", C.c_float, R.MAXMRK)[:m + 1] = np.r_[z, -1.0]
+    R.arr("markt", C.c_byte, R.MAXMRK)[:m + 1] = np.r_[mtype, 2]
+    R.arr("markk", C.c_float, R.MAXMRK)[:m + 1] = np.r_[markk, 0.0]
+    for name, a in dict(tk=tk, ro=ro, cp=cp, kt=kt, ht=ht).items():
+        R.f32(name)[:N] = a
+    R.f32("et")[:N] = 0
+    R.f64("pr")[:N] = 0
+    R.lib.titerate(C.c_int(0))
+    tk_out = R.f32("tk")[:N].copy()
+    markk_out = R.arr("markk", C.c_float, R.MAXMRK)[:m].copy()
+    np.savez_compressed(os.path.join(HERE, "g6_heat.npz"), n=n, gx=gx, gy=gy, gz=gz, bondm=cfg["bondm"],
+                        bondv=cfg["bondv"], bondn=cfg["bondn"], ro=ro, cp=cp, kt=kt, ht=ht, tk=tk, dt=dt,
+                        maxtkstep=maxtkstep, heatmin=1e-2, sweeps=2000, x=x, y=y, z=z, mtype=mtype, markk=markk, zdeep=zdeep,
+                        tdeep=tdeep, heatdif=1.0, tk_out=tk_out, markk_out=markk_out)
+    print("g6_heat.npz written")
+
+
 if __name__ == "__main__":
     R = Ref()
     R.lib.refh_set_threads(1)
-    which = sys.argv[1:] or ["g1", "g3", "g5"]
+    which = sys.argv[1:] or ["g1", "g3", "g5", "g6"]
     if "g1" in which:
         g1(R)
     if "g3" in which:
@@ -246,3 +307,5 @@ if __name__ == "__main__":
         g3(R, 25)
     if "g5" in which:
         g5(R)
+    if "g6" in which:
+        g6(R)
