@@ -150,7 +150,8 @@ def main():
     def run(prec, timing=False, cap=2000):
         S.set_solution(z, z, z, z)
         S.set_timing(timing)
-        st = S.solve(check_conv=cap >= 2000, precision=prec, rtol=1e-8, krylov=a.krylov, max_outer=20,
+        full = cap >= 2000
+        st = S.solve(check_conv=full, precision=prec, rtol=1e-8, krylov=a.krylov, max_outer=20 if full else 1,
                      max_vcycles=cap, verbose=a.verbose)
         return st
 

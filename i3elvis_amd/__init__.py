@@ -323,6 +323,136 @@ class Markers:
         self._tbc = (bt, bv, bn)
         check(lib().i3g_markers_set_tbc(self.h, _p(bt), _p(bv), _p(bn), C.c_long(len(bn))), "markers_set_tbc")
 
+    def update_tk(self, tk, tk2, kt, ro, cp, heatdif, timestep, zdeep=1e30, tdeep=0.0):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (tk, tk2, kt, ro, cp)]
+        check(lib().i3g_markers_update_tk(self.h, *[_p(a) for a in arrs], C.c_double(heatdif),
+                                          C.c_double(timestep), C.c_double(zdeep), C.c_double(tdeep)),
+              "markers_update_tk")
+
     def advect_rk4(self, dt, vx, vy, vz):
         v = [np.ascontiguousarray(a, np.float32) for a in (vx, vy, vz)]
         check(lib().i3g_markers_advect_rk4(self.h, C.c_double(dt), *[_p(a) for a in v]), "markers_rk4")
+
+
+# ---- heat -------------------------------------------------------------------
+
+class HeatParams(C.Structure):
+    _fields_ = [("timestep", C.c_double), ("maxtkstep", C.c_double), ("heatmin", C.c_double),
+                ("t0koef", C.c_double), ("sweeps", C.c_int), ("max_sweeps", C.c_int),
+                ("timesum", C.c_double), ("timebeg", C.c_double), ("timeend", C.c_double),
+                ("tempend", C.c_double), ("verbose", C.c_int)]
+
+
+class HeatStats(C.Structure):
+    _fields_ = [("converged", C.c_int), ("substeps", C.c_int), ("sweeps", C.c_int),
+                ("heatsum", C.c_double), ("bondsum", C.c_double), ("seconds", C.c_double)]
+
+
+class Heat:
+    """Device heat context for one level-0 grid (i3g_heat_create .. destroy)."""
+
+    def __init__(self, gx, gy, gz, bondm, bondv, bondn):
+        self._keep = [np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz)] + [
+            np.ascontiguousarray(bondm, np.int32), np.ascontiguousarray(bondv, np.float32),
+            np.ascontiguousarray(bondn, np.int32)]
+        gx, gy, gz, bm, bv, bn = self._keep
+        self.N = len(gx) * len(gy) * len(gz)
+        s = Setup(len(gx), len(gy), len(gz), _p(gx), _p(gy), _p(gz), 0, _p(bm), _p(bv), _p(bn), len(bn))
+        self.h = C.c_void_p()
+        check(lib().i3g_heat_create(C.byref(s), C.byref(self.h)), "heat_create")
+
+    def close(self):
+        if self.h:
+            lib().i3g_heat_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, tk, ro, cp, kt, ht, check_conv=True, **kw):
+        """Returns (new tk float32, HeatStats); tk is not modified."""
+        p = HeatParams()
+        lib().i3g_heat_default_params(C.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        out = np.array(tk, np.float32, copy=True)
+        fl = [np.ascontiguousarray(a, np.float32) for a in (ro, cp, kt, ht)]
+        st = HeatStats()
+        rc = lib().i3g_heat_solve(self.h, _p(out), *[_p(a) for a in fl], C.byref(p), C.byref(st))
+        if rc != -4 or check_conv:
+            check(rc, "heat_solve")
+        return out, st
+
+
+# ---- learned warm start -----------------------------------------------------
+
+class WarmStartNet:
+    """Device encoder-decoder (i3g_warmstart_*) on an (nz, nx, ny) node grid."""
+
+    def __init__(self, nz, nx, ny, seed=0):
+        self.shape = (nz, nx, ny)
+        self.N = nz * nx * ny
+        self.h = C.c_void_p()
+        check(lib().i3g_warmstart_create(C.c_long(nz), C.c_long(nx), C.c_long(ny), C.c_ulonglong(seed),
+                                         C.byref(self.h)), "warmstart_create")
+        lib().i3g_warmstart_nparams.restype = C.c_long
+        self.nparams = lib().i3g_warmstart_nparams(self.h)
+
+    def close(self):
+        if self.h:
+            lib().i3g_warmstart_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.nparams, np.float32)
+        check(lib().i3g_warmstart_get_params(self.h, _p(p)), "warmstart_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        check(lib().i3g_warmstart_set_params(self.h, _p(p)), "warmstart_set_params")
+
+    def infer(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(self.N, 2)
+        y = np.zeros((self.N, 4), np.float32)
+        check(lib().i3g_warmstart_infer(self.h, _p(x), _p(y)), "warmstart_infer")
+        return y
+
+    def grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(self.N, 2)
+        t = np.ascontiguousarray(t, np.float32).reshape(self.N, 4)
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_double()
+        check(lib().i3g_warmstart_grad(self.h, _p(x), _p(t), C.byref(loss), _p(g)), "warmstart_grad")
+        return loss.value, g
+
+    def train_step(self, xs, ts, lr=1e-3):
+        xs = np.ascontiguousarray(xs, np.float32)
+        ts = np.ascontiguousarray(ts, np.float32)
+        b = xs.size // (2 * self.N)
+        loss = C.c_double()
+        check(lib().i3g_warmstart_train_step(self.h, C.c_int(b), _p(xs), _p(ts), C.c_double(lr), C.byref(loss)),
+              "warmstart_train_step")
+        return loss.value
+
+
+WARMSTART_LAYERS = [(2, 16, 1), (16, 32, 2), (32, 64, 2), (64, 64, 2), (64, 32, 1), (32, 16, 1), (16, 4, 1)]
+
+
+def unflatten_params(p):
+    """device parameter vector -> [W0, b0, W1, b1, ...] with W [27, cout, cin]"""
+    out, o = [], 0
+    for ci, co, _ in WARMSTART_LAYERS:
+        out.append(np.asarray(p[o:o + 27 * co * ci]).reshape(27, co, ci)); o += 27 * co * ci
+        out.append(np.asarray(p[o:o + co])); o += co
+    return out

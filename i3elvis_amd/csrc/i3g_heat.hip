@@ -1,0 +1,449 @@
+/*
+ * i3g heat path on MI355X: the implicit temperature step of the reference
+ * (titerate, heat3mg.c:287-588) kept resident on the device.  Level-0 rows
+ * of heaterr/tbonderr are evaluated matrix-free in FP64 and relaxed by
+ * red-black Gauss-Seidel (the colour-ordered form of gausmatt's sweep,
+ * heat3mg.c:906-961) until the reference's heat norm reaches heatmin.
+ * C ABI in include/i3g.h.
+ */
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/i3g.h"
+
+/* products and sums rounded one at a time, as the reference's x86-64 build */
+#pragma clang fp contract(off)
+
+#define TCHK(x)                                                                \
+	do {                                                                   \
+		hipError_t e_ = (x);                                           \
+		if (e_ != hipSuccess) {                                        \
+			fprintf(stderr, "i3g heat: HIP error %s at %s:%d\n",  \
+				hipGetErrorString(e_), __FILE__, __LINE__);    \
+			return I3G_EDEVICE;                                    \
+		}                                                              \
+	} while (0)
+
+#define NB 256
+#define MAXDEPTH 8
+
+struct i3g_heat {
+	hipStream_t s = nullptr;
+	int nx = 0, ny = 0, nz = 0;
+	long N = 0;
+	double *gx = nullptr, *gy = nullptr, *gz = nullptr;
+	/* boundary rows per node: partner (-2 none, -1 constant only),
+	 * constant, coefficient; boundary node lists ordered by dependency
+	 * depth (a partner's row is applied before the rows that read it) */
+	int *bp = nullptr;
+	float *bc = nullptr, *bk = nullptr;
+	int *blist = nullptr;
+	std::vector<long> boff;   /* boff[d]..boff[d+1]: nodes of depth d */
+	/* fields */
+	float *kt = nullptr, *rocp = nullptr, *ht = nullptr, *tk0 = nullptr;
+	double *T = nullptr, *b = nullptr, *diag = nullptr;
+	double *part = nullptr, *res = nullptr;
+	std::vector<int> hb;      /* host copy of the T block of bondm */
+};
+
+template <typename T> static T *halloc(size_t n, int *err)
+{
+	void *p = nullptr;
+	if (n == 0) n = 1;
+	if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) { *err = I3G_ENOMEM; return nullptr; }
+	return (T *)p;
+}
+
+/* node index from the (plane, in-plane) launch; y fastest as the reference */
+#define NODE_IDX                                                               \
+	int t = blockIdx.x * blockDim.x + threadIdx.x;                         \
+	int m3 = blockIdx.y;                                                   \
+	int m1 = t / ny, m2 = t - m1 * ny;                                     \
+	if (m1 >= nx) return;                                                  \
+	long sx = ny, sz = (long)nx * ny;                                      \
+	long v3 = m3 * sz + m1 * sx + m2;
+
+/* Face conductances of an interior row (heaterr ynerr=1, heat3mg.c:182-220):
+ * k_f = (k_a + k_b)/2 with the float sum of the reference, times
+ * 2/(g[m+1]-g[m-1]) / (face spacing). */
+struct Cond {
+	double xm, xp, ym, yp, zm, zp;
+};
+
+__device__ __forceinline__ Cond conds(const float *__restrict__ kt, const double *gx, const double *gy,
+				      const double *gz, int m1, int m2, int m3, long v3, long sx, long sz)
+{
+	Cond c;
+	float k3 = kt[v3];
+	double xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]);
+	double ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+	double zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+	c.xm = (double)(kt[v3 - sx] + k3) / 2.0 * xkf / (gx[m1] - gx[m1 - 1]);
+	c.xp = (double)(k3 + kt[v3 + sx]) / 2.0 * xkf / (gx[m1 + 1] - gx[m1]);
+	c.ym = (double)(kt[v3 - 1] + k3) / 2.0 * ykf / (gy[m2] - gy[m2 - 1]);
+	c.yp = (double)(k3 + kt[v3 + 1]) / 2.0 * ykf / (gy[m2 + 1] - gy[m2]);
+	c.zm = (double)(kt[v3 - sz] + k3) / 2.0 * zkf / (gz[m3] - gz[m3 - 1]);
+	c.zp = (double)(k3 + kt[v3 + sz]) / 2.0 * zkf / (gz[m3 + 1] - gz[m3]);
+	return c;
+}
+
+/* Right-hand side and diagonal of every row of one sub-step; boundary rows
+ * take their constant, on the bottom line replaced by the plume ramp of
+ * titerate (heat3mg.c:444-455). */
+__global__ void k_heat_setup(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz,
+			     const float *__restrict__ kt, const float *__restrict__ rocp,
+			     const float *__restrict__ ht, const float *__restrict__ tk0, const int *__restrict__ bp,
+			     const float *__restrict__ bc, double dt, int plume, double timebeg, double timeend,
+			     double timesum, double tempend, double *b, double *diag, double *T)
+{
+	NODE_IDX
+	T[v3] = (double)tk0[v3];
+	if (bp[v3] != -2) {
+		double c = bc[v3];
+		if (plume && m2 == ny - 1)
+			c = plume == 2 ? tempend : c + (tempend - c) / (timeend - timebeg) * (timesum - timebeg);
+		b[v3] = c;
+		diag[v3] = 1.0;
+		return;
+	}
+	double rc = (double)rocp[v3];
+	Cond c = conds(kt, gx, gy, gz, m1, m2, m3, v3, sx, sz);
+	b[v3] = (double)ht[v3] + (double)tk0[v3] / dt * rc;
+	double d = rc / dt;
+	d += c.xm; d += c.xp; d += c.ym; d += c.yp; d += c.zm; d += c.zp;
+	diag[v3] = d;
+}
+
+/* One colour of the red-black relaxation of the interior rows:
+ * T -= (A T - b) / diag * t0koef (gausmatt, heat3mg.c:933-940). */
+__global__ void k_heat_sweep(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz,
+			     const float *__restrict__ kt, const int *__restrict__ bp, const double *__restrict__ b,
+			     const double *__restrict__ diag, double *T, int colour, double t0koef)
+{
+	NODE_IDX
+	if (((m1 + m2 + m3) & 1) != colour || bp[v3] != -2) return;
+	Cond c = conds(kt, gx, gy, gz, m1, m2, m3, v3, sx, sz);
+	double r = diag[v3] * T[v3] - b[v3];
+	r -= c.xm * T[v3 - sx]; r -= c.xp * T[v3 + sx];
+	r -= c.ym * T[v3 - 1];  r -= c.yp * T[v3 + 1];
+	r -= c.zm * T[v3 - sz]; r -= c.zp * T[v3 + sz];
+	T[v3] -= r / diag[v3] * t0koef;
+}
+
+/* Boundary rows of one dependency depth: T = c + k T[partner]. */
+__global__ void k_heat_bc(long n, const int *__restrict__ list, const int *__restrict__ bp,
+			  const float *__restrict__ bk, const double *__restrict__ b, double *T, double t0koef)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int v = list[i];
+	int p = bp[v];
+	double r = T[v] - b[v];
+	if (p >= 0) r -= (double)bk[v] * T[p];
+	T[v] -= r * t0koef;
+}
+
+/* Sums of the squared heat-row errors in temperature units (heaterr
+ * ynerr=2 times the step, heat3mg.c:512-537) and of the boundary-row errors
+ * (tbonderr ynerr=1), per block. */
+__global__ void k_heat_norm(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz,
+			    const float *__restrict__ kt, const float *__restrict__ rocp, const int *__restrict__ bp,
+			    const float *__restrict__ bk, const double *__restrict__ b,
+			    const double *__restrict__ diag, const double *__restrict__ T, double dt, double *part)
+{
+	__shared__ double sh[2][NB / 64];
+	double h = 0, q = 0;
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int m3 = blockIdx.y;
+	int m1 = t / ny, m2 = t - m1 * ny;
+	if (m1 < nx) {
+		long sx = ny, sz = (long)nx * ny;
+		long v3 = m3 * sz + m1 * sx + m2;
+		int p = bp[v3];
+		if (p != -2) {
+			double r = T[v3] - b[v3];
+			if (p >= 0) r -= (double)bk[v3] * T[p];
+			q = r * r;
+		} else {
+			Cond c = conds(kt, gx, gy, gz, m1, m2, m3, v3, sx, sz);
+			double r = diag[v3] * T[v3] - b[v3];
+			r -= c.xm * T[v3 - sx]; r -= c.xp * T[v3 + sx];
+			r -= c.ym * T[v3 - 1];  r -= c.yp * T[v3 + 1];
+			r -= c.zm * T[v3 - sz]; r -= c.zp * T[v3 + sz];
+			r = r / (double)rocp[v3] * dt;
+			h = r * r;
+		}
+	}
+	for (int o = 32; o > 0; o >>= 1) {
+		h += __shfl_down(h, o, 64);
+		q += __shfl_down(q, o, 64);
+	}
+	int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (lane == 0) { sh[0][w] = h; sh[1][w] = q; }
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double a = 0, c = 0;
+		for (int k = 0; k < NB / 64; k++) { a += sh[0][k]; c += sh[1][k]; }
+		long blk = (long)blockIdx.y * gridDim.x + blockIdx.x;
+		part[2 * blk] = a;
+		part[2 * blk + 1] = c;
+	}
+}
+
+__global__ void k_heat_sum(long nblk, const double *part, double *res)
+{
+	__shared__ double sh[2][NB];
+	double a = 0, c = 0;
+	for (long i = threadIdx.x; i < nblk; i += NB) {
+		a += part[2 * i];
+		c += part[2 * i + 1];
+	}
+	sh[0][threadIdx.x] = a;
+	sh[1][threadIdx.x] = c;
+	__syncthreads();
+	for (int o = NB / 2; o > 0; o >>= 1) {
+		if ((int)threadIdx.x < o) {
+			sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
+			sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) { res[0] = sh[0][0]; res[1] = sh[1][0]; }
+}
+
+/* end of a sub-step: the reference stores T as float (tk[] = sol0[]) and
+ * starts the next sub-step from that float field */
+__global__ void k_heat_store(long N, double *T, float *tk0)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= N) return;
+	float f = (float)T[i];
+	tk0[i] = f;
+	T[i] = f;
+}
+
+/* ------------------------------------------------------------------ */
+/* C ABI                                                              */
+/* ------------------------------------------------------------------ */
+
+extern "C" int i3g_heat_destroy(i3g_heat *h)
+{
+	if (!h) return I3G_OK;
+	if (h->s) hipStreamSynchronize(h->s);
+	void *ps[] = {h->gx, h->gy, h->gz, h->bp, h->bc, h->bk, h->blist, h->kt, h->rocp, h->ht, h->tk0,
+		      h->T, h->b, h->diag, h->part, h->res};
+	for (void *p : ps)
+		if (p) hipFree(p);
+	if (h->s) hipStreamDestroy(h->s);
+	delete h;
+	return I3G_OK;
+}
+
+extern "C" int i3g_heat_create(const i3g_setup *su, i3g_heat **out)
+{
+	if (!su || !out || !su->gx || !su->gy || !su->gz || !su->bondm || !su->bondv || !su->bondn)
+		return I3G_EINVAL;
+	if (su->nx < 3 || su->ny < 3 || su->nz < 3 || su->nrec < 1) return I3G_EINVAL;
+	long N = su->nx * su->ny * su->nz;
+	if (N > 400000000L) return I3G_ECAP;
+	*out = nullptr;
+	i3g_heat *h = new i3g_heat();
+	h->nx = (int)su->nx; h->ny = (int)su->ny; h->nz = (int)su->nz; h->N = N;
+	/* boundary rows of the T block (bondm + 8*nodenum) */
+	std::vector<int> bp(N, -2);
+	std::vector<float> bc(N, 0.f), bk(N, 0.f);
+	const int *bm = su->bondm + 8 * N;
+	for (long i = 0; i < N; i++) {
+		long r = bm[i];
+		if (r <= 0) continue;
+		if (r >= su->nrec) { delete h; return I3G_EBC; }
+		long p = (long)su->bondn[r] - 1;
+		if (p < -1 || p >= N) { delete h; return I3G_EBC; }
+		bp[i] = (int)p;
+		bc[i] = su->bondv[2 * r];
+		bk[i] = su->bondv[2 * r + 1];
+	}
+	/* a boundary node on the domain surface must exist for every surface
+	 * node: the interior stencil reads all six neighbours */
+	for (long m3 = 0; m3 < su->nz; m3++)
+		for (long m1 = 0; m1 < su->nx; m1++)
+			for (long m2 = 0; m2 < su->ny; m2++) {
+				bool surf = m1 == 0 || m2 == 0 || m3 == 0 || m1 == su->nx - 1 || m2 == su->ny - 1 ||
+					    m3 == su->nz - 1;
+				long v = m3 * su->nx * su->ny + m1 * su->ny + m2;
+				if (surf && bp[v] == -2) { delete h; return I3G_EBC; }
+			}
+	/* dependency depth of the boundary rows */
+	std::vector<int> depth(N, -1);
+	std::vector<std::vector<int>> lists(MAXDEPTH);
+	for (long i = 0; i < N; i++) {
+		if (bp[i] == -2) continue;
+		int d = 0;
+		long j = i;
+		while (bp[j] >= 0 && bp[bp[j]] != -2) {
+			j = bp[j];
+			if (++d >= MAXDEPTH) { delete h; return I3G_EBC; }
+		}
+		depth[i] = d;
+		lists[d].push_back((int)i);
+	}
+	std::vector<int> flat;
+	h->boff.push_back(0);
+	for (int d = 0; d < MAXDEPTH; d++) {
+		flat.insert(flat.end(), lists[d].begin(), lists[d].end());
+		h->boff.push_back((long)flat.size());
+	}
+	int err = 0;
+	if (hipStreamCreate(&h->s) != hipSuccess) { delete h; return I3G_EDEVICE; }
+	h->gx = halloc<double>(su->nx, &err); h->gy = halloc<double>(su->ny, &err); h->gz = halloc<double>(su->nz, &err);
+	h->bp = halloc<int>(N, &err); h->bc = halloc<float>(N, &err); h->bk = halloc<float>(N, &err);
+	h->blist = halloc<int>(flat.size(), &err);
+	h->kt = halloc<float>(N, &err); h->rocp = halloc<float>(N, &err); h->ht = halloc<float>(N, &err);
+	h->tk0 = halloc<float>(N, &err);
+	h->T = halloc<double>(N, &err); h->b = halloc<double>(N, &err); h->diag = halloc<double>(N, &err);
+	long nblk = ((long)su->nx * su->ny + NB - 1) / NB * su->nz;
+	h->part = halloc<double>(2 * nblk, &err);
+	h->res = halloc<double>(2, &err);
+	if (err) { i3g_heat_destroy(h); return err; }
+	if (hipMemcpy(h->gx, su->gx, su->nx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(h->gy, su->gy, su->ny * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(h->gz, su->gz, su->nz * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(h->bp, bp.data(), N * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(h->bc, bc.data(), N * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(h->bk, bk.data(), N * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+	    (flat.size() &&
+	     hipMemcpy(h->blist, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)) {
+		i3g_heat_destroy(h);
+		return I3G_EDEVICE;
+	}
+	h->hb.assign(bm, bm + N);
+	*out = h;
+	return I3G_OK;
+}
+
+extern "C" int i3g_heat_default_params(i3g_heat_params *p)
+{
+	if (!p) return I3G_EINVAL;
+	p->timestep = 0;
+	p->maxtkstep = 25.0;  /* mode.t3c maxtkstep (K) */
+	p->heatmin = 1e-3;    /* mode.t3c HEATMIN */
+	p->t0koef = 1.0;
+	p->sweeps = 30;       /* multittt[0][0] */
+	p->max_sweeps = 100000;
+	p->timesum = 0;
+	p->timebeg = 1e300;
+	p->timeend = 1e300;
+	p->tempend = 0;
+	p->verbose = 0;
+	return I3G_OK;
+}
+
+static int bc_apply(i3g_heat *h, double t0koef)
+{
+	for (size_t d = 0; d + 1 < h->boff.size(); d++) {
+		long n = h->boff[d + 1] - h->boff[d];
+		if (n <= 0) continue;
+		hipLaunchKernelGGL(k_heat_bc, dim3((unsigned)((n + NB - 1) / NB)), dim3(NB), 0, h->s, n,
+				   (const int *)(h->blist + h->boff[d]), (const int *)h->bp, (const float *)h->bk,
+				   (const double *)h->b, h->T, t0koef);
+	}
+	return I3G_OK;
+}
+
+extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const float *cp, const float *kt,
+			      const float *ht, const i3g_heat_params *p, i3g_heat_stats *st)
+{
+	if (!h || !tk || !ro || !cp || !kt || !ht || !p) return I3G_EINVAL;
+	if (!(p->t0koef > 0 && p->t0koef < 2) || p->sweeps < 1 || p->max_sweeps < 1) return I3G_EINVAL;
+	auto t_start = std::chrono::steady_clock::now();
+	long N = h->N;
+	i3g_heat_stats s{};
+	/* temperature step limit (heat3mg.c:327-368) from max |H/(ro cp)| over
+	 * the non-boundary nodes */
+	std::vector<float> rocp(N);
+	double maxrate = 0;
+	for (long i = 0; i < N; i++) {
+		rocp[i] = ro[i] * cp[i];
+		if (h->hb[i] == 0) {
+			double r = std::fabs((double)ht[i] / (double)rocp[i]);
+			if (r > maxrate) maxrate = r;
+		}
+	}
+	if (p->timestep <= 0) {
+		/* the reference returns without a solve when the step is zero */
+		if (st) *st = s;
+		return p->timestep == 0 ? I3G_OK : I3G_EINVAL;
+	}
+	double tkstep = 1e30;
+	if (maxrate > 0 && p->maxtkstep > 0) tkstep = p->maxtkstep / maxrate;
+	if (maxrate > 0 && p->maxtkstep == 0) {
+		if (st) *st = s;
+		return I3G_OK;   /* tktimestep == 0: the reference returns */
+	}
+	TCHK(hipMemcpyAsync(h->kt, kt, N * sizeof(float), hipMemcpyHostToDevice, h->s));
+	TCHK(hipMemcpyAsync(h->rocp, rocp.data(), N * sizeof(float), hipMemcpyHostToDevice, h->s));
+	TCHK(hipMemcpyAsync(h->ht, ht, N * sizeof(float), hipMemcpyHostToDevice, h->s));
+	TCHK(hipMemcpyAsync(h->tk0, tk, N * sizeof(float), hipMemcpyHostToDevice, h->s));
+	int plume = p->timesum > p->timebeg ? (p->timesum >= p->timeend ? 2 : 1) : 0;
+	dim3 gg((unsigned)(((long)h->nx * h->ny + NB - 1) / NB), (unsigned)h->nz);
+	long nblk = (long)gg.x * gg.y;
+	double total = p->timestep, done = 0;
+	double hres[2] = {0, 0};
+	int rc = I3G_OK;
+	while (done < total) {
+		double dt = total < tkstep ? total : tkstep;
+		if (dt > total - done) dt = total - done;
+		hipLaunchKernelGGL(k_heat_setup, gg, dim3(NB), 0, h->s, h->nx, h->ny, h->nz, (const double *)h->gx,
+				   (const double *)h->gy, (const double *)h->gz, (const float *)h->kt,
+				   (const float *)h->rocp, (const float *)h->ht, (const float *)h->tk0,
+				   (const int *)h->bp, (const float *)h->bc, dt, plume, p->timebeg, p->timeend, p->timesum,
+				   p->tempend, h->b, h->diag,
+				   h->T);
+		int sw = 0;
+		for (;;) {
+			for (int k = 0; k < p->sweeps; k++) {
+				for (int col = 0; col < 2; col++) {
+					hipLaunchKernelGGL(k_heat_sweep, gg, dim3(NB), 0, h->s, h->nx, h->ny, h->nz,
+							   (const double *)h->gx, (const double *)h->gy,
+							   (const double *)h->gz, (const float *)h->kt, (const int *)h->bp,
+							   (const double *)h->b, (const double *)h->diag, h->T, col,
+							   p->t0koef);
+					bc_apply(h, p->t0koef);
+				}
+			}
+			sw += p->sweeps;
+			hipLaunchKernelGGL(k_heat_norm, gg, dim3(NB), 0, h->s, h->nx, h->ny, h->nz, (const double *)h->gx,
+					   (const double *)h->gy, (const double *)h->gz, (const float *)h->kt,
+					   (const float *)h->rocp, (const int *)h->bp, (const float *)h->bk,
+					   (const double *)h->b, (const double *)h->diag, (const double *)h->T, dt, h->part);
+			hipLaunchKernelGGL(k_heat_sum, dim3(1), dim3(NB), 0, h->s, nblk, (const double *)h->part, h->res);
+			TCHK(hipGetLastError());
+			TCHK(hipMemcpyAsync(hres, h->res, 2 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+			TCHK(hipStreamSynchronize(h->s));
+			long nb = h->boff.back();
+			s.heatsum = std::sqrt(hres[0] / (double)(N - nb));
+			s.bondsum = nb ? std::sqrt(hres[1] / (double)nb) : 0;
+			if (p->verbose >= 2)
+				fprintf(stderr, "i3g heat: sub-step %d sweeps %d heatsum %.3e bondsum %.3e\n", s.substeps + 1,
+					sw, s.heatsum, s.bondsum);
+			if (!(s.heatsum > p->heatmin)) break;
+			if (sw >= p->max_sweeps) { rc = I3G_ENOCONV; break; }
+		}
+		s.sweeps += sw;
+		hipLaunchKernelGGL(k_heat_store, dim3((unsigned)((N + NB - 1) / NB)), dim3(NB), 0, h->s, N, h->T, h->tk0);
+		s.substeps++;
+		done += dt;
+		if (rc) break;
+	}
+	TCHK(hipGetLastError());
+	TCHK(hipMemcpyAsync(tk, h->tk0, N * sizeof(float), hipMemcpyDeviceToHost, h->s));
+	TCHK(hipStreamSynchronize(h->s));
+	s.converged = rc == I3G_OK;
+	s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+	if (st) *st = s;
+	return rc;
+}

@@ -62,6 +62,11 @@ struct i3g_markers {
 	int *tbc_p = nullptr;
 	float *tbc_c = nullptr, *tbc_k = nullptr, *tk1 = nullptr;
 	int has_tbc = 0;
+	/* marker temperature update: node fields tk, tk2, kt, ro, cp (5 x N),
+	 * per-marker diffusion change and flag, averaged nodal change */
+	float *tnod = nullptr;
+	double *dtk = nullptr, *sn = nullptr;
+	uint8_t *tin = nullptr;
 };
 
 /* ------------------------------------------------------------------ */
@@ -431,6 +436,163 @@ __global__ void k_tkbc(long N, const int *__restrict__ p, const float *__restric
 }
 
 /* ------------------------------------------------------------------ */
+/* Marker temperature update after the heat solve                     */
+/* ------------------------------------------------------------------ */
+
+/* tkcalc1 (mark3mg.c:2844-2910): cell, spacings, node numbers and weights
+ * of the 8 nodes around a point inside the grid, in the reference's node
+ * order (z, then x, then y). */
+__device__ __forceinline__ void tk_weights(const double *gx, const double *gy, const double *gz, int nx, int ny,
+					   int nz, double x, double y, double z, int *cell, double *d, long *wn,
+					   double *wi)
+{
+	int a0 = serch(gx, nx, x), b0 = serch(gy, ny, y), c0 = serch(gz, nz, z);
+	cell[0] = a0; cell[1] = b0; cell[2] = c0;
+	d[0] = gx[a0 + 1] - gx[a0];
+	d[1] = gy[b0 + 1] - gy[b0];
+	d[2] = gz[c0 + 1] - gz[c0];
+	int q = 0;
+	for (int c = c0; c <= c0 + 1; c++) {
+		double ddz = 1.0 - fabs((gz[c] - z) / d[2]);
+		for (int a = a0; a <= a0 + 1; a++) {
+			double ddx = 1.0 - fabs((gx[a] - x) / d[0]);
+			for (int b = b0; b <= b0 + 1; b++) {
+				double ddy = 1.0 - fabs((gy[b] - y) / d[1]);
+				wi[q] = ddx * ddy * ddz;
+				wn[q] = (long)c * nx * ny + (long)a * ny + b;
+				q++;
+			}
+		}
+	}
+}
+
+/* Per marker (titerate, heat3mg.c:604-701): fluid markers (type 50..99)
+ * take the new grid temperature; markers of type < 50 strictly inside the
+ * grid are initialised when new, relaxed by the subgrid diffusion and given
+ * the interpolated temperature change.  tin flags the markers whose
+ * diffusion change dtk enters the nodal average. */
+__global__ void k_tk_marker(long n, const float *__restrict__ x, const float *__restrict__ y,
+			    const float *__restrict__ z, const signed char *__restrict__ type, float *markk,
+			    const double *gx, const double *gy, const double *gz, int nx, int ny, int nz,
+			    const float *__restrict__ tk, const float *__restrict__ tk2, const float *__restrict__ kt,
+			    const float *__restrict__ ro, const float *__restrict__ cp, double heatdif, double dt,
+			    double zdeep, double tdeep, double *dtk, uint8_t *tin)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	tin[i] = 0;
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	float fx = x[i], fy = y[i], fz = z[i];
+	int mt = type[i];
+	if (!(fx > 0 && fy > 0 && fz > 0 && (double)fx < xs && (double)fy < ys && (double)fz < zs)) return;
+	if (mt >= 100) return;
+	int cell[3];
+	double d[3], wi[8];
+	long wn[8];
+	tk_weights(gx, gy, gz, nx, ny, nz, fx, fy, fz, cell, d, wn, wi);
+	double e2 = 0, e3 = 0;
+	for (int q = 0; q < 8; q++) {
+		e2 += wi[q] * tk[wn[q]];
+		e3 += wi[q] * tk2[wn[q]];
+	}
+	if (mt >= 50) {
+		markk[i] = (float)e2;
+		return;
+	}
+	float mk = markk[i];
+	if (mk <= 0) {
+		mk = (float)e3;
+		if (fy > zdeep && mk < tdeep) mk = (float)tdeep;
+	}
+	if (heatdif != 0) {
+		double mkt = 0, rocp = 0, mtk = tk[wn[0]];
+		for (int q = 0; q < 8; q++) {
+			mkt += kt[wn[q]] * wi[q];
+			rocp += (ro[wn[q]] * cp[wn[q]]) * wi[q];
+			mtk = fmax(mtk, (double)tk[wn[q]]);
+		}
+		if ((cell[0] <= 0 || cell[0] >= nx - 2 || cell[1] <= 0 || cell[1] >= ny - 2 || cell[2] <= 0 ||
+		     cell[2] >= nz - 2) &&
+		    ((double)mk + e2 - e3) > mtk)
+			mk = (float)(mtk - e2 + e3);
+		double dd = e3 - (double)mk;
+		double d1 = -heatdif * mkt / rocp * 2.0 * (1.0 / d[0] / d[0] + 1.0 / d[1] / d[1] + 1.0 / d[2] / d[2]) * dt;
+		if (d1 < -150.0) d1 = -150.0;
+		d1 = dd * (1.0 - exp(d1));
+		mk += (float)d1;
+		dtk[i] = d1;
+		tin[i] = 1;
+	}
+	mk += (float)(e2 - e3);
+	markk[i] = mk;
+}
+
+/* Per node: average of the markers' diffusion changes, weights summed in
+ * marker id order over the (up to) 8 cells around the node, as the
+ * reference's serial scatter (heat3mg.c:687-699, 711-728). */
+__global__ void __launch_bounds__(256)
+k_tk_gather(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz, const int *start,
+	    const int *cnt, const int *__restrict__ id, const float *__restrict__ x, const float *__restrict__ y,
+	    const float *__restrict__ z, const uint8_t *__restrict__ tin, const double *__restrict__ dtk, double *sn)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int m3 = blockIdx.y;
+	int m1 = t / ny, m2 = t - m1 * ny;
+	if (m1 >= nx) return;
+	long nd = (long)m3 * nx * ny + (long)m1 * ny + m2;
+	int ps[8], pe[8], ca[8], cb[8], cc[8];
+	int ncell = 0;
+	for (int c = m3 - 1; c <= m3; c++)
+		for (int a = m1 - 1; a <= m1; a++)
+			for (int b = m2 - 1; b <= m2; b++) {
+				if (a < 0 || b < 0 || c < 0 || a > nx - 2 || b > ny - 2 || c > nz - 2) continue;
+				long k = (long)c * nx * ny + (long)a * ny + b;
+				ps[ncell] = start[k];
+				pe[ncell] = start[k] + cnt[k];
+				ca[ncell] = a; cb[ncell] = b; cc[ncell] = c;
+				ncell++;
+			}
+	double s0 = 0, s1 = 0;
+	while (true) {
+		int best = -1, bid = 0x7fffffff;
+		for (int q = 0; q < ncell; q++)
+			if (ps[q] < pe[q]) {
+				int v = id[ps[q]];
+				if (v < bid) { bid = v; best = q; }
+			}
+		if (best < 0) break;
+		int mk = ps[best]++;
+		if (!tin[mk]) continue;
+		int a = ca[best], b = cb[best], c = cc[best];
+		double dx = gx[a + 1] - gx[a], dy = gy[b + 1] - gy[b], dz = gz[c + 1] - gz[c];
+		double ddz = 1.0 - fabs((gz[m3] - (double)z[mk]) / dz);
+		double ddx = 1.0 - fabs((gx[m1] - (double)x[mk]) / dx);
+		double ddy = 1.0 - fabs((gy[m2] - (double)y[mk]) / dy);
+		double w = ddx * ddy * ddz;
+		s1 += dtk[mk] * w;
+		s0 += w;
+	}
+	if (s0 != 0) s1 /= s0;
+	sn[nd] = s1;
+}
+
+/* Per marker: antidiffusion, markk -= sum of the averaged nodal changes
+ * (heat3mg.c:733-776). */
+__global__ void k_tk_anti(long n, const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+			  const uint8_t *__restrict__ tin, float *markk, const double *gx, const double *gy,
+			  const double *gz, int nx, int ny, int nz, const double *__restrict__ sn)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n || !tin[i]) return;
+	int cell[3];
+	double d[3], wi[8], a = 0;
+	long wn[8];
+	tk_weights(gx, gy, gz, nx, ny, nz, x[i], y[i], z[i], cell, d, wn, wi);
+	for (int q = 0; q < 8; q++) a -= sn[wn[q]] * wi[q];
+	markk[i] += (float)a;
+}
+
+/* ------------------------------------------------------------------ */
 /* RK4 advection                                                      */
 /* ------------------------------------------------------------------ */
 
@@ -587,7 +749,7 @@ extern "C" int i3g_markers_destroy(i3g_markers *m)
 	void *ps[] = {m->gx, m->gy, m->gz, m->x, m->y, m->z, m->tk, m->eta, m->type, m->id, m->x2, m->y2, m->z2,
 		      m->tk2, m->eta2, m->type2, m->id2, m->key, m->c1, m->c2, m->c3, m->perm, m->cnt, m->start,
 		      m->cur, m->bsum, m->pnu, m->pro, m->pcp, m->pkt, m->pht, m->pbb, m->ptk, m->pin, m->pr, m->fld,
-		      m->vel, m->rocks, m->bad, m->tbc_p, m->tbc_c, m->tbc_k, m->tk1};
+		      m->vel, m->rocks, m->bad, m->tbc_p, m->tbc_c, m->tbc_k, m->tk1, m->tnod, m->dtk, m->sn, m->tin};
 	for (void *p : ps)
 		if (p) hipFree(p);
 	if (m->s) hipStreamDestroy(m->s);
@@ -798,6 +960,45 @@ extern "C" int i3g_markers_set_tbc(i3g_markers *m, const int *bondm_t, const flo
 	MCHK(hipMemcpyAsync(m->tbc_k, k.data(), N * sizeof(float), hipMemcpyHostToDevice, m->s));
 	MCHK(hipStreamSynchronize(m->s));
 	m->has_tbc = 1;
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_update_tk(i3g_markers *m, const float *tk, const float *tk2, const float *kt,
+				     const float *ro, const float *cp, double heatdif, double timestep, double zdeep,
+				     double tdeep)
+{
+	if (!m || !tk || !tk2 || !kt || !ro || !cp || !(heatdif >= 0)) return I3G_EINVAL;
+	if (heatdif != 0 && !m->sorted) return I3G_EINVAL;
+	long N = m->N, n = m->n;
+	int err = 0;
+	if (!m->tnod) {
+		m->tnod = mk_alloc<float>(5 * N, &err);
+		m->sn = mk_alloc<double>(N, &err);
+		m->dtk = mk_alloc<double>(m->cap, &err);
+		m->tin = mk_alloc<uint8_t>(m->cap, &err);
+		if (err) return err;
+	}
+	const float *src[5] = {tk, tk2, kt, ro, cp};
+	for (int q = 0; q < 5; q++)
+		MCHK(hipMemcpyAsync(m->tnod + q * N, src[q], N * sizeof(float), hipMemcpyHostToDevice, m->s));
+	const float *f = m->tnod;
+	if (n)
+		hipLaunchKernelGGL(k_tk_marker, g1(n), dim3(256), 0, m->s, n, (const float *)m->x, (const float *)m->y,
+				   (const float *)m->z, (const signed char *)m->type, m->tk, (const double *)m->gx,
+				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, f, f + N, f + 2 * N,
+				   f + 3 * N, f + 4 * N, heatdif, timestep, zdeep, tdeep, m->dtk, m->tin);
+	if (n && heatdif != 0) {
+		dim3 gg((unsigned)(((long)m->nx * m->ny + 255) / 256), (unsigned)m->nz);
+		hipLaunchKernelGGL(k_tk_gather, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz, (const double *)m->gx,
+				   (const double *)m->gy, (const double *)m->gz, (const int *)m->start, (const int *)m->cnt,
+				   (const int *)m->id, (const float *)m->x, (const float *)m->y, (const float *)m->z,
+				   (const uint8_t *)m->tin, (const double *)m->dtk, m->sn);
+		hipLaunchKernelGGL(k_tk_anti, g1(n), dim3(256), 0, m->s, n, (const float *)m->x, (const float *)m->y,
+				   (const float *)m->z, (const uint8_t *)m->tin, m->tk, (const double *)m->gx,
+				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, (const double *)m->sn);
+	}
+	MCHK(hipGetLastError());
+	MCHK(hipStreamSynchronize(m->s));
 	return I3G_OK;
 }
 

@@ -194,6 +194,50 @@ int i3g_kernel_stats(i3g_ctx *ctx, char *names, int name_len, double *avg_ms,
 		     long *calls, double *bytes, int cap);
 
 /* ------------------------------------------------------------------ */
+/* Heat                                                               */
+/* ------------------------------------------------------------------ */
+
+/* Device-resident implicit temperature step on the level-0 grid; replaces
+ * the solve part of titerate (heat3mg.c:287-588): heaterr rows (ynerr=1,
+ * heat3mg.c:160-225) without adiabatic and shear heating, tbonderr rows
+ * from the T block of bondm (heat3mg.c:794-843), relaxation
+ * T -= (A T - b)/diag * t0koef in red-black order, convergence when the
+ * reference's heat norm (RMS over non-boundary nodes of the row error in
+ * K, heat3mg.c:512-560) is <= heatmin. */
+typedef struct i3g_heat i3g_heat;
+
+typedef struct {
+	double timestep;   /* s; 0 returns at once as the reference */
+	double maxtkstep;  /* K: sub-step = maxtkstep / max|H/(ro cp)| (heat3mg.c:358-384) */
+	double heatmin;    /* K (HEATMIN) */
+	double t0koef;     /* relaxation factor, (0,2) */
+	int sweeps;        /* red-black sweeps between norm checks (multittt[0][0]) */
+	int max_sweeps;    /* per sub-step; I3G_ENOCONV beyond */
+	double timesum, timebeg, timeend, tempend; /* bottom-line plume ramp (heat3mg.c:444-455) */
+	int verbose;
+} i3g_heat_params;
+
+typedef struct {
+	int converged;
+	int substeps;
+	int sweeps;        /* total over the sub-steps */
+	double heatsum;    /* final heat norm (K) */
+	double bondsum;    /* final boundary-row norm (K) */
+	double seconds;
+} i3g_heat_stats;
+
+/* bondm/bondv/bondn of the setup are read for the T block (bondm +
+ * 8*nodenum); every surface node must carry a T boundary row. */
+int i3g_heat_create(const i3g_setup *s, i3g_heat **out);
+int i3g_heat_destroy(i3g_heat *h);
+int i3g_heat_default_params(i3g_heat_params *p);
+/* tk (in/out, float, level-0 nodes) is advanced by p->timestep; ro, cp,
+ * kt, ht are the node fields of ronurecalc.  The start temperature (the
+ * reference's tk2) is the caller's input copy. */
+int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const float *cp, const float *kt,
+		   const float *ht, const i3g_heat_params *p, i3g_heat_stats *st);
+
+/* ------------------------------------------------------------------ */
 /* Markers                                                            */
 /* ------------------------------------------------------------------ */
 
@@ -266,11 +310,54 @@ int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int nrock, const
 int i3g_markers_set_tbc(i3g_markers *m, const int *bondm_t, const float *bondv, const int *bondn,
 			long nrec);
 
+/* Marker temperature update after the heat solve; replaces the marker
+ * part of titerate (heat3mg.c:592-782, interpolation tkcalc1
+ * mark3mg.c:2844-2910): markers of type 50..99 inside the grid take the
+ * interpolated new temperature; markers of type < 50 strictly inside the
+ * grid are initialised from tk2 when markk <= 0 (floor tdeep below zdeep),
+ * relaxed towards the grid by the subgrid diffusion (heatdif > 0) whose
+ * nodal average is removed again, and given the interpolated change
+ * tk - tk2.  tk: temperature after the solve, tk2: before (host, level-0
+ * nodes).  Markers must be sorted when heatdif > 0; the nodal averages
+ * follow the reference's serial marker order, so the result is
+ * deterministic. */
+int i3g_markers_update_tk(i3g_markers *m, const float *tk, const float *tk2, const float *kt,
+			  const float *ro, const float *cp, double heatdif, double timestep, double zdeep,
+			  double tdeep);
+
 /* RK4 advection of the resident markers by dt in the staggered velocity
  * field (vxyzcalc1 + movemark, mark3mg.c:215-284, 2347-2530); markers that
  * leave the grid get tk = eta = 0 (mark3mg.c:280-284). */
 int i3g_markers_advect_rk4(i3g_markers *m, double dt, const float *vx, const float *vy,
 			   const float *vz);
+
+/* ------------------------------------------------------------------ */
+/* Learned warm start                                                  */
+/* ------------------------------------------------------------------ */
+
+/* 3D convolutional encoder-decoder (levels of 16/32/64/64 channels, 3x3x3
+ * kernels, ReLU, strided convolutions down, node-linear upsampling with
+ * additive skips up) mapping two node fields to four.  No counterpart in
+ * the reference: it supplies the initial guess that vpiterate takes from
+ * the previous step's vx/vy/vz/pr (move3mg.c:1585-1700).  BF16 matrix-core
+ * products with FP32 accumulation; FP32 parameters and Adam.
+ * Grids: nz, nx, ny = 8k+1 (three halvings).  Fields are node-major with
+ * channels fastest: in [nodes][2], out/target [nodes][4]; the caller
+ * chooses their scaling.  Parameter vector: per layer W [27][cout][cin]
+ * (tap 9 tz + 3 tx + ty) then b [cout], layers in forward order. */
+typedef struct i3g_net i3g_net;
+
+int i3g_warmstart_create(long nz, long nx, long ny, unsigned long long seed, i3g_net **out);
+int i3g_warmstart_destroy(i3g_net *net);
+long i3g_warmstart_nparams(const i3g_net *net);
+int i3g_warmstart_get_params(i3g_net *net, float *params);
+int i3g_warmstart_set_params(i3g_net *net, const float *params);
+int i3g_warmstart_infer(i3g_net *net, const float *in, float *out);
+/* gradient of the mean squared error of one sample (grad may be NULL) */
+int i3g_warmstart_grad(i3g_net *net, const float *in, const float *target, double *loss, float *grad);
+/* one Adam step (beta 0.9/0.999, eps 1e-8) on the mean gradient of a batch */
+int i3g_warmstart_train_step(i3g_net *net, int batch, const float *in, const float *target, double lr,
+			     double *loss);
 
 const char *i3g_error_string(int code);
 
