@@ -267,6 +267,9 @@ struct i3g_ctx {
 	hipGraph_t cg_graph[2] = {nullptr, nullptr};
 	std::vector<double> cg_key[2];
 	int use_graph = 1;
+	/* XCD-contiguous plane slabs for the level-0 smoother (I3G_XCD_REMAP=0
+	 * switches it off) */
+	int xcd_remap = 1;
 };
 
 /* ---- event timing of named kernels ------------------------------------ */
@@ -640,6 +643,10 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 	if (rc) { delete c; return rc; }
 	if (s->nx * s->ny * s->nz > 400000000L) { delete c; return I3G_ECAP; }
 	c->nlev = c->h.nlev;
+	{
+		const char *e = getenv("I3G_XCD_REMAP");
+		if (e && e[0] == '0') c->xcd_remap = 0;
+	}
 	if (hipStreamCreate(&c->s) != hipSuccess) { delete c; return I3G_EDEVICE; }
 	c->L.resize(c->nlev);
 	for (int l = 0; l < c->nlev; l++) {
@@ -796,11 +803,14 @@ static void sweep(i3g_ctx *c, int l, T *x, const T *b, const i3g_stokes_params *
 	Level &L = c->L[l];
 	LevelView<T> v = view<T>(L);
 	dim3 hg = half_grid(L.nx, L.ny, L.nz);
+	int chunks = (int)hg.x, remap = c->xcd_remap && l == 0;
+	if (remap) hg = dim3((unsigned)((((long)chunks * L.nz + 7) / 8) * 8));
 	double by = (double)L.N * (sizeof(T) * 2 + 1 + sizeof(T) * 4 + sizeof(T) * 3) / 2;
 	for (int comp = 1; comp <= 3; comp++)
 		for (int col = 0; col < 2; col++) {
 			Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "smooth_vel_f64_L0" : "smooth_vel_f32_L0") : "smooth_vel_coarse", by);
-			hipLaunchKernelGGL(k_smooth_vel<T>, hg, dim3(256), 0, c->s, v, x, b, comp, col, (T)p->v0koef);
+			hipLaunchKernelGGL(k_smooth_vel<T>, hg, dim3(256), 0, c->s, v, x, b, comp, col, (T)p->v0koef, remap,
+					   chunks);
 		}
 	{
 		Timed tm(c, l == 0 ? "smooth_p_L0" : "smooth_p_coarse", 0);
@@ -1440,14 +1450,30 @@ static int solve_impl(i3g_ctx *c, const i3g_stokes_params *p, i3g_stats *st)
 	return loc.converged ? I3G_OK : I3G_ENOCONV;
 }
 
+/* unknowns that do not exist on the staggered grid are zero */
+__global__ void k_zero_missing(double *x, long N, const uint8_t *flags)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= N) return;
+	unsigned fl = flags[t];
+	for (int q = 0; q < 4; q++)
+		if (!(fl & FL_EX(q))) x[q * N + t] = 0.0;
+}
+
 extern "C" int i3g_stokes_set_solution(i3g_ctx *c, const double *P, const double *vx, const double *vy,
 				       const double *vz)
 {
 	if (!c || !P || !vx || !vy || !vz) return I3G_EINVAL;
-	long N = c->L[0].N;
+	Level &L = c->L[0];
+	long N = L.N;
 	const double *src[4] = {P, vx, vy, vz};
 	for (int q = 0; q < 4; q++)
 		HCHK(hipMemcpyAsync(c->sol + q * N, src[q], N * sizeof(double), hipMemcpyHostToDevice, c->s));
+	/* the guess as the solver sees it: missing unknowns zero, boundary rows
+	 * satisfied */
+	hipLaunchKernelGGL(k_zero_missing, lin_grid(N), dim3(256), 0, c->s, c->sol, N, (const uint8_t *)L.flags);
+	if (c->coef_ready) bc_enforce<double>(c, L, c->sol, 0);
+	HCHK(hipGetLastError());
 	HCHK(hipStreamSynchronize(c->s));
 	return I3G_OK;
 }

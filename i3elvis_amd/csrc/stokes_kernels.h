@@ -217,13 +217,34 @@ k_residual(LevelView<T> L, const X *__restrict__ x, const T *__restrict__ b,
  * colour ((i+j+k) mod 2 == colour): x += v0koef (b - A x)/diag, the
  * gausmat4 update (move3mg.c:3121-3124).  BC rows are left to k_bc.
  * Threads cover every other y index of each (x,z) line. */
+/* Work-group -> (plane, chunk) map.  remap 0: grid (chunks, planes).
+ * remap 1: a 1D grid of 8*ceil(chunks*planes/8) groups; the hardware hands
+ * groups to the 8 XCDs round-robin, so group g is given work item
+ * (g % 8) * per + g / 8: every XCD sweeps a contiguous slab of planes in
+ * order and finds the neighbouring planes in its own L2. */
+__device__ __forceinline__ bool block_work(int remap, int chunks, int planes, int *chunk, int *plane)
+{
+	if (!remap) {
+		*chunk = blockIdx.x;
+		*plane = blockIdx.y;
+		return true;
+	}
+	int per = gridDim.x >> 3;
+	int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+	if (w >= chunks * planes) return false;
+	*plane = w / chunks;
+	*chunk = w - *plane * chunks;
+	return true;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_smooth_vel(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int comp,
-	     int colour, T v0koef)
+	     int colour, T v0koef, int remap, int chunks)
 {
-	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	int k = blockIdx.y;
+	int ch, k;
+	if (!block_work(remap, chunks, L.nz, &ch, &k)) return;
+	int t = ch * blockDim.x + threadIdx.x;
 	int hy = (L.ny + 1) >> 1;
 	int i = t / hy, tj = t - i * hy;
 	if (i >= L.nx) return;

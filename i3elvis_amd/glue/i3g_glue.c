@@ -1,0 +1,264 @@
+/*
+ * i3g_glue.c -- drop-in glue between a program built on the reference's
+ * global state (head3mg.c) and the i3g C ABI (include/i3g.h).
+ *
+ * Use: compile this file in the same translation unit as the program's
+ * globals, after head3mg.c (the reference builds i3mg.c as one unit that
+ * #includes its other sources), link libi3g.so, and call
+ *     i3g_vpiterate(m0, f0)  where the program calls vpiterate(m0, f0)
+ *     i3g_titerate(m0)       where it calls titerate(m0)
+ *     i3g_ronurecalc()       where it calls ronurecalc()
+ *     i3g_movemark()         where it calls movemark()
+ * (i3mg.c:98-137).  The functions read and write the same globals as the
+ * routines they replace; device state is created on first use and kept
+ * between steps (re-created when the grid changes).
+ *
+ * What each replaces:
+ *   i3g_vpiterate  the Stokes solve of vpiterate (move3mg.c:1585-2636):
+ *                  solve to i3g_glue_rtol (relative FP64 residual) from the
+ *                  current vx/vy/vz/pr, pressure gauge (pxinit..pinit),
+ *                  stress/strain fields by the program's own sxxcalc..syzcalc
+ *                  in value mode, and the marker time-step limit
+ *                  (move3mg.c:2609-2632).
+ *   i3g_titerate   the heat step of titerate (heat3mg.c:287-588) without
+ *                  adiabatic/shear heating, then the marker temperature
+ *                  update (heat3mg.c:592-782).
+ *   i3g_ronurecalc the node fields of ronurecalc (mark3mg.c:404-1015) for
+ *                  Newtonian rock types, with tkrecalc.
+ *   i3g_movemark   the RK4 position update of movemark (mark3mg.c:215-284).
+ * Errors print a message and exit, as the reference does on fatal
+ * conditions.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "i3g.h"
+
+double i3g_glue_rtol = 1e-8;
+int i3g_glue_verbose = 0;
+
+static struct {
+	i3g_ctx *ctx;
+	i3g_heat *heat;
+	i3g_markers *mk;
+	long nx, ny, nz, mcap, nrec;
+	int multinum;
+	double gsum;
+	double *P, *X, *Y, *Z;
+	float *tkstart;
+} G;
+
+static void glue_fail(const char *what, int rc)
+{
+	fprintf(stderr, "EXIT PROGRAM: i3g %s failed: %s (%d)\n", what, i3g_error_string(rc), rc);
+	exit(1);
+}
+
+/* grid lines and BC table size identify the grid; a change re-creates the
+ * device state */
+static double grid_sum(void)
+{
+	double s = 0;
+	long i;
+	for (i = 0; i < xnumx; i++) s += gx[i] * (i + 1);
+	for (i = 0; i < ynumy; i++) s += gy[i] * (i + 2);
+	for (i = 0; i < znumz; i++) s += gz[i] * (i + 3);
+	return s;
+}
+
+static void glue_grid(void)
+{
+	int rc;
+	double gs = grid_sum();
+	if (G.ctx && G.nx == xnumx && G.ny == ynumy && G.nz == znumz && G.multinum == multinum &&
+	    G.nrec == bondnum + 1 && G.gsum == gs)
+		return;
+	if (G.ctx) i3g_destroy(G.ctx);
+	if (G.heat) i3g_heat_destroy(G.heat);
+	if (G.mk) i3g_markers_destroy(G.mk);
+	free(G.P); free(G.X); free(G.Y); free(G.Z); free(G.tkstart);
+	G.ctx = NULL; G.heat = NULL; G.mk = NULL; G.mcap = 0;
+	i3g_setup s;
+	s.nx = xnumx; s.ny = ynumy; s.nz = znumz;
+	s.gx = gx; s.gy = gy; s.gz = gz;
+	s.multinum = multinum;
+	s.bondm = bondm;
+	s.bondv = &bondv1[0][0];
+	s.bondn = bondn1;
+	s.nrec = bondnum + 1;
+	if ((rc = i3g_create(&s, &G.ctx))) glue_fail("create", rc);
+	if ((rc = i3g_heat_create(&s, &G.heat))) glue_fail("heat_create", rc);
+	G.nx = xnumx; G.ny = ynumy; G.nz = znumz; G.multinum = multinum; G.nrec = bondnum + 1; G.gsum = gs;
+	long N = xnumx * ynumy * znumz;
+	G.P = malloc(sizeof(double) * N); G.X = malloc(sizeof(double) * N);
+	G.Y = malloc(sizeof(double) * N); G.Z = malloc(sizeof(double) * N);
+	G.tkstart = malloc(sizeof(float) * N);
+	if (!G.P || !G.X || !G.Y || !G.Z || !G.tkstart) glue_fail("host allocation", I3G_ENOMEM);
+}
+
+/* markers of the program, uploaded in their order (ids = positions) */
+static void glue_markers(void)
+{
+	int rc;
+	if (!G.mk || G.mcap < marknum) {
+		if (G.mk) i3g_markers_destroy(G.mk);
+		G.mcap = marknum + marknum / 4 + 1024;
+		if ((rc = i3g_markers_create(xnumx, ynumy, znumz, gx, gy, gz, G.mcap, &G.mk))) glue_fail("markers_create", rc);
+	}
+	if ((rc = i3g_markers_upload(G.mk, marknum, markx, marky, markz, markk, markv, (const signed char *)markt, NULL)))
+		glue_fail("markers_upload", rc);
+}
+
+static void glue_markers_back(void)
+{
+	int rc;
+	long n = marknum, i;
+	float *buf = malloc(sizeof(float) * 5 * (n ? n : 1));
+	signed char *t = malloc(n ? n : 1);
+	int *id = malloc(sizeof(int) * (n ? n : 1));
+	if (!buf || !t || !id) glue_fail("host allocation", I3G_ENOMEM);
+	if ((rc = i3g_markers_download(G.mk, buf, buf + n, buf + 2 * n, buf + 3 * n, buf + 4 * n, t, id)))
+		glue_fail("markers_download", rc);
+	for (i = 0; i < n; i++) {
+		long j = id[i];
+		markx[j] = buf[i]; marky[j] = buf[n + i]; markz[j] = buf[2 * n + i];
+		markk[j] = buf[3 * n + i]; markv[j] = buf[4 * n + i];
+	}
+	free(buf); free(t); free(id);
+}
+
+void i3g_vpiterate(int m0, int f0)
+{
+	long N, n, m1, m2, m3;
+	int rc;
+	(void)m0; (void)f0;
+	glue_grid();
+	N = xnumx * ynumy * znumz;
+	double phys[6] = {GXKOEF, GYKOEF, GZKOEF, nubeg, nuend, (double)viscmod};
+	if ((rc = i3g_set_coefficients(G.ctx, nu, nxx, nxy, nxz, nyz, ro, phys))) glue_fail("set_coefficients", rc);
+	for (n = 0; n < N; n++) {
+		G.P[n] = pr[n]; G.X[n] = vx[n]; G.Y[n] = vy[n]; G.Z[n] = vz[n];
+	}
+	i3g_stokes_params p;
+	i3g_stats st;
+	i3g_stokes_default_params(&p);
+	p.rtol = i3g_glue_rtol;
+	p.pxinit = (long)pxinit; p.pyinit = (long)pyinit; p.pzinit = (long)pzinit; p.pinit = pinit;
+	p.verbose = i3g_glue_verbose;
+	rc = i3g_stokes_solve(G.ctx, G.P, G.X, G.Y, G.Z, &p, &st);
+	if (rc && rc != I3G_ENOCONV) glue_fail("stokes_solve", rc);
+	if (rc == I3G_ENOCONV) fprintf(stderr, "i3g: Stokes solve stopped at relative residual %.3e\n", st.rel_res);
+	double mnx = 1e300, mxx = -1e300, mny = 1e300, mxy = -1e300, mnz = 1e300, mxz = -1e300;
+	for (n = 0; n < N; n++) {
+		vx[n] = (float)G.X[n]; vy[n] = (float)G.Y[n]; vz[n] = (float)G.Z[n]; pr[n] = G.P[n];
+	}
+	/* value ranges over the existing velocity nodes, then the marker time
+	 * step limit maxxystep / |v|max (move3mg.c:2609-2632) */
+	for (m3 = 0; m3 < znumz; m3++)
+		for (m1 = 0; m1 < xnumx; m1++)
+			for (m2 = 0; m2 < ynumy; m2++) {
+				n = m3 * xnumx * ynumy + m1 * ynumy + m2;
+				if (m2 < ynumy - 1 && m3 < znumz - 1) {
+					if (vx[n] < mnx) mnx = vx[n];
+					if (vx[n] > mxx) mxx = vx[n];
+				}
+				if (m1 < xnumx - 1 && m3 < znumz - 1) {
+					if (vy[n] < mny) mny = vy[n];
+					if (vy[n] > mxy) mxy = vy[n];
+				}
+				if (m1 < xnumx - 1 && m2 < ynumy - 1) {
+					if (vz[n] < mnz) mnz = vz[n];
+					if (vz[n] > mxz) mxz = vz[n];
+				}
+			}
+	double vm[3] = {mxx > -mnx ? mxx : -mnx, mxy > -mny ? mxy : -mny, mxz > -mnz ? mxz : -mnz};
+	for (int a = 0; a < 3; a++)
+		if (vm[a] > 0 && timestep > maxxystep / vm[a]) timestep = maxxystep / vm[a];
+	/* stress and strain-rate fields from the solution (value mode of the
+	 * program's own stress routines, as at the end of vpiterate) */
+	{
+		long un1[MAXPOS];
+		double ui1[MAXPOS], eps1[100];
+		for (m3 = 0; m3 < znumz; m3++)
+			for (m1 = 0; m1 < xnumx; m1++)
+				for (m2 = 0; m2 < ynumy; m2++) {
+					n = m3 * xnumx * ynumy + m1 * ynumy + m2;
+					if (m1 > 0 && m2 > 0 && m3 > 0) {
+						sxx[n] = sxxcalc(m1, m2, m3, 0, 0, eps1, un1, ui1); exx[n] = eps1[0];
+						syy[n] = syycalc(m1, m2, m3, 0, 0, eps1, un1, ui1); eyy[n] = eps1[0];
+						szz[n] = szzcalc(m1, m2, m3, 0, 0, eps1, un1, ui1); ezz[n] = eps1[0];
+					}
+					if (m1 > 0 && m1 < xnumx - 1 && m2 > 0 && m2 < ynumy - 1 && m3 < znumz - 1) {
+						sxy[n] = sxycalc(m1, m2, m3, 0, 0, eps1, un1, ui1); exy[n] = eps1[0];
+					}
+					if (m1 > 0 && m1 < xnumx - 1 && m2 < ynumy - 1 && m3 > 0 && m3 < znumz - 1) {
+						sxz[n] = sxzcalc(m1, m2, m3, 0, 0, eps1, un1, ui1); exz[n] = eps1[0];
+					}
+					if (m1 < xnumx - 1 && m2 > 0 && m2 < ynumy - 1 && m3 > 0 && m3 < znumz - 1) {
+						syz[n] = syzcalc(m1, m2, m3, 0, 0, eps1, un1, ui1); eyz[n] = eps1[0];
+					}
+				}
+	}
+}
+
+void i3g_titerate(int m0)
+{
+	long N, n;
+	int rc;
+	(void)m0;
+	glue_grid();
+	N = xnumx * ynumy * znumz;
+	for (n = 0; n < N; n++) G.tkstart[n] = tk[n];
+	i3g_heat_params hp;
+	i3g_heat_stats hs;
+	i3g_heat_default_params(&hp);
+	hp.timestep = timestep;
+	hp.maxtkstep = maxtkstep;
+	hp.heatmin = HEATMIN;
+	hp.t0koef = t0koef;
+	hp.sweeps = multittt[0][0] > 0 ? multittt[0][0] : 1;
+	hp.timesum = timesum; hp.timebeg = timebeg; hp.timeend = timeend; hp.tempend = tempend;
+	hp.verbose = i3g_glue_verbose;
+	rc = i3g_heat_solve(G.heat, tk, ro, cp, kt, ht, &hp, &hs);
+	if (rc && rc != I3G_ENOCONV) glue_fail("heat_solve", rc);
+	if (hs.substeps == 0) return; /* zero step: the reference returns before the marker update */
+	glue_markers();
+	if ((rc = i3g_markers_sort(G.mk, NULL))) glue_fail("markers_sort", rc);
+	rc = i3g_markers_update_tk(G.mk, tk, G.tkstart, kt, ro, cp, heatdif, timestep, zdeep, tdeep);
+	if (rc) glue_fail("markers_update_tk", rc);
+	glue_markers_back();
+}
+
+void i3g_ronurecalc(void)
+{
+	int rc, t;
+	glue_grid();
+	glue_markers();
+	if ((rc = i3g_markers_sort(G.mk, NULL))) glue_fail("markers_sort", rc);
+	if ((rc = i3g_markers_set_tbc(G.mk, bondm + nodenum8, &bondv1[0][0], bondn1, bondnum + 1)))
+		glue_fail("markers_set_tbc", rc);
+	static i3g_rock rocks[64];
+	int nr = rocknum > 0 && rocknum <= 64 ? rocknum : 64;
+	for (t = 0; t < nr; t++) {
+		rocks[t].nu0 = marknu[t]; rocks[t].dh = markdh[t]; rocks[t].dv = markdv[t];
+		rocks[t].n0 = markn0[t]; rocks[t].n1 = markn1[t];
+		rocks[t].ro = markro[t]; rocks[t].bro = markbb[t]; rocks[t].aro = markaa[t];
+		rocks[t].cp = markcp[t]; rocks[t].kt = markkt[t]; rocks[t].kf = markkf[t]; rocks[t].kp = markkp[t];
+		rocks[t].ht = markht[t];
+	}
+	i3g_mk_params mp;
+	mp.nubeg = nubeg; mp.nuend = nuend; mp.nukoef = nukoef; mp.densimod = densimod; mp.timestep = timestep;
+	mp.vdeep = vdeep; mp.nudeep = nudeep; mp.viscmod = viscmod;
+	rc = i3g_markers_to_nodes(G.mk, rocks, nr, &mp, pr, nu, ro, cp, kt, ht, et, tk, nxx, nxy, nxz, nyz);
+	if (rc) glue_fail("markers_to_nodes", rc);
+	glue_markers_back();
+}
+
+void i3g_movemark(void)
+{
+	int rc;
+	glue_grid();
+	glue_markers();
+	if ((rc = i3g_markers_advect_rk4(G.mk, timestep, vx, vy, vz))) glue_fail("markers_advect_rk4", rc);
+	glue_markers_back();
+}

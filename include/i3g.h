@@ -180,6 +180,9 @@ int i3g_stokes_solve_resident(i3g_ctx *ctx, const i3g_stokes_params *p,
 			      i3g_stats *st);
 int i3g_stokes_get_solution(i3g_ctx *ctx, double *P, double *vx, double *vy,
 			    double *vz);
+/* Sets the resident iterate; unknowns that do not exist on the staggered
+ * grid are zeroed and, once coefficients are set, the boundary rows are
+ * applied (the guess as the solver starts from it). */
 int i3g_stokes_set_solution(i3g_ctx *ctx, const double *P, const double *vx,
 			    const double *vy, const double *vz);
 

@@ -14,15 +14,17 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 PATH = os.path.join(_HERE, "_ref", "libi3ref.so")
+# the same reference unit plus the product's drop-in glue (oracle/Makefile: glue)
+GLUE_PATH = os.path.join(_HERE, "_ref", "libi3ref_glue.so")
 
 
-def available():
-    return os.path.isfile(PATH)
+def available(path=PATH):
+    return os.path.isfile(path)
 
 
 class Ref:
-    def __init__(self):
-        self.lib = C.CDLL(PATH)
+    def __init__(self, path=PATH):
+        self.lib = C.CDLL(path)
         caps = (C.c_long * 11)()
         self.lib.refh_caps(caps)
         (self.MAXXLN, self.MAXYLN, self.MAXZLN, self.MAXNOD, self.MAXPPP,

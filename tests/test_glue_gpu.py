@@ -1,0 +1,161 @@
+"""GPU tests of the drop-in glue (i3elvis_amd/glue/i3g_glue.c): the
+reference's own global state, set up through the reference harness built
+with the glue (oracle/_ref/libi3ref_glue.so), is advanced by
+i3g_vpiterate / i3g_titerate / i3g_ronurecalc / i3g_movemark, and the
+globals are compared with the reference's outputs (golden fixtures) and the
+oracle.  This is the drop-in check: same globals in, same globals out."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden
+from oracle.refh import GLUE_PATH, Ref, available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not available(GLUE_PATH), reason="glue harness not built (needs the reference sources)")]
+
+_R = None
+
+
+def ref():
+    global _R
+    if _R is None:
+        _R = Ref(GLUE_PATH)
+        _R.lib.refh_set_threads(1)
+    return _R
+
+
+def test_glue_vpiterate_matches_reference_converged(gpu):
+    import i3elvis_amd as I
+    n = 17
+    print("case: glue vpiterate", n, flush=True)
+    g = golden(f"g3_converged_c0_{n}.npz")
+    cfg = I.make_config(0, n, n, n, seed=int(g["seed"]))
+    R = ref()
+    N = n ** 3
+    R.setup_grid(cfg["gx"], cfg["gy"], cfg["gz"], cfg["multinum"])
+    R.set_bc(cfg["bondm"], cfg["bondv"], cfg["bondn"])
+    R.set_physics(g=cfg["g"], nubeg=cfg["nubeg"], nuend=cfg["nuend"])
+    R.set("bondnum", len(cfg["bondn"]) - 1, C.c_long)
+    for k in ("nu", "ro", "nxx", "nxy", "nxz", "nyz"):
+        R.f32(k)[:N] = cfg[k]
+    for k in ("vx", "vy", "vz"):
+        R.f32(k)[:N] = 0
+    R.f64("pr")[:N] = 0
+    for k, v in dict(pxinit=2, pyinit=1, pzinit=2).items():
+        R.set(k, v, C.c_long)
+    R.set("pinit", 0.0)
+    R.set("timestep", 1e30)
+    R.set("maxxystep", 500.0)
+    R.lib.i3g_vpiterate(C.c_int(0), C.c_int(0))
+    sol = g["sol"].reshape(N, 4)
+    ex = g["exists"].reshape(N, 4)
+    got = np.stack([R.f64("pr")[:N], R.f32("vx")[:N], R.f32("vy")[:N], R.f32("vz")[:N]], 1)
+    for c in range(4):
+        e = ex[:, c]
+        rel = np.linalg.norm(got[e, c] - sol[e, c]) / np.linalg.norm(sol[e, c])
+        print("component", c, "rel L2", rel, flush=True)
+        assert rel < 1e-6, (c, rel)
+    vmax = max(np.abs(R.f32(k)[:N]).max() for k in ("vx", "vy", "vz"))
+    assert R.get("timestep") <= 500.0 / vmax * (1 + 1e-6)
+    # the stress fields were filled by the reference's own routines
+    out = (C.c_double * 3)()
+    R.lib.refh_stress(0, 5, 6, 7, out)
+    assert np.isclose(R.f32("sxx", N)[7 * n * n + 5 * n + 6], out[0], rtol=1e-6)
+
+
+def test_glue_titerate_matches_reference(gpu):
+    g = golden("g6_heat.npz")
+    n = int(g["n"])
+    N = n ** 3
+    print("case: glue titerate", n, flush=True)
+    R = ref()
+    R.setup_grid(g["gx"], g["gy"], g["gz"], 0)
+    R.set_bc(g["bondm"], g["bondv"], g["bondn"])
+    R.set("bondnum", len(g["bondn"]) - 1, C.c_long)
+    for k, v in dict(HEATMIN=1e-8, t0koef=1.0, heatdif=float(g["heatdif"]), timestep=float(g["dt"]),
+                     maxtkstep=float(g["maxtkstep"]), timesum=0.0, timebeg=1e30, timeend=2e30, tempend=0.0,
+                     zdeep=float(g["zdeep"]), tdeep=float(g["tdeep"])).items():
+        R.set(k, v)
+    R.i32("multittt", 100)[0] = 10
+    m = len(g["x"])
+    R.set("marknum", m, C.c_long)
+    R.arr("markx", C.c_float, R.MAXMRK)[:m] = g["x"]
+    R.arr("marky", C.c_float, R.MAXMRK)[:m] = g["y"]
+    R.arr("markz", C.c_float, R.MAXMRK)[:m] = g["z"]
+    R.arr("markt", C.c_byte, R.MAXMRK)[:m] = g["mtype"]
+    R.arr("markk", C.c_float, R.MAXMRK)[:m] = g["markk"]
+    R.arr("markv", C.c_float, R.MAXMRK)[:m] = 0
+    for k in ("tk", "ro", "cp", "kt", "ht"):
+        R.f32(k)[:N] = g[k]
+    R.lib.i3g_titerate(C.c_int(0))
+    tk = R.f32("tk")[:N].copy()
+    rel = np.linalg.norm(tk.astype(np.float64) - g["tk_out"]) / np.linalg.norm(g["tk_out"])
+    print("tk rel L2", rel, flush=True)
+    assert rel < 1e-7
+    mk = R.arr("markk", C.c_float, R.MAXMRK)[:m]
+    assert np.allclose(mk, g["markk_out"], rtol=1e-6, atol=0)
+
+
+def test_glue_ronurecalc_matches_reference(gpu):
+    g = golden("g5_markers.npz")
+    nx, ny, nz = len(g["gx"]), len(g["gy"]), len(g["gz"])
+    N = nx * ny * nz
+    print("case: glue ronurecalc", N, flush=True)
+    R = ref()
+    R.setup_grid(g["gx"], g["gy"], g["gz"], 0)
+    R.set_bc(g["bondm"], g["bondv"], g["bondn"])
+    R.set("bondnum", len(g["bondn"]) - 1, C.c_long)
+    m = len(g["x"])
+    R.set("marknum", m, C.c_long)
+    for name, a in (("markx", g["x"]), ("marky", g["y"]), ("markz", g["z"]), ("markk", g["tk"]), ("markv", g["eta"])):
+        R.arr(name, C.c_float, R.MAXMRK)[:m] = a
+    R.arr("markt", C.c_byte, R.MAXMRK)[:m] = g["mtype"]
+    names = ("marknu", "markdh", "markdv", "markn0", "markn1", "markro", "markbb", "markaa", "markcp", "markkt",
+             "markkf", "markkp", "markht")
+    for q, name in enumerate(names):
+        a = R.arr(name, C.c_double, R.MAXTMR)
+        a[:] = 0
+        a[:len(g["rocks"])] = g["rocks"][:, q]
+    R.set("rocknum", len(g["rocks"]), C.c_int)
+    R.set("densimod", 1, C.c_int)
+    R.set("viscmod", 0, C.c_int)
+    for k, v in dict(vdeep=float(g["vdeep"]), nudeep=float(g["nudeep"]), nukoef=0.0, nubeg=float(g["nubeg"]),
+                     nuend=float(g["nuend"]), timestep=float(g["timestep"])).items():
+        R.set(k, v)
+    R.f64("pr")[:N] = g["pr"]
+    gn = ["nu", "ro", "cp", "kt", "ht", "et", "tk", "nxx", "nxy", "nxz", "nyz"]
+    for q, k in enumerate(gn):
+        R.f32(k)[:N] = g["init"][q]
+    R.lib.i3g_ronurecalc()
+    for q, k in enumerate(gn):
+        a = R.f32(k)[:N].astype(np.float64)
+        b = g["out"][q].astype(np.float64)
+        rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+        assert rel.max() <= 2e-7, (k, rel.max())
+    assert np.allclose(R.arr("markv", C.c_float, R.MAXMRK)[:m], g["eta_out"], rtol=2e-7, atol=0)
+
+
+def test_glue_movemark_matches_oracle(gpu):
+    g = golden("g5_markers.npz")
+    nx, ny, nz = len(g["gx"]), len(g["gy"]), len(g["gz"])
+    N = nx * ny * nz
+    R = ref()
+    R.setup_grid(g["gx"], g["gy"], g["gz"], 0)
+    m = len(g["x"])
+    R.set("marknum", m, C.c_long)
+    for name, a in (("markx", g["x"]), ("marky", g["y"]), ("markz", g["z"]), ("markk", g["tk"]), ("markv", g["eta"])):
+        R.arr(name, C.c_float, R.MAXMRK)[:m] = a
+    R.arr("markt", C.c_byte, R.MAXMRK)[:m] = g["mtype"]
+    R.f32("vx")[:N] = g["vx"]; R.f32("vy")[:N] = g["vy"]; R.f32("vz")[:N] = g["vz"]
+    dt = 2e12
+    R.set("timestep", dt)
+    R.lib.i3g_movemark()
+    ox, oy, oz, _, _ = oracle.rk4(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"], g["vx"], g["vy"], g["vz"],
+                                  g["gx"], g["gy"], g["gz"], dt)
+    for name, b in (("markx", ox), ("marky", oy), ("markz", oz)):
+        a = R.arr(name, C.c_float, R.MAXMRK)[:m]
+        ulp = np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+        assert ulp.max() <= 1, (name, ulp.max())
