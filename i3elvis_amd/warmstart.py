@@ -74,9 +74,11 @@ class WarmStart:
         np.savez_compressed(path, params=self.net.get_params(), n=self.n, vscale=self.vscale, pscale=self.pscale)
 
     @classmethod
-    def load(cls, path):
+    def load(cls, path, n=None):
+        """weights do not depend on the grid size: n (8k+1) may differ from
+        the training grid"""
         d = np.load(path)
-        w = cls(int(d["n"]))
+        w = cls(int(d["n"]) if n is None else n)
         w.net.set_params(d["params"])
         w.vscale, w.pscale = float(d["vscale"]), float(d["pscale"])
         return w
