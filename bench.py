@@ -180,15 +180,16 @@ def main():
     learned = {}
 
     def run_learned(timing=False):
-        """network guess (device inference; its wall time, PCIe copies
-        included, is added to the step), fallback selection, mixed solve"""
+        """residual-optimal learned guess (device inference plus host-side
+        combination; its wall time, PCIe copies included, is added to the
+        step), fallback selection, mixed solve"""
         t = time.time()
-        g = ws.guess(cfg)
+        g, info = W.learned_guess(S, cfg, ws)
         tg = time.time() - t
         S.set_timing(timing)
         st, which, (rg, rf) = W.solve_with_guess(S, cfg, g, check_conv=True, precision=32, rtol=1e-8,
                                                  krylov=a.krylov, max_outer=20, verbose=a.verbose)
-        learned.update(start=which, rel0_net=rg, rel0_zero=rf, infer_wall_s=round(tg, 4))
+        learned.update(info, start=which, rel0_net=rg, rel0_zero=rf, guess_wall_s=round(tg, 4))
         return st, tg
 
     if a.profile_vcycles:

@@ -270,6 +270,8 @@ struct i3g_ctx {
 	/* XCD-contiguous plane slabs for the level-0 smoother (I3G_XCD_REMAP=0
 	 * switches it off) */
 	int xcd_remap = 1;
+	/* tiled z-marching level-0 smoother (I3G_SMOOTH_TILE=XR,zlen; 0 off) */
+	int smooth_tile = 0, tile_xr = 8, tile_zlen = 16;
 };
 
 /* ---- event timing of named kernels ------------------------------------ */
@@ -646,6 +648,13 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 	{
 		const char *e = getenv("I3G_XCD_REMAP");
 		if (e && e[0] == '0') c->xcd_remap = 0;
+		e = getenv("I3G_SMOOTH_TILE");
+		int xr = 0, zl = 0;
+		if (e && sscanf(e, "%d,%d", &xr, &zl) == 2 && xr >= 1 && xr <= 64 && zl >= 1 && zl <= 512) {
+			c->smooth_tile = 1;
+			c->tile_xr = xr;
+			c->tile_zlen = zl;
+		}
 	}
 	if (hipStreamCreate(&c->s) != hipSuccess) { delete c; return I3G_EDEVICE; }
 	c->L.resize(c->nlev);
@@ -805,12 +814,21 @@ static void sweep(i3g_ctx *c, int l, T *x, const T *b, const i3g_stokes_params *
 	dim3 hg = half_grid(L.nx, L.ny, L.nz);
 	int chunks = (int)hg.x, remap = c->xcd_remap && l == 0;
 	if (remap) hg = dim3((unsigned)((((long)chunks * L.nz + 7) / 8) * 8));
+	/* tiled z-marching variant on level 0 */
+	int tile = c->smooth_tile && l == 0;
+	int XR = c->tile_xr, zlen = c->tile_zlen;
+	int xtiles = (L.nx + XR - 1) / XR, zsegs = (L.nz + zlen - 1) / zlen;
+	dim3 tg((unsigned)((((long)xtiles * zsegs + 7) / 8) * 8));
 	double by = (double)L.N * (sizeof(T) * 2 + 1 + sizeof(T) * 4 + sizeof(T) * 3) / 2;
 	for (int comp = 1; comp <= 3; comp++)
 		for (int col = 0; col < 2; col++) {
 			Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "smooth_vel_f64_L0" : "smooth_vel_f32_L0") : "smooth_vel_coarse", by);
-			hipLaunchKernelGGL(k_smooth_vel<T>, hg, dim3(256), 0, c->s, v, x, b, comp, col, (T)p->v0koef, remap,
-					   chunks);
+			if (tile)
+				hipLaunchKernelGGL(k_smooth_vel_tile<T>, tg, dim3(256), 0, c->s, v, x, b, comp, col,
+						   (T)p->v0koef, xtiles, zsegs, XR, zlen);
+			else
+				hipLaunchKernelGGL(k_smooth_vel<T>, hg, dim3(256), 0, c->s, v, x, b, comp, col,
+						   (T)p->v0koef, remap, chunks);
 		}
 	{
 		Timed tm(c, l == 0 ? "smooth_p_L0" : "smooth_p_coarse", 0);

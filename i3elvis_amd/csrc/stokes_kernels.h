@@ -261,6 +261,45 @@ k_smooth_vel(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int com
 	x[u] += v0koef * res / d;
 }
 
+/* The same relaxation with each work-group marching through a tile of
+ * XR x-rows (all y) over zlen consecutive planes: the planes k-1, k+1 and
+ * the neighbouring x-rows a tile reads were mostly just read by the same
+ * group, so they come from its CU's caches instead of HBM.  Groups are
+ * mapped XCD-contiguously (block_work remap 1) over (x-tiles, z-segments),
+ * so neighbouring tiles share an L2.  Updates of one colour are
+ * independent, so the result equals the plane-parallel launch bit for
+ * bit. */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_smooth_vel_tile(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int comp,
+		  int colour, T v0koef, int xtiles, int zsegs, int XR, int zlen)
+{
+	int xt, zs;
+	if (!block_work(1, xtiles, zsegs, &xt, &zs)) return;
+	int hy = (L.ny + 1) >> 1;
+	int i0 = xt * XR;
+	int k0 = zs * zlen, k1 = min(L.nz, k0 + zlen);
+	int npos = XR * hy;
+	long N = L.N;
+	for (int k = k0; k < k1; k++) {
+		for (int p = threadIdx.x; p < npos; p += blockDim.x) {
+			int di = p / hy, tj = p - di * hy;
+			int i = i0 + di;
+			if (i >= L.nx) break;
+			int j = 2 * tj + ((i + k + colour) & 1);
+			if (j >= L.ny) continue;
+			long n = (long)k * L.nx * L.ny + (long)i * L.ny + j;
+			unsigned fl = L.flags[n];
+			if (!(fl & FL_EX(comp)) || (fl & FL_BC(comp))) continue;
+			T d;
+			T ax = row_ax<T, T>(L, x, x + N, x + 2 * N, x + 3 * N, comp, i, j, k, &d);
+			long u = comp * N + n;
+			T res = (b ? b[u] : (T)0) - ax;
+			x[u] += v0koef * res / d;
+		}
+	}
+}
+
 /* Pressure relaxation P -= p0koef * eta_cell * (div v - b_P)
  * (move3mg.c:3105-3111); BC rows are left to k_bc. */
 template <typename T>

@@ -53,7 +53,7 @@ def main():
     ws = W.WarmStart(n, seed=1)
     ws.vscale, ws.pscale = vscale, pscale
     t1 = time.time()
-    losses = ws.train(data, a.epochs, batch=a.batch, lr=a.lr, log=log)
+    losses = ws.train(data, a.epochs, batch=a.batch, lr=a.lr, log=log, checkpoint=a.out or None)
     tt = time.time() - t1
     if a.out:
         ws.save(a.out)
@@ -67,25 +67,35 @@ def main():
         S.set_solution(z, z, z, z)
         st0 = S.solve(precision=32, rtol=1e-8)
         ref = S.get_solution()
-        g = ws.guess(cfg)
-        gerr = [float(np.linalg.norm(a1 - b1) / max(np.linalg.norm(b1), 1e-300)) for a1, b1 in zip(g, ref)]
+        graw = ws.guess(cfg)
+        gerr = [float(np.linalg.norm(a1 - b1) / max(np.linalg.norm(b1), 1e-300)) for a1, b1 in zip(graw, ref)]
+        t = time.time()
+        g, info = W.learned_guess(S, cfg, ws)
+        tg = time.time() - t
         st1, which, (rg, rf) = W.solve_with_guess(S, cfg, g, precision=32, rtol=1e-8)
+        # the lithostatic pressure alone, for comparison
+        plith = W.lithostatic(cfg)
+        S.set_solution(plith, z, z, z)
+        st2 = S.solve(precision=32, rtol=1e-8)
         sol1 = S.get_solution()
         err = max(np.linalg.norm(a1 - b1) / max(np.linalg.norm(b1), 1e-300) for a1, b1 in zip(sol1[1:], ref[1:]))
         rows.append(dict(seed=s, zero_vcycles=st0.vcycles, zero_s=round(st0.seconds, 3), net_vcycles=st1.vcycles,
-                         net_s=round(st1.seconds, 3), start=which, rel0_net=rg, rel0_zero=rf,
-                         guess_rel_err_P_vx_vy_vz=gerr, vel_rel_diff=float(err)))
-        log(f"eval seed {s}: zero {st0.vcycles} V-cycles {st0.seconds:.2f}s | net ({which}, r0 {rg:.2e} vs "
-            f"{rf:.2e}, guess error P/vx/vy/vz {' '.join(f'{e:.2f}' for e in gerr)}) {st1.vcycles} V-cycles "
-            f"{st1.seconds:.2f}s")
+                         net_s=round(st1.seconds, 3), guess_s=round(tg, 3), start=which, rel0_net=rg, rel0_zero=rf,
+                         lith_vcycles=st2.vcycles, lith_s=round(st2.seconds, 3), guess=info,
+                         raw_guess_rel_err_P_vx_vy_vz=gerr, vel_rel_diff=float(err)))
+        log(f"eval seed {s}: zero {st0.vcycles} V-cycles {st0.seconds:.2f}s | lithostatic {st2.vcycles} V-cycles "
+            f"(r0 {info['rel_lithostatic']:.2e}) | learned ({which}, r0 {rg:.2e}, coef P {info['coef_dP']:.2f} "
+            f"v {info['coef_v']:.2f}; raw error P/vx/vy/vz {' '.join(f'{e:.2f}' for e in gerr)}) {st1.vcycles} "
+            f"V-cycles {st1.seconds:.2f}s + guess {tg:.2f}s")
         S.close()
     zv = np.mean([r["zero_vcycles"] for r in rows])
     nv = np.mean([r["net_vcycles"] for r in rows])
+    lv = np.mean([r["lith_vcycles"] for r in rows])
     zs = np.mean([r["zero_s"] for r in rows])
     ns = np.mean([r["net_s"] for r in rows])
     print(json.dumps(dict(n=n, train=a.train, epochs=a.epochs, label_s=round(tl, 1), train_s=round(tt, 1),
                           loss_first=losses[0], loss_last=losses[-1], vscale=vscale, pscale=pscale,
-                          eval=rows, mean_zero_vcycles=zv, mean_net_vcycles=nv, mean_zero_s=zs, mean_net_s=ns,
+                          eval=rows, mean_zero_vcycles=zv, mean_lith_vcycles=lv, mean_net_vcycles=nv, mean_zero_s=zs, mean_net_s=ns,
                           vcycle_gain=zv / nv if nv else None, time_gain=zs / ns if ns else None)))
 
 

@@ -27,6 +27,25 @@ def features(cfg):
     return np.stack([lg, dr], -1).astype(np.float32)
 
 
+def lithostatic(cfg, gauge=(2, 1, 2)):
+    """P = integral of rho g_y dy down each (x, z) column at the pressure
+    points (cell centres, m1, m2, m3 >= 1: the cell's four corner densities
+    averaged), shifted to zero at the gauge cell"""
+    nx, ny, nz = cfg["nx"], cfg["ny"], cfg["nz"]
+    ro = cfg["ro"].astype(np.float64).reshape(nz, nx, ny)
+    gy = cfg["gy"]
+    # cell-centre density in the (x, z) plane at every node row m2
+    rc = np.zeros_like(ro)
+    rc[1:, 1:, :] = 0.25 * (ro[1:, 1:, :] + ro[:-1, 1:, :] + ro[1:, :-1, :] + ro[:-1, :-1, :])
+    yc = np.r_[0.0, 0.5 * (gy[1:] + gy[:-1])]
+    P = np.zeros_like(ro)
+    # between pressure rows m2-1 and m2 (m2 >= 2) the node row m2-1 lies
+    for m2 in range(2, ny):
+        P[:, :, m2] = P[:, :, m2 - 1] + cfg["g"][1] * rc[:, :, m2 - 1] * (yc[m2] - yc[m2 - 1])
+    P -= P[gauge[2], gauge[0], gauge[1]]
+    return P.reshape(-1)
+
+
 def hydrostatic(cfg, gauge=(2, 1, 2)):
     """rho0 g_y (y - y_gauge) at every node (y is the vertical axis)"""
     ny, nx, nz = cfg["ny"], cfg["nx"], cfg["nz"]
@@ -87,8 +106,9 @@ class WarmStart:
         y = self.net.infer(features(cfg))
         return solution_from(cfg, y, self.vscale, self.pscale)
 
-    def train(self, data, epochs, batch=4, lr=1e-3, log=None, seed=0):
-        """data: list of (features [N,2], targets [N,4]); returns losses"""
+    def train(self, data, epochs, batch=4, lr=1e-3, log=None, seed=0, checkpoint=None):
+        """data: list of (features [N,2], targets [N,4]); returns losses.
+        checkpoint: path written after every epoch"""
         rng = np.random.default_rng(seed)
         losses = []
         for ep in range(epochs):
@@ -100,9 +120,66 @@ class WarmStart:
                 ts = np.stack([data[i][1] for i in idx])
                 tot += self.net.train_step(xs, ts, lr=lr) * len(idx)
             losses.append(tot / len(data))
+            if checkpoint:
+                self.save(checkpoint)
             if log:
                 log(f"epoch {ep + 1}/{epochs} loss {losses[-1]:.4e}")
         return losses
+
+
+def best_combination(S, base, dirs):
+    """guess = base + sum_i a_i dirs[i] with the a_i minimising the FP64
+    weighted residual norm; the squared norm is a quadratic in a, fixed by
+    its values at 1 + 2k + k(k-1)/2 points (k = len(dirs)).  Returns
+    (guess, a, rel)."""
+    k = len(dirs)
+
+    def comb(a):
+        return [b + sum(ai * d[c] for ai, d in zip(a, dirs)) for c, b in enumerate(base)]
+
+    def f(a):
+        nr = S.residual(*comb(a))[1]
+        return nr[0] ** 2, nr[1]
+
+    f0, bn = f([0.0] * k)
+    g = np.zeros(k)
+    H = np.zeros((k, k))
+    e = np.eye(k)
+    fp = [f(list(e[i]))[0] for i in range(k)]
+    fm = [f(list(-e[i]))[0] for i in range(k)]
+    for i in range(k):
+        g[i] = (fp[i] - fm[i]) / 2
+        H[i, i] = fp[i] + fm[i] - 2 * f0
+    for i in range(k):
+        for j in range(i + 1, k):
+            fij = f(list(e[i] + e[j]))[0]
+            H[i, j] = H[j, i] = fij - f0 - g[i] - g[j] - (H[i, i] + H[j, j]) / 2
+    # f(a) = f0 + g.a + a.H.a / 2
+    try:
+        a = -np.linalg.solve(H, g)
+    except np.linalg.LinAlgError:
+        a = np.zeros(k)
+    guess = comb(list(a))
+    nr = S.residual(*guess)[1]
+    if not nr[0] ** 2 < f0:
+        a, guess, nr = np.zeros(k), comb([0.0] * k), (np.sqrt(f0), bn)
+    return guess, a, nr[0] / nr[1]
+
+
+def learned_guess(S, cfg, ws):
+    """The network's guess made residual-optimal: lithostatic pressure plus
+    the network's dynamic pressure and its velocities, each scaled by the
+    coefficient that minimises the FP64 residual.  Returns (guess, info)."""
+    n = cfg["nx"] * cfg["ny"] * cfg["nz"]
+    z = np.zeros(n)
+    plith = lithostatic(cfg)
+    y = ws.net.infer(features(cfg)).astype(np.float64)
+    dP = [y[:, 3] * ws.pscale, z, z, z]
+    dv = [z, y[:, 0] * ws.vscale, y[:, 1] * ws.vscale, y[:, 2] * ws.vscale]
+    base = [plith, z, z, z]
+    rl = S.residual(*base)[1]
+    guess, a, rel = best_combination(S, base, [dP, dv])
+    return guess, dict(rel_lithostatic=rl[0] / rl[1], coef_dP=float(a[0]), coef_v=float(a[1]), rel_learned=rel)
 
 
 def initial_rel(S, sol):

@@ -1,0 +1,47 @@
+"""GPU test of the learned warm start: the committed network (trained on
+the device by the project's own kernels, python -m
+i3elvis_amd.train_warmstart) against the solve from a zero guess on
+held-out configuration-1 seeds at 129^3.  The guess is the residual-optimal
+combination of the lithostatic pressure and the network's outputs, with the
+zero guess as fallback, so it can never start worse; the test reports the
+V-cycles saved."""
+import os
+
+import numpy as np
+import pytest
+
+import i3elvis_amd as I
+from i3elvis_amd import warmstart as W
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(I.__file__), "data", "warmstart_c1_129.npz")
+
+
+@pytest.mark.skipif(not os.path.isfile(WEIGHTS), reason="no trained weights")
+def test_learned_guess_never_worse_and_gain(gpu):
+    n = 129
+    ws = W.WarmStart.load(WEIGHTS)
+    zs, ls = [], []
+    for seed in (2100, 2101):
+        print("case: warm start seed", seed, flush=True)
+        cfg = I.make_config(1, n, n, n, seed=seed)
+        S = I.Stokes(cfg)
+        S.set_coefficients()
+        z = np.zeros(n ** 3)
+        S.set_solution(z, z, z, z)
+        st0 = S.solve(precision=32, rtol=1e-8)
+        ref = S.get_solution()
+        g, info = W.learned_guess(S, cfg, ws)
+        st1, which, (rg, rf) = W.solve_with_guess(S, cfg, g, precision=32, rtol=1e-8)
+        sol = S.get_solution()
+        print(f"seed {seed}: zero {st0.vcycles} V-cycles, learned start ({which}, r0 {rg:.2e} vs {rf:.2e}, "
+              f"lithostatic r0 {info['rel_lithostatic']:.2e}, coef P {info['coef_dP']:.3f} v {info['coef_v']:.3f}) "
+              f"{st1.vcycles} V-cycles", flush=True)
+        assert st1.converged and rg <= rf
+        for a, b in zip(sol[1:], ref[1:]):
+            assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
+        zs.append(st0.vcycles)
+        ls.append(st1.vcycles)
+        S.close()
+    assert sum(ls) <= sum(zs)
