@@ -270,8 +270,9 @@ struct i3g_ctx {
 	/* XCD-contiguous plane slabs for the level-0 smoother (I3G_XCD_REMAP=0
 	 * switches it off) */
 	int xcd_remap = 1;
-	/* tiled z-marching level-0 smoother (I3G_SMOOTH_TILE=XR,zlen; 0 off) */
-	int smooth_tile = 0, tile_xr = 8, tile_zlen = 16;
+	/* velocity smoother: 0 component-by-colour launches, 1 fused red/black
+	 * phases (I3G_SMOOTHER selects) */
+	int smoother = 0;
 };
 
 /* ---- event timing of named kernels ------------------------------------ */
@@ -648,13 +649,8 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 	{
 		const char *e = getenv("I3G_XCD_REMAP");
 		if (e && e[0] == '0') c->xcd_remap = 0;
-		e = getenv("I3G_SMOOTH_TILE");
-		int xr = 0, zl = 0;
-		if (e && sscanf(e, "%d,%d", &xr, &zl) == 2 && xr >= 1 && xr <= 64 && zl >= 1 && zl <= 512) {
-			c->smooth_tile = 1;
-			c->tile_xr = xr;
-			c->tile_zlen = zl;
-		}
+		e = getenv("I3G_SMOOTHER");
+		if (e && (e[0] == '0' || e[0] == '1')) c->smoother = e[0] - '0';
 	}
 	if (hipStreamCreate(&c->s) != hipSuccess) { delete c; return I3G_EDEVICE; }
 	c->L.resize(c->nlev);
@@ -814,22 +810,31 @@ static void sweep(i3g_ctx *c, int l, T *x, const T *b, const i3g_stokes_params *
 	dim3 hg = half_grid(L.nx, L.ny, L.nz);
 	int chunks = (int)hg.x, remap = c->xcd_remap && l == 0;
 	if (remap) hg = dim3((unsigned)((((long)chunks * L.nz + 7) / 8) * 8));
-	/* tiled z-marching variant on level 0 */
-	int tile = c->smooth_tile && l == 0;
-	int XR = c->tile_xr, zlen = c->tile_zlen;
-	int xtiles = (L.nx + XR - 1) / XR, zsegs = (L.nz + zlen - 1) / zlen;
-	dim3 tg((unsigned)((((long)xtiles * zsegs + 7) / 8) * 8));
+	if (c->smoother == 1) {
+		/* fused red/black phases through the level's residual buffer (free
+		 * during smoothing) */
+		T *s3 = lr<T>(L) + L.N;
+		double by3 = (double)L.N * (14 * sizeof(T) + 1);
+		dim3 pg = plane_grid(L.nx, L.ny, L.nz);
+		{
+			Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "smooth_uvw_f64_L0" : "smooth_uvw_f32_L0") : "smooth_uvw_coarse", by3);
+			hipLaunchKernelGGL(k_smooth_uvw<T>, pg, dim3(256), 0, c->s, v, (const T *)x, (const T *)(x + L.N), s3,
+					   b, 0, (T)p->v0koef);
+		}
+		{
+			Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "smooth_uvw_f64_L0" : "smooth_uvw_f32_L0") : "smooth_uvw_coarse", by3);
+			hipLaunchKernelGGL(k_smooth_uvw<T>, pg, dim3(256), 0, c->s, v, (const T *)x, (const T *)s3, x + L.N,
+					   b, 1, (T)p->v0koef);
+		}
+	} else {
 	double by = (double)L.N * (sizeof(T) * 2 + 1 + sizeof(T) * 4 + sizeof(T) * 3) / 2;
 	for (int comp = 1; comp <= 3; comp++)
 		for (int col = 0; col < 2; col++) {
 			Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "smooth_vel_f64_L0" : "smooth_vel_f32_L0") : "smooth_vel_coarse", by);
-			if (tile)
-				hipLaunchKernelGGL(k_smooth_vel_tile<T>, tg, dim3(256), 0, c->s, v, x, b, comp, col,
-						   (T)p->v0koef, xtiles, zsegs, XR, zlen);
-			else
-				hipLaunchKernelGGL(k_smooth_vel<T>, hg, dim3(256), 0, c->s, v, x, b, comp, col,
-						   (T)p->v0koef, remap, chunks);
+			hipLaunchKernelGGL(k_smooth_vel<T>, hg, dim3(256), 0, c->s, v, x, b, comp, col, (T)p->v0koef, remap,
+					   chunks);
 		}
+	}
 	{
 		Timed tm(c, l == 0 ? "smooth_p_L0" : "smooth_p_coarse", 0);
 		hipLaunchKernelGGL(k_smooth_p<T>, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, v, x, b,

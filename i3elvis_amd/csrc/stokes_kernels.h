@@ -261,42 +261,39 @@ k_smooth_vel(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int com
 	x[u] += v0koef * res / d;
 }
 
-/* The same relaxation with each work-group marching through a tile of
- * XR x-rows (all y) over zlen consecutive planes: the planes k-1, k+1 and
- * the neighbouring x-rows a tile reads were mostly just read by the same
- * group, so they come from its CU's caches instead of HBM.  Groups are
- * mapped XCD-contiguously (block_work remap 1) over (x-tiles, z-segments),
- * so neighbouring tiles share an L2.  Updates of one colour are
- * independent, so the result equals the plane-parallel launch bit for
- * bit. */
+/* One colour phase of the fused velocity relaxation: at the nodes of one
+ * colour ((i+j+k) mod 2) the three velocity components are relaxed with the
+ * gausmat4 update (x += v0koef (b - A x)/diag) from the state vin (all
+ * reads come from vin, so same-colour couplings between components see
+ * the values at the start of the phase); the other colour's values are
+ * copied.  vout = state after the phase; P is read from P.  Two launches
+ * (red vin=x vout=s, black vin=s vout=x) replace the six component-colour
+ * launches of k_smooth_vel and read every plane twice instead of six
+ * times.  Deterministic: no value is read and written in the same launch. */
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_smooth_vel_tile(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int comp,
-		  int colour, T v0koef, int xtiles, int zsegs, int XR, int zlen)
+k_smooth_uvw(LevelView<T> L, const T *__restrict__ P, const T *__restrict__ vin, T *__restrict__ vout,
+	     const T *__restrict__ b, int colour, T v0koef)
 {
-	int xt, zs;
-	if (!block_work(1, xtiles, zsegs, &xt, &zs)) return;
-	int hy = (L.ny + 1) >> 1;
-	int i0 = xt * XR;
-	int k0 = zs * zlen, k1 = min(L.nz, k0 + zlen);
-	int npos = XR * hy;
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int k = blockIdx.y;
+	int i = t / L.ny, j = t - i * L.ny;
+	if (i >= L.nx) return;
 	long N = L.N;
-	for (int k = k0; k < k1; k++) {
-		for (int p = threadIdx.x; p < npos; p += blockDim.x) {
-			int di = p / hy, tj = p - di * hy;
-			int i = i0 + di;
-			if (i >= L.nx) break;
-			int j = 2 * tj + ((i + k + colour) & 1);
-			if (j >= L.ny) continue;
-			long n = (long)k * L.nx * L.ny + (long)i * L.ny + j;
-			unsigned fl = L.flags[n];
-			if (!(fl & FL_EX(comp)) || (fl & FL_BC(comp))) continue;
+	long n = (long)k * L.nx * L.ny + (long)i * L.ny + j;
+	bool mine = ((i + j + k) & 1) == colour;
+	unsigned fl = L.flags[n];
+	const T *vx = vin, *vy = vin + N, *vz = vin + 2 * N;
+	for (int comp = 1; comp <= 3; comp++) {
+		long u = (long)(comp - 1) * N + n;
+		T v = vin[u];
+		if (mine && (fl & FL_EX(comp)) && !(fl & FL_BC(comp))) {
 			T d;
-			T ax = row_ax<T, T>(L, x, x + N, x + 2 * N, x + 3 * N, comp, i, j, k, &d);
-			long u = comp * N + n;
-			T res = (b ? b[u] : (T)0) - ax;
-			x[u] += v0koef * res / d;
+			T ax = row_ax<T, T>(L, P, vx, vy, vz, comp, i, j, k, &d);
+			T res = (b ? b[(long)comp * N + n] : (T)0) - ax;
+			v += v0koef * res / d;
 		}
+		vout[u] = v;
 	}
 }
 
