@@ -143,7 +143,7 @@ __global__ void k_ronurestrict(int fnx, int fny, int cnx, int cny, int cnz,
 			       const float *__restrict__ nuf, float *__restrict__ nuc,
 			       const int *xs, const int *xi, const double *xw,
 			       const int *ys, const int *yi, const double *yw,
-			       const int *zs, const int *zi, const double *zw)
+			       const int *zs, const int *zi, const double *zw, int mode)
 {
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	int K = blockIdx.y;
@@ -151,16 +151,19 @@ __global__ void k_ronurestrict(int fnx, int fny, int cnx, int cny, int cnz,
 	if (I >= cnx) return;
 	long szf = (long)fnx * fny;
 	float acc = 0;
-	double sw = 0;
+	double sw = 0, dacc = 0;
 	for (int a = zs[K]; a < zs[K + 1]; a++)
 		for (int b = xs[I]; b < xs[I + 1]; b++)
 			for (int e = ys[J]; e < ys[J + 1]; e++) {
 				double swt = xw[b] * yw[e] * zw[a];
 				long nf = zi[a] * szf + (long)xi[b] * fny + yi[e];
-				acc = (float)((double)acc + (double)nuf[nf] * swt);
+				if (mode == 0) acc = (float)((double)acc + (double)nuf[nf] * swt);
+				else if (mode == 1) dacc += log((double)nuf[nf]) * swt;
+				else dacc += swt / (double)nuf[nf];
 				sw += swt;
 			}
-	nuc[(long)K * cnx * cny + (long)I * cny + J] = (float)((double)acc / sw);
+	double v = mode == 0 ? (double)acc / sw : mode == 1 ? exp(dacc / sw) : sw / dacc;
+	nuc[(long)K * cnx * cny + (long)I * cny + J] = (float)v;
 }
 
 /* ------------------------------------------------------------------ */
@@ -270,9 +273,10 @@ struct i3g_ctx {
 	/* XCD-contiguous plane slabs for the level-0 smoother (I3G_XCD_REMAP=0
 	 * switches it off) */
 	int xcd_remap = 1;
-	/* velocity smoother: 0 component-by-colour launches, 1 fused red/black
-	 * phases (I3G_SMOOTHER selects) */
-	int smoother = 0;
+	/* viscosity averaging of the coarse (V-cycle only) levels:
+	 * 0 arithmetic as the reference, 1 geometric, 2 harmonic
+	 * (I3G_COARSE_VISC selects) */
+	int coarse_visc = 0;
 };
 
 /* ---- event timing of named kernels ------------------------------------ */
@@ -649,8 +653,8 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 	{
 		const char *e = getenv("I3G_XCD_REMAP");
 		if (e && e[0] == '0') c->xcd_remap = 0;
-		e = getenv("I3G_SMOOTHER");
-		if (e && (e[0] == '0' || e[0] == '1')) c->smoother = e[0] - '0';
+		e = getenv("I3G_COARSE_VISC");
+		if (e && e[0] >= '0' && e[0] <= '2') c->coarse_visc = e[0] - '0';
 	}
 	if (hipStreamCreate(&c->s) != hipSuccess) { delete c; return I3G_EDEVICE; }
 	c->L.resize(c->nlev);
@@ -762,21 +766,24 @@ extern "C" int i3g_set_coefficients(i3g_ctx *c, const float *nu, const float *nx
 	HCHK(hipMemcpyAsync(c->nxz, nxz, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
 	HCHK(hipMemcpyAsync(c->nyz, nyz, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
 	HCHK(hipMemcpyAsync(c->ro, ro, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
-	/* coarse viscosities (viscmod 0 weights; log/harmonic means are
-	 * applied by transforming nu before and after, as the reference) */
+	/* The level-0 operator uses viscmod 0 (the reference's arithmetic
+	 * averages).  The coarse levels exist only for the V-cycle; their
+	 * viscosities are restricted and averaged by c->coarse_visc (0 the
+	 * reference's arithmetic ronurestrict, 1 geometric, 2 harmonic). */
+	if (viscmod != 0) return I3G_EINVAL; /* log/harmonic level-0 averages not built */
+	int cm = c->coarse_visc;
 	for (int l = 1; l < c->nlev; l++) {
 		Level &F = c->L[l - 1], &Cl = c->L[l];
 		const Xfer &X = c->X[l - 1];
-		if (viscmod != 0) return I3G_EINVAL; /* log/harmonic restriction not built */
 		hipLaunchKernelGGL(k_ronurestrict, plane_grid(Cl.nx, Cl.ny, Cl.nz), dim3(256), 0, c->s,
 				   F.nx, F.ny, Cl.nx, Cl.ny, Cl.nz, F.nu, Cl.nu, X.ax[0].istart[3],
 				   X.ax[0].iidx[3], X.ax[0].iw64[3], X.ax[1].istart[3], X.ax[1].iidx[3],
-				   X.ax[1].iw64[3], X.ax[2].istart[3], X.ax[2].iidx[3], X.ax[2].iw64[3]);
+				   X.ax[1].iw64[3], X.ax[2].istart[3], X.ax[2].iidx[3], X.ax[2].iw64[3], cm);
 	}
 	for (int l = 0; l < c->nlev; l++) {
 		Level &L = c->L[l];
 		hipLaunchKernelGGL(k_eta, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, L.nx, L.ny, L.nz,
-				   l == 0 ? 1 : 0, L.nu, c->nxx, c->nxy, c->nxz, c->nyz, viscmod, phys[3],
+				   l == 0 ? 1 : 0, L.nu, c->nxx, c->nxy, c->nxz, c->nyz, l == 0 ? viscmod : cm, phys[3],
 				   phys[4], L.eta64, L.eta32, L.wP64, L.wP32, L.g[0], L.g[1], L.g[2]);
 	}
 	Level &L0 = c->L[0];
@@ -810,23 +817,6 @@ static void sweep(i3g_ctx *c, int l, T *x, const T *b, const i3g_stokes_params *
 	dim3 hg = half_grid(L.nx, L.ny, L.nz);
 	int chunks = (int)hg.x, remap = c->xcd_remap && l == 0;
 	if (remap) hg = dim3((unsigned)((((long)chunks * L.nz + 7) / 8) * 8));
-	if (c->smoother == 1) {
-		/* fused red/black phases through the level's residual buffer (free
-		 * during smoothing) */
-		T *s3 = lr<T>(L) + L.N;
-		double by3 = (double)L.N * (14 * sizeof(T) + 1);
-		dim3 pg = plane_grid(L.nx, L.ny, L.nz);
-		{
-			Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "smooth_uvw_f64_L0" : "smooth_uvw_f32_L0") : "smooth_uvw_coarse", by3);
-			hipLaunchKernelGGL(k_smooth_uvw<T>, pg, dim3(256), 0, c->s, v, (const T *)x, (const T *)(x + L.N), s3,
-					   b, 0, (T)p->v0koef);
-		}
-		{
-			Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "smooth_uvw_f64_L0" : "smooth_uvw_f32_L0") : "smooth_uvw_coarse", by3);
-			hipLaunchKernelGGL(k_smooth_uvw<T>, pg, dim3(256), 0, c->s, v, (const T *)x, (const T *)s3, x + L.N,
-					   b, 1, (T)p->v0koef);
-		}
-	} else {
 	double by = (double)L.N * (sizeof(T) * 2 + 1 + sizeof(T) * 4 + sizeof(T) * 3) / 2;
 	for (int comp = 1; comp <= 3; comp++)
 		for (int col = 0; col < 2; col++) {
@@ -834,7 +824,6 @@ static void sweep(i3g_ctx *c, int l, T *x, const T *b, const i3g_stokes_params *
 			hipLaunchKernelGGL(k_smooth_vel<T>, hg, dim3(256), 0, c->s, v, x, b, comp, col, (T)p->v0koef, remap,
 					   chunks);
 		}
-	}
 	{
 		Timed tm(c, l == 0 ? "smooth_p_L0" : "smooth_p_coarse", 0);
 		hipLaunchKernelGGL(k_smooth_p<T>, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, v, x, b,

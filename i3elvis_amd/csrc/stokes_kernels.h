@@ -261,42 +261,6 @@ k_smooth_vel(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int com
 	x[u] += v0koef * res / d;
 }
 
-/* One colour phase of the fused velocity relaxation: at the nodes of one
- * colour ((i+j+k) mod 2) the three velocity components are relaxed with the
- * gausmat4 update (x += v0koef (b - A x)/diag) from the state vin (all
- * reads come from vin, so same-colour couplings between components see
- * the values at the start of the phase); the other colour's values are
- * copied.  vout = state after the phase; P is read from P.  Two launches
- * (red vin=x vout=s, black vin=s vout=x) replace the six component-colour
- * launches of k_smooth_vel and read every plane twice instead of six
- * times.  Deterministic: no value is read and written in the same launch. */
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_smooth_uvw(LevelView<T> L, const T *__restrict__ P, const T *__restrict__ vin, T *__restrict__ vout,
-	     const T *__restrict__ b, int colour, T v0koef)
-{
-	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	int k = blockIdx.y;
-	int i = t / L.ny, j = t - i * L.ny;
-	if (i >= L.nx) return;
-	long N = L.N;
-	long n = (long)k * L.nx * L.ny + (long)i * L.ny + j;
-	bool mine = ((i + j + k) & 1) == colour;
-	unsigned fl = L.flags[n];
-	const T *vx = vin, *vy = vin + N, *vz = vin + 2 * N;
-	for (int comp = 1; comp <= 3; comp++) {
-		long u = (long)(comp - 1) * N + n;
-		T v = vin[u];
-		if (mine && (fl & FL_EX(comp)) && !(fl & FL_BC(comp))) {
-			T d;
-			T ax = row_ax<T, T>(L, P, vx, vy, vz, comp, i, j, k, &d);
-			T res = (b ? b[(long)comp * N + n] : (T)0) - ax;
-			v += v0koef * res / d;
-		}
-		vout[u] = v;
-	}
-}
-
 /* Pressure relaxation P -= p0koef * eta_cell * (div v - b_P)
  * (move3mg.c:3105-3111); BC rows are left to k_bc. */
 template <typename T>
