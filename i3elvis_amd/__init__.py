@@ -207,6 +207,18 @@ class Stokes:
             check(rc, "stokes_solve")
         return st
 
+    def strain_stress(self):
+        """(eps, sig): 6 float planes each (xx yy zz xy xz yz) of the resident solution"""
+        out = [np.zeros(self.N, np.float32) for _ in range(12)]
+        check(lib().i3g_stokes_strain_stress(self.ctx, *[_p(a) for a in out]), "strain_stress")
+        return np.stack(out[:6]), np.stack(out[6:])
+
+    def native_norms(self, divvmax=0.0):
+        """the reference's stoksum, stoksum1, stoksum2, contsum, contsumb, bondsum, maxvxyz"""
+        out = np.zeros(7)
+        check(lib().i3g_stokes_native_norms(self.ctx, C.c_double(divvmax), _p(out)), "native_norms")
+        return dict(zip(("stoksum", "stoksum1", "stoksum2", "contsum", "contsumb", "bondsum", "maxvxyz"), out))
+
     def set_timing(self, on=True):
         check(lib().i3g_set_timing(self.ctx, C.c_int(1 if on else 0)))
 

@@ -277,6 +277,7 @@ struct i3g_ctx {
 	 * 0 arithmetic as the reference, 1 geometric, 2 harmonic
 	 * (I3G_COARSE_VISC selects) */
 	int coarse_visc = 0;
+	float *st12 = nullptr;   /* strain rates and stresses, 12 level-0 planes */
 };
 
 /* ---- event timing of named kernels ------------------------------------ */
@@ -742,7 +743,7 @@ extern "C" int i3g_destroy(i3g_ctx *c)
 				for (void *p : ps)
 					if (p) hipFree(p);
 			}
-	void *ps[] = {c->rhs0, c->sol, c->ro, c->nxx, c->nxy, c->nxz, c->nyz, c->partials, c->dscal, c->e64};
+	void *ps[] = {c->rhs0, c->sol, c->ro, c->nxx, c->nxy, c->nxz, c->nyz, c->partials, c->dscal, c->e64, c->st12};
 	for (void *p : ps)
 		if (p) hipFree(p);
 	if (c->hscal) hipHostFree(c->hscal);
@@ -1520,6 +1521,272 @@ extern "C" int i3g_stokes_solve(i3g_ctx *c, double *P, double *vx, double *vy, d
 	if (rs && rs != I3G_ENOCONV) return rs;
 	rc = i3g_stokes_get_solution(c, P, vx, vy, vz);
 	return rc ? rc : rs;
+}
+
+/* ------------------------------------------------------------------ */
+/* Strain rates, stresses and the reference's native norms            */
+/* ------------------------------------------------------------------ */
+
+/* Value mode of sxxcalc..syzcalc (move3mg.c:3-710) over level 0, as the
+ * loop at the end of vpiterate (:1691-1764): velocities rounded to float
+ * (the reference's vx/vy/vz), differences in float, eps = diff * 1/h in
+ * double, sigma = 2 eta eps, both stored as float.  st: 12 planes
+ * exx eyy ezz exy exz eyz sxx syy szz sxy sxz syz; entries a component
+ * does not have are left untouched. */
+__global__ void k_strain_stress(LevelView<double> L, const double *__restrict__ sol, float *st)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int m3 = blockIdx.y;
+	int m1 = t / L.ny, m2 = t - m1 * L.ny;
+	if (m1 >= L.nx) return;
+	long N = L.N, sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = m3 * sz + m1 * sx + m2;
+	const double *vx = sol + N, *vy = sol + 2 * N, *vz = sol + 3 * N;
+#define F(a) ((float)(a))
+	int nx = L.nx, ny = L.ny, nz = L.nz;
+	if (m1 > 0 && m2 > 0 && m3 > 0) {
+		long v0 = n - sz - sx - 1;
+		double e;
+		e = (double)(F(vx[v0 + sx]) - F(vx[v0])) * L.ihx[m1 - 1];
+		st[n] = (float)e; st[6 * N + n] = (float)(2.0 * L.eC[n] * e);
+		e = (double)(F(vy[v0 + 1]) - F(vy[v0])) * L.ihy[m2 - 1];
+		st[N + n] = (float)e; st[7 * N + n] = (float)(2.0 * L.eC[n] * e);
+		e = (double)(F(vz[v0 + sz]) - F(vz[v0])) * L.ihz[m3 - 1];
+		st[2 * N + n] = (float)e; st[8 * N + n] = (float)(2.0 * L.eC[n] * e);
+	}
+	/* icx[m] = 1/(g[m+1]-g[m-1]) */
+	if (m1 > 0 && m1 < nx - 1 && m2 > 0 && m2 < ny - 1 && m3 < nz - 1) {
+		double e = (double)(F(vx[n]) - F(vx[n - 1])) * L.icy[m2] + (double)(F(vy[n]) - F(vy[n - sx])) * L.icx[m1];
+		st[3 * N + n] = (float)e; st[9 * N + n] = (float)(2.0 * L.eXY[n] * e);
+	}
+	if (m1 > 0 && m1 < nx - 1 && m2 < ny - 1 && m3 > 0 && m3 < nz - 1) {
+		double e = (double)(F(vx[n]) - F(vx[n - sz])) * L.icz[m3] + (double)(F(vz[n]) - F(vz[n - sx])) * L.icx[m1];
+		st[4 * N + n] = (float)e; st[10 * N + n] = (float)(2.0 * L.eXZ[n] * e);
+	}
+	if (m1 < nx - 1 && m2 > 0 && m2 < ny - 1 && m3 > 0 && m3 < nz - 1) {
+		double e = (double)(F(vy[n]) - F(vy[n - sz])) * L.icz[m3] + (double)(F(vz[n]) - F(vz[n - 1])) * L.icy[m2];
+		st[5 * N + n] = (float)e; st[11 * N + n] = (float)(2.0 * L.eYZ[n] * e);
+	}
+#undef F
+}
+
+__device__ __forceinline__ double nueff48(const float *nu, long a0, long a1, long a2, long a3, long b0, long b1,
+					  long b2, long b3, long c0, long c1, long c2, long c3, double nubeg,
+					  double nuend)
+{
+	float s1 = nu[a0] + nu[a1] + nu[a2] + nu[a3];
+	float s2 = nu[b0] + nu[b1] + nu[b2] + nu[b3];
+	double e = ((double)s1 + 10.0 * (double)s2 + nu[c0] + nu[c1] + nu[c2] + nu[c3]) / 48.0;
+	return e > nuend ? nuend : (e < nubeg ? nubeg : e);
+}
+
+/* Per-block sums of the norm loop of vpiterate (move3mg.c:2405-2560) on
+ * level 0: momentum rows (x/y/zstokserr ynerr=1 from the stored float
+ * stresses, double pressure, 48-weight viscosity for eps1[0]), continuity
+ * rows (continerr ynerr=1) split by the P row's BC flag, velocity BC rows
+ * (bonderr ynerr=1), and the float velocity ranges.  part[blk*16 + q]:
+ * 0 sum ival^2, 1 sum eps^2, 2 sum eps^2 ival^2, 3 count, 4 cont^2,
+ * 5 count, 6 contb^2, 7 count, 8 bond^2, 9 count, 10..15 min/max vx vy vz. */
+#define NB_NORM 256
+__global__ void __launch_bounds__(NB_NORM)
+k_native_norms(LevelView<double> L, const double *__restrict__ sol, const float *__restrict__ st,
+	       const float *__restrict__ nu, const float *__restrict__ ro, double g0, double g1, double g2,
+	       double nubeg, double nuend, const int *__restrict__ bcid, BcRec<double> bc, double *part)
+{
+	double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+	double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int m3 = blockIdx.y;
+	int m1 = t / L.ny, m2 = t - m1 * L.ny;
+	long N = L.N, sx = L.ny, sz = (long)L.nx * L.ny;
+	int nx = L.nx, ny = L.ny, nz = L.nz;
+	if (m1 < nx) {
+		long n = m3 * sz + m1 * sx + m2;
+		const double *P = sol;
+		const float *sxx = st + 6 * N, *syy = st + 7 * N, *szz = st + 8 * N;
+		const float *sxy = st + 9 * N, *sxz = st + 10 * N, *syz = st + 11 * N;
+		unsigned fl = L.flags[n];
+		if (m1 && m2 && m3) {
+			long v0 = n - sz - sx - 1;
+			const double *vx = sol + N, *vy = sol + 2 * N, *vz = sol + 3 * N;
+			double xkf = L.ihx[m1 - 1], ykf = L.ihy[m2 - 1], zkf = L.ihz[m3 - 1];
+			double lc = (double)((float)vx[v0 + sx] - (float)vx[v0]) * xkf +
+				    (double)((float)vy[v0 + 1] - (float)vy[v0]) * ykf +
+				    (double)((float)vz[v0 + sz] - (float)vz[v0]) * zkf;
+			double skf = (1.0 / xkf + 1.0 / ykf + 1.0 / zkf) / 3.0;
+			lc = (lc - 0) * skf;
+			if (!(fl & FL_BC(0))) { acc[4] += lc * lc; acc[5] += 1; }
+			else { acc[6] += lc * lc; acc[7] += 1; }
+		}
+		for (int c = 1; c <= 3; c++) {
+			if (!(fl & FL_EX(c))) continue;
+			double vv = (double)(float)sol[(long)c * N + n];
+			mn[c - 1] = fmin(mn[c - 1], vv);
+			mx[c - 1] = fmax(mx[c - 1], vv);
+			if (fl & FL_BC(c)) {
+				int r = bcid[(long)c * N + n];
+				double lb = sol[(long)c * N + n] - bc.c[r];
+				if (bc.partner[r] >= 0) lb -= bc.k[r] * sol[bc.partner[r]];
+				acc[8] += lb * lb; acc[9] += 1;
+				continue;
+			}
+			long v[8] = {n, n + 1, n + sx, n + sx + 1, n + sz, n + sz + 1, n + sz + sx, n + sz + sx + 1};
+			double xkf, ykf, zkf, right, left, ne;
+			if (c == 1) {
+				xkf = 2.0 * L.icx[m1]; ykf = L.ihy[m2]; zkf = L.ihz[m3];
+				right = -g0 * (double)(ro[v[0]] + ro[v[1]] + ro[v[4]] + ro[v[5]]) / 4.0;
+				ne = nueff48(nu, v[0] - sx, v[1] - sx, v[4] - sx, v[5] - sx, v[0], v[1], v[4], v[5], v[2],
+					     v[3], v[6], v[7], nubeg, nuend);
+				left = (double)(sxx[v[7]] - sxx[v[5]]) * xkf;
+				left += (double)(sxy[v[1]] - sxy[v[0]]) * ykf;
+				left += (double)(sxz[v[4]] - sxz[v[0]]) * zkf;
+				left -= (P[v[7]] - P[v[5]]) * xkf;
+			} else if (c == 2) {
+				xkf = L.ihx[m1]; ykf = 2.0 * L.icy[m2]; zkf = L.ihz[m3];
+				right = -g1 * (double)(ro[v[0]] + ro[v[2]] + ro[v[4]] + ro[v[6]]) / 4.0;
+				ne = nueff48(nu, v[0] - 1, v[2] - 1, v[4] - 1, v[6] - 1, v[0], v[2], v[4], v[6], v[1], v[3],
+					     v[5], v[7], nubeg, nuend);
+				left = (double)(syy[v[7]] - syy[v[6]]) * ykf;
+				left += (double)(sxy[v[2]] - sxy[v[0]]) * xkf;
+				left += (double)(syz[v[4]] - syz[v[0]]) * zkf;
+				left -= (P[v[7]] - P[v[6]]) * ykf;
+			} else {
+				xkf = L.ihx[m1]; ykf = L.ihy[m2]; zkf = 2.0 * L.icz[m3];
+				right = -g2 * (double)(ro[v[0]] + ro[v[1]] + ro[v[2]] + ro[v[3]]) / 4.0;
+				ne = nueff48(nu, v[0] - sz, v[1] - sz, v[2] - sz, v[3] - sz, v[0], v[1], v[2], v[3], v[4],
+					     v[5], v[6], v[7], nubeg, nuend);
+				left = (double)(szz[v[7]] - szz[v[3]]) * zkf;
+				left += (double)(sxz[v[2]] - sxz[v[0]]) * xkf;
+				left += (double)(syz[v[1]] - syz[v[0]]) * ykf;
+				left -= (P[v[7]] - P[v[3]]) * zkf;
+			}
+			double skf1 = 1.0 / ne * (1.0 / xkf / xkf + 1.0 / ykf / ykf + 1.0 / zkf / zkf) / 3.0;
+			left = left - right;
+			double e = left * skf1;
+			acc[0] += left * left; acc[1] += e * e; acc[2] += e * e * left * left; acc[3] += 1;
+		}
+	}
+	__shared__ double sh[NB_NORM / 64][16];
+	int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	for (int q = 0; q < 10; q++) {
+		double v = acc[q];
+		for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+		if (lane == 0) sh[w][q] = v;
+	}
+	for (int q = 0; q < 3; q++) {
+		double a = mn[q], b = mx[q];
+		for (int o = 32; o > 0; o >>= 1) {
+			a = fmin(a, __shfl_down(a, o, 64));
+			b = fmax(b, __shfl_down(b, o, 64));
+		}
+		if (lane == 0) { sh[w][10 + 2 * q] = a; sh[w][11 + 2 * q] = b; }
+	}
+	__syncthreads();
+	if (threadIdx.x < 16) {
+		int q = threadIdx.x;
+		double v = sh[0][q];
+		for (int k = 1; k < NB_NORM / 64; k++) {
+			double u = sh[k][q];
+			if (q < 10) v += u;
+			else if ((q - 10) % 2 == 0) v = fmin(v, u);
+			else v = fmax(v, u);
+		}
+		long blk = (long)blockIdx.y * gridDim.x + blockIdx.x;
+		part[blk * 16 + q] = v;
+	}
+}
+
+__global__ void k_native_final(long nblk, const double *part, double *out)
+{
+	__shared__ double sh[256][16];
+	double v[16];
+	for (int q = 0; q < 16; q++) v[q] = q < 10 ? 0.0 : ((q - 10) % 2 == 0 ? 1e300 : -1e300);
+	for (long b = threadIdx.x; b < nblk; b += 256)
+		for (int q = 0; q < 16; q++) {
+			double u = part[b * 16 + q];
+			if (q < 10) v[q] += u;
+			else if ((q - 10) % 2 == 0) v[q] = fmin(v[q], u);
+			else v[q] = fmax(v[q], u);
+		}
+	for (int q = 0; q < 16; q++) sh[threadIdx.x][q] = v[q];
+	__syncthreads();
+	for (int o = 128; o > 0; o >>= 1) {
+		if ((int)threadIdx.x < o)
+			for (int q = 0; q < 16; q++) {
+				double a = sh[threadIdx.x][q], b = sh[threadIdx.x + o][q];
+				sh[threadIdx.x][q] = q < 10 ? a + b : ((q - 10) % 2 == 0 ? fmin(a, b) : fmax(a, b));
+			}
+		__syncthreads();
+	}
+	if (threadIdx.x < 16) out[threadIdx.x] = sh[0][threadIdx.x];
+}
+
+static int ensure_st(i3g_ctx *c)
+{
+	if (c->st12) return I3G_OK;
+	int err = 0;
+	c->st12 = dalloc<float>(12 * c->L[0].N, &err);
+	if (err) return err;
+	HCHK(hipMemsetAsync(c->st12, 0, 12 * c->L[0].N * sizeof(float), c->s));
+	return I3G_OK;
+}
+
+extern "C" int i3g_stokes_strain_stress(i3g_ctx *c, float *exx, float *eyy, float *ezz, float *exy, float *exz,
+					float *eyz, float *sxx, float *syy, float *szz, float *sxy, float *sxz,
+					float *syz)
+{
+	if (!c || !c->coef_ready) return I3G_EINVAL;
+	int rc = ensure_st(c);
+	if (rc) return rc;
+	Level &L = c->L[0];
+	hipLaunchKernelGGL(k_strain_stress, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, view<double>(L),
+			   (const double *)c->sol, c->st12);
+	HCHK(hipGetLastError());
+	float *dst[12] = {exx, eyy, ezz, exy, exz, eyz, sxx, syy, szz, sxy, sxz, syz};
+	for (int q = 0; q < 12; q++)
+		if (dst[q])
+			HCHK(hipMemcpyAsync(dst[q], c->st12 + (long)q * L.N, L.N * sizeof(float), hipMemcpyDeviceToHost,
+					    c->s));
+	HCHK(hipStreamSynchronize(c->s));
+	return I3G_OK;
+}
+
+extern "C" int i3g_stokes_native_norms(i3g_ctx *c, double divvmax, double *out)
+{
+	if (!c || !out || !c->coef_ready) return I3G_EINVAL;
+	int rc = ensure_st(c);
+	if (rc) return rc;
+	Level &L = c->L[0];
+	dim3 pg = plane_grid(L.nx, L.ny, L.nz);
+	long nblk = (long)pg.x * pg.y;
+	hipLaunchKernelGGL(k_strain_stress, pg, dim3(256), 0, c->s, view<double>(L), (const double *)c->sol, c->st12);
+	double *part = nullptr;
+	int err = 0;
+	part = dalloc<double>(16 * nblk + 16, &err);
+	if (err) return err;
+	hipLaunchKernelGGL(k_native_norms, pg, dim3(NB_NORM), 0, c->s, view<double>(L), (const double *)c->sol,
+			   (const float *)c->st12, (const float *)L.nu, (const float *)c->ro, c->phys[0], c->phys[1],
+			   c->phys[2], c->phys[3], c->phys[4], (const int *)L.bcid, bcview<double>(L), part);
+	hipLaunchKernelGGL(k_native_final, dim3(1), dim3(256), 0, c->s, nblk, (const double *)part, part + 16 * nblk);
+	double h[16];
+	hipError_t e1 = hipGetLastError();
+	hipError_t e2 = hipMemcpyAsync(h, part + 16 * nblk, 16 * sizeof(double), hipMemcpyDeviceToHost, c->s);
+	hipError_t e3 = hipStreamSynchronize(c->s);
+	hipFree(part);
+	if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return I3G_EDEVICE;
+	/* maxvxyz (move3mg.c:1766-1770) */
+	double dvx = h[11] - h[10], dvy = h[13] - h[12], dvz = h[15] - h[14];
+	double mv = sqrt((dvx * dvx + dvy * dvy + dvz * dvz) / 3.0);
+	if (mv <= 0) mv = 1e-8;
+	if (divvmax > 0 && mv > divvmax) mv = divvmax;
+	out[0] = sqrt(h[0] / h[3]);
+	out[1] = sqrt(h[1] / h[3]) / mv;
+	out[2] = sqrt(h[2] / h[3]) / mv;
+	out[3] = h[5] > 0 ? sqrt(h[4] / h[5]) / mv : 0;
+	out[4] = h[7] > 0 ? sqrt(h[6] / h[7]) / mv : 0;
+	out[5] = h[9] > 0 ? sqrt(h[8] / h[9]) : 0;
+	out[6] = mv;
+	return I3G_OK;
 }
 
 extern "C" int i3g_set_timing(i3g_ctx *c, int on)

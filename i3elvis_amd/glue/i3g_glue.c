@@ -17,9 +17,9 @@
  *   i3g_vpiterate  the Stokes solve of vpiterate (move3mg.c:1585-2636):
  *                  solve to i3g_glue_rtol (relative FP64 residual) from the
  *                  current vx/vy/vz/pr, pressure gauge (pxinit..pinit),
- *                  stress/strain fields by the program's own sxxcalc..syzcalc
- *                  in value mode, and the marker time-step limit
- *                  (move3mg.c:2609-2632).
+ *                  the stress/strain-rate fields (value mode of
+ *                  sxxcalc..syzcalc, on the device) and the marker
+ *                  time-step limit (move3mg.c:2609-2632).
  *   i3g_titerate   the heat step of titerate (heat3mg.c:287-588) without
  *                  adiabatic/shear heating, then the marker temperature
  *                  update (heat3mg.c:592-782).
@@ -132,6 +132,7 @@ void i3g_vpiterate(int m0, int f0)
 	long N, n, m1, m2, m3;
 	int rc;
 	(void)m0; (void)f0;
+	if (xnumx * ynumy * znumz > MAXNNN) glue_fail("grid above the program's stress arrays", I3G_ECAP);
 	glue_grid();
 	N = xnumx * ynumy * znumz;
 	double phys[6] = {GXKOEF, GYKOEF, GZKOEF, nubeg, nuend, (double)viscmod};
@@ -174,31 +175,10 @@ void i3g_vpiterate(int m0, int f0)
 	double vm[3] = {mxx > -mnx ? mxx : -mnx, mxy > -mny ? mxy : -mny, mxz > -mnz ? mxz : -mnz};
 	for (int a = 0; a < 3; a++)
 		if (vm[a] > 0 && timestep > maxxystep / vm[a]) timestep = maxxystep / vm[a];
-	/* stress and strain-rate fields from the solution (value mode of the
-	 * program's own stress routines, as at the end of vpiterate) */
-	{
-		long un1[MAXPOS];
-		double ui1[MAXPOS], eps1[100];
-		for (m3 = 0; m3 < znumz; m3++)
-			for (m1 = 0; m1 < xnumx; m1++)
-				for (m2 = 0; m2 < ynumy; m2++) {
-					n = m3 * xnumx * ynumy + m1 * ynumy + m2;
-					if (m1 > 0 && m2 > 0 && m3 > 0) {
-						sxx[n] = sxxcalc(m1, m2, m3, 0, 0, eps1, un1, ui1); exx[n] = eps1[0];
-						syy[n] = syycalc(m1, m2, m3, 0, 0, eps1, un1, ui1); eyy[n] = eps1[0];
-						szz[n] = szzcalc(m1, m2, m3, 0, 0, eps1, un1, ui1); ezz[n] = eps1[0];
-					}
-					if (m1 > 0 && m1 < xnumx - 1 && m2 > 0 && m2 < ynumy - 1 && m3 < znumz - 1) {
-						sxy[n] = sxycalc(m1, m2, m3, 0, 0, eps1, un1, ui1); exy[n] = eps1[0];
-					}
-					if (m1 > 0 && m1 < xnumx - 1 && m2 < ynumy - 1 && m3 > 0 && m3 < znumz - 1) {
-						sxz[n] = sxzcalc(m1, m2, m3, 0, 0, eps1, un1, ui1); exz[n] = eps1[0];
-					}
-					if (m1 < xnumx - 1 && m2 > 0 && m2 < ynumy - 1 && m3 > 0 && m3 < znumz - 1) {
-						syz[n] = syzcalc(m1, m2, m3, 0, 0, eps1, un1, ui1); eyz[n] = eps1[0];
-					}
-				}
-	}
+	/* stress and strain-rate fields of the solution (value mode of
+	 * sxxcalc..syzcalc, computed on the device) */
+	rc = i3g_stokes_strain_stress(G.ctx, exx, eyy, ezz, exy, exz, eyz, sxx, syy, szz, sxy, sxz, syz);
+	if (rc) glue_fail("strain_stress", rc);
 }
 
 void i3g_titerate(int m0)

@@ -186,6 +186,22 @@ int i3g_stokes_get_solution(i3g_ctx *ctx, double *P, double *vx, double *vy,
 int i3g_stokes_set_solution(i3g_ctx *ctx, const double *P, const double *vx,
 			    const double *vy, const double *vz);
 
+/* Strain rates and stresses of the resident solution: the value mode of
+ * sxxcalc..syzcalc (move3mg.c:3-710) as in the loop at the end of vpiterate
+ * (move3mg.c:1691-1764), from the velocities rounded to float (the
+ * reference's vx/vy/vz).  Host outputs of nodenum floats, each may be NULL;
+ * entries where a component does not exist are 0. */
+int i3g_stokes_strain_stress(i3g_ctx *ctx, float *exx, float *eyy, float *ezz, float *exy,
+			     float *exz, float *eyz, float *sxx, float *syy, float *szz,
+			     float *sxy, float *sxz, float *syz);
+
+/* The reference's own convergence norms of the resident solution
+ * (vpiterate, move3mg.c:2405-2596): out[0] stoksum, [1] stoksum1,
+ * [2] stoksum2, [3] contsum, [4] contsumb, [5] bondsum, [6] maxvxyz
+ * (divvmax > 0 caps it, DIVVMAX).  Momentum rows use the float stresses of
+ * i3g_stokes_strain_stress and the double pressure, as the reference. */
+int i3g_stokes_native_norms(i3g_ctx *ctx, double divvmax, double *out);
+
 /* Enables (1) or disables (0) HIP-event timing of the named kernels on the
  * context's stream, and clears the collected statistics. */
 int i3g_set_timing(i3g_ctx *ctx, int on);

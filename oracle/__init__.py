@@ -254,3 +254,49 @@ def mk_tk_update(x, y, z, mtype, markk, gx, gy, gz, tk, tk2, kt, ro, cp, heatdif
                           _p(gy), C.c_long(len(gy)), _p(gz), C.c_long(len(gz)), *[_p(a) for a in nodes],
                           C.c_double(heatdif), C.c_double(dt), C.c_double(zdeep), C.c_double(tdeep))
     return markk
+
+
+def err_rows(L, nu, ro, g, nubeg, nuend, st12, pr, vx, vy, vz, bondm, bondv, bondn, sol):
+    N = L.nx * L.ny * L.nz
+    rows = np.zeros((5, 4 * N, 2))
+    g = np.asarray(g, np.float64)
+    st12 = np.ascontiguousarray(st12, np.float32)
+    lib().or_err_rows(C.byref(L), _p(nu), _p(ro), _p(g), C.c_double(nubeg), C.c_double(nuend), _p(st12),
+                      _p(np.ascontiguousarray(pr, np.float64)), _p(vx), _p(vy), _p(vz), _p(bondm), _p(bondv),
+                      _p(bondn), _p(np.ascontiguousarray(sol, np.float64)), _p(rows))
+    return rows
+
+
+def native_norms(rows, maxvxyz, bondm=None):
+    """vpiterate's stoksum, stoksum1, stoksum2, contsum, contsumb, bondsum
+    (move3mg.c:2405-2596) from residual-mode rows; maxvxyz as the reference
+    (RMS velocity range, 1e-8 when zero).  bondm splits the continuity rows
+    into free (contsum) and BC (contsumb) pressure cells."""
+    mom = np.concatenate([rows[c][np.isfinite(rows[c][:, 0])] for c in (1, 2, 3)])
+    ival, ival1 = mom[:, 0], mom[:, 1] / maxvxyz
+    ok = np.isfinite(rows[0][:, 0])
+    cont = rows[0][:, 0] / maxvxyz
+    out = dict(stoksum=np.sqrt(np.mean(ival ** 2)), stoksum1=np.sqrt(np.mean(ival1 ** 2)),
+               stoksum2=np.sqrt(np.mean(ival1 ** 2 * ival ** 2)),
+               bondsum=np.sqrt(np.mean(rows[4][np.isfinite(rows[4][:, 0]), 0] ** 2)), cont=cont[ok])
+    if bondm is not None:
+        bc = np.asarray(bondm)[:len(cont)] != 0
+        out["contsum"] = np.sqrt(np.mean(cont[ok & ~bc] ** 2))
+        out["contsumb"] = np.sqrt(np.mean(cont[ok & bc] ** 2)) if np.any(ok & bc) else 0.0
+    return out
+
+
+def maxvxyz(vx, vy, vz, nx, ny, nz, divvmax=0.0):
+    """RMS of the float velocity ranges over existing unknowns
+    (move3mg.c:1648-1672, 1766-1770)"""
+    m1 = (np.arange(nx * ny * nz) // ny) % nx
+    m2 = np.arange(nx * ny * nz) % ny
+    m3 = np.arange(nx * ny * nz) // (nx * ny)
+    sel = [(m2 < ny - 1) & (m3 < nz - 1), (m1 < nx - 1) & (m3 < nz - 1), (m1 < nx - 1) & (m2 < ny - 1)]
+    d = [float(np.float32(v)[s].max()) - float(np.float32(v)[s].min()) for v, s in zip((vx, vy, vz), sel)]
+    mv = np.sqrt(sum(x * x for x in d) / 3.0)
+    if mv <= 0:
+        mv = 1e-8
+    if divvmax > 0 and mv > divvmax:
+        mv = divvmax
+    return mv

@@ -1074,3 +1074,98 @@ void or_mk_tk_update(long n, const float *x, const float *y, const float *z, con
 	free(s0);
 	free(s1);
 }
+
+/* ---- native norms (test infrastructure) -------------------------------- */
+
+/* 48-weight viscosity of a momentum row (x/y/zstokserr ynerr=1,
+ * move3mg.c:770, 966, 1165): the reference sums the first four node values
+ * in float, the ten-fold group in float, the rest in double. */
+static double nueff48(const float *nu, long a0, long a1, long a2, long a3, long b0, long b1, long b2, long b3,
+		      long c0, long c1, long c2, long c3, double nubeg, double nuend)
+{
+	float s1 = nu[a0] + nu[a1] + nu[a2] + nu[a3];
+	float s2 = nu[b0] + nu[b1] + nu[b2] + nu[b3];
+	double e = ((double)s1 + 10.0 * (double)s2 + nu[c0] + nu[c1] + nu[c2] + nu[c3]) / 48.0;
+	if (e > nuend) e = nuend;
+	if (e < nubeg) e = nubeg;
+	return e;
+}
+
+void or_err_rows(const orl_t *L, const float *nu, const float *ro, const double *g, double nubeg, double nuend,
+		 const float *st12, const double *pr, const float *vx, const float *vy, const float *vz,
+		 const int *bondm, const float *bondv, const int *bondn, const double *sol, double *rows)
+{
+	long nx = L->nx, ny = L->ny, nz = L->nz, N = nx * ny * nz, sx = ny, sz = nx * ny;
+	const double *gx = L->gx, *gy = L->gy, *gz = L->gz;
+	const float *sxx = st12 + 6 * N, *syy = st12 + 7 * N, *szz = st12 + 8 * N;
+	const float *sxy = st12 + 9 * N, *sxz = st12 + 10 * N, *syz = st12 + 11 * N;
+	long m1, m2, m3, i;
+	for (i = 0; i < 5 * 4 * N * 2; i++) rows[i] = NAN;
+#define ROW(k, u, a, b) do { rows[(((long)(k) * 4 * N) + (u)) * 2] = (a); rows[(((long)(k) * 4 * N) + (u)) * 2 + 1] = (b); } while (0)
+	for (m3 = 0; m3 < nz; m3++)
+	for (m1 = 0; m1 < nx; m1++)
+	for (m2 = 0; m2 < ny; m2++) {
+		long n = m3 * sz + m1 * sx + m2;
+		if (m1 && m2 && m3) {
+			/* continerr ynerr=1 (move3mg.c:1342-1348) */
+			double xkf = 1.0 / (gx[m1] - gx[m1 - 1]), ykf = 1.0 / (gy[m2] - gy[m2 - 1]),
+			       zkf = 1.0 / (gz[m3] - gz[m3 - 1]);
+			long v0 = n - sz - sx - 1;
+			double lc = (vx[v0 + sx] - vx[v0]) * xkf + (vy[v0 + 1] - vy[v0]) * ykf + (vz[v0 + sz] - vz[v0]) * zkf;
+			double skf = (1.0 / xkf + 1.0 / ykf + 1.0 / zkf) / 3.0;
+			ROW(0, 4 * n, (lc - 0) * skf, 0.0);
+		}
+		for (int c = 1; c <= 3; c++) {
+			int ok = c == 1 ? (m2 < ny - 1 && m3 < nz - 1) : c == 2 ? (m1 < nx - 1 && m3 < nz - 1)
+									: (m1 < nx - 1 && m2 < ny - 1);
+			if (!ok) continue;
+			long u = 4 * n + c;
+			int rec = bondm[u];
+			if (rec) {
+				double lb = sol[u] - bondv[2 * rec];
+				if (bondn[rec]) lb -= bondv[2 * rec + 1] * sol[bondn[rec] - 1];
+				ROW(4, u, lb, 0.0);
+				continue;
+			}
+			long v[8];
+			v[0] = n; v[1] = n + 1; v[2] = n + sx; v[3] = n + sx + 1;
+			v[4] = n + sz; v[5] = n + sz + 1; v[6] = n + sz + sx; v[7] = n + sz + sx + 1;
+			double xkf, ykf, zkf, right, left, ne;
+			if (c == 1) {
+				xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]); ykf = 1.0 / (gy[m2 + 1] - gy[m2]);
+				zkf = 1.0 / (gz[m3 + 1] - gz[m3]);
+				right = -g[0] * (ro[v[0]] + ro[v[1]] + ro[v[4]] + ro[v[5]]) / 4.0;
+				ne = nueff48(nu, v[0] - sx, v[1] - sx, v[4] - sx, v[5] - sx, v[0], v[1], v[4], v[5], v[2], v[3],
+					     v[6], v[7], nubeg, nuend);
+				left = (sxx[v[7]] - sxx[v[5]]) * xkf;
+				left += (sxy[v[1]] - sxy[v[0]]) * ykf;
+				left += (sxz[v[4]] - sxz[v[0]]) * zkf;
+				left -= (pr[v[7]] - pr[v[5]]) * xkf;
+			} else if (c == 2) {
+				xkf = 1.0 / (gx[m1 + 1] - gx[m1]); ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+				zkf = 1.0 / (gz[m3 + 1] - gz[m3]);
+				right = -g[1] * (ro[v[0]] + ro[v[2]] + ro[v[4]] + ro[v[6]]) / 4.0;
+				ne = nueff48(nu, v[0] - 1, v[2] - 1, v[4] - 1, v[6] - 1, v[0], v[2], v[4], v[6], v[1], v[3],
+					     v[5], v[7], nubeg, nuend);
+				left = (syy[v[7]] - syy[v[6]]) * ykf;
+				left += (sxy[v[2]] - sxy[v[0]]) * xkf;
+				left += (syz[v[4]] - syz[v[0]]) * zkf;
+				left -= (pr[v[7]] - pr[v[6]]) * ykf;
+			} else {
+				xkf = 1.0 / (gx[m1 + 1] - gx[m1]); ykf = 1.0 / (gy[m2 + 1] - gy[m2]);
+				zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+				right = -g[2] * (ro[v[0]] + ro[v[1]] + ro[v[2]] + ro[v[3]]) / 4.0;
+				ne = nueff48(nu, v[0] - sz, v[1] - sz, v[2] - sz, v[3] - sz, v[0], v[1], v[2], v[3], v[4], v[5],
+					     v[6], v[7], nubeg, nuend);
+				left = (szz[v[7]] - szz[v[3]]) * zkf;
+				left += (sxz[v[2]] - sxz[v[0]]) * xkf;
+				left += (syz[v[1]] - syz[v[0]]) * ykf;
+				left -= (pr[v[7]] - pr[v[3]]) * zkf;
+			}
+			double skf1 = 1.0 / ne * (1.0 / xkf / xkf + 1.0 / ykf / ykf + 1.0 / zkf / zkf) / 3.0;
+			left = left - right;
+			ROW(c, u, left, left * skf1);
+		}
+	}
+#undef ROW
+}

@@ -136,4 +136,15 @@ void or_mk_tk_update(long n, const float *x, const float *y, const float *z, con
 		     const float *tk2, const float *kt, const float *ro, const float *cp, double heatdif, double dt,
 		     double zdeep, double tdeep);
 
+/* ---- native norms (test infrastructure) -------------------------------- */
+
+/* Residual-mode rows of vpiterate's norm loop (move3mg.c:2405-2560):
+ * rows[k][u][0..1] for kind k = 0 continerr (u = 4n), 1..3 x/y/zstokserr
+ * (return value, eps1[0] = value * skf1) on non-BC rows, 4 bonderr of the
+ * BC velocity rows; NaN elsewhere.  st12: exx..eyz, sxx..syz float planes;
+ * sol: interleaved double solution (for bonderr). */
+void or_err_rows(const orl_t *L, const float *nu, const float *ro, const double *g, double nubeg, double nuend,
+		 const float *st12, const double *pr, const float *vx, const float *vy, const float *vz,
+		 const int *bondm, const float *bondv, const int *bondn, const double *sol, double *rows);
+
 #endif

@@ -197,3 +197,50 @@ void refh_vxyzcalc1(double x, double y, double z, double *out)
 /* The reference's scatter loops race between OpenMP threads
  * (move3mg.c:2662, :3185; mark3mg.c:460); golden vectors use one thread. */
 void refh_set_threads(int n) { omp_set_num_threads(n); }
+
+/* Value-mode strain rates and stresses of level 0 into the reference's own
+ * sxx..syz / exx..eyz arrays, the loop at the end of vpiterate
+ * (move3mg.c:1691-1764), from the current vx/vy/vz. */
+void refh_fill_stress(void)
+{
+	long m1, m2, m3, n;
+	long un1[MAXPOS];
+	double ui1[MAXPOS], eps1[100];
+	for (m3 = 0; m3 < mgz[0]; m3++)
+	for (m1 = 0; m1 < mgx[0]; m1++)
+	for (m2 = 0; m2 < mgy[0]; m2++) {
+		n = m3 * mgxy[0] + m1 * mgy[0] + m2;
+		if (m1 > 0 && m2 > 0 && m3 > 0) {
+			sxx[n] = sxxcalc(m1, m2, m3, 0, 0, eps1, un1, ui1); exx[n] = eps1[0];
+			syy[n] = syycalc(m1, m2, m3, 0, 0, eps1, un1, ui1); eyy[n] = eps1[0];
+			szz[n] = szzcalc(m1, m2, m3, 0, 0, eps1, un1, ui1); ezz[n] = eps1[0];
+		}
+		if (m1 > 0 && m1 < mgx[0] - 1 && m2 > 0 && m2 < mgy[0] - 1 && m3 < mgz[0] - 1) {
+			sxy[n] = sxycalc(m1, m2, m3, 0, 0, eps1, un1, ui1); exy[n] = eps1[0];
+		}
+		if (m1 > 0 && m1 < mgx[0] - 1 && m2 < mgy[0] - 1 && m3 > 0 && m3 < mgz[0] - 1) {
+			sxz[n] = sxzcalc(m1, m2, m3, 0, 0, eps1, un1, ui1); exz[n] = eps1[0];
+		}
+		if (m1 < mgx[0] - 1 && m2 > 0 && m2 < mgy[0] - 1 && m3 > 0 && m3 < mgz[0] - 1) {
+			syz[n] = syzcalc(m1, m2, m3, 0, 0, eps1, un1, ui1); eyz[n] = eps1[0];
+		}
+	}
+}
+
+/* Residual-mode rows of the norm loop of vpiterate (move3mg.c:2405-2560):
+ * kind 0 continerr, 1..3 x/y/zstokserr, 4 bonderr of sol0 unknown
+ * 4 n + comp (comp = m1 here).  out[0] = return value, out[1] = eps1[0]. */
+void refh_err_row(int kind, long m1, long m2, long m3, double *out)
+{
+	long un1[MAXPOS];
+	double ui1[MAXPOS], eps1[100];
+	eps1[0] = 0;
+	switch (kind) {
+	case 0: out[0] = continerr(m1, m2, m3, 1, 0, eps1, un1, ui1); break;
+	case 1: out[0] = xstokserr(m1, m2, m3, 1, 0, eps1, un1, ui1); break;
+	case 2: out[0] = ystokserr(m1, m2, m3, 1, 0, eps1, un1, ui1); break;
+	case 3: out[0] = zstokserr(m1, m2, m3, 1, 0, eps1, un1, ui1); break;
+	default: out[0] = bonderr(m1, 1, eps1, un1, ui1); break;
+	}
+	out[1] = eps1[0];
+}

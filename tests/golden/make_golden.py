@@ -296,10 +296,71 @@ def g6(R, n=17):
     print("g6_heat.npz written")
 
 
+def g7(R):
+    """Value-mode strain rates/stresses (move3mg.c:3-710) and the
+    residual-mode rows behind vpiterate's native norms (continerr,
+    x/y/zstokserr with ynerr=1, bonderr; move3mg.c:718-1579, 2405-2560) on a
+    random non-uniform grid with random fields."""
+    import ctypes as C
+    rng = np.random.default_rng(707)
+    nx, ny, nz = 12, 10, 11
+    gx = np.cumsum(np.r_[0, rng.uniform(0.6, 1.4, nx - 1)]) * 3e3
+    gy = np.cumsum(np.r_[0, rng.uniform(0.6, 1.4, ny - 1)]) * 3e3
+    gz = np.cumsum(np.r_[0, rng.uniform(0.6, 1.4, nz - 1)]) * 3e3
+    N = nx * ny * nz
+    h = I.hierarchy(nx, ny, nz, 0)
+    rules, nr = I.freeslip_rules()
+    bondm, bondv, bondn = I.expand_bc(rules, nr, h)
+    f = {k: (10 ** rng.uniform(19, 23, N)).astype(np.float32) for k in ("nu", "nxx", "nxy", "nxz", "nyz")}
+    f["ro"] = rng.uniform(3000, 3400, N).astype(np.float32)
+    R.setup_grid(gx, gy, gz, 0)
+    R.set_bc(bondm, bondv, bondn)
+    R.set_physics(g=(0.3, 9.81, -0.2), nubeg=1e19, nuend=1e23)
+    for k, v in f.items():
+        R.f32(k)[:N] = v
+    vel = {k: (rng.standard_normal(N) * 1e-9).astype(np.float32) for k in ("vx", "vy", "vz")}
+    pr = rng.standard_normal(N) * 1e8
+    for k, v in vel.items():
+        R.f32(k)[:N] = v
+    R.f64("pr")[:N] = pr
+    sol = np.stack([pr, vel["vx"].astype(np.float64), vel["vy"].astype(np.float64),
+                    vel["vz"].astype(np.float64)], 1).ravel()
+    R.f64("sol0", R.MAXPPP)[:4 * N] = sol
+    R.lib.refh_fill_stress()
+    names = ["exx", "eyy", "ezz", "exy", "exz", "eyz", "sxx", "syy", "szz", "sxy", "sxz", "syz"]
+    for k in names:
+        R.f32(k, R.MAXNNN)[:N] = 0
+    R.lib.refh_fill_stress()
+    stress = np.stack([R.f32(k, R.MAXNNN)[:N].copy() for k in names])
+    rows = np.full((5, 4 * N, 2), np.nan)
+    out = (C.c_double * 2)()
+    for m3 in range(nz):
+        for m1 in range(nx):
+            for m2 in range(ny):
+                n = m3 * nx * ny + m1 * ny + m2
+                if m1 and m2 and m3:
+                    R.lib.refh_err_row(0, m1, m2, m3, out)
+                    rows[0, 4 * n] = list(out)
+                for c, ok in ((1, m2 < ny - 1 and m3 < nz - 1), (2, m1 < nx - 1 and m3 < nz - 1),
+                              (3, m1 < nx - 1 and m2 < ny - 1)):
+                    if not ok:
+                        continue
+                    if bondm[4 * n + c] == 0:
+                        R.lib.refh_err_row(c, m1, m2, m3, out)
+                        rows[c, 4 * n + c] = list(out)
+                    else:
+                        R.lib.refh_err_row(4, 4 * n + c, 0, 0, out)
+                        rows[4, 4 * n + c] = list(out)
+    np.savez_compressed(os.path.join(HERE, "g7_stress_norms.npz"), gx=gx, gy=gy, gz=gz, bondm=bondm, bondv=bondv,
+                        bondn=bondn, g=np.array([0.3, 9.81, -0.2]), nubeg=1e19, nuend=1e23, **f, **vel, pr=pr,
+                        stress=stress, rows=rows)
+    print("g7_stress_norms.npz written")
+
+
 if __name__ == "__main__":
     R = Ref()
     R.lib.refh_set_threads(1)
-    which = sys.argv[1:] or ["g1", "g3", "g5", "g6"]
+    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7"]
     if "g1" in which:
         g1(R)
     if "g3" in which:
@@ -309,3 +370,5 @@ if __name__ == "__main__":
         g5(R)
     if "g6" in which:
         g6(R)
+    if "g7" in which:
+        g7(R)
