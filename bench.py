@@ -222,6 +222,8 @@ def main():
     wall = time.time() - t0
     kst = S.kernel_stats()
     mixed_t = float(np.mean(times))
+    # the reference's own convergence norms of the product path's result
+    native = {k: float(v) for k, v in S.native_norms().items()}
 
     fp64 = None
     if not a.no_fp64:
@@ -262,6 +264,7 @@ def main():
         "fp64_mg": fp64, "speedup_vs_fp64_mg": (round(fp64["seconds"] / mixed_t, 3) if fp64 else None),
         "learned": (dict(learned, weights=os.path.relpath(wpath, ROOT)) if ws is not None else None),
         "mixed_zero_guess": mixed_zero,
+        "native_norms": native,
         "roofline": roof, "cpu_baseline": cpu,
         "kernels": {k: dict(avg_ms=round(v["avg_ms"], 5), calls=v["calls"]) for k, v in kst.items()},
     }

@@ -274,8 +274,8 @@ def native_norms(rows, maxvxyz, bondm=None):
     into free (contsum) and BC (contsumb) pressure cells."""
     mom = np.concatenate([rows[c][np.isfinite(rows[c][:, 0])] for c in (1, 2, 3)])
     ival, ival1 = mom[:, 0], mom[:, 1] / maxvxyz
-    ok = np.isfinite(rows[0][:, 0])
-    cont = rows[0][:, 0] / maxvxyz
+    ok = np.isfinite(rows[0][0::4, 0])     # continuity rows live at u = 4 n
+    cont = rows[0][0::4, 0] / maxvxyz
     out = dict(stoksum=np.sqrt(np.mean(ival ** 2)), stoksum1=np.sqrt(np.mean(ival1 ** 2)),
                stoksum2=np.sqrt(np.mean(ival1 ** 2 * ival ** 2)),
                bondsum=np.sqrt(np.mean(rows[4][np.isfinite(rows[4][:, 0]), 0] ** 2)), cont=cont[ok])
