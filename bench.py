@@ -119,6 +119,59 @@ def _cpu_baseline_once(n, cap_s, threads):
                 vcycles_per_s=cycles / dt, rel_reached=rel, n=n)
 
 
+def other_paths(I, np, cfg257):
+    """Device throughput of the other hot-path kernels (HIP events around the
+    kernels, host copies excluded): marker cell index, sort, marker -> node
+    transfer, RK4 and temperature update on a 129^3 grid with 8 markers per
+    cell (16.8M markers); one implicit heat step on the 257^3 grid."""
+    out = {}
+    n = 129
+    cfg = I.make_config(1, n, n, n, seed=12345)
+    gx, gy, gz = cfg["gx"], cfg["gy"], cfg["gz"]
+    rng = np.random.default_rng(4)
+    cells = np.stack(np.meshgrid(np.arange(n - 1), np.arange(n - 1), np.arange(n - 1), indexing="ij"), -1).reshape(-1, 3)
+    c = np.repeat(cells, 8, 0)
+    x = (gx[c[:, 0]] + rng.uniform(0.01, 0.99, len(c)) * (gx[1] - gx[0])).astype(np.float32)
+    y = (gy[c[:, 1]] + rng.uniform(0.01, 0.99, len(c)) * (gy[1] - gy[0])).astype(np.float32)
+    z = (gz[c[:, 2]] + rng.uniform(0.01, 0.99, len(c)) * (gz[1] - gz[0])).astype(np.float32)
+    perm = rng.permutation(len(x))
+    x, y, z = x[perm], y[perm], z[perm]
+    m = len(x)
+    N = n ** 3
+    M = I.Markers(gx, gy, gz, m)
+    M.upload(x, y, z, rng.uniform(600, 1600, m).astype(np.float32), np.zeros(m, np.float32),
+             rng.integers(2, 7, m).astype(np.int8))
+    res = {}
+    M.cell_index()
+    res["cell_index"] = (M.last_ms(), m * 16.0)
+    M.sort()
+    res["sort"] = (M.last_ms(), m * (8.0 + 2 * 21))
+    rocks = [[1e21, 0, 0, 1e18, 1e25, 3300, 3e-5, 1e-3, 1000, 3, 0, 0, 1e-8]] * 2 + \
+        [[2e16, 3e5, 4.0, 1e19, 1e24, 3300, 3e-5, 1e-3, 1000, 0.73, 1293, 4e-6, 2e-8]] * 5
+    M.to_nodes(rocks, np.full(N, 1e8), np.zeros((11, N), np.float32), 1e18, 1e25, 0.0, 1, 0.0)
+    res["to_nodes"] = (M.last_ms(), m * 40.0 + N * 44.0)
+    tk = rng.uniform(600, 1600, N).astype(np.float32)
+    one = np.ones(N, np.float32)
+    M.update_tk(tk + 1, tk, 3 * one, 3300 * one, 1000 * one, 1.0, 1e12)
+    res["update_tk"] = (M.last_ms(), m * 48.0 + N * 28.0)
+    v = [(rng.standard_normal(N) * 1e-9).astype(np.float32) for _ in range(3)]
+    M.advect_rk4(1e11, *v)
+    res["rk4"] = (M.last_ms(), m * 24.0 + N * 12.0)
+    M.close()
+    out["markers"] = {k: dict(ms=round(ms, 3), markers_per_s=round(m / (ms * 1e-3), 1),
+                              gbs_algorithmic=round(by / (ms * 1e-3) / 1e9, 1)) for k, (ms, by) in res.items()}
+    out["markers"]["count"] = m
+    c2 = cfg257
+    N2 = c2["nx"] * c2["ny"] * c2["nz"]
+    H = I.Heat(c2["gx"], c2["gy"], c2["gz"], c2["bondm"], c2["bondv"], c2["bondn"])
+    one = np.ones(N2, np.float32)
+    _, st = H.solve(c2["tk"], c2["ro"], 1000 * one, 3 * one, 1e-7 * one, timestep=3e13, heatmin=1e-6, sweeps=10)
+    H.close()
+    out["heat_257"] = dict(device_s=round(st.device_seconds, 4), sweeps=st.sweeps, substeps=st.substeps,
+                           heatsum=st.heatsum, converged=st.converged)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +182,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--krylov", type=int, default=400)
     ap.add_argument("--no-learned", action="store_true", help="mixed precision from a zero guess, no network")
+    ap.add_argument("--no-extras", action="store_true", help="skip the marker and heat measurements")
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB three times with a plain device copy (HBM counter calibration)")
@@ -242,6 +296,8 @@ def main():
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=committed_traffic(n), kernel="k_smooth_vel<float> (level 0)",
                     avg_ms=dom["avg_ms"], launches=dom["calls"], algorithmic_bytes=by)
 
+    extras = None if a.no_extras else other_paths(I, np, cfg)
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -265,6 +321,7 @@ def main():
         "learned": (dict(learned, weights=os.path.relpath(wpath, ROOT)) if ws is not None else None),
         "mixed_zero_guess": mixed_zero,
         "native_norms": native,
+        "other_paths": extras,
         "roofline": roof, "cpu_baseline": cpu,
         "kernels": {k: dict(avg_ms=round(v["avg_ms"], 5), calls=v["calls"]) for k, v in kst.items()},
     }

@@ -287,6 +287,11 @@ class Markers:
     def n(self):
         return lib().i3g_markers_count(self.h)
 
+    def last_ms(self):
+        """device time of the last operation's kernels"""
+        lib().i3g_markers_last_ms.restype = C.c_double
+        return lib().i3g_markers_last_ms(self.h)
+
     def upload(self, x, y, z, tk, eta, mtype, ids=None):
         arrs = [np.ascontiguousarray(a, np.float32) for a in (x, y, z, tk, eta)]
         t = np.ascontiguousarray(mtype, np.int8)
@@ -357,7 +362,8 @@ class HeatParams(C.Structure):
 
 class HeatStats(C.Structure):
     _fields_ = [("converged", C.c_int), ("substeps", C.c_int), ("sweeps", C.c_int),
-                ("heatsum", C.c_double), ("bondsum", C.c_double), ("seconds", C.c_double)]
+                ("heatsum", C.c_double), ("bondsum", C.c_double), ("seconds", C.c_double),
+                ("device_seconds", C.c_double)]
 
 
 class Heat:

@@ -391,6 +391,10 @@ extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const flo
 	int plume = p->timesum > p->timebeg ? (p->timesum >= p->timeend ? 2 : 1) : 0;
 	dim3 gg((unsigned)(((long)h->nx * h->ny + NB - 1) / NB), (unsigned)h->nz);
 	long nblk = (long)gg.x * gg.y;
+	hipEvent_t e0, e1;
+	TCHK(hipEventCreate(&e0));
+	TCHK(hipEventCreate(&e1));
+	TCHK(hipEventRecord(e0, h->s));
 	double total = p->timestep, done = 0;
 	double hres[2] = {0, 0};
 	int rc = I3G_OK;
@@ -440,8 +444,15 @@ extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const flo
 		if (rc) break;
 	}
 	TCHK(hipGetLastError());
+	TCHK(hipEventRecord(e1, h->s));
 	TCHK(hipMemcpyAsync(tk, h->tk0, N * sizeof(float), hipMemcpyDeviceToHost, h->s));
 	TCHK(hipStreamSynchronize(h->s));
+	{
+		float ms = 0;
+		if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) s.device_seconds = ms * 1e-3;
+		hipEventDestroy(e0);
+		hipEventDestroy(e1);
+	}
 	s.converged = rc == I3G_OK;
 	s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 	if (st) *st = s;

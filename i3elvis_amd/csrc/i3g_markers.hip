@@ -67,7 +67,19 @@ struct i3g_markers {
 	float *tnod = nullptr;
 	double *dtk = nullptr, *sn = nullptr;
 	uint8_t *tin = nullptr;
+	/* device time of the last operation's kernels (host copies excluded) */
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	float last_ms = 0;
 };
+
+static void mark_begin(i3g_markers *m) { hipEventRecord(m->ev0, m->s); }
+static void mark_end(i3g_markers *m) { hipEventRecord(m->ev1, m->s); }
+/* after a synchronisation of the stream */
+static void mark_read(i3g_markers *m)
+{
+	float ms = 0;
+	if (hipEventElapsedTime(&ms, m->ev0, m->ev1) == hipSuccess) m->last_ms = ms;
+}
 
 /* ------------------------------------------------------------------ */
 /* Cell search                                                        */
@@ -717,6 +729,10 @@ extern "C" int i3g_markers_create(long nx, long ny, long nz, const double *gx, c
 	m->cap = capacity;
 	int err = 0;
 	if (hipStreamCreate(&m->s) != hipSuccess) { delete m; return I3G_EDEVICE; }
+	if (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess) {
+		i3g_markers_destroy(m);
+		return I3G_EDEVICE;
+	}
 	m->gx = mk_alloc<double>(nx, &err); m->gy = mk_alloc<double>(ny, &err); m->gz = mk_alloc<double>(nz, &err);
 	long c = capacity;
 	m->x = mk_alloc<float>(c, &err); m->y = mk_alloc<float>(c, &err); m->z = mk_alloc<float>(c, &err);
@@ -752,12 +768,16 @@ extern "C" int i3g_markers_destroy(i3g_markers *m)
 		      m->vel, m->rocks, m->bad, m->tbc_p, m->tbc_c, m->tbc_k, m->tk1, m->tnod, m->dtk, m->sn, m->tin};
 	for (void *p : ps)
 		if (p) hipFree(p);
+	if (m->ev0) hipEventDestroy(m->ev0);
+	if (m->ev1) hipEventDestroy(m->ev1);
 	if (m->s) hipStreamDestroy(m->s);
 	delete m;
 	return I3G_OK;
 }
 
 extern "C" long i3g_markers_count(const i3g_markers *m) { return m ? m->n : I3G_EINVAL; }
+
+extern "C" double i3g_markers_last_ms(const i3g_markers *m) { return m ? (double)m->last_ms : -1.0; }
 
 extern "C" int i3g_markers_upload(i3g_markers *m, long n, const float *x, const float *y, const float *z,
 				  const float *tk, const float *eta, const signed char *type, const int *id)
@@ -814,14 +834,17 @@ static int cell_index(i3g_markers *m, int with_count)
 extern "C" int i3g_markers_cell_index(i3g_markers *m, int *m1, int *m2, int *m3, int *key)
 {
 	if (!m) return I3G_EINVAL;
+	mark_begin(m);
 	int rc = cell_index(m, 0);
 	if (rc) return rc;
+	mark_end(m);
 	long n = m->n;
 	if (m1) MCHK(hipMemcpyAsync(m1, m->c1, n * sizeof(int), hipMemcpyDeviceToHost, m->s));
 	if (m2) MCHK(hipMemcpyAsync(m2, m->c2, n * sizeof(int), hipMemcpyDeviceToHost, m->s));
 	if (m3) MCHK(hipMemcpyAsync(m3, m->c3, n * sizeof(int), hipMemcpyDeviceToHost, m->s));
 	if (key) MCHK(hipMemcpyAsync(key, m->key, n * sizeof(int), hipMemcpyDeviceToHost, m->s));
 	MCHK(hipStreamSynchronize(m->s));
+	mark_read(m);
 	return I3G_OK;
 }
 
@@ -837,6 +860,7 @@ extern "C" int i3g_markers_sort(i3g_markers *m, long *order)
 {
 	if (!m) return I3G_EINVAL;
 	long n = m->n, N = m->N;
+	mark_begin(m);
 	int rc = cell_index(m, 1);
 	if (rc) return rc;
 	long nb = (N + SCAN_B - 1) / SCAN_B;
@@ -861,6 +885,7 @@ extern "C" int i3g_markers_sort(i3g_markers *m, long *order)
 		if (rc) return rc;
 	}
 	MCHK(hipGetLastError());
+	mark_end(m);
 	if (order && n) {
 		std::vector<int> p(n);
 		MCHK(hipMemcpyAsync(p.data(), m->perm, n * sizeof(int), hipMemcpyDeviceToHost, m->s));
@@ -868,6 +893,7 @@ extern "C" int i3g_markers_sort(i3g_markers *m, long *order)
 		for (long i = 0; i < n; i++) order[i] = p[i];
 	}
 	MCHK(hipStreamSynchronize(m->s));
+	mark_read(m);
 	m->sorted = 1;
 	return I3G_OK;
 }
@@ -897,6 +923,7 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 	MCHK(hipMemcpyAsync(m->pr, pr, N * sizeof(double), hipMemcpyHostToDevice, m->s));
 	for (int q = 0; q < 11; q++)
 		MCHK(hipMemcpyAsync(m->fld + q * N, host[q], N * sizeof(float), hipMemcpyHostToDevice, m->s));
+	mark_begin(m);
 	if (n)
 		hipLaunchKernelGGL(k_marker_props, g1(n), dim3(256), 0, m->s, n, (const float *)m->x, (const float *)m->y,
 				   (const float *)m->z, (const float *)m->tk, m->eta, (const signed char *)m->type,
@@ -917,10 +944,12 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 				   (const float *)m->tbc_k, (const float *)m->tk1, m->fld + 6 * N);
 	}
 	MCHK(hipGetLastError());
+	mark_end(m);
 	int bad = 0;
 	MCHK(hipMemcpyAsync(&bad, m->bad, sizeof(int), hipMemcpyDeviceToHost, m->s));
 	MCHK(hipStreamSynchronize(m->s));
 	if (bad) return I3G_EINVAL;
+	mark_read(m);
 	for (int q = 0; q < 11; q++)
 		MCHK(hipMemcpyAsync(host[q], m->fld + q * N, N * sizeof(float), hipMemcpyDeviceToHost, m->s));
 	MCHK(hipStreamSynchronize(m->s));
@@ -982,6 +1011,7 @@ extern "C" int i3g_markers_update_tk(i3g_markers *m, const float *tk, const floa
 	for (int q = 0; q < 5; q++)
 		MCHK(hipMemcpyAsync(m->tnod + q * N, src[q], N * sizeof(float), hipMemcpyHostToDevice, m->s));
 	const float *f = m->tnod;
+	mark_begin(m);
 	if (n)
 		hipLaunchKernelGGL(k_tk_marker, g1(n), dim3(256), 0, m->s, n, (const float *)m->x, (const float *)m->y,
 				   (const float *)m->z, (const signed char *)m->type, m->tk, (const double *)m->gx,
@@ -998,7 +1028,9 @@ extern "C" int i3g_markers_update_tk(i3g_markers *m, const float *tk, const floa
 				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, (const double *)m->sn);
 	}
 	MCHK(hipGetLastError());
+	mark_end(m);
 	MCHK(hipStreamSynchronize(m->s));
+	mark_read(m);
 	return I3G_OK;
 }
 
@@ -1014,13 +1046,16 @@ extern "C" int i3g_markers_advect_rk4(i3g_markers *m, double dt, const float *vx
 	MCHK(hipMemcpyAsync(m->vel, vx, N * sizeof(float), hipMemcpyHostToDevice, m->s));
 	MCHK(hipMemcpyAsync(m->vel + N, vy, N * sizeof(float), hipMemcpyHostToDevice, m->s));
 	MCHK(hipMemcpyAsync(m->vel + 2 * N, vz, N * sizeof(float), hipMemcpyHostToDevice, m->s));
+	mark_begin(m);
 	if (n)
 		hipLaunchKernelGGL(k_rk4, g1(n), dim3(256), 0, m->s, n, m->x, m->y, m->z, m->tk, m->eta,
 				   (const signed char *)m->type, (const float *)m->vel, (const float *)(m->vel + N),
 				   (const float *)(m->vel + 2 * N), (const double *)m->gx, (const double *)m->gy,
 				   (const double *)m->gz, m->nx, m->ny, m->nz, dt);
 	MCHK(hipGetLastError());
+	mark_end(m);
 	MCHK(hipStreamSynchronize(m->s));
+	mark_read(m);
 	m->sorted = 0;
 	return I3G_OK;
 }

@@ -242,7 +242,8 @@ typedef struct {
 	int sweeps;        /* total over the sub-steps */
 	double heatsum;    /* final heat norm (K) */
 	double bondsum;    /* final boundary-row norm (K) */
-	double seconds;
+	double seconds;    /* wall time of the call */
+	double device_seconds; /* device time of the solve kernels (host copies excluded) */
 } i3g_heat_stats;
 
 /* bondm/bondv/bondn of the setup are read for the T block (bondm +
@@ -298,6 +299,10 @@ int i3g_markers_upload(i3g_markers *m, long n, const float *x, const float *y, c
 int i3g_markers_download(i3g_markers *m, float *x, float *y, float *z, float *tk, float *eta,
 			 signed char *type, int *id);
 long i3g_markers_count(const i3g_markers *m);
+/* Device time (ms) of the kernels of the last cell_index, sort, to_nodes,
+ * update_tk or advect_rk4 call (HIP events on the set's stream; host
+ * copies excluded). */
+double i3g_markers_last_ms(const i3g_markers *m);
 
 /* Cell of every marker by the reference's binary searches m1serch/m2serch/
  * m3serch (mark3mg.c:1021-1087) after clamping the coordinate to the
