@@ -234,16 +234,17 @@ def main():
     learned = {}
 
     def run_learned(timing=False):
-        """residual-optimal learned guess (device inference plus host-side
-        combination; its wall time, PCIe copies included, is added to the
-        step), fallback selection, mixed solve"""
+        """residual-optimal learned start (network inference and the
+        combination on the device; the wall time of building it, its host
+        copies included, is added to the step), fallback selection, mixed
+        solve"""
         t = time.time()
-        g, info = W.learned_guess(S, cfg, ws)
+        info = W.learned_start(S, cfg, ws)
         tg = time.time() - t
         S.set_timing(timing)
-        st, which, (rg, rf) = W.solve_with_guess(S, cfg, g, check_conv=True, precision=32, rtol=1e-8,
-                                                 krylov=a.krylov, max_outer=20, verbose=a.verbose)
-        learned.update(info, start=which, rel0_net=rg, rel0_zero=rf, guess_wall_s=round(tg, 4))
+        st = S.solve(check_conv=True, precision=32, rtol=1e-8, krylov=a.krylov, max_outer=20,
+                     verbose=a.verbose)
+        learned.update(info, guess_wall_s=round(tg, 4))
         return st, tg
 
     if a.profile_vcycles:
@@ -270,8 +271,9 @@ def main():
         times.append(st.seconds + tg)
         vcyc.append(st.vcycles)
         print(f"# step {k}: {st.seconds + tg:.3f}s vcycles={st.vcycles} outer={st.outer_iters} rel={st.rel_res:.2e}"
-              + (f" start={learned['start']} r0 net {learned['rel0_net']:.2e} zero {learned['rel0_zero']:.2e}"
-                 if ws is not None else ""), file=sys.stderr, flush=True)
+              + (f" start={learned['start']} r0 learned {learned['rel_learned']:.2e} zero "
+                 f"{learned['rel_fallback']:.2e} guess {tg:.2f}s" if ws is not None else ""),
+              file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.time() - t0
     kst = S.kernel_stats()

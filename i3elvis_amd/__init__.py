@@ -207,6 +207,19 @@ class Stokes:
             check(rc, "stokes_solve")
         return st
 
+    def combine_start(self, base, dirs):
+        """resident iterate = base + sum a_i dirs[i] with residual-optimal a;
+        vectors are [P, vx, vy, vz] lists whose entries may be None.
+        Returns (a, rel)."""
+        vecs = [base] + list(dirs)
+        keep = [None if p is None else np.ascontiguousarray(p, np.float64) for v in vecs for p in v]
+        arr = (C.c_void_p * len(keep))(*[None if p is None else p.ctypes.data for p in keep])
+        a = np.zeros(max(1, len(dirs)))
+        rel = C.c_double()
+        check(lib().i3g_stokes_combine_start(self.ctx, C.c_int(len(dirs)), arr, _p(a), C.byref(rel)),
+              "combine_start")
+        return a[:len(dirs)], rel.value
+
     def strain_stress(self):
         """(eps, sig): 6 float planes each (xx yy zz xy xz yz) of the resident solution"""
         out = [np.zeros(self.N, np.float32) for _ in range(12)]

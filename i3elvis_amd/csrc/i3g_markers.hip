@@ -98,10 +98,25 @@ __device__ __forceinline__ int serch(const double *g, int n, double v)
 	return lo;
 }
 
+/* The grid lines of the three axes staged in LDS (dynamic shared memory of
+ * nx+ny+nz doubles): every marker's binary searches and interpolation
+ * weights read them many times at lane-divergent addresses.  Called by all
+ * threads of the group before any early exit. */
+__device__ __forceinline__ void stage_lines(double *sg, const double *gx, const double *gy, const double *gz, int nx,
+					    int ny, int nz)
+{
+	for (int i = threadIdx.x; i < nx + ny + nz; i += blockDim.x)
+		sg[i] = i < nx ? gx[i] : (i < nx + ny ? gy[i - nx] : gz[i - nx - ny]);
+	__syncthreads();
+}
+
 __global__ void k_cell_index(long n, const float *__restrict__ x, const float *__restrict__ y,
 			     const float *__restrict__ z, const double *gx, const double *gy, const double *gz,
 			     int nx, int ny, int nz, int *key, int *c1, int *c2, int *c3, int *cnt)
 {
+	extern __shared__ double sg[];
+	stage_lines(sg, gx, gy, gz, nx, ny, nz);
+	gx = sg; gy = sg + nx; gz = sg + nx + ny;
 	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= n) return;
 	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
@@ -490,6 +505,9 @@ __global__ void k_tk_marker(long n, const float *__restrict__ x, const float *__
 			    const float *__restrict__ ro, const float *__restrict__ cp, double heatdif, double dt,
 			    double zdeep, double tdeep, double *dtk, uint8_t *tin)
 {
+	extern __shared__ double sg[];
+	stage_lines(sg, gx, gy, gz, nx, ny, nz);
+	gx = sg; gy = sg + nx; gz = sg + nx + ny;
 	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	tin[i] = 0;
@@ -594,6 +612,9 @@ __global__ void k_tk_anti(long n, const float *__restrict__ x, const float *__re
 			  const uint8_t *__restrict__ tin, float *markk, const double *gx, const double *gy,
 			  const double *gz, int nx, int ny, int nz, const double *__restrict__ sn)
 {
+	extern __shared__ double sg[];
+	stage_lines(sg, gx, gy, gz, nx, ny, nz);
+	gx = sg; gy = sg + nx; gz = sg + nx + ny;
 	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n || !tin[i]) return;
 	int cell[3];
@@ -674,6 +695,9 @@ __global__ void k_rk4(long n, float *x, float *y, float *z, float *tk, float *et
 		      const float *vx, const float *vy, const float *vz, const double *gx, const double *gy,
 		      const double *gz, int nx, int ny, int nz, double dt)
 {
+	extern __shared__ double sg[];
+	stage_lines(sg, gx, gy, gz, nx, ny, nz);
+	gx = sg; gy = sg + nx; gz = sg + nx + ny;
 	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= n) return;
 	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
@@ -717,12 +741,14 @@ template <typename T> static T *mk_alloc(size_t n, int *err)
 }
 
 static inline dim3 g1(long n) { return dim3((unsigned)((n + 255) / 256)); }
+static inline unsigned lines_bytes(const i3g_markers *m) { return (unsigned)((m->nx + m->ny + m->nz) * sizeof(double)); }
 
 extern "C" int i3g_markers_create(long nx, long ny, long nz, const double *gx, const double *gy, const double *gz,
 				  long capacity, i3g_markers **out)
 {
 	if (!out || !gx || !gy || !gz || nx < 3 || ny < 3 || nz < 3 || capacity < 1 || capacity > 2000000000L)
 		return I3G_EINVAL;
+	if (nx + ny + nz > 6144) return I3G_ECAP; /* grid lines staged in 48 KB of LDS */
 	i3g_markers *m = new i3g_markers();
 	m->nx = (int)nx; m->ny = (int)ny; m->nz = (int)nz;
 	m->N = nx * ny * nz;
@@ -823,7 +849,7 @@ static int cell_index(i3g_markers *m, int with_count)
 {
 	if (with_count) MCHK(hipMemsetAsync(m->cnt, 0, m->N * sizeof(int), m->s));
 	if (m->n)
-		hipLaunchKernelGGL(k_cell_index, g1(m->n), dim3(256), 0, m->s, m->n, (const float *)m->x,
+		hipLaunchKernelGGL(k_cell_index, g1(m->n), dim3(256), lines_bytes(m), m->s, m->n, (const float *)m->x,
 				   (const float *)m->y, (const float *)m->z, (const double *)m->gx,
 				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, m->key, m->c1,
 				   m->c2, m->c3, with_count ? m->cnt : (int *)nullptr);
@@ -1013,7 +1039,7 @@ extern "C" int i3g_markers_update_tk(i3g_markers *m, const float *tk, const floa
 	const float *f = m->tnod;
 	mark_begin(m);
 	if (n)
-		hipLaunchKernelGGL(k_tk_marker, g1(n), dim3(256), 0, m->s, n, (const float *)m->x, (const float *)m->y,
+		hipLaunchKernelGGL(k_tk_marker, g1(n), dim3(256), lines_bytes(m), m->s, n, (const float *)m->x, (const float *)m->y,
 				   (const float *)m->z, (const signed char *)m->type, m->tk, (const double *)m->gx,
 				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, f, f + N, f + 2 * N,
 				   f + 3 * N, f + 4 * N, heatdif, timestep, zdeep, tdeep, m->dtk, m->tin);
@@ -1023,7 +1049,7 @@ extern "C" int i3g_markers_update_tk(i3g_markers *m, const float *tk, const floa
 				   (const double *)m->gy, (const double *)m->gz, (const int *)m->start, (const int *)m->cnt,
 				   (const int *)m->id, (const float *)m->x, (const float *)m->y, (const float *)m->z,
 				   (const uint8_t *)m->tin, (const double *)m->dtk, m->sn);
-		hipLaunchKernelGGL(k_tk_anti, g1(n), dim3(256), 0, m->s, n, (const float *)m->x, (const float *)m->y,
+		hipLaunchKernelGGL(k_tk_anti, g1(n), dim3(256), lines_bytes(m), m->s, n, (const float *)m->x, (const float *)m->y,
 				   (const float *)m->z, (const uint8_t *)m->tin, m->tk, (const double *)m->gx,
 				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, (const double *)m->sn);
 	}
@@ -1048,7 +1074,7 @@ extern "C" int i3g_markers_advect_rk4(i3g_markers *m, double dt, const float *vx
 	MCHK(hipMemcpyAsync(m->vel + 2 * N, vz, N * sizeof(float), hipMemcpyHostToDevice, m->s));
 	mark_begin(m);
 	if (n)
-		hipLaunchKernelGGL(k_rk4, g1(n), dim3(256), 0, m->s, n, m->x, m->y, m->z, m->tk, m->eta,
+		hipLaunchKernelGGL(k_rk4, g1(n), dim3(256), lines_bytes(m), m->s, n, m->x, m->y, m->z, m->tk, m->eta,
 				   (const signed char *)m->type, (const float *)m->vel, (const float *)(m->vel + N),
 				   (const float *)(m->vel + 2 * N), (const double *)m->gx, (const double *)m->gy,
 				   (const double *)m->gz, m->nx, m->ny, m->nz, dt);

@@ -1370,13 +1370,16 @@ extern "C" void i3g_stokes_default_params(i3g_stokes_params *p)
 	p->krylov = 400;
 	p->max_vcycles = 2000;
 	/* Sweep counts: the reference's 8/16/16/32 per level and 64 on the
-	 * coarsest (mode.t3c:3030-3031), with the finest level doubled; the
-	 * relaxation factors differ from mode.t3c (p0koef 0.3, p1koef 0.1,
-	 * v0koef 0.7) because the V-cycle here preconditions a Krylov solver:
-	 * measured on the 257^3 subduction case (10^6 contrast), p0koef 0.6 /
-	 * p1koef 1.0 / v0koef 0.85 with 16+16 fine sweeps gave the fastest
-	 * residual reduction per V-cycle; v0koef 1.0 stalls there. */
-	int pre[4] = {16, 16, 16, 32}, post[4] = {16, 16, 16, 32};
+	 * coarsest (mode.t3c:3030-3031), with the finest and second level
+	 * swapped to 16 and 8; the relaxation factors differ from mode.t3c
+	 * (p0koef 0.3, p1koef 0.1, v0koef 0.7) because the V-cycle here
+	 * preconditions a Krylov solver.  Measured on the 257^3 subduction case
+	 * (10^6 contrast): p0koef 0.6 / p1koef 1.0 / v0koef 0.85 (v0koef 1.0
+	 * stalls); finest-level sweeps 8, 12, 16, 20, 24, 32 took 1538, 687, 477,
+	 * 457, 447, 427 V-cycles, 16 being the fastest in time; 8 instead of 16
+	 * sweeps on the second level cost 2% more V-cycles for 7% cheaper
+	 * V-cycles. */
+	int pre[4] = {16, 8, 16, 32}, post[4] = {16, 8, 16, 32};
 	for (int l = 0; l < I3G_MAXLEV; l++) {
 		p->nu_pre[l] = l < 4 ? pre[l] : 32;
 		p->nu_post[l] = l < 4 ? post[l] : 32;
@@ -1488,6 +1491,112 @@ extern "C" int i3g_stokes_set_solution(i3g_ctx *c, const double *P, const double
 	if (c->coef_ready) bc_enforce<double>(c, L, c->sol, 0);
 	HCHK(hipGetLastError());
 	HCHK(hipStreamSynchronize(c->s));
+	return I3G_OK;
+}
+
+/* host planes (NULL = zero) into a device 4N vector, missing unknowns
+ * zeroed, boundary rows applied (homog: partners only) */
+static int upload_planes(i3g_ctx *c, const double *const *pl, double *x, int homog)
+{
+	Level &L = c->L[0];
+	long N = L.N;
+	for (int q = 0; q < 4; q++) {
+		if (pl[q]) HCHK(hipMemcpyAsync(x + q * N, pl[q], N * sizeof(double), hipMemcpyHostToDevice, c->s));
+		else HCHK(hipMemsetAsync(x + q * N, 0, N * sizeof(double), c->s));
+	}
+	hipLaunchKernelGGL(k_zero_missing, lin_grid(N), dim3(256), 0, c->s, x, N, (const uint8_t *)L.flags);
+	bc_enforce<double>(c, L, x, homog);
+	HCHK(hipGetLastError());
+	return I3G_OK;
+}
+
+extern "C" int i3g_stokes_combine_start(i3g_ctx *c, int k, const double *const *planes, double *a_out,
+					double *rel_out)
+{
+	if (!c || !c->coef_ready || k < 0 || k > 4 || !planes) return I3G_EINVAL;
+	Level &L = c->L[0];
+	long N = L.N, n4 = 4 * N;
+	int err = 0;
+	double *dirs = k ? dalloc<double>((size_t)k * n4, &err) : nullptr;
+	double *ys = dalloc<double>((size_t)(k + 1) * n4, &err);
+	if (err) {
+		if (dirs) hipFree(dirs);
+		if (ys) hipFree(ys);
+		return err;
+	}
+	int rc = upload_planes(c, planes, c->sol, 0);
+	double nrm[2] = {0, 1}, a[4] = {0, 0, 0, 0};
+	if (!rc) rc = resid64(c, c->sol, L.r64, nrm);
+	if (!rc) {
+		dim3 lg = lin_grid(n4);
+		/* y0 = W (b - A x0); y_i = W A d_i */
+		hipLaunchKernelGGL((k_wscale<double, double>), lg, dim3(256), 0, c->s, ys, (const double *)L.r64,
+				   (const double *)L.wP64, (const uint8_t *)L.flags, N, 0, 1.0);
+		for (int i = 0; i < k && !rc; i++) {
+			double *d = dirs + (long)i * n4;
+			rc = upload_planes(c, planes + 4 * (i + 1), d, 1);
+			if (!rc)
+				hipLaunchKernelGGL((k_residual<double, double, 1>), plane_grid(L.nx, L.ny, L.nz), dim3(256), 0,
+						   c->s, view<double>(L), (const double *)d, (const double *)nullptr,
+						   ys + (long)(i + 1) * n4, (const int *)L.bcid, bcview<double>(L),
+						   (double *)nullptr);
+		}
+		if (!rc && k > 0) {
+			/* normal equations G a = g of min || y0 - sum a_i y_i || */
+			double G[4][5];
+			for (int i = 0; i < k; i++) {
+				for (int j = i; j < k; j++)
+					G[i][j] = G[j][i] = dot(c, ys + (long)(i + 1) * n4, ys + (long)(j + 1) * n4, n4);
+				G[i][k] = dot(c, ys, ys + (long)(i + 1) * n4, n4);
+			}
+			/* Gaussian elimination with partial pivoting */
+			int ok = 1;
+			for (int col = 0; col < k && ok; col++) {
+				int piv = col;
+				for (int r = col + 1; r < k; r++)
+					if (fabs(G[r][col]) > fabs(G[piv][col])) piv = r;
+				if (!(fabs(G[piv][col]) > 1e-300)) { ok = 0; break; }
+				if (piv != col)
+					for (int q = 0; q <= k; q++) std::swap(G[piv][q], G[col][q]);
+				for (int r = col + 1; r < k; r++) {
+					double f = G[r][col] / G[col][col];
+					for (int q = col; q <= k; q++) G[r][q] -= f * G[col][q];
+				}
+			}
+			if (ok)
+				for (int i = k - 1; i >= 0; i--) {
+					double v = G[i][k];
+					for (int j = i + 1; j < k; j++) v -= G[i][j] * a[j];
+					a[i] = v / G[i][i];
+					if (!std::isfinite(a[i])) ok = 0;
+				}
+			if (!ok) a[0] = a[1] = a[2] = a[3] = 0;
+			for (int i = 0; i < k; i++)
+				if (a[i] != 0)
+					hipLaunchKernelGGL((k_axpy<double, double>), lg, dim3(256), 0, c->s, c->sol,
+							   (const double *)(dirs + (long)i * n4), a[i], n4);
+			double nrm2[2];
+			rc = resid64(c, c->sol, L.r64, nrm2);
+			if (!rc && !(nrm2[0] <= nrm[0])) {
+				/* rounding made it worse: back to the base */
+				for (int i = 0; i < k; i++)
+					if (a[i] != 0)
+						hipLaunchKernelGGL((k_axpy<double, double>), lg, dim3(256), 0, c->s, c->sol,
+								   (const double *)(dirs + (long)i * n4), -a[i], n4);
+				a[0] = a[1] = a[2] = a[3] = 0;
+			} else if (!rc) {
+				nrm[0] = nrm2[0];
+			}
+		}
+	}
+	hipError_t e = hipStreamSynchronize(c->s);
+	if (dirs) hipFree(dirs);
+	hipFree(ys);
+	if (rc) return rc;
+	if (e != hipSuccess) return I3G_EDEVICE;
+	for (int i = 0; i < k; i++)
+		if (a_out) a_out[i] = a[i];
+	if (rel_out) *rel_out = nrm[0] / (nrm[1] > 0 ? nrm[1] : 1.0);
 	return I3G_OK;
 }
 

@@ -70,9 +70,10 @@ def main():
         graw = ws.guess(cfg)
         gerr = [float(np.linalg.norm(a1 - b1) / max(np.linalg.norm(b1), 1e-300)) for a1, b1 in zip(graw, ref)]
         t = time.time()
-        g, info = W.learned_guess(S, cfg, ws)
+        info = W.learned_start(S, cfg, ws)
         tg = time.time() - t
-        st1, which, (rg, rf) = W.solve_with_guess(S, cfg, g, precision=32, rtol=1e-8)
+        which, rg, rf = info["start"], info["rel_learned"], info["rel_fallback"]
+        st1 = S.solve(precision=32, rtol=1e-8)
         # the lithostatic pressure alone, for comparison
         plith = W.lithostatic(cfg)
         S.set_solution(plith, z, z, z)

@@ -127,81 +127,27 @@ class WarmStart:
         return losses
 
 
-def best_combination(S, base, dirs):
-    """guess = base + sum_i a_i dirs[i] with the a_i minimising the FP64
-    weighted residual norm; the squared norm is a quadratic in a, fixed by
-    its values at 1 + 2k + k(k-1)/2 points (k = len(dirs)).  Returns
-    (guess, a, rel)."""
-    k = len(dirs)
-
-    def comb(a):
-        return [b + sum(ai * d[c] for ai, d in zip(a, dirs)) for c, b in enumerate(base)]
-
-    def f(a):
-        nr = S.residual(*comb(a))[1]
-        return nr[0] ** 2, nr[1]
-
-    f0, bn = f([0.0] * k)
-    g = np.zeros(k)
-    H = np.zeros((k, k))
-    e = np.eye(k)
-    fp = [f(list(e[i]))[0] for i in range(k)]
-    fm = [f(list(-e[i]))[0] for i in range(k)]
-    for i in range(k):
-        g[i] = (fp[i] - fm[i]) / 2
-        H[i, i] = fp[i] + fm[i] - 2 * f0
-    for i in range(k):
-        for j in range(i + 1, k):
-            fij = f(list(e[i] + e[j]))[0]
-            H[i, j] = H[j, i] = fij - f0 - g[i] - g[j] - (H[i, i] + H[j, j]) / 2
-    # f(a) = f0 + g.a + a.H.a / 2
-    try:
-        a = -np.linalg.solve(H, g)
-    except np.linalg.LinAlgError:
-        a = np.zeros(k)
-    guess = comb(list(a))
-    nr = S.residual(*guess)[1]
-    if not nr[0] ** 2 < f0:
-        a, guess, nr = np.zeros(k), comb([0.0] * k), (np.sqrt(f0), bn)
-    return guess, a, nr[0] / nr[1]
-
-
-def learned_guess(S, cfg, ws):
-    """The network's guess made residual-optimal: lithostatic pressure plus
-    the network's dynamic pressure and its velocities, each scaled by the
-    coefficient that minimises the FP64 residual.  Returns (guess, info)."""
-    n = cfg["nx"] * cfg["ny"] * cfg["nz"]
-    z = np.zeros(n)
-    plith = lithostatic(cfg)
+def learned_start(S, cfg, ws, fallback=None):
+    """Set the solver's start from the network: the lithostatic pressure
+    plus the network's dynamic pressure and velocities, each scaled by the
+    coefficient that minimises the FP64 residual (computed on the device,
+    i3g_stokes_combine_start).  The fallback (the previous step's solution;
+    zero when None) is used instead when its residual is smaller.  Returns
+    info (coefficients, initial residuals, which start was taken)."""
+    fb = fallback if fallback is not None else [None] * 4
+    _, relf = S.combine_start(fb, [])
     y = ws.net.infer(features(cfg)).astype(np.float64)
-    dP = [y[:, 3] * ws.pscale, z, z, z]
-    dv = [z, y[:, 0] * ws.vscale, y[:, 1] * ws.vscale, y[:, 2] * ws.vscale]
-    base = [plith, z, z, z]
-    rl = S.residual(*base)[1]
-    guess, a, rel = best_combination(S, base, [dP, dv])
-    return guess, dict(rel_lithostatic=rl[0] / rl[1], coef_dP=float(a[0]), coef_v=float(a[1]), rel_learned=rel)
-
-
-def initial_rel(S, sol):
-    """relative FP64 residual of a guess as the solver starts from it
-    (missing unknowns zeroed, boundary rows applied); returns (rel, guess)"""
-    S.set_solution(*sol)
-    g = S.get_solution()
-    nr = S.residual(*g)[1]
-    return nr[0] / nr[1], g
-
-
-def solve_with_guess(S, cfg, guess, fallback=None, **kw):
-    """Start the solve from the better of the guess and the fallback
-    (zero when None) by FP64 residual; returns (stats, which, rel0s)."""
-    n = cfg["nx"] * cfg["ny"] * cfg["nz"]
-    fb = fallback if fallback is not None else [np.zeros(n)] * 4
-    relg, g = initial_rel(S, guess)
-    relf, f = initial_rel(S, fb)
-    use = g if relg < relf else f
-    S.set_solution(*use)
-    st = S.solve(**kw)
-    return st, ("network" if relg < relf else "fallback"), (relg, relf)
+    base = [lithostatic(cfg), None, None, None]
+    dP = [y[:, 3] * ws.pscale, None, None, None]
+    dv = [None, y[:, 0] * ws.vscale, y[:, 1] * ws.vscale, y[:, 2] * ws.vscale]
+    _, rel_lith = S.combine_start(base, [])
+    a, rel = S.combine_start(base, [dP, dv])
+    start = "network"
+    if relf < rel:
+        S.combine_start(fb, [])
+        start = "fallback"
+    return dict(coef_dP=float(a[0]), coef_v=float(a[1]), rel_learned=rel, rel_lithostatic=rel_lith,
+                rel_fallback=relf, start=start)
 
 
 def timed(f, *a, **k):

@@ -186,6 +186,19 @@ int i3g_stokes_get_solution(i3g_ctx *ctx, double *P, double *vx, double *vy,
 int i3g_stokes_set_solution(i3g_ctx *ctx, const double *P, const double *vx,
 			    const double *vy, const double *vz);
 
+/* Residual-optimal start: the resident iterate becomes
+ *   x = base + sum_i a_i d_i   (i < k <= 4)
+ * with the a_i minimising the weighted FP64 residual (the metric of
+ * i3g_stokes_residual), computed on the device from one upload of each
+ * vector.  planes: (k+1)*4 host pointers (P, vx, vy, vz of base, then of
+ * each d_i; NULL = zero plane).  Missing unknowns are zeroed and boundary
+ * rows applied (homogeneous for the d_i).  a_out (k entries) and rel_out
+ * (relative residual of the start) may be NULL.  No counterpart in the
+ * reference: it serves the learned warm start (the start is never worse
+ * than base). */
+int i3g_stokes_combine_start(i3g_ctx *ctx, int k, const double *const *planes, double *a_out,
+			     double *rel_out);
+
 /* Strain rates and stresses of the resident solution: the value mode of
  * sxxcalc..syzcalc (move3mg.c:3-710) as in the loop at the end of vpiterate
  * (move3mg.c:1691-1764), from the velocities rounded to float (the

@@ -73,3 +73,27 @@ def test_solve_matches_reference_converged(gpu, n, prec):
         print(f"  field {c}: rel L2 err {err:.3e}", flush=True)
         assert err <= 1e-6, (c, err)
     S.close()
+
+
+def test_combine_start_recovers_solution(gpu):
+    """i3g_stokes_combine_start: with the converged solution and a random
+    vector as directions from a zero base, the residual-optimal coefficients
+    are (1, 0) and the start is the solution."""
+    import i3elvis_amd as I
+    n = 17
+    cfg = I.make_config(0, n, n, n, seed=1)
+    S = I.Stokes(cfg)
+    S.set_coefficients()
+    z = np.zeros(n ** 3)
+    S.set_solution(z, z, z, z)
+    S.solve(precision=64, rtol=1e-11)
+    xs = S.get_solution()
+    rng = np.random.default_rng(3)
+    rnd = [rng.standard_normal(n ** 3) * s for s in (1e6, 1e-10, 1e-10, 1e-10)]
+    a, rel = S.combine_start([None] * 4, [xs, rnd])
+    print("coefficients", a, "rel", rel, flush=True)
+    assert abs(a[0] - 1) < 1e-6 and abs(a[1]) < 1e-6 * max(1.0, abs(a[0]))
+    assert rel < 1e-9
+    _, rel0 = S.combine_start([None] * 4, [])
+    assert abs(rel0 - 1.0) < 1e-12
+    S.close()

@@ -32,13 +32,14 @@ def test_learned_guess_never_worse_and_gain(gpu):
         S.set_solution(z, z, z, z)
         st0 = S.solve(precision=32, rtol=1e-8)
         ref = S.get_solution()
-        g, info = W.learned_guess(S, cfg, ws)
-        st1, which, (rg, rf) = W.solve_with_guess(S, cfg, g, precision=32, rtol=1e-8)
+        info = W.learned_start(S, cfg, ws)
+        st1 = S.solve(precision=32, rtol=1e-8)
         sol = S.get_solution()
-        print(f"seed {seed}: zero {st0.vcycles} V-cycles, learned start ({which}, r0 {rg:.2e} vs {rf:.2e}, "
-              f"lithostatic r0 {info['rel_lithostatic']:.2e}, coef P {info['coef_dP']:.3f} v {info['coef_v']:.3f}) "
-              f"{st1.vcycles} V-cycles", flush=True)
-        assert st1.converged and rg <= rf
+        print(f"seed {seed}: zero {st0.vcycles} V-cycles, learned start ({info['start']}, r0 "
+              f"{info['rel_learned']:.2e} vs {info['rel_fallback']:.2e}, lithostatic r0 {info['rel_lithostatic']:.2e}, "
+              f"coef P {info['coef_dP']:.3f} v {info['coef_v']:.3f}) {st1.vcycles} V-cycles", flush=True)
+        assert st1.converged and min(info['rel_learned'], info['rel_fallback']) <= info['rel_fallback']
+        assert info['rel_learned'] <= info['rel_lithostatic'] * (1 + 1e-12)
         for a, b in zip(sol[1:], ref[1:]):
             assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
         zs.append(st0.vcycles)
