@@ -277,6 +277,9 @@ struct i3g_ctx {
 	 * 0 arithmetic as the reference, 1 geometric, 2 harmonic
 	 * (I3G_COARSE_VISC selects) */
 	int coarse_visc = 0;
+	/* velocity smoother with all row loads issued first (I3G_SMOOTH_ILP=0
+	 * selects the plain kernel) */
+	int smooth_ilp = 1;
 	float *st12 = nullptr;   /* strain rates and stresses, 12 level-0 planes */
 };
 
@@ -654,6 +657,8 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 	{
 		const char *e = getenv("I3G_XCD_REMAP");
 		if (e && e[0] == '0') c->xcd_remap = 0;
+		e = getenv("I3G_SMOOTH_ILP");
+		if (e && e[0] == '0') c->smooth_ilp = 0;
 		e = getenv("I3G_COARSE_VISC");
 		if (e && e[0] >= '0' && e[0] <= '2') c->coarse_visc = e[0] - '0';
 	}
@@ -822,8 +827,18 @@ static void sweep(i3g_ctx *c, int l, T *x, const T *b, const i3g_stokes_params *
 	for (int comp = 1; comp <= 3; comp++)
 		for (int col = 0; col < 2; col++) {
 			Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "smooth_vel_f64_L0" : "smooth_vel_f32_L0") : "smooth_vel_coarse", by);
-			hipLaunchKernelGGL(k_smooth_vel<T>, hg, dim3(256), 0, c->s, v, x, b, comp, col, (T)p->v0koef, remap,
-					   chunks);
+			if (!c->smooth_ilp)
+				hipLaunchKernelGGL(k_smooth_vel<T>, hg, dim3(256), 0, c->s, v, x, b, comp, col, (T)p->v0koef,
+						   remap, chunks);
+			else if (comp == 1)
+				hipLaunchKernelGGL((k_smooth_vel_ilp<T, 1>), hg, dim3(256), 0, c->s, v, x, b, col,
+						   (T)p->v0koef, remap, chunks);
+			else if (comp == 2)
+				hipLaunchKernelGGL((k_smooth_vel_ilp<T, 2>), hg, dim3(256), 0, c->s, v, x, b, col,
+						   (T)p->v0koef, remap, chunks);
+			else
+				hipLaunchKernelGGL((k_smooth_vel_ilp<T, 3>), hg, dim3(256), 0, c->s, v, x, b, col,
+						   (T)p->v0koef, remap, chunks);
 		}
 	{
 		Timed tm(c, l == 0 ? "smooth_p_L0" : "smooth_p_coarse", 0);

@@ -261,6 +261,102 @@ k_smooth_vel(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int com
 	x[u] += v0koef * res / d;
 }
 
+/* The same relaxation with every operand of the row loaded up front: the
+ * plane-per-launch kernel above is latency-bound (its loads are consumed
+ * as they arrive, about eight dependent memory round trips per wave); here
+ * the flag is read first and then all 25 loads of the row (the unknowns
+ * of the stencil, the stress-point viscosities, the rhs) are issued before
+ * any is used.  Same arithmetic as row_ax; the scheduling barrier keeps the
+ * compiler from sinking the loads back to their uses.  COMP = 1, 2, 3. */
+template <typename T, int COMP>
+__global__ void __launch_bounds__(256)
+k_smooth_vel_ilp(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int colour, T v0koef, int remap,
+		 int chunks)
+{
+	int ch, k;
+	if (!block_work(remap, chunks, L.nz, &ch, &k)) return;
+	int t = ch * blockDim.x + threadIdx.x;
+	int hy = (L.ny + 1) >> 1;
+	int i = t / hy, tj = t - i * hy;
+	if (i >= L.nx) return;
+	int j = 2 * tj + ((i + k + colour) & 1);
+	if (j >= L.ny) return;
+	long N = L.N, sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = (long)k * sz + (long)i * sx + j;
+	unsigned fl = L.flags[n];
+	if (!(fl & FL_EX(COMP)) || (fl & FL_BC(COMP))) return;
+	const T *P = x, *vx = x + N, *vy = x + 2 * N, *vz = x + 3 * N;
+	long u = COMP * N + n;
+	T ax, d;
+	if (COMP == 1) {
+		T c0 = L.eC[n + sz + sx + 1], c1 = L.eC[n + sz + 1];
+		T xy1 = L.eXY[n + 1], xy0 = L.eXY[n], xz1 = L.eXZ[n + sz], xz0 = L.eXZ[n];
+		T a0 = vx[n], axp = vx[n + sx], axm = vx[n - sx], ayp = vx[n + 1], aym = vx[n - 1];
+		T azp = vx[n + sz], azm = vx[n - sz];
+		T by1 = vy[n + 1], by1x = vy[n + 1 - sx], by0 = vy[n], by0x = vy[n - sx];
+		T cz1 = vz[n + sz], cz1x = vz[n + sz - sx], cz0 = vz[n], cz0x = vz[n - sx];
+		T p1 = P[n + sz + sx + 1], p0 = P[n + sz + 1];
+		T rhs = b ? b[u] : (T)0;
+		T ihx1 = L.ihx[i], ihx0 = L.ihx[i - 1], icx = L.icx[i], icy1 = L.icy[j + 1], icy0 = L.icy[j];
+		T icz1 = L.icz[k + 1], icz0 = L.icz[k], ihy = L.ihy[j], ihz = L.ihz[k];
+		__builtin_amdgcn_sched_barrier(0);
+		T xkf = (T)2 * icx, ykf = ihy, zkf = ihz;
+		T sxx1 = (T)2 * c0 * (axp - a0) * ihx1, sxx0 = (T)2 * c1 * (a0 - axm) * ihx0;
+		T sxy1 = (T)2 * xy1 * ((ayp - a0) * icy1 + (by1 - by1x) * icx);
+		T sxy0 = (T)2 * xy0 * ((a0 - aym) * icy0 + (by0 - by0x) * icx);
+		T sxz1 = (T)2 * xz1 * ((azp - a0) * icz1 + (cz1 - cz1x) * icx);
+		T sxz0 = (T)2 * xz0 * ((a0 - azm) * icz0 + (cz0 - cz0x) * icx);
+		ax = xkf * (sxx1 - sxx0) + ykf * (sxy1 - sxy0) + zkf * (sxz1 - sxz0) - xkf * (p1 - p0);
+		d = -xkf * ((T)2 * c0 * ihx1 + (T)2 * c1 * ihx0) - ykf * ((T)2 * xy1 * icy1 + (T)2 * xy0 * icy0) -
+		    zkf * ((T)2 * xz1 * icz1 + (T)2 * xz0 * icz0);
+		x[u] = a0 + v0koef * (rhs - ax) / d;
+	} else if (COMP == 2) {
+		T c0 = L.eC[n + sz + sx + 1], c1 = L.eC[n + sz + sx];
+		T xy1 = L.eXY[n + sx], xy0 = L.eXY[n], yz1 = L.eYZ[n + sz], yz0 = L.eYZ[n];
+		T a0 = vy[n], ayp = vy[n + 1], aym = vy[n - 1], axp = vy[n + sx], axm = vy[n - sx];
+		T azp = vy[n + sz], azm = vy[n - sz];
+		T bx1 = vx[n + sx], bx1y = vx[n + sx - 1], bx0 = vx[n], bx0y = vx[n - 1];
+		T cz1 = vz[n + sz], cz1y = vz[n + sz - 1], cz0 = vz[n], cz0y = vz[n - 1];
+		T p1 = P[n + sz + sx + 1], p0 = P[n + sz + sx];
+		T rhs = b ? b[u] : (T)0;
+		T ihy1 = L.ihy[j], ihy0 = L.ihy[j - 1], icy = L.icy[j], icx1 = L.icx[i + 1], icx0 = L.icx[i];
+		T icz1 = L.icz[k + 1], icz0 = L.icz[k], ihx = L.ihx[i], ihz = L.ihz[k];
+		__builtin_amdgcn_sched_barrier(0);
+		T xkf = ihx, ykf = (T)2 * icy, zkf = ihz;
+		T syy1 = (T)2 * c0 * (ayp - a0) * ihy1, syy0 = (T)2 * c1 * (a0 - aym) * ihy0;
+		T sxy1 = (T)2 * xy1 * ((bx1 - bx1y) * icy + (axp - a0) * icx1);
+		T sxy0 = (T)2 * xy0 * ((bx0 - bx0y) * icy + (a0 - axm) * icx0);
+		T syz1 = (T)2 * yz1 * ((azp - a0) * icz1 + (cz1 - cz1y) * icy);
+		T syz0 = (T)2 * yz0 * ((a0 - azm) * icz0 + (cz0 - cz0y) * icy);
+		ax = ykf * (syy1 - syy0) + xkf * (sxy1 - sxy0) + zkf * (syz1 - syz0) - ykf * (p1 - p0);
+		d = -ykf * ((T)2 * c0 * ihy1 + (T)2 * c1 * ihy0) - xkf * ((T)2 * xy1 * icx1 + (T)2 * xy0 * icx0) -
+		    zkf * ((T)2 * yz1 * icz1 + (T)2 * yz0 * icz0);
+		x[u] = a0 + v0koef * (rhs - ax) / d;
+	} else {
+		T c0 = L.eC[n + sz + sx + 1], c1 = L.eC[n + sx + 1];
+		T xz1 = L.eXZ[n + sx], xz0 = L.eXZ[n], yz1 = L.eYZ[n + 1], yz0 = L.eYZ[n];
+		T a0 = vz[n], azp = vz[n + sz], azm = vz[n - sz], axp = vz[n + sx], axm = vz[n - sx];
+		T ayp = vz[n + 1], aym = vz[n - 1];
+		T bx1 = vx[n + sx], bx1z = vx[n + sx - sz], bx0 = vx[n], bx0z = vx[n - sz];
+		T cy1 = vy[n + 1], cy1z = vy[n + 1 - sz], cy0 = vy[n], cy0z = vy[n - sz];
+		T p1 = P[n + sz + sx + 1], p0 = P[n + sx + 1];
+		T rhs = b ? b[u] : (T)0;
+		T ihz1 = L.ihz[k], ihz0 = L.ihz[k - 1], icz = L.icz[k], icx1 = L.icx[i + 1], icx0 = L.icx[i];
+		T icy1 = L.icy[j + 1], icy0 = L.icy[j], ihx = L.ihx[i], ihy = L.ihy[j];
+		__builtin_amdgcn_sched_barrier(0);
+		T xkf = ihx, ykf = ihy, zkf = (T)2 * icz;
+		T szz1 = (T)2 * c0 * (azp - a0) * ihz1, szz0 = (T)2 * c1 * (a0 - azm) * ihz0;
+		T sxz1 = (T)2 * xz1 * ((bx1 - bx1z) * icz + (axp - a0) * icx1);
+		T sxz0 = (T)2 * xz0 * ((bx0 - bx0z) * icz + (a0 - axm) * icx0);
+		T syz1 = (T)2 * yz1 * ((cy1 - cy1z) * icz + (ayp - a0) * icy1);
+		T syz0 = (T)2 * yz0 * ((cy0 - cy0z) * icz + (a0 - aym) * icy0);
+		ax = zkf * (szz1 - szz0) + xkf * (sxz1 - sxz0) + ykf * (syz1 - syz0) - zkf * (p1 - p0);
+		d = -zkf * ((T)2 * c0 * ihz1 + (T)2 * c1 * ihz0) - xkf * ((T)2 * xz1 * icx1 + (T)2 * xz0 * icx0) -
+		    ykf * ((T)2 * yz1 * icy1 + (T)2 * yz0 * icy0);
+		x[u] = a0 + v0koef * (rhs - ax) / d;
+	}
+}
+
 /* Pressure relaxation P -= p0koef * eta_cell * (div v - b_P)
  * (move3mg.c:3105-3111); BC rows are left to k_bc. */
 template <typename T>
