@@ -220,6 +220,32 @@ class Stokes:
               "combine_start")
         return a[:len(dirs)], rel.value
 
+    def level_nodes(self, level):
+        c = self.cfg
+        h = c.get("hier") or hierarchy(c["nx"], c["ny"], c["nz"], c["multinum"])
+        return int(h.mgx[level]) * int(h.mgy[level]) * int(h.mgz[level])
+
+    def restrict(self, level, fine):
+        """vprestrict(level): fine residual planes [P, vx, vy, vz] of
+        level-1 -> coarse right-hand side planes of `level` (FP64)"""
+        keep = [np.ascontiguousarray(a, np.float64) for a in fine]
+        out = [np.zeros(self.level_nodes(level)) for _ in range(4)]
+        fin = (C.c_void_p * 4)(*[a.ctypes.data for a in keep])
+        cou = (C.c_void_p * 4)(*[a.ctypes.data for a in out])
+        check(lib().i3g_stokes_restrict(self.ctx, C.c_int(level), fin, cou), "stokes_restrict")
+        return out
+
+    def prolong(self, level, coarse, fine, v1koef, p1koef):
+        """vpprolong(level, 0): returns fine + the weighted coarse correction
+        (planes [P, vx, vy, vz], FP64)"""
+        keep = [np.ascontiguousarray(a, np.float64) for a in coarse]
+        out = [np.array(a, np.float64, copy=True) for a in fine]
+        cou = (C.c_void_p * 4)(*[a.ctypes.data for a in keep])
+        fin = (C.c_void_p * 4)(*[a.ctypes.data for a in out])
+        check(lib().i3g_stokes_prolong(self.ctx, C.c_int(level), cou, fin, C.c_double(v1koef),
+                                       C.c_double(p1koef)), "stokes_prolong")
+        return out
+
     def strain_stress(self):
         """(eps, sig): 6 float planes each (xx yy zz xy xz yz) of the resident solution"""
         out = [np.zeros(self.N, np.float32) for _ in range(12)]

@@ -280,6 +280,9 @@ struct i3g_ctx {
 	/* velocity smoother with all row loads issued first (I3G_SMOOTH_ILP=0
 	 * selects the plain kernel) */
 	int smooth_ilp = 1;
+	/* level transfer maps: 0 the reference's index rules, 1 geometric
+	 * bracketing (I3G_XFER_GEO=1) */
+	int xfer_geo = 0;
 	float *st12 = nullptr;   /* strain rates and stresses, 12 level-0 planes */
 };
 
@@ -342,6 +345,12 @@ static inline dim3 plane_grid(int nx, int ny, int nz)
 	long pl = (long)nx * ny;
 	return dim3((unsigned)((pl + 255) / 256), (unsigned)nz);
 }
+/* one thread per node and unknown kind: z of the grid is the kind */
+static inline dim3 comp_grid(int nx, int ny, int nz)
+{
+	long pl = (long)nx * ny;
+	return dim3((unsigned)((pl + 255) / 256), (unsigned)nz, 4);
+}
 static inline dim3 half_grid(int nx, int ny, int nz)
 {
 	long pl = (long)nx * ((ny + 1) / 2);
@@ -400,7 +409,31 @@ static double xw1(long m, long base, long cc, const double *gf, const double *gc
 	return 1.0 - dd;
 }
 
-static int build_axis(AxisMapDev &A, long nf, long nc, const double *gf, const double *gc)
+/* Position of the point of a transfer kind on a line set: node kinds at
+ * the line, mid kind between lines m and m+1, centre kind between m-1 and m */
+static double kind_pos(const double *g, long m, int kind)
+{
+	if (kind == 1) return (g[m] + g[m + 1]) / 2.0;
+	if (kind == 2) return (g[m - 1] + g[m]) / 2.0;
+	return g[m];
+}
+
+/* Geometric alternative to xbase: the coarse pair that brackets the fine
+ * point (largest admissible base whose coarse point is not above it).
+ * Equal to xbase wherever the coarse lines are every second fine line;
+ * differs next to the copied outer lines when (n-5) is odd, where xbase
+ * pairs a fine line with a coarse pair that does not contain it. */
+static long xbase_geo(long m, long nc, const double *gf, const double *gc, int kind)
+{
+	long lo = kind == 2 ? 1 : 0, hi = kind == 1 ? nc - 3 : nc - 2;
+	double p = kind_pos(gf, m, kind);
+	long b = lo;
+	for (long c = lo; c <= hi; c++)
+		if (kind_pos(gc, c, kind) <= p) b = c;
+	return b;
+}
+
+static int build_axis(AxisMapDev &A, long nf, long nc, const double *gf, const double *gc, int geo)
 {
 	int err = 0;
 	for (int kind = 0; kind < 4; kind++) {
@@ -412,12 +445,18 @@ static int build_axis(AxisMapDev &A, long nf, long nc, const double *gf, const d
 		for (long m = 0; m < nf; m++) {
 			if (kind == 1 && m >= nf - 1) continue;
 			if (kind == 2 && m < 1) continue;
-			long b = xbase(m, nf, nc, gf, gc, kind);
+			long b = geo ? xbase_geo(m, nc, gf, gc, kind) : xbase(m, nf, nc, gf, gc, kind);
 			if (b < 0 || b + 1 >= nc || (kind == 1 && b + 2 >= nc) || (kind == 2 && b < 1))
 				return I3G_EINVAL;
 			base[m] = (int)b;
 			wlo[m] = xw1(m, b, b, gf, gc, kind);
 			whi[m] = xw1(m, b, b + 1, gf, gc, kind);
+			if (geo) {
+				/* interpolation inside the pair, the nearest coarse
+				 * point beyond the outermost one */
+				wlo[m] = wlo[m] < 0 ? 0 : wlo[m] > 1 ? 1 : wlo[m];
+				whi[m] = whi[m] < 0 ? 0 : whi[m] > 1 ? 1 : whi[m];
+			}
 			inv[b].push_back({(int)m, wlo[m]});
 			inv[b + 1].push_back({(int)m, whi[m]});
 		}
@@ -425,7 +464,19 @@ static int build_axis(AxisMapDev &A, long nf, long nc, const double *gf, const d
 		std::vector<double> w;
 		for (long c = 0; c < nc; c++) {
 			st[c] = (int)idx.size();
-			for (auto &p : inv[c]) { idx.push_back(p.first); w.push_back(p.second); }
+			if (inv[c].size() > XFER_MAXL) return I3G_EINVAL; /* k_restrict's register list */
+			double wsum = 0, wabs = 0;
+			for (auto &p : inv[c]) {
+				idx.push_back(p.first);
+				w.push_back(p.second);
+				wsum += p.second;
+				wabs += fabs(p.second);
+			}
+			/* the node-viscosity restriction divides by the product of the
+			 * three 1-D weight sums; a vanishing sum (the outer-line rule
+			 * of an even-sized level can give weights -1, 0, 1, 0) would
+			 * make the coarse viscosity non-finite */
+			if (kind == 3 && fabs(wsum) <= 1e-9 * wabs) return I3G_EINVAL;
 		}
 		st[nc] = (int)idx.size();
 		std::vector<float> wlo32(wlo.begin(), wlo.end()), whi32(whi.begin(), whi.end()), w32(w.begin(), w.end());
@@ -661,6 +712,8 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		if (e && e[0] == '0') c->smooth_ilp = 0;
 		e = getenv("I3G_COARSE_VISC");
 		if (e && e[0] >= '0' && e[0] <= '2') c->coarse_visc = e[0] - '0';
+		e = getenv("I3G_XFER_GEO");
+		if (e && e[0] == '1') c->xfer_geo = 1;
 	}
 	if (hipStreamCreate(&c->s) != hipSuccess) { delete c; return I3G_EDEVICE; }
 	c->L.resize(c->nlev);
@@ -673,7 +726,7 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		Level &F = c->L[l], &Cl = c->L[l + 1];
 		for (int a = 0; a < 3; a++) {
 			long nf = (long)F.hg[a].size(), nc = (long)Cl.hg[a].size();
-			rc = build_axis(c->X[l].ax[a], nf, nc, F.hg[a].data(), Cl.hg[a].data());
+			rc = build_axis(c->X[l].ax[a], nf, nc, F.hg[a].data(), Cl.hg[a].data(), c->xfer_geo);
 			if (rc) { i3g_destroy(c); return rc; }
 		}
 	}
@@ -907,7 +960,7 @@ template <typename T> static void vcycle(i3g_ctx *c, int l, const i3g_stokes_par
 	Level &Cl = c->L[l + 1];
 	{
 		Timed tm(c, "restrict", 0);
-		hipLaunchKernelGGL(k_restrict<T>, plane_grid(Cl.nx, Cl.ny, Cl.nz), dim3(256), 0, c->s, view<T>(L),
+		hipLaunchKernelGGL(k_restrict<T>, comp_grid(Cl.nx, Cl.ny, Cl.nz), dim3(256), 0, c->s, view<T>(L),
 				   view<T>(Cl), xview<T>(c->X[l]), (const T *)lr<T>(L), lb<T>(Cl), lx<T>(Cl));
 	}
 	if (l == 0 && c->use_graph) coarse_cycle_graph<T>(c, p);
@@ -1507,6 +1560,54 @@ extern "C" int i3g_stokes_set_solution(i3g_ctx *c, const double *P, const double
 	HCHK(hipGetLastError());
 	HCHK(hipStreamSynchronize(c->s));
 	return I3G_OK;
+}
+
+/* ---- level transfers on host planes (FP64) ----------------------------- */
+
+static int planes_in(i3g_ctx *c, const double *const *pl, double *x, long N)
+{
+	for (int q = 0; q < 4; q++) {
+		if (!pl[q]) return I3G_EINVAL;
+		HCHK(hipMemcpyAsync(x + q * N, pl[q], N * sizeof(double), hipMemcpyHostToDevice, c->s));
+	}
+	return I3G_OK;
+}
+
+static int planes_out(i3g_ctx *c, double *const *pl, const double *x, long N)
+{
+	for (int q = 0; q < 4; q++) {
+		if (!pl[q]) return I3G_EINVAL;
+		HCHK(hipMemcpyAsync(pl[q], x + q * N, N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+	}
+	HCHK(hipStreamSynchronize(c->s));
+	return I3G_OK;
+}
+
+extern "C" int i3g_stokes_restrict(i3g_ctx *c, int level, const double *const *fine, double *const *coarse)
+{
+	if (!c || !c->coef_ready || !fine || !coarse || level < 1 || level >= c->nlev) return I3G_EINVAL;
+	Level &F = c->L[level - 1], &Cl = c->L[level];
+	int rc = planes_in(c, fine, F.r64, F.N);
+	if (rc) return rc;
+	hipLaunchKernelGGL(k_restrict<double>, comp_grid(Cl.nx, Cl.ny, Cl.nz), dim3(256), 0, c->s, view<double>(F),
+			   view<double>(Cl), xview<double>(c->X[level - 1]), (const double *)F.r64, Cl.b64, Cl.x64);
+	HCHK(hipGetLastError());
+	return planes_out(c, coarse, Cl.b64, Cl.N);
+}
+
+extern "C" int i3g_stokes_prolong(i3g_ctx *c, int level, const double *const *coarse, double *const *fine,
+				  double v1koef, double p1koef)
+{
+	if (!c || !c->coef_ready || !fine || !coarse || level < 1 || level >= c->nlev) return I3G_EINVAL;
+	Level &F = c->L[level - 1], &Cl = c->L[level];
+	int rc = planes_in(c, coarse, Cl.x64, Cl.N);
+	if (!rc) rc = planes_in(c, (const double *const *)fine, F.x64, F.N);
+	if (rc) return rc;
+	hipLaunchKernelGGL(k_prolong<double>, plane_grid(F.nx, F.ny, F.nz), dim3(256), 0, c->s, view<double>(F),
+			   view<double>(Cl), xview<double>(c->X[level - 1]), (const double *)Cl.x64, F.x64, v1koef,
+			   p1koef);
+	HCHK(hipGetLastError());
+	return planes_out(c, fine, F.x64, F.N);
 }
 
 /* host planes (NULL = zero) into a device 4N vector, missing unknowns

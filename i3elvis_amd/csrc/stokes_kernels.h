@@ -409,65 +409,95 @@ template <typename T> struct XferView {
 
 __device__ __constant__ int c_kind[4][3] = {{2, 2, 2}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
 
-/* Residual restriction in gather form; accumulation order equals the
- * reference's serial scatter (fine nodes in ascending index).  Coarse x is
- * zeroed (move3mg.c:3508-3518). */
+/* Longest inverse list of one axis (fine lines gathered by one coarse
+ * line); the host checks the maps against it. */
+#define XFER_MAXL 6
+
+/* Residual restriction in gather form, one thread per coarse unknown
+ * (component c = blockIdx.z).  The coarse line's y-list is held in
+ * registers and the fine flag/value loads of a whole y-list are issued
+ * together.  Accumulation order equals the reference's serial scatter
+ * (fine nodes in ascending index) and nothing is contracted, so in FP64
+ * the result is the reference's to the bit.  Coarse x is zeroed
+ * (move3mg.c:3508-3518). */
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_restrict(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__ rf,
 	   T *__restrict__ bc_out, T *__restrict__ xc)
 {
+#pragma clang fp contract(off)
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	int K = blockIdx.y;
+	int K = blockIdx.y, c = blockIdx.z;
 	int I = t / Cl.ny, J = t - I * Cl.ny;
 	if (I >= Cl.nx) return;
 	long Nc = Cl.N, Nf = F.N;
 	long nc = (long)K * Cl.nx * Cl.ny + (long)I * Cl.ny + J;
 	unsigned flc = Cl.flags[nc];
-	long sxf = F.ny, szf = (long)F.nx * F.ny;
-	for (int c = 0; c < 4; c++) {
-		T acc = 0, sw = 0;
-		if ((flc & FL_EX(c)) && !(flc & FL_BC(c))) {
-			int kx = c_kind[c][0], ky = c_kind[c][1], kz = c_kind[c][2];
-			int z0 = M.ax[2].istart[kz][K], z1 = M.ax[2].istart[kz][K + 1];
-			int x0 = M.ax[0].istart[kx][I], x1 = M.ax[0].istart[kx][I + 1];
-			int y0 = M.ax[1].istart[ky][J], y1 = M.ax[1].istart[ky][J + 1];
-			T ndc = (T)(float)Cl.eC[nc];
-			for (int a = z0; a < z1; a++) {
-				int kf = M.ax[2].iidx[kz][a];
-				T wz = M.ax[2].iw[kz][a];
-				for (int bq = x0; bq < x1; bq++) {
-					int ifn = M.ax[0].iidx[kx][bq];
-					T wx = M.ax[0].iw[kx][bq];
-					for (int e = y0; e < y1; e++) {
-						int jf = M.ax[1].iidx[ky][e];
-						T wy = M.ax[1].iw[ky][e];
-						long nf = kf * szf + ifn * sxf + jf;
-						unsigned flf = F.flags[nf];
-						if (!(flf & FL_EX(c)) || (flf & FL_BC(c))) continue;
-						T swt = wx * wy * wz;
-						if (c == 0)
-							acc += rf[nf] * (T)(float)F.eC[nf] / ndc * swt;
-						else
-							acc += rf[c * Nf + nf] * swt;
-						sw += swt;
+	T acc = 0;
+	if ((flc & FL_EX(c)) && !(flc & FL_BC(c))) {
+		int kx = c_kind[c][0], ky = c_kind[c][1], kz = c_kind[c][2];
+		int z0 = M.ax[2].istart[kz][K], z1 = M.ax[2].istart[kz][K + 1];
+		int x0 = M.ax[0].istart[kx][I], x1 = M.ax[0].istart[kx][I + 1];
+		int y0 = M.ax[1].istart[ky][J], ny = M.ax[1].istart[ky][J + 1] - y0;
+		long sxf = F.ny, szf = (long)F.nx * F.ny;
+		int jf[XFER_MAXL];
+		T wy[XFER_MAXL];
+#pragma unroll
+		for (int e = 0; e < XFER_MAXL; e++) {
+			jf[e] = e < ny ? M.ax[1].iidx[ky][y0 + e] : 0;
+			wy[e] = e < ny ? M.ax[1].iw[ky][y0 + e] : (T)0;
+		}
+		const T *r = rf + c * Nf;
+		T ndc = (T)(float)Cl.eC[nc];
+		T sw = 0;
+		for (int a = z0; a < z1; a++) {
+			long kf = M.ax[2].iidx[kz][a];
+			T wz = M.ax[2].iw[kz][a];
+			for (int bq = x0; bq < x1; bq++) {
+				long row = kf * szf + (long)M.ax[0].iidx[kx][bq] * sxf;
+				T wx = M.ax[0].iw[kx][bq];
+				unsigned fl[XFER_MAXL];
+				T q[XFER_MAXL], ef[XFER_MAXL];
+#pragma unroll
+				for (int e = 0; e < XFER_MAXL; e++) {
+					fl[e] = 0;
+					q[e] = 0;
+					ef[e] = 0;
+					if (e < ny) {
+						fl[e] = F.flags[row + jf[e]];
+						q[e] = r[row + jf[e]];
+						if (c == 0) ef[e] = F.eC[row + jf[e]];
 					}
 				}
+#pragma unroll
+				for (int e = 0; e < XFER_MAXL; e++) {
+					if (!(fl[e] & FL_EX(c)) || (fl[e] & FL_BC(c))) continue;
+					T swt = wx * wy[e] * wz;
+					if (c == 0)
+						acc += q[e] * (T)(float)ef[e] / ndc * swt;
+					else
+						acc += q[e] * swt;
+					sw += swt;
+				}
 			}
-			if (sw != (T)0) acc /= sw;
 		}
-		bc_out[c * Nc + nc] = acc;
-		xc[c * Nc + nc] = 0;
+		if (sw != (T)0) acc /= sw;
 	}
+	bc_out[c * Nc + nc] = acc;
+	xc[c * Nc + nc] = 0;
 }
 
 /* Correction prolongation x_f += P x_c (move3mg.c:2760-3060), per fine
- * node; v1koef for velocities, p1koef for pressure. */
+ * node, the four unknown kinds in one thread (one flag load); the eight
+ * coarse values of a kind are loaded before they are summed.  v1koef for
+ * velocities, p1koef for pressure; the reference's term order, nothing
+ * contracted. */
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_prolong(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__ xc,
 	  T *__restrict__ xf, T v1koef, T p1koef)
 {
+#pragma clang fp contract(off)
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	int k = blockIdx.y;
 	int i = t / F.ny, j = t - i * F.ny;
@@ -484,13 +514,24 @@ k_prolong(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__ 
 		T wy[2] = {M.ax[1].wlo[ky][j], M.ax[1].whi[ky][j]};
 		T wz[2] = {M.ax[2].wlo[kz][k], M.ax[2].whi[kz][k]};
 		T kf = c ? v1koef : p1koef;
-		T acc = xf[c * Nf + nf];
 		const T *xcc = xc + c * Nc;
+		T v[8];
+#pragma unroll
 		for (int a = 0; a < 2; a++)
+#pragma unroll
 			for (int bq = 0; bq < 2; bq++)
+#pragma unroll
+				for (int e = 0; e < 2; e++)
+					v[a * 4 + bq * 2 + e] = xcc[(bz + a) * szc + (bx + bq) * sxc + (by + e)];
+		T acc = xf[c * Nf + nf];
+#pragma unroll
+		for (int a = 0; a < 2; a++)
+#pragma unroll
+			for (int bq = 0; bq < 2; bq++)
+#pragma unroll
 				for (int e = 0; e < 2; e++) {
 					T swt = wx[bq] * wy[e] * wz[a];
-					acc += xcc[(bz + a) * szc + (bx + bq) * sxc + (by + e)] * swt * kf;
+					acc += v[a * 4 + bq * 2 + e] * swt * kf;
 				}
 		xf[c * Nf + nf] = acc;
 	}

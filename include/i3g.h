@@ -135,6 +135,24 @@ int i3g_stokes_residual(i3g_ctx *ctx, const double *P, const double *vx,
 			const double *vy, const double *vz, double *rP,
 			double *rvx, double *rvy, double *rvz, double *norms);
 
+/* Level transfers of the V-cycle, in FP64, on host planes (P, vx, vy, vz;
+ * SoA, one plane per unknown kind, level sizes).  `level` is the coarse
+ * level (1 <= level < levels), as the reference's argument.
+ *
+ * i3g_stokes_restrict replaces vprestrict(level) (move3mg.c:3165-3520):
+ * coarse right-hand side = weight-normalised trilinear gather of the fine
+ * residual per staggered component, fine and coarse BC rows skipped, the P
+ * residual scaled by nd_fine/nd_coarse; coarse BC and missing rows get 0.
+ * i3g_stokes_prolong replaces vpprolong(level, 0) (move3mg.c:2748-3063):
+ * fine += v1koef (velocities) or p1koef (P) times the trilinear gather of
+ * the coarse correction, for the existing fine unknowns.
+ * Both need i3g_set_coefficients first and overwrite the FP64 multigrid
+ * work buffers of the two levels (not the resident solution). */
+int i3g_stokes_restrict(i3g_ctx *ctx, int level, const double *const *fine,
+			double *const *coarse);
+int i3g_stokes_prolong(i3g_ctx *ctx, int level, const double *const *coarse,
+		       double *const *fine, double v1koef, double p1koef);
+
 typedef struct {
 	double rtol;          /* relative weighted residual target (1e-8) */
 	int precision;        /* 64: plain FP64 multigrid; 32: FP32 V-cycles

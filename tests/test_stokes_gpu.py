@@ -122,3 +122,105 @@ def test_load_first_smoother_matches_plain_kernel(gpu, monkeypatch, prec, tol):
         d = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
         print(f"  field {c}: rel L2 diff {d:.3e}", flush=True)
         assert d <= tol, (c, d)
+
+
+def _transfer_mask(exists, bondm, mgp, N, c):
+    return exists[c::4] & (bondm[4 * mgp:4 * (mgp + N)][c::4] == 0)
+
+
+def test_transfers_match_reference_bitwise(gpu):
+    """i3g_stokes_restrict / i3g_stokes_prolong (FP64) against vprestrict /
+    vpprolong run by the reference on the same inputs (g1 fixture, 13x11x12
+    -> 9x7x8): equal to the bit on every existing non-BC coarse row and on
+    every existing fine unknown; the other coarse rows are 0 and missing
+    fine unknowns are untouched."""
+    g = golden("g1_operator.npz")
+    cfg = _cfg_from_g1(g)
+    print("case: g1 transfers", cfg["nx"], cfg["ny"], cfg["nz"], flush=True)
+    S = I.Stokes(cfg)
+    S.set_coefficients()
+    rr = g["restr_in"]
+    out = S.restrict(1, [rr[c::4] for c in range(4)])
+    p1, N1 = int(g["mgp"][1]), len(out[0])
+    for c in range(4):
+        m = _transfer_mask(g["exists1"], g["bondm"], p1, N1, c)
+        ref = g["restr_out"][c::4]
+        print(f"  restrict comp {c}: rows {int(m.sum())} max|diff| {np.abs(out[c][m] - ref[m]).max():.3e}", flush=True)
+        assert np.array_equal(out[c][m], ref[m]), c
+        assert np.all(out[c][~m] == 0), c
+    xf, xc = g["prol_xf"], g["prol_xc"]
+    pf = S.prolong(1, [xc[c::4] for c in range(4)], [xf[c::4] for c in range(4)], float(g["v1koef"]),
+                   float(g["p1koef"]))
+    for c in range(4):
+        m = g["exists0"][c::4]
+        ref = g["prol_out"][c::4]
+        print(f"  prolong comp {c}: max|diff| {np.abs(pf[c][m] - ref[m]).max():.3e}", flush=True)
+        assert np.array_equal(pf[c][m], ref[m]), c
+        assert np.array_equal(pf[c][~m], xf[c::4][~m]), c
+    S.close()
+
+
+def test_transfers_match_oracle_all_levels(gpu):
+    """Restriction and prolongation between every pair of levels of a 129^3
+    hierarchy (129 -> 67 -> 36 -> 20 -> 12; the 67 -> 36 maps have
+    five-line inverse lists) against the oracle's vprestrict / vpprolong,
+    with the oracle's own coarse viscosities (ronurestrict) for the P
+    scaling: bit-equal on existing non-BC coarse rows and existing fine
+    unknowns."""
+    n = 129
+    cfg = I.make_config(2, n, n, n, seed=3)
+    h = cfg["hier"]
+    print("case: transfers c2 n=129 levels", cfg["multinum"] + 1, flush=True)
+    S = I.Stokes(cfg)
+    S.set_coefficients()
+    rng = np.random.default_rng(11)
+    Ls, nds = [], []
+    nu = cfg["nu"]
+    for l in range(cfg["multinum"] + 1):
+        lines = I.level_lines(h, l, cfg["gx"], cfg["gy"], cfg["gz"])
+        L = oracle.level(int(h.mgx[l]), int(h.mgy[l]), int(h.mgz[l]), int(h.mgp[l]), l, *lines)
+        if l == 0:
+            et = oracle.eta(L, nu, cfg["nxx"], cfg["nxy"], cfg["nxz"], cfg["nyz"], 0, cfg["nubeg"], cfg["nuend"])
+        else:
+            nu = oracle.ronurestrict(Ls[-1], L, nu, 0)
+            et = oracle.eta(L, nu, None, None, None, None, 0, cfg["nubeg"], cfg["nuend"])
+        Ls.append(L)
+        nds.append(et[0].astype(np.float32))
+    for l in range(1, cfg["multinum"] + 1):
+        F, Cl = Ls[l - 1], Ls[l]
+        Nf, Nc = F.nx * F.ny * F.nz, Cl.nx * Cl.ny * Cl.nz
+        rf = rng.standard_normal(4 * Nf)
+        out = S.restrict(l, [rf[c::4] for c in range(4)])
+        ref = oracle.vprestrict(F, Cl, cfg["bondm"], rf, nds[l - 1], nds[l])
+        assert S.level_nodes(l) == Nc
+        exc = np.zeros(4 * Nc, bool)
+        mz, rem = np.divmod(np.arange(Nc), Cl.nx * Cl.ny)
+        mx, my = np.divmod(rem, Cl.ny)
+        exc[0::4] = (mx > 0) & (my > 0) & (mz > 0)
+        exc[1::4] = (my < Cl.ny - 1) & (mz < Cl.nz - 1)
+        exc[2::4] = (mx < Cl.nx - 1) & (mz < Cl.nz - 1)
+        exc[3::4] = (mx < Cl.nx - 1) & (my < Cl.ny - 1)
+        for c in range(4):
+            m = _transfer_mask(exc, cfg["bondm"], int(h.mgp[l]), Nc, c)
+            d = np.abs(out[c][m] - ref[c::4][m]).max()
+            print(f"  level {l} restrict comp {c}: max|diff| {d:.3e}", flush=True)
+            assert np.array_equal(out[c][m], ref[c::4][m]), (l, c)
+        xc = rng.standard_normal(4 * Nc)
+        xf = rng.standard_normal(4 * Nf)
+        pf = S.prolong(l, [xc[c::4] for c in range(4)], [xf[c::4] for c in range(4)], 0.7, 1.0)
+        pref = oracle.vpprolong(F, Cl, xc, xf, 0.7, 1.0)
+        for c in range(4):
+            d = np.abs(pf[c] - pref[c::4]).max()
+            print(f"  level {l} prolong comp {c}: max|diff| {d:.3e}", flush=True)
+            assert np.array_equal(pf[c], pref[c::4]), (l, c)
+    S.close()
+
+
+def test_create_rejects_degenerate_viscosity_restriction(gpu):
+    """On a 68^3 grid the reference's coarse lines (68 -> 36) give a coarse
+    line whose ronurestrict weights sum to zero (fine weights -1, 0, 1, 0),
+    i.e. a division by zero in the coarse viscosity; the context refuses
+    such a hierarchy instead of producing non-finite viscosities."""
+    cfg = I.make_config(2, 68, 68, 68, seed=3)
+    with pytest.raises(I.I3GError, match="invalid"):
+        I.Stokes(cfg)
