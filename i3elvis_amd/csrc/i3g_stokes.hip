@@ -283,8 +283,14 @@ struct i3g_ctx {
 	/* level transfer maps: 0 the reference's index rules, 1 geometric
 	 * bracketing (I3G_XFER_GEO=1) */
 	int xfer_geo = 0;
+	/* coarsest-level sweeps in one work-group from LDS (k_coarsest) when
+	 * they fit; I3G_FUSED_BOTTOM=0 switches it off */
+	int bottom = 0;
+	size_t coarsest_lds_max = 0;
 	float *st12 = nullptr;   /* strain rates and stresses, 12 level-0 planes */
 };
+
+static int choose_bottom(i3g_ctx *c);
 
 /* ---- event timing of named kernels ------------------------------------ */
 
@@ -714,6 +720,8 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		if (e && e[0] >= '0' && e[0] <= '2') c->coarse_visc = e[0] - '0';
 		e = getenv("I3G_XFER_GEO");
 		if (e && e[0] == '1') c->xfer_geo = 1;
+		e = getenv("I3G_FUSED_BOTTOM");
+		if (e && e[0] == '0') c->bottom = -1;
 	}
 	if (hipStreamCreate(&c->s) != hipSuccess) { delete c; return I3G_EDEVICE; }
 	c->L.resize(c->nlev);
@@ -730,6 +738,8 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 			if (rc) { i3g_destroy(c); return rc; }
 		}
 	}
+	if (c->bottom == 0) c->bottom = choose_bottom(c);
+	else c->bottom = 0;
 	int err = 0;
 	long N0 = c->L[0].N;
 	c->rhs0 = dalloc<double>(4 * N0, &err);
@@ -912,6 +922,60 @@ template <typename T> static void residual_level(i3g_ctx *c, int l, const T *x, 
 /* V-cycle on level l: x_l (in/out), b_l given (level buffers). */
 template <typename T> static void vcycle(i3g_ctx *c, int l, const i3g_stokes_params *p);
 
+static size_t coarsest_lds(const Level &L, size_t word)
+{
+	return (size_t)L.N * (8 * word + 1);
+}
+
+/* LDS budget of the coarsest-level kernel: the device's per-work-group
+ * limit, with the opt-in above 64 KiB set for both precisions; 0 = off */
+static int choose_bottom(i3g_ctx *c)
+{
+	const Level &L = c->L[c->nlev - 1];
+	if (c->nlev < 2 || L.depth_start.size() > COARSEST_MAXD + 2) return 0;
+	int dev = 0, lim = 0;
+	if (hipGetDevice(&dev) != hipSuccess ||
+	    hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+		return 0;
+	size_t need = coarsest_lds(L, sizeof(double));
+	if (need > (size_t)lim) need = coarsest_lds(L, sizeof(float));
+	if (need > (size_t)lim) return 0;
+	if (need > 64 * 1024) {
+		if (hipFuncSetAttribute((const void *)k_coarsest<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+					(int)need) != hipSuccess ||
+		    hipFuncSetAttribute((const void *)k_coarsest<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+					(int)need) != hipSuccess) {
+			hipGetLastError();
+			need = 64 * 1024;
+		}
+	}
+	c->coarsest_lds_max = need;
+	return 1;
+}
+
+/* the coarsest level's sweeps in one launch when its data fits the LDS
+ * budget in precision T; false: use the launched sweeps */
+template <typename T> static bool coarsest_launch(i3g_ctx *c, const i3g_stokes_params *p, int sweeps)
+{
+	Level &L = c->L[c->nlev - 1];
+	size_t lds = coarsest_lds(L, sizeof(T));
+	if (c->bottom <= 0 || lds > c->coarsest_lds_max) return false;
+	CoarsestArgs<T> A;
+	memset(&A, 0, sizeof(A));
+	A.v = view<T>(L);
+	A.x = lx<T>(L);
+	A.b = lb<T>(L);
+	A.eta = A.v.eC;
+	A.bc = bcview<T>(L);
+	A.nd = (int)L.depth_start.size() - 2;
+	for (size_t d = 0; d < L.depth_start.size(); d++) A.ds[d] = L.depth_start[d];
+	A.sweeps = sweeps;
+	A.v0koef = (T)p->v0koef;
+	A.p0koef = (T)p->p0koef;
+	hipLaunchKernelGGL(k_coarsest<T>, dim3(1), dim3(COARSEST_THREADS), lds, c->s, A);
+	return true;
+}
+
 /* The coarse levels (>= 1) of a V-cycle do many small launches; they are
  * captured once into a HIP graph (fixed level buffers, no memory-set
  * nodes) and replayed.  Re-captured when the smoothing parameters change. */
@@ -952,6 +1016,7 @@ template <typename T> static void vcycle(i3g_ctx *c, int l, const i3g_stokes_par
 	T *x = lx<T>(L), *b = lb<T>(L);
 	if (l == c->nlev - 1) {
 		int ns = p->nu_coarse > 0 ? p->nu_coarse : 64;
+		if (coarsest_launch<T>(c, p, ns)) return;
 		for (int s = 0; s < ns; s++) sweep<T>(c, l, x, b, p);
 		return;
 	}
@@ -1446,13 +1511,15 @@ extern "C" void i3g_stokes_default_params(i3g_stokes_params *p)
 	 * stalls); finest-level sweeps 8, 12, 16, 20, 24, 32 took 1538, 687, 477,
 	 * 457, 447, 427 V-cycles, 16 being the fastest in time; 8 instead of 16
 	 * sweeps on the second level cost 2% more V-cycles for 7% cheaper
-	 * V-cycles. */
+	 * V-cycles.  The coarsest level (smoothed from LDS in one work-group)
+	 * gets 128 sweeps: 457 instead of 477 V-cycles for 0.5 ms more per
+	 * V-cycle; 256 gain nothing further. */
 	int pre[4] = {16, 8, 16, 32}, post[4] = {16, 8, 16, 32};
 	for (int l = 0; l < I3G_MAXLEV; l++) {
 		p->nu_pre[l] = l < 4 ? pre[l] : 32;
 		p->nu_post[l] = l < 4 ? post[l] : 32;
 	}
-	p->nu_coarse = 64;
+	p->nu_coarse = 128;
 	p->inner_m = 10;
 	p->inner_tol = 0.1;
 	p->p0koef = 0.6; p->p1koef = 1.0; p->v0koef = 0.85; p->v1koef = 0.7;

@@ -168,6 +168,42 @@ __device__ __forceinline__ void block_sum2(double a, double b, double *out2)
  * by wP) for the convergence metric. Block = 256 threads over the xy plane,
  * grid.y = z. */
 template <typename T, typename X, int MODE>
+__device__ __forceinline__ void residual_node(const LevelView<T> &L, const X *__restrict__ x,
+					      const T *__restrict__ b, T *__restrict__ r,
+					      const int *__restrict__ bcid, const BcRec<T> &bc, int i, int j, int k,
+					      bool sums, double *sr, double *sb)
+{
+	long N = L.N;
+	long n = (long)k * L.nx * L.ny + (long)i * L.ny + j;
+	unsigned fl = L.flags[n];
+	const X *P = x, *vx = x + N, *vy = x + 2 * N, *vz = x + 3 * N;
+#pragma unroll
+	for (int c = 0; c < 4; c++) {
+		T res = 0;
+		long u = c * N + n;
+		if (fl & FL_EX(c)) {
+			if (fl & FL_BC(c)) {
+				int rec = bcid[u];
+				int p = bc.partner[rec];
+				T ax = (T)x[u] - (p >= 0 ? bc.k[rec] * (T)x[p] : (T)0);
+				res = MODE ? ax : (b ? b[u] : (T)0) - ax;
+			} else {
+				T ax = row_ax<T, X>(L, P, vx, vy, vz, c, i, j, k, (T *)0);
+				T bb = b ? b[u] : (T)0;
+				res = MODE ? ax : bb - ax;
+				if (MODE && c == 0) res *= L.wP[n];
+				if (sums) {
+					double w = c == 0 ? (double)L.wP[n] : 1.0;
+					*sr += ((double)res * w) * ((double)res * w);
+					*sb += ((double)bb * w) * ((double)bb * w);
+				}
+			}
+		}
+		r[u] = res;
+	}
+}
+
+template <typename T, typename X, int MODE>
 __global__ void __launch_bounds__(256)
 k_residual(LevelView<T> L, const X *__restrict__ x, const T *__restrict__ b,
 	   T *__restrict__ r, const int *__restrict__ bcid, BcRec<T> bc,
@@ -177,36 +213,7 @@ k_residual(LevelView<T> L, const X *__restrict__ x, const T *__restrict__ b,
 	int k = blockIdx.y;
 	int i = t / L.ny, j = t - i * L.ny;
 	double sr = 0, sb = 0;
-	if (i < L.nx) {
-		long N = L.N;
-		long n = (long)k * L.nx * L.ny + (long)i * L.ny + j;
-		unsigned fl = L.flags[n];
-		const X *P = x, *vx = x + N, *vy = x + 2 * N, *vz = x + 3 * N;
-#pragma unroll
-		for (int c = 0; c < 4; c++) {
-			T res = 0;
-			long u = c * N + n;
-			if (fl & FL_EX(c)) {
-				if (fl & FL_BC(c)) {
-					int rec = bcid[u];
-					int p = bc.partner[rec];
-					T ax = (T)x[u] - (p >= 0 ? bc.k[rec] * (T)x[p] : (T)0);
-					res = MODE ? ax : (b ? b[u] : (T)0) - ax;
-				} else {
-					T ax = row_ax<T, X>(L, P, vx, vy, vz, c, i, j, k, (T *)0);
-					T bb = b ? b[u] : (T)0;
-					res = MODE ? ax : bb - ax;
-					if (MODE && c == 0) res *= L.wP[n];
-					if (partials) {
-						double w = c == 0 ? (double)L.wP[n] : 1.0;
-						sr += ((double)res * w) * ((double)res * w);
-						sb += ((double)bb * w) * ((double)bb * w);
-					}
-				}
-			}
-			r[u] = res;
-		}
-	}
+	if (i < L.nx) residual_node<T, X, MODE>(L, x, b, r, bcid, bc, i, j, k, partials != nullptr, &sr, &sb);
 	if (partials) {
 		long bid = (long)blockIdx.y * gridDim.x + blockIdx.x;
 		block_sum2<T>(sr, sb, partials + 2 * bid);
@@ -269,18 +276,9 @@ k_smooth_vel(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int com
  * any is used.  Same arithmetic as row_ax; the scheduling barrier keeps the
  * compiler from sinking the loads back to their uses.  COMP = 1, 2, 3. */
 template <typename T, int COMP>
-__global__ void __launch_bounds__(256)
-k_smooth_vel_ilp(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int colour, T v0koef, int remap,
-		 int chunks)
+__device__ __forceinline__ void vel_update(const LevelView<T> &L, T *__restrict__ x, const T *__restrict__ b,
+					   int i, int j, int k, T v0koef)
 {
-	int ch, k;
-	if (!block_work(remap, chunks, L.nz, &ch, &k)) return;
-	int t = ch * blockDim.x + threadIdx.x;
-	int hy = (L.ny + 1) >> 1;
-	int i = t / hy, tj = t - i * hy;
-	if (i >= L.nx) return;
-	int j = 2 * tj + ((i + k + colour) & 1);
-	if (j >= L.ny) return;
 	long N = L.N, sx = L.ny, sz = (long)L.nx * L.ny;
 	long n = (long)k * sz + (long)i * sx + j;
 	unsigned fl = L.flags[n];
@@ -357,16 +355,28 @@ k_smooth_vel_ilp(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int
 	}
 }
 
+template <typename T, int COMP>
+__global__ void __launch_bounds__(256)
+k_smooth_vel_ilp(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int colour, T v0koef, int remap,
+		 int chunks)
+{
+	int ch, k;
+	if (!block_work(remap, chunks, L.nz, &ch, &k)) return;
+	int t = ch * blockDim.x + threadIdx.x;
+	int hy = (L.ny + 1) >> 1;
+	int i = t / hy, tj = t - i * hy;
+	if (i >= L.nx) return;
+	int j = 2 * tj + ((i + k + colour) & 1);
+	if (j >= L.ny) return;
+	vel_update<T, COMP>(L, x, b, i, j, k, v0koef);
+}
+
 /* Pressure relaxation P -= p0koef * eta_cell * (div v - b_P)
  * (move3mg.c:3105-3111); BC rows are left to k_bc. */
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_smooth_p(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, T p0koef)
+__device__ __forceinline__ void p_update(const LevelView<T> &L, T *__restrict__ x, const T *__restrict__ b, int i,
+					 int j, int k, T p0koef)
 {
-	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	int k = blockIdx.y;
-	int i = t / L.ny, j = t - i * L.ny;
-	if (i >= L.nx) return;
 	long N = L.N;
 	long n = (long)k * L.nx * L.ny + (long)i * L.ny + j;
 	unsigned fl = L.flags[n];
@@ -376,17 +386,34 @@ k_smooth_p(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, T p0koef)
 	x[n] -= res * L.eC[n] * p0koef;
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_smooth_p(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, T p0koef)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int k = blockIdx.y;
+	int i = t / L.ny, j = t - i * L.ny;
+	if (i >= L.nx) return;
+	p_update<T>(L, x, b, i, j, k, p0koef);
+}
+
 /* Boundary rows x_u = c + k x_partner for the records [start, end) of one
  * dependency depth (partners of depth-d rows are interior or depth < d). */
+template <typename T>
+__device__ __forceinline__ void bc_update(T *__restrict__ x, const BcRec<T> &bc, int t, int homog)
+{
+	int p = bc.partner[t];
+	T v = homog ? (T)0 : bc.c[t];
+	if (p >= 0) v += bc.k[t] * x[p];
+	x[bc.u[t]] = v;
+}
+
 template <typename T>
 __global__ void k_bc(T *__restrict__ x, BcRec<T> bc, int start, int end, int homog)
 {
 	int t = start + blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= end) return;
-	int p = bc.partner[t];
-	T v = homog ? (T)0 : bc.c[t];
-	if (p >= 0) v += bc.k[t] * x[p];
-	x[bc.u[t]] = v;
+	bc_update<T>(x, bc, t, homog);
 }
 
 /* Per-axis transfer maps between a fine and a coarse level for the three
@@ -421,15 +448,11 @@ __device__ __constant__ int c_kind[4][3] = {{2, 2, 2}, {0, 1, 1}, {1, 0, 1}, {1,
  * the result is the reference's to the bit.  Coarse x is zeroed
  * (move3mg.c:3508-3518). */
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_restrict(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__ rf,
-	   T *__restrict__ bc_out, T *__restrict__ xc)
+__device__ __forceinline__ void restrict_node(const LevelView<T> &F, const LevelView<T> &Cl, const XferView<T> &M,
+					      const T *__restrict__ rf, T *__restrict__ bc_out, T *__restrict__ xc,
+					      int I, int J, int K, int c)
 {
 #pragma clang fp contract(off)
-	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	int K = blockIdx.y, c = blockIdx.z;
-	int I = t / Cl.ny, J = t - I * Cl.ny;
-	if (I >= Cl.nx) return;
 	long Nc = Cl.N, Nf = F.N;
 	long nc = (long)K * Cl.nx * Cl.ny + (long)I * Cl.ny + J;
 	unsigned flc = Cl.flags[nc];
@@ -487,21 +510,28 @@ k_restrict(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__
 	xc[c * Nc + nc] = 0;
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_restrict(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__ rf,
+	   T *__restrict__ bc_out, T *__restrict__ xc)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int I = t / Cl.ny, J = t - I * Cl.ny;
+	if (I >= Cl.nx) return;
+	restrict_node<T>(F, Cl, M, rf, bc_out, xc, I, J, blockIdx.y, blockIdx.z);
+}
+
 /* Correction prolongation x_f += P x_c (move3mg.c:2760-3060), per fine
  * node, the four unknown kinds in one thread (one flag load); the eight
  * coarse values of a kind are loaded before they are summed.  v1koef for
  * velocities, p1koef for pressure; the reference's term order, nothing
  * contracted. */
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_prolong(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__ xc,
-	  T *__restrict__ xf, T v1koef, T p1koef)
+__device__ __forceinline__ void prolong_node(const LevelView<T> &F, const LevelView<T> &Cl, const XferView<T> &M,
+					     const T *__restrict__ xc, T *__restrict__ xf, int i, int j, int k,
+					     T v1koef, T p1koef)
 {
 #pragma clang fp contract(off)
-	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	int k = blockIdx.y;
-	int i = t / F.ny, j = t - i * F.ny;
-	if (i >= F.nx) return;
 	long Nc = Cl.N, Nf = F.N;
 	long nf = (long)k * F.nx * F.ny + (long)i * F.ny + j;
 	unsigned fl = F.flags[nf];
@@ -535,6 +565,100 @@ k_prolong(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__ 
 				}
 		xf[c * Nf + nf] = acc;
 	}
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_prolong(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__ xc,
+	  T *__restrict__ xf, T v1koef, T p1koef)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int i = t / F.ny, j = t - i * F.ny;
+	if (i >= F.nx) return;
+	prolong_node<T>(F, Cl, M, xc, xf, i, j, blockIdx.y, v1koef, p1koef);
+}
+
+/* ---- the coarsest level in one work-group, from LDS --------------------- */
+
+/* The coarsest level (12^3 in the usual hierarchies) is smoothed nu_coarse
+ * times per V-cycle: as launches, each colour pass, pressure pass and BC
+ * depth costs a few microseconds for well under a microsecond of work.
+ * k_coarsest runs all of its sweeps in one work-group with the unknowns,
+ * the four viscosity planes and the flags held in LDS (the right-hand side,
+ * line factors and BC records stay in global memory): every launch becomes
+ * a work-group-strided loop over the same per-node functions followed by a
+ * barrier, in launch order, so the result is the launched sweeps' bit for
+ * bit. */
+#define COARSEST_THREADS 1024
+#define COARSEST_MAXD 8
+
+template <typename T> struct CoarsestArgs {
+	LevelView<T> v;
+	T *x;
+	const T *b;
+	const T *eta;                    /* eC, eXY, eXZ, eYZ planes */
+	BcRec<T> bc;
+	int nd;                          /* BC depth ranges 1..nd */
+	int ds[COARSEST_MAXD + 2];
+	int sweeps;
+	T v0koef, p0koef;
+};
+
+template <typename T>
+__device__ void wg_sweep(const LevelView<T> &L, T *x, const T *b, const CoarsestArgs<T> &A)
+{
+	int hy = (L.ny + 1) >> 1;
+	int plane = L.nx * hy;
+	int tot = L.nz * plane;
+#pragma unroll
+	for (int comp = 1; comp <= 3; comp++)
+		for (int col = 0; col < 2; col++) {
+			for (int t = threadIdx.x; t < tot; t += COARSEST_THREADS) {
+				int k = t / plane, rem = t - k * plane;
+				int i = rem / hy, tj = rem - i * hy;
+				int j = 2 * tj + ((i + k + col) & 1);
+				if (j >= L.ny) continue;
+				if (comp == 1) vel_update<T, 1>(L, x, b, i, j, k, A.v0koef);
+				else if (comp == 2) vel_update<T, 2>(L, x, b, i, j, k, A.v0koef);
+				else vel_update<T, 3>(L, x, b, i, j, k, A.v0koef);
+			}
+			__syncthreads();
+		}
+	int pl = L.nx * L.ny, N = (int)L.N;
+	for (int t = threadIdx.x; t < N; t += COARSEST_THREADS) {
+		int k = t / pl, rem = t - k * pl;
+		int i = rem / L.ny, j = rem - i * L.ny;
+		p_update<T>(L, x, b, i, j, k, A.p0koef);
+	}
+	__syncthreads();
+	for (int d = 1; d <= A.nd; d++) {
+		for (int t = A.ds[d] + threadIdx.x; t < A.ds[d + 1]; t += COARSEST_THREADS) bc_update<T>(x, A.bc, t, 1);
+		__syncthreads();
+	}
+}
+
+/* dynamic LDS: 4N unknowns, 4N viscosities (T), N flag bytes */
+template <typename T>
+__global__ void __launch_bounds__(COARSEST_THREADS) k_coarsest(CoarsestArgs<T> A)
+{
+	extern __shared__ __align__(16) unsigned char lds[];
+	int N = (int)A.v.N;
+	T *sx = (T *)lds, *se = sx + 4 * N;
+	uint8_t *sf = (uint8_t *)(se + 4 * N);
+	for (int t = threadIdx.x; t < 4 * N; t += COARSEST_THREADS) {
+		sx[t] = A.x[t];
+		se[t] = A.eta[t];
+	}
+	for (int t = threadIdx.x; t < N; t += COARSEST_THREADS) sf[t] = A.v.flags[t];
+	__syncthreads();
+	LevelView<T> L = A.v;
+	L.eC = se;
+	L.eXY = se + N;
+	L.eXZ = se + 2 * N;
+	L.eYZ = se + 3 * N;
+	L.flags = sf;
+	for (int s = 0; s < A.sweeps; s++) wg_sweep<T>(L, sx, A.b, A);
+	for (int t = threadIdx.x; t < 4 * N; t += COARSEST_THREADS) A.x[t] = sx[t];
 }
 
 /* ---- vector helpers (Krylov, precision conversion) --------------------- */
