@@ -287,6 +287,9 @@ struct i3g_ctx {
 	 * they fit; I3G_FUSED_BOTTOM=0 switches it off */
 	int bottom = 0;
 	size_t coarsest_lds_max = 0;
+	/* BC rows of all depths in one launch (k_bc_chain); I3G_BC_CHAIN=0
+	 * selects one launch per depth */
+	int bc_chain = 1;
 	float *st12 = nullptr;   /* strain rates and stresses, 12 level-0 planes */
 };
 
@@ -722,6 +725,8 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		if (e && e[0] == '1') c->xfer_geo = 1;
 		e = getenv("I3G_FUSED_BOTTOM");
 		if (e && e[0] == '0') c->bottom = -1;
+		e = getenv("I3G_BC_CHAIN");
+		if (e && e[0] == '0') c->bc_chain = 0;
 	}
 	if (hipStreamCreate(&c->s) != hipSuccess) { delete c; return I3G_EDEVICE; }
 	c->L.resize(c->nlev);
@@ -871,6 +876,14 @@ extern "C" int i3g_set_coefficients(i3g_ctx *c, const float *nu, const float *nx
 template <typename T> static void bc_enforce(i3g_ctx *c, Level &L, T *x, int homog)
 {
 	BcRec<T> bc = bcview<T>(L);
+	int nd = (int)L.depth_start.size() - 2;
+	if (c->bc_chain && nd <= BC_MAXCHAIN) {
+		int a = L.depth_start[1], b = L.depth_start[nd + 1];
+		if (b > a)
+			hipLaunchKernelGGL(k_bc_chain<T>, dim3((b - a + 255) / 256), dim3(256), 0, c->s, x, bc,
+					   (const int *)L.bcid, a, b, homog);
+		return;
+	}
 	for (size_t d = 1; d + 1 < L.depth_start.size(); d++) {
 		int a = L.depth_start[d], b = L.depth_start[d + 1];
 		if (b > a)

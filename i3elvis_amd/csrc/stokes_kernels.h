@@ -416,6 +416,41 @@ __global__ void k_bc(T *__restrict__ x, BcRec<T> bc, int start, int end, int hom
 	bc_update<T>(x, bc, t, homog);
 }
 
+/* All BC records [start, end) in one launch: a record whose partner is
+ * itself a BC row follows the partner chain (at most BC_MAXCHAIN records,
+ * checked on the host) down to an interior unknown and evaluates it
+ * innermost first, each step the expression of bc_update, so every value
+ * equals what the depth-by-depth launches store.  Only interior unknowns
+ * are read, only BC unknowns written: no ordering between threads. */
+#define BC_MAXCHAIN 8
+template <typename T>
+__global__ void k_bc_chain(T *__restrict__ x, BcRec<T> bc, const int *__restrict__ bcid, int start, int end,
+			   int homog)
+{
+	int t = start + blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= end) return;
+	int chain[BC_MAXCHAIN];
+	int m = 0, r = t;
+	for (;;) {
+		chain[m++] = r;
+		int p = bc.partner[r];
+		if (p < 0 || m == BC_MAXCHAIN) break;
+		int rp = bcid[p];
+		if (rp < 0) break;
+		r = rp;
+	}
+	int ri = chain[m - 1], pi = bc.partner[ri];
+	T v = homog ? (T)0 : bc.c[ri];
+	if (pi >= 0) v += bc.k[ri] * x[pi];
+	for (int q = m - 2; q >= 0; q--) {
+		int rq = chain[q];
+		T w = homog ? (T)0 : bc.c[rq];
+		w += bc.k[rq] * v;
+		v = w;
+	}
+	x[bc.u[t]] = v;
+}
+
 /* Per-axis transfer maps between a fine and a coarse level for the three
  * staggering kinds (0 node line, 1 mid-point, 2 cell centre), built on the
  * host from the m10min/m20min/m30min rules of vprestrict/vpprolong

@@ -248,3 +248,30 @@ def test_fused_bottom_matches_launched_levels(gpu, monkeypatch, n, prec):
         S.close()
     for c, (a, b) in enumerate(zip(*sols)):
         assert np.array_equal(a, b), (c, np.abs(a - b).max())
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_bc_chain_matches_depth_launches(gpu, monkeypatch, prec):
+    """BC rows of all dependency depths in one launch (k_bc_chain, partner
+    chains evaluated innermost first) against one launch per depth
+    (I3G_BC_CHAIN=0): equal to the bit after a few V-cycles, and for the
+    inhomogeneous rows applied by set_solution."""
+    n = 33
+    cfg = I.make_config(2, n, n, n, seed=9)
+    sols = []
+    rng = np.random.default_rng(5)
+    guess = [rng.standard_normal(n ** 3) * s for s in (1e6, 1e-10, 1e-10, 1e-10)]
+    for mode in ("0", "1"):
+        monkeypatch.setenv("I3G_BC_CHAIN", mode)
+        S = I.Stokes(cfg)
+        S.set_coefficients()
+        S.set_solution(*guess)
+        g = S.get_solution()
+        st = S.solve(precision=prec, rtol=1e-8, max_outer=1, max_vcycles=4, check_conv=False)
+        print(f"case: bc chain={mode} prec={prec} vcycles={st.vcycles} rel={st.rel_res:.3e}", flush=True)
+        sols.append((g, S.get_solution()))
+        S.close()
+    for (ga, a), (gb, b) in [(sols[0], sols[1])]:
+        for c in range(4):
+            assert np.array_equal(ga[c], gb[c]), ("guess", c)
+            assert np.array_equal(a[c], b[c]), ("solve", c)
