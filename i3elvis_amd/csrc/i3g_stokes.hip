@@ -290,6 +290,7 @@ struct i3g_ctx {
 	/* BC rows of all depths in one launch (k_bc_chain); I3G_BC_CHAIN=0
 	 * selects one launch per depth */
 	int bc_chain = 1;
+	long reorth = 0;         /* inner Gram-Schmidt second passes (diagnostic) */
 	float *st12 = nullptr;   /* strain rates and stresses, 12 level-0 planes */
 };
 
@@ -1325,21 +1326,31 @@ static int inner_fgmres(i3g_ctx *c, const R *r, T *out, int m, double tol, const
 			hipLaunchKernelGGL((k_residual<T, T, 1>), plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, view<T>(L),
 					   (const T *)Z[j], (const T *)nullptr, w, L.bcid, bcview<T>(L), (double *)nullptr);
 		}
-		/* classical Gram-Schmidt, twice, coefficients to the host */
-		double *h1 = c->dh, *h2 = c->dh + c->kr_m; /* kr_m >= m + 1 */
-		for (int pass = 0; pass < 2; pass++) {
-			double *hh = pass ? h2 : h1;
-			multidot_dev<T, T>(c, (const T *const *)dV, j + 1, (const T *)w, n, hh);
-			hipLaunchKernelGGL((k_multiaxpy<T, T>), lg, dim3(256), 0, c->s, w, (const T *const *)dV, (const double *)hh,
-					   j + 1, n, -1.0);
-		}
+		/* classical Gram-Schmidt; a second pass only when the first
+		 * removed more than half of ||w||^2 (as in the outer GCR).  The
+		 * first pass also returns ||w||^2 (w = V[j+1] is the last vector
+		 * of the set). */
+		double *h1 = c->dh, *h2 = c->dh + c->kr_m; /* kr_m >= m + 2 */
+		multidot_dev<T, T>(c, (const T *const *)dV, j + 2, (const T *)w, n, h1);
+		hipLaunchKernelGGL((k_multiaxpy<T, T>), lg, dim3(256), 0, c->s, w, (const T *const *)dV, (const double *)h1,
+				   j + 1, n, -1.0);
 		multidot_dev<T, T>(c, (const T *const *)(dV + j + 1), 1, (const T *)w, n, dh);
-		hipMemcpyAsync(c->hscal + 16, h1, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, c->s);
-		hipMemcpyAsync(c->hscal + 16 + m + 1, h2, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, c->s);
+		hipMemcpyAsync(c->hscal + 16, h1, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, c->s);
 		hipMemcpyAsync(c->hscal + 8, dh, sizeof(double), hipMemcpyDeviceToHost, c->s);
-		hipLaunchKernelGGL(k_scale_rsqrt<T>, lg, dim3(256), 0, c->s, w, (const double *)dh, n);
 		hipStreamSynchronize(c->s);
-		for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = c->hscal[16 + i] + c->hscal[16 + m + 1 + i];
+		for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = c->hscal[16 + i];
+		if (c->hscal[8] < 0.5 * c->hscal[16 + j + 1]) {
+			multidot_dev<T, T>(c, (const T *const *)dV, j + 1, (const T *)w, n, h2);
+			hipLaunchKernelGGL((k_multiaxpy<T, T>), lg, dim3(256), 0, c->s, w, (const T *const *)dV,
+					   (const double *)h2, j + 1, n, -1.0);
+			multidot_dev<T, T>(c, (const T *const *)(dV + j + 1), 1, (const T *)w, n, dh);
+			hipMemcpyAsync(c->hscal + 16 + m + 2, h2, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, c->s);
+			hipMemcpyAsync(c->hscal + 8, dh, sizeof(double), hipMemcpyDeviceToHost, c->s);
+			hipStreamSynchronize(c->s);
+			for (int i = 0; i <= j; i++) H[(size_t)i * m + j] += c->hscal[16 + m + 2 + i];
+			c->reorth++;
+		}
+		hipLaunchKernelGGL(k_scale_rsqrt<T>, lg, dim3(256), 0, c->s, w, (const double *)dh, n);
 		double hn = sqrt(c->hscal[8]);
 		H[(size_t)(j + 1) * m + j] = hn;
 		for (int i = 0; i < j; i++) {
@@ -1551,7 +1562,7 @@ static int solve_impl(i3g_ctx *c, const i3g_stokes_params *p, i3g_stats *st)
 	int rc = p->precision == 64 ? ensure_krylov<double>(c, m) : ensure_krylov<float>(c, m);
 	if (rc) return rc;
 	m = c->kr_m;
-	if (p->inner_m + 1 > m) return I3G_EINVAL;
+	if (p->inner_m + 2 > m) return I3G_EINVAL; /* inner coefficients fit below the second-pass block */
 	hipEvent_t t0, t1;
 	hipEventCreate(&t0);
 	hipEventCreate(&t1);
@@ -1587,7 +1598,9 @@ static int solve_impl(i3g_ctx *c, const i3g_stokes_params *p, i3g_stats *st)
 		rel = nrm[0] / bnorm;
 		outer++;
 		if (loc.nhist < 256) loc.hist[loc.nhist++] = rel;
-		if (p->verbose) fprintf(stderr, "i3g outer %d: rel %.3e (inner est %.2e, cycles %d)\n", outer, rel, red, cycles);
+		if (p->verbose)
+			fprintf(stderr, "i3g outer %d: rel %.3e (inner est %.2e, cycles %d, second GS passes %ld)\n", outer,
+				rel, red, cycles, c->reorth);
 		if (!std::isfinite(rel)) break;
 	}
 	/* pressure gauge (FP64) */
