@@ -44,6 +44,27 @@ def smoother_bytes_per_launch(N, word):
     return N * (8 * word + 1 + word / 2)
 
 
+class _stdout_to_stderr:
+    """Sends what C code prints to file descriptor 1 (the reference prints
+    progress with printf) to stderr, so that stdout carries only the JSON
+    line; C stdio buffers are flushed before the descriptor is restored."""
+
+    def __enter__(self):
+        import ctypes
+        self._libc = ctypes.CDLL(None)
+        sys.stdout.flush()
+        self._saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        self._libc.fflush(None)
+        sys.stdout.flush()
+        os.dup2(self._saved, 1)
+        os.close(self._saved)
+        return False
+
+
 def cpu_baseline(n=33, cap_s=25.0, threads=None):
     """Runs _cpu_baseline_once with all host threads; the reference's
     Gauss-Seidel sweeps race between OpenMP threads (move3mg.c:3089) and on
@@ -51,11 +72,12 @@ def cpu_baseline(n=33, cap_s=25.0, threads=None):
     until the iteration stays finite."""
     import math
     threads = threads or min(os.cpu_count() or 1, 16)
-    while True:
-        r = _cpu_baseline_once(n, cap_s, threads)
-        if r is None or math.isfinite(r["rel_reached"]) or threads == 1:
-            return r
-        threads = max(1, threads // 2)
+    with _stdout_to_stderr():
+        while True:
+            r = _cpu_baseline_once(n, cap_s, threads)
+            if r is None or math.isfinite(r["rel_reached"]) or threads == 1:
+                return r
+            threads = max(1, threads // 2)
 
 
 def _cpu_baseline_once(n, cap_s, threads):
