@@ -429,6 +429,20 @@ class Heat:
         except Exception:
             pass
 
+    def set_sources(self, adiabyn=0, frictyn=0, gk=(0.0, 9.81, 0.0), ysize=0.0, stp100=0.0, vx=None, vy=None,
+                    vz=None, pr=None, et=None, sig=None, eps=None):
+        """Adiabatic (adiabyn 1/2) and shear (frictyn) heating terms of the
+        following solves (i3g_heat_set_sources); sig/eps: 6 planes each."""
+        f = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        keep = [f(vx), f(vy), f(vz), None if pr is None else np.ascontiguousarray(pr, np.float64), f(et)]
+        s6 = [f(a) for a in sig] if sig is not None else None
+        e6 = [f(a) for a in eps] if eps is not None else None
+        sp = (C.c_void_p * 6)(*[a.ctypes.data for a in s6]) if s6 else None
+        ep = (C.c_void_p * 6)(*[a.ctypes.data for a in e6]) if e6 else None
+        gkd = np.ascontiguousarray(gk, np.float64)
+        check(lib().i3g_heat_set_sources(self.h, C.c_int(adiabyn), C.c_int(frictyn), _p(gkd), C.c_double(ysize),
+                                         C.c_double(stp100), *[_p(a) for a in keep], sp, ep), "heat_set_sources")
+
     def solve(self, tk, ro, cp, kt, ht, check_conv=True, **kw):
         """Returns (new tk float32, HeatStats); tk is not modified."""
         p = HeatParams()

@@ -18,6 +18,8 @@
 /* products and sums rounded one at a time, as the reference's x86-64 build */
 #pragma clang fp contract(off)
 
+#include "grid_interp.h"
+
 #define TCHK(x)                                                                \
 	do {                                                                   \
 		hipError_t e_ = (x);                                           \
@@ -48,6 +50,14 @@ struct i3g_heat {
 	double *T = nullptr, *b = nullptr, *diag = nullptr;
 	double *part = nullptr, *res = nullptr;
 	std::vector<int> hb;      /* host copy of the T block of bondm */
+	/* adiabatic and shear heating of heaterr's right-hand side
+	 * (i3g_heat_set_sources): fields on the device and the source as the
+	 * reference keeps it in tk1 (float) */
+	int adiabyn = 0, frictyn = 0;
+	double gk[3] = {0, 0, 0}, ysize = 0, stp100 = 0;
+	float *vx = nullptr, *vy = nullptr, *vz = nullptr, *et = nullptr, *ro = nullptr, *se = nullptr;
+	double *pr = nullptr;
+	float *src = nullptr;
 };
 
 template <typename T> static T *halloc(size_t n, int *err)
@@ -91,12 +101,122 @@ __device__ __forceinline__ Cond conds(const float *__restrict__ kt, const double
 	return c;
 }
 
+/* heaterr in right-hand-side mode (ynerr=0, heat3mg.c:54-160) at the
+ * interior nodes: src = radiogenic heat + adiabatic heating (adiabyn 1:
+ * alpha T rho (g.v); 2: alpha T (v.grad P) with the trilinear P gradient
+ * between the eight surrounding cell centres) + shear heating (frictyn:
+ * sigma:eps, normal terms from the eight centres, shear terms from the two
+ * edge nodes along the third axis), the terms only where rho > 2000 and
+ * y < ysize - stp100; v at the node is vxyzcalc1's.  Products of two float
+ * fields stay float as in the reference.  src is stored as float (tk1);
+ * each block writes the largest |src / (rho cp)| of its interior nodes. */
+struct HeatSrc {
+	int adiabyn, frictyn;
+	double gk0, gk1, gk2, ysize, stp100;
+	const float *ht, *rocp, *ro, *tk0, *et, *vx, *vy, *vz, *se;
+	const double *pr;
+};
+
+__global__ void k_heat_source(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz,
+			      const int *__restrict__ bp, HeatSrc S, float *src, double *part)
+{
+	__shared__ double sh[NB / 64];
+	double mx = 0;
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int m3 = blockIdx.y;
+	int m1 = t / ny, m2 = t - m1 * ny;
+	if (m1 < nx) {
+		long sx = ny, sz = (long)nx * ny, N = sz * nz;
+		long v3 = m3 * sz + m1 * sx + m2;
+		if (bp[v3] == -2) {
+			double xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]);
+			double ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+			double zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+			double rocpkf = (double)S.rocp[v3];
+			double ival = 0, x = 0, y = 0, z = 0, v[3] = {0, 0, 0};
+			ival += (double)S.ht[v3];
+			if (S.adiabyn == 2 || S.frictyn) {
+				x = (gx[m1] - gx[m1 - 1]) / 2.0 * xkf;
+				y = (gy[m2] - gy[m2 - 1]) / 2.0 * ykf;
+				z = (gz[m3] - gz[m3 - 1]) / 2.0 * zkf;
+			}
+			if (S.adiabyn) vxyz(S.vx, S.vy, S.vz, gx, gy, gz, nx, ny, nz, gx[m1], gy[m2], gz[m3], v);
+			bool act = S.ro[v3] > 2e+3f && gy[m2] < S.ysize - S.stp100;
+			/* cell-centre corners, y fastest: c0 = v3, c1 = +y, c2 = +x,
+			 * c3 = +x+y, c4 = +z, ... c7 = +x+y+z */
+			long c0 = v3, c1 = v3 + 1, c2 = v3 + sx, c3 = c2 + 1;
+			long c4 = v3 + sz, c5 = c4 + 1, c6 = c4 + sx, c7 = c6 + 1;
+			if (S.adiabyn == 2 && act) {
+				const double *pr = S.pr;
+				double dpdx = xkf * ((1.0 - y) * (1.0 - z) * (pr[c2] - pr[c0]) + y * (1.0 - z) * (pr[c3] - pr[c1]) +
+						     (1.0 - y) * z * (pr[c6] - pr[c4]) + y * z * (pr[c7] - pr[c5]));
+				double dpdy = ykf * ((1.0 - x) * (1.0 - z) * (pr[c1] - pr[c0]) + x * (1.0 - z) * (pr[c3] - pr[c2]) +
+						     (1.0 - x) * z * (pr[c5] - pr[c4]) + x * z * (pr[c7] - pr[c6]));
+				double dpdz = zkf * ((1.0 - x) * (1.0 - y) * (pr[c4] - pr[c0]) + x * (1.0 - y) * (pr[c6] - pr[c2]) +
+						     (1.0 - x) * y * (pr[c5] - pr[c1]) + x * y * (pr[c7] - pr[c3]));
+				float at = S.et[v3] * S.tk0[v3];
+				ival += (double)at * (v[0] * dpdx + v[1] * dpdy + v[2] * dpdz);
+			}
+			if (S.adiabyn == 1 && act) {
+				float atr = S.et[v3] * S.tk0[v3] * S.ro[v3];
+				ival += (double)atr * (v[0] * S.gk0 + v[1] * S.gk1 + v[2] * S.gk2);
+			}
+			if (S.frictyn && act) {
+				const float *sxx = S.se, *syy = sxx + N, *szz = syy + N, *sxy = szz + N, *sxz = sxy + N,
+					    *syz = sxz + N, *exx = syz + N, *eyy = exx + N, *ezz = eyy + N, *exy = ezz + N,
+					    *exz = exy + N, *eyz = exz + N;
+				long cc[8] = {c0, c1, c2, c3, c4, c5, c6, c7};
+				double e = 0;
+#pragma unroll
+				for (int q = 0; q < 8; q++) {
+					long u = cc[q];
+					float sn = sxx[u] * exx[u] + syy[u] * eyy[u] + szz[u] * ezz[u];
+					double wx = (q >> 1) & 1 ? x : 1.0 - x, wy = q & 1 ? y : 1.0 - y,
+					       wz = (q >> 2) & 1 ? z : 1.0 - z;
+					e += wx * wy * wz * (double)sn;
+				}
+				e += 2.0 * ((1.0 - z) * (double)(sxy[v3 - sz] * exy[v3 - sz]) + z * (double)(sxy[v3] * exy[v3]));
+				e += 2.0 * ((1.0 - y) * (double)(sxz[v3 - 1] * exz[v3 - 1]) + y * (double)(sxz[v3] * exz[v3]));
+				e += 2.0 * ((1.0 - x) * (double)(syz[v3 - sx] * eyz[v3 - sx]) + x * (double)(syz[v3] * eyz[v3]));
+				ival += e;
+			}
+			src[v3] = (float)ival;
+			mx = fabs(ival / rocpkf);
+		} else {
+			src[v3] = 0.0f;
+		}
+	}
+	for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
+	int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (lane == 0) sh[w] = mx;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double a = 0;
+		for (int k = 0; k < NB / 64; k++) a = fmax(a, sh[k]);
+		part[(long)blockIdx.y * gridDim.x + blockIdx.x] = a;
+	}
+}
+
+__global__ void k_heat_max(long nblk, const double *part, double *res)
+{
+	__shared__ double sh[NB];
+	double a = 0;
+	for (long i = threadIdx.x; i < nblk; i += NB) a = fmax(a, part[i]);
+	sh[threadIdx.x] = a;
+	__syncthreads();
+	for (int o = NB / 2; o > 0; o >>= 1) {
+		if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) res[0] = sh[0];
+}
+
 /* Right-hand side and diagonal of every row of one sub-step; boundary rows
  * take their constant, on the bottom line replaced by the plume ramp of
- * titerate (heat3mg.c:444-455). */
+ * titerate (heat3mg.c:444-455).  src: the heat source as tk1 holds it. */
 __global__ void k_heat_setup(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz,
 			     const float *__restrict__ kt, const float *__restrict__ rocp,
-			     const float *__restrict__ ht, const float *__restrict__ tk0, const int *__restrict__ bp,
+			     const float *__restrict__ src, const float *__restrict__ tk0, const int *__restrict__ bp,
 			     const float *__restrict__ bc, double dt, int plume, double timebeg, double timeend,
 			     double timesum, double tempend, double *b, double *diag, double *T)
 {
@@ -112,7 +232,7 @@ __global__ void k_heat_setup(int nx, int ny, int nz, const double *gx, const dou
 	}
 	double rc = (double)rocp[v3];
 	Cond c = conds(kt, gx, gy, gz, m1, m2, m3, v3, sx, sz);
-	b[v3] = (double)ht[v3] + (double)tk0[v3] / dt * rc;
+	b[v3] = (double)src[v3] + (double)tk0[v3] / dt * rc;
 	double d = rc / dt;
 	d += c.xm; d += c.xp; d += c.ym; d += c.yp; d += c.zm; d += c.zp;
 	diag[v3] = d;
@@ -235,7 +355,7 @@ extern "C" int i3g_heat_destroy(i3g_heat *h)
 	if (!h) return I3G_OK;
 	if (h->s) hipStreamSynchronize(h->s);
 	void *ps[] = {h->gx, h->gy, h->gz, h->bp, h->bc, h->bk, h->blist, h->kt, h->rocp, h->ht, h->tk0,
-		      h->T, h->b, h->diag, h->part, h->res};
+		      h->T, h->b, h->diag, h->part, h->res, h->vx, h->vy, h->vz, h->et, h->ro, h->se, h->pr, h->src};
 	for (void *p : ps)
 		if (p) hipFree(p);
 	if (h->s) hipStreamDestroy(h->s);
@@ -354,6 +474,47 @@ static int bc_apply(i3g_heat *h, double t0koef)
 	return I3G_OK;
 }
 
+extern "C" int i3g_heat_set_sources(i3g_heat *h, int adiabyn, int frictyn, const double *gk, double ysize,
+				    double stp100, const float *vx, const float *vy, const float *vz, const double *pr,
+				    const float *et, const float *const *sig, const float *const *eps)
+{
+	if (!h || adiabyn < 0 || adiabyn > 2 || frictyn < 0 || frictyn > 1) return I3G_EINVAL;
+	h->adiabyn = adiabyn;
+	h->frictyn = frictyn;
+	if (!adiabyn && !frictyn) return I3G_OK;
+	if (!gk || !et || (adiabyn && (!vx || !vy || !vz)) || (adiabyn == 2 && !pr) || (frictyn && (!sig || !eps))) {
+		h->adiabyn = h->frictyn = 0;
+		return I3G_EINVAL;
+	}
+	long N = h->N;
+	int err = 0;
+	if (!h->vx) {
+		h->vx = halloc<float>(N, &err); h->vy = halloc<float>(N, &err); h->vz = halloc<float>(N, &err);
+		h->et = halloc<float>(N, &err); h->ro = halloc<float>(N, &err); h->se = halloc<float>(12 * N, &err);
+		h->pr = halloc<double>(N, &err); h->src = halloc<float>(N, &err);
+		if (err) { h->adiabyn = h->frictyn = 0; return err; }
+	}
+	for (int q = 0; q < 3; q++) h->gk[q] = gk[q];
+	h->ysize = ysize;
+	h->stp100 = stp100;
+	size_t fb = N * sizeof(float);
+	TCHK(hipMemcpyAsync(h->et, et, fb, hipMemcpyHostToDevice, h->s));
+	if (adiabyn) {
+		TCHK(hipMemcpyAsync(h->vx, vx, fb, hipMemcpyHostToDevice, h->s));
+		TCHK(hipMemcpyAsync(h->vy, vy, fb, hipMemcpyHostToDevice, h->s));
+		TCHK(hipMemcpyAsync(h->vz, vz, fb, hipMemcpyHostToDevice, h->s));
+	}
+	if (adiabyn == 2) TCHK(hipMemcpyAsync(h->pr, pr, N * sizeof(double), hipMemcpyHostToDevice, h->s));
+	if (frictyn)
+		for (int q = 0; q < 6; q++) {
+			if (!sig[q] || !eps[q]) { h->adiabyn = h->frictyn = 0; return I3G_EINVAL; }
+			TCHK(hipMemcpyAsync(h->se + q * N, sig[q], fb, hipMemcpyHostToDevice, h->s));
+			TCHK(hipMemcpyAsync(h->se + (6 + q) * N, eps[q], fb, hipMemcpyHostToDevice, h->s));
+		}
+	TCHK(hipStreamSynchronize(h->s));
+	return I3G_OK;
+}
+
 extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const float *cp, const float *kt,
 			      const float *ht, const i3g_heat_params *p, i3g_heat_stats *st)
 {
@@ -362,16 +523,34 @@ extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const flo
 	auto t_start = std::chrono::steady_clock::now();
 	long N = h->N;
 	i3g_heat_stats s{};
-	/* temperature step limit (heat3mg.c:327-368) from max |H/(ro cp)| over
-	 * the non-boundary nodes */
+	/* temperature step limit (heat3mg.c:327-368) from max |src/(ro cp)|
+	 * over the non-boundary nodes; src = H, or computed on the device with
+	 * the adiabatic/shear terms */
 	std::vector<float> rocp(N);
+	for (long i = 0; i < N; i++) rocp[i] = ro[i] * cp[i];
+	bool terms = h->adiabyn || h->frictyn;
+	dim3 gg((unsigned)(((long)h->nx * h->ny + NB - 1) / NB), (unsigned)h->nz);
+	long nblk = (long)gg.x * gg.y;
 	double maxrate = 0;
-	for (long i = 0; i < N; i++) {
-		rocp[i] = ro[i] * cp[i];
-		if (h->hb[i] == 0) {
-			double r = std::fabs((double)ht[i] / (double)rocp[i]);
-			if (r > maxrate) maxrate = r;
-		}
+	if (terms) {
+		TCHK(hipMemcpyAsync(h->rocp, rocp.data(), N * sizeof(float), hipMemcpyHostToDevice, h->s));
+		TCHK(hipMemcpyAsync(h->ht, ht, N * sizeof(float), hipMemcpyHostToDevice, h->s));
+		TCHK(hipMemcpyAsync(h->tk0, tk, N * sizeof(float), hipMemcpyHostToDevice, h->s));
+		TCHK(hipMemcpyAsync(h->ro, ro, N * sizeof(float), hipMemcpyHostToDevice, h->s));
+		HeatSrc S{h->adiabyn, h->frictyn, h->gk[0], h->gk[1], h->gk[2], h->ysize, h->stp100, h->ht, h->rocp,
+			  h->ro, h->tk0, h->et, h->vx, h->vy, h->vz, h->se, h->pr};
+		hipLaunchKernelGGL(k_heat_source, gg, dim3(NB), 0, h->s, h->nx, h->ny, h->nz, (const double *)h->gx,
+				   (const double *)h->gy, (const double *)h->gz, (const int *)h->bp, S, h->src, h->part);
+		hipLaunchKernelGGL(k_heat_max, dim3(1), dim3(NB), 0, h->s, nblk, (const double *)h->part, h->res);
+		TCHK(hipGetLastError());
+		TCHK(hipMemcpyAsync(&maxrate, h->res, sizeof(double), hipMemcpyDeviceToHost, h->s));
+		TCHK(hipStreamSynchronize(h->s));
+	} else {
+		for (long i = 0; i < N; i++)
+			if (h->hb[i] == 0) {
+				double r = std::fabs((double)ht[i] / (double)rocp[i]);
+				if (r > maxrate) maxrate = r;
+			}
 	}
 	if (p->timestep <= 0) {
 		/* the reference returns without a solve when the step is zero */
@@ -389,8 +568,6 @@ extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const flo
 	TCHK(hipMemcpyAsync(h->ht, ht, N * sizeof(float), hipMemcpyHostToDevice, h->s));
 	TCHK(hipMemcpyAsync(h->tk0, tk, N * sizeof(float), hipMemcpyHostToDevice, h->s));
 	int plume = p->timesum > p->timebeg ? (p->timesum >= p->timeend ? 2 : 1) : 0;
-	dim3 gg((unsigned)(((long)h->nx * h->ny + NB - 1) / NB), (unsigned)h->nz);
-	long nblk = (long)gg.x * gg.y;
 	hipEvent_t e0, e1;
 	TCHK(hipEventCreate(&e0));
 	TCHK(hipEventCreate(&e1));
@@ -403,7 +580,7 @@ extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const flo
 		if (dt > total - done) dt = total - done;
 		hipLaunchKernelGGL(k_heat_setup, gg, dim3(NB), 0, h->s, h->nx, h->ny, h->nz, (const double *)h->gx,
 				   (const double *)h->gy, (const double *)h->gz, (const float *)h->kt,
-				   (const float *)h->rocp, (const float *)h->ht, (const float *)h->tk0,
+				   (const float *)h->rocp, (const float *)(terms ? h->src : h->ht), (const float *)h->tk0,
 				   (const int *)h->bp, (const float *)h->bc, dt, plume, p->timebeg, p->timeend, p->timesum,
 				   p->tempend, h->b, h->diag,
 				   h->T);

@@ -20,9 +20,9 @@
  *                  the stress/strain-rate fields (value mode of
  *                  sxxcalc..syzcalc, on the device) and the marker
  *                  time-step limit (move3mg.c:2609-2632).
- *   i3g_titerate   the heat step of titerate (heat3mg.c:287-588) without
- *                  adiabatic/shear heating, then the marker temperature
- *                  update (heat3mg.c:592-782).
+ *   i3g_titerate   the heat step of titerate (heat3mg.c:287-588) with the
+ *                  adiabyn/frictyn heating terms, then the marker
+ *                  temperature update (heat3mg.c:592-782).
  *   i3g_ronurecalc the node fields of ronurecalc (mark3mg.c:404-1015) for
  *                  Newtonian rock types, with tkrecalc.
  *   i3g_movemark   the RK4 position update of movemark (mark3mg.c:215-284).
@@ -199,6 +199,15 @@ void i3g_titerate(int m0)
 	hp.sweeps = multittt[0][0] > 0 ? multittt[0][0] : 1;
 	hp.timesum = timesum; hp.timebeg = timebeg; hp.timeend = timeend; hp.tempend = tempend;
 	hp.verbose = i3g_glue_verbose;
+	{
+		/* heaterr's adiabatic and shear heating from the current
+		 * velocity, pressure and stress/strain-rate fields */
+		double gk[3] = {GXKOEF, GYKOEF, GZKOEF};
+		const float *sig[6] = {sxx, syy, szz, sxy, sxz, syz};
+		const float *eps[6] = {exx, eyy, ezz, exy, exz, eyz};
+		rc = i3g_heat_set_sources(G.heat, adiabyn, frictyn, gk, ysize, stp100, vx, vy, vz, pr, et, sig, eps);
+		if (rc) glue_fail("heat_set_sources", rc);
+	}
 	rc = i3g_heat_solve(G.heat, tk, ro, cp, kt, ht, &hp, &hs);
 	if (rc && rc != I3G_ENOCONV) glue_fail("heat_solve", rc);
 	if (hs.substeps == 0) return; /* zero step: the reference returns before the marker update */

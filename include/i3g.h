@@ -282,6 +282,19 @@ typedef struct {
 int i3g_heat_create(const i3g_setup *s, i3g_heat **out);
 int i3g_heat_destroy(i3g_heat *h);
 int i3g_heat_default_params(i3g_heat_params *p);
+/* Adiabatic and shear heating in heaterr's right-hand side
+ * (heat3mg.c:67-160; the reference's adiabyn / frictyn switches) for the
+ * following i3g_heat_solve calls: adiabyn 1 adds alpha T rho (g.v), 2 adds
+ * alpha T (v.grad P); frictyn 1 adds sigma:eps; both only where
+ * rho > 2000 and y < ysize - stp100, with v from vxyzcalc1 at the node and
+ * T the temperature at the start of the step.  gk: GXKOEF, GYKOEF, GZKOEF;
+ * vx, vy, vz: staggered velocities (float); pr: pressure (double); et: the
+ * expansion coefficient (float); sig / eps: 6 float planes each (sxx syy
+ * szz sxy sxz syz, exx .. eyz) as the reference stores them.  Fields are
+ * copied at the call.  adiabyn = frictyn = 0 switches the terms off. */
+int i3g_heat_set_sources(i3g_heat *h, int adiabyn, int frictyn, const double *gk, double ysize,
+			 double stp100, const float *vx, const float *vy, const float *vz, const double *pr,
+			 const float *et, const float *const *sig, const float *const *eps);
 /* tk (in/out, float, level-0 nodes) is advanced by p->timestep; ro, cp,
  * kt, ht are the node fields of ronurecalc.  The start temperature (the
  * reference's tk2) is the caller's input copy. */

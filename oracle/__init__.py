@@ -197,13 +197,14 @@ class HeatSys(C.Structure):
                 ("tk0", C.c_void_p), ("dt", C.c_double),
                 ("bondm_t", C.c_void_p), ("bondv", C.c_void_p), ("bondn", C.c_void_p),
                 ("timesum", C.c_double), ("timebeg", C.c_double), ("timeend", C.c_double),
-                ("tempend", C.c_double)]
+                ("tempend", C.c_double), ("src", C.c_void_p)]
 
 
 def heat_sys(gx, gy, gz, ro, cp, kt, ht, tk0, dt, bondm, bondv, bondn, timesum=0.0, timebeg=1e30,
-             timeend=1e30, tempend=0.0):
+             timeend=1e30, tempend=0.0, src=None):
     """Level-0 heat system descriptor; bondm may be the full reference table
-    (its T block at 8*nodenum is taken) or the T block itself."""
+    (its T block at 8*nodenum is taken) or the T block itself.  src: the
+    heaterr source per node (heat_source), default the radiogenic ht."""
     gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
     N = len(gx) * len(gy) * len(gz)
     bondm = np.asarray(bondm)
@@ -212,9 +213,37 @@ def heat_sys(gx, gy, gz, ro, cp, kt, ht, tk0, dt, bondm, bondv, bondn, timesum=0
         bt, np.ascontiguousarray(bondv, np.float32), np.ascontiguousarray(bondn, np.int32)]
     H = HeatSys(len(gx), len(gy), len(gz), *[_p(a) for a in keep[:3]], *[_p(a) for a in keep[3:8]], dt,
                 *[_p(a) for a in keep[8:]], timesum, timebeg, timeend, tempend)
+    if src is not None:
+        src = np.ascontiguousarray(src, np.float32)
+        keep.append(src)
+        H.src = src.ctypes.data
     H._keep = keep
     H.N = N
     return H
+
+
+def heat_source(gx, gy, gz, ro, cp, ht, tk0, et, vx, vy, vz, pr, s6, e6, bondm, adiabyn, frictyn,
+                gk=(0.0, 9.81, 0.0), ysize=None, stp100=0.0):
+    """(src, rate): heaterr's right-hand side with adiabatic (adiabyn 1/2)
+    and shear (frictyn) heating as stored in tk1 (float32), and the double
+    rate src/(ro cp) it returns, per level-0 node."""
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
+    N = len(gx) * len(gy) * len(gz)
+    bondm = np.asarray(bondm)
+    bt = np.ascontiguousarray(bondm[8 * N:9 * N] if len(bondm) > N else bondm, np.int32)
+    f = [np.ascontiguousarray(a, np.float32) for a in (ro, cp, ht, tk0, et, vx, vy, vz)]
+    prd = np.ascontiguousarray(pr, np.float64)
+    s6 = [np.ascontiguousarray(a, np.float32) for a in s6]
+    e6 = [np.ascontiguousarray(a, np.float32) for a in e6]
+    sp = (C.c_void_p * 6)(*[a.ctypes.data for a in s6])
+    ep = (C.c_void_p * 6)(*[a.ctypes.data for a in e6])
+    gkd = np.ascontiguousarray(gk, np.float64)
+    src, rate = np.zeros(N, np.float32), np.zeros(N)
+    lib().or_heat_source(C.c_long(len(gx)), C.c_long(len(gy)), C.c_long(len(gz)), _p(gx), _p(gy), _p(gz),
+                         *[_p(a) for a in f], _p(prd), sp, ep, _p(bt), C.c_int(adiabyn), C.c_int(frictyn),
+                         _p(gkd), C.c_double(gy[-1] if ysize is None else ysize), C.c_double(stp100),
+                         _p(src), _p(rate))
+    return src, rate
 
 
 def heat_matrix(H):

@@ -864,7 +864,7 @@ static int heat_row(const or_heat_t *H, long m1, long m2, long m3, int *col, dou
 	long v[7] = {v3 - nxy, v3 - ny, v3 - 1, v3, v3 + 1, v3 + ny, v3 + nxy};
 	double rocp = (float)(H->ro[v3] * H->cp[v3]);
 	double iv;
-	*rhs = (double)H->ht[v3] + H->tk0[v3] / H->dt * rocp;
+	*rhs = (H->src ? H->src[v3] : (double)H->ht[v3]) + H->tk0[v3] / H->dt * rocp;
 	val[0] = rocp / H->dt;
 	col[0] = (int)v3;
 	iv = (float)(kt[v[1]] + kt[v[3]]) / 2.0 * xkf / (gx[m1] - gx[m1 - 1]);
@@ -902,7 +902,7 @@ double or_heat_maxrate(const or_heat_t *H)
 	for (v = 0; v < N; v++)
 		if (H->bondm_t[v] == 0) {
 			double rocp = (float)(H->ro[v] * H->cp[v]);
-			double r = fabs((double)H->ht[v] / rocp);
+			double r = fabs((H->src ? H->src[v] : (double)H->ht[v]) / rocp);
 			if (r > mx) mx = r;
 		}
 	return mx;
@@ -951,7 +951,7 @@ void or_heat_norms(const or_heat_t *H, const double *x, double *out)
 				e += (double)(tk[v3] - tk[v[0]]) * iv;
 				iv = (float)(kt[v[3]] + kt[v[6]]) / 2.0 * zkf / (gz[m3 + 1] - gz[m3]) / rocp;
 				e -= (double)(tk[v[6]] - tk[v3]) * iv;
-				e -= (double)H->ht[v3] / rocp; /* tk1 = H (no adiabatic/shear terms) */
+				e -= (H->src ? H->src[v3] : (double)H->ht[v3]) / rocp; /* tk1 */
 				e *= H->dt;
 				hs += e * e;
 				hn += 1;
@@ -1168,4 +1168,73 @@ void or_err_rows(const orl_t *L, const float *nu, const float *ro, const double 
 		}
 	}
 #undef ROW
+}
+
+/* heat3mg.c:54-160.  Float products stay float where the reference
+ * multiplies two float arrays (et*tk0, ro*cp, sigma*eps). */
+void or_heat_source(long nx, long ny, long nz, const double *gx, const double *gy, const double *gz,
+		    const float *ro, const float *cp, const float *ht, const float *tk0, const float *et,
+		    const float *vx, const float *vy, const float *vz, const double *pr, const float *const *s6,
+		    const float *const *e6, const int *bondm_t, int adiabyn, int frictyn, const double *gk,
+		    double ysize, double stp100, float *src, double *rate)
+{
+	long nxy = nx * ny, m1, m2, m3;
+	for (m3 = 0; m3 < nz; m3++)
+	for (m1 = 0; m1 < nx; m1++)
+	for (m2 = 0; m2 < ny; m2++) {
+		long v3 = m3 * nxy + m1 * ny + m2;
+		src[v3] = 0;
+		rate[v3] = 0;
+		if (bondm_t[v3] != 0) continue;
+		double xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]);
+		double ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+		double zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+		double rocpkf = (float)(ro[v3] * cp[v3]);
+		double ival = 0, x = 0, y = 0, z = 0, v[3] = {0, 0, 0};
+		/* P cell centres around the node, corner order y fastest, then x,
+		 * then z: c[0] = v3, c[1] = +y, c[2] = +x, ..., c[7] = +x+y+z */
+		long c[8];
+		int q;
+		for (q = 0; q < 8; q++) c[q] = v3 + (q & 1) + ((q >> 1) & 1) * ny + ((q >> 2) & 1) * nxy;
+		ival += ht[v3];
+		if (adiabyn == 2 || frictyn) {
+			x = (gx[m1] - gx[m1 - 1]) / 2.0 * xkf;
+			y = (gy[m2] - gy[m2 - 1]) / 2.0 * ykf;
+			z = (gz[m3] - gz[m3 - 1]) / 2.0 * zkf;
+		}
+		if (adiabyn) or_vxyz(vx, vy, vz, gx, nx, gy, ny, gz, nz, gx[m1], gy[m2], gz[m3], v);
+		int act = ro[v3] > 2e+3 && gy[m2] < ysize - stp100;
+		if (adiabyn == 2 && act) {
+			/* trilinear P gradient between the cell centres */
+			double dpdx = xkf * ((1.0 - y) * (1.0 - z) * (pr[c[2]] - pr[c[0]]) + y * (1.0 - z) * (pr[c[3]] - pr[c[1]]) +
+					     (1.0 - y) * z * (pr[c[6]] - pr[c[4]]) + y * z * (pr[c[7]] - pr[c[5]]));
+			double dpdy = ykf * ((1.0 - x) * (1.0 - z) * (pr[c[1]] - pr[c[0]]) + x * (1.0 - z) * (pr[c[3]] - pr[c[2]]) +
+					     (1.0 - x) * z * (pr[c[5]] - pr[c[4]]) + x * z * (pr[c[7]] - pr[c[6]]));
+			double dpdz = zkf * ((1.0 - x) * (1.0 - y) * (pr[c[4]] - pr[c[0]]) + x * (1.0 - y) * (pr[c[6]] - pr[c[2]]) +
+					     (1.0 - x) * y * (pr[c[5]] - pr[c[1]]) + x * y * (pr[c[7]] - pr[c[3]]));
+			float at = et[v3] * tk0[v3];
+			ival += at * (v[0] * dpdx + v[1] * dpdy + v[2] * dpdz);
+		}
+		if (adiabyn == 1 && act) {
+			float atr = et[v3] * tk0[v3] * ro[v3];
+			ival += atr * (v[0] * gk[0] + v[1] * gk[1] + v[2] * gk[2]);
+		}
+		if (frictyn && act) {
+			/* normal terms: trilinear over the eight centres */
+			double wx[2] = {1.0 - x, x}, wy[2] = {1.0 - y, y}, wz[2] = {1.0 - z, z};
+			double e = 0;
+			for (q = 0; q < 8; q++) {
+				long u = c[q];
+				float se = s6[0][u] * e6[0][u] + s6[1][u] * e6[1][u] + s6[2][u] * e6[2][u];
+				e += wx[(q >> 1) & 1] * wy[q & 1] * wz[(q >> 2) & 1] * se;
+			}
+			/* shear terms: the two edge nodes along the third axis */
+			e += 2.0 * ((1.0 - z) * (s6[3][v3 - nxy] * e6[3][v3 - nxy]) + z * (s6[3][v3] * e6[3][v3]));
+			e += 2.0 * ((1.0 - y) * (s6[4][v3 - 1] * e6[4][v3 - 1]) + y * (s6[4][v3] * e6[4][v3]));
+			e += 2.0 * ((1.0 - x) * (s6[5][v3 - ny] * e6[5][v3 - ny]) + x * (s6[5][v3] * e6[5][v3]));
+			ival += e;
+		}
+		src[v3] = (float)ival;
+		rate[v3] = ival / rocpkf;
+	}
 }

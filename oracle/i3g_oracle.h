@@ -121,7 +121,22 @@ typedef struct {
 	const float *bondv;             /* bondv1[r][0..1] at [2r..2r+1] */
 	const int *bondn;               /* bondn1[r] = partner node + 1 or 0 */
 	double timesum, timebeg, timeend, tempend; /* bottom-line plume override */
+	const float *src;               /* heaterr source as stored in tk1 (float), NULL = ht */
 } or_heat_t;
+
+/* heaterr right-hand-side mode (ynerr=0, heat3mg.c:54-160) at every
+ * non-boundary level-0 node: src = radiogenic heat + adiabatic heating
+ * (adiabyn 1: alpha T rho (g.v), 2: alpha T (v.grad P)) + shear heating
+ * (frictyn: sigma:eps), the two terms only where rho > 2000 and
+ * y < ysize - stp100; rate = src / (ro cp).  tk0 is the temperature at the
+ * start of the step, v at the node is vxyzcalc1's, s6/e6 are sxx syy szz
+ * sxy sxz syz and exx .. eyz.  src is stored as float, as tk1; rate is the
+ * double quotient heaterr returns.  Boundary nodes get 0. */
+void or_heat_source(long nx, long ny, long nz, const double *gx, const double *gy, const double *gz,
+		    const float *ro, const float *cp, const float *ht, const float *tk0, const float *et,
+		    const float *vx, const float *vy, const float *vz, const double *pr, const float *const *s6,
+		    const float *const *e6, const int *bondm_t, int adiabyn, int frictyn, const double *gk,
+		    double ysize, double stp100, float *src, double *rate);
 
 /* Rows of the level-0 heat system (7 slots per node: cnt entries of
  * cols/vals at 7*node) and right-hand side, heat3mg.c:160-225, 818-841. */

@@ -244,3 +244,14 @@ void refh_err_row(int kind, long m1, long m2, long m3, double *out)
 	}
 	out[1] = eps1[0];
 }
+
+/* heaterr in right-hand-side mode (ynerr=0, heat3mg.c:54-160) at one level-0
+ * node: out[0] = the returned rate ival/(ro cp), out[1] = the source it
+ * stores in tk1 (radiogenic + adiabatic + shear heating). */
+void refh_heat_rhs(long m1, long m2, long m3, double *out)
+{
+	long un1[MAXPOS];
+	double ui1[MAXPOS], eps1[100];
+	out[0] = heaterr(m1, m2, m3, 0, 0, eps1, un1, ui1);
+	out[1] = tk1[m3 * mgxy[0] + m1 * mgy[0] + m2];
+}

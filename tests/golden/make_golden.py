@@ -357,10 +357,81 @@ def g7(R):
     print("g7_stress_norms.npz written")
 
 
+def g8(R, n=17):
+    """heaterr's right-hand side with adiabatic and shear heating
+    (heat3mg.c:54-160) at every interior node for (adiabyn, frictyn) =
+    (1, 0), (2, 1), (0, 1), and one titerate step (heat3mg.c:287-588) with
+    adiabyn 2 and frictyn 1, through the reference on configuration-0
+    geometry with random fields (some rock below 2000 kg/m^3, a top layer
+    above ysize - stp100)."""
+    import ctypes as C
+    rng = np.random.default_rng(808)
+    cfg = I.make_config(0, n, n, n, seed=1)
+    gx, gy, gz = cfg["gx"], cfg["gy"], cfg["gz"]
+    N = n ** 3
+    R.setup_grid(gx, gy, gz, 0)
+    R.set_bc(cfg["bondm"], cfg["bondv"], cfg["bondn"])
+    f32 = lambda a: np.asarray(a, np.float32)
+    ro = f32(rng.uniform(1800, 3400, N))
+    cp = f32(rng.uniform(900, 1300, N))
+    kt = f32(rng.uniform(1.5, 4.5, N))
+    ht = f32(rng.uniform(0, 2e-6, N))
+    tk = f32(cfg["tk"] + rng.uniform(-50, 50, N))
+    et = f32(rng.uniform(1e-5, 4e-5, N))
+    vel = {k: f32(rng.standard_normal(N) * 1e-9) for k in ("vx", "vy", "vz")}
+    yy = np.tile(gy, n * n)  # y of node m3*n*n + m1*n + m2
+    pr = 3300.0 * 9.81 * yy + rng.uniform(-1e6, 1e6, N)
+    s6 = {k: f32(rng.standard_normal(N) * 1e6) for k in ("sxx", "syy", "szz", "sxy", "sxz", "syz")}
+    e6 = {k: f32(rng.standard_normal(N) * 1e-14) for k in ("exx", "eyy", "ezz", "exy", "exz", "eyz")}
+    gk = np.array([0.3, 9.81, -0.2])
+    ysize, stp100 = float(gy[-1]), float(0.15 * gy[-1])
+    for k, v in dict(ysize=ysize, stp100=stp100, GXKOEF=gk[0], GYKOEF=gk[1], GZKOEF=gk[2]).items():
+        R.set(k, v)
+    for name, a in dict(tk=tk, tk0=tk, ro=ro, cp=cp, kt=kt, ht=ht, et=et, **vel, **s6, **e6).items():
+        R.f32(name)[:N] = a
+    R.f64("pr")[:N] = pr
+    bt = cfg["bondm"][8 * N:9 * N]
+    out = (C.c_double * 2)()
+    res = {}
+    for a, fr in ((1, 0), (2, 1), (0, 1)):
+        R.set("adiabyn", a, C.c_int)
+        R.set("frictyn", fr, C.c_int)
+        rate = np.zeros(N)
+        src = np.zeros(N, np.float32)
+        for m3 in range(n):
+            for m1 in range(n):
+                for m2 in range(n):
+                    v = m3 * n * n + m1 * n + m2
+                    if bt[v] != 0:
+                        continue
+                    R.lib.refh_heat_rhs(C.c_long(m1), C.c_long(m2), C.c_long(m3), out)
+                    rate[v], src[v] = out[0], out[1]
+        res[f"rate_{a}{fr}"] = rate
+        res[f"src_{a}{fr}"] = src
+    # one implicit step with the terms on, three sub-steps
+    dt = 1e13
+    maxtkstep = np.abs(res["rate_21"]).max() * dt / 2.5
+    for k, v in dict(HEATMIN=1e-2, t0koef=1.0, t1koef=1.0, heatdif=0.0, timestep=dt, maxtkstep=maxtkstep,
+                     timesum=0.0, timebeg=1e30, timeend=2e30, tempend=0.0).items():
+        R.set(k, v)
+    for k, v in dict(multinumt=0, multicyct=1, frictyn=1, adiabyn=2).items():
+        R.set(k, v, C.c_int)
+    R.i32("multittt", 100)[0] = 2000
+    R.set("marknum", 0, C.c_long)
+    R.f32("tk")[:N] = tk
+    R.lib.titerate(C.c_int(0))
+    tk_out = R.f32("tk")[:N].copy()
+    np.savez_compressed(os.path.join(HERE, "g8_heat_sources.npz"), n=n, gx=gx, gy=gy, gz=gz, bondm=cfg["bondm"],
+                        bondv=cfg["bondv"], bondn=cfg["bondn"], ro=ro, cp=cp, kt=kt, ht=ht, tk=tk, et=et, pr=pr,
+                        gk=gk, ysize=ysize, stp100=stp100, dt=dt, maxtkstep=maxtkstep, heatmin=1e-2, sweeps=2000,
+                        tk_out=tk_out, **vel, **s6, **e6, **res)
+    print("g8_heat_sources.npz written")
+
+
 if __name__ == "__main__":
     R = Ref()
     R.lib.refh_set_threads(1)
-    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7"]
+    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8"]
     if "g1" in which:
         g1(R)
     if "g3" in which:
@@ -372,3 +443,5 @@ if __name__ == "__main__":
         g6(R)
     if "g7" in which:
         g7(R)
+    if "g8" in which:
+        g8(R)

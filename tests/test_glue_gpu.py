@@ -159,3 +159,46 @@ def test_glue_movemark_matches_oracle(gpu):
         a = R.arr(name, C.c_float, R.MAXMRK)[:m]
         ulp = np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
         assert ulp.max() <= 1, (name, ulp.max())
+
+
+def test_glue_titerate_with_heating_terms_matches_reference(gpu):
+    """i3g_titerate on the reference's globals with adiabyn 2 and frictyn 1
+    (g8, written from the reference's own titerate): the glue passes
+    vx/vy/vz, pr, et and the stress/strain-rate fields to the device heat
+    step, which reproduces the reference temperature."""
+    g = golden("g8_heat_sources.npz")
+    n = int(g["n"])
+    N = n ** 3
+    print("case: glue titerate with heating terms", n, flush=True)
+    R = ref()
+    R.setup_grid(g["gx"], g["gy"], g["gz"], 0)
+    R.set_bc(g["bondm"], g["bondv"], g["bondn"])
+    R.set("bondnum", len(g["bondn"]) - 1, C.c_long)
+    for k, v in dict(HEATMIN=1e-8, t0koef=1.0, heatdif=0.0, timestep=float(g["dt"]), maxtkstep=float(g["maxtkstep"]),
+                     timesum=0.0, timebeg=1e30, timeend=2e30, tempend=0.0, zdeep=1e30, tdeep=0.0,
+                     ysize=float(g["ysize"]), stp100=float(g["stp100"]), GXKOEF=float(g["gk"][0]),
+                     GYKOEF=float(g["gk"][1]), GZKOEF=float(g["gk"][2])).items():
+        R.set(k, v)
+    R.set("adiabyn", 2, C.c_int)
+    R.set("frictyn", 1, C.c_int)
+    R.i32("multittt", 100)[0] = 10
+    # a few markers inside the grid (their update follows the solve)
+    rng = np.random.default_rng(3)
+    m = 64
+    R.set("marknum", m, C.c_long)
+    for k, gl in (("markx", g["gx"]), ("marky", g["gy"]), ("markz", g["gz"])):
+        R.arr(k, C.c_float, R.MAXMRK)[:m] = rng.uniform(gl[1], gl[-2], m)
+    R.arr("markt", C.c_byte, R.MAXMRK)[:m] = 2
+    R.arr("markk", C.c_float, R.MAXMRK)[:m] = 1000.0
+    R.arr("markv", C.c_float, R.MAXMRK)[:m] = 0
+    for k in ("tk", "ro", "cp", "kt", "ht", "et", "vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz",
+              "exx", "eyy", "ezz", "exy", "exz", "eyz"):
+        R.f32(k)[:N] = g[k]
+    R.f64("pr")[:N] = g["pr"]
+    R.lib.i3g_titerate(C.c_int(0))
+    R.set("adiabyn", 0, C.c_int)
+    R.set("frictyn", 0, C.c_int)
+    tk = R.f32("tk")[:N].copy()
+    rel = np.linalg.norm(tk.astype(np.float64) - g["tk_out"]) / np.linalg.norm(g["tk_out"])
+    print("tk rel L2", rel, flush=True)
+    assert rel < 1e-7

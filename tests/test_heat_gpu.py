@@ -118,3 +118,56 @@ def test_marker_tk_update_no_diffusion_unsorted(gpu):
                               float(g["zdeep"]), float(g["tdeep"]))
     assert np.array_equal(d["tk"], ref)
     M.close()
+
+
+def _g8_sources(g):
+    return dict(gk=g["gk"], ysize=float(g["ysize"]), stp100=float(g["stp100"]), vx=g["vx"], vy=g["vy"], vz=g["vz"],
+                pr=g["pr"], et=g["et"], sig=[g[k] for k in ("sxx", "syy", "szz", "sxy", "sxz", "syz")],
+                eps=[g[k] for k in ("exx", "eyy", "ezz", "exy", "exz", "eyz")])
+
+
+def test_heat_step_with_sources_matches_reference(gpu):
+    """titerate with adiabatic (adiabyn 2) and shear (frictyn 1) heating
+    (g8, written from the compiled reference): same three sub-steps, the
+    converged temperature within 1e-7 relative L2."""
+    g = golden("g8_heat_sources.npz")
+    print("case: g8 heat step with sources", flush=True)
+    H = _heat(g)
+    H.set_sources(2, 1, **_g8_sources(g))
+    tk, st = H.solve(g["tk"], g["ro"], g["cp"], g["kt"], g["ht"], timestep=float(g["dt"]),
+                     maxtkstep=float(g["maxtkstep"]), heatmin=1e-8, sweeps=10)
+    ref = g["tk_out"]
+    rel = np.linalg.norm(tk.astype(np.float64) - ref) / np.linalg.norm(ref)
+    print("substeps", st.substeps, "sweeps", st.sweeps, "rel L2", rel, "max", np.abs(tk - ref).max(), flush=True)
+    assert st.converged and st.substeps == 3
+    assert rel < 1e-7
+    assert np.abs(tk - ref).max() < 2e-4
+    H.close()
+
+
+def test_heat_sources_adiabyn1_and_off_match_oracle(gpu):
+    """adiabyn 1 (alpha T rho g.v) against the oracle's source and a
+    direct solve of its rows; then the terms switched off again give the
+    plain step."""
+    g = golden("g8_heat_sources.npz")
+    H = _heat(g)
+    src_kw = _g8_sources(g)
+    s6, e6 = src_kw["sig"], src_kw["eps"]
+    for a, fr in ((1, 0), (0, 0)):
+        print(f"case: g8 sources adiabyn={a} frictyn={fr}", flush=True)
+        H.set_sources(a, fr, **src_kw)
+        dt = 3e12
+        tk, st = H.solve(g["tk"], g["ro"], g["cp"], g["kt"], g["ht"], timestep=dt, maxtkstep=1e9, heatmin=1e-8,
+                         sweeps=10)
+        src, rate = oracle.heat_source(g["gx"], g["gy"], g["gz"], g["ro"], g["cp"], g["ht"], g["tk"], g["et"], g["vx"],
+                                       g["vy"], g["vz"], g["pr"], s6, e6, g["bondm"], a, fr, gk=g["gk"],
+                                       ysize=float(g["ysize"]), stp100=float(g["stp100"]))
+        Hs = oracle.heat_sys(g["gx"], g["gy"], g["gz"], g["ro"], g["cp"], g["kt"], g["ht"], g["tk"], dt, g["bondm"],
+                             g["bondv"], g["bondn"], src=src)
+        A, b = oracle.heat_matrix(Hs)
+        ref = spl.spsolve(A.tocsc(), b)
+        rel = np.linalg.norm(tk - ref) / np.linalg.norm(ref)
+        print("  substeps", st.substeps, "rel L2", rel, flush=True)
+        assert st.converged and st.substeps == 1
+        assert rel < 1e-7
+    H.close()

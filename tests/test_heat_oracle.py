@@ -149,3 +149,107 @@ def test_marker_tk_update_no_diffusion_is_interpolated_change():
     xi = np.interp(g["x"][inside], gx, np.arange(len(gx)))
     expect = mk0[inside] + (5.0 + 0.25 * xi)
     assert np.abs(mk[inside] - expect).max() < 1e-3
+
+
+# ---- adiabatic and shear heating (g8) --------------------------------------
+
+def _g8_source(g, a, fr, **kw):
+    s6 = [g[k] for k in ("sxx", "syy", "szz", "sxy", "sxz", "syz")]
+    e6 = [g[k] for k in ("exx", "eyy", "ezz", "exy", "exz", "eyz")]
+    args = dict(gk=g["gk"], ysize=float(g["ysize"]), stp100=float(g["stp100"]))
+    args.update(kw)
+    return oracle.heat_source(g["gx"], g["gy"], g["gz"], g["ro"], g["cp"], g["ht"], g["tk"], g["et"], g["vx"], g["vy"],
+                              g["vz"], g["pr"], s6, e6, g["bondm"], a, fr, **args)
+
+
+def test_heat_source_matches_reference_bitwise():
+    """heaterr ynerr=0 of the compiled reference (g8): the stored tk1 and
+    the returned rate at every interior node, for adiabyn 1 and 2 and
+    frictyn 1, equal to the bit."""
+    g = golden("g8_heat_sources.npz")
+    for a, fr in ((1, 0), (2, 1), (0, 1)):
+        src, rate = _g8_source(g, a, fr)
+        assert np.array_equal(src, g[f"src_{a}{fr}"]), (a, fr)
+        assert np.array_equal(rate, g[f"rate_{a}{fr}"]), (a, fr)
+
+
+def test_heat_source_terms_switch_and_mask():
+    """Without adiabyn/frictyn the source is the radiogenic heat; the terms
+    act only where rho > 2000 and y < ysize - stp100 (plausible slips --
+    a dropped mask or a wrong sign of the gradient -- change these)."""
+    g = golden("g8_heat_sources.npz")
+    N = len(g["ro"])
+    bt = g["bondm"][8 * N:9 * N]
+    inner = bt == 0
+    src0, _ = _g8_source(g, 0, 0)
+    assert np.array_equal(src0[inner], g["ht"][inner])
+    n = int(g["n"])
+    y = np.tile(g["gy"], n * n)
+    off = inner & ((g["ro"] <= 2000) | (y >= float(g["ysize"]) - float(g["stp100"])))
+    assert off.sum() > 100 and (inner & ~off).sum() > 100
+    for a, fr in ((1, 0), (2, 1)):
+        src, _ = _g8_source(g, a, fr)
+        assert np.array_equal(src[off], g["ht"][off])
+        assert np.abs(src[inner & ~off] - g["ht"][inner & ~off]).max() > 0
+
+
+def test_adiabatic_term_closed_forms():
+    """adiabyn 1 in a uniform upward flow: alpha T rho g.v exactly; adiabyn
+    2 with a linear pressure P = c y and the same flow: alpha T c v_y."""
+    g = golden("g8_heat_sources.npz")
+    N = len(g["ro"])
+    bt = g["bondm"][8 * N:9 * N]
+    n = int(g["n"])
+    y = np.tile(g["gy"], n * n)
+    one = np.ones(N, np.float32)
+    sel = (bt == 0) & (y < float(g["ysize"]) - float(g["stp100"]))
+    z6 = [0 * one] * 6
+    vy = 2e-9
+    ht = 0 * one
+    src1, _ = oracle.heat_source(g["gx"], g["gy"], g["gz"], 3000 * one, 1000 * one, ht, 1000 * one, 3e-5 * one, 0 * one,
+                                 vy * one, 0 * one, 0 * y, z6, z6, g["bondm"], 1, 0, gk=(0.0, 9.81, 0.0),
+                                 ysize=float(g["ysize"]), stp100=float(g["stp100"]))
+    expect1 = np.float32(np.float32(3e-5) * np.float32(1000) * np.float32(3000)) * (vy * 9.81)
+    assert np.allclose(src1[sel], expect1, rtol=1e-6)
+    c = 3.2e4
+    src2, _ = oracle.heat_source(g["gx"], g["gy"], g["gz"], 3000 * one, 1000 * one, ht, 1000 * one, 3e-5 * one, 0 * one,
+                                 vy * one, 0 * one, c * y, z6, z6, g["bondm"], 2, 0, gk=(0.0, 9.81, 0.0),
+                                 ysize=float(g["ysize"]), stp100=float(g["stp100"]))
+    # the P nodes sit at cell centres: (y_c(m2+1) - y_c(m2)) ... c y is linear, so dP/dy = c in the interior
+    m2 = np.tile(np.arange(n), n * n)
+    sel2 = sel & (m2 < n - 2)
+    assert np.allclose(src2[sel2], np.float32(3e-5 * 1000.0) * c * vy, rtol=1e-6)
+
+
+def test_heat_solve_with_sources_matches_reference():
+    """titerate with adiabyn 2 and frictyn 1 (g8): the oracle rows with the
+    oracle source (tk1) and the sub-steps from its rate reproduce the
+    reference's temperature."""
+    g = golden("g8_heat_sources.npz")
+    src, rate = _g8_source(g, 2, 1)
+    dt, maxtkstep = float(g["dt"]), float(g["maxtkstep"])
+    step = maxtkstep / np.abs(rate).max()
+    subs, s = [], 0.0
+    while s < dt:
+        h = min(dt, step)
+        h = min(h, dt - s)
+        subs.append(h)
+        s += h
+    assert len(subs) == 3
+    tk = g["tk"].astype(np.float32)
+    for h in subs:
+        H = oracle.heat_sys(g["gx"], g["gy"], g["gz"], g["ro"], g["cp"], g["kt"], g["ht"], tk, h, g["bondm"], g["bondv"],
+                            g["bondn"], src=src)
+        A, b = oracle.heat_matrix(H)
+        tk = spl.spsolve(A.tocsc(), b).astype(np.float32)
+    ref = g["tk_out"]
+    assert np.linalg.norm(tk.astype(np.float64) - ref) / np.linalg.norm(ref) < 1e-7
+    assert np.abs(tk - ref).max() < 2e-4
+    # the sources matter: without them the step differs clearly
+    tk0 = g["tk"].astype(np.float32)
+    for h in subs:
+        H = oracle.heat_sys(g["gx"], g["gy"], g["gz"], g["ro"], g["cp"], g["kt"], g["ht"], tk0, h, g["bondm"],
+                            g["bondv"], g["bondn"])
+        A, b = oracle.heat_matrix(H)
+        tk0 = spl.spsolve(A.tocsc(), b).astype(np.float32)
+    assert np.abs(tk0 - ref).max() > 100 * np.abs(tk - ref).max()
