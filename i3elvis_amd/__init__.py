@@ -358,10 +358,10 @@ class Markers:
         return order
 
     def to_nodes(self, rocks, pr, fld, nubeg, nuend, nukoef=0.0, densimod=1, timestep=1.0, vdeep=1e30,
-                 nudeep=1e30):
+                 nudeep=1e30, viscmod=0):
         fld = np.array(fld, np.float32, copy=True)
         pr = np.ascontiguousarray(pr, np.float64)
-        prm = MkParams(nubeg, nuend, nukoef, densimod, timestep, vdeep, nudeep, 0)
+        prm = MkParams(nubeg, nuend, nukoef, densimod, timestep, vdeep, nudeep, viscmod)
         ra = rocks_array(rocks)
         check(lib().i3g_markers_to_nodes(self.h, ra, C.c_int(len(rocks)), C.byref(prm), _p(pr),
                                          *[_p(fld[q]) for q in range(11)]), "markers_to_nodes")

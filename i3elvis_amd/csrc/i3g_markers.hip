@@ -342,7 +342,7 @@ k_marker_gather(int nx, int ny, int nz, const double *gx, const double *gy, cons
 		const int *start, const int *cnt, const int *__restrict__ id, const float *__restrict__ x,
 		const float *__restrict__ y, const float *__restrict__ z, const uint8_t *pin, const double *pnu,
 		const double *pro, const double *pcp, const double *pkt, const double *pht, const double *pbb,
-		const double *ptk, double nukoef, double nubeg, double nuend, float *fld)
+		const double *ptk, double nukoef, double nubeg, double nuend, int viscmod, float *fld)
 {
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	int m3 = blockIdx.y;
@@ -380,7 +380,9 @@ k_marker_gather(int nx, int ny, int nz, const double *gx, const double *gy, cons
 		double ddx = 1.0 - fabs((gx[m1] - (double)x[mk]) / dx);
 		double ddy = 1.0 - fabs((gy[m2] - (double)y[mk]) / dy);
 		double swt = ddx * ddy * ddz;
-		double mnu = pnu[mk];
+		/* viscmod 0/1/2: arithmetic, geometric (log) or harmonic (inverse)
+		 * averaging of the marker viscosity (mark3mg.c:848-900) */
+		double mnu = viscmod == 1 ? log(pnu[mk]) : viscmod == 2 ? 1.0 / pnu[mk] : pnu[mk];
 		if (ddx > nukoef && ddy > nukoef && ddz > nukoef) {
 			anu = (float)((double)anu + mnu * swt);
 			w1 += swt;
@@ -417,11 +419,15 @@ k_marker_gather(int nx, int ny, int nz, const double *gx, const double *gy, cons
 		}
 	}
 	float *f = fld;
-	if (w1 > 0.01) f[nd] = (float)((double)anu / w1);
-	if (wxx > 0.01) f[7 * N + nd] = (float)((double)axx / wxx);
-	if (wxy > 0.01) f[8 * N + nd] = (float)((double)axy / wxy);
-	if (wxz > 0.01) f[9 * N + nd] = (float)((double)axz / wxz);
-	if (wyz > 0.01) f[10 * N + nd] = (float)((double)ayz / wyz);
+	/* the mean back from the log / inverse (mark3mg.c:946-981) */
+#define VMEAN(a, w) (viscmod == 1 ? (float)exp((double)(float)((double)(a) / (w)))                    \
+		     : viscmod == 2 ? (float)(1.0 / (double)(float)((double)(a) / (w))) : (float)((double)(a) / (w)))
+	if (w1 > 0.01) f[nd] = VMEAN(anu, w1);
+	if (wxx > 0.01) f[7 * N + nd] = VMEAN(axx, wxx);
+	if (wxy > 0.01) f[8 * N + nd] = VMEAN(axy, wxy);
+	if (wxz > 0.01) f[9 * N + nd] = VMEAN(axz, wxz);
+	if (wyz > 0.01) f[10 * N + nd] = VMEAN(ayz, wyz);
+#undef VMEAN
 	if (w0 > 0.01) {
 		float rn = (float)((double)aro / w0);
 		f[N + nd] = rn;
@@ -856,7 +862,7 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 				    float *tk, float *nxx, float *nxy, float *nxz, float *nyz)
 {
 	if (!m || !rocks || !prm || !pr || nrock < 1 || nrock > MAXROCK) return I3G_EINVAL;
-	if (prm->viscmod != 0 || prm->densimod < 0 || prm->densimod > 1) return I3G_EINVAL;
+	if (prm->viscmod < 0 || prm->viscmod > 2 || prm->densimod < 0 || prm->densimod > 1) return I3G_EINVAL;
 	if (!m->sorted) return I3G_EINVAL;
 	float *host[11] = {nu, ro, cp, kt, ht, et, tk, nxx, nxy, nxz, nyz};
 	for (int q = 0; q < 11; q++)
@@ -890,7 +896,7 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 			   (const int *)m->id, (const float *)m->x, (const float *)m->y, (const float *)m->z,
 			   (const uint8_t *)m->pin, (const double *)m->pnu, (const double *)m->pro, (const double *)m->pcp,
 			   (const double *)m->pkt, (const double *)m->pht, (const double *)m->pbb, (const double *)m->ptk,
-			   prm->nukoef, prm->nubeg, prm->nuend, m->fld);
+			   prm->nukoef, prm->nubeg, prm->nuend, prm->viscmod, m->fld);
 	if (m->has_tbc) {
 		MCHK(hipMemcpyAsync(m->tk1, m->fld + 6 * N, N * sizeof(float), hipMemcpyDeviceToDevice, m->s));
 		hipLaunchKernelGGL(k_tkbc, g1(N), dim3(256), 0, m->s, N, (const int *)m->tbc_p, (const float *)m->tbc_c,

@@ -145,25 +145,29 @@ __global__ void k_ronurestrict(int fnx, int fny, int cnx, int cny, int cnz,
 			       const int *ys, const int *yi, const double *yw,
 			       const int *zs, const int *zi, const double *zw, int mode)
 {
+#pragma clang fp contract(off)
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	int K = blockIdx.y;
 	int I = t / cny, J = t - I * cny;
 	if (I >= cnx) return;
 	long szf = (long)fnx * fny;
 	float acc = 0;
-	double sw = 0, dacc = 0;
+	double sw = 0;
 	for (int a = zs[K]; a < zs[K + 1]; a++)
 		for (int b = xs[I]; b < xs[I + 1]; b++)
 			for (int e = ys[J]; e < ys[J + 1]; e++) {
 				double swt = xw[b] * yw[e] * zw[a];
 				long nf = zi[a] * szf + (long)xi[b] * fny + yi[e];
-				if (mode == 0) acc = (float)((double)acc + (double)nuf[nf] * swt);
-				else if (mode == 1) dacc += log((double)nuf[nf]) * swt;
-				else dacc += swt / (double)nuf[nf];
+				/* viscmod 0/1/2: the value, its log or its inverse */
+				double term = mode == 0 ? (double)nuf[nf] * swt
+					      : mode == 1 ? log((double)nuf[nf]) * swt : 1.0 / (double)nuf[nf] * swt;
+				acc = (float)((double)acc + term);
 				sw += swt;
 			}
-	double v = mode == 0 ? (double)acc / sw : mode == 1 ? exp(dacc / sw) : sw / dacc;
-	nuc[(long)K * cnx * cny + (long)I * cny + J] = (float)v;
+	float v = (float)((double)acc / sw);
+	if (mode == 1) v = (float)exp((double)v);
+	if (mode == 2) v = (float)(1.0 / (double)v);
+	nuc[(long)K * cnx * cny + (long)I * cny + J] = v;
 }
 
 /* ------------------------------------------------------------------ */
@@ -274,9 +278,9 @@ struct i3g_ctx {
 	 * switches it off) */
 	int xcd_remap = 1;
 	/* viscosity averaging of the coarse (V-cycle only) levels:
-	 * 0 arithmetic as the reference, 1 geometric, 2 harmonic
-	 * (I3G_COARSE_VISC selects) */
-	int coarse_visc = 0;
+	 * -1 the reference's viscmod, or 0 arithmetic, 1 geometric, 2
+	 * harmonic (I3G_COARSE_VISC selects) */
+	int coarse_visc = -1;
 	/* velocity smoother with all row loads issued first (I3G_SMOOTH_ILP=0
 	 * selects the plain kernel) */
 	int smooth_ilp = 1;
@@ -841,12 +845,10 @@ extern "C" int i3g_set_coefficients(i3g_ctx *c, const float *nu, const float *nx
 	HCHK(hipMemcpyAsync(c->nxz, nxz, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
 	HCHK(hipMemcpyAsync(c->nyz, nyz, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
 	HCHK(hipMemcpyAsync(c->ro, ro, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
-	/* The level-0 operator uses viscmod 0 (the reference's arithmetic
-	 * averages).  The coarse levels exist only for the V-cycle; their
-	 * viscosities are restricted and averaged by c->coarse_visc (0 the
-	 * reference's arithmetic ronurestrict, 1 geometric, 2 harmonic). */
-	if (viscmod != 0) return I3G_EINVAL; /* log/harmonic level-0 averages not built */
-	int cm = c->coarse_visc;
+	/* viscmod selects the arithmetic, geometric or harmonic averages of
+	 * the level-0 stress-point viscosities and of ronurestrict, as in the
+	 * reference; I3G_COARSE_VISC overrides the coarse levels' choice */
+	int cm = c->coarse_visc >= 0 ? c->coarse_visc : viscmod;
 	for (int l = 1; l < c->nlev; l++) {
 		Level &F = c->L[l - 1], &Cl = c->L[l];
 		const Xfer &X = c->X[l - 1];

@@ -428,10 +428,86 @@ def g8(R, n=17):
     print("g8_heat_sources.npz written")
 
 
+def g9(R):
+    """viscmod 1 (geometric) and 2 (harmonic) averaging through the
+    reference: the level-0 and level-1 Stokes rows (sxxcalc..syzcalc
+    viscosity means, move3mg.c:36-48, 402-414, 517-529, 632-644) and
+    ronurestrict (move3mg.c:2642-2741) on the g1 grid and fields, and
+    ronurecalc's node fields (mark3mg.c:848-981) on the g5 markers."""
+    import ctypes as C
+    g = np.load(os.path.join(HERE, "g1_operator.npz"))
+    rng = np.random.default_rng(909)
+    nx, ny, nz, mn = int(g["nx"]), int(g["ny"]), int(g["nz"]), int(g["multinum"])
+    N0 = nx * ny * nz
+    R.setup_grid(g["gx"], g["gy"], g["gz"], mn)
+    R.set_bc(g["bondm"], g["bondv"], g["bondn"])
+    out = {}
+    for vm in (1, 2):
+        R.set_physics(g=(0.3, 9.81, -0.2), nubeg=1e19, nuend=1e23, viscmod=vm)
+        for k in ("nu", "nxx", "nxy", "nxz", "nyz", "ro"):
+            R.f32(k)[:N0] = g[k]
+        A0, rhs0, ex0 = reference_system(R, 0)
+        x0 = rng.standard_normal(4 * N0) * np.tile([1e7, 1e-9, 1e-9, 1e-9], N0)
+        x0[~ex0] = 0
+        R.lib.refh_ronurestrict(1)
+        p1, N1 = int(R.mgp[1]), int(R.mgn[1])
+        nu1 = R.f32("nu")[p1:p1 + N1].copy()
+        A1, rhs1, ex1 = reference_system(R, 1)
+        x1 = rng.standard_normal(4 * N1) * np.tile([1e7, 1e-9, 1e-9, 1e-9], N1)
+        x1[~ex1] = 0
+        b1 = rng.standard_normal(4 * N1)
+        b1[~ex1] = 0
+        out.update({f"x0_{vm}": x0, f"rhs0_{vm}": rhs0, f"r0_{vm}": rhs0 - A0 @ x0, f"exists0_{vm}": ex0,
+                    f"nu1_{vm}": nu1, f"x1_{vm}": x1, f"b1_{vm}": b1, f"r1_{vm}": b1 - A1 @ x1,
+                    f"exists1_{vm}": ex1})
+    # ronurecalc with the g5 markers
+    m5 = np.load(os.path.join(HERE, "g5_markers.npz"))
+    gx, gy, gz = m5["gx"], m5["gy"], m5["gz"]
+    N = len(gx) * len(gy) * len(gz)
+    R.setup_grid(gx, gy, gz, 0)
+    R.set_bc(m5["bondm"], m5["bondv"], m5["bondn"])
+    n = len(m5["x"])
+    gnames = ["nu", "ro", "cp", "kt", "ht", "et", "tk", "nxx", "nxy", "nxz", "nyz"]
+    names = dict(nu0="marknu", dh="markdh", dv="markdv", n0="markn0", n1="markn1", ro="markro", bro="markbb",
+                 aro="markaa", cp="markcp", kt="markkt", kf="markkf", kp="markkp", ht="markht")
+    fields = ("nu0", "dh", "dv", "n0", "n1", "ro", "bro", "aro", "cp", "kt", "kf", "kp", "ht")
+    for vm in (1, 2):
+        R.set("marknum", n, C.c_long)
+        R.arr("markx", C.c_float, R.MAXMRK)[:n] = m5["x"]
+        R.arr("marky", C.c_float, R.MAXMRK)[:n] = m5["y"]
+        R.arr("markz", C.c_float, R.MAXMRK)[:n] = m5["z"]
+        R.arr("markk", C.c_float, R.MAXMRK)[:n] = m5["tk"]
+        R.arr("markv", C.c_float, R.MAXMRK)[:n] = m5["eta"]
+        R.arr("markt", C.c_byte, R.MAXMRK)[:n] = m5["mtype"]
+        R.arr("markrr", C.c_float, R.MAXMRK)[:n] = 0
+        R.arr("marke", C.c_float, R.MAXMRK)[:n] = 0
+        for q, f in enumerate(fields):
+            a = R.arr(names[f], C.c_double, R.MAXTMR)
+            a[:] = 0
+            a[:len(m5["rocks"])] = m5["rocks"][:, q]
+        for gname in ("markss", "markll", "marka0", "marka1", "markb0", "markb1", "marke0", "marke1"):
+            R.arr(gname, C.c_double, R.MAXTMR)[:] = 0
+        R.arr("markmm", C.c_double, R.MAXTMR)[:] = 1.0
+        R.set("densimod", 1, C.c_int)
+        for k, v in dict(eroslev=-1e30, sedilev=1e30, waterlev=1e30, vdeep=float(m5["vdeep"]),
+                         nudeep=float(m5["nudeep"]), nukoef=0.0, nubeg=float(m5["nubeg"]), nuend=float(m5["nuend"]),
+                         timestep=float(m5["timestep"])).items():
+            R.set(k, v)
+        R.set("viscmod", vm, C.c_int)
+        R.f64("pr")[:N] = m5["pr"]
+        for q, gname in enumerate(gnames):
+            R.f32(gname)[:N] = m5["init"][q]
+        R.lib.refh_ronurecalc()
+        out[f"mk_out_{vm}"] = np.stack([R.f32(gname)[:N].copy() for gname in gnames])
+    R.set("viscmod", 0, C.c_int)
+    np.savez_compressed(os.path.join(HERE, "g9_viscmod.npz"), **out)
+    print("g9_viscmod.npz written")
+
+
 if __name__ == "__main__":
     R = Ref()
     R.lib.refh_set_threads(1)
-    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8"]
+    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9"]
     if "g1" in which:
         g1(R)
     if "g3" in which:
@@ -445,3 +521,5 @@ if __name__ == "__main__":
         g7(R)
     if "g8" in which:
         g8(R)
+    if "g9" in which:
+        g9(R)

@@ -154,3 +154,26 @@ def test_large_marker_set_deterministic(gpu):
     assert np.allclose(f1[1], 3300, rtol=1e-5)
     assert np.allclose(f1[3], 3, rtol=1e-5)
     M.close()
+
+
+@pytest.mark.parametrize("vm", [1, 2])
+def test_to_nodes_viscmod_matches_reference(gpu, vm):
+    """ronurecalc with geometric (1) / harmonic (2) averaging of the marker
+    viscosities (mark3mg.c:848-981) against the reference's node fields
+    (g9, the g5 markers): within 2 float ulps-level relative error."""
+    g = golden("g5_markers.npz")
+    r = golden("g9_viscmod.npz")
+    print(f"case: g9 to_nodes viscmod={vm}", flush=True)
+    M = _markers(g)
+    M.sort()
+    M.set_tbc(g["bondm"], g["bondv"], g["bondn"])
+    rocks = [list(x) for x in g["rocks"]]
+    fld = M.to_nodes(rocks, g["pr"], g["init"], float(g["nubeg"]), float(g["nuend"]), 0.0, 1,
+                     float(g["timestep"]), float(g["vdeep"]), float(g["nudeep"]), viscmod=vm)
+    ref = r[f"mk_out_{vm}"]
+    for q in range(11):
+        a, b = fld[q], ref[q]
+        rel = np.abs(a.astype(np.float64) - b) / np.maximum(np.abs(b), 1e-30)
+        assert rel.max() <= 4e-7, (vm, q, rel.max())
+    assert not np.array_equal(ref[0], g["out"][0])
+    M.close()

@@ -123,3 +123,28 @@ def test_golden_solution_satisfies_oracle_system(n):
     r = oracle.residual(L, et, cfg["bondm"], cfg["bondv"], cfg["bondn"], b, g["sol"])
     w = oracle.wnorms(L, et[0], cfg["bondm"], b, r)
     assert w[0] / w[1] < 1e-10
+
+
+@pytest.mark.parametrize("vm", [1, 2])
+def test_viscmod_rows_and_restriction_match_reference(vm):
+    """viscmod 1 (geometric) and 2 (harmonic) means of the stress-point
+    viscosities and of ronurestrict against the reference (g9): level-0 and
+    level-1 residual rows to 1e-12 of the row scale, the restricted coarse
+    viscosity bitwise."""
+    g = golden("g1_operator.npz")
+    r = golden("g9_viscmod.npz")
+    L0, L1 = _levels(g)
+    et0 = oracle.eta(L0, g["nu"], g["nxx"], g["nxy"], g["nxz"], g["nyz"], vm, float(g["nubeg"]), float(g["nuend"]))
+    nu1 = oracle.ronurestrict(L0, L1, g["nu"], vm)
+    assert np.array_equal(nu1, r[f"nu1_{vm}"])
+    et1 = oracle.eta(L1, nu1, None, None, None, None, vm, float(g["nubeg"]), float(g["nuend"]))
+    for et, L, x, b, rref, ex in ((et0, L0, r[f"x0_{vm}"], r[f"rhs0_{vm}"], r[f"r0_{vm}"], r[f"exists0_{vm}"]),
+                                  (et1, L1, r[f"x1_{vm}"], r[f"b1_{vm}"], r[f"r1_{vm}"], r[f"exists1_{vm}"])):
+        res = oracle.residual(L, et, g["bondm"], g["bondv"], g["bondn"], b, x)
+        for c in range(4):
+            d = np.abs(res[c::4] - rref[c::4])[ex[c::4]]
+            s = np.abs(rref[c::4]).max() + np.abs(b[c::4]).max()
+            assert d.max() <= 1e-12 * s, (vm, c, d.max(), s)
+    # the means differ from the arithmetic one (a plausible slip: ignoring viscmod)
+    et00 = oracle.eta(L0, g["nu"], g["nxx"], g["nxy"], g["nxz"], g["nyz"], 0, float(g["nubeg"]), float(g["nuend"]))
+    assert not np.array_equal(et0[1], et00[1])

@@ -160,6 +160,63 @@ def test_transfers_match_reference_bitwise(gpu):
     S.close()
 
 
+@pytest.mark.parametrize("vm", [1, 2])
+def test_residual_viscmod_matches_reference_rows(gpu, vm):
+    """Level-0 FP64 residual with geometric (1) / harmonic (2) viscosity
+    means against the reference's rows (g9) to 1e-12 of the row scale."""
+    g = golden("g1_operator.npz")
+    r = golden("g9_viscmod.npz")
+    cfg = _cfg_from_g1(g)
+    cfg["viscmod"] = vm
+    print(f"case: g9 residual viscmod={vm}", flush=True)
+    S = I.Stokes(cfg)
+    S.set_coefficients()
+    x = r[f"x0_{vm}"]
+    res, _ = S.residual(x[0::4], x[1::4], x[2::4], x[3::4])
+    ex = r[f"exists0_{vm}"]
+    for c in range(4):
+        ref = r[f"r0_{vm}"][c::4]
+        s = np.abs(ref).max() + np.abs(r[f"rhs0_{vm}"][c::4]).max()
+        d = np.abs(res[c] - ref)[ex[c::4]].max()
+        assert d <= 1e-12 * s, (vm, c, d, s)
+    S.close()
+
+
+@pytest.mark.parametrize("vm", [1, 2])
+def test_solve_viscmod_satisfies_oracle_system(gpu, vm):
+    """A converged solve with viscmod 1 / 2 (level-0 means and the coarse
+    levels' ronurestrict) is a solution of the oracle's system with the
+    same means: its oracle-weighted relative residual is <= 1e-8."""
+    n = 25
+    cfg = I.make_config(1, n, n, n, seed=4)
+    cfg["viscmod"] = vm
+    print(f"case: solve c1 n={n} viscmod={vm}", flush=True)
+    S = I.Stokes(cfg)
+    S.set_coefficients()
+    z = np.zeros(n ** 3)
+    S.set_solution(z, z, z, z)
+    st = S.solve(precision=32, rtol=1e-9, max_outer=40)
+    print("  vcycles", st.vcycles, "rel", st.rel_res, flush=True)
+    assert st.converged
+    P, vx, vy, vz = S.get_solution()
+    S.close()
+    L = oracle.level(n, n, n, 0, 0, cfg["gx"], cfg["gy"], cfg["gz"])
+    et = oracle.eta(L, cfg["nu"], cfg["nxx"], cfg["nxy"], cfg["nxz"], cfg["nyz"], vm, cfg["nubeg"], cfg["nuend"])
+    b = oracle.rhs(L, cfg["ro"], cfg["g"], cfg["bondm"], cfg["bondv"])
+    x = np.zeros(4 * n ** 3)
+    for c, f in enumerate((P, vx, vy, vz)):
+        x[c::4] = f
+    res = oracle.residual(L, et, cfg["bondm"], cfg["bondv"], cfg["bondn"], b, x)
+    w = oracle.wnorms(L, et[0], cfg["bondm"], b, res)
+    print("  oracle rel residual", w[0] / w[1], flush=True)
+    assert w[0] / w[1] <= 1e-8
+    # and it is not a solution of the arithmetic-mean system
+    et0 = oracle.eta(L, cfg["nu"], cfg["nxx"], cfg["nxy"], cfg["nxz"], cfg["nyz"], 0, cfg["nubeg"], cfg["nuend"])
+    res0 = oracle.residual(L, et0, cfg["bondm"], cfg["bondv"], cfg["bondn"], b, x)
+    w0 = oracle.wnorms(L, et0[0], cfg["bondm"], b, res0)
+    assert w0[0] / w0[1] > 1e-6
+
+
 def test_transfers_match_oracle_all_levels(gpu):
     """Restriction and prolongation between every pair of levels of a 129^3
     hierarchy (129 -> 67 -> 36 -> 20 -> 12; the 67 -> 36 maps have
