@@ -367,6 +367,34 @@ class Markers:
                                          *[_p(fld[q]) for q in range(11)]), "markers_to_nodes")
         return fld
 
+    def set_strain(self, e):
+        e = np.ascontiguousarray(e, np.float32)
+        check(lib().i3g_markers_set_strain(self.h, _p(e)), "markers_set_strain")
+
+    def get_strain(self):
+        e = np.zeros(self.n, np.float32)
+        check(lib().i3g_markers_get_strain(self.h, _p(e)), "markers_get_strain")
+        return e
+
+    def epscalc(self, st12, pr):
+        """(n, 14): eps xx yy zz xy xz yz, P, sigma xx .. yz, dissipation at
+        every marker (current order)"""
+        st = [np.ascontiguousarray(a, np.float32) for a in st12]
+        sp = (C.c_void_p * 12)(*[a.ctypes.data for a in st])
+        prd = np.ascontiguousarray(pr, np.float64)
+        out = np.zeros((self.n, 14))
+        check(lib().i3g_markers_epscalc(self.h, sp, _p(prd), _p(out)), "markers_epscalc")
+        return out
+
+    def update_props(self, rocks, nubeg, nuend, timestep, healrate, st12, pr):
+        st = [np.ascontiguousarray(a, np.float32) for a in st12]
+        sp = (C.c_void_p * 12)(*[a.ctypes.data for a in st])
+        prd = np.ascontiguousarray(pr, np.float64)
+        prm = MkParams(nubeg, nuend, 0.0, 1, timestep, 1e30, 1e30, 0)
+        ra = rocks_array(rocks)
+        check(lib().i3g_markers_update_props(self.h, ra, C.c_int(len(rocks)), C.byref(prm), C.c_double(healrate), sp,
+                                             _p(prd)), "markers_update_props")
+
     def set_tbc(self, bondm, bondv, bondn):
         """Temperature boundary rows from the reference-layout tables (bondm
         over all levels or just level 0's nine blocks); None switches off."""

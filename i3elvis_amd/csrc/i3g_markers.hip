@@ -38,9 +38,10 @@ struct i3g_markers {
 	double *gx = nullptr, *gy = nullptr, *gz = nullptr;
 	/* SoA marker state (current order) and a second buffer for permuting */
 	float *x = nullptr, *y = nullptr, *z = nullptr, *tk = nullptr, *eta = nullptr;
+	float *e = nullptr;      /* accumulated strain (marke) */
 	signed char *type = nullptr;
 	int *id = nullptr;
-	float *x2 = nullptr, *y2 = nullptr, *z2 = nullptr, *tk2 = nullptr, *eta2 = nullptr;
+	float *x2 = nullptr, *y2 = nullptr, *z2 = nullptr, *tk2 = nullptr, *eta2 = nullptr, *e2 = nullptr;
 	signed char *type2 = nullptr;
 	int *id2 = nullptr;
 	int *key = nullptr, *c1 = nullptr, *c2 = nullptr, *c3 = nullptr;
@@ -57,6 +58,7 @@ struct i3g_markers {
 	double *pr = nullptr;
 	float *fld = nullptr;    /* 11 x N float in/out fields */
 	float *vel = nullptr;    /* 3 x N velocities */
+	float *st12 = nullptr;   /* 12 x N strain rates and stresses (epscalc1) */
 	i3g_rock *rocks = nullptr;
 	int *bad = nullptr;
 	/* temperature boundary rows of level 0 (tkrecalc): per node partner
@@ -621,6 +623,153 @@ __global__ void k_tk_anti(long n, const float *__restrict__ x, const float *__re
 }
 
 /* ------------------------------------------------------------------ */
+/* Strain rates / stresses at markers and movemark's property update  */
+/* ------------------------------------------------------------------ */
+
+/* One axis of an epscalc1 interpolation block (mark3mg.c:2575-2600,
+ * 2646-2662): points at cell centres (kind 0: base cell + 1 past the
+ * mid-line, in [1, n-2]), grid lines (kind 1: base in [1, n-3]) or
+ * mid-lines (kind 2: base cell - 1 below the mid-line, in [0, n-3]);
+ * w[0..1]: the weights of base and base+1 clamped to [0, 1]. */
+__device__ __forceinline__ int eps_axis(const double *g, int n, int cell, double x, int kind, double *w)
+{
+	int b = cell;
+	double d, p0, p1;
+	if (kind == 0) {
+		if (x > (g[b] + g[b + 1]) / 2.0) b += 1;
+		if (b < 1) b = 1;
+		if (b > n - 2) b = n - 2;
+		d = (g[b + 1] - g[b - 1]) / 2.0;
+		p0 = (g[b - 1] + g[b]) / 2.0;
+		p1 = (g[b] + g[b + 1]) / 2.0;
+	} else if (kind == 1) {
+		if (b < 1) b = 1;
+		if (b > n - 3) b = n - 3;
+		d = g[b + 1] - g[b];
+		p0 = g[b];
+		p1 = g[b + 1];
+	} else {
+		if (x < (g[b] + g[b + 1]) / 2.0) b -= 1;
+		if (b < 0) b = 0;
+		if (b > n - 3) b = n - 3;
+		d = (g[b + 2] - g[b]) / 2.0;
+		p0 = (g[b] + g[b + 1]) / 2.0;
+		p1 = (g[b + 1] + g[b + 2]) / 2.0;
+	}
+	double dd0 = 1.0 - (-((p0 - x) / d)), dd1 = 1.0 - (p1 - x) / d;
+	w[0] = dd0 < 0 ? 0 : dd0 > 1.0 ? 1.0 : dd0;
+	w[1] = dd1 < 0 ? 0 : dd1 > 1.0 ? 1.0 : dd1;
+	return b;
+}
+
+/* epscalc1 (mark3mg.c:2535-2838) at a point: o[0..5] eps xx yy zz xy xz
+ * yz, o[6] P, o[7..12] sigma, o[13] dissipation; normal components and P
+ * from the eight surrounding cell centres, shear components from their
+ * edge nodes; the reference's summation order. */
+__device__ void epscalc(const float *st12, long N, const double *pr, const double *gx, const double *gy,
+			const double *gz, int nx, int ny, int nz, double x, double y, double z, double *o)
+{
+	const int K[4][3] = {{0, 0, 0}, {1, 1, 2}, {1, 2, 1}, {2, 1, 1}};
+	for (int q = 0; q < 14; q++) o[q] = 0;
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	if (x < 0) x = 0;
+	if (x > xs) x = xs;
+	if (y < 0) y = 0;
+	if (y > ys) y = ys;
+	if (z < 0) z = 0;
+	if (z > zs) z = zs;
+	int m1 = serch(gx, nx, x), m2 = serch(gy, ny, y), m3 = serch(gz, nz, z);
+	long sx = ny, sz = (long)nx * ny;
+	for (int blk = 0; blk < 4; blk++) {
+		double wx[2], wy[2], wz[2];
+		int a = eps_axis(gx, nx, m1, x, K[blk][0], wx);
+		int b = eps_axis(gy, ny, m2, y, K[blk][1], wy);
+		int c = eps_axis(gz, nz, m3, z, K[blk][2], wz);
+		for (int k = 0; k < 2; k++)
+			for (int i = 0; i < 2; i++)
+				for (int j = 0; j < 2; j++) {
+					long u = (c + k) * sz + (long)(a + i) * sx + (b + j);
+					double swt = wx[i] * wy[j] * wz[k];
+					if (blk == 0) {
+						float e0 = st12[u], e1 = st12[N + u], e2 = st12[2 * N + u];
+						float s0 = st12[6 * N + u], s1 = st12[7 * N + u], s2 = st12[8 * N + u];
+						o[0] += e0 * swt;
+						o[1] += e1 * swt;
+						o[2] += e2 * swt;
+						o[7] += s0 * swt;
+						o[8] += s1 * swt;
+						o[9] += s2 * swt;
+						o[13] += (s0 * e0 + s1 * e1 + s2 * e2) * swt;
+						o[6] += pr[u] * swt;
+					} else {
+						float es = st12[(2 + blk) * N + u], ss = st12[(8 + blk) * N + u];
+						o[2 + blk] += es * swt;
+						o[9 + blk] += ss * swt;
+						o[13] += 2.0 * ss * es * swt;
+					}
+				}
+	}
+}
+
+/* markers that movemark moves (mark3mg.c:61, outgrid 0) */
+__device__ __forceinline__ bool mk_moves(float fx, float fy, float fz, int mt, double xs, double ys, double zs)
+{
+	return (fx >= 0 && fy >= 0 && fz >= 0 && (double)fx <= xs && (double)fy <= ys && (double)fz <= zs) || mt < 50 ||
+	       mt >= 100;
+}
+
+__global__ void k_epscalc(long n, const float *__restrict__ x, const float *__restrict__ y,
+			  const float *__restrict__ z, const float *st12, long N, const double *pr, const double *gx,
+			  const double *gy, const double *gz, int nx, int ny, int nz, double *out)
+{
+	extern __shared__ double sg[];
+	stage_lines(sg, gx, gy, gz, nx, ny, nz);
+	gx = sg; gy = sg + nx; gz = sg + nx + ny;
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	double o[14];
+	epscalc(st12, N, pr, gx, gy, gz, nx, ny, nz, (double)x[t], (double)y[t], (double)z[t], o);
+	for (int q = 0; q < 14; q++) out[t * 14 + q] = o[q];
+}
+
+/* movemark's marker properties ahead of the position update
+ * (mark3mg.c:61-145) for the Newtonian rock types: markers of type < 50
+ * with T > 0 get eta = viscalc(T, P) with P from epscalc1 at the marker;
+ * with no brittle strength viscalc marks them as not yielding (e = -|e|,
+ * mark3mg.c:1179, 1648-1659), so the strain is not grown by dt * eii and
+ * heals by dt * healrate toward zero. */
+__global__ void k_mk_props(long n, const float *__restrict__ x, const float *__restrict__ y,
+			   const float *__restrict__ z, const float *__restrict__ tk, float *eta, float *e,
+			   const signed char *__restrict__ type, const float *st12, long N, const double *pr,
+			   const double *gx, const double *gy, const double *gz, int nx, int ny, int nz,
+			   const i3g_rock *rocks, int nrock, double nubeg, double nuend, double dt, double healrate,
+			   int *bad)
+{
+	extern __shared__ double sg[];
+	stage_lines(sg, gx, gy, gz, nx, ny, nz);
+	gx = sg; gy = sg + nx; gz = sg + nx + ny;
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	float fx = x[t], fy = y[t], fz = z[t];
+	int mt = type[t];
+	if (!mk_moves(fx, fy, fz, mt, gx[nx - 1], gy[ny - 1], gz[nz - 1])) return;
+	double mtk = (double)tk[t];
+	if (!(mt < 50 && mtk > 0)) return;
+	if (mt < 2 || mt >= nrock || mt == 7 || mt == 8 || (mt >= 16 && mt <= 18) || mt > 20) {
+		atomicOr(bad, 1);
+		return;
+	}
+	double o[14];
+	epscalc(st12, N, pr, gx, gy, gz, nx, ny, nz, (double)fx, (double)fy, (double)fz, o);
+	double mpb = o[6] * 1e-5;
+	eta[t] = (float)visc_ab(rocks[mt], mtk, mpb, nubeg, nuend);
+	float ev = -fabsf(e[t]);
+	ev += dt * healrate;
+	if (ev > 0) ev = 0;
+	e[t] = ev;
+}
+
+/* ------------------------------------------------------------------ */
 /* RK4 advection                                                      */
 /* ------------------------------------------------------------------ */
 
@@ -696,6 +845,7 @@ extern "C" int i3g_markers_create(long nx, long ny, long nz, const double *gx, c
 	long c = capacity;
 	m->x = mk_alloc<float>(c, &err); m->y = mk_alloc<float>(c, &err); m->z = mk_alloc<float>(c, &err);
 	m->tk = mk_alloc<float>(c, &err); m->eta = mk_alloc<float>(c, &err);
+	m->e = mk_alloc<float>(c, &err); m->e2 = mk_alloc<float>(c, &err);
 	m->type = mk_alloc<signed char>(c, &err); m->id = mk_alloc<int>(c, &err);
 	m->x2 = mk_alloc<float>(c, &err); m->y2 = mk_alloc<float>(c, &err); m->z2 = mk_alloc<float>(c, &err);
 	m->tk2 = mk_alloc<float>(c, &err); m->eta2 = mk_alloc<float>(c, &err);
@@ -724,7 +874,8 @@ extern "C" int i3g_markers_destroy(i3g_markers *m)
 	void *ps[] = {m->gx, m->gy, m->gz, m->x, m->y, m->z, m->tk, m->eta, m->type, m->id, m->x2, m->y2, m->z2,
 		      m->tk2, m->eta2, m->type2, m->id2, m->key, m->c1, m->c2, m->c3, m->perm, m->cnt, m->start,
 		      m->cur, m->bsum, m->pnu, m->pro, m->pcp, m->pkt, m->pht, m->pbb, m->ptk, m->pin, m->pr, m->fld,
-		      m->vel, m->rocks, m->bad, m->tbc_p, m->tbc_c, m->tbc_k, m->tk1, m->tnod, m->dtk, m->sn, m->tin};
+		      m->vel, m->rocks, m->bad, m->tbc_p, m->tbc_c, m->tbc_k, m->tk1, m->tnod, m->dtk, m->sn, m->tin,
+		      m->e, m->e2, m->st12};
 	for (void *p : ps)
 		if (p) hipFree(p);
 	if (m->ev0) hipEventDestroy(m->ev0);
@@ -750,6 +901,7 @@ extern "C" int i3g_markers_upload(i3g_markers *m, long n, const float *x, const 
 	MCHK(hipMemcpyAsync(m->tk, tk, n * sizeof(float), hipMemcpyHostToDevice, m->s));
 	MCHK(hipMemcpyAsync(m->eta, eta, n * sizeof(float), hipMemcpyHostToDevice, m->s));
 	MCHK(hipMemcpyAsync(m->type, type, n * sizeof(signed char), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemsetAsync(m->e, 0, (n ? n : 1) * sizeof(float), m->s));
 	if (id) {
 		MCHK(hipMemcpyAsync(m->id, id, n * sizeof(int), hipMemcpyHostToDevice, m->s));
 	} else {
@@ -837,6 +989,7 @@ extern "C" int i3g_markers_sort(i3g_markers *m, long *order)
 		permute(m, m->z, m->z2);
 		permute(m, m->tk, m->tk2);
 		permute(m, m->eta, m->eta2);
+		permute(m, m->e, m->e2);
 		permute(m, m->type, m->type2);
 		permute(m, m->id, m->id2);
 		/* cell indices in the new order */
@@ -1017,4 +1170,92 @@ extern "C" int i3g_markers_advect_rk4(i3g_markers *m, double dt, const float *vx
 	mark_read(m);
 	m->sorted = 0;
 	return I3G_OK;
+}
+
+extern "C" int i3g_markers_set_strain(i3g_markers *m, const float *e)
+{
+	if (!m || !e) return I3G_EINVAL;
+	if (m->n) MCHK(hipMemcpyAsync(m->e, e, m->n * sizeof(float), hipMemcpyHostToDevice, m->s));
+	MCHK(hipStreamSynchronize(m->s));
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_get_strain(i3g_markers *m, float *e)
+{
+	if (!m || !e) return I3G_EINVAL;
+	if (m->n) MCHK(hipMemcpyAsync(e, m->e, m->n * sizeof(float), hipMemcpyDeviceToHost, m->s));
+	MCHK(hipStreamSynchronize(m->s));
+	return I3G_OK;
+}
+
+/* the twelve strain-rate / stress planes and P into device buffers */
+static int upload_st12(i3g_markers *m, const float *const *st12, const double *pr)
+{
+	long N = m->N;
+	int err = 0;
+	if (!m->st12) {
+		m->st12 = mk_alloc<float>(12 * N, &err);
+		if (err) return err;
+	}
+	if (!m->pr) {
+		m->pr = mk_alloc<double>(N, &err);
+		if (err) return err;
+	}
+	for (int q = 0; q < 12; q++) {
+		if (!st12[q]) return I3G_EINVAL;
+		MCHK(hipMemcpyAsync(m->st12 + q * N, st12[q], N * sizeof(float), hipMemcpyHostToDevice, m->s));
+	}
+	MCHK(hipMemcpyAsync(m->pr, pr, N * sizeof(double), hipMemcpyHostToDevice, m->s));
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_epscalc(i3g_markers *m, const float *const *st12, const double *pr, double *out)
+{
+	if (!m || !st12 || !pr || !out) return I3G_EINVAL;
+	long n = m->n;
+	int rc = upload_st12(m, st12, pr);
+	if (rc) return rc;
+	int err = 0;
+	if (n) {
+		double *d = mk_alloc<double>(14 * n, &err);
+		if (err) return err;
+		mark_begin(m);
+		hipLaunchKernelGGL(k_epscalc, g1(n), dim3(256), lines_bytes(m), m->s, n, (const float *)m->x,
+				   (const float *)m->y, (const float *)m->z, (const float *)m->st12, m->N, (const double *)m->pr,
+				   (const double *)m->gx, (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, d);
+		mark_end(m);
+		hipError_t e1 = hipGetLastError();
+		hipError_t e2 = e1 == hipSuccess ? hipMemcpyAsync(out, d, 14 * n * sizeof(double), hipMemcpyDeviceToHost, m->s)
+						 : e1;
+		hipError_t e3 = hipStreamSynchronize(m->s);
+		hipFree(d);
+		if (e2 != hipSuccess || e3 != hipSuccess) return I3G_EDEVICE;
+		mark_read(m);
+	}
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_update_props(i3g_markers *m, const i3g_rock *rocks, int nrock, const i3g_mk_params *prm,
+					double healrate, const float *const *st12, const double *pr)
+{
+	if (!m || !rocks || !prm || !st12 || !pr || nrock < 1 || nrock > MAXROCK) return I3G_EINVAL;
+	long n = m->n;
+	int rc = upload_st12(m, st12, pr);
+	if (rc) return rc;
+	MCHK(hipMemcpyAsync(m->rocks, rocks, nrock * sizeof(i3g_rock), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemsetAsync(m->bad, 0, sizeof(int), m->s));
+	mark_begin(m);
+	if (n)
+		hipLaunchKernelGGL(k_mk_props, g1(n), dim3(256), lines_bytes(m), m->s, n, (const float *)m->x,
+				   (const float *)m->y, (const float *)m->z, (const float *)m->tk, m->eta, m->e,
+				   (const signed char *)m->type, (const float *)m->st12, m->N, (const double *)m->pr,
+				   (const double *)m->gx, (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz,
+				   (const i3g_rock *)m->rocks, nrock, prm->nubeg, prm->nuend, prm->timestep, healrate, m->bad);
+	MCHK(hipGetLastError());
+	mark_end(m);
+	int bad = 0;
+	MCHK(hipMemcpyAsync(&bad, m->bad, sizeof(int), hipMemcpyDeviceToHost, m->s));
+	MCHK(hipStreamSynchronize(m->s));
+	mark_read(m);
+	return bad ? I3G_EINVAL : I3G_OK;
 }

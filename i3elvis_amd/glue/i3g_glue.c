@@ -25,7 +25,9 @@
  *                  temperature update (heat3mg.c:592-782).
  *   i3g_ronurecalc the node fields of ronurecalc (mark3mg.c:404-1015) for
  *                  Newtonian rock types, with tkrecalc.
- *   i3g_movemark   the RK4 position update of movemark (mark3mg.c:215-284).
+ *   i3g_movemark   movemark (mark3mg.c:2-397) for the Newtonian rock types:
+ *                  viscosity and accumulated strain at the old positions
+ *                  (epscalc1 + viscalc), then the RK4 position update.
  * Errors print a message and exit, as the reference does on fatal
  * conditions.
  */
@@ -109,22 +111,39 @@ static void glue_markers(void)
 		glue_fail("markers_upload", rc);
 }
 
-static void glue_markers_back(void)
+/* device markers back into markx..markv (by id); with_strain also marke */
+static void glue_markers_back(int with_strain)
 {
 	int rc;
 	long n = marknum, i;
-	float *buf = malloc(sizeof(float) * 5 * (n ? n : 1));
+	float *buf = malloc(sizeof(float) * 6 * (n ? n : 1));
 	signed char *t = malloc(n ? n : 1);
 	int *id = malloc(sizeof(int) * (n ? n : 1));
 	if (!buf || !t || !id) glue_fail("host allocation", I3G_ENOMEM);
 	if ((rc = i3g_markers_download(G.mk, buf, buf + n, buf + 2 * n, buf + 3 * n, buf + 4 * n, t, id)))
 		glue_fail("markers_download", rc);
+	if (with_strain && (rc = i3g_markers_get_strain(G.mk, buf + 5 * n))) glue_fail("markers_get_strain", rc);
 	for (i = 0; i < n; i++) {
 		long j = id[i];
 		markx[j] = buf[i]; marky[j] = buf[n + i]; markz[j] = buf[2 * n + i];
 		markk[j] = buf[3 * n + i]; markv[j] = buf[4 * n + i];
+		if (with_strain) marke[j] = buf[5 * n + i];
 	}
 	free(buf); free(t); free(id);
+}
+
+/* the rock table of the program (marknu.. markht) for the device */
+static int glue_rocks(i3g_rock *rocks)
+{
+	int t, nr = rocknum > 0 && rocknum <= 64 ? rocknum : 64;
+	for (t = 0; t < nr; t++) {
+		rocks[t].nu0 = marknu[t]; rocks[t].dh = markdh[t]; rocks[t].dv = markdv[t];
+		rocks[t].n0 = markn0[t]; rocks[t].n1 = markn1[t];
+		rocks[t].ro = markro[t]; rocks[t].bro = markbb[t]; rocks[t].aro = markaa[t];
+		rocks[t].cp = markcp[t]; rocks[t].kt = markkt[t]; rocks[t].kf = markkf[t]; rocks[t].kp = markkp[t];
+		rocks[t].ht = markht[t];
+	}
+	return nr;
 }
 
 void i3g_vpiterate(int m0, int f0)
@@ -215,7 +234,7 @@ void i3g_titerate(int m0)
 	if ((rc = i3g_markers_sort(G.mk, NULL))) glue_fail("markers_sort", rc);
 	rc = i3g_markers_update_tk(G.mk, tk, G.tkstart, kt, ro, cp, heatdif, timestep, zdeep, tdeep);
 	if (rc) glue_fail("markers_update_tk", rc);
-	glue_markers_back();
+	glue_markers_back(0);
 }
 
 void i3g_ronurecalc(void)
@@ -227,27 +246,31 @@ void i3g_ronurecalc(void)
 	if ((rc = i3g_markers_set_tbc(G.mk, bondm + nodenum8, &bondv1[0][0], bondn1, bondnum + 1)))
 		glue_fail("markers_set_tbc", rc);
 	static i3g_rock rocks[64];
-	int nr = rocknum > 0 && rocknum <= 64 ? rocknum : 64;
-	for (t = 0; t < nr; t++) {
-		rocks[t].nu0 = marknu[t]; rocks[t].dh = markdh[t]; rocks[t].dv = markdv[t];
-		rocks[t].n0 = markn0[t]; rocks[t].n1 = markn1[t];
-		rocks[t].ro = markro[t]; rocks[t].bro = markbb[t]; rocks[t].aro = markaa[t];
-		rocks[t].cp = markcp[t]; rocks[t].kt = markkt[t]; rocks[t].kf = markkf[t]; rocks[t].kp = markkp[t];
-		rocks[t].ht = markht[t];
-	}
+	int nr = glue_rocks(rocks);
+	(void)t;
 	i3g_mk_params mp;
 	mp.nubeg = nubeg; mp.nuend = nuend; mp.nukoef = nukoef; mp.densimod = densimod; mp.timestep = timestep;
 	mp.vdeep = vdeep; mp.nudeep = nudeep; mp.viscmod = viscmod;
 	rc = i3g_markers_to_nodes(G.mk, rocks, nr, &mp, pr, nu, ro, cp, kt, ht, et, tk, nxx, nxy, nxz, nyz);
 	if (rc) glue_fail("markers_to_nodes", rc);
-	glue_markers_back();
+	glue_markers_back(0);
 }
 
 void i3g_movemark(void)
 {
 	int rc;
+	static i3g_rock rocks[64];
 	glue_grid();
 	glue_markers();
+	if ((rc = i3g_markers_set_strain(G.mk, marke))) glue_fail("markers_set_strain", rc);
+	/* viscosity and accumulated strain at the old positions (epscalc1 +
+	 * viscalc), then the RK4 move */
+	int nr = glue_rocks(rocks);
+	i3g_mk_params mp;
+	mp.nubeg = nubeg; mp.nuend = nuend; mp.nukoef = nukoef; mp.densimod = densimod; mp.timestep = timestep;
+	mp.vdeep = vdeep; mp.nudeep = nudeep; mp.viscmod = viscmod;
+	const float *st12[12] = {exx, eyy, ezz, exy, exz, eyz, sxx, syy, szz, sxy, sxz, syz};
+	if ((rc = i3g_markers_update_props(G.mk, rocks, nr, &mp, healrate, st12, pr))) glue_fail("markers_update_props", rc);
 	if ((rc = i3g_markers_advect_rk4(G.mk, timestep, vx, vy, vz))) glue_fail("markers_advect_rk4", rc);
-	glue_markers_back();
+	glue_markers_back(1);
 }

@@ -369,6 +369,32 @@ int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int nrock, const
 			 const double *pr, float *nu, float *ro, float *cp, float *kt, float *ht,
 			 float *et, float *tk, float *nxx, float *nxy, float *nxz, float *nyz);
 
+/* Accumulated strain of the markers (the reference's marke), in the
+ * current marker order (that of i3g_markers_download); upload sets it to 0.
+ * Sorting keeps it with its marker. */
+int i3g_markers_set_strain(i3g_markers *m, const float *e);
+int i3g_markers_get_strain(i3g_markers *m, float *e);
+
+/* epscalc1 (mark3mg.c:2535-2838) at every marker: strain rates, stresses,
+ * pressure and dissipation by trilinear interpolation with weights clamped
+ * to [0,1] (normal components and P from the cell centres, shear
+ * components from their edge nodes).  st12: 12 float node planes exx eyy
+ * ezz exy exz eyz sxx syy szz sxy sxz syz as the reference stores them;
+ * pr: pressure (double).  out: 14 doubles per marker in the current order
+ * (eps xx yy zz xy xz yz, P, sigma xx .. yz, dissipation). */
+int i3g_markers_epscalc(i3g_markers *m, const float *const *st12, const double *pr, double *out);
+
+/* movemark's property update ahead of the position update
+ * (mark3mg.c:61-145) for the Newtonian rock types: every marker of type
+ * < 50 with T > 0 gets eta = viscalc (branches A/B and limits) at the
+ * pressure epscalc1 gives at the marker; viscalc marks it as not yielding
+ * (these rocks have no brittle strength), so its accumulated strain is
+ * made non-positive and heals by timestep * healrate toward 0.  prm: nubeg,
+ * nuend, timestep.  Types with other rheology branches return I3G_EINVAL
+ * (as i3g_markers_to_nodes).  Then i3g_markers_advect_rk4 moves them. */
+int i3g_markers_update_props(i3g_markers *m, const i3g_rock *rocks, int nrock, const i3g_mk_params *prm,
+			     double healrate, const float *const *st12, const double *pr);
+
 /* Temperature boundary rows applied at the end of every marker -> node
  * transfer, as ronurecalc calls tkrecalc (mark3mg.c:1007, heat3mg.c:849-899):
  * bondm_t is the level-0 T block of the reference's bondm (bondm +

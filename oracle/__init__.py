@@ -173,6 +173,36 @@ def vxyz(vx, vy, vz, gx, gy, gz, x, y, z):
     return v
 
 
+def epscalc(st12, pr, gx, gy, gz, x, y, z):
+    """epscalc1 at a point: [exx eyy ezz exy exz eyz, P, sxx .. syz, dissipation]"""
+    st = [np.ascontiguousarray(a, np.float32) for a in st12]
+    sp = (C.c_void_p * 12)(*[a.ctypes.data for a in st])
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
+    prd = np.ascontiguousarray(pr, np.float64)
+    out = np.zeros(14)
+    lib().or_epscalc(sp, _p(prd), _p(gx), C.c_long(len(gx)), _p(gy), C.c_long(len(gy)), _p(gz), C.c_long(len(gz)),
+                     C.c_double(x), C.c_double(y), C.c_double(z), _p(out))
+    return out
+
+
+def mk_props(x, y, z, mtype, tk, eta, e, rocks, st12, pr, gx, gy, gz, nubeg, nuend, dt, healrate):
+    """movemark's (eta, e) update of the Newtonian markers; returns copies"""
+    eta = np.array(eta, np.float32, copy=True)
+    e = np.array(e, np.float32, copy=True)
+    st = [np.ascontiguousarray(a, np.float32) for a in st12]
+    sp = (C.c_void_p * 12)(*[a.ctypes.data for a in st])
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
+    f = [np.ascontiguousarray(a, np.float32) for a in (x, y, z)]
+    t = np.ascontiguousarray(mtype, np.int8)
+    tkf = np.ascontiguousarray(tk, np.float32)
+    prd = np.ascontiguousarray(pr, np.float64)
+    ra = rocks_array(rocks)
+    lib().or_mk_props(C.c_long(len(x)), *[_p(a) for a in f], _p(t), _p(tkf), _p(eta), _p(e), ra, sp, _p(prd), _p(gx),
+                      C.c_long(len(gx)), _p(gy), C.c_long(len(gy)), _p(gz), C.c_long(len(gz)), C.c_double(nubeg),
+                      C.c_double(nuend), C.c_double(dt), C.c_double(healrate))
+    return eta, e
+
+
 def rk4(x, y, z, tk, eta, mtype, vx, vy, vz, gx, gy, gz, dt):
     x, y, z, tk, eta = (np.array(a, np.float32, copy=True) for a in (x, y, z, tk, eta))
     gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))

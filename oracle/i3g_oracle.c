@@ -1238,3 +1238,116 @@ void or_heat_source(long nx, long ny, long nz, const double *gx, const double *g
 		rate[v3] = ival / rocpkf;
 	}
 }
+
+/* One axis of an epscalc1 interpolation block: the interpolation points are
+ * cell centres (kind 0, base cell + 1 past the mid-line, base in
+ * [1, n-2]), grid lines (kind 1, base in [1, n-3]) or mid-lines (kind 2,
+ * base cell - 1 below the mid-line, base in [0, n-3]); w[0..1] are the
+ * clamped weights of base and base+1 (mark3mg.c:2575-2600, 2646-2662). */
+static long eps_axis(const double *g, long n, long cell, double x, int kind, double *w)
+{
+	long b = cell;
+	double d, p0, p1;
+	if (kind == 0) {
+		if (x > (g[b] + g[b + 1]) / 2.0) b += 1;
+		if (b < 1) b = 1;
+		if (b > n - 2) b = n - 2;
+		d = (g[b + 1] - g[b - 1]) / 2.0;
+		p0 = (g[b - 1] + g[b]) / 2.0;
+		p1 = (g[b] + g[b + 1]) / 2.0;
+	} else if (kind == 1) {
+		if (b < 1) b = 1;
+		if (b > n - 3) b = n - 3;
+		d = g[b + 1] - g[b];
+		p0 = g[b];
+		p1 = g[b + 1];
+	} else {
+		if (x < (g[b] + g[b + 1]) / 2.0) b -= 1;
+		if (b < 0) b = 0;
+		if (b > n - 3) b = n - 3;
+		d = (g[b + 2] - g[b]) / 2.0;
+		p0 = (g[b] + g[b + 1]) / 2.0;
+		p1 = (g[b + 1] + g[b + 2]) / 2.0;
+	}
+	double dd0 = 1.0 - (-((p0 - x) / d)), dd1 = 1.0 - (p1 - x) / d;
+	w[0] = dd0 < 0 ? 0 : dd0 > 1.0 ? 1.0 : dd0;
+	w[1] = dd1 < 0 ? 0 : dd1 > 1.0 ? 1.0 : dd1;
+	return b;
+}
+
+void or_epscalc(const float *const *st12, const double *pr, const double *gx, long nx, const double *gy, long ny,
+		const double *gz, long nz, double x, double y, double z, double *out)
+{
+	/* axis kinds of the four blocks: centres, xy edges, xz edges, yz edges */
+	static const int K[4][3] = {{0, 0, 0}, {1, 1, 2}, {1, 2, 1}, {2, 1, 1}};
+	const float *e[6], *s[6];
+	long nxy = nx * ny, q, blk;
+	for (q = 0; q < 6; q++) { e[q] = st12[q]; s[q] = st12[6 + q]; }
+	for (q = 0; q < 14; q++) out[q] = 0;
+	x = clampx(x, gx[nx - 1]); y = clampx(y, gy[ny - 1]); z = clampx(z, gz[nz - 1]);
+	long m1 = serch(gx, nx, x), m2 = serch(gy, ny, y), m3 = serch(gz, nz, z);
+	for (blk = 0; blk < 4; blk++) {
+		double wx[2], wy[2], wz[2];
+		long a = eps_axis(gx, nx, m1, x, K[blk][0], wx);
+		long b = eps_axis(gy, ny, m2, y, K[blk][1], wy);
+		long c = eps_axis(gz, nz, m3, z, K[blk][2], wz);
+		long i, j, k;
+		for (k = 0; k < 2; k++)
+		for (i = 0; i < 2; i++)
+		for (j = 0; j < 2; j++) {
+			long u = (c + k) * nxy + (a + i) * ny + (b + j);
+			double swt = wx[i] * wy[j] * wz[k];
+			if (blk == 0) {
+				out[0] += e[0][u] * swt;
+				out[1] += e[1][u] * swt;
+				out[2] += e[2][u] * swt;
+				out[7] += s[0][u] * swt;
+				out[8] += s[1][u] * swt;
+				out[9] += s[2][u] * swt;
+				out[13] += (s[0][u] * e[0][u] + s[1][u] * e[1][u] + s[2][u] * e[2][u]) * swt;
+				out[6] += pr[u] * swt;
+			} else {
+				/* blk 1: xy, 2: xz, 3: yz */
+				out[2 + blk] += e[2 + blk][u] * swt;
+				out[9 + blk] += s[2 + blk][u] * swt;
+				out[13] += 2.0 * s[2 + blk][u] * e[2 + blk][u] * swt;
+			}
+		}
+	}
+}
+
+/* mark3mg.c:61-145 (topography, fluid markers and the non-Newtonian
+ * strain-rate correction left out: they do not change eta or e of the
+ * Newtonian rock types) */
+void or_mk_props(long n, const float *x, const float *y, const float *z, const signed char *type, const float *tk,
+		 float *eta, float *e, const or_rock_t *rocks, const float *const *st12, const double *pr,
+		 const double *gx, long nx, const double *gy, long ny, const double *gz, long nz, double nubeg,
+		 double nuend, double dt, double healrate)
+{
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	long i;
+	for (i = 0; i < n; i++) {
+		int mt = type[i];
+		if (!((x[i] >= 0 && y[i] >= 0 && z[i] >= 0 && x[i] <= xs && y[i] <= ys && z[i] <= zs) || mt < 50 || mt >= 100))
+			continue;
+		double o[14];
+		or_epscalc(st12, pr, gx, nx, gy, ny, gz, nz, x[i], y[i], z[i], o);
+		double eii = pow(0.5 * (o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) + o[3] * o[3] + o[4] * o[4] + o[5] * o[5], 0.5);
+		double mpb = o[6] * 1e-5, mtk = tk[i];
+		if (mt < 50 && mtk > 0) {
+			eta[i] = (float)or_visc(&rocks[mt], mtk, mpb, nubeg, nuend);
+			/* viscalc marks the marker as not yielding (its brittle
+			 * inverse viscosity is zero for these rocks,
+			 * mark3mg.c:1179, 1648-1659) */
+			e[i] = -fabsf(e[i]);
+			if (e[i] > 0) e[i] += dt * eii;
+			if (e[i] > 0) {
+				e[i] -= dt * healrate;
+				if (e[i] < 0) e[i] = 0;
+			} else {
+				e[i] += dt * healrate;
+				if (e[i] > 0) e[i] = 0;
+			}
+		}
+	}
+}

@@ -105,6 +105,27 @@ void or_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, const s
 	    const float *vx, const float *vy, const float *vz, const double *gx, long nx, const double *gy,
 	    long ny, const double *gz, long nz, double dt);
 
+/* epscalc1 (mark3mg.c:2535-2838): strain rates, stresses, pressure and
+ * dissipation at a point by trilinear interpolation with weights clamped
+ * to [0,1]: normal components and P from the cell centres, the shear
+ * components from their edge nodes.  st12: exx eyy ezz exy exz eyz, sxx
+ * .. syz planes.  out[0..5] eps xx yy zz xy xz yz, out[6] P,
+ * out[7..12] sigma, out[13] dissipation sigma:eps. */
+void or_epscalc(const float *const *st12, const double *pr, const double *gx, long nx, const double *gy, long ny,
+		const double *gz, long nz, double x, double y, double z, double *out);
+
+/* movemark's marker properties ahead of the position update
+ * (mark3mg.c:61-145) for the Newtonian rock types: for the markers that
+ * move, epscalc1 at the marker gives the strain-rate invariant and the
+ * pressure; markers of type < 50 with T > 0 get eta = viscalc(T, P),
+ * viscalc marks them as not yielding (e = -|e|, no brittle strength for
+ * these rocks), so e is not grown by dt*eii and heals by dt*healrate
+ * toward zero. */
+void or_mk_props(long n, const float *x, const float *y, const float *z, const signed char *type, const float *tk,
+		 float *eta, float *e, const or_rock_t *rocks, const float *const *st12, const double *pr,
+		 const double *gx, long nx, const double *gy, long ny, const double *gz, long nz, double nubeg,
+		 double nuend, double dt, double healrate);
+
 /* tkrecalc (heat3mg.c:846-899): temperature boundary rows on level 0 from
  * a copy of tk (reference bondm layout, T records at 8*nodenum). */
 void or_tkrecalc(long nodenum, const int *bondm, const float *bondv, const int *bondn, float *tk);

@@ -255,3 +255,15 @@ void refh_heat_rhs(long m1, long m2, long m3, double *out)
 	out[0] = heaterr(m1, m2, m3, 0, 0, eps1, un1, ui1);
 	out[1] = tk1[m3 * mgxy[0] + m1 * mgy[0] + m2];
 }
+
+/* epscalc1 (mark3mg.c:2535-2838) at a point: out[0..5] eps1[4..9],
+ * out[6] eps1[10] (P), out[7..12] eps1[54..59], out[13] eps1[60]. */
+void refh_epscalc1(double x, double y, double z, double *out)
+{
+	double eps1[100];
+	long wn1[100];
+	int q;
+	epscalc1(x, y, z, 0, eps1, wn1);
+	for (q = 0; q < 7; q++) out[q] = eps1[4 + q];
+	for (q = 0; q < 7; q++) out[7 + q] = eps1[54 + q];
+}

@@ -504,10 +504,89 @@ def g9(R):
     print("g9_viscmod.npz written")
 
 
+def g10(R):
+    """Run with stdin closed (< /dev/null): with random, unrelated stress
+    and strain-rate fields the dissipation can be negative, where epscalc1
+    prints and waits for a key (mark3mg.c:2834).
+
+    epscalc1 (mark3mg.c:2535-2838) at random points of the g5 grid,
+    including points outside the grid and on grid and mid lines, and one
+    movemark step (mark3mg.c:2-397) of the g5 markers: viscosity from
+    viscalc at the marker pressure of epscalc1, accumulated strain with
+    healing, RK4 positions."""
+    import ctypes as C
+    m5 = np.load(os.path.join(HERE, "g5_markers.npz"))
+    rng = np.random.default_rng(1010)
+    gx, gy, gz = m5["gx"], m5["gy"], m5["gz"]
+    nx, ny, nz = len(gx), len(gy), len(gz)
+    N = nx * ny * nz
+    R.setup_grid(gx, gy, gz, 0)
+    R.set_bc(m5["bondm"], m5["bondv"], m5["bondn"])
+    names = ("exx", "eyy", "ezz", "exy", "exz", "eyz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    st = {k: (rng.standard_normal(N) * (1e-14 if k[0] == "e" else 1e6)).astype(np.float32) for k in names}
+    pr = rng.uniform(1e7, 1e9, N)
+    for k, a in st.items():
+        R.f32(k)[:N] = a
+    R.f64("pr")[:N] = pr
+    npt = 600
+    pts = np.stack([rng.uniform(-2e3, gx[-1] + 2e3, npt), rng.uniform(-2e3, gy[-1] + 2e3, npt),
+                    rng.uniform(-2e3, gz[-1] + 2e3, npt)], 1)
+    pts[:30, 0] = gx[rng.integers(0, nx, 30)]
+    pts[30:60, 1] = (gy[:-1] + gy[1:])[rng.integers(0, ny - 1, 30)] / 2.0
+    pts[60:90, 2] = gz[rng.integers(0, nz, 30)]
+    buf = (C.c_double * 14)()
+    eps = np.zeros((npt, 14))
+    for q, p in enumerate(pts):
+        R.lib.refh_epscalc1(C.c_double(p[0]), C.c_double(p[1]), C.c_double(p[2]), buf)
+        eps[q] = list(buf)
+    # one movemark step of the g5 markers
+    n = len(m5["x"])
+    marke = np.where(rng.uniform(size=n) < 0.4, 0.0, rng.uniform(-0.3, 0.6, n)).astype(np.float32)
+    markv = m5["eta"].copy()
+    vel = {k: (rng.standard_normal(N) * 1e-9).astype(np.float32) for k in ("vx", "vy", "vz")}
+    for k, a in vel.items():
+        R.f32(k)[:N] = a
+    R.set("marknum", n, C.c_long)
+    R.arr("markx", C.c_float, R.MAXMRK)[:n] = m5["x"]
+    R.arr("marky", C.c_float, R.MAXMRK)[:n] = m5["y"]
+    R.arr("markz", C.c_float, R.MAXMRK)[:n] = m5["z"]
+    R.arr("markk", C.c_float, R.MAXMRK)[:n] = m5["tk"]
+    R.arr("markv", C.c_float, R.MAXMRK)[:n] = markv
+    R.arr("markt", C.c_byte, R.MAXMRK)[:n] = m5["mtype"]
+    R.arr("marke", C.c_float, R.MAXMRK)[:n] = marke
+    for gname in ("markd", "markw", "markex", "markft", "markrr", "markto"):
+        R.arr(gname, C.c_float, R.MAXMRK)[:n] = 0
+    rnames = dict(nu0="marknu", dh="markdh", dv="markdv", n0="markn0", n1="markn1", ro="markro", bro="markbb",
+                  aro="markaa", cp="markcp", kt="markkt", kf="markkf", kp="markkp", ht="markht")
+    fields = ("nu0", "dh", "dv", "n0", "n1", "ro", "bro", "aro", "cp", "kt", "kf", "kp", "ht")
+    for q, f in enumerate(fields):
+        a = R.arr(rnames[f], C.c_double, R.MAXTMR)
+        a[:] = 0
+        a[:len(m5["rocks"])] = m5["rocks"][:, q]
+    for gname in ("markss", "markll", "marka0", "marka1", "markb0", "markb1", "marke0", "marke1"):
+        R.arr(gname, C.c_double, R.MAXTMR)[:] = 0
+    R.arr("markmm", C.c_double, R.MAXTMR)[:] = 1.0
+    dt = 3e11
+    for k, v in dict(timestep=dt, healrate=2e-15, nubeg=float(m5["nubeg"]), nuend=float(m5["nuend"]), zdeep=1e30,
+                     vdeep=1e30, tdeep=0.0, stp100=1e30, vyfluid=0.0, vymelt=0.0,
+                     timesum=0.0).items():
+        R.set(k, v)
+    R.set("outgrid", 0, C.c_int)
+    R.set("markmod", 0, C.c_int)
+    R.lib.refh_movemark()
+    out = dict(x_out=R.arr("markx", C.c_float, R.MAXMRK)[:n].copy(), y_out=R.arr("marky", C.c_float, R.MAXMRK)[:n].copy(),
+               z_out=R.arr("markz", C.c_float, R.MAXMRK)[:n].copy(), tk_out=R.arr("markk", C.c_float, R.MAXMRK)[:n].copy(),
+               eta_out=R.arr("markv", C.c_float, R.MAXMRK)[:n].copy(), e_out=R.arr("marke", C.c_float, R.MAXMRK)[:n].copy(),
+               type_out=R.arr("markt", C.c_byte, R.MAXMRK)[:n].copy())
+    np.savez_compressed(os.path.join(HERE, "g10_eps_movemark.npz"), pts=pts, eps=eps, pr=pr, **st, **vel,
+                        marke=marke, markv=markv, dt=dt, healrate=2e-15, **out)
+    print("g10_eps_movemark.npz written")
+
+
 if __name__ == "__main__":
     R = Ref()
     R.lib.refh_set_threads(1)
-    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9"]
+    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9", "g10"]
     if "g1" in which:
         g1(R)
     if "g3" in which:
@@ -523,3 +602,5 @@ if __name__ == "__main__":
         g8(R)
     if "g9" in which:
         g9(R)
+    if "g10" in which:
+        g10(R)

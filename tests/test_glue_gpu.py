@@ -144,6 +144,8 @@ def test_glue_movemark_matches_oracle(gpu):
     N = nx * ny * nz
     R = ref()
     R.setup_grid(g["gx"], g["gy"], g["gz"], 0)
+    R.set_bc(g["bondm"], g["bondv"], g["bondn"])
+    R.set("bondnum", len(g["bondn"]) - 1, C.c_long)
     m = len(g["x"])
     R.set("marknum", m, C.c_long)
     for name, a in (("markx", g["x"]), ("marky", g["y"]), ("markz", g["z"]), ("markk", g["tk"]), ("markv", g["eta"])):
@@ -202,3 +204,52 @@ def test_glue_titerate_with_heating_terms_matches_reference(gpu):
     rel = np.linalg.norm(tk.astype(np.float64) - g["tk_out"]) / np.linalg.norm(g["tk_out"])
     print("tk rel L2", rel, flush=True)
     assert rel < 1e-7
+
+
+def test_glue_movemark_matches_reference(gpu):
+    """i3g_movemark on the reference's globals (g10 inputs: the g5 markers
+    with random accumulated strain, stress/strain-rate, pressure and
+    velocity fields): viscosity and strain from epscalc1 + viscalc at the
+    old positions, then RK4, against the reference's own movemark (g10)."""
+    g = golden("g5_markers.npz")
+    r = golden("g10_eps_movemark.npz")
+    nx, ny, nz = len(g["gx"]), len(g["gy"]), len(g["gz"])
+    N = nx * ny * nz
+    print("case: glue movemark vs reference", len(g["x"]), flush=True)
+    R = ref()
+    R.setup_grid(g["gx"], g["gy"], g["gz"], 0)
+    R.set_bc(g["bondm"], g["bondv"], g["bondn"])
+    R.set("bondnum", len(g["bondn"]) - 1, C.c_long)
+    m = len(g["x"])
+    R.set("marknum", m, C.c_long)
+    for name, a in (("markx", g["x"]), ("marky", g["y"]), ("markz", g["z"]), ("markk", g["tk"]), ("markv", r["markv"]),
+                    ("marke", r["marke"])):
+        R.arr(name, C.c_float, R.MAXMRK)[:m] = a
+    R.arr("markt", C.c_byte, R.MAXMRK)[:m] = g["mtype"]
+    names = dict(nu0="marknu", dh="markdh", dv="markdv", n0="markn0", n1="markn1", ro="markro", bro="markbb",
+                 aro="markaa", cp="markcp", kt="markkt", kf="markkf", kp="markkp", ht="markht")
+    fields = ("nu0", "dh", "dv", "n0", "n1", "ro", "bro", "aro", "cp", "kt", "kf", "kp", "ht")
+    for q, f in enumerate(fields):
+        a = R.arr(names[f], C.c_double, R.MAXTMR)
+        a[:] = 0
+        a[:len(g["rocks"])] = g["rocks"][:, q]
+    R.set("rocknum", len(g["rocks"]), C.c_int)
+    for k in ("exx", "eyy", "ezz", "exy", "exz", "eyz", "sxx", "syy", "szz", "sxy", "sxz", "syz", "vx", "vy", "vz"):
+        R.f32(k)[:N] = r[k]
+    R.f64("pr")[:N] = r["pr"]
+    for k, v in dict(timestep=float(r["dt"]), healrate=float(r["healrate"]), nubeg=float(g["nubeg"]),
+                     nuend=float(g["nuend"]), vdeep=1e30, nudeep=1e30, nukoef=0.0).items():
+        R.set(k, v)
+    R.set("viscmod", 0, C.c_int)
+    R.set("densimod", 1, C.c_int)
+    R.lib.i3g_movemark()
+    eta = R.arr("markv", C.c_float, R.MAXMRK)[:m]
+    rel = np.abs(eta.astype(np.float64) - r["eta_out"]) / np.maximum(np.abs(r["eta_out"]), 1e-30)
+    print("eta max rel", rel.max(), flush=True)
+    assert rel.max() <= 2e-7
+    assert np.array_equal(R.arr("marke", C.c_float, R.MAXMRK)[:m], r["e_out"])
+    for name, k in (("markx", "x_out"), ("marky", "y_out"), ("markz", "z_out")):
+        a = R.arr(name, C.c_float, R.MAXMRK)[:m]
+        ulp = np.abs(a.view(np.int32).astype(np.int64) - r[k].view(np.int32).astype(np.int64))
+        assert ulp.max() <= 1, (name, ulp.max())
+    assert np.array_equal(R.arr("markk", C.c_float, R.MAXMRK)[:m] == 0, r["tk_out"] == 0)

@@ -177,3 +177,58 @@ def test_to_nodes_viscmod_matches_reference(gpu, vm):
         assert rel.max() <= 4e-7, (vm, q, rel.max())
     assert not np.array_equal(ref[0], g["out"][0])
     M.close()
+
+
+_ST12 = ("exx", "eyy", "ezz", "exy", "exz", "eyz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+
+
+def test_epscalc_matches_oracle_bitwise(gpu):
+    """i3g_markers_epscalc at 600 markers (points of g10: inside, outside
+    -- clamped -- and on grid and mid lines) against the oracle's epscalc1
+    restatement, itself equal to the reference to the bit: equal to the
+    bit (the device uses the same summation order, nothing contracted)."""
+    g = golden("g5_markers.npz")
+    r = golden("g10_eps_movemark.npz")
+    pts = r["pts"].astype(np.float32)
+    n = len(pts)
+    print("case: epscalc at", n, "markers", flush=True)
+    M = I.Markers(g["gx"], g["gy"], g["gz"], n)
+    M.upload(pts[:, 0], pts[:, 1], pts[:, 2], np.ones(n, np.float32), np.zeros(n, np.float32), np.full(n, 2, np.int8))
+    st = [r[k] for k in _ST12]
+    out = M.epscalc(st, r["pr"])
+    ref = np.array([oracle.epscalc(st, r["pr"], g["gx"], g["gy"], g["gz"], *p.astype(np.float64)) for p in pts])
+    d = np.abs(out - ref).max(axis=0)
+    print("max |diff| per output", d, flush=True)
+    assert np.array_equal(out, ref)
+    M.close()
+
+
+def test_movemark_props_and_rk4_match_reference(gpu):
+    """One movemark step of the g5 markers through the device
+    (i3g_markers_update_props, then i3g_markers_advect_rk4) against the
+    reference's own movemark (g10): viscosity from viscalc at the marker
+    pressure of epscalc1 (2e-7, the device exp), accumulated strain
+    (bitwise), positions (1 ulp) and the out-of-grid reset."""
+    g = golden("g5_markers.npz")
+    r = golden("g10_eps_movemark.npz")
+    print("case: movemark props + rk4", len(g["x"]), flush=True)
+    M = _markers(g)
+    n = len(g["x"])
+    M.upload(g["x"], g["y"], g["z"], g["tk"], r["markv"], g["mtype"])
+    M.set_strain(r["marke"])
+    rocks = [list(x) for x in g["rocks"]]
+    dt = float(r["dt"])
+    M.update_props(rocks, float(g["nubeg"]), float(g["nuend"]), dt, float(r["healrate"]), [r[k] for k in _ST12], r["pr"])
+    M.advect_rk4(dt, r["vx"], r["vy"], r["vz"])
+    d = M.download()
+    e = M.get_strain()
+    assert np.array_equal(d["id"], np.arange(n))
+    assert np.array_equal(e, r["e_out"])
+    rel = np.abs(d["eta"].astype(np.float64) - r["eta_out"]) / np.maximum(np.abs(r["eta_out"]), 1e-30)
+    print("eta max rel", rel.max(), "exact", int((d["eta"] == r["eta_out"]).sum()), "of", n, flush=True)
+    assert rel.max() <= 2e-7
+    for k in ("x", "y", "z"):
+        ulp = np.abs(d[k].view(np.int32).astype(np.int64) - r[k + "_out"].view(np.int32).astype(np.int64))
+        assert ulp.max() <= 1, (k, ulp.max())
+    assert np.array_equal(d["tk"] == 0, r["tk_out"] == 0)
+    M.close()

@@ -61,3 +61,36 @@ def test_rk4_uniform_flow_translates():
                      vx, vy, vz, gx, gx, gx, dt)
     assert np.array_equal(out[0], (x.astype(np.float64) + dt * np.float64(np.float32(1e-9))).astype(np.float32))
     assert np.array_equal(out[1], (y.astype(np.float64) + dt * np.float64(np.float32(-2e-9))).astype(np.float32))
+
+
+def test_epscalc_matches_reference_bitwise():
+    """epscalc1 (strain rates, stresses, P, dissipation at a point with
+    clamped trilinear weights) against the reference at 600 points of the
+    g5 grid, including points outside the grid (clamped) and on grid and
+    mid lines (g10): equal to the bit."""
+    g = golden("g5_markers.npz")
+    r = golden("g10_eps_movemark.npz")
+    st = [r[k] for k in ("exx", "eyy", "ezz", "exy", "exz", "eyz", "sxx", "syy", "szz", "sxy", "sxz", "syz")]
+    out = np.array([oracle.epscalc(st, r["pr"], g["gx"], g["gy"], g["gz"], *p) for p in r["pts"]])
+    assert np.array_equal(out, r["eps"])
+
+
+def test_epscalc_closed_forms():
+    """Constant fields are reproduced exactly inside the grid (weights of
+    each block sum to one), and the dissipation of constant fields is
+    sigma:eps with the shear terms doubled."""
+    g = golden("g5_markers.npz")
+    N = len(g["pr"])
+    vals = [1e-15, 2e-15, -3e-15, 4e-15, 5e-15, -6e-15, 1e6, 2e6, 3e6, -4e6, 5e6, 6e6]
+    st = [np.full(N, v, np.float32) for v in vals]
+    pr = np.full(N, 3e8)
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        p = [rng.uniform(g[k][2], g[k][-3]) for k in ("gx", "gy", "gz")]
+        out = oracle.epscalc(st, pr, g["gx"], g["gy"], g["gz"], *p)
+        f = np.float32(vals)
+        assert np.allclose(out[:6], f[:6], rtol=1e-12)
+        assert np.isclose(out[6], 3e8, rtol=1e-12)
+        assert np.allclose(out[7:13], f[6:], rtol=1e-12)
+        dis = float(f[6] * f[0] + f[7] * f[1] + f[8] * f[2]) + 2 * sum(float(f[9 + q]) * float(f[3 + q]) for q in range(3))
+        assert np.isclose(out[13], dis, rtol=1e-6)
