@@ -367,6 +367,10 @@ class Markers:
                                          *[_p(fld[q]) for q in range(11)]), "markers_to_nodes")
         return fld
 
+    def set_depth_rules(self, zdeep=1e30, vdeep=1e30, tdeep=0.0):
+        check(lib().i3g_markers_set_depth_rules(self.h, C.c_double(zdeep), C.c_double(vdeep), C.c_double(tdeep)),
+              "markers_set_depth_rules")
+
     def set_strain(self, e):
         e = np.ascontiguousarray(e, np.float32)
         check(lib().i3g_markers_set_strain(self.h, _p(e)), "markers_set_strain")

@@ -59,6 +59,8 @@ struct i3g_markers {
 	float *fld = nullptr;    /* 11 x N float in/out fields */
 	float *vel = nullptr;    /* 3 x N velocities */
 	float *st12 = nullptr;   /* 12 x N strain rates and stresses (epscalc1) */
+	/* movemark's depth rules (inactive by default) */
+	double zdeep = 1e30, vdeep = 1e30, tdeep = 0;
 	i3g_rock *rocks = nullptr;
 	int *bad = nullptr;
 	/* temperature boundary rows of level 0 (tkrecalc): per node partner
@@ -773,9 +775,17 @@ __global__ void k_mk_props(long n, const float *__restrict__ x, const float *__r
 /* RK4 advection                                                      */
 /* ------------------------------------------------------------------ */
 
-__global__ void k_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, const signed char *type,
+/* zdeep / vdeep / tdeep of movemark's depth rules (mark3mg.c:254-276):
+ * below zdeep a sinking marker becomes type 10 (at least tdeep when it is
+ * already 10), any other below zdeep type 9; a marker rising through
+ * vdeep or zdeep becomes type 10 with T at least tdeep. */
+struct DepthRules {
+	double zdeep, vdeep, tdeep;
+};
+
+__global__ void k_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, signed char *type,
 		      const float *vx, const float *vy, const float *vz, const double *gx, const double *gy,
-		      const double *gz, int nx, int ny, int nz, double dt)
+		      const double *gz, int nx, int ny, int nz, double dt, DepthRules D)
 {
 	extern __shared__ double sg[];
 	stage_lines(sg, gx, gy, gz, nx, ny, nz);
@@ -799,10 +809,21 @@ __global__ void k_rk4(long n, float *x, float *y, float *z, float *tk, float *et
 	double v[3];
 	for (int q = 0; q < 3; q++) v[q] = (v1[q] + 2.0 * v2[q] + 2.0 * v3[q] + v4[q]) / 6.0;
 	if (mt < 100) {
+		float ftk = tk[t];
+		double yold = fy;
+		if ((double)fy > D.zdeep && v[1] < 0 && (double)ftk < D.tdeep && mt == 10) ftk = (float)D.tdeep;
+		if ((double)fy > D.zdeep && v[1] < 0 && mt != 10) mt = 10;
+		if ((double)fy > D.zdeep && mt != 10 && mt != 9) mt = 9;
 		fx = (float)((double)fx + (dt * v[0] - 0.0));
 		fy = (float)((double)fy + dt * v[1]);
 		fz = (float)((double)fz + dt * v[2]);
 		x[t] = fx; y[t] = fy; z[t] = fz;
+		if ((yold > D.vdeep && (double)fy < D.vdeep) || (yold > D.zdeep && (double)fy < D.zdeep)) {
+			if ((double)ftk < D.tdeep) ftk = (float)D.tdeep;
+			mt = 10;
+		}
+		tk[t] = ftk;
+		type[t] = (signed char)mt;
 		if (fx < 0 || fy < 0 || fz < 0 || (double)fx > xs || (double)fy > ys || (double)fz > zs) {
 			tk[t] = 0;
 			eta[t] = 0;
@@ -1161,9 +1182,9 @@ extern "C" int i3g_markers_advect_rk4(i3g_markers *m, double dt, const float *vx
 	mark_begin(m);
 	if (n)
 		hipLaunchKernelGGL(k_rk4, g1(n), dim3(256), lines_bytes(m), m->s, n, m->x, m->y, m->z, m->tk, m->eta,
-				   (const signed char *)m->type, (const float *)m->vel, (const float *)(m->vel + N),
+				   m->type, (const float *)m->vel, (const float *)(m->vel + N),
 				   (const float *)(m->vel + 2 * N), (const double *)m->gx, (const double *)m->gy,
-				   (const double *)m->gz, m->nx, m->ny, m->nz, dt);
+				   (const double *)m->gz, m->nx, m->ny, m->nz, dt, DepthRules{m->zdeep, m->vdeep, m->tdeep});
 	MCHK(hipGetLastError());
 	mark_end(m);
 	MCHK(hipStreamSynchronize(m->s));
@@ -1258,4 +1279,13 @@ extern "C" int i3g_markers_update_props(i3g_markers *m, const i3g_rock *rocks, i
 	MCHK(hipStreamSynchronize(m->s));
 	mark_read(m);
 	return bad ? I3G_EINVAL : I3G_OK;
+}
+
+extern "C" int i3g_markers_set_depth_rules(i3g_markers *m, double zdeep, double vdeep, double tdeep)
+{
+	if (!m) return I3G_EINVAL;
+	m->zdeep = zdeep;
+	m->vdeep = vdeep;
+	m->tdeep = tdeep;
+	return I3G_OK;
 }

@@ -111,7 +111,8 @@ static void glue_markers(void)
 		glue_fail("markers_upload", rc);
 }
 
-/* device markers back into markx..markv (by id); with_strain also marke */
+/* device markers back into markx..markv (by id); with_strain (after
+ * movemark) also marke and markt */
 static void glue_markers_back(int with_strain)
 {
 	int rc;
@@ -127,7 +128,10 @@ static void glue_markers_back(int with_strain)
 		long j = id[i];
 		markx[j] = buf[i]; marky[j] = buf[n + i]; markz[j] = buf[2 * n + i];
 		markk[j] = buf[3 * n + i]; markv[j] = buf[4 * n + i];
-		if (with_strain) marke[j] = buf[5 * n + i];
+		if (with_strain) {
+			marke[j] = buf[5 * n + i];
+			markt[j] = t[i]; /* the depth rules of movemark change types */
+		}
 	}
 	free(buf); free(t); free(id);
 }
@@ -271,6 +275,7 @@ void i3g_movemark(void)
 	mp.vdeep = vdeep; mp.nudeep = nudeep; mp.viscmod = viscmod;
 	const float *st12[12] = {exx, eyy, ezz, exy, exz, eyz, sxx, syy, szz, sxy, sxz, syz};
 	if ((rc = i3g_markers_update_props(G.mk, rocks, nr, &mp, healrate, st12, pr))) glue_fail("markers_update_props", rc);
+	if ((rc = i3g_markers_set_depth_rules(G.mk, zdeep, vdeep, tdeep))) glue_fail("markers_set_depth_rules", rc);
 	if ((rc = i3g_markers_advect_rk4(G.mk, timestep, vx, vy, vz))) glue_fail("markers_advect_rk4", rc);
 	glue_markers_back(1);
 }

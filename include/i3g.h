@@ -369,6 +369,13 @@ int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int nrock, const
 			 const double *pr, float *nu, float *ro, float *cp, float *kt, float *ht,
 			 float *et, float *tk, float *nxx, float *nxy, float *nxz, float *nyz);
 
+/* movemark's depth rules for the following i3g_markers_advect_rk4 calls
+ * (mark3mg.c:254-276): a marker below zdeep that sinks becomes type 10 (T
+ * raised to tdeep if it already was), any other marker below zdeep type 9;
+ * a marker rising through vdeep or zdeep becomes type 10 with T at least
+ * tdeep.  Default zdeep = vdeep = 1e30 (off). */
+int i3g_markers_set_depth_rules(i3g_markers *m, double zdeep, double vdeep, double tdeep);
+
 /* Accumulated strain of the markers (the reference's marke), in the
  * current marker order (that of i3g_markers_download); upload sets it to 0.
  * Sorting keeps it with its marker. */

@@ -203,12 +203,19 @@ def mk_props(x, y, z, mtype, tk, eta, e, rocks, st12, pr, gx, gy, gz, nubeg, nue
     return eta, e
 
 
-def rk4(x, y, z, tk, eta, mtype, vx, vy, vz, gx, gy, gz, dt):
+def rk4(x, y, z, tk, eta, mtype, vx, vy, vz, gx, gy, gz, dt, depth=None):
+    """RK4 move; depth=(zdeep, vdeep, tdeep) also applies movemark's depth
+    rules and returns the new types as a sixth value"""
     x, y, z, tk, eta = (np.array(a, np.float32, copy=True) for a in (x, y, z, tk, eta))
+    t = np.array(mtype, np.int8, copy=True)
     gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
-    lib().or_rk4(C.c_long(len(x)), _p(x), _p(y), _p(z), _p(tk), _p(eta), _p(mtype), _p(vx), _p(vy), _p(vz),
+    zd, vd, td = depth if depth is not None else (1e30, 1e30, 0.0)
+    v = [np.ascontiguousarray(a, np.float32) for a in (vx, vy, vz)]
+    lib().or_rk4(C.c_long(len(x)), _p(x), _p(y), _p(z), _p(tk), _p(eta), _p(t), *[_p(a) for a in v],
                  _p(gx), C.c_long(len(gx)), _p(gy), C.c_long(len(gy)), _p(gz), C.c_long(len(gz)),
-                 C.c_double(dt))
+                 C.c_double(dt), C.c_double(zd), C.c_double(vd), C.c_double(td))
+    if depth is not None:
+        return x, y, z, tk, eta, t
     return x, y, z, tk, eta
 
 

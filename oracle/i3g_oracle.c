@@ -782,9 +782,9 @@ void or_vxyz(const float *vx, const float *vy, const float *vz, const double *gx
 	}
 }
 
-void or_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, const signed char *type,
+void or_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, signed char *type,
 	    const float *vx, const float *vy, const float *vz, const double *gx, long nx, const double *gy,
-	    long ny, const double *gz, long nz, double dt)
+	    long ny, const double *gz, long nz, double dt, double zdeep, double vdeep, double tdeep)
 {
 	long i;
 	int q;
@@ -800,9 +800,17 @@ void or_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, const s
 		or_vxyz(vx, vy, vz, gx, nx, gy, ny, gz, nz, px + v3[0] * dt, py + v3[1] * dt, pz + v3[2] * dt, v4);
 		for (q = 0; q < 3; q++) v[q] = (v1[q] + 2.0 * v2[q] + 2.0 * v3[q] + v4[q]) / 6.0;
 		if (mt < 100) {
+			double yold = y[i];
+			if (y[i] > zdeep && v[1] < 0 && tk[i] < tdeep && type[i] == 10) tk[i] = (float)tdeep;
+			if (y[i] > zdeep && v[1] < 0 && type[i] != 10) type[i] = 10;
+			if (y[i] > zdeep && type[i] != 10 && type[i] != 9) type[i] = 9;
 			x[i] += dt * v[0] - 0.0;
 			y[i] += dt * v[1];
 			z[i] += dt * v[2];
+			if ((yold > vdeep && y[i] < vdeep) || (yold > zdeep && y[i] < zdeep)) {
+				if (tk[i] < tdeep) tk[i] = (float)tdeep;
+				type[i] = 10;
+			}
 			if (x[i] < 0 || y[i] < 0 || z[i] < 0 || x[i] > xs || y[i] > ys || z[i] > zs) {
 				tk[i] = 0;
 				eta[i] = 0;

@@ -100,10 +100,11 @@ void or_vxyz(const float *vx, const float *vy, const float *vz, const double *gx
 	     long ny, const double *gz, long nz, double x, double y, double z, double *v);
 
 /* movemark RK4 position update (mark3mg.c:215-284) for markers of type <
- * 50 inside the grid; out-of-grid markers get tk = eta = 0. */
-void or_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, const signed char *type,
+ * 100 that move, with the depth rules (zdeep, vdeep, tdeep; 1e30 = off,
+ * mark3mg.c:254-276); out-of-grid markers get tk = eta = 0. */
+void or_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, signed char *type,
 	    const float *vx, const float *vy, const float *vz, const double *gx, long nx, const double *gy,
-	    long ny, const double *gz, long nz, double dt);
+	    long ny, const double *gz, long nz, double dt, double zdeep, double vdeep, double tdeep);
 
 /* epscalc1 (mark3mg.c:2535-2838): strain rates, stresses, pressure and
  * dissipation at a point by trilinear interpolation with weights clamped

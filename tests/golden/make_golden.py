@@ -583,10 +583,71 @@ def g10(R):
     print("g10_eps_movemark.npz written")
 
 
+def g11(R):
+    """One movemark step (mark3mg.c:2-397) of the g5 markers with the depth
+    rules active (zdeep, vdeep inside the grid, tdeep above some marker
+    temperatures) and velocities large enough for markers to cross them;
+    inputs as g10 otherwise.  Run with stdin closed (see g10)."""
+    import ctypes as C
+    m5 = np.load(os.path.join(HERE, "g5_markers.npz"))
+    r10 = np.load(os.path.join(HERE, "g10_eps_movemark.npz"))
+    rng = np.random.default_rng(1111)
+    gx, gy, gz = m5["gx"], m5["gy"], m5["gz"]
+    N = len(gx) * len(gy) * len(gz)
+    R.setup_grid(gx, gy, gz, 0)
+    R.set_bc(m5["bondm"], m5["bondv"], m5["bondn"])
+    names = ("exx", "eyy", "ezz", "exy", "exz", "eyz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    for k in names:
+        R.f32(k)[:N] = r10[k]
+    R.f64("pr")[:N] = r10["pr"]
+    vel = {k: (rng.standard_normal(N) * 2e-8).astype(np.float32) for k in ("vx", "vy", "vz")}
+    for k, a in vel.items():
+        R.f32(k)[:N] = a
+    n = len(m5["x"])
+    mtype = m5["mtype"].copy()
+    R.set("marknum", n, C.c_long)
+    R.arr("markx", C.c_float, R.MAXMRK)[:n] = m5["x"]
+    R.arr("marky", C.c_float, R.MAXMRK)[:n] = m5["y"]
+    R.arr("markz", C.c_float, R.MAXMRK)[:n] = m5["z"]
+    R.arr("markk", C.c_float, R.MAXMRK)[:n] = m5["tk"]
+    R.arr("markv", C.c_float, R.MAXMRK)[:n] = r10["markv"]
+    R.arr("markt", C.c_byte, R.MAXMRK)[:n] = mtype
+    R.arr("marke", C.c_float, R.MAXMRK)[:n] = r10["marke"]
+    for gname in ("markd", "markw", "markex", "markft", "markrr", "markto"):
+        R.arr(gname, C.c_float, R.MAXMRK)[:n] = 0
+    rnames = dict(nu0="marknu", dh="markdh", dv="markdv", n0="markn0", n1="markn1", ro="markro", bro="markbb",
+                  aro="markaa", cp="markcp", kt="markkt", kf="markkf", kp="markkp", ht="markht")
+    fields = ("nu0", "dh", "dv", "n0", "n1", "ro", "bro", "aro", "cp", "kt", "kf", "kp", "ht")
+    for q, f in enumerate(fields):
+        a = R.arr(rnames[f], C.c_double, R.MAXTMR)
+        a[:] = 0
+        a[:len(m5["rocks"])] = m5["rocks"][:, q]
+    for gname in ("markss", "markll", "marka0", "marka1", "markb0", "markb1", "marke0", "marke1"):
+        R.arr(gname, C.c_double, R.MAXTMR)[:] = 0
+    R.arr("markmm", C.c_double, R.MAXTMR)[:] = 1.0
+    dt = 3e11
+    zdeep, vdeep, tdeep = 0.55 * gy[-1], 0.7 * gy[-1], 1200.0
+    for k, v in dict(timestep=dt, healrate=2e-15, nubeg=float(m5["nubeg"]), nuend=float(m5["nuend"]), zdeep=zdeep,
+                     vdeep=vdeep, tdeep=tdeep, stp100=1e30, vyfluid=0.0, vymelt=0.0, timesum=0.0).items():
+        R.set(k, v)
+    R.set("outgrid", 0, C.c_int)
+    R.set("markmod", 0, C.c_int)
+    R.lib.refh_movemark()
+    out = dict(x_out=R.arr("markx", C.c_float, R.MAXMRK)[:n].copy(), y_out=R.arr("marky", C.c_float, R.MAXMRK)[:n].copy(),
+               z_out=R.arr("markz", C.c_float, R.MAXMRK)[:n].copy(), tk_out=R.arr("markk", C.c_float, R.MAXMRK)[:n].copy(),
+               eta_out=R.arr("markv", C.c_float, R.MAXMRK)[:n].copy(), e_out=R.arr("marke", C.c_float, R.MAXMRK)[:n].copy(),
+               type_out=R.arr("markt", C.c_byte, R.MAXMRK)[:n].copy())
+    print("types changed:", int((out["type_out"] != mtype).sum()), "tk raised:",
+          int((out["tk_out"] > m5["tk"]).sum()))
+    np.savez_compressed(os.path.join(HERE, "g11_movemark_depth.npz"), **vel, dt=dt, healrate=2e-15, zdeep=zdeep,
+                        vdeep=vdeep, tdeep=tdeep, **out)
+    print("g11_movemark_depth.npz written")
+
+
 if __name__ == "__main__":
     R = Ref()
     R.lib.refh_set_threads(1)
-    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9", "g10"]
+    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9", "g10", "g11"]
     if "g1" in which:
         g1(R)
     if "g3" in which:
@@ -604,3 +665,5 @@ if __name__ == "__main__":
         g9(R)
     if "g10" in which:
         g10(R)
+    if "g11" in which:
+        g11(R)

@@ -206,13 +206,22 @@ def test_glue_titerate_with_heating_terms_matches_reference(gpu):
     assert rel < 1e-7
 
 
-def test_glue_movemark_matches_reference(gpu):
+@pytest.mark.parametrize("case", ["g10", "g11"])
+def test_glue_movemark_matches_reference(gpu, case):
     """i3g_movemark on the reference's globals (g10 inputs: the g5 markers
     with random accumulated strain, stress/strain-rate, pressure and
-    velocity fields): viscosity and strain from epscalc1 + viscalc at the
-    old positions, then RK4, against the reference's own movemark (g10)."""
+    velocity fields; g11: faster flow with the zdeep/vdeep depth rules
+    active): viscosity and strain from epscalc1 + viscalc at the old
+    positions, then RK4 with the depth rules, against the reference's own
+    movemark."""
     g = golden("g5_markers.npz")
-    r = golden("g10_eps_movemark.npz")
+    r10 = golden("g10_eps_movemark.npz")
+    r = dict(r10)
+    depth = dict(zdeep=1e30, vdeep=1e30, tdeep=0.0)
+    if case == "g11":
+        r11 = golden("g11_movemark_depth.npz")
+        r.update(dict(r11))
+        depth = dict(zdeep=float(r11["zdeep"]), vdeep=float(r11["vdeep"]), tdeep=float(r11["tdeep"]))
     nx, ny, nz = len(g["gx"]), len(g["gy"]), len(g["gz"])
     N = nx * ny * nz
     print("case: glue movemark vs reference", len(g["x"]), flush=True)
@@ -238,7 +247,7 @@ def test_glue_movemark_matches_reference(gpu):
         R.f32(k)[:N] = r[k]
     R.f64("pr")[:N] = r["pr"]
     for k, v in dict(timestep=float(r["dt"]), healrate=float(r["healrate"]), nubeg=float(g["nubeg"]),
-                     nuend=float(g["nuend"]), vdeep=1e30, nudeep=1e30, nukoef=0.0).items():
+                     nuend=float(g["nuend"]), nudeep=1e30, nukoef=0.0, **depth).items():
         R.set(k, v)
     R.set("viscmod", 0, C.c_int)
     R.set("densimod", 1, C.c_int)
@@ -253,3 +262,4 @@ def test_glue_movemark_matches_reference(gpu):
         ulp = np.abs(a.view(np.int32).astype(np.int64) - r[k].view(np.int32).astype(np.int64))
         assert ulp.max() <= 1, (name, ulp.max())
     assert np.array_equal(R.arr("markk", C.c_float, R.MAXMRK)[:m] == 0, r["tk_out"] == 0)
+    assert np.array_equal(R.arr("markt", C.c_byte, R.MAXMRK)[:m], r["type_out"])

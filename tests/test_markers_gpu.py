@@ -232,3 +232,32 @@ def test_movemark_props_and_rk4_match_reference(gpu):
         assert ulp.max() <= 1, (k, ulp.max())
     assert np.array_equal(d["tk"] == 0, r["tk_out"] == 0)
     M.close()
+
+
+def test_movemark_depth_rules_match_reference(gpu):
+    """The device movemark step with the depth rules active (g11): types,
+    temperatures and positions as the reference's movemark."""
+    g = golden("g5_markers.npz")
+    r10 = golden("g10_eps_movemark.npz")
+    r = golden("g11_movemark_depth.npz")
+    print("case: movemark with depth rules", len(g["x"]), flush=True)
+    M = _markers(g)
+    M.upload(g["x"], g["y"], g["z"], g["tk"], r10["markv"], g["mtype"])
+    M.set_strain(r10["marke"])
+    dt = float(r["dt"])
+    M.update_props([list(x) for x in g["rocks"]], float(g["nubeg"]), float(g["nuend"]), dt, float(r["healrate"]),
+                   [r10[k] for k in _ST12], r10["pr"])
+    M.set_depth_rules(float(r["zdeep"]), float(r["vdeep"]), float(r["tdeep"]))
+    M.advect_rk4(dt, r["vx"], r["vy"], r["vz"])
+    d = M.download()
+    assert np.array_equal(d["type"], r["type_out"])
+    assert np.array_equal(M.get_strain(), r["e_out"])
+    for k in ("x", "y", "z"):
+        ulp = np.abs(d[k].view(np.int32).astype(np.int64) - r[k + "_out"].view(np.int32).astype(np.int64))
+        assert ulp.max() <= 1, (k, ulp.max())
+    ok = d["tk"] == r["tk_out"]
+    print("tk exact", int(ok.sum()), "of", len(ok), flush=True)
+    assert np.array_equal(d["tk"] == 0, r["tk_out"] == 0)
+    assert np.allclose(d["tk"], r["tk_out"], rtol=1e-7, atol=0)
+    M.set_depth_rules()
+    M.close()

@@ -94,3 +94,24 @@ def test_epscalc_closed_forms():
         assert np.allclose(out[7:13], f[6:], rtol=1e-12)
         dis = float(f[6] * f[0] + f[7] * f[1] + f[8] * f[2]) + 2 * sum(float(f[9 + q]) * float(f[3 + q]) for q in range(3))
         assert np.isclose(out[13], dis, rtol=1e-6)
+
+
+def test_movemark_with_depth_rules_matches_reference():
+    """movemark of the g5 markers with zdeep/vdeep inside the grid (g11,
+    written from the reference): the oracle's property update and RK4 with
+    the depth rules reproduce viscosity, strain, positions, temperatures
+    and the changed types bit for bit."""
+    g = golden("g5_markers.npz")
+    r10 = golden("g10_eps_movemark.npz")
+    r = golden("g11_movemark_depth.npz")
+    st = [r10[k] for k in ("exx", "eyy", "ezz", "exy", "exz", "eyz", "sxx", "syy", "szz", "sxy", "sxz", "syz")]
+    rocks = [dict(zip(oracle.ROCK_FIELDS, row)) for row in g["rocks"]]
+    dt = float(r["dt"])
+    eta, e = oracle.mk_props(g["x"], g["y"], g["z"], g["mtype"], g["tk"], r10["markv"], r10["marke"], rocks, st, r10["pr"],
+                             g["gx"], g["gy"], g["gz"], float(g["nubeg"]), float(g["nuend"]), dt, float(r["healrate"]))
+    x, y, z, tk, eta, t = oracle.rk4(g["x"], g["y"], g["z"], g["tk"], eta, g["mtype"], r["vx"], r["vy"], r["vz"], g["gx"],
+                                     g["gy"], g["gz"], dt, depth=(float(r["zdeep"]), float(r["vdeep"]), float(r["tdeep"])))
+    for a, b in ((x, r["x_out"]), (y, r["y_out"]), (z, r["z_out"]), (tk, r["tk_out"]), (eta, r["eta_out"]),
+                 (e, r["e_out"]), (t, r["type_out"])):
+        assert np.array_equal(a, b)
+    assert (t != g["mtype"]).sum() > 100
