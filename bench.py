@@ -44,40 +44,37 @@ def smoother_bytes_per_launch(N, word):
     return N * (8 * word + 1 + word / 2)
 
 
-class _stdout_to_stderr:
-    """Sends what C code prints to file descriptor 1 (the reference prints
-    progress with printf) to stderr, so that stdout carries only the JSON
-    line; C stdio buffers are flushed before the descriptor is restored."""
-
-    def __enter__(self):
-        import ctypes
-        self._libc = ctypes.CDLL(None)
-        sys.stdout.flush()
-        self._saved = os.dup(1)
-        os.dup2(2, 1)
-        return self
-
-    def __exit__(self, *exc):
-        self._libc.fflush(None)
-        sys.stdout.flush()
-        os.dup2(self._saved, 1)
-        os.close(self._saved)
-        return False
-
-
 def cpu_baseline(n=33, cap_s=25.0, threads=None):
+    """The CPU baseline in a child process (it uses no GPU): the reference
+    prints progress with C stdio, which must not reach this process's
+    stdout, where the JSON line goes.  The child prints its result as the
+    last line of its stdout."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", "--n", str(n), "--cap", str(cap_s)]
+    if threads:
+        cmd += ["--threads", str(threads)]
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=sys.stderr, timeout=cap_s * 4 + 120, cwd=ROOT)
+    lines = [l for l in p.stdout.decode(errors="replace").splitlines() if l.startswith("{")]
+    if p.returncode != 0 or not lines:
+        raise RuntimeError(f"cpu baseline child failed (rc {p.returncode})")
+    r = json.loads(lines[-1])
+    return r or None
+
+
+def _cpu_baseline_child(n, cap_s, threads):
     """Runs _cpu_baseline_once with all host threads; the reference's
     Gauss-Seidel sweeps race between OpenMP threads (move3mg.c:3089) and on
     small grids many threads can diverge, so the thread count is halved
     until the iteration stays finite."""
     import math
     threads = threads or min(os.cpu_count() or 1, 16)
-    with _stdout_to_stderr():
-        while True:
-            r = _cpu_baseline_once(n, cap_s, threads)
-            if r is None or math.isfinite(r["rel_reached"]) or threads == 1:
-                return r
-            threads = max(1, threads // 2)
+    while True:
+        r = _cpu_baseline_once(n, cap_s, threads)
+        if r is None or math.isfinite(r["rel_reached"]) or threads == 1:
+            break
+        threads = max(1, threads // 2)
+    sys.stdout.flush()
+    print(json.dumps(r), flush=True)
 
 
 def _cpu_baseline_once(n, cap_s, threads):
@@ -210,10 +207,16 @@ def main():
                     help="copy 1 GiB three times with a plain device copy (HBM counter calibration)")
     ap.add_argument("--profile-vcycles", type=int, default=0,
                     help="profiling run: one mixed-precision solve capped at this many V-cycles, no JSON metric")
+    ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cap", type=float, default=25.0, help=argparse.SUPPRESS)
+    ap.add_argument("--threads", type=int, default=0, help=argparse.SUPPRESS)
     a = ap.parse_args()
+    if a.cpu_baseline_child:
+        _cpu_baseline_child(a.n, a.cap, a.threads or None)
+        return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
-                                   "(z-slab decomposition, see DESIGN.md)", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this build measures one GPU; sharding over several GPUs is design-only "
+                                   "(z-slab decomposition)", "n_gpus": a.gpus}))
         sys.exit(2)
 
     import numpy as np

@@ -274,9 +274,9 @@ struct i3g_ctx {
 	hipGraph_t cg_graph[2] = {nullptr, nullptr};
 	std::vector<double> cg_key[2];
 	int use_graph = 1;
-	/* XCD-contiguous plane slabs for the level-0 smoother (I3G_XCD_REMAP=0
-	 * switches it off) */
-	int xcd_remap = 1;
+	/* XCD-contiguous plane slabs for the smoother on every level of 64
+	 * planes or more (I3G_XCD_REMAP=1: level 0 only, =0 off) */
+	int xcd_remap = 2;
 	/* viscosity averaging of the coarse (V-cycle only) levels:
 	 * -1 the reference's viscmod, or 0 arithmetic, 1 geometric, 2
 	 * harmonic (I3G_COARSE_VISC selects) */
@@ -721,7 +721,7 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 	c->nlev = c->h.nlev;
 	{
 		const char *e = getenv("I3G_XCD_REMAP");
-		if (e && e[0] == '0') c->xcd_remap = 0;
+		if (e && e[0] >= '0' && e[0] <= '2') c->xcd_remap = e[0] - '0';
 		e = getenv("I3G_SMOOTH_ILP");
 		if (e && e[0] == '0') c->smooth_ilp = 0;
 		e = getenv("I3G_COARSE_VISC");
@@ -900,7 +900,7 @@ static void sweep(i3g_ctx *c, int l, T *x, const T *b, const i3g_stokes_params *
 	Level &L = c->L[l];
 	LevelView<T> v = view<T>(L);
 	dim3 hg = half_grid(L.nx, L.ny, L.nz);
-	int chunks = (int)hg.x, remap = c->xcd_remap && l == 0;
+	int chunks = (int)hg.x, remap = c->xcd_remap && (l == 0 || (c->xcd_remap == 2 && L.nz >= 64));
 	if (remap) hg = dim3((unsigned)((((long)chunks * L.nz + 7) / 8) * 8));
 	double by = (double)L.N * (sizeof(T) * 2 + 1 + sizeof(T) * 4 + sizeof(T) * 3) / 2;
 	for (int comp = 1; comp <= 3; comp++)
@@ -1539,8 +1539,10 @@ extern "C" void i3g_stokes_default_params(i3g_stokes_params *p)
 	 * sweeps on the second level cost 2% more V-cycles for 7% cheaper
 	 * V-cycles.  The coarsest level (smoothed from LDS in one work-group)
 	 * gets 128 sweeps: 457 instead of 477 V-cycles for 0.5 ms more per
-	 * V-cycle; 256 gain nothing further. */
-	int pre[4] = {16, 8, 16, 32}, post[4] = {16, 8, 16, 32};
+	 * V-cycle; 256 gain nothing further.  8 instead of 16 sweeps on the
+	 * third level kept the count (447) for 2.4% less time; 16 instead of
+	 * 32 on the fourth and fifth cost 16% more V-cycles. */
+	int pre[4] = {16, 8, 8, 32}, post[4] = {16, 8, 8, 32};
 	for (int l = 0; l < I3G_MAXLEV; l++) {
 		p->nu_pre[l] = l < 4 ? pre[l] : 32;
 		p->nu_post[l] = l < 4 ? post[l] : 32;

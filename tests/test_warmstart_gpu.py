@@ -30,10 +30,14 @@ def test_learned_guess_never_worse_and_gain(gpu):
         S.set_coefficients()
         z = np.zeros(n ** 3)
         S.set_solution(z, z, z, z)
-        st0 = S.solve(precision=32, rtol=1e-8)
+        # 1e-9: two solves from different starts that each reach 1e-8
+        # differ by up to ~1e-6 in the smallest velocity component (its
+        # conditioning), so both go one decade further before the fields
+        # are compared at the 1e-6 bar
+        st0 = S.solve(precision=32, rtol=1e-9)
         ref = S.get_solution()
         info = W.learned_start(S, cfg, ws)
-        st1 = S.solve(precision=32, rtol=1e-8)
+        st1 = S.solve(precision=32, rtol=1e-9)
         sol = S.get_solution()
         print(f"seed {seed}: zero {st0.vcycles} V-cycles, learned start ({info['start']}, r0 "
               f"{info['rel_learned']:.2e} vs {info['rel_fallback']:.2e}, lithostatic r0 {info['rel_lithostatic']:.2e}, "
