@@ -207,6 +207,8 @@ def main():
                     help="copy 1 GiB three times with a plain device copy (HBM counter calibration)")
     ap.add_argument("--profile-vcycles", type=int, default=0,
                     help="profiling run: one mixed-precision solve capped at this many V-cycles, no JSON metric")
+    ap.add_argument("--other-paths-only", action="store_true",
+                    help="profiling run: only the marker and heat measurements, printed as JSON, no metric")
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cap", type=float, default=25.0, help=argparse.SUPPRESS)
     ap.add_argument("--threads", type=int, default=0, help=argparse.SUPPRESS)
@@ -236,6 +238,9 @@ def main():
     t = time.time()
     cfg = I.make_config(2, n, n, n, seed=2)
     print(f"# config built in {time.time() - t:.1f}s multinum={cfg['multinum']}", file=sys.stderr, flush=True)
+    if a.other_paths_only:
+        print(json.dumps({"other_paths": other_paths(I, np, cfg)}))
+        return
     S = I.Stokes(cfg)
     S.set_coefficients()
     z = np.zeros(n ** 3)

@@ -50,10 +50,8 @@ struct i3g_markers {
 	int *cnt = nullptr, *start = nullptr, *cur = nullptr;
 	int *bsum = nullptr;     /* scan block sums */
 	int sorted = 0;
-	/* per-marker properties for the node transfer */
-	double *pnu = nullptr, *pro = nullptr, *pcp = nullptr, *pkt = nullptr, *pht = nullptr, *pbb = nullptr;
-	double *ptk = nullptr;
-	uint8_t *pin = nullptr;
+	/* per-marker records for the node transfer (sorted order) */
+	struct MkRec *prec = nullptr;
 	/* node-field staging */
 	double *pr = nullptr;
 	float *fld = nullptr;    /* 11 x N float in/out fields */
@@ -71,7 +69,8 @@ struct i3g_markers {
 	/* marker temperature update: node fields tk, tk2, kt, ro, cp (5 x N),
 	 * per-marker diffusion change and flag, averaged nodal change */
 	float *tnod = nullptr;
-	double *dtk = nullptr, *sn = nullptr;
+	double *sn = nullptr;
+	struct TkRec *trec = nullptr; /* per-marker records for the nodal average (sorted order) */
 	uint8_t *tin = nullptr;
 	/* device time of the last operation's kernels (host copies excluded) */
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -286,12 +285,21 @@ __device__ double visc_ab(const i3g_rock &r, double mtk, double mpb, double nube
 	return nueff;
 }
 
+/* One marker's transfer record, written by k_marker_props in sorted order
+ * and read whole by k_marker_gather: 80 bytes on one or two cache lines
+ * per marker instead of one line per property array. */
+struct __align__(16) MkRec {
+	float x, y, z;
+	int in; /* participates in the transfer */
+	double nu, ro, cp, kt, ht, bb, tk;
+	double pad;
+};
+
 __global__ void k_marker_props(long n, const float *__restrict__ x, const float *__restrict__ y,
 			       const float *__restrict__ z, const float *__restrict__ tk, float *eta,
 			       const signed char *__restrict__ type, const int *c1, const int *c2, const int *c3,
 			       const double *pr, const double *gx, const double *gy, const double *gz, int nx, int ny,
-			       int nz, const i3g_rock *rocks, int nrock, i3g_mk_params prm, double *pnu, double *pro,
-			       double *pcp, double *pkt, double *pht, double *pbb, double *ptk, uint8_t *pin, int *bad)
+			       int nz, const i3g_rock *rocks, int nrock, i3g_mk_params prm, MkRec *rec, int *bad)
 {
 	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= n) return;
@@ -300,13 +308,14 @@ __global__ void k_marker_props(long n, const float *__restrict__ x, const float 
 	int mt = type[t];
 	/* marker participates (mark3mg.c:460-461) */
 	int in = fx > 0 && fy > 0 && fz > 0 && fx < xs && fy < ys && fz < zs && mt < 50 && tk[t] > 0 && mt < nrock;
-	pin[t] = (uint8_t)in;
+	MkRec *R = rec + t;
+	R->x = fx; R->y = fy; R->z = fz; R->in = in;
 	if (!in) return;
 	/* types with air/water, eclogite or melt branches in the reference
 	 * (mark3mg.c:463-474, 530-641, 660-686) are not handled here */
 	if (mt < 2 || mt == 7 || mt == 8 || (mt >= 16 && mt <= 18) || mt > 20) {
 		atomicOr(bad, 1);
-		pin[t] = 0;
+		R->in = 0;
 		return;
 	}
 	const i3g_rock r = rocks[mt];
@@ -330,12 +339,76 @@ __global__ void k_marker_props(long n, const float *__restrict__ x, const float 
 	/* viscosity caps of mark3mg.c:790-793 */
 	if ((double)fy > prm.vdeep && mnu > prm.nudeep) mnu = prm.nudeep;
 	if (fy < 5e+4 && fx > xs - 1e+4 && fz > 1.5e+5 && fz < 8.7e+5 && mnu > 1e+19) mnu = 1e+19;
-	pnu[t] = mnu; pro[t] = mro; pcp[t] = mcp; pkt[t] = mkt; pht[t] = mht; pbb[t] = mbb; ptk[t] = mtk;
+	R->nu = mnu; R->ro = mro; R->cp = mcp; R->kt = mkt; R->ht = mht; R->bb = mbb; R->tk = mtk;
 }
 
 /* ------------------------------------------------------------------ */
 /* Marker -> node gather                                              */
 /* ------------------------------------------------------------------ */
+
+/* The markers of the (up to) eight cells around node (m1, m2, m3) in
+ * increasing marker id: the cells' id-sorted runs merged.  Slot
+ * q = 4*dc + 2*da + db is cell (m1-1+da, m2-1+db, m3-1+dc); a cell outside
+ * the grid is an empty run.  The run heads' ids stay in registers and every
+ * slot index is static, so one step issues a single id load (the advanced
+ * run's next head), independent of the chosen marker's own loads. */
+struct CellMerge {
+	int ps[8], pe[8], hid[8];
+	__device__ __forceinline__ void init(int nx, int ny, int nz, int m1, int m2, int m3, const int *start,
+					     const int *cnt, const int *id)
+	{
+#pragma unroll
+		for (int q = 0; q < 8; q++) {
+			int a = m1 - 1 + ((q >> 1) & 1), b = m2 - 1 + (q & 1), c = m3 - 1 + (q >> 2);
+			bool ok = a >= 0 && b >= 0 && c >= 0 && a <= nx - 2 && b <= ny - 2 && c <= nz - 2;
+			long k = ok ? (long)c * nx * ny + (long)a * ny + b : 0;
+			int s0 = ok ? start[k] : 0;
+			ps[q] = s0;
+			pe[q] = ok ? s0 + cnt[k] : 0;
+		}
+#pragma unroll
+		for (int q = 0; q < 8; q++) hid[q] = ps[q] < pe[q] ? id[ps[q]] : 0x7fffffff;
+	}
+	/* next marker (sorted position mk) and its slot, and the load of that
+	 * run's next head id into nid; false when every run is done.
+	 * commit(slot, nid) after the marker's work applies the new head, so
+	 * the id load overlaps the marker's own loads. */
+	__device__ __forceinline__ bool pop(const int *id, int &mk, int &slot, int &nid)
+	{
+		int best = 0, bid = hid[0];
+#pragma unroll
+		for (int q = 1; q < 8; q++)
+			if (hid[q] < bid) { bid = hid[q]; best = q; }
+		if (bid == 0x7fffffff) return false;
+		int p = ps[0], e = pe[0];
+#pragma unroll
+		for (int q = 1; q < 8; q++) {
+			p = q == best ? ps[q] : p;
+			e = q == best ? pe[q] : e;
+		}
+		nid = p + 1 < e ? id[p + 1] : 0x7fffffff;
+		/* unconditional selects: every store keeps a static index, so the
+		 * arrays stay in registers */
+#pragma unroll
+		for (int q = 0; q < 8; q++) ps[q] = q == best ? p + 1 : ps[q];
+		mk = p;
+		slot = best;
+		return true;
+	}
+	__device__ __forceinline__ void commit(int slot, int nid)
+	{
+#pragma unroll
+		for (int q = 0; q < 8; q++) hid[q] = q == slot ? nid : hid[q];
+	}
+};
+
+/* widths of the two cells on either side of node index m along one axis
+ * (zero where the cell is outside the grid: its slots are empty runs) */
+__device__ __forceinline__ void side_widths(const double *g, int m, int n, double w[2])
+{
+	w[0] = m > 0 ? g[m] - g[m - 1] : 0.0;
+	w[1] = m < n - 1 ? g[m + 1] - g[m] : 0.0;
+}
 
 /* Per node: merge the markers of the (up to) 8 cells around it by id and
  * accumulate the reference's weights (mark3mg.c:818-933) in that order;
@@ -343,10 +416,8 @@ __global__ void k_marker_props(long n, const float *__restrict__ x, const float 
  * 3 kt, 4 ht, 5 et, 6 tk, 7 nxx, 8 nxy, 9 nxz, 10 nyz. */
 __global__ void __launch_bounds__(256)
 k_marker_gather(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz,
-		const int *start, const int *cnt, const int *__restrict__ id, const float *__restrict__ x,
-		const float *__restrict__ y, const float *__restrict__ z, const uint8_t *pin, const double *pnu,
-		const double *pro, const double *pcp, const double *pkt, const double *pht, const double *pbb,
-		const double *ptk, double nukoef, double nubeg, double nuend, int viscmod, float *fld)
+		const int *start, const int *cnt, const int *__restrict__ id, const MkRec *__restrict__ rec,
+		double nukoef, double nubeg, double nuend, int viscmod, float *fld)
 {
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	int m3 = blockIdx.y;
@@ -354,39 +425,36 @@ k_marker_gather(int nx, int ny, int nz, const double *gx, const double *gy, cons
 	if (m1 >= nx) return;
 	long N = (long)nx * ny * nz;
 	long nd = (long)m3 * nx * ny + (long)m1 * ny + m2;
-	int ps[8], pe[8], ca[8], cb[8], cc[8];
-	int ncell = 0;
-	for (int c = m3 - 1; c <= m3; c++)
-		for (int a = m1 - 1; a <= m1; a++)
-			for (int b = m2 - 1; b <= m2; b++) {
-				if (a < 0 || b < 0 || c < 0 || a > nx - 2 || b > ny - 2 || c > nz - 2) continue;
-				long k = (long)c * nx * ny + (long)a * ny + b;
-				ps[ncell] = start[k];
-				pe[ncell] = start[k] + cnt[k];
-				ca[ncell] = a; cb[ncell] = b; cc[ncell] = c;
-				ncell++;
-			}
+	CellMerge cm;
+	cm.init(nx, ny, nz, m1, m2, m3, start, cnt, id);
+	double wx[2], wy[2], wz[2];
+	side_widths(gx, m1, nx, wx);
+	side_widths(gy, m2, ny, wy);
+	side_widths(gz, m3, nz, wz);
+	double gxm = gx[m1], gym = gy[m2], gzm = gz[m3];
 	float anu = 0, aro = 0, acp = 0, akt = 0, aht = 0, aet = 0, atk = 0, axx = 0, axy = 0, axz = 0, ayz = 0;
 	double w1 = 0, w0 = 0, wxx = 0, wxy = 0, wxz = 0, wyz = 0;
-	while (true) {
-		int best = -1, bid = 0x7fffffff;
-		for (int q = 0; q < ncell; q++)
-			if (ps[q] < pe[q]) {
-				int v = id[ps[q]];
-				if (v < bid) { bid = v; best = q; }
-			}
-		if (best < 0) break;
-		int mk = ps[best]++;
-		if (!pin[mk]) continue;
-		int a = ca[best], b = cb[best], c = cc[best];
-		double dx = gx[a + 1] - gx[a], dy = gy[b + 1] - gy[b], dz = gz[c + 1] - gz[c];
-		double ddz = 1.0 - fabs((gz[m3] - (double)z[mk]) / dz);
-		double ddx = 1.0 - fabs((gx[m1] - (double)x[mk]) / dx);
-		double ddy = 1.0 - fabs((gy[m2] - (double)y[mk]) / dy);
+	int mk, slot, nid;
+	while (cm.pop(id, mk, slot, nid)) {
+		/* every load of the step issued before one wait (the empty asm
+		 * consumes them ahead of the branch on the participation flag) */
+		const MkRec r = rec[mk];
+		int in = r.in;
+		float xm = r.x, ym = r.y, zm = r.z;
+		double qnu = r.nu, qro = r.ro, qcp = r.cp, qkt = r.kt, qht = r.ht, qbb = r.bb, qtk = r.tk;
+		asm volatile("" ::"v"(in), "v"(xm), "v"(ym), "v"(zm), "v"(qnu), "v"(qro), "v"(qcp), "v"(qkt), "v"(qht),
+			     "v"(qbb), "v"(qtk));
+		if (in) {
+		int da = (slot >> 1) & 1, db = slot & 1, dc = slot >> 2;
+		int a = m1 - 1 + da, b = m2 - 1 + db, c = m3 - 1 + dc;
+		double dx = wx[da], dy = wy[db], dz = wz[dc];
+		double ddz = 1.0 - fabs((gzm - (double)zm) / dz);
+		double ddx = 1.0 - fabs((gxm - (double)xm) / dx);
+		double ddy = 1.0 - fabs((gym - (double)ym) / dy);
 		double swt = ddx * ddy * ddz;
 		/* viscmod 0/1/2: arithmetic, geometric (log) or harmonic (inverse)
 		 * averaging of the marker viscosity (mark3mg.c:848-900) */
-		double mnu = viscmod == 1 ? log(pnu[mk]) : viscmod == 2 ? 1.0 / pnu[mk] : pnu[mk];
+		double mnu = viscmod == 1 ? log(qnu) : viscmod == 2 ? 1.0 / qnu : qnu;
 		if (ddx > nukoef && ddy > nukoef && ddz > nukoef) {
 			anu = (float)((double)anu + mnu * swt);
 			w1 += swt;
@@ -412,15 +480,16 @@ k_marker_gather(int nx, int ny, int nz, const double *gx, const double *gy, cons
 			wyz += s1;
 		}
 		if (swt > 0) {
-			double mro = pro[mk];
-			aro = (float)((double)aro + mro * swt);
-			acp = (float)((double)acp + pcp[mk] * mro * swt);
-			akt = (float)((double)akt + pkt[mk] * swt);
-			aht = (float)((double)aht + pht[mk] * swt);
-			aet = (float)((double)aet + pbb[mk] * swt);
-			atk = (float)((double)atk + ptk[mk] * swt);
+			aro = (float)((double)aro + qro * swt);
+			acp = (float)((double)acp + qcp * qro * swt);
+			akt = (float)((double)akt + qkt * swt);
+			aht = (float)((double)aht + qht * swt);
+			aet = (float)((double)aet + qbb * swt);
+			atk = (float)((double)atk + qtk * swt);
 			w0 += swt;
 		}
+		}
+		cm.commit(slot, nid);
 	}
 	float *f = fld;
 	/* the mean back from the log / inverse (mark3mg.c:946-981) */
@@ -496,13 +565,20 @@ __device__ __forceinline__ void tk_weights(const double *gx, const double *gy, c
  * take the new grid temperature; markers of type < 50 strictly inside the
  * grid are initialised when new, relaxed by the subgrid diffusion and given
  * the interpolated temperature change.  tin flags the markers whose
- * diffusion change dtk enters the nodal average. */
+ * diffusion change enters the nodal average; the record carries the flag,
+ * the position and the change for k_tk_gather. */
+struct __align__(16) TkRec {
+	float x, y, z;
+	int in;
+	double dtk, pad;
+};
+
 __global__ void k_tk_marker(long n, const float *__restrict__ x, const float *__restrict__ y,
 			    const float *__restrict__ z, const signed char *__restrict__ type, float *markk,
 			    const double *gx, const double *gy, const double *gz, int nx, int ny, int nz,
 			    const float *__restrict__ tk, const float *__restrict__ tk2, const float *__restrict__ kt,
 			    const float *__restrict__ ro, const float *__restrict__ cp, double heatdif, double dt,
-			    double zdeep, double tdeep, double *dtk, uint8_t *tin)
+			    double zdeep, double tdeep, TkRec *rec, uint8_t *tin)
 {
 	extern __shared__ double sg[];
 	stage_lines(sg, gx, gy, gz, nx, ny, nz);
@@ -512,6 +588,8 @@ __global__ void k_tk_marker(long n, const float *__restrict__ x, const float *__
 	tin[i] = 0;
 	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
 	float fx = x[i], fy = y[i], fz = z[i];
+	TkRec *R = rec + i;
+	R->x = fx; R->y = fy; R->z = fz; R->in = 0;
 	int mt = type[i];
 	if (!(fx > 0 && fy > 0 && fz > 0 && (double)fx < xs && (double)fy < ys && (double)fz < zs)) return;
 	if (mt >= 100) return;
@@ -549,7 +627,8 @@ __global__ void k_tk_marker(long n, const float *__restrict__ x, const float *__
 		if (d1 < -150.0) d1 = -150.0;
 		d1 = dd * (1.0 - exp(d1));
 		mk += (float)d1;
-		dtk[i] = d1;
+		R->dtk = d1;
+		R->in = 1;
 		tin[i] = 1;
 	}
 	mk += (float)(e2 - e3);
@@ -561,45 +640,38 @@ __global__ void k_tk_marker(long n, const float *__restrict__ x, const float *__
  * reference's serial scatter (heat3mg.c:687-699, 711-728). */
 __global__ void __launch_bounds__(256)
 k_tk_gather(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz, const int *start,
-	    const int *cnt, const int *__restrict__ id, const float *__restrict__ x, const float *__restrict__ y,
-	    const float *__restrict__ z, const uint8_t *__restrict__ tin, const double *__restrict__ dtk, double *sn)
+	    const int *cnt, const int *__restrict__ id, const TkRec *__restrict__ rec, double *sn)
 {
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	int m3 = blockIdx.y;
 	int m1 = t / ny, m2 = t - m1 * ny;
 	if (m1 >= nx) return;
 	long nd = (long)m3 * nx * ny + (long)m1 * ny + m2;
-	int ps[8], pe[8], ca[8], cb[8], cc[8];
-	int ncell = 0;
-	for (int c = m3 - 1; c <= m3; c++)
-		for (int a = m1 - 1; a <= m1; a++)
-			for (int b = m2 - 1; b <= m2; b++) {
-				if (a < 0 || b < 0 || c < 0 || a > nx - 2 || b > ny - 2 || c > nz - 2) continue;
-				long k = (long)c * nx * ny + (long)a * ny + b;
-				ps[ncell] = start[k];
-				pe[ncell] = start[k] + cnt[k];
-				ca[ncell] = a; cb[ncell] = b; cc[ncell] = c;
-				ncell++;
-			}
+	CellMerge cm;
+	cm.init(nx, ny, nz, m1, m2, m3, start, cnt, id);
+	double wx[2], wy[2], wz[2];
+	side_widths(gx, m1, nx, wx);
+	side_widths(gy, m2, ny, wy);
+	side_widths(gz, m3, nz, wz);
+	double gxm = gx[m1], gym = gy[m2], gzm = gz[m3];
 	double s0 = 0, s1 = 0;
-	while (true) {
-		int best = -1, bid = 0x7fffffff;
-		for (int q = 0; q < ncell; q++)
-			if (ps[q] < pe[q]) {
-				int v = id[ps[q]];
-				if (v < bid) { bid = v; best = q; }
-			}
-		if (best < 0) break;
-		int mk = ps[best]++;
-		if (!tin[mk]) continue;
-		int a = ca[best], b = cb[best], c = cc[best];
-		double dx = gx[a + 1] - gx[a], dy = gy[b + 1] - gy[b], dz = gz[c + 1] - gz[c];
-		double ddz = 1.0 - fabs((gz[m3] - (double)z[mk]) / dz);
-		double ddx = 1.0 - fabs((gx[m1] - (double)x[mk]) / dx);
-		double ddy = 1.0 - fabs((gy[m2] - (double)y[mk]) / dy);
-		double w = ddx * ddy * ddz;
-		s1 += dtk[mk] * w;
-		s0 += w;
+	int mk, slot, nid;
+	while (cm.pop(id, mk, slot, nid)) {
+		const TkRec r = rec[mk];
+		int in = r.in;
+		float xm = r.x, ym = r.y, zm = r.z;
+		double qtk = r.dtk;
+		asm volatile("" ::"v"(in), "v"(xm), "v"(ym), "v"(zm), "v"(qtk));
+		if (in) {
+			double dx = wx[(slot >> 1) & 1], dy = wy[slot & 1], dz = wz[slot >> 2];
+			double ddz = 1.0 - fabs((gzm - (double)zm) / dz);
+			double ddx = 1.0 - fabs((gxm - (double)xm) / dx);
+			double ddy = 1.0 - fabs((gym - (double)ym) / dy);
+			double w = ddx * ddy * ddz;
+			s1 += qtk * w;
+			s0 += w;
+		}
+		cm.commit(slot, nid);
 	}
 	if (s0 != 0) s1 /= s0;
 	sn[nd] = s1;
@@ -894,8 +966,8 @@ extern "C" int i3g_markers_destroy(i3g_markers *m)
 	hipDeviceSynchronize();
 	void *ps[] = {m->gx, m->gy, m->gz, m->x, m->y, m->z, m->tk, m->eta, m->type, m->id, m->x2, m->y2, m->z2,
 		      m->tk2, m->eta2, m->type2, m->id2, m->key, m->c1, m->c2, m->c3, m->perm, m->cnt, m->start,
-		      m->cur, m->bsum, m->pnu, m->pro, m->pcp, m->pkt, m->pht, m->pbb, m->ptk, m->pin, m->pr, m->fld,
-		      m->vel, m->rocks, m->bad, m->tbc_p, m->tbc_c, m->tbc_k, m->tk1, m->tnod, m->dtk, m->sn, m->tin,
+		      m->cur, m->bsum, m->prec, m->pr, m->fld,
+		      m->vel, m->rocks, m->bad, m->tbc_p, m->tbc_c, m->tbc_k, m->tk1, m->tnod, m->trec, m->sn, m->tin,
 		      m->e, m->e2, m->st12};
 	for (void *p : ps)
 		if (p) hipFree(p);
@@ -1043,11 +1115,9 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 		if (!host[q]) return I3G_EINVAL;
 	int err = 0;
 	long n = m->n, N = m->N;
-	if (!m->pnu) {
+	if (!m->prec) {
 		long c = m->cap;
-		m->pnu = mk_alloc<double>(c, &err); m->pro = mk_alloc<double>(c, &err); m->pcp = mk_alloc<double>(c, &err);
-		m->pkt = mk_alloc<double>(c, &err); m->pht = mk_alloc<double>(c, &err); m->pbb = mk_alloc<double>(c, &err);
-		m->ptk = mk_alloc<double>(c, &err); m->pin = mk_alloc<uint8_t>(c, &err);
+		m->prec = mk_alloc<MkRec>(c, &err);
 		m->pr = mk_alloc<double>(N, &err); m->fld = mk_alloc<float>(11 * N, &err);
 		if (err) return err;
 	}
@@ -1062,15 +1132,12 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 				   (const float *)m->z, (const float *)m->tk, m->eta, (const signed char *)m->type,
 				   (const int *)m->c1, (const int *)m->c2, (const int *)m->c3, (const double *)m->pr,
 				   (const double *)m->gx, (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz,
-				   (const i3g_rock *)m->rocks, nrock, *prm, m->pnu, m->pro, m->pcp, m->pkt, m->pht, m->pbb,
-				   m->ptk, m->pin, m->bad);
+				   (const i3g_rock *)m->rocks, nrock, *prm, m->prec, m->bad);
 	dim3 gg((unsigned)(((long)m->nx * m->ny + 255) / 256), (unsigned)m->nz);
 	hipLaunchKernelGGL(k_marker_gather, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz, (const double *)m->gx,
 			   (const double *)m->gy, (const double *)m->gz, (const int *)m->start, (const int *)m->cnt,
-			   (const int *)m->id, (const float *)m->x, (const float *)m->y, (const float *)m->z,
-			   (const uint8_t *)m->pin, (const double *)m->pnu, (const double *)m->pro, (const double *)m->pcp,
-			   (const double *)m->pkt, (const double *)m->pht, (const double *)m->pbb, (const double *)m->ptk,
-			   prm->nukoef, prm->nubeg, prm->nuend, prm->viscmod, m->fld);
+			   (const int *)m->id, (const MkRec *)m->prec, prm->nukoef, prm->nubeg, prm->nuend, prm->viscmod,
+			   m->fld);
 	if (m->has_tbc) {
 		MCHK(hipMemcpyAsync(m->tk1, m->fld + 6 * N, N * sizeof(float), hipMemcpyDeviceToDevice, m->s));
 		hipLaunchKernelGGL(k_tkbc, g1(N), dim3(256), 0, m->s, N, (const int *)m->tbc_p, (const float *)m->tbc_c,
@@ -1136,7 +1203,7 @@ extern "C" int i3g_markers_update_tk(i3g_markers *m, const float *tk, const floa
 	if (!m->tnod) {
 		m->tnod = mk_alloc<float>(5 * N, &err);
 		m->sn = mk_alloc<double>(N, &err);
-		m->dtk = mk_alloc<double>(m->cap, &err);
+		m->trec = mk_alloc<TkRec>(m->cap, &err);
 		m->tin = mk_alloc<uint8_t>(m->cap, &err);
 		if (err) return err;
 	}
@@ -1149,13 +1216,12 @@ extern "C" int i3g_markers_update_tk(i3g_markers *m, const float *tk, const floa
 		hipLaunchKernelGGL(k_tk_marker, g1(n), dim3(256), lines_bytes(m), m->s, n, (const float *)m->x, (const float *)m->y,
 				   (const float *)m->z, (const signed char *)m->type, m->tk, (const double *)m->gx,
 				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, f, f + N, f + 2 * N,
-				   f + 3 * N, f + 4 * N, heatdif, timestep, zdeep, tdeep, m->dtk, m->tin);
+				   f + 3 * N, f + 4 * N, heatdif, timestep, zdeep, tdeep, m->trec, m->tin);
 	if (n && heatdif != 0) {
 		dim3 gg((unsigned)(((long)m->nx * m->ny + 255) / 256), (unsigned)m->nz);
 		hipLaunchKernelGGL(k_tk_gather, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz, (const double *)m->gx,
 				   (const double *)m->gy, (const double *)m->gz, (const int *)m->start, (const int *)m->cnt,
-				   (const int *)m->id, (const float *)m->x, (const float *)m->y, (const float *)m->z,
-				   (const uint8_t *)m->tin, (const double *)m->dtk, m->sn);
+				   (const int *)m->id, (const TkRec *)m->trec, m->sn);
 		hipLaunchKernelGGL(k_tk_anti, g1(n), dim3(256), lines_bytes(m), m->s, n, (const float *)m->x, (const float *)m->y,
 				   (const float *)m->z, (const uint8_t *)m->tin, m->tk, (const double *)m->gx,
 				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, (const double *)m->sn);

@@ -1593,8 +1593,16 @@ static int solve_impl(i3g_ctx *c, const i3g_stokes_params *p, i3g_stats *st)
 		hipMemsetAsync(e, 0, n * sizeof(double), c->s);
 		if (p->precision == 64)
 			cycles += gcr<double, double>(c, G, r, e, tgt, p->max_vcycles, p, &red);
-		else
-			cycles += gcr<float, double>(c, G, r, e, std::max(p->inner_rtol, tgt), p->max_vcycles, p, &red);
+		else {
+			/* an FP32 pass reduces the scaled residual to about 1e-7;
+			 * when that reaches the final target with margin, aim
+			 * for it at once (a restart costs the kept directions'
+			 * head start: 487 instead of 437 V-cycles from a start
+			 * at 3.4e-2), else stop at inner_rtol and refine */
+			double fin = 0.8 * p->rtol / rel;
+			double itgt = fin >= 1e-7 ? fin : std::max(p->inner_rtol, tgt);
+			cycles += gcr<float, double>(c, G, r, e, itgt, p->max_vcycles, p, &red);
+		}
 		hipLaunchKernelGGL(k_add_scaled<double>, lg, dim3(256), 0, c->s, c->sol, (const double *)e, nrm[0], n);
 		bc_enforce<double>(c, L, c->sol, 0);
 		rc = resid64(c, c->sol, L.r64, nrm);
