@@ -164,7 +164,7 @@ def other_paths(I, np, cfg257):
     M.cell_index()
     res["cell_index"] = (M.last_ms(), m * 16.0)
     M.sort()
-    res["sort"] = (M.last_ms(), m * (8.0 + 2 * 21))
+    res["sort"] = (M.last_ms(), m * (8.0 + 2 * 29))
     rocks = [[1e21, 0, 0, 1e18, 1e25, 3300, 3e-5, 1e-3, 1000, 3, 0, 0, 1e-8]] * 2 + \
         [[2e16, 3e5, 4.0, 1e19, 1e24, 3300, 3e-5, 1e-3, 1000, 0.73, 1293, 4e-6, 2e-8]] * 5
     M.to_nodes(rocks, np.full(N, 1e8), np.zeros((11, N), np.float32), 1e18, 1e25, 0.0, 1, 0.0)

@@ -239,19 +239,53 @@ __global__ void k_heat_setup(int nx, int ny, int nz, const double *gx, const dou
 }
 
 /* One colour of the red-black relaxation of the interior rows:
- * T -= (A T - b) / diag * t0koef (gausmatt, heat3mg.c:933-940). */
-__global__ void k_heat_sweep(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz,
-			     const float *__restrict__ kt, const int *__restrict__ bp, const double *__restrict__ b,
-			     const double *__restrict__ diag, double *T, int colour, double t0koef)
+ * T -= (A T - b) / diag * t0koef (gausmatt, heat3mg.c:933-940).
+ * Threads cover only the colour's nodes (every other y index of a line);
+ * the boundary code is read first and then all 17 operands of the row are
+ * loaded before any is used (the scheduling barrier keeps them together);
+ * the arithmetic is conds() and the update term for term.  Work-group
+ * g of the 1-D grid (8 * ceil(chunks * nz / 8) groups, handed to the 8
+ * XCDs round-robin) takes item (g % 8) * per + g / 8, so every XCD sweeps a
+ * contiguous slab of planes and finds the neighbouring planes in its L2. */
+__global__ void __launch_bounds__(NB)
+k_heat_sweep(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz,
+	     const float *__restrict__ kt, const int *__restrict__ bp, const double *__restrict__ b,
+	     const double *__restrict__ diag, double *T, int colour, double t0koef, int chunks)
 {
-	NODE_IDX
-	if (((m1 + m2 + m3) & 1) != colour || bp[v3] != -2) return;
-	Cond c = conds(kt, gx, gy, gz, m1, m2, m3, v3, sx, sz);
-	double r = diag[v3] * T[v3] - b[v3];
-	r -= c.xm * T[v3 - sx]; r -= c.xp * T[v3 + sx];
-	r -= c.ym * T[v3 - 1];  r -= c.yp * T[v3 + 1];
-	r -= c.zm * T[v3 - sz]; r -= c.zp * T[v3 + sz];
-	T[v3] -= r / diag[v3] * t0koef;
+	int per = gridDim.x >> 3;
+	int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+	if (w >= chunks * nz) return;
+	int m3 = w / chunks, ch = w - m3 * chunks;
+	int t = ch * blockDim.x + threadIdx.x;
+	int hy = (ny + 1) >> 1;
+	int m1 = t / hy, tj = t - m1 * hy;
+	if (m1 >= nx) return;
+	int m2 = 2 * tj + ((m1 + m3 + colour) & 1);
+	if (m2 >= ny) return;
+	long sx = ny, sz = (long)nx * ny;
+	long v3 = m3 * sz + m1 * sx + m2;
+	if (bp[v3] != -2) return; /* boundary rows (every outer node) */
+	float k3 = kt[v3], kxm = kt[v3 - sx], kxp = kt[v3 + sx], kym = kt[v3 - 1], kyp = kt[v3 + 1];
+	float kzm = kt[v3 - sz], kzp = kt[v3 + sz];
+	double t0 = T[v3], txm = T[v3 - sx], txp = T[v3 + sx], tym = T[v3 - 1], typ = T[v3 + 1];
+	double tzm = T[v3 - sz], tzp = T[v3 + sz];
+	double dg = diag[v3], bb = b[v3];
+	double gx0 = gx[m1 - 1], gx1 = gx[m1], gx2 = gx[m1 + 1];
+	double gy0 = gy[m2 - 1], gy1 = gy[m2], gy2 = gy[m2 + 1];
+	double gz0 = gz[m3 - 1], gz1 = gz[m3], gz2 = gz[m3 + 1];
+	__builtin_amdgcn_sched_barrier(0);
+	double xkf = 2.0 / (gx2 - gx0), ykf = 2.0 / (gy2 - gy0), zkf = 2.0 / (gz2 - gz0);
+	double cxm = (double)(kxm + k3) / 2.0 * xkf / (gx1 - gx0);
+	double cxp = (double)(k3 + kxp) / 2.0 * xkf / (gx2 - gx1);
+	double cym = (double)(kym + k3) / 2.0 * ykf / (gy1 - gy0);
+	double cyp = (double)(k3 + kyp) / 2.0 * ykf / (gy2 - gy1);
+	double czm = (double)(kzm + k3) / 2.0 * zkf / (gz1 - gz0);
+	double czp = (double)(k3 + kzp) / 2.0 * zkf / (gz2 - gz1);
+	double r = dg * t0 - bb;
+	r -= cxm * txm; r -= cxp * txp;
+	r -= cym * tym; r -= cyp * typ;
+	r -= czm * tzm; r -= czp * tzp;
+	T[v3] = t0 - r / dg * t0koef;
 }
 
 /* Boundary rows of one dependency depth: T = c + k T[partner]. */
@@ -531,6 +565,9 @@ extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const flo
 	bool terms = h->adiabyn || h->frictyn;
 	dim3 gg((unsigned)(((long)h->nx * h->ny + NB - 1) / NB), (unsigned)h->nz);
 	long nblk = (long)gg.x * gg.y;
+	/* sweep grid: one thread per node of a colour, XCD-contiguous planes */
+	int swchunks = (int)(((long)h->nx * ((h->ny + 1) / 2) + NB - 1) / NB);
+	dim3 gsw((unsigned)(8 * (((long)swchunks * h->nz + 7) / 8)));
 	double maxrate = 0;
 	if (terms) {
 		TCHK(hipMemcpyAsync(h->rocp, rocp.data(), N * sizeof(float), hipMemcpyHostToDevice, h->s));
@@ -588,11 +625,11 @@ extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const flo
 		for (;;) {
 			for (int k = 0; k < p->sweeps; k++) {
 				for (int col = 0; col < 2; col++) {
-					hipLaunchKernelGGL(k_heat_sweep, gg, dim3(NB), 0, h->s, h->nx, h->ny, h->nz,
+					hipLaunchKernelGGL(k_heat_sweep, gsw, dim3(NB), 0, h->s, h->nx, h->ny, h->nz,
 							   (const double *)h->gx, (const double *)h->gy,
 							   (const double *)h->gz, (const float *)h->kt, (const int *)h->bp,
 							   (const double *)h->b, (const double *)h->diag, h->T, col,
-							   p->t0koef);
+							   p->t0koef, swchunks);
 					bc_apply(h, p->t0koef);
 				}
 			}

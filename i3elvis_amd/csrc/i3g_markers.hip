@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "../../include/i3g.h"
@@ -207,10 +208,25 @@ __global__ void k_cellsort(long N, const int *start, const int *cnt, int *perm, 
 	}
 }
 
-template <typename T> __global__ void k_gather(long n, const int *perm, const T *src, T *dst)
+/* The sorted order applied to every per-marker array in one pass: the
+ * permutation is read once and all eight values of a marker are loaded
+ * before the first store. */
+struct MkArrays {
+	float *x, *y, *z, *tk, *eta, *e;
+	signed char *type;
+	int *id;
+};
+__global__ void k_permute(long n, const int *__restrict__ perm, MkArrays s, MkArrays d)
 {
 	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t < n) dst[t] = src[perm[t]];
+	if (t >= n) return;
+	int p = perm[t];
+	float x = s.x[p], y = s.y[p], z = s.z[p], tk = s.tk[p], eta = s.eta[p], e = s.e[p];
+	signed char ty = s.type[p];
+	int id = s.id[p];
+	d.x[t] = x; d.y[t] = y; d.z[t] = z; d.tk[t] = tk; d.eta[t] = eta; d.e[t] = e;
+	d.type[t] = ty;
+	d.id[t] = id;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1052,12 +1068,15 @@ extern "C" int i3g_markers_cell_index(i3g_markers *m, int *m1, int *m2, int *m3,
 	return I3G_OK;
 }
 
-template <typename T> static void permute(i3g_markers *m, T *&a, T *&b)
+/* gather every per-marker array into the second buffers, then swap */
+static void permute_all(i3g_markers *m)
 {
-	hipLaunchKernelGGL(k_gather<T>, g1(m->n), dim3(256), 0, m->s, m->n, (const int *)m->perm, (const T *)a, b);
-	T *t = a;
-	a = b;
-	b = t;
+	MkArrays s{m->x, m->y, m->z, m->tk, m->eta, m->e, m->type, m->id};
+	MkArrays d{m->x2, m->y2, m->z2, m->tk2, m->eta2, m->e2, m->type2, m->id2};
+	hipLaunchKernelGGL(k_permute, g1(m->n), dim3(256), 0, m->s, m->n, (const int *)m->perm, s, d);
+	std::swap(m->x, m->x2); std::swap(m->y, m->y2); std::swap(m->z, m->z2);
+	std::swap(m->tk, m->tk2); std::swap(m->eta, m->eta2); std::swap(m->e, m->e2);
+	std::swap(m->type, m->type2); std::swap(m->id, m->id2);
 }
 
 extern "C" int i3g_markers_sort(i3g_markers *m, long *order)
@@ -1077,14 +1096,7 @@ extern "C" int i3g_markers_sort(i3g_markers *m, long *order)
 				   m->perm);
 		hipLaunchKernelGGL(k_cellsort, g1(N), dim3(256), 0, m->s, N, (const int *)m->start, (const int *)m->cnt,
 				   m->perm, (const int *)m->id);
-		permute(m, m->x, m->x2);
-		permute(m, m->y, m->y2);
-		permute(m, m->z, m->z2);
-		permute(m, m->tk, m->tk2);
-		permute(m, m->eta, m->eta2);
-		permute(m, m->e, m->e2);
-		permute(m, m->type, m->type2);
-		permute(m, m->id, m->id2);
+		permute_all(m);
 		/* cell indices in the new order */
 		rc = cell_index(m, 0);
 		if (rc) return rc;
