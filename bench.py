@@ -46,22 +46,31 @@ def smoother_bytes_per_launch(N, word):
 
 def cpu_baseline(n=33, cap_s=25.0, threads=None):
     """The CPU baseline in a child process (it uses no GPU): the reference
-    prints progress with C stdio, which must not reach this process's
-    stdout, where the JSON line goes.  The child prints its result as the
-    last line of its stdout."""
+    prints progress with C stdio (partial lines, flushed whenever its buffer
+    fills), which must not reach this process's stdout, where the JSON line
+    goes.  The child's output goes to stderr and its result to a file."""
     import subprocess
-    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", "--n", str(n), "--cap", str(cap_s)]
-    if threads:
-        cmd += ["--threads", str(threads)]
-    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=sys.stderr, timeout=cap_s * 4 + 120, cwd=ROOT)
-    lines = [l for l in p.stdout.decode(errors="replace").splitlines() if l.startswith("{")]
-    if p.returncode != 0 or not lines:
+    import tempfile
+    fd, out = tempfile.mkstemp(suffix=".json")
+    os.close(fd)
+    try:
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", "--n", str(n), "--cap",
+               str(cap_s), "--result-file", out]
+        if threads:
+            cmd += ["--threads", str(threads)]
+        sys.stderr.flush()
+        p = subprocess.run(cmd, stdout=sys.stderr.fileno(), stderr=sys.stderr.fileno(), timeout=cap_s * 4 + 120,
+                           cwd=ROOT)
+        with open(out) as f:
+            text = f.read().strip()
+    finally:
+        os.unlink(out)
+    if p.returncode != 0 or not text:
         raise RuntimeError(f"cpu baseline child failed (rc {p.returncode})")
-    r = json.loads(lines[-1])
-    return r or None
+    return json.loads(text) or None
 
 
-def _cpu_baseline_child(n, cap_s, threads):
+def _cpu_baseline_child(n, cap_s, threads, result_file):
     """Runs _cpu_baseline_once with all host threads; the reference's
     Gauss-Seidel sweeps race between OpenMP threads (move3mg.c:3089) and on
     small grids many threads can diverge, so the thread count is halved
@@ -73,8 +82,8 @@ def _cpu_baseline_child(n, cap_s, threads):
         if r is None or math.isfinite(r["rel_reached"]) or threads == 1:
             break
         threads = max(1, threads // 2)
-    sys.stdout.flush()
-    print(json.dumps(r), flush=True)
+    with open(result_file, "w") as f:
+        f.write(json.dumps(r))
 
 
 def _cpu_baseline_once(n, cap_s, threads):
@@ -212,9 +221,10 @@ def main():
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cap", type=float, default=25.0, help=argparse.SUPPRESS)
     ap.add_argument("--threads", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--result-file", default="", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.cpu_baseline_child:
-        _cpu_baseline_child(a.n, a.cap, a.threads or None)
+        _cpu_baseline_child(a.n, a.cap, a.threads or None, a.result_file)
         return
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; sharding over several GPUs is design-only "

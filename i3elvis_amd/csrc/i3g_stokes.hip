@@ -1533,8 +1533,13 @@ extern "C" void i3g_stokes_default_params(i3g_stokes_params *p)
 	 * swapped to 16 and 8; the relaxation factors differ from mode.t3c
 	 * (p0koef 0.3, p1koef 0.1, v0koef 0.7) because the V-cycle here
 	 * preconditions a Krylov solver.  Measured on the 257^3 subduction case
-	 * (10^6 contrast): p0koef 0.6 / p1koef 1.0 / v0koef 0.85 (v0koef 1.0
-	 * stalls); finest-level sweeps 8, 12, 16, 20, 24, 32 took 1538, 687, 477,
+	 * (10^6 contrast) with the V-cycle below: v0koef 0.75 / 0.85 / 0.95 /
+	 * 1.0 / 1.05 took 597 / 447 / 406 / 406 / 419 V-cycles, p0koef
+	 * 0.5 / 0.6 / 0.7 477 / 447 / 426, v1koef 0.6 / 0.7 / 0.85 487 / 447 /
+	 * 448; v0koef 0.95, p0koef 0.7, v1koef 0.85 together 366 (its
+	 * neighbours 356-387), and on 129^3 21 instead of 28 (32-blob cube) and
+	 * 261 instead of 331 (subduction) against 0.85 / 0.6 / 0.7.
+	 * Finest-level sweeps 8, 12, 16, 20, 24, 32 took 1538, 687, 477,
 	 * 457, 447, 427 V-cycles, 16 being the fastest in time; 8 instead of 16
 	 * sweeps on the second level cost 2% more V-cycles for 7% cheaper
 	 * V-cycles.  The coarsest level (smoothed from LDS in one work-group)
@@ -1550,7 +1555,7 @@ extern "C" void i3g_stokes_default_params(i3g_stokes_params *p)
 	p->nu_coarse = 128;
 	p->inner_m = 10;
 	p->inner_tol = 0.1;
-	p->p0koef = 0.6; p->p1koef = 1.0; p->v0koef = 0.85; p->v1koef = 0.7;
+	p->p0koef = 0.7; p->p1koef = 1.0; p->v0koef = 0.95; p->v1koef = 0.85;
 	p->inner_rtol = 1e-6;
 	p->warm_start = 0;
 	p->pxinit = 2; p->pyinit = 1; p->pzinit = 2; p->pinit = 0;
