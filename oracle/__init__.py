@@ -1,9 +1,10 @@
 """Oracle — TEST INFRASTRUCTURE ONLY.
 
 A plain-C FP64 restatement of the reference Stokes operator, transfers and
-norms (``oracle/i3g_oracle.c``), loaded through ctypes.  Only ``tests/``,
-``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg may use it,
-and only as the checker: the product path never calls into this package.
+norms (``oracle/i3g_oracle.c``), loaded through ctypes.  Only ``tests/``
+and ``__graft_entry__.smoke()`` (to check results) and ``bench.py``'s
+CPU-baseline leg (the timed CPU side) may use it: the product path never
+calls into this package.
 
 Vectors use the reference's ``sol0`` layout (node-major, 4 unknowns per node:
 P, vx, vy, vz).  Boundary tables use the reference's global layout

@@ -2,8 +2,9 @@
  * i3g oracle — TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C, FP64, single-threaded restatement of the reference algorithm
- * for the Stokes hot path.  Only tests/, __graft_entry__.smoke() and
- * bench.py's CPU-baseline leg may load it, and only as the checker.
+ * for the Stokes hot path.  Only tests/ and __graft_entry__.smoke() (to
+ * check results) and bench.py's CPU-baseline leg (the timed CPU side) may
+ * load it.
  * Vectors use the reference's sol0 layout: unknown (node n, comp c) at
  * n*4+c with c = 0:P 1:vx 2:vy 3:vz; a level's node n is level-local.
  * Boundary tables are the reference's global layout (bondm over all levels,
