@@ -21,9 +21,24 @@ __device__ __forceinline__ int serch(const double *g, int n, double v)
 	return lo;
 }
 
-/* staggered trilinear velocity at a point, vxyzcalc1 (mark3mg.c:2347-2530) */
+/* The same index found by walking from a nearby start i (0 <= i <= n-2),
+ * for points close to the previous one: upward while g[i+1] is not above
+ * v, then downward while g[i] is above v.  For increasing lines this ends
+ * at the largest i <= n-2 with g[i] <= v (0 below g[0]) and, as the
+ * bisection, at n-2 for a NaN v. */
+__device__ __forceinline__ int serch_from(const double *g, int n, double v, int i)
+{
+	while (i < n - 2 && !(g[i + 1] > v)) i++;
+	while (i > 0 && g[i] > v) i--;
+	return i;
+}
+
+/* staggered trilinear velocity at a point, vxyzcalc1 (mark3mg.c:2347-2530);
+ * cell[3]: the node-line indices of the point, returned, and when
+ * from_hint is set also the start of the search (the previous point's) */
 __device__ inline void vxyz(const float *vx, const float *vy, const float *vz, const double *gx, const double *gy,
-		     const double *gz, int nx, int ny, int nz, double x, double y, double z, double *v)
+			    const double *gz, int nx, int ny, int nz, double x, double y, double z, double *v,
+			    int *cell = nullptr, bool from_hint = false)
 {
 #pragma clang fp contract(off)
 	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
@@ -33,7 +48,21 @@ __device__ inline void vxyz(const float *vx, const float *vy, const float *vz, c
 	if (y > ys) y = ys;
 	if (z < 0) z = 0;
 	if (z > zs) z = zs;
-	int m1 = serch(gx, nx, x), m2 = serch(gy, ny, y), m3 = serch(gz, nz, z);
+	int m1, m2, m3;
+	if (cell && from_hint) {
+		m1 = serch_from(gx, nx, x, cell[0]);
+		m2 = serch_from(gy, ny, y, cell[1]);
+		m3 = serch_from(gz, nz, z, cell[2]);
+	} else {
+		m1 = serch(gx, nx, x);
+		m2 = serch(gy, ny, y);
+		m3 = serch(gz, nz, z);
+	}
+	if (cell) {
+		cell[0] = m1;
+		cell[1] = m2;
+		cell[2] = m3;
+	}
 	long sx = ny, sz = (long)nx * ny;
 	for (int comp = 0; comp < 3; comp++) {
 		int a = m1, b = m2, c = m3;
