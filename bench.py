@@ -274,12 +274,12 @@ def main():
     learned = {}
 
     def run_learned(timing=False):
-        """residual-optimal learned start (network inference and the
-        combination on the device; the wall time of building it, its host
-        copies included, is added to the step), fallback selection, mixed
-        solve"""
+        """residual-optimal learned start built on the device from the
+        resident fields (features, network inference, lithostatic pressure,
+        combination, zero-start fallback; its wall time is added to the
+        step), mixed solve"""
         t = time.time()
-        info = W.learned_start(S, cfg, ws)
+        info = W.learned_start_device(S, ws)
         tg = time.time() - t
         S.set_timing(timing)
         st = S.solve(check_conv=True, precision=32, rtol=1e-8, krylov=a.krylov, max_outer=20,

@@ -220,6 +220,18 @@ class Stokes:
               "combine_start")
         return a[:len(dirs)], rel.value
 
+    def learned_start(self, net, pscale, vscale, gauge=(2, 1, 2)):
+        """i3g_stokes_learned_start: the learned start built on the device
+        (features, inference, lithostatic pressure, residual-optimal
+        combination, zero-start fallback).  Returns the info dict."""
+        info = np.zeros(6)
+        g = (C.c_long * 3)(*[int(v) for v in gauge])
+        check(lib().i3g_stokes_learned_start(self.ctx, net.h, C.c_double(pscale), C.c_double(vscale), g, _p(info)),
+              "stokes_learned_start")
+        return dict(coef_dP=float(info[0]), coef_v=float(info[1]), rel_learned=float(info[2]),
+                    rel_lithostatic=float(info[3]), rel_fallback=float(info[4]),
+                    start="network" if info[5] else "fallback")
+
     def level_nodes(self, level):
         c = self.cfg
         h = c.get("hier") or hierarchy(c["nx"], c["ny"], c["nz"], c["multinum"])

@@ -684,6 +684,19 @@ extern "C" int i3g_warmstart_infer(i3g_net *q, const float *in, float *out)
 	return I3G_OK;
 }
 
+extern "C" int i3g_warmstart_infer_device(i3g_net *q, long n, const float *d_in, float *d_out)
+{
+	if (!q || !d_in || !d_out) return I3G_EINVAL;
+	long P0 = vox(q->dims[0]);
+	if (n != P0) return I3G_EINVAL;
+	NCHK(hipMemcpyAsync(q->xin, d_in, P0 * 2 * sizeof(float), hipMemcpyDeviceToDevice, q->s));
+	int rc = forward(q);
+	if (rc) return rc;
+	NCHK(hipMemcpyAsync(d_out, q->y, P0 * 4 * sizeof(float), hipMemcpyDeviceToDevice, q->s));
+	NCHK(hipStreamSynchronize(q->s));
+	return I3G_OK;
+}
+
 extern "C" int i3g_warmstart_grad(i3g_net *q, const float *in, const float *target, double *loss, float *grad)
 {
 	if (!q || !in || !target) return I3G_EINVAL;

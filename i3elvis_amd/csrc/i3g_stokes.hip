@@ -1722,12 +1722,13 @@ extern "C" int i3g_stokes_prolong(i3g_ctx *c, int level, const double *const *co
 
 /* host planes (NULL = zero) into a device 4N vector, missing unknowns
  * zeroed, boundary rows applied (homog: partners only) */
-static int upload_planes(i3g_ctx *c, const double *const *pl, double *x, int homog)
+static int upload_planes(i3g_ctx *c, const double *const *pl, double *x, int homog,
+			 hipMemcpyKind kind = hipMemcpyHostToDevice)
 {
 	Level &L = c->L[0];
 	long N = L.N;
 	for (int q = 0; q < 4; q++) {
-		if (pl[q]) HCHK(hipMemcpyAsync(x + q * N, pl[q], N * sizeof(double), hipMemcpyHostToDevice, c->s));
+		if (pl[q]) HCHK(hipMemcpyAsync(x + q * N, pl[q], N * sizeof(double), kind, c->s));
 		else HCHK(hipMemsetAsync(x + q * N, 0, N * sizeof(double), c->s));
 	}
 	hipLaunchKernelGGL(k_zero_missing, lin_grid(N), dim3(256), 0, c->s, x, N, (const uint8_t *)L.flags);
@@ -1736,8 +1737,8 @@ static int upload_planes(i3g_ctx *c, const double *const *pl, double *x, int hom
 	return I3G_OK;
 }
 
-extern "C" int i3g_stokes_combine_start(i3g_ctx *c, int k, const double *const *planes, double *a_out,
-					double *rel_out)
+static int combine_core(i3g_ctx *c, int k, const double *const *planes, hipMemcpyKind kind, double *a_out,
+			double *rel_out)
 {
 	if (!c || !c->coef_ready || k < 0 || k > 4 || !planes) return I3G_EINVAL;
 	Level &L = c->L[0];
@@ -1750,7 +1751,7 @@ extern "C" int i3g_stokes_combine_start(i3g_ctx *c, int k, const double *const *
 		if (ys) hipFree(ys);
 		return err;
 	}
-	int rc = upload_planes(c, planes, c->sol, 0);
+	int rc = upload_planes(c, planes, c->sol, 0, kind);
 	double nrm[2] = {0, 1}, a[4] = {0, 0, 0, 0};
 	if (!rc) rc = resid64(c, c->sol, L.r64, nrm);
 	if (!rc) {
@@ -1760,7 +1761,7 @@ extern "C" int i3g_stokes_combine_start(i3g_ctx *c, int k, const double *const *
 				   (const double *)L.wP64, (const uint8_t *)L.flags, N, 0, 1.0);
 		for (int i = 0; i < k && !rc; i++) {
 			double *d = dirs + (long)i * n4;
-			rc = upload_planes(c, planes + 4 * (i + 1), d, 1);
+			rc = upload_planes(c, planes + 4 * (i + 1), d, 1, kind);
 			if (!rc)
 				hipLaunchKernelGGL((k_residual<double, double, 1>), plane_grid(L.nx, L.ny, L.nz), dim3(256), 0,
 						   c->s, view<double>(L), (const double *)d, (const double *)nullptr,
@@ -1823,6 +1824,132 @@ extern "C" int i3g_stokes_combine_start(i3g_ctx *c, int k, const double *const *
 	for (int i = 0; i < k; i++)
 		if (a_out) a_out[i] = a[i];
 	if (rel_out) *rel_out = nrm[0] / (nrm[1] > 0 ? nrm[1] : 1.0);
+	return I3G_OK;
+}
+
+extern "C" int i3g_stokes_combine_start(i3g_ctx *c, int k, const double *const *planes, double *a_out,
+					double *rel_out)
+{
+	return combine_core(c, k, planes, hipMemcpyHostToDevice, a_out, rel_out);
+}
+
+/* Learned start on the device.  Network features per node: (log10
+ * max(nu, 1) - 21) / 2 and (rho - 3300) / 50. */
+__global__ void k_ws_features(long N, const float *__restrict__ nu, const float *__restrict__ ro, float *f)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= N) return;
+	double v = (double)nu[t];
+	f[2 * t] = (float)((log10(v > 1.0 ? v : 1.0) - 21.0) / 2.0);
+	f[2 * t + 1] = (float)(((double)ro[t] - 3300.0) / 50.0);
+}
+
+/* Lithostatic pressure, one thread per (m1, m3) column: the integral of
+ * rho g_y down the column at the pressure points (cell centres, m1, m3 >= 1,
+ * the cell's four corner densities in the x-z plane averaged; zero for
+ * m2 < 2 and on the m1 = 0 and m3 = 0 columns). */
+__global__ void k_ws_lithostatic(int nx, int ny, int nz, const float *__restrict__ ro, const double *gy, double g,
+				 double *P)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= nx * nz) return;
+	int m3 = t / nx, m1 = t - m3 * nx;
+	long sx = ny, sz = (long)nx * ny;
+	long base = m3 * sz + m1 * sx;
+	double acc = 0;
+	P[base] = 0;
+	if (ny > 1) P[base + 1] = 0;
+	for (int m2 = 2; m2 < ny; m2++) {
+		double rc = 0;
+		if (m1 >= 1 && m3 >= 1) {
+			long q = base + m2 - 1;
+			rc = 0.25 * ((((double)ro[q] + (double)ro[q - sz]) + (double)ro[q - sx]) + (double)ro[q - sz - sx]);
+		}
+		double y1 = 0.5 * (gy[m2] + gy[m2 - 1]), y0 = m2 - 1 >= 1 ? 0.5 * (gy[m2 - 1] + gy[m2 - 2]) : 0.0;
+		acc = acc + g * rc * (y1 - y0);
+		P[base + m2] = acc;
+	}
+}
+
+__global__ void k_ws_shift(long N, double *P, const double *ref)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < N) P[t] -= *ref;
+}
+
+__global__ void k_ws_copy1(const double *src, double *dst) { *dst = *src; }
+
+/* scaled network outputs: plane q of out gets ch[q] * s[q] (channel
+ * ch = 3 for P, 0..2 for vx..vz) */
+__global__ void k_ws_outputs(long N, const float *__restrict__ y, double ps, double vs, double *dP, double *dv)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= N) return;
+	dP[t] = (double)y[4 * t + 3] * ps;
+	dv[t] = (double)y[4 * t] * vs;
+	dv[N + t] = (double)y[4 * t + 1] * vs;
+	dv[2 * N + t] = (double)y[4 * t + 2] * vs;
+}
+
+extern "C" int i3g_stokes_learned_start(i3g_ctx *c, i3g_net *net, double pscale, double vscale,
+					const long gauge[3], double *info)
+{
+	if (!c || !c->coef_ready || !net || !gauge || !info) return I3G_EINVAL;
+	Level &L = c->L[0];
+	long N = L.N;
+	long g3 = gauge[2] * L.nx * L.ny + gauge[0] * L.ny + gauge[1];
+	if (gauge[0] < 0 || gauge[0] >= L.nx || gauge[1] < 0 || gauge[1] >= L.ny || gauge[2] < 0 || gauge[2] >= L.nz)
+		return I3G_EINVAL;
+	int err = 0;
+	float *feat = dalloc<float>(2 * N, &err), *y = dalloc<float>(4 * N, &err);
+	double *lith = dalloc<double>(N + 1, &err), *dP = dalloc<double>(N, &err), *dv = dalloc<double>(3 * N, &err);
+	int rc = err;
+	double relf = 1, rel_lith = 1, rel = 1, a[2] = {0, 0};
+	int start = 0;
+	if (!rc) {
+		/* the zero start's residual: the fallback */
+		const double *zero[4] = {nullptr, nullptr, nullptr, nullptr};
+		rc = combine_core(c, 0, zero, hipMemcpyDeviceToDevice, nullptr, &relf);
+	}
+	if (!rc) {
+		dim3 lg = lin_grid(N);
+		hipLaunchKernelGGL(k_ws_features, lg, dim3(256), 0, c->s, N, (const float *)L.nu, (const float *)c->ro,
+				   feat);
+		hipLaunchKernelGGL(k_ws_lithostatic, dim3((unsigned)((L.nx * L.nz + 255) / 256)), dim3(256), 0, c->s,
+				   L.nx, L.ny, L.nz, (const float *)c->ro, (const double *)L.g[1], c->phys[1], lith);
+		hipLaunchKernelGGL(k_ws_copy1, dim3(1), dim3(1), 0, c->s, (const double *)(lith + g3), lith + N);
+		hipLaunchKernelGGL(k_ws_shift, lg, dim3(256), 0, c->s, N, lith, (const double *)(lith + N));
+		if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->s) != hipSuccess) rc = I3G_EDEVICE;
+	}
+	if (!rc) rc = i3g_warmstart_infer_device(net, N, feat, y);
+	if (!rc) {
+		hipLaunchKernelGGL(k_ws_outputs, lin_grid(N), dim3(256), 0, c->s, N, (const float *)y, pscale, vscale, dP,
+				   dv);
+		const double *base[4] = {lith, nullptr, nullptr, nullptr};
+		rc = combine_core(c, 0, base, hipMemcpyDeviceToDevice, nullptr, &rel_lith);
+	}
+	if (!rc) {
+		const double *pl[12] = {lith, nullptr, nullptr, nullptr, dP, nullptr, nullptr, nullptr,
+					nullptr, dv, dv + N, dv + 2 * N};
+		rc = combine_core(c, 2, pl, hipMemcpyDeviceToDevice, a, &rel);
+		start = 1;
+		if (!rc && relf < rel) {
+			const double *zero[4] = {nullptr, nullptr, nullptr, nullptr};
+			rc = combine_core(c, 0, zero, hipMemcpyDeviceToDevice, nullptr, nullptr);
+			start = 0;
+		}
+	}
+	hipStreamSynchronize(c->s);
+	void *ps[] = {feat, y, lith, dP, dv};
+	for (void *p : ps)
+		if (p) hipFree(p);
+	if (rc) return rc;
+	info[0] = a[0];
+	info[1] = a[1];
+	info[2] = rel;
+	info[3] = rel_lith;
+	info[4] = relf;
+	info[5] = start;
 	return I3G_OK;
 }
 

@@ -150,6 +150,13 @@ def learned_start(S, cfg, ws, fallback=None):
                 rel_fallback=relf, start=start)
 
 
+def learned_start_device(S, ws, gauge=(2, 1, 2)):
+    """learned_start with the zero fallback, built entirely on the device
+    from the context's own fields (i3g_stokes_learned_start): no host
+    arrays and no host-device copies of fields"""
+    return S.learned_start(ws.net, ws.pscale, ws.vscale, gauge)
+
+
 def timed(f, *a, **k):
     t = time.time()
     r = f(*a, **k)

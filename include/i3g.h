@@ -454,6 +454,24 @@ long i3g_warmstart_nparams(const i3g_net *net);
 int i3g_warmstart_get_params(i3g_net *net, float *params);
 int i3g_warmstart_set_params(i3g_net *net, const float *params);
 int i3g_warmstart_infer(i3g_net *net, const float *in, float *out);
+/* The same on device buffers (n = nodes of the network's grid, else
+ * I3G_EINVAL): d_in n x 2 features, d_out n x 4 outputs, node order.  No
+ * reference counterpart: it keeps the learned start of the Stokes solve
+ * on the device (i3g_stokes_learned_start). */
+int i3g_warmstart_infer_device(i3g_net *net, long n, const float *d_in, float *d_out);
+
+/* Learned start of the resident Stokes iterate, all on the device: network
+ * features from the context's level-0 viscosity and density, inference,
+ * the lithostatic pressure (rho g_y integrated down each column at the
+ * pressure points, zero at gauge = {m1, m2, m3}), then the residual-optimal
+ * combination x = lithostatic + a0 (network P * pscale) + a1 (network
+ * velocities * vscale), as i3g_stokes_combine_start; the zero start is
+ * taken instead when its residual is smaller.  The network must be on the
+ * context's level-0 grid.  info[6]: a0, a1, relative residual of the
+ * combination, of the lithostatic start alone, of the zero start, and 1 if
+ * the combination was taken (0: zero start).  No reference counterpart. */
+int i3g_stokes_learned_start(i3g_ctx *ctx, i3g_net *net, double pscale, double vscale, const long gauge[3],
+			     double *info);
 /* gradient of the mean squared error of one sample (grad may be NULL) */
 int i3g_warmstart_grad(i3g_net *net, const float *in, const float *target, double *loss, float *grad);
 /* one Adam step (beta 0.9/0.999, eps 1e-8) on the mean gradient of a batch */

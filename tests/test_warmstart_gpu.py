@@ -50,3 +50,36 @@ def test_learned_guess_never_worse_and_gain(gpu):
         ls.append(st1.vcycles)
         S.close()
     assert sum(ls) <= sum(zs)
+
+
+@pytest.mark.skipif(not os.path.isfile(WEIGHTS), reason="no trained weights")
+@pytest.mark.parametrize("cf", [1, 2])
+def test_device_learned_start_matches_host_pipeline(gpu, cf):
+    """i3g_stokes_learned_start (features, inference, lithostatic pressure
+    and the combination all on the device) against the host pipeline
+    (learned_start: numpy features and lithostatic pressure, host copies):
+    the same start up to the rounding of the features (rel L2 1e-6), the
+    same residuals and coefficients; then a solve from it converges."""
+    n = 65
+    ws = W.WarmStart.load(WEIGHTS, n=n)
+    cfg = I.make_config(cf, n, n, n, seed=2102)
+    S = I.Stokes(cfg)
+    S.set_coefficients()
+    print(f"case: device learned start cfg {cf} n {n}", flush=True)
+    host = W.learned_start(S, cfg, ws)
+    xh = S.get_solution()
+    dev = W.learned_start_device(S, ws)
+    xd = S.get_solution()
+    print(f"  host {host}\n  device {dev}", flush=True)
+    assert dev["start"] == host["start"]
+    for k in ("rel_learned", "rel_lithostatic", "rel_fallback"):
+        assert abs(dev[k] - host[k]) <= 1e-6 * abs(host[k]) + 1e-15, (k, dev[k], host[k])
+    for k in ("coef_dP", "coef_v"):
+        assert abs(dev[k] - host[k]) <= 1e-4 * max(abs(host[k]), 1e-3), (k, dev[k], host[k])
+    for c in range(4):
+        d = np.linalg.norm(xd[c] - xh[c]) / max(np.linalg.norm(xh[c]), 1e-300)
+        print(f"  field {c}: rel L2 diff {d:.3e}", flush=True)
+        assert d <= 1e-6, (c, d)
+    st = S.solve(precision=32, rtol=1e-8)
+    assert st.converged
+    S.close()
