@@ -258,6 +258,30 @@ class Stokes:
                                        C.c_double(p1koef)), "stokes_prolong")
         return out
 
+    def smooth_phase(self, level, phase, precision, x, b, v0koef, p0koef):
+        """one colour phase of the smoother on planes [P, vx, vy, vz] of
+        `level` (i3g_stokes_smooth_phase); returns the new planes"""
+        out = [np.array(a, np.float64, copy=True) for a in x]
+        keep = [np.ascontiguousarray(a, np.float64) for a in b]
+        xp = (C.c_void_p * 4)(*[a.ctypes.data for a in out])
+        bp = (C.c_void_p * 4)(*[a.ctypes.data for a in keep])
+        check(lib().i3g_stokes_smooth_phase(self.ctx, C.c_int(level), C.c_int(phase), C.c_int(precision), xp, bp,
+                                            C.c_double(v0koef), C.c_double(p0koef)), "stokes_smooth_phase")
+        return out
+
+    def smooth(self, level, precision, nsweeps, x, b, v0koef, p0koef, variant=0):
+        """nsweeps smoother sweeps (i3g_stokes_smooth; variant 0 the
+        context's default, 1 the seven phase launches, 2 one launch per
+        velocity component)"""
+        out = [np.array(a, np.float64, copy=True) for a in x]
+        keep = [np.ascontiguousarray(a, np.float64) for a in b]
+        xp = (C.c_void_p * 4)(*[a.ctypes.data for a in out])
+        bp = (C.c_void_p * 4)(*[a.ctypes.data for a in keep])
+        check(lib().i3g_stokes_smooth(self.ctx, C.c_int(level), C.c_int(precision), C.c_int(nsweeps),
+                                      C.c_int(variant), xp, bp, C.c_double(v0koef), C.c_double(p0koef)),
+              "stokes_smooth")
+        return out
+
     def strain_stress(self):
         """(eps, sig): 6 float planes each (xx yy zz xy xz yz) of the resident solution"""
         out = [np.zeros(self.N, np.float32) for _ in range(12)]

@@ -219,6 +219,11 @@ struct Level {
 	float *nu;               /* level viscosity at nodes */
 	double *x64, *b64, *r64;
 	float *x32, *b32, *r32;
+	/* second unknown vector of the out-of-place sweeps (k_sweep_comp) and
+	 * its tiling */
+	double *x64b;
+	float *x32b;
+	FusedGeom fg;
 	std::vector<double> hg[3];
 };
 
@@ -294,6 +299,15 @@ struct i3g_ctx {
 	/* BC rows of all depths in one launch (k_bc_chain); I3G_BC_CHAIN=0
 	 * selects one launch per depth */
 	int bc_chain = 1;
+	/* one launch per velocity component and sweep (k_sweep_comp, out of
+	 * place, pairs of sweeps) when I3G_FUSED_SWEEP=1; the default is the
+	 * seven phase launches, which measured faster at 257^3 (1.03 against
+	 * 1.28 ms a sweep: the component kernels' re-reads of the other
+	 * fields miss L2 and their per-tile plane chains hide latency worse).
+	 * Tile targets (x, y, z-slab) from I3G_FUSED_TILE=tx,ty,tz */
+	int fused = 0;
+	int ftile[3] = {16, 64, 16};
+	int fused_minz = 64; /* levels with fewer planes keep the phase launches (I3G_FUSED_MINZ) */
 	long reorth = 0;         /* inner Gram-Schmidt second passes (diagnostic) */
 	float *st12 = nullptr;   /* strain rates and stresses, 12 level-0 planes */
 };
@@ -635,7 +649,20 @@ static int build_level(i3g_ctx *c, int l, const i3g_setup *s)
 	L.x32 = dalloc<float>(4 * N, &err);
 	L.b32 = dalloc<float>(4 * N, &err);
 	L.r32 = dalloc<float>(4 * N, &err);
+	L.x64b = dalloc<double>(4 * N, &err);
+	L.x32b = dalloc<float>(4 * N, &err);
 	if (err) return err;
+	{
+		/* balanced tiles of about ftile[] nodes (x, y) and slabs of about
+		 * ftile[2] planes */
+		FusedGeom &G = L.fg;
+		G.ntx = (int)((nx + c->ftile[0] - 1) / c->ftile[0]);
+		G.nty = (int)((ny + c->ftile[1] - 1) / c->ftile[1]);
+		G.nzs = (int)((nz + c->ftile[2] - 1) / c->ftile[2]);
+		G.txm = (int)((nx + G.ntx - 1) / G.ntx);
+		G.tym = (int)((ny + G.nty - 1) / G.nty);
+		G.remap = 0;
+	}
 	if (upload(L.flags, fl.data(), N) || upload(L.bcid, bcid.data(), 4 * N) ||
 	    upload(L.bc_u, bu.data(), L.nbc) || upload(L.bc_p, bp.data(), L.nbc) ||
 	    upload(L.bc_c64, bcv.data(), L.nbc) || upload(L.bc_k64, bkv.data(), L.nbc) ||
@@ -643,6 +670,8 @@ static int build_level(i3g_ctx *c, int l, const i3g_setup *s)
 		return I3G_EDEVICE;
 	HCHK(hipMemset(L.x64, 0, 4 * N * sizeof(double)));
 	HCHK(hipMemset(L.x32, 0, 4 * N * sizeof(float)));
+	HCHK(hipMemset(L.x64b, 0, 4 * N * sizeof(double)));
+	HCHK(hipMemset(L.x32b, 0, 4 * N * sizeof(float)));
 	return I3G_OK;
 }
 
@@ -702,6 +731,9 @@ template <> XferView<float> xview<float>(const Xfer &X)
 template <typename T> static T *lx(Level &L);
 template <> double *lx<double>(Level &L) { return L.x64; }
 template <> float *lx<float>(Level &L) { return L.x32; }
+template <typename T> static T *lxb(Level &L);
+template <> double *lxb<double>(Level &L) { return L.x64b; }
+template <> float *lxb<float>(Level &L) { return L.x32b; }
 template <typename T> static T *lb(Level &L);
 template <> double *lb<double>(Level &L) { return L.b64; }
 template <> float *lb<float>(Level &L) { return L.b32; }
@@ -732,6 +764,17 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		if (e && e[0] == '0') c->bottom = -1;
 		e = getenv("I3G_BC_CHAIN");
 		if (e && e[0] == '0') c->bc_chain = 0;
+		e = getenv("I3G_FUSED_SWEEP");
+		if (e) c->fused = e[0] == '1';
+		e = getenv("I3G_FUSED_MINZ");
+		if (e) c->fused_minz = atoi(e);
+		e = getenv("I3G_FUSED_TILE");
+		if (e) {
+			int a = 0, b = 0, d = 0;
+			if (sscanf(e, "%d,%d,%d", &a, &b, &d) == 3 && a > 0 && b > 0 && d > 0) {
+				c->ftile[0] = a; c->ftile[1] = b; c->ftile[2] = d;
+			}
+		}
 	}
 	if (hipStreamCreate(&c->s) != hipSuccess) { delete c; return I3G_EDEVICE; }
 	c->L.resize(c->nlev);
@@ -808,7 +851,8 @@ extern "C" int i3g_destroy(i3g_ctx *c)
 		void *ps[] = {L.g[0], L.g[1], L.g[2], L.ih64[0], L.ih64[1], L.ih64[2], L.ic64[0], L.ic64[1],
 			      L.ic64[2], L.ih32[0], L.ih32[1], L.ih32[2], L.ic32[0], L.ic32[1], L.ic32[2],
 			      L.eta64, L.eta32, L.wP64, L.wP32, L.flags, L.bcid, L.bc_u, L.bc_p, L.bc_c64,
-			      L.bc_k64, L.bc_c32, L.bc_k32, L.nu, L.x64, L.b64, L.r64, L.x32, L.b32, L.r32};
+			      L.bc_k64, L.bc_c32, L.bc_k32, L.nu, L.x64, L.b64, L.r64, L.x32, L.b32, L.r32,
+			      L.x64b, L.x32b};
 		for (void *p : ps)
 			if (p) hipFree(p);
 	}
@@ -927,6 +971,76 @@ static void sweep(i3g_ctx *c, int l, T *x, const T *b, const i3g_stokes_params *
 	bc_enforce<T>(c, L, x, 1);
 }
 
+/* One sweep xin -> xout in three launches (k_sweep_comp): vx, vy, vz with
+ * P; then the BC rows of xout.  Equals sweep() on xin to the bit. */
+template <typename T> static size_t fused_lds(const Level &L)
+{
+	return (size_t)4 * (L.fg.txm + 4) * (L.fg.tym + 4) * sizeof(T);
+}
+
+template <typename T>
+static void fused_sweep(i3g_ctx *c, int l, const T *xin, T *xout, const T *b, const i3g_stokes_params *p)
+{
+	Level &L = c->L[l];
+	LevelView<T> v = view<T>(L);
+	long N = L.N;
+	FusedGeom G = L.fg;
+	G.remap = c->xcd_remap && (l == 0 || (c->xcd_remap == 2 && L.nz >= 64));
+	int total = G.ntx * G.nty * G.nzs;
+	dim3 grid = G.remap ? dim3((unsigned)(((total + 7) / 8) * 8)) : dim3((unsigned)total);
+	size_t lds = fused_lds<T>(L);
+	if (lds > 64 * 1024) {
+		/* opt in to the larger LDS allocation once per kernel */
+		static size_t granted[2] = {0, 0};
+		size_t &gr = granted[sizeof(T) == 8];
+		if (lds > gr) {
+			hipFuncSetAttribute((const void *)k_sweep_comp<T, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+			hipFuncSetAttribute((const void *)k_sweep_comp<T, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+			hipFuncSetAttribute((const void *)k_sweep_comp<T, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+			gr = lds;
+		}
+	}
+	T v0 = (T)p->v0koef, p0 = (T)p->p0koef;
+	/* compulsory bytes of the component launches: own component read and
+	 * written, the other three unknown planes, three viscosity planes, the
+	 * rhs and the flag byte (the vz launch adds P written and its rhs) */
+	double w = sizeof(T), by = (double)N * (2 * w + 3 * w + 3 * w + w + 1);
+	const char *n1 = l == 0 ? (sizeof(T) == 8 ? "sweep_vx_f64_L0" : "sweep_vx_f32_L0") : "sweep_coarse";
+	const char *n2 = l == 0 ? (sizeof(T) == 8 ? "sweep_vy_f64_L0" : "sweep_vy_f32_L0") : "sweep_coarse";
+	const char *n3 = l == 0 ? (sizeof(T) == 8 ? "sweep_vzp_f64_L0" : "sweep_vzp_f32_L0") : "sweep_coarse";
+	{
+		Timed tm(c, n1, by);
+		hipLaunchKernelGGL((k_sweep_comp<T, 1>), grid, dim3(256), lds, c->s, v, xin, xin + N, xin + 2 * N,
+				   xin + 3 * N, b, xout + N, (T *)nullptr, G, v0, p0);
+	}
+	{
+		Timed tm(c, n2, by);
+		hipLaunchKernelGGL((k_sweep_comp<T, 2>), grid, dim3(256), lds, c->s, v, xin, (const T *)(xout + N),
+				   xin + 2 * N, xin + 3 * N, b, xout + 2 * N, (T *)nullptr, G, v0, p0);
+	}
+	{
+		Timed tm(c, n3, by + (double)N * 2 * w);
+		hipLaunchKernelGGL((k_sweep_comp<T, 3>), grid, dim3(256), lds, c->s, v, xin, (const T *)(xout + N),
+				   (const T *)(xout + 2 * N), xin + 3 * N, b, xout + 3 * N, xout, G, v0, p0);
+	}
+	bc_enforce<T>(c, L, xout, 1);
+}
+
+/* ns sweeps on the level's unknowns: pairs of out-of-place sweeps through
+ * the second vector (so the result lands back in x), an odd one in place */
+template <typename T> static void smooth(i3g_ctx *c, int l, int ns, const i3g_stokes_params *p)
+{
+	Level &L = c->L[l];
+	T *x = lx<T>(L), *xb = lxb<T>(L), *b = lb<T>(L);
+	int s = 0;
+	if (c->fused && L.nz >= c->fused_minz)
+		for (; s + 2 <= ns; s += 2) {
+			fused_sweep<T>(c, l, x, xb, b, p);
+			fused_sweep<T>(c, l, xb, x, b, p);
+		}
+	for (; s < ns; s++) sweep<T>(c, l, x, b, p);
+}
+
 template <typename T> static void residual_level(i3g_ctx *c, int l, const T *x, const T *b, T *r)
 {
 	Level &L = c->L[l];
@@ -1033,10 +1147,10 @@ template <typename T> static void vcycle(i3g_ctx *c, int l, const i3g_stokes_par
 	if (l == c->nlev - 1) {
 		int ns = p->nu_coarse > 0 ? p->nu_coarse : 64;
 		if (coarsest_launch<T>(c, p, ns)) return;
-		for (int s = 0; s < ns; s++) sweep<T>(c, l, x, b, p);
+		smooth<T>(c, l, ns, p);
 		return;
 	}
-	for (int s = 0; s < p->nu_pre[l]; s++) sweep<T>(c, l, x, b, p);
+	smooth<T>(c, l, p->nu_pre[l], p);
 	residual_level<T>(c, l, x, b, lr<T>(L));
 	Level &Cl = c->L[l + 1];
 	{
@@ -1052,7 +1166,7 @@ template <typename T> static void vcycle(i3g_ctx *c, int l, const i3g_stokes_par
 				   view<T>(Cl), xview<T>(c->X[l]), (const T *)lx<T>(Cl), x, (T)p->v1koef,
 				   (T)p->p1koef);
 	}
-	for (int s = 0; s < p->nu_post[l]; s++) sweep<T>(c, l, x, b, p);
+	smooth<T>(c, l, p->nu_post[l], p);
 }
 
 /* ---- reductions -------------------------------------------------------- */
@@ -1718,6 +1832,106 @@ extern "C" int i3g_stokes_prolong(i3g_ctx *c, int level, const double *const *co
 			   p1koef);
 	HCHK(hipGetLastError());
 	return planes_out(c, fine, F.x64, F.N);
+}
+
+/* ---- the smoother on host planes (FP64 or rounded to FP32) -------------- */
+
+/* x and b planes of level l into the level's work vectors of precision T */
+template <typename T> static int smooth_in(i3g_ctx *c, Level &L, const double *const *x, const double *const *b)
+{
+	int rc = planes_in(c, (const double *const *)x, L.x64, L.N);
+	if (!rc) rc = planes_in(c, b, L.b64, L.N);
+	if (rc) return rc;
+	if (sizeof(T) == 4) {
+		long n = 4 * L.N;
+		hipLaunchKernelGGL((k_copy_scale<float, double>), lin_grid(n), dim3(256), 0, c->s, L.x32,
+				   (const double *)L.x64, 1.0, n);
+		hipLaunchKernelGGL((k_copy_scale<float, double>), lin_grid(n), dim3(256), 0, c->s, L.b32,
+				   (const double *)L.b64, 1.0, n);
+	}
+	return I3G_OK;
+}
+
+template <typename T> static int smooth_out(i3g_ctx *c, Level &L, double *const *x)
+{
+	if (sizeof(T) == 4) {
+		long n = 4 * L.N;
+		hipLaunchKernelGGL((k_copy_scale<double, float>), lin_grid(n), dim3(256), 0, c->s, L.x64,
+				   (const float *)L.x32, 1.0, n);
+	}
+	HCHK(hipGetLastError());
+	return planes_out(c, x, L.x64, L.N);
+}
+
+template <typename T>
+static int smooth_phase_impl(i3g_ctx *c, int level, int phase, double *const *x, const double *const *b,
+			     const i3g_stokes_params *p)
+{
+	Level &L = c->L[level];
+	int rc = smooth_in<T>(c, L, x, b);
+	if (rc) return rc;
+	LevelView<T> v = view<T>(L);
+	T *xx = lx<T>(L);
+	const T *bb = lb<T>(L);
+	if (phase == 6) {
+		hipLaunchKernelGGL(k_smooth_p<T>, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, v, xx, bb,
+				   (T)p->p0koef);
+	} else {
+		dim3 hg = half_grid(L.nx, L.ny, L.nz);
+		int chunks = (int)hg.x, col = phase & 1;
+		T v0 = (T)p->v0koef;
+		if (phase / 2 == 0)
+			hipLaunchKernelGGL((k_smooth_vel_ilp<T, 1>), hg, dim3(256), 0, c->s, v, xx, bb, col, v0, 0, chunks);
+		else if (phase / 2 == 1)
+			hipLaunchKernelGGL((k_smooth_vel_ilp<T, 2>), hg, dim3(256), 0, c->s, v, xx, bb, col, v0, 0, chunks);
+		else
+			hipLaunchKernelGGL((k_smooth_vel_ilp<T, 3>), hg, dim3(256), 0, c->s, v, xx, bb, col, v0, 0, chunks);
+	}
+	return smooth_out<T>(c, L, x);
+}
+
+extern "C" int i3g_stokes_smooth_phase(i3g_ctx *c, int level, int phase, int precision, double *const *x,
+				       const double *const *b, double v0koef, double p0koef)
+{
+	if (!c || !c->coef_ready || !x || !b || level < 0 || level >= c->nlev || phase < 0 || phase > 6 ||
+	    (precision != 32 && precision != 64))
+		return I3G_EINVAL;
+	i3g_stokes_params p;
+	i3g_stokes_default_params(&p);
+	p.v0koef = v0koef;
+	p.p0koef = p0koef;
+	return precision == 64 ? smooth_phase_impl<double>(c, level, phase, x, b, &p)
+			       : smooth_phase_impl<float>(c, level, phase, x, b, &p);
+}
+
+template <typename T>
+static int smooth_impl(i3g_ctx *c, int level, int nsweeps, int variant, double *const *x, const double *const *b,
+		       const i3g_stokes_params *p)
+{
+	Level &L = c->L[level];
+	int rc = smooth_in<T>(c, L, x, b);
+	if (rc) return rc;
+	int keep = c->fused, keepz = c->fused_minz;
+	if (variant == 1) c->fused = 0;
+	if (variant == 2) { c->fused = 1; c->fused_minz = 0; }
+	smooth<T>(c, level, nsweeps, p);
+	c->fused = keep;
+	c->fused_minz = keepz;
+	return smooth_out<T>(c, L, x);
+}
+
+extern "C" int i3g_stokes_smooth(i3g_ctx *c, int level, int precision, int nsweeps, int variant, double *const *x,
+				 const double *const *b, double v0koef, double p0koef)
+{
+	if (!c || !c->coef_ready || !x || !b || level < 0 || level >= c->nlev || nsweeps < 0 || variant < 0 ||
+	    variant > 2 || (precision != 32 && precision != 64))
+		return I3G_EINVAL;
+	i3g_stokes_params p;
+	i3g_stokes_default_params(&p);
+	p.v0koef = v0koef;
+	p.p0koef = p0koef;
+	return precision == 64 ? smooth_impl<double>(c, level, nsweeps, variant, x, b, &p)
+			       : smooth_impl<float>(c, level, nsweeps, variant, x, b, &p);
 }
 
 /* host planes (NULL = zero) into a device 4N vector, missing unknowns

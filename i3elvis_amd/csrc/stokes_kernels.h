@@ -274,27 +274,36 @@ k_smooth_vel(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int com
  * the flag is read first and then all 25 loads of the row (the unknowns
  * of the stencil, the stress-point viscosities, the rhs) are issued before
  * any is used.  Same arithmetic as row_ax; the scheduling barrier keeps the
- * compiler from sinking the loads back to their uses.  COMP = 1, 2, 3. */
-template <typename T, int COMP>
-__device__ __forceinline__ void vel_update(const LevelView<T> &L, T *__restrict__ x, const T *__restrict__ b,
-					   int i, int j, int k, T v0koef)
+ * compiler from sinking the loads back to their uses.  COMP = 1, 2, 3.
+ *
+ * vel_relax returns the relaxed value x + v0koef (b - A x)/diag of
+ * component COMP at node n = (i,j,k) (gausmat4, move3mg.c:3121-3124).  The
+ * component's own seven stencil values come through `own(di, dj, dk)`
+ * (offsets from (i,j,k)); the cross-component unknowns, P and the
+ * viscosities are read from global memory.  Every smoother kernel calls this
+ * one function, so that they all round alike. */
+template <typename T> struct OwnGlobal {
+	const T *a;
+	long n, sx, sz;
+	__device__ __forceinline__ T operator()(int di, int dj, int dk) const { return a[n + di * sx + dj + dk * sz]; }
+};
+
+template <typename T, int COMP, class Own>
+__device__ __forceinline__ T vel_relax(const LevelView<T> &L, const T *__restrict__ P, const T *__restrict__ vx,
+				       const T *__restrict__ vy, const T *__restrict__ vz, T rhs, int i, int j, int k,
+				       long n, T v0koef, const Own &own)
 {
-	long N = L.N, sx = L.ny, sz = (long)L.nx * L.ny;
-	long n = (long)k * sz + (long)i * sx + j;
-	unsigned fl = L.flags[n];
-	if (!(fl & FL_EX(COMP)) || (fl & FL_BC(COMP))) return;
-	const T *P = x, *vx = x + N, *vy = x + 2 * N, *vz = x + 3 * N;
-	long u = COMP * N + n;
-	T ax, d;
+	long sx = L.ny, sz = (long)L.nx * L.ny;
+	T ax, d, a0;
 	if (COMP == 1) {
 		T c0 = L.eC[n + sz + sx + 1], c1 = L.eC[n + sz + 1];
 		T xy1 = L.eXY[n + 1], xy0 = L.eXY[n], xz1 = L.eXZ[n + sz], xz0 = L.eXZ[n];
-		T a0 = vx[n], axp = vx[n + sx], axm = vx[n - sx], ayp = vx[n + 1], aym = vx[n - 1];
-		T azp = vx[n + sz], azm = vx[n - sz];
+		a0 = own(0, 0, 0);
+		T axp = own(1, 0, 0), axm = own(-1, 0, 0), ayp = own(0, 1, 0), aym = own(0, -1, 0);
+		T azp = own(0, 0, 1), azm = own(0, 0, -1);
 		T by1 = vy[n + 1], by1x = vy[n + 1 - sx], by0 = vy[n], by0x = vy[n - sx];
 		T cz1 = vz[n + sz], cz1x = vz[n + sz - sx], cz0 = vz[n], cz0x = vz[n - sx];
 		T p1 = P[n + sz + sx + 1], p0 = P[n + sz + 1];
-		T rhs = b ? b[u] : (T)0;
 		T ihx1 = L.ihx[i], ihx0 = L.ihx[i - 1], icx = L.icx[i], icy1 = L.icy[j + 1], icy0 = L.icy[j];
 		T icz1 = L.icz[k + 1], icz0 = L.icz[k], ihy = L.ihy[j], ihz = L.ihz[k];
 		__builtin_amdgcn_sched_barrier(0);
@@ -307,16 +316,15 @@ __device__ __forceinline__ void vel_update(const LevelView<T> &L, T *__restrict_
 		ax = xkf * (sxx1 - sxx0) + ykf * (sxy1 - sxy0) + zkf * (sxz1 - sxz0) - xkf * (p1 - p0);
 		d = -xkf * ((T)2 * c0 * ihx1 + (T)2 * c1 * ihx0) - ykf * ((T)2 * xy1 * icy1 + (T)2 * xy0 * icy0) -
 		    zkf * ((T)2 * xz1 * icz1 + (T)2 * xz0 * icz0);
-		x[u] = a0 + v0koef * (rhs - ax) / d;
 	} else if (COMP == 2) {
 		T c0 = L.eC[n + sz + sx + 1], c1 = L.eC[n + sz + sx];
 		T xy1 = L.eXY[n + sx], xy0 = L.eXY[n], yz1 = L.eYZ[n + sz], yz0 = L.eYZ[n];
-		T a0 = vy[n], ayp = vy[n + 1], aym = vy[n - 1], axp = vy[n + sx], axm = vy[n - sx];
-		T azp = vy[n + sz], azm = vy[n - sz];
+		a0 = own(0, 0, 0);
+		T ayp = own(0, 1, 0), aym = own(0, -1, 0), axp = own(1, 0, 0), axm = own(-1, 0, 0);
+		T azp = own(0, 0, 1), azm = own(0, 0, -1);
 		T bx1 = vx[n + sx], bx1y = vx[n + sx - 1], bx0 = vx[n], bx0y = vx[n - 1];
 		T cz1 = vz[n + sz], cz1y = vz[n + sz - 1], cz0 = vz[n], cz0y = vz[n - 1];
 		T p1 = P[n + sz + sx + 1], p0 = P[n + sz + sx];
-		T rhs = b ? b[u] : (T)0;
 		T ihy1 = L.ihy[j], ihy0 = L.ihy[j - 1], icy = L.icy[j], icx1 = L.icx[i + 1], icx0 = L.icx[i];
 		T icz1 = L.icz[k + 1], icz0 = L.icz[k], ihx = L.ihx[i], ihz = L.ihz[k];
 		__builtin_amdgcn_sched_barrier(0);
@@ -329,16 +337,15 @@ __device__ __forceinline__ void vel_update(const LevelView<T> &L, T *__restrict_
 		ax = ykf * (syy1 - syy0) + xkf * (sxy1 - sxy0) + zkf * (syz1 - syz0) - ykf * (p1 - p0);
 		d = -ykf * ((T)2 * c0 * ihy1 + (T)2 * c1 * ihy0) - xkf * ((T)2 * xy1 * icx1 + (T)2 * xy0 * icx0) -
 		    zkf * ((T)2 * yz1 * icz1 + (T)2 * yz0 * icz0);
-		x[u] = a0 + v0koef * (rhs - ax) / d;
 	} else {
 		T c0 = L.eC[n + sz + sx + 1], c1 = L.eC[n + sx + 1];
 		T xz1 = L.eXZ[n + sx], xz0 = L.eXZ[n], yz1 = L.eYZ[n + 1], yz0 = L.eYZ[n];
-		T a0 = vz[n], azp = vz[n + sz], azm = vz[n - sz], axp = vz[n + sx], axm = vz[n - sx];
-		T ayp = vz[n + 1], aym = vz[n - 1];
+		a0 = own(0, 0, 0);
+		T azp = own(0, 0, 1), azm = own(0, 0, -1), axp = own(1, 0, 0), axm = own(-1, 0, 0);
+		T ayp = own(0, 1, 0), aym = own(0, -1, 0);
 		T bx1 = vx[n + sx], bx1z = vx[n + sx - sz], bx0 = vx[n], bx0z = vx[n - sz];
 		T cy1 = vy[n + 1], cy1z = vy[n + 1 - sz], cy0 = vy[n], cy0z = vy[n - sz];
 		T p1 = P[n + sz + sx + 1], p0 = P[n + sx + 1];
-		T rhs = b ? b[u] : (T)0;
 		T ihz1 = L.ihz[k], ihz0 = L.ihz[k - 1], icz = L.icz[k], icx1 = L.icx[i + 1], icx0 = L.icx[i];
 		T icy1 = L.icy[j + 1], icy0 = L.icy[j], ihx = L.ihx[i], ihy = L.ihy[j];
 		__builtin_amdgcn_sched_barrier(0);
@@ -351,8 +358,22 @@ __device__ __forceinline__ void vel_update(const LevelView<T> &L, T *__restrict_
 		ax = zkf * (szz1 - szz0) + xkf * (sxz1 - sxz0) + ykf * (syz1 - syz0) - zkf * (p1 - p0);
 		d = -zkf * ((T)2 * c0 * ihz1 + (T)2 * c1 * ihz0) - xkf * ((T)2 * xz1 * icx1 + (T)2 * xz0 * icx0) -
 		    ykf * ((T)2 * yz1 * icy1 + (T)2 * yz0 * icy0);
-		x[u] = a0 + v0koef * (rhs - ax) / d;
 	}
+	return a0 + v0koef * (rhs - ax) / d;
+}
+
+template <typename T, int COMP>
+__device__ __forceinline__ void vel_update(const LevelView<T> &L, T *__restrict__ x, const T *__restrict__ b,
+					   int i, int j, int k, T v0koef)
+{
+	long N = L.N, sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = (long)k * sz + (long)i * sx + j;
+	unsigned fl = L.flags[n];
+	if (!(fl & FL_EX(COMP)) || (fl & FL_BC(COMP))) return;
+	long u = COMP * N + n;
+	T rhs = b ? b[u] : (T)0;
+	OwnGlobal<T> own{x + COMP * N, n, sx, sz};
+	x[u] = vel_relax<T, COMP>(L, x, x + N, x + 2 * N, x + 3 * N, rhs, i, j, k, n, v0koef, own);
 }
 
 template <typename T, int COMP>
@@ -372,18 +393,30 @@ k_smooth_vel_ilp(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int
 }
 
 /* Pressure relaxation P -= p0koef * eta_cell * (div v - b_P)
- * (move3mg.c:3105-3111); BC rows are left to k_bc. */
+ * (move3mg.c:3105-3111); BC rows are left to k_bc.  p_relax returns the
+ * relaxed value from the old p; vz comes through `vzf(di, dj, dk)`. */
+template <typename T, class VZ>
+__device__ __forceinline__ T p_relax(const LevelView<T> &L, const T *__restrict__ vx, const T *__restrict__ vy,
+				     const VZ &vzf, T p, T rhs, int i, int j, int k, long n, T p0koef)
+{
+	long sx = L.ny, sz = (long)L.nx * L.ny;
+	long v0 = n - sz - sx - 1;
+	T ax = ((T)vx[v0 + sx] - (T)vx[v0]) * L.ihx[i - 1] + ((T)vy[v0 + 1] - (T)vy[v0]) * L.ihy[j - 1] +
+	       (vzf(-1, -1, 0) - vzf(-1, -1, -1)) * L.ihz[k - 1];
+	T res = ax - rhs;
+	return p - res * L.eC[n] * p0koef;
+}
+
 template <typename T>
 __device__ __forceinline__ void p_update(const LevelView<T> &L, T *__restrict__ x, const T *__restrict__ b, int i,
 					 int j, int k, T p0koef)
 {
-	long N = L.N;
-	long n = (long)k * L.nx * L.ny + (long)i * L.ny + j;
+	long N = L.N, sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = (long)k * sz + (long)i * sx + j;
 	unsigned fl = L.flags[n];
 	if (!(fl & FL_EX(0)) || (fl & FL_BC(0))) return;
-	T ax = row_ax<T, T>(L, x, x + N, x + 2 * N, x + 3 * N, 0, i, j, k, (T *)0);
-	T res = ax - (b ? b[n] : (T)0);
-	x[n] -= res * L.eC[n] * p0koef;
+	OwnGlobal<T> vzf{x + 3 * N, n, sx, sz};
+	x[n] = p_relax<T>(L, x + N, x + 2 * N, vzf, x[n], b ? b[n] : (T)0, i, j, k, n, p0koef);
 }
 
 template <typename T>
@@ -395,6 +428,142 @@ k_smooth_p(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, T p0koef)
 	int i = t / L.ny, j = t - i * L.ny;
 	if (i >= L.nx) return;
 	p_update<T>(L, x, b, i, j, k, p0koef);
+}
+
+/* ---- both colours of one velocity component in one pass ----------------- */
+
+/* The colour-ordered sweep (vx colour 0, vx colour 1, vy 0, vy 1, vz 0, vz 1,
+ * P) as three launches, one per velocity component, the pressure update
+ * folded into the vz launch.  Each launch reads the sweep's input vector
+ * and writes a second vector (out of place), so every old value a work-group
+ * needs stays readable while the others write.
+ *
+ * A work-group owns an (x, y) tile of a z-slab and marches up through the
+ * slab plane by plane.  The component's old values of the tile plus two
+ * rings are staged in an LDS ring of four planes.  At plane kk it relaxes
+ * colour 0 on the tile plus one ring (the ring's values are the neighbouring
+ * work-group's, recomputed here from the same old inputs, bit for bit),
+ * then colour 1 of plane kk-1 on the tile from the colour-0 values in LDS,
+ * then stores plane kk-1 of the tile, and in the vz launch relaxes P of
+ * plane kk-1 (the tile shifted by +1 in x and y: P of cell (i,j,k) reads vz
+ * at (i-1,j-1,k) and (i-1,j-1,k-1)).  The vz launch relaxes colour 1 one
+ * plane past its slab (not stored) so that P of the slab's top plane sees
+ * the new vz.  Every node update goes through vel_relax / p_relax, so the
+ * result equals the seven phase launches of the in-place sweep to the bit.
+ * Unknowns that are not relaxed (boundary rows, missing unknowns) are
+ * copied to the output; P of the planes i = 0, j = 0, k = 0 (no unknowns)
+ * is never written and stays zero. */
+struct FusedGeom {
+	int ntx, nty, nzs; /* tiles in x, y; slabs in z */
+	int txm, tym;      /* largest tile extents */
+	int remap;         /* XCD-contiguous work order */
+};
+
+template <typename T> struct OwnLds {
+	const T *m, *z, *p; /* slots of planes k-1, k, k+1 at node (i,j) */
+	int rs;             /* row stride (y) */
+	__device__ __forceinline__ T operator()(int di, int dj, int dk) const
+	{
+		const T *s = dk < 0 ? m : (dk > 0 ? p : z);
+		return s[di * rs + dj];
+	}
+};
+
+template <typename T, int COMP>
+__global__ void __launch_bounds__(256)
+k_sweep_comp(LevelView<T> L, const T *__restrict__ P, const T *__restrict__ vx, const T *__restrict__ vy,
+	     const T *__restrict__ vz, const T *__restrict__ b, T *__restrict__ out, T *__restrict__ pout,
+	     FusedGeom G, T v0koef, T p0koef)
+{
+	extern __shared__ __align__(16) unsigned char lds_raw[];
+	T *ring = (T *)lds_raw;
+	int w, dummy;
+	int total = G.ntx * G.nty * G.nzs;
+	if (!block_work(G.remap, total, 1, &w, &dummy)) return;
+	int tx = w % G.ntx, rest = w / G.ntx;
+	int ty = rest % G.nty, sl = rest / G.nty;
+	const int nx = L.nx, ny = L.ny, nz = L.nz;
+	const long N = L.N, sx = ny, sz = (long)nx * ny;
+	const int i0 = (int)((long)tx * nx / G.ntx), i1 = (int)((long)(tx + 1) * nx / G.ntx);
+	const int j0 = (int)((long)ty * ny / G.nty), j1 = (int)((long)(ty + 1) * ny / G.nty);
+	const int k0 = (int)((long)sl * nz / G.nzs), k1 = (int)((long)(sl + 1) * nz / G.nzs);
+	const int rs = G.tym + 4, slot = (G.txm + 4) * rs;
+	const T *own_old = COMP == 1 ? vx : (COMP == 2 ? vy : vz);
+	const T *bown = b ? b + COMP * N : nullptr;
+	/* region bounds: load (tile + 2 rings), colour 0 (tile + 1 ring) */
+	const int la = max(i0 - 2, 0), lb = min(i1 + 2, nx), lc = max(j0 - 2, 0), ld = min(j1 + 2, ny);
+	const int ca = max(i0 - 1, 0), cb = min(i1 + 1, nx), cc = max(j0 - 1, 0), cd = min(j1 + 1, ny);
+	const int c1end = COMP == 3 ? min(k1 + 1, nz) : k1;
+	auto lidx = [&](int i, int j) { return (i - i0 + 2) * rs + (j - j0 + 2); };
+	auto load_plane = [&](int q) {
+		T *s = ring + (q & 3) * slot;
+		int wl = ld - lc, cnt = (lb - la) * wl;
+		long base = (long)q * sz;
+		for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+			int ii = t / wl, jj = t - ii * wl;
+			int i = la + ii, j = lc + jj;
+			s[lidx(i, j)] = own_old[base + (long)i * sx + j];
+		}
+	};
+	/* relax colour `col` of plane kq on [ia,ib) x [ja,jb) */
+	auto relax = [&](int kq, int col, int ia, int ib, int ja, int jb) {
+		T *sz0 = ring + (kq & 3) * slot;
+		const T *sm = ring + ((kq + 3) & 3) * slot, *sp = ring + ((kq + 1) & 3) * slot;
+		int hw = (jb - ja + 1) >> 1, cnt = (ib - ia) * hw;
+		for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+			int ii = t / hw, jj = t - ii * hw;
+			int i = ia + ii;
+			int j = ja + ((col - i - kq - ja) & 1) + 2 * jj;
+			if (j >= jb) continue;
+			long n = (long)kq * sz + (long)i * sx + j;
+			unsigned fl = L.flags[n];
+			if (!(fl & FL_EX(COMP)) || (fl & FL_BC(COMP))) continue;
+			T rhs = bown ? bown[n] : (T)0;
+			int li = lidx(i, j);
+			OwnLds<T> own{sm + li, sz0 + li, sp + li, rs};
+			sz0[li] = vel_relax<T, COMP>(L, P, vx, vy, vz, rhs, i, j, kq, n, v0koef, own);
+		}
+	};
+	for (int q = k0 - 2; q < k0; q++)
+		if (q >= 0) load_plane(q);
+	for (int kk = k0 - 1; kk <= c1end; kk++) {
+		if (kk + 1 < nz) load_plane(kk + 1);
+		__syncthreads();
+		if (kk >= 0 && kk < nz) relax(kk, 0, ca, cb, cc, cd);
+		__syncthreads();
+		int kc = kk - 1;
+		if (kc >= k0 && kc < c1end) relax(kc, 1, i0, i1, j0, j1);
+		__syncthreads();
+		if (kc >= k0 && kc < k1) {
+			const T *s = ring + (kc & 3) * slot;
+			int wt = j1 - j0, cnt = (i1 - i0) * wt;
+			long base = (long)kc * sz;
+			for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+				int ii = t / wt, jj = t - ii * wt;
+				int i = i0 + ii, j = j0 + jj;
+				out[base + (long)i * sx + j] = s[lidx(i, j)];
+			}
+		}
+		if (COMP == 3 && kc >= k0 + 1 && kc <= k1 && kc >= 1 && kc < nz) {
+			const T *s0 = ring + (kc & 3) * slot, *s1 = ring + ((kc + 3) & 3) * slot;
+			int pa = max(i0 + 1, 1), pb = min(i1 + 1, nx), pc = max(j0 + 1, 1), pd = min(j1 + 1, ny);
+			int wt = pd - pc, cnt = (pb - pa) * wt;
+			for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+				int ii = t / wt, jj = t - ii * wt;
+				int i = pa + ii, j = pc + jj;
+				long n = (long)kc * sz + (long)i * sx + j;
+				unsigned fl = L.flags[n];
+				T pold = P[n];
+				if ((fl & FL_EX(0)) && !(fl & FL_BC(0))) {
+					int li = lidx(i, j);
+					OwnLds<T> vzf{s1 + li, s0 + li, s0 + li, rs};
+					pold = p_relax<T>(L, vx, vy, vzf, pold, b ? b[n] : (T)0, i, j, kc, n, p0koef);
+				}
+				pout[n] = pold;
+			}
+		}
+		__syncthreads();
+	}
 }
 
 /* Boundary rows x_u = c + k x_partner for the records [start, end) of one

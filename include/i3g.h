@@ -153,6 +153,32 @@ int i3g_stokes_restrict(i3g_ctx *ctx, int level, const double *const *fine,
 int i3g_stokes_prolong(i3g_ctx *ctx, int level, const double *const *coarse,
 		       double *const *fine, double v1koef, double p1koef);
 
+/* The smoother of the V-cycle on host planes of one level (P, vx, vy, vz;
+ * SoA), replacing gausmat4's relaxation loop (move3mg.c:3087-3131) with its
+ * update rules: velocity rows x += v0koef (b - A x) / a_ii (a_ii the
+ * row's own coefficient, the first entry of the reference's row), pressure
+ * rows P += p0koef eta_cell (b - A x); boundary rows are not relaxed.
+ *
+ * i3g_stokes_smooth_phase runs one phase of the colour-ordered sweep:
+ * phase 0..5 relaxes velocity component 1 + phase/2 on the nodes with
+ * (m1 + m2 + m3) % 2 == phase % 2, phase 6 every pressure row (no row of a
+ * phase couples to another row of the same phase, so its result does not
+ * depend on the order).  Nothing else changes.
+ * i3g_stokes_smooth runs nsweeps whole sweeps (the seven phases in order,
+ * then the BC rows x_u = k x_partner, homogeneous as inside the V-cycle)
+ * with the product's kernels: variant 0 the context's default, 1 the seven
+ * phase launches, 2 one launch per velocity component (the pressure rows in
+ * the vz launch; pairs of sweeps out of place, an odd last sweep as
+ * phases).  precision 64 or 32 (the planes are rounded to float on the
+ * device, and back).  Both need i3g_set_coefficients first and overwrite
+ * the level's multigrid work buffers. */
+int i3g_stokes_smooth_phase(i3g_ctx *ctx, int level, int phase, int precision,
+			    double *const *x, const double *const *b,
+			    double v0koef, double p0koef);
+int i3g_stokes_smooth(i3g_ctx *ctx, int level, int precision, int nsweeps,
+		      int variant, double *const *x, const double *const *b,
+		      double v0koef, double p0koef);
+
 typedef struct {
 	double rtol;          /* relative weighted residual target (1e-8) */
 	int precision;        /* 64: plain FP64 multigrid; 32: FP32 V-cycles

@@ -118,9 +118,19 @@ def vpprolong(F, Cl, x_c, x_f, v1koef, p1koef):
 
 
 def gs_sweep(L, etas, bondm, bondv, bondn, b, x, p0koef, v0koef):
+    """one lexicographic gausmat4 sweep (move3mg.c:3087-3131), interleaved x"""
     x = np.array(x, np.float64, copy=True)
     lib().or_gs_sweep(C.byref(L), *[_p(e) for e in etas], _p(bondm), _p(bondv),
                       _p(bondn), _p(b), _p(x), C.c_double(p0koef), C.c_double(v0koef))
+    return x
+
+
+def colour_phase(L, etas, bondm, b, x, phase, p0koef, v0koef):
+    """one phase of the colour-ordered sweep (gausmat4's update rules on a
+    colour of one velocity component, or on every P row), interleaved x"""
+    x = np.array(x, np.float64, copy=True)
+    lib().or_colour_phase(C.byref(L), *[_p(e) for e in etas], _p(bondm), _p(b), _p(x), C.c_int(phase),
+                          C.c_double(p0koef), C.c_double(v0koef))
     return x
 
 

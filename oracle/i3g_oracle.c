@@ -553,6 +553,36 @@ void or_gs_sweep(const orl_t *L, const double *etaC, const double *etaXY,
 	}
 }
 
+/* One phase of the colour-ordered sweep with the gausmat4 update rules
+ * (move3mg.c:3100-3128): phase 0..5 relaxes velocity component 1 + phase/2
+ * on the nodes with (m1+m2+m3) % 2 == phase % 2 by x += v0koef (b - A x)/a_ii,
+ * phase 6 every pressure row by P += p0koef eta_cell (b - A x); A x from the
+ * state before the phase (no row of a phase couples to another of the same
+ * phase).  Boundary-condition rows are not relaxed. */
+void or_colour_phase(const orl_t *L, const double *etaC, const double *etaXY,
+		     const double *etaXZ, const double *etaYZ, const int *bondm,
+		     const double *b, double *x, int phase, double p0koef, double v0koef)
+{
+	long N = L->nx * L->ny * L->nz, m1, m2, m3;
+	int comp = phase == 6 ? 0 : 1 + phase / 2, col = phase & 1;
+	double *nx_ = malloc(sizeof(double) * 4 * N);
+	ctx_t C = {L, etaC, etaXY, etaXZ, etaYZ, x};
+	memcpy(nx_, x, sizeof(double) * 4 * N);
+	for (m3 = 0; m3 < L->nz; m3++)
+	for (m1 = 0; m1 < L->nx; m1++)
+	for (m2 = 0; m2 < L->ny; m2++) {
+		long n = NODE(L, m1, m2, m3), u = n * 4 + comp;
+		double ax, d;
+		if (!exists(L, m1, m2, m3, comp) || bondm[(L->mgp + n) * 4 + comp]) continue;
+		if (comp && ((m1 + m2 + m3) & 1) != col) continue;
+		ax = row_ax(&C, m1, m2, m3, comp, &d);
+		if (comp) nx_[u] = x[u] + v0koef * (b[u] - ax) / d;
+		else nx_[u] = x[u] + p0koef * etaC[n] * (b[u] - ax);
+	}
+	memcpy(x, nx_, sizeof(double) * 4 * N);
+	free(nx_);
+}
+
 /* ------------------------------------------------------------------ */
 /* Markers                                                            */
 /* ------------------------------------------------------------------ */

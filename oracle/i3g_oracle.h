@@ -76,6 +76,14 @@ void or_gs_sweep(const orl_t *L, const double *etaC, const double *etaXY,
 		 const float *bondv, const int *bondn, const double *b, double *x,
 		 double p0koef, double v0koef);
 
+/* One phase of the colour-ordered sweep with the gausmat4 update rules
+ * (move3mg.c:3100-3128): phase 0..5 = velocity component 1 + phase/2 on the
+ * nodes with (m1+m2+m3) % 2 == phase % 2, phase 6 = every pressure row.
+ * BC rows untouched; x updated in place from A x of the state before. */
+void or_colour_phase(const orl_t *L, const double *etaC, const double *etaXY,
+		     const double *etaXZ, const double *etaYZ, const int *bondm,
+		     const double *b, double *x, int phase, double p0koef, double v0koef);
+
 /* ---- markers (test infrastructure) ---------------------------------- */
 
 typedef struct {

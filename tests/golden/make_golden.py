@@ -12,9 +12,15 @@ inputs and outputs only.
                     vpprolong outputs on random vectors (move3mg.c:718-1579,
                     2642-3520)
   g3_converged_*.npz  converged FP64 solution of the reference's assembled
-                    level-0 system (configuration 0 geometry) by an
-                    equilibrated sparse direct solve, pressure gauge
-                    P(2,1,2) = 0 as vpiterate (move3mg.c:2248-2269)
+                    level-0 system (configuration 0 geometry; c2: the
+                    subduction layout) by an equilibrated sparse direct
+                    solve, pressure gauge P(2,1,2) = 0 as vpiterate
+                    (move3mg.c:2248-2269)
+  g12_smoother.npz  on the g1 inputs: the reference's own coefficient of each
+                    assembled row's unknown (the first row entry, gausmat4's
+                    divisor ival1, move3mg.c:3096-3124) and the state after
+                    one gausmat4 sweep on level 0 from g1's x0 (one thread,
+                    lexicographic, move3mg.c:3087-3131)
 """
 import os
 import sys
@@ -122,6 +128,38 @@ def g3(R, n, which=0, seed=1):
     np.savez_compressed(os.path.join(HERE, f"g3_converged_c{which}_{n}.npz"), n=n, which=which, seed=seed,
                         sol=x, rhs=rhs, exists=ex, resid_max=np.abs(r).max())
     print(f"g3_converged_c{which}_{n}.npz written; max |b - A x| = {np.abs(r).max():.3e}")
+
+
+def g12(R):
+    g = dict(np.load(os.path.join(HERE, "g1_operator.npz")))
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    N0 = nx * ny * nz
+    R.setup_grid(g["gx"], g["gy"], g["gz"], int(g["multinum"]))
+    R.set_bc(g["bondm"], g["bondv"], g["bondn"])
+    v0koef, p0koef = 0.7, 0.3
+    R.set_physics(g=tuple(g["g"]), nubeg=float(g["nubeg"]), nuend=float(g["nuend"]), v0koef=v0koef, p0koef=p0koef)
+    for k in ("nu", "nxx", "nxy", "nxz", "nyz", "ro"):
+        R.f32(k)[:N0] = g[k]
+    rowptr, cols, vals, rhs = R.assemble_level(0)
+    assert np.array_equal(rhs, g["rhs0"])
+    has = np.diff(rowptr) > 0
+    # velocity and BC rows start with their own unknown; continuity rows
+    # have none (no P diagonal) and use nd * p0koef instead
+    vel = has & ((np.arange(4 * N0) % 4 != 0) | (g["bondm"][:4 * N0] != 0))
+    diag = np.zeros(4 * N0)
+    diag[vel] = vals[rowptr[:-1][vel]]
+    assert np.array_equal(cols[rowptr[:-1][vel]], np.arange(4 * N0)[vel]), "first row entry is not the row's unknown"
+    R.lib.refh_ronurestrict(1)
+    R.lib.refh_store_all_levels()
+    sol0 = R.f64("sol0", R.MAXPPP)
+    sol0[:] = 0
+    sol0[:4 * N0] = g["x0"]
+    R.lib.refh_gausmat4(1, 0)
+    gs_out = sol0[:4 * N0].copy()
+    nd0 = R.f32("nd")[:N0].copy()
+    np.savez_compressed(os.path.join(HERE, "g12_smoother.npz"), diag0=diag, gs_out=gs_out, nd0=nd0,
+                        v0koef=v0koef, p0koef=p0koef)
+    print("g12_smoother.npz written")
 
 
 ROCKS = [  # types 0,1 unused (air/water branches), 2..6 used
@@ -647,7 +685,7 @@ def g11(R):
 if __name__ == "__main__":
     R = Ref()
     R.lib.refh_set_threads(1)
-    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9", "g10", "g11"]
+    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9", "g10", "g11", "g12", "g3c2"]
     if "g1" in which:
         g1(R)
     if "g3" in which:
@@ -667,3 +705,8 @@ if __name__ == "__main__":
         g10(R)
     if "g11" in which:
         g11(R)
+    if "g12" in which:
+        g12(R)
+    if "g3c2" in which:
+        g3(R, 25, which=2, seed=2)
+        g3(R, 33, which=2, seed=2)

@@ -148,3 +148,96 @@ def test_viscmod_rows_and_restriction_match_reference(vm):
     # the means differ from the arithmetic one (a plausible slip: ignoring viscmod)
     et00 = oracle.eta(L0, g["nu"], g["nxx"], g["nxy"], g["nxz"], g["nyz"], 0, float(g["nubeg"]), float(g["nuend"]))
     assert not np.array_equal(et0[1], et00[1])
+
+
+def _phase_nodes(nx, ny, nz, phase):
+    """interleaved indices of the rows relaxed by one colour phase"""
+    k, i, j = np.meshgrid(np.arange(nz), np.arange(nx), np.arange(ny), indexing="ij")
+    n = (k * nx * ny + i * ny + j).ravel()
+    if phase == 6:
+        return n * 4
+    sel = ((i + j + k).ravel() & 1) == (phase & 1)
+    return n[sel] * 4 + 1 + phase // 2
+
+
+@pytest.mark.parametrize("phase", range(7))
+def test_colour_phase_matches_reference_rule(phase):
+    """One phase of the colour-ordered sweep against gausmat4's update rule
+    (move3mg.c:3100-3128) evaluated from the reference's own assembled rows
+    (g1 residual r0 = b - A x0, g12 row coefficient a_ii and nd): velocity
+    x += v0koef r / a_ii on one colour of one component, pressure
+    P += p0koef nd r on every continuity row; every other entry unchanged
+    bitwise."""
+    g = golden("g1_operator.npz")
+    s = golden("g12_smoother.npz")
+    L0, _ = _levels(g)
+    et = _etas(g, L0, 0, g["nu"])
+    v0, p0 = float(s["v0koef"]), float(s["p0koef"])
+    x0, r0 = g["x0"], g["r0"]
+    out = oracle.colour_phase(L0, et, g["bondm"], g["rhs0"], x0, phase, p0, v0)
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    rows = _phase_nodes(nx, ny, nz, phase)
+    bc = g["bondm"][:4 * nx * ny * nz] != 0
+    rows = rows[g["exists0"][rows] & ~bc[rows]]
+    if phase == 6:
+        want = p0 * s["nd0"][rows // 4].astype(np.float64) * r0[rows]
+    else:
+        want = v0 * r0[rows] / s["diag0"][rows]
+    got = out[rows] - x0[rows]
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max(), np.abs(got - want).max()
+    others = np.ones(len(x0), bool)
+    others[rows] = False
+    assert np.array_equal(out[others], x0[others])
+    # the reference rule differs from a sign slip or the other colour
+    assert np.abs(got + want).max() > 0.5 * np.abs(want).max()
+
+
+def test_lexicographic_sweep_matches_reference_gausmat4():
+    """or_gs_sweep (lexicographic, the four rows of a node in P, vx, vy, vz
+    order) equals one sweep of the reference's gausmat4 on level 0 (g12, one
+    thread) from g1's x0."""
+    g = golden("g1_operator.npz")
+    s = golden("g12_smoother.npz")
+    L0, _ = _levels(g)
+    et = _etas(g, L0, 0, g["nu"])
+    out = oracle.gs_sweep(L0, et, g["bondm"], g["bondv"], g["bondn"], g["rhs0"], g["x0"], float(s["p0koef"]),
+                          float(s["v0koef"]))
+    ref = s["gs_out"]
+    for c in range(4):
+        d = np.abs(out[c::4] - ref[c::4]).max()
+        assert d <= 1e-10 * np.abs(ref[c::4] - g["x0"][c::4]).max(), (c, d)
+
+
+def test_wnorms_closed_form():
+    """The convergence metric's weights: a unit residual on one continuity
+    row gives exactly eta_C / mean(dx, dy, dz) of its cell, on a momentum row
+    1, on a BC row 0; b enters the same way."""
+    g = golden("g1_operator.npz")
+    L0, _ = _levels(g)
+    et = _etas(g, L0, 0, g["nu"])
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    N = nx * ny * nz
+    bc = g["bondm"][:4 * N] != 0
+    ex = g["exists0"]
+    gx, gy, gz = g["gx"], g["gy"], g["gz"]
+    rng = np.random.default_rng(5)
+    zero = np.zeros(4 * N)
+    cont = np.where(ex & ~bc & (np.arange(4 * N) % 4 == 0))[0]
+    for u in rng.choice(cont, 5, replace=False):
+        n = u // 4
+        k, rem = divmod(n, nx * ny)
+        i, j = divmod(rem, ny)
+        h = ((gx[i] - gx[i - 1]) + (gy[j] - gy[j - 1]) + (gz[k] - gz[k - 1])) / 3.0
+        e = zero.copy()
+        e[u] = 1.0
+        w = oracle.wnorms(L0, et[0], g["bondm"], zero, e)
+        assert w[0] == pytest.approx(et[0][n] / h, rel=1e-14) and w[1] == 0
+        wb = oracle.wnorms(L0, et[0], g["bondm"], e, zero)
+        assert wb[1] == pytest.approx(et[0][n] / h, rel=1e-14) and wb[0] == 0
+    mom = np.where(ex & ~bc & (np.arange(4 * N) % 4 != 0))[0]
+    e = zero.copy()
+    e[mom[7]] = 1.0
+    assert oracle.wnorms(L0, et[0], g["bondm"], zero, e)[0] == 1.0
+    e = zero.copy()
+    e[np.where(bc & ex)[0][3]] = 1.0
+    assert oracle.wnorms(L0, et[0], g["bondm"], zero, e)[0] == 0.0

@@ -108,6 +108,7 @@ def test_load_first_smoother_matches_plain_kernel(gpu, monkeypatch, prec, tol):
     n = 25
     cfg = I.make_config(2, n, n, n, seed=5)
     sols = []
+    monkeypatch.setenv("I3G_FUSED_SWEEP", "0")  # the phase launches, where the two kernels differ
     for mode in ("0", "1"):
         monkeypatch.setenv("I3G_SMOOTH_ILP", mode)
         S = I.Stokes(cfg)
