@@ -224,6 +224,9 @@ struct Level {
 	double *x64b;
 	float *x32b;
 	FusedGeom fg;
+	/* pressure relaxation scale (k_pscale); null: eta_cell */
+	double *pS64;
+	float *pS32;
 	std::vector<double> hg[3];
 };
 
@@ -286,6 +289,10 @@ struct i3g_ctx {
 	 * -1 the reference's viscmod, or 0 arithmetic, 1 geometric, 2
 	 * harmonic (I3G_COARSE_VISC selects) */
 	int coarse_visc = -1;
+	int coarse_visc_maxlev = 99; /* I3G_COARSE_VISC applies to levels 1..this */
+	/* pressure relaxation scale: 0 eta_cell (gausmat4's nd), 1 the inverse
+	 * local Schur-complement diagonal (I3G_PSCALE) */
+	int pscale = 0;
 	/* velocity smoother with all row loads issued first (I3G_SMOOTH_ILP=0
 	 * selects the plain kernel) */
 	int smooth_ilp = 1;
@@ -684,6 +691,7 @@ template <> LevelView<double> view<double>(const Level &L)
 	v.icx = L.ic64[0]; v.icy = L.ic64[1]; v.icz = L.ic64[2];
 	v.eC = L.eta64; v.eXY = L.eta64 + L.N; v.eXZ = L.eta64 + 2 * L.N; v.eYZ = L.eta64 + 3 * L.N;
 	v.flags = L.flags; v.wP = L.wP64;
+	v.pS = L.pS64 ? L.pS64 : v.eC;
 	return v;
 }
 template <> LevelView<float> view<float>(const Level &L)
@@ -694,6 +702,7 @@ template <> LevelView<float> view<float>(const Level &L)
 	v.icx = L.ic32[0]; v.icy = L.ic32[1]; v.icz = L.ic32[2];
 	v.eC = L.eta32; v.eXY = L.eta32 + L.N; v.eXZ = L.eta32 + 2 * L.N; v.eYZ = L.eta32 + 3 * L.N;
 	v.flags = L.flags; v.wP = L.wP32;
+	v.pS = L.pS32 ? L.pS32 : v.eC;
 	return v;
 }
 template <typename T> static BcRec<T> bcview(const Level &L);
@@ -758,6 +767,10 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		if (e && e[0] == '0') c->smooth_ilp = 0;
 		e = getenv("I3G_COARSE_VISC");
 		if (e && e[0] >= '0' && e[0] <= '2') c->coarse_visc = e[0] - '0';
+		e = getenv("I3G_PSCALE");
+		if (e) c->pscale = atoi(e);
+		e = getenv("I3G_COARSE_VISC_MAXLEV");
+		if (e) c->coarse_visc_maxlev = atoi(e);
 		e = getenv("I3G_XFER_GEO");
 		if (e && e[0] == '1') c->xfer_geo = 1;
 		e = getenv("I3G_FUSED_BOTTOM");
@@ -852,7 +865,7 @@ extern "C" int i3g_destroy(i3g_ctx *c)
 			      L.ic64[2], L.ih32[0], L.ih32[1], L.ih32[2], L.ic32[0], L.ic32[1], L.ic32[2],
 			      L.eta64, L.eta32, L.wP64, L.wP32, L.flags, L.bcid, L.bc_u, L.bc_p, L.bc_c64,
 			      L.bc_k64, L.bc_c32, L.bc_k32, L.nu, L.x64, L.b64, L.r64, L.x32, L.b32, L.r32,
-			      L.x64b, L.x32b};
+			      L.x64b, L.x32b, L.pS64, L.pS32};
 		for (void *p : ps)
 			if (p) hipFree(p);
 	}
@@ -892,10 +905,12 @@ extern "C" int i3g_set_coefficients(i3g_ctx *c, const float *nu, const float *nx
 	/* viscmod selects the arithmetic, geometric or harmonic averages of
 	 * the level-0 stress-point viscosities and of ronurestrict, as in the
 	 * reference; I3G_COARSE_VISC overrides the coarse levels' choice */
-	int cm = c->coarse_visc >= 0 ? c->coarse_visc : viscmod;
+	int cm0 = c->coarse_visc >= 0 ? c->coarse_visc : viscmod;
+	auto cmode = [&](int l) { return l <= c->coarse_visc_maxlev ? cm0 : viscmod; };
 	for (int l = 1; l < c->nlev; l++) {
 		Level &F = c->L[l - 1], &Cl = c->L[l];
 		const Xfer &X = c->X[l - 1];
+		int cm = cmode(l);
 		hipLaunchKernelGGL(k_ronurestrict, plane_grid(Cl.nx, Cl.ny, Cl.nz), dim3(256), 0, c->s,
 				   F.nx, F.ny, Cl.nx, Cl.ny, Cl.nz, F.nu, Cl.nu, X.ax[0].istart[3],
 				   X.ax[0].iidx[3], X.ax[0].iw64[3], X.ax[1].istart[3], X.ax[1].iidx[3],
@@ -904,9 +919,26 @@ extern "C" int i3g_set_coefficients(i3g_ctx *c, const float *nu, const float *nx
 	for (int l = 0; l < c->nlev; l++) {
 		Level &L = c->L[l];
 		hipLaunchKernelGGL(k_eta, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, L.nx, L.ny, L.nz,
-				   l == 0 ? 1 : 0, L.nu, c->nxx, c->nxy, c->nxz, c->nyz, l == 0 ? viscmod : cm, phys[3],
+				   l == 0 ? 1 : 0, L.nu, c->nxx, c->nxy, c->nxz, c->nyz, l == 0 ? viscmod : cmode(l), phys[3],
 				   phys[4], L.eta64, L.eta32, L.wP64, L.wP32, L.g[0], L.g[1], L.g[2]);
 	}
+	if (c->pscale)
+		for (int l = 0; l < c->nlev; l++) {
+			Level &L = c->L[l];
+			int err = 0;
+			if (!L.pS64) L.pS64 = dalloc<double>(L.N, &err);
+			if (!L.pS32) L.pS32 = dalloc<float>(L.N, &err);
+			if (err) return err;
+			/* computed from the eta_cell view (pS unset yet) */
+			double *p64 = L.pS64;
+			float *p32 = L.pS32;
+			L.pS64 = nullptr;
+			L.pS32 = nullptr;
+			LevelView<double> v = view<double>(L);
+			L.pS64 = p64;
+			L.pS32 = p32;
+			hipLaunchKernelGGL(k_pscale<double>, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, v, p64, p32);
+		}
 	Level &L0 = c->L[0];
 	hipLaunchKernelGGL(k_rhs, plane_grid(L0.nx, L0.ny, L0.nz), dim3(256), 0, c->s, L0.nx, L0.ny, L0.nz,
 			   c->ro, phys[0], phys[1], phys[2], L0.flags, L0.bcid, L0.bc_c64, c->rhs0);
@@ -964,11 +996,14 @@ static void sweep(i3g_ctx *c, int l, T *x, const T *b, const i3g_stokes_params *
 						   (T)p->v0koef, remap, chunks);
 		}
 	{
-		Timed tm(c, l == 0 ? "smooth_p_L0" : "smooth_p_coarse", 0);
+		Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "smooth_p_f64_L0" : "smooth_p_f32_L0") : "smooth_p_coarse", 0);
 		hipLaunchKernelGGL(k_smooth_p<T>, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, v, x, b,
 				   (T)p->p0koef);
 	}
-	bc_enforce<T>(c, L, x, 1);
+	{
+		Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "bc_f64_L0" : "bc_f32_L0") : "bc_coarse", 0);
+		bc_enforce<T>(c, L, x, 1);
+	}
 }
 
 /* One sweep xin -> xout in three launches (k_sweep_comp): vx, vy, vz with

@@ -110,12 +110,13 @@ def test_constant_viscosity_hydrostatic_closed_form():
     assert np.abs(r2[~bc]).max() > 1.0
 
 
-@pytest.mark.parametrize("n", [17, 25])
-def test_golden_solution_satisfies_oracle_system(n):
-    """The reference-converged solution zeroes the oracle's residual."""
+@pytest.mark.parametrize("which,n", [(0, 17), (0, 25), (2, 25), (2, 33)])
+def test_golden_solution_satisfies_oracle_system(which, n):
+    """The reference-converged solution zeroes the oracle's residual
+    (configuration 0 and the subduction layout)."""
     import i3elvis_amd as I
-    g = golden(f"g3_converged_c0_{n}.npz")
-    cfg = I.make_config(0, n, n, n, seed=int(g["seed"]))
+    g = golden(f"g3_converged_c{which}_{n}.npz")
+    cfg = I.make_config(which, n, n, n, seed=int(g["seed"]))
     L = oracle.level(n, n, n, 0, 0, cfg["gx"], cfg["gy"], cfg["gz"])
     et = oracle.eta(L, cfg["nu"], cfg["nxx"], cfg["nxy"], cfg["nxz"], cfg["nyz"], 0, cfg["nubeg"], cfg["nuend"])
     b = oracle.rhs(L, cfg["ro"], cfg["g"], cfg["bondm"], cfg["bondv"])

@@ -49,20 +49,25 @@ def test_residual_fp64_matches_reference_rows(gpu):
     S.close()
 
 
-@pytest.mark.parametrize("n", [17, 25])
+@pytest.mark.parametrize("which,n", [(0, 17), (0, 25), (2, 25), (2, 33)])
 @pytest.mark.parametrize("prec", [64, 32])
-def test_solve_matches_reference_converged(gpu, n, prec):
-    g = golden(f"g3_converged_c0_{n}.npz")
-    cfg = I.make_config(0, n, n, n, seed=int(g["seed"]))
-    print(f"case: solve c0 n={n} prec={prec} multinum={cfg['multinum']}", flush=True)
+def test_solve_matches_reference_converged(gpu, which, n, prec):
+    """Converged fields against the direct solve of the reference's own
+    assembled level-0 system (g3): configuration 0 (sphere) and the
+    subduction layout of the 257^3 headline workload (10^6 contrast, weak
+    channel) at 25^3 and 33^3, mixed precision and FP64, each field within
+    1e-6 relative L2."""
+    g = golden(f"g3_converged_c{which}_{n}.npz")
+    cfg = I.make_config(which, n, n, n, seed=int(g["seed"]))
+    print(f"case: solve c{which} n={n} prec={prec} multinum={cfg['multinum']}", flush=True)
     S = I.Stokes(cfg)
     S.set_coefficients()
     N = n ** 3
     z = np.zeros(N)
     S.set_solution(z, z, z, z)
-    st = S.solve(precision=prec, rtol=1e-9, max_outer=40)
+    st = S.solve(precision=prec, rtol=1e-9 if which == 0 else 1e-10, max_outer=40, max_vcycles=4000)
     print("  outer", st.outer_iters, "vcycles", st.vcycles, "rel", st.rel_res, "s", st.seconds, flush=True)
-    assert st.converged and st.rel_res <= 1e-9
+    assert st.converged and st.rel_res <= (1e-9 if which == 0 else 1e-10)
     P, vx, vy, vz = S.get_solution()
     sol = g["sol"]
     ex = g["exists"]
