@@ -224,9 +224,6 @@ struct Level {
 	double *x64b;
 	float *x32b;
 	FusedGeom fg;
-	/* pressure relaxation scale (k_pscale); null: eta_cell */
-	double *pS64;
-	float *pS32;
 	std::vector<double> hg[3];
 };
 
@@ -290,9 +287,6 @@ struct i3g_ctx {
 	 * harmonic (I3G_COARSE_VISC selects) */
 	int coarse_visc = -1;
 	int coarse_visc_maxlev = 99; /* I3G_COARSE_VISC applies to levels 1..this */
-	/* pressure relaxation scale: 0 eta_cell (gausmat4's nd), 1 the inverse
-	 * local Schur-complement diagonal (I3G_PSCALE) */
-	int pscale = 0;
 	/* velocity smoother with all row loads issued first (I3G_SMOOTH_ILP=0
 	 * selects the plain kernel) */
 	int smooth_ilp = 1;
@@ -691,7 +685,6 @@ template <> LevelView<double> view<double>(const Level &L)
 	v.icx = L.ic64[0]; v.icy = L.ic64[1]; v.icz = L.ic64[2];
 	v.eC = L.eta64; v.eXY = L.eta64 + L.N; v.eXZ = L.eta64 + 2 * L.N; v.eYZ = L.eta64 + 3 * L.N;
 	v.flags = L.flags; v.wP = L.wP64;
-	v.pS = L.pS64 ? L.pS64 : v.eC;
 	return v;
 }
 template <> LevelView<float> view<float>(const Level &L)
@@ -702,7 +695,6 @@ template <> LevelView<float> view<float>(const Level &L)
 	v.icx = L.ic32[0]; v.icy = L.ic32[1]; v.icz = L.ic32[2];
 	v.eC = L.eta32; v.eXY = L.eta32 + L.N; v.eXZ = L.eta32 + 2 * L.N; v.eYZ = L.eta32 + 3 * L.N;
 	v.flags = L.flags; v.wP = L.wP32;
-	v.pS = L.pS32 ? L.pS32 : v.eC;
 	return v;
 }
 template <typename T> static BcRec<T> bcview(const Level &L);
@@ -767,8 +759,6 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		if (e && e[0] == '0') c->smooth_ilp = 0;
 		e = getenv("I3G_COARSE_VISC");
 		if (e && e[0] >= '0' && e[0] <= '2') c->coarse_visc = e[0] - '0';
-		e = getenv("I3G_PSCALE");
-		if (e) c->pscale = atoi(e);
 		e = getenv("I3G_COARSE_VISC_MAXLEV");
 		if (e) c->coarse_visc_maxlev = atoi(e);
 		e = getenv("I3G_XFER_GEO");
@@ -865,7 +855,7 @@ extern "C" int i3g_destroy(i3g_ctx *c)
 			      L.ic64[2], L.ih32[0], L.ih32[1], L.ih32[2], L.ic32[0], L.ic32[1], L.ic32[2],
 			      L.eta64, L.eta32, L.wP64, L.wP32, L.flags, L.bcid, L.bc_u, L.bc_p, L.bc_c64,
 			      L.bc_k64, L.bc_c32, L.bc_k32, L.nu, L.x64, L.b64, L.r64, L.x32, L.b32, L.r32,
-			      L.x64b, L.x32b, L.pS64, L.pS32};
+			      L.x64b, L.x32b};
 		for (void *p : ps)
 			if (p) hipFree(p);
 	}
@@ -922,23 +912,6 @@ extern "C" int i3g_set_coefficients(i3g_ctx *c, const float *nu, const float *nx
 				   l == 0 ? 1 : 0, L.nu, c->nxx, c->nxy, c->nxz, c->nyz, l == 0 ? viscmod : cmode(l), phys[3],
 				   phys[4], L.eta64, L.eta32, L.wP64, L.wP32, L.g[0], L.g[1], L.g[2]);
 	}
-	if (c->pscale)
-		for (int l = 0; l < c->nlev; l++) {
-			Level &L = c->L[l];
-			int err = 0;
-			if (!L.pS64) L.pS64 = dalloc<double>(L.N, &err);
-			if (!L.pS32) L.pS32 = dalloc<float>(L.N, &err);
-			if (err) return err;
-			/* computed from the eta_cell view (pS unset yet) */
-			double *p64 = L.pS64;
-			float *p32 = L.pS32;
-			L.pS64 = nullptr;
-			L.pS32 = nullptr;
-			LevelView<double> v = view<double>(L);
-			L.pS64 = p64;
-			L.pS32 = p32;
-			hipLaunchKernelGGL(k_pscale<double>, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, v, p64, p32);
-		}
 	Level &L0 = c->L[0];
 	hipLaunchKernelGGL(k_rhs, plane_grid(L0.nx, L0.ny, L0.nz), dim3(256), 0, c->s, L0.nx, L0.ny, L0.nz,
 			   c->ro, phys[0], phys[1], phys[2], L0.flags, L0.bcid, L0.bc_c64, c->rhs0);

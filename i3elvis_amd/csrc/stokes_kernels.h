@@ -26,8 +26,6 @@ template <typename T> struct LevelView {
 	const T *eC, *eXY, *eXZ, *eYZ;
 	const uint8_t *flags;
 	const T *wP; /* continuity-row weight eta_cell / mean cell size (P plane) */
-	const T *pS; /* pressure relaxation scale of each cell (eta_cell, or the
-		      * inverse diagonal of B D^-1 B^T) */
 };
 
 template <typename T> struct Vec4 {
@@ -133,63 +131,6 @@ __device__ __forceinline__ T row_ax(const LevelView<T> &L, const X *P, const X *
 		if (diag) *diag = d;
 	}
 	return ax;
-}
-
-/* Own coefficient a_ii of the velocity row of component comp at (i,j,k)
- * (the diagonal of row_ax) */
-template <typename T>
-__device__ __forceinline__ T vel_diag(const LevelView<T> &L, int comp, int i, int j, int k)
-{
-	long sx = L.ny, sz = (long)L.nx * L.ny;
-	long n = k * sz + i * sx + j;
-	if (comp == 1)
-		return -(T)2 * L.icx[i] * ((T)2 * L.eC[n + sz + sx + 1] * L.ihx[i] + (T)2 * L.eC[n + sz + 1] * L.ihx[i - 1]) -
-		       L.ihy[j] * ((T)2 * L.eXY[n + 1] * L.icy[j + 1] + (T)2 * L.eXY[n] * L.icy[j]) -
-		       L.ihz[k] * ((T)2 * L.eXZ[n + sz] * L.icz[k + 1] + (T)2 * L.eXZ[n] * L.icz[k]);
-	if (comp == 2)
-		return -(T)2 * L.icy[j] * ((T)2 * L.eC[n + sz + sx + 1] * L.ihy[j] + (T)2 * L.eC[n + sz + sx] * L.ihy[j - 1]) -
-		       L.ihx[i] * ((T)2 * L.eXY[n + sx] * L.icx[i + 1] + (T)2 * L.eXY[n] * L.icx[i]) -
-		       L.ihz[k] * ((T)2 * L.eYZ[n + sz] * L.icz[k + 1] + (T)2 * L.eYZ[n] * L.icz[k]);
-	return -(T)2 * L.icz[k] * ((T)2 * L.eC[n + sz + sx + 1] * L.ihz[k] + (T)2 * L.eC[n + sx + 1] * L.ihz[k - 1]) -
-	       L.ihx[i] * ((T)2 * L.eXZ[n + sx] * L.icx[i + 1] + (T)2 * L.eXZ[n] * L.icx[i]) -
-	       L.ihy[j] * ((T)2 * L.eYZ[n + 1] * L.icy[j + 1] + (T)2 * L.eYZ[n] * L.icy[j]);
-}
-
-/* Pressure relaxation scale of every cell: the inverse of the diagonal of
- * the local Schur complement B D^-1 B^T, i.e. 1 / sum over the six faces of
- * b_f^2 / |a_ff| (b_f the continuity row's coefficient of the face
- * velocity, a_ff that velocity row's own coefficient; boundary rows are
- * held by their condition and add nothing).  With one viscosity on a
- * uniform grid this is 4/3 eta; at a viscosity jump it follows the stiffer
- * neighbour that holds the face velocities.  Cells without a free face keep
- * eta_cell. */
-template <typename T>
-__global__ void k_pscale(LevelView<T> L, T *__restrict__ pS64, float *__restrict__ pS32)
-{
-	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	int k = blockIdx.y;
-	int i = t / L.ny, j = t - i * L.ny;
-	if (i >= L.nx) return;
-	long sx = L.ny, sz = (long)L.nx * L.ny;
-	long n = k * sz + i * sx + j;
-	T out = L.eC[n];
-	if (i >= 1 && j >= 1 && k >= 1) {
-		T acc = 0;
-		struct F { int c, a, b, d; T w; } f[6] = {
-			{1, i, j - 1, k - 1, L.ihx[i - 1]}, {1, i - 1, j - 1, k - 1, L.ihx[i - 1]},
-			{2, i - 1, j, k - 1, L.ihy[j - 1]}, {2, i - 1, j - 1, k - 1, L.ihy[j - 1]},
-			{3, i - 1, j - 1, k, L.ihz[k - 1]}, {3, i - 1, j - 1, k - 1, L.ihz[k - 1]}};
-		for (int q = 0; q < 6; q++) {
-			long m = (long)f[q].d * sz + (long)f[q].a * sx + f[q].b;
-			unsigned fl = L.flags[m];
-			if (!(fl & FL_EX(f[q].c)) || (fl & FL_BC(f[q].c))) continue;
-			T d = vel_diag<T>(L, f[q].c, f[q].a, f[q].b, f[q].d);
-			acc += f[q].w * f[q].w / (d < 0 ? -d : d);
-		}
-		if (acc > 0) out = (T)1 / acc;
-	}
-	pS64[n] = out;
-	pS32[n] = (float)out;
 }
 
 /* Boundary-condition record of one unknown (bonderr, move3mg.c:1532-1579):
@@ -463,7 +404,7 @@ __device__ __forceinline__ T p_relax(const LevelView<T> &L, const T *__restrict_
 	T ax = ((T)vx[v0 + sx] - (T)vx[v0]) * L.ihx[i - 1] + ((T)vy[v0 + 1] - (T)vy[v0]) * L.ihy[j - 1] +
 	       (vzf(-1, -1, 0) - vzf(-1, -1, -1)) * L.ihz[k - 1];
 	T res = ax - rhs;
-	return p - res * L.pS[n] * p0koef;
+	return p - res * L.eC[n] * p0koef;
 }
 
 template <typename T>
