@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -239,6 +240,7 @@ struct KStat {
 
 struct i3g_ctx {
 	hipStream_t s = nullptr;
+	int dev = 0;                 /* HIP device the context lives on */
 	i3g_hier h;
 	int nlev = 0;
 	std::vector<Level> L;
@@ -742,6 +744,31 @@ template <typename T> static T *lr(Level &L);
 template <> double *lr<double>(Level &L) { return L.r64; }
 template <> float *lr<float>(Level &L) { return L.r32; }
 
+/* ---- scratch of the entry point that allocates nothing ------------------- */
+
+/* Device memory for i3g_stokes_residual_noalloc: the per-block partial sums
+ * of the residual norms and their final reduction.  Every GPU the library
+ * runs on holds its own copy until the process ends.  In host code the
+ * array's name is not a device address: the address of each device's copy
+ * is looked up once per device (scratch_init, from i3g_create with that
+ * device current) and only that pointer is handed to kernels. */
+#define I3G_SCRATCH_BYTES ((size_t)4 << 20)
+#define I3G_SCRATCH_MAXDEV 64
+static __device__ __align__(256) unsigned char scratch[I3G_SCRATCH_BYTES];
+static std::mutex scratch_mu;
+static void *scratch_dev[I3G_SCRATCH_MAXDEV];
+
+static int scratch_init(int dev)
+{
+	if (dev < 0 || dev >= I3G_SCRATCH_MAXDEV) return I3G_EDEVICE;
+	std::lock_guard<std::mutex> g(scratch_mu);
+	if (scratch_dev[dev]) return I3G_OK;
+	void *ptr = nullptr;
+	if (hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return I3G_EDEVICE;
+	scratch_dev[dev] = ptr;
+	return I3G_OK;
+}
+
 extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 {
 	if (!s || !out || !s->gx || !s->gy || !s->gz || !s->bondm || !s->bondv || !s->bondn)
@@ -779,6 +806,7 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 			}
 		}
 	}
+	if (hipGetDevice(&c->dev) != hipSuccess || scratch_init(c->dev) != I3G_OK) { delete c; return I3G_EDEVICE; }
 	if (hipStreamCreate(&c->s) != hipSuccess) { delete c; return I3G_EDEVICE; }
 	c->L.resize(c->nlev);
 	for (int l = 0; l < c->nlev; l++) {
@@ -1242,6 +1270,41 @@ extern "C" int i3g_stokes_residual(i3g_ctx *c, const double *P, const double *vx
 		if (dst[q]) HCHK(hipMemcpyAsync(dst[q], L.r64 + q * N, N * sizeof(double), hipMemcpyDeviceToHost, c->s));
 	HCHK(hipStreamSynchronize(c->s));
 	if (norms) { norms[0] = nrm[0]; norms[1] = nrm[1]; }
+	return I3G_OK;
+}
+
+__global__ void k_sqrt2(const double *in, double *out)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0) {
+		out[0] = sqrt(in[0]);
+		out[1] = sqrt(in[1]);
+	}
+}
+
+/* i3g_stokes_residual on device buffers with nothing allocated: the block
+ * partial sums and their reduction live in the static scratch (at most
+ * I3G_SCRATCH_BYTES: 2 doubles per block of 256 nodes, plus 2), the
+ * residual goes to d_r or, when d_r is NULL, to the level's own work
+ * vector.  Same kernels and block partition as i3g_stokes_residual, so the
+ * results are the same to the bit.  Enqueued on `stream` (NULL: the
+ * context's stream), not synchronised.  Calls that overlap on one GPU share
+ * its scratch and race. */
+extern "C" int i3g_stokes_residual_noalloc(i3g_ctx *c, const double *d_x, double *d_r, double *d_norms, void *stream)
+{
+	if (!c || !c->coef_ready || !d_x || !d_norms) return I3G_EINVAL;
+	if (c->dev < 0 || c->dev >= I3G_SCRATCH_MAXDEV || !scratch_dev[c->dev]) return I3G_EDEVICE;
+	Level &L = c->L[0];
+	dim3 g = plane_grid(L.nx, L.ny, L.nz);
+	long nb = (long)g.x * g.y;
+	if ((size_t)(2 * nb + 2) * sizeof(double) > I3G_SCRATCH_BYTES) return I3G_ECAP;
+	double *part = (double *)scratch_dev[c->dev];
+	hipStream_t st = stream ? (hipStream_t)stream : c->s;
+	double *r = d_r ? d_r : L.r64;
+	hipLaunchKernelGGL((k_residual<double, double, 0>), g, dim3(256), 0, st, view<double>(L), d_x,
+			   (const double *)c->rhs0, r, L.bcid, bcview<double>(L), part);
+	hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, (const double *)part, nb, 2, part + 2 * nb);
+	hipLaunchKernelGGL(k_sqrt2, dim3(1), dim3(64), 0, st, (const double *)(part + 2 * nb), d_norms);
+	HCHK(hipGetLastError());
 	return I3G_OK;
 }
 

@@ -135,6 +135,20 @@ int i3g_stokes_residual(i3g_ctx *ctx, const double *P, const double *vx,
 			const double *vy, const double *vz, double *rP,
 			double *rvx, double *rvy, double *rvz, double *norms);
 
+/* i3g_stokes_residual without any allocation, on device buffers: d_x is
+ * the level-0 vector (4 planes P, vx, vy, vz of nodenum doubles), d_r the
+ * residual out (same layout; NULL: the context's work vector), d_norms 2
+ * doubles out ([0] ||r||_w, [1] ||b||_w).  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Its partial sums live in a static
+ * device array of 4 MiB per GPU, so the level-0 grid may have at most
+ * 262143 blocks of 256 nodes (about 67 million nodes; larger grids return
+ * I3G_ECAP).  Enqueued on `stream` (a hipStream_t; NULL = the context's
+ * stream) and not synchronised.  Calls that overlap on one GPU race on that
+ * array. */
+int i3g_stokes_residual_noalloc(i3g_ctx *ctx, const double *d_x, double *d_r,
+				double *d_norms, void *stream);
+
 /* Level transfers of the V-cycle, in FP64, on host planes (P, vx, vy, vz;
  * SoA, one plane per unknown kind, level sizes).  `level` is the coarse
  * level (1 <= level < levels), as the reference's argument.
