@@ -65,9 +65,13 @@ def test_solve_matches_reference_converged(gpu, which, n, prec):
     N = n ** 3
     z = np.zeros(N)
     S.set_solution(z, z, z, z)
-    st = S.solve(precision=prec, rtol=1e-9 if which == 0 else 1e-10, max_outer=40, max_vcycles=4000)
+    # the subduction system is ill-conditioned (10^6 contrast): a relative
+    # residual of 5e-11 still leaves the pressure ~1e-6 off, so it is
+    # solved to 1e-12
+    rtol = 1e-9 if which == 0 else 1e-12
+    st = S.solve(precision=prec, rtol=rtol, max_outer=40, max_vcycles=6000)
     print("  outer", st.outer_iters, "vcycles", st.vcycles, "rel", st.rel_res, "s", st.seconds, flush=True)
-    assert st.converged and st.rel_res <= (1e-9 if which == 0 else 1e-10)
+    assert st.converged and st.rel_res <= rtol
     P, vx, vy, vz = S.get_solution()
     sol = g["sol"]
     ex = g["exists"]
