@@ -228,6 +228,55 @@ def other_paths(I, np, cfg257):
     return out
 
 
+def time_loop(I, np, steps=3, cap_s=90.0, nx=257, ny=257, nz=129, per_axis=2, learned=None):
+    """The resident time-step loop on the 257x257x129 subduction layout with
+    per_axis^3 markers per cell (2: 67M markers) and Arrhenius viscosity
+    (E = 300 kJ/mol, capped 1e18-1e25 Pa s): vpiterate (mixed-precision
+    solve from the previous step's solution), titerate, movemark (RK4),
+    ronurecalc (re-sort + marker -> node) per step, every field on the
+    device.  Runs `steps` steps or until cap_s has passed."""
+    t = time.time()
+    cfg = I.make_config(2, nx, ny, nz, seed=4)
+    x, y, z, tk, mt = I.make_markers(cfg, per_axis, seed=4)
+    eta_ref = [0, 0, 1e20, 1e23, 1e23, 1e17]
+    rocks = I.arrhenius_rocks(eta_ref, ro=[3300, 3300, 3250, 3300, 3350, 3250])
+    dx = cfg["gx"][1] - cfg["gx"][0]
+    p = I.model_params(rtol=1e-8, maxtmstep=3e12, maxxystep=0.5 * dx, heatdif=0.0, nubeg=cfg["nubeg"],
+                       nuend=cfg["nuend"])
+    net = None
+    if learned is not None:
+        p.learned, p.pscale, p.vscale = 1, learned.pscale, learned.vscale
+        net = learned.net
+    M = I.Model(cfg, len(x))
+    M.set_markers(x, y, z, tk, mt, rocks)
+    nmk = len(x)
+    del x, y, z, tk, mt
+    t0 = time.time()
+    M.init(p)
+    setup_s = time.time() - t
+    rows = []
+    t1 = time.time()
+    while len(rows) < steps and time.time() - t1 < cap_s:
+        st = M.step(p, net)
+        rows.append(st.as_dict())
+        print(f"# loop step {len(rows) - 1}: {st.seconds:.3f}s stokes {st.t_stokes:.3f}s ({st.vcycles} V-cycles, "
+              f"rel {st.rel_res:.1e}) heat {st.t_heat:.3f}s rk4 {st.t_rk4:.3f}s sort {st.t_sort:.3f}s "
+              f"transfer {st.t_transfer:.3f}s dt {st.timestep:.2e}", file=sys.stderr, flush=True)
+    M.close()
+    mean = lambda k: float(np.mean([r[k] for r in rows])) if rows else None
+    sec = mean("seconds")
+    # per-stage algorithmic HBM bytes: markers M * bytes, nodes N * bytes
+    N = nx * ny * nz
+    stage_bytes = dict(t_rk4=nmk * 24.0 + N * 12.0, t_sort=nmk * (8.0 + 2 * 29), t_transfer=nmk * 40.0 + N * 44.0)
+    gbs = {k: round(b / mean(k) / 1e9, 1) for k, b in stage_bytes.items() if rows and mean(k) > 0}
+    return dict(workload=f"time loop {nx}x{ny}x{nz} subduction, Arrhenius E=300 kJ/mol, {nmk} markers",
+                steps=len(rows), steps_per_s=(round(1.0 / sec, 4) if sec else None), mean_step_s=sec,
+                stage_mean_s={k: mean(k) for k in ("t_stokes", "t_heat", "t_mktk", "t_rk4", "t_sort", "t_transfer")},
+                stage_gbs_algorithmic=gbs, vcycles=[r["vcycles"] for r in rows],
+                rel_res=[r["rel_res"] for r in rows], timestep=[r["timestep"] for r in rows],
+                setup_s=round(setup_s, 2), init_transfer_s=round(t1 - t0, 3))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +289,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="mixed precision from a zero guess, no network")
     ap.add_argument("--no-extras", action="store_true", help="skip the marker and heat measurements")
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--loop-steps", type=int, default=3,
+                    help="steps of the 257x257x129 time loop reported under time_loop (0: skip)")
+    ap.add_argument("--loop-only", action="store_true", help="only the time loop, printed as JSON, no metric")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB three times with a plain device copy (HBM counter calibration)")
     ap.add_argument("--profile-vcycles", type=int, default=0,
@@ -278,6 +330,9 @@ def main():
     print(f"# config built in {time.time() - t:.1f}s multinum={cfg['multinum']}", file=sys.stderr, flush=True)
     if a.other_paths_only:
         print(json.dumps({"other_paths": other_paths(I, np, cfg)}))
+        return
+    if a.loop_only:
+        print(json.dumps({"time_loop": time_loop(I, np, steps=a.loop_steps or 20, cap_s=1e9)}))
         return
     S = I.Stokes(cfg)
     S.set_coefficients()
@@ -393,6 +448,11 @@ def main():
                                                    1)))
 
     extras = None if a.no_extras else other_paths(I, np, cfg)
+    loop = None
+    if a.loop_steps > 0 and not a.no_extras:
+        S.close()
+        S = None
+        loop = time_loop(I, np, steps=a.loop_steps)
 
     cpu = None
     if not a.no_cpu:
@@ -424,12 +484,13 @@ def main():
         "field_error_vs_fp64": field_err, "gpu_fp64_rel_residual": gpu_rel,
         "learned": (dict(learned, weights=os.path.relpath(wpath, ROOT)) if ws is not None else None),
         "native_norms": native,
-        "other_paths": extras,
+        "other_paths": extras, "time_loop": loop,
         "roofline": roof, "cpu_baseline": cpu,
         "kernels": {k: dict(avg_ms=round(v["avg_ms"], 5), calls=v["calls"]) for k, v in kst.items()},
     }
     print(json.dumps(out))
-    S.close()
+    if S is not None:
+        S.close()
 
 
 if __name__ == "__main__":

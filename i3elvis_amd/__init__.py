@@ -258,6 +258,14 @@ class Stokes:
                                        C.c_double(p1koef)), "stokes_prolong")
         return out
 
+    def fields(self):
+        """(vx, vy, vz float32, pr float64, vmax[3]) of the resident solution
+        (i3g_stokes_fields)"""
+        out = [np.zeros(self.N, np.float32) for _ in range(3)] + [np.zeros(self.N)]
+        vmax = np.zeros(3)
+        check(lib().i3g_stokes_fields(self.ctx, *[_p(a) for a in out], _p(vmax)), "stokes_fields")
+        return out[0], out[1], out[2], out[3], vmax
+
     def smooth_phase(self, level, phase, precision, x, b, v0koef, p0koef):
         """one colour phase of the smoother on planes [P, vx, vy, vz] of
         `level` (i3g_stokes_smooth_phase); returns the new planes"""
@@ -595,3 +603,227 @@ def unflatten_params(p):
         out.append(np.asarray(p[o:o + 27 * co * ci]).reshape(27, co, ci)); o += 27 * co * ci
         out.append(np.asarray(p[o:o + co])); o += co
     return out
+
+
+# ---- the time-step loop ------------------------------------------------------
+
+def make_markers(cfg, per_axis, seed=4):
+    """seeded markers of configuration cfg['which'] (i3g_gen_markers):
+    x, y, z, tk (float32) and type (int8)"""
+    args = (C.c_int(cfg["which"]), C.c_long(cfg["nx"]), C.c_long(cfg["ny"]), C.c_long(cfg["nz"]),
+            C.c_ulonglong(seed), _p(cfg["gx"]), _p(cfg["gy"]), _p(cfg["gz"]), C.c_int(per_axis))
+    lib().i3g_gen_markers.restype = C.c_long
+    n = check(lib().i3g_gen_markers(*args, None, None, None, None, None), "gen_markers")
+    out = [np.zeros(n, np.float32) for _ in range(4)] + [np.zeros(n, np.int8)]
+    check(lib().i3g_gen_markers(*args, *[_p(a) for a in out]), "gen_markers")
+    return out
+
+
+def arrhenius_rocks(eta_ref, t_ref=1600.0, e_act=3.0e5, n0=1e18, n1=1e25, ro=None, bro=3e-5, aro=1e-3, cp=1000.0,
+                    kt=3.0, ht=0.0):
+    """Rock table (types 0..len-1) with Arrhenius viscosity
+    eta = A exp(E / (R T)) (viscalc branch B, mark3mg.c:1278-1299), A chosen
+    so that eta(t_ref) = eta_ref[type], capped to [n0, n1]"""
+    rocks = []
+    for t, e in enumerate(eta_ref):
+        a = e / np.exp(e_act / (8.314 * t_ref)) if e > 0 else 0.0
+        rocks.append(dict(nu0=a, dh=e_act if e > 0 else 0.0, dv=0.0, n0=n0, n1=n1,
+                          ro=(ro[t] if ro is not None else 3300.0), bro=bro, aro=aro, cp=cp, kt=kt, kf=0.0, kp=0.0,
+                          ht=ht))
+    return rocks
+
+
+class ModelParams(C.Structure):
+    _fields_ = [("mk", MkParams), ("stokes", StokesParams), ("heat", HeatParams), ("maxtmstep", C.c_double),
+                ("maxxystep", C.c_double), ("heatdif", C.c_double), ("pscale", C.c_double), ("vscale", C.c_double),
+                ("learned", C.c_int)]
+
+
+class ModelStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("t_stokes", C.c_double), ("t_heat", C.c_double), ("t_mktk", C.c_double),
+                ("t_rk4", C.c_double), ("t_sort", C.c_double), ("t_transfer", C.c_double), ("vcycles", C.c_int),
+                ("outer", C.c_int), ("rel_res", C.c_double), ("timestep", C.c_double), ("timesum", C.c_double),
+                ("learned_taken", C.c_int), ("heat_substeps", C.c_int), ("heat_sweeps", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def model_params(rtol=1e-8, maxtmstep=1e12, maxxystep=None, heatdif=0.0, nubeg=1e18, nuend=1e25, densimod=1,
+                 heatmin=1e-6, learned=False, pscale=1.0, vscale=1.0, **stokes_kw):
+    """ModelParams with the library's defaults for the solve and the heat
+    step"""
+    p = ModelParams()
+    p.mk.nubeg, p.mk.nuend, p.mk.nukoef, p.mk.densimod = nubeg, nuend, 0.0, densimod
+    p.mk.timestep, p.mk.vdeep, p.mk.nudeep, p.mk.viscmod = 0.0, 1e30, 1e30, 0
+    lib().i3g_stokes_default_params(C.byref(p.stokes))
+    p.stokes.rtol = rtol
+    for k, v in stokes_kw.items():
+        setattr(p.stokes, k, v)
+    lib().i3g_heat_default_params(C.byref(p.heat))
+    p.heat.heatmin = heatmin
+    p.maxtmstep = maxtmstep
+    p.maxxystep = maxxystep if maxxystep is not None else 0.0
+    p.heatdif = heatdif
+    p.pscale, p.vscale, p.learned = pscale, vscale, 1 if learned else 0
+    return p
+
+
+class Model:
+    """The resident time-step loop (i3g_model_*): vpiterate, titerate,
+    movemark, ronurecalc per step on one GPU."""
+
+    def __init__(self, cfg, capacity):
+        self.cfg = cfg
+        self.N = cfg["nx"] * cfg["ny"] * cfg["nz"]
+        self._keep = [np.ascontiguousarray(cfg[k]) for k in ("gx", "gy", "gz", "bondm", "bondv", "bondn")]
+        gx, gy, gz, bm, bv, bn = self._keep
+        s = Setup(cfg["nx"], cfg["ny"], cfg["nz"], _p(gx), _p(gy), _p(gz), cfg["multinum"],
+                  _p(bm), _p(bv), _p(bn), len(bn))
+        self.h = C.c_void_p()
+        check(lib().i3g_model_create(C.byref(s), C.c_long(capacity), C.byref(self.h)), "model_create")
+        phys = np.array([cfg["g"][0], cfg["g"][1], cfg["g"][2], cfg["nubeg"], cfg["nuend"], cfg["viscmod"]])
+        check(lib().i3g_model_set_physics(self.h, _p(phys)), "model_set_physics")
+
+    def close(self):
+        if self.h:
+            lib().i3g_model_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_markers(self, x, y, z, tk, mtype, rocks):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (x, y, z, tk)]
+        mt = np.ascontiguousarray(mtype, np.int8)
+        ra = rocks_array(rocks)
+        check(lib().i3g_model_set_markers(self.h, C.c_long(len(arrs[0])), *[_p(a) for a in arrs], _p(mt), ra,
+                                          C.c_int(len(rocks))), "model_set_markers")
+
+    def init(self, params, pr=None):
+        pp = None if pr is None else _p(np.ascontiguousarray(pr, np.float64))
+        check(lib().i3g_model_init(self.h, C.byref(params), pp), "model_init")
+
+    def step(self, params, net=None):
+        st = ModelStats()
+        check(lib().i3g_model_step(self.h, C.byref(params), net.h if net is not None else None, C.byref(st)),
+              "model_step")
+        return st
+
+    def get(self):
+        N = self.N
+        out = dict(vx=np.zeros(N, np.float32), vy=np.zeros(N, np.float32), vz=np.zeros(N, np.float32),
+                   pr=np.zeros(N), tk=np.zeros(N, np.float32), nu=np.zeros(N, np.float32))
+        check(lib().i3g_model_get(self.h, *[_p(out[k]) for k in ("vx", "vy", "vz", "pr", "tk", "nu")]), "model_get")
+        return out
+
+    def stokes_ctx(self):
+        lib().i3g_model_stokes.restype = C.c_void_p
+        return lib().i3g_model_stokes(self.h)
+
+    def markers_handle(self):
+        lib().i3g_model_markers.restype = C.c_void_p
+        return lib().i3g_model_markers(self.h)
+
+
+# ---- run files of the reference (host only) ---------------------------------
+
+_MODE_D1 = ("tkpor", "zmpor", "vyfluid", "vymelt", "dmwamin", "tdeep", "dtdeep", "drdeep", "zdeep", "vdeep",
+            "nudeep")
+_MODE_D2 = ("dxwater", "dywater", "dzwater", "dserp", "yserp", "tserp", "lambmlt", "ydike", "dydike", "nusseltno")
+_MODE_D3 = ("circdepth", "circtemp", "healrate", "meltmin", "maxwater", "maxpmelt", "teclmin", "teclmax", "xvolc",
+            "xtkplut")
+
+
+class ModeC(C.Structure):
+    _fields_ = ([("fl0cap", C.c_int), ("fl0out", C.c_void_p), ("fl0otp", C.c_void_p), ("fl0cyc", C.c_void_p),
+                 ("fl0stp", C.c_void_p), ("fln3", C.c_int), ("fl1in", C.c_char * 64), ("fl1itp", C.c_int),
+                 ("fl0num", C.c_int)]
+                + [(n, C.c_int) for n in ("printmod", "movemod", "tempmod", "markmod", "gridmod", "outgrid",
+                                          "densimod", "filesjob")]
+                + [(n, C.c_double) for n in ("stp100", "eroslev", "sedilev", "waterlev")]
+                + [("cyc1max", C.c_int), ("multinum", C.c_int), ("multicyc", C.c_int),
+                   ("multinnn", (C.c_int * MAXLEV) * 2)]
+                + [(n, C.c_double) for n in ("DIVVMIN", "STOKSMIN", "DIVVMAX", "STOKSMAX")]
+                + [(n, C.c_double) for n in ("p0koef", "p1koef", "p2koef", "v0koef", "v1koef", "nubeg", "nuend",
+                                             "nukoef")]
+                + [("viscmod", C.c_int), ("spheryn", C.c_int)]
+                + [("cyc2max", C.c_int), ("multinumt", C.c_int), ("multicyct", C.c_int),
+                   ("multittt", (C.c_int * MAXLEV) * 2)]
+                + [(n, C.c_double) for n in ("HEATMIN", "t0koef", "t1koef", "heatdif")]
+                + [("frictyn", C.c_int), ("adiabyn", C.c_int)]
+                + [(n, C.c_double) for n in _MODE_D1 + _MODE_D2 + _MODE_D3]
+                + [(n, C.c_double) for n in ("timebeg", "timeend", "tempbeg", "convend")])
+
+
+def read_mode(mode_path, file_path=None, cap=5000):
+    """The parameter deck (i3g_mode_read): a dict of the reference's
+    loadconf variables plus the output-file table"""
+    m = ModeC()
+    out = np.zeros((cap, 64), np.uint8)
+    otp = np.zeros(cap, np.int32)
+    cyc = np.zeros((cap, 4), np.int32)
+    stp = np.zeros((cap, 11))
+    m.fl0cap = cap
+    m.fl0out, m.fl0otp, m.fl0cyc, m.fl0stp = (a.ctypes.data for a in (out, otp, cyc, stp))
+    check(lib().i3g_mode_read(str(mode_path).encode(), None if file_path is None else str(file_path).encode(),
+                              C.byref(m)), "mode_read")
+    k = m.fl0num
+    d = {f: getattr(m, f) for f, _ in ModeC._fields_ if f not in ("fl0cap", "fl0out", "fl0otp", "fl0cyc", "fl0stp",
+                                                                   "multinnn", "multittt", "fl1in")}
+    d["fl1in"] = m.fl1in.decode()
+    d["fl0out"] = [bytes(out[i]).split(b"\0", 1)[0].decode() for i in range(k)]
+    d["fl0otp"], d["fl0cyc"], d["fl0stp"] = otp[:k].copy(), cyc[:k].copy(), stp[:k].copy()
+    d["multinnn"] = np.array([[m.multinnn[a][b] for b in range(m.multinum + 1)] for a in range(2)])
+    d["multittt"] = np.array([[m.multittt[a][b] for b in range(m.multinumt + 1)] for a in range(2)])
+    return d
+
+
+class PrnC(C.Structure):
+    _fields_ = ([(n, C.c_long) for n in ("xnumx", "ynumy", "znumz", "mnumx", "mnumy", "mnumz")]
+                + [(n, C.c_double) for n in ("xsize", "ysize", "zsize")]
+                + [(n, C.c_long) for n in ("pxinit", "pyinit", "pzinit")]
+                + [(n, C.c_double) for n in ("pinit", "gxkoef", "gykoef", "gzkoef")]
+                + [("rocknum", C.c_int), ("bondnum", C.c_long), ("marknum", C.c_long), ("n0", C.c_int)]
+                + [(n, C.c_double) for n in ("timesum_years", "gridcur", "gridtot")]
+                + [(n, C.c_void_p) for n in ("rock", "node", "gx", "gy", "gz", "bond_at", "bond_rec",
+                                             "bond_partner", "bond_v", "mark", "markt")])
+
+
+PRN_ARRAYS = ("rock", "node", "gx", "gy", "gz", "bond_at", "bond_rec", "bond_partner", "bond_v", "mark", "markt")
+
+
+def read_prn(path):
+    """The .prn state file (i3g_prn_read): header scalars and arrays (rock
+    24 x rocknum, node 16 x nodenum, mark 12 x marknum, bond_v bondnum x 2)"""
+    h = PrnC()
+    check(lib().i3g_prn_read_header(str(path).encode(), C.byref(h)), "prn_read_header")
+    N = h.xnumx * h.ynumy * h.znumz
+    a = dict(rock=np.zeros((24, h.rocknum)), node=np.zeros((16, N), np.float32), gx=np.zeros(h.xnumx, np.float32),
+             gy=np.zeros(h.ynumy, np.float32), gz=np.zeros(h.znumz, np.float32),
+             bond_at=np.zeros(h.bondnum, np.int64), bond_rec=np.zeros(h.bondnum, np.int64),
+             bond_partner=np.zeros(h.bondnum, np.int64), bond_v=np.zeros((h.bondnum, 2), np.float32),
+             mark=np.zeros((12, h.marknum), np.float32), markt=np.zeros(h.marknum, np.int8))
+    for k in PRN_ARRAYS:
+        setattr(h, k, a[k].ctypes.data if a[k].size else None)
+    check(lib().i3g_prn_read(str(path).encode(), C.byref(h)), "prn_read")
+    d = {f: getattr(h, f) for f, _ in PrnC._fields_ if f not in PRN_ARRAYS}
+    d.update(a)
+    return d
+
+
+def write_prn(path, d):
+    """i3g_prn_write from a dict shaped as read_prn returns"""
+    h = PrnC()
+    for f, _ in PrnC._fields_:
+        if f not in PRN_ARRAYS:
+            setattr(h, f, d[f])
+    dt = dict(rock=np.float64, node=np.float32, gx=np.float32, gy=np.float32, gz=np.float32, bond_at=np.int64,
+              bond_rec=np.int64, bond_partner=np.int64, bond_v=np.float32, mark=np.float32, markt=np.int8)
+    keep = {k: np.ascontiguousarray(d[k], dt[k]) for k in PRN_ARRAYS}
+    for k in PRN_ARRAYS:
+        setattr(h, k, keep[k].ctypes.data if keep[k].size else None)
+    check(lib().i3g_prn_write(str(path).encode(), C.byref(h)), "prn_write")

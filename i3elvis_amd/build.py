@@ -15,7 +15,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 HIP_SOURCES = ["i3g_stokes.hip", "i3g_markers.hip", "i3g_heat.hip", "i3g_net.hip"]
-C_SOURCES = ["i3g_host.c"]
+C_SOURCES = ["i3g_host.c", "i3g_model.c", "i3g_io.c"]
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
@@ -34,14 +34,16 @@ def build(verbose=False):
     objs = []
     for s in C_SOURCES:
         o = os.path.join(BUILD, s + ".o")
-        _run(["gcc", "-O2", "-fPIC", "-fopenmp", "-Wall", "-c", os.path.join(CSRC, s), "-o", o])
+        _run(["gcc", "-O2", "-fPIC", "-fopenmp", "-Wall", "-D__HIP_PLATFORM_AMD__",
+              "-I" + os.path.join(ROCM, "include"), "-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     for s in HIP_SOURCES:
         o = os.path.join(BUILD, s + ".o")
         _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-fopenmp", "-lm"])
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs +
+         ["-fopenmp", "-lm"])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

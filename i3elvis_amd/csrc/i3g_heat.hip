@@ -197,6 +197,39 @@ __global__ void k_heat_source(int nx, int ny, int nz, const double *gx, const do
 	}
 }
 
+/* rocp = ro * cp (float, as the reference's product of two float fields),
+ * cp arriving in rocp */
+__global__ void k_rocp(long N, const float *__restrict__ ro, float *__restrict__ rocp)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < N) rocp[t] = ro[t] * rocp[t];
+}
+
+/* block maxima of |H / (ro cp)| over the non-boundary nodes (the step
+ * limit of heat3mg.c:327-368 without adiabatic and shear terms) */
+__global__ void __launch_bounds__(NB) k_rate_part(int nx, int ny, int nz, const float *__restrict__ ht,
+						  const float *__restrict__ rocp, const int *__restrict__ bp, double *part)
+{
+	__shared__ double sh[NB / 64];
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int m3 = blockIdx.y;
+	int m1 = t / ny, m2 = t - m1 * ny;
+	double mx = 0;
+	if (m1 < nx) {
+		long v3 = (long)m3 * nx * ny + (long)m1 * ny + m2;
+		if (bp[v3] == -2) mx = fabs((double)ht[v3] / (double)rocp[v3]);
+	}
+	for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
+	int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (lane == 0) sh[w] = mx;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double a = 0;
+		for (int k = 0; k < NB / 64; k++) a = fmax(a, sh[k]);
+		part[(long)blockIdx.y * gridDim.x + blockIdx.x] = a;
+	}
+}
+
 __global__ void k_heat_max(long nblk, const double *part, double *res)
 {
 	__shared__ double sh[NB];
@@ -457,6 +490,7 @@ extern "C" int i3g_heat_create(const i3g_setup *su, i3g_heat **out)
 	h->bp = halloc<int>(N, &err); h->bc = halloc<float>(N, &err); h->bk = halloc<float>(N, &err);
 	h->blist = halloc<int>(flat.size(), &err);
 	h->kt = halloc<float>(N, &err); h->rocp = halloc<float>(N, &err); h->ht = halloc<float>(N, &err);
+	h->ro = halloc<float>(N, &err);
 	h->tk0 = halloc<float>(N, &err);
 	h->T = halloc<double>(N, &err); h->b = halloc<double>(N, &err); h->diag = halloc<double>(N, &err);
 	long nblk = ((long)su->nx * su->ny + NB - 1) / NB * su->nz;
@@ -524,7 +558,7 @@ extern "C" int i3g_heat_set_sources(i3g_heat *h, int adiabyn, int frictyn, const
 	int err = 0;
 	if (!h->vx) {
 		h->vx = halloc<float>(N, &err); h->vy = halloc<float>(N, &err); h->vz = halloc<float>(N, &err);
-		h->et = halloc<float>(N, &err); h->ro = halloc<float>(N, &err); h->se = halloc<float>(12 * N, &err);
+		h->et = halloc<float>(N, &err); h->se = halloc<float>(12 * N, &err);
 		h->pr = halloc<double>(N, &err); h->src = halloc<float>(N, &err);
 		if (err) { h->adiabyn = h->frictyn = 0; return err; }
 	}
@@ -559,9 +593,7 @@ extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const flo
 	i3g_heat_stats s{};
 	/* temperature step limit (heat3mg.c:327-368) from max |src/(ro cp)|
 	 * over the non-boundary nodes; src = H, or computed on the device with
-	 * the adiabatic/shear terms */
-	std::vector<float> rocp(N);
-	for (long i = 0; i < N; i++) rocp[i] = ro[i] * cp[i];
+	 * the adiabatic/shear terms.  The fields may be host or device arrays. */
 	bool terms = h->adiabyn || h->frictyn;
 	dim3 gg((unsigned)(((long)h->nx * h->ny + NB - 1) / NB), (unsigned)h->nz);
 	long nblk = (long)gg.x * gg.y;
@@ -569,26 +601,25 @@ extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const flo
 	int swchunks = (int)(((long)h->nx * ((h->ny + 1) / 2) + NB - 1) / NB);
 	dim3 gsw((unsigned)(8 * (((long)swchunks * h->nz + 7) / 8)));
 	double maxrate = 0;
+	TCHK(hipMemcpyAsync(h->rocp, cp, N * sizeof(float), hipMemcpyDefault, h->s));
+	TCHK(hipMemcpyAsync(h->ro, ro, N * sizeof(float), hipMemcpyDefault, h->s));
+	TCHK(hipMemcpyAsync(h->ht, ht, N * sizeof(float), hipMemcpyDefault, h->s));
+	TCHK(hipMemcpyAsync(h->tk0, tk, N * sizeof(float), hipMemcpyDefault, h->s));
+	TCHK(hipMemcpyAsync(h->kt, kt, N * sizeof(float), hipMemcpyDefault, h->s));
+	hipLaunchKernelGGL(k_rocp, dim3((unsigned)((N + NB - 1) / NB)), dim3(NB), 0, h->s, N, (const float *)h->ro, h->rocp);
 	if (terms) {
-		TCHK(hipMemcpyAsync(h->rocp, rocp.data(), N * sizeof(float), hipMemcpyHostToDevice, h->s));
-		TCHK(hipMemcpyAsync(h->ht, ht, N * sizeof(float), hipMemcpyHostToDevice, h->s));
-		TCHK(hipMemcpyAsync(h->tk0, tk, N * sizeof(float), hipMemcpyHostToDevice, h->s));
-		TCHK(hipMemcpyAsync(h->ro, ro, N * sizeof(float), hipMemcpyHostToDevice, h->s));
 		HeatSrc S{h->adiabyn, h->frictyn, h->gk[0], h->gk[1], h->gk[2], h->ysize, h->stp100, h->ht, h->rocp,
 			  h->ro, h->tk0, h->et, h->vx, h->vy, h->vz, h->se, h->pr};
 		hipLaunchKernelGGL(k_heat_source, gg, dim3(NB), 0, h->s, h->nx, h->ny, h->nz, (const double *)h->gx,
 				   (const double *)h->gy, (const double *)h->gz, (const int *)h->bp, S, h->src, h->part);
-		hipLaunchKernelGGL(k_heat_max, dim3(1), dim3(NB), 0, h->s, nblk, (const double *)h->part, h->res);
-		TCHK(hipGetLastError());
-		TCHK(hipMemcpyAsync(&maxrate, h->res, sizeof(double), hipMemcpyDeviceToHost, h->s));
-		TCHK(hipStreamSynchronize(h->s));
 	} else {
-		for (long i = 0; i < N; i++)
-			if (h->hb[i] == 0) {
-				double r = std::fabs((double)ht[i] / (double)rocp[i]);
-				if (r > maxrate) maxrate = r;
-			}
+		hipLaunchKernelGGL(k_rate_part, gg, dim3(NB), 0, h->s, h->nx, h->ny, h->nz, (const float *)h->ht,
+				   (const float *)h->rocp, (const int *)h->bp, h->part);
 	}
+	hipLaunchKernelGGL(k_heat_max, dim3(1), dim3(NB), 0, h->s, nblk, (const double *)h->part, h->res);
+	TCHK(hipGetLastError());
+	TCHK(hipMemcpyAsync(&maxrate, h->res, sizeof(double), hipMemcpyDeviceToHost, h->s));
+	TCHK(hipStreamSynchronize(h->s));
 	if (p->timestep <= 0) {
 		/* the reference returns without a solve when the step is zero */
 		if (st) *st = s;
@@ -600,10 +631,6 @@ extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const flo
 		if (st) *st = s;
 		return I3G_OK;   /* tktimestep == 0: the reference returns */
 	}
-	TCHK(hipMemcpyAsync(h->kt, kt, N * sizeof(float), hipMemcpyHostToDevice, h->s));
-	TCHK(hipMemcpyAsync(h->rocp, rocp.data(), N * sizeof(float), hipMemcpyHostToDevice, h->s));
-	TCHK(hipMemcpyAsync(h->ht, ht, N * sizeof(float), hipMemcpyHostToDevice, h->s));
-	TCHK(hipMemcpyAsync(h->tk0, tk, N * sizeof(float), hipMemcpyHostToDevice, h->s));
 	int plume = p->timesum > p->timebeg ? (p->timesum >= p->timeend ? 2 : 1) : 0;
 	hipEvent_t e0, e1;
 	TCHK(hipEventCreate(&e0));
@@ -659,7 +686,7 @@ extern "C" int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const flo
 	}
 	TCHK(hipGetLastError());
 	TCHK(hipEventRecord(e1, h->s));
-	TCHK(hipMemcpyAsync(tk, h->tk0, N * sizeof(float), hipMemcpyDeviceToHost, h->s));
+	TCHK(hipMemcpyAsync(tk, h->tk0, N * sizeof(float), hipMemcpyDefault, h->s));
 	TCHK(hipStreamSynchronize(h->s));
 	{
 		float ms = 0;

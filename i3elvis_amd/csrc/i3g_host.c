@@ -329,6 +329,70 @@ int i3g_gen_config(int which, long nx, long ny, long nz, unsigned long long seed
 	return I3G_OK;
 }
 
+/* Markers of the seeded configurations: per_axis^3 markers per cell at
+ * jittered sub-cell centres (the in3mg.c:182-184 analogue: offset
+ * U(-1/2, 1/2) of the marker spacing per axis), ids in cell order (z, x, y
+ * cells, then the sub-cell index).  Rock type from the layout at the
+ * marker: configuration 2 gives 2 asthenosphere, 3 lithosphere, 4 slab,
+ * 5 weak channel; configurations 0 and 1 give 2 background, 3 inclusion /
+ * positive blob anomaly.  Temperature from the layout's profile. */
+long i3g_gen_markers(int which, long nx, long ny, long nz, unsigned long long seed, const double *gx,
+		     const double *gy, const double *gz, int per_axis, float *x, float *y, float *z, float *tk,
+		     signed char *type)
+{
+	cfg_t c;
+	long k;
+	if (which < 0 || which > 2 || nx < 2 || ny < 2 || nz < 2 || per_axis < 1 || !gx || !gy || !gz) return I3G_EINVAL;
+	memset(&c, 0, sizeof(c));
+	c.which = which;
+	c.xs = gx[nx - 1]; c.ys = gy[ny - 1]; c.zs = gz[nz - 1];
+	if (which == 1) {
+		c.nblob = 32;
+		for (k = 0; k < 32; k++) {
+			c.bx[k] = urand(seed, 1, k) * c.xs;
+			c.by[k] = urand(seed, 2, k) * c.ys;
+			c.bz[k] = urand(seed, 3, k) * c.zs;
+			c.bs[k] = 20e3 + 40e3 * urand(seed, 4, k);
+			c.ba[k] = -2.0 + 4.0 * urand(seed, 5, k);
+		}
+	}
+	long pc = (long)per_axis * per_axis * per_axis;
+	long ncell = (nx - 1) * (ny - 1) * (nz - 1);
+	if (!x) return ncell * pc;
+	if (!y || !z || !tk || !type) return I3G_EINVAL;
+#pragma omp parallel for
+	for (k = 0; k < nz - 1; k++) {
+		long i, j, q;
+		for (i = 0; i < nx - 1; i++)
+			for (j = 0; j < ny - 1; j++)
+				for (q = 0; q < pc; q++) {
+					long cell = (k * (nx - 1) + i) * (ny - 1) + j, id = cell * pc + q;
+					int a = (int)(q / (per_axis * per_axis)), b = (int)((q / per_axis) % per_axis),
+					    d = (int)(q % per_axis);
+					double sx = (gx[i + 1] - gx[i]) / per_axis, sy = (gy[j + 1] - gy[j]) / per_axis,
+					       sz = (gz[k + 1] - gz[k]) / per_axis;
+					double px = gx[i] + (a + 0.5 + (urand(seed, 11, id) - 0.5)) * sx;
+					double py = gy[j] + (d + 0.5 + (urand(seed, 12, id) - 0.5)) * sy;
+					double pz = gz[k] + (b + 0.5 + (urand(seed, 13, id) - 0.5)) * sz;
+					double e, r, t;
+					material(&c, px, py, pz, &e, &r, &t);
+					int ty = 2;
+					if (which == 2) {
+						if (e < 1e18) ty = 5;
+						else if (e > 1e22) ty = r > 3325.0 ? 4 : 3;
+					} else if (which == 0) {
+						ty = e > 1e22 ? 3 : 2;
+					} else {
+						ty = r < 3300.0 ? 3 : 2;
+					}
+					x[id] = (float)px; y[id] = (float)py; z[id] = (float)pz;
+					tk[id] = (float)t;
+					type[id] = (signed char)ty;
+				}
+	}
+	return ncell * pc;
+}
+
 const char *i3g_error_string(int code)
 {
 	switch (code) {

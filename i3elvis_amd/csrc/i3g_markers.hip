@@ -1138,11 +1138,11 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 		m->pr = mk_alloc<double>(N, &err); m->fld = mk_alloc<float>(11 * N, &err);
 		if (err) return err;
 	}
-	MCHK(hipMemcpyAsync(m->rocks, rocks, nrock * sizeof(i3g_rock), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->rocks, rocks, nrock * sizeof(i3g_rock), hipMemcpyDefault, m->s));
 	MCHK(hipMemsetAsync(m->bad, 0, sizeof(int), m->s));
-	MCHK(hipMemcpyAsync(m->pr, pr, N * sizeof(double), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->pr, pr, N * sizeof(double), hipMemcpyDefault, m->s));
 	for (int q = 0; q < 11; q++)
-		MCHK(hipMemcpyAsync(m->fld + q * N, host[q], N * sizeof(float), hipMemcpyHostToDevice, m->s));
+		MCHK(hipMemcpyAsync(m->fld + q * N, host[q], N * sizeof(float), hipMemcpyDefault, m->s));
 	mark_begin(m);
 	if (n)
 		hipLaunchKernelGGL(k_marker_props, g1(n), dim3(256), 0, m->s, n, (const float *)m->x, (const float *)m->y,
@@ -1168,7 +1168,7 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 	if (bad) return I3G_EINVAL;
 	mark_read(m);
 	for (int q = 0; q < 11; q++)
-		MCHK(hipMemcpyAsync(host[q], m->fld + q * N, N * sizeof(float), hipMemcpyDeviceToHost, m->s));
+		MCHK(hipMemcpyAsync(host[q], m->fld + q * N, N * sizeof(float), hipMemcpyDefault, m->s));
 	MCHK(hipStreamSynchronize(m->s));
 	return I3G_OK;
 }
@@ -1226,7 +1226,7 @@ extern "C" int i3g_markers_update_tk(i3g_markers *m, const float *tk, const floa
 	}
 	const float *src[5] = {tk, tk2, kt, ro, cp};
 	for (int q = 0; q < 5; q++)
-		MCHK(hipMemcpyAsync(m->tnod + q * N, src[q], N * sizeof(float), hipMemcpyHostToDevice, m->s));
+		MCHK(hipMemcpyAsync(m->tnod + q * N, src[q], N * sizeof(float), hipMemcpyDefault, m->s));
 	const float *f = m->tnod;
 	mark_begin(m);
 	if (n)
@@ -1259,9 +1259,9 @@ extern "C" int i3g_markers_advect_rk4(i3g_markers *m, double dt, const float *vx
 		m->vel = mk_alloc<float>(3 * N, &err);
 		if (err) return err;
 	}
-	MCHK(hipMemcpyAsync(m->vel, vx, N * sizeof(float), hipMemcpyHostToDevice, m->s));
-	MCHK(hipMemcpyAsync(m->vel + N, vy, N * sizeof(float), hipMemcpyHostToDevice, m->s));
-	MCHK(hipMemcpyAsync(m->vel + 2 * N, vz, N * sizeof(float), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->vel, vx, N * sizeof(float), hipMemcpyDefault, m->s));
+	MCHK(hipMemcpyAsync(m->vel + N, vy, N * sizeof(float), hipMemcpyDefault, m->s));
+	MCHK(hipMemcpyAsync(m->vel + 2 * N, vz, N * sizeof(float), hipMemcpyDefault, m->s));
 	mark_begin(m);
 	if (n)
 		hipLaunchKernelGGL(k_rk4, g1(n), dim3(256), lines_bytes(m), m->s, n, m->x, m->y, m->z, m->tk, m->eta,

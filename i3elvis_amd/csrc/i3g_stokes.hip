@@ -311,6 +311,11 @@ struct i3g_ctx {
 	int fused = 0;
 	int ftile[3] = {16, 64, 16};
 	int fused_minz = 64; /* levels with fewer planes keep the phase launches (I3G_FUSED_MINZ) */
+	/* learned start and start combination work buffers (kept across calls) */
+	float *ws_feat = nullptr, *ws_y = nullptr;
+	double *ws_lith = nullptr, *ws_dP = nullptr, *ws_dv = nullptr;
+	double *cb_dirs = nullptr, *cb_ys = nullptr;
+	int cb_k = -1;
 	long reorth = 0;         /* inner Gram-Schmidt second passes (diagnostic) */
 	float *st12 = nullptr;   /* strain rates and stresses, 12 level-0 planes */
 };
@@ -370,6 +375,8 @@ static void flush_timing(i3g_ctx *c)
 /* ------------------------------------------------------------------ */
 /* Setup                                                              */
 /* ------------------------------------------------------------------ */
+
+static inline dim3 lin_grid(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 static inline dim3 plane_grid(int nx, int ny, int nz)
 {
@@ -896,7 +903,9 @@ extern "C" int i3g_destroy(i3g_ctx *c)
 				for (void *p : ps)
 					if (p) hipFree(p);
 			}
-	void *ps[] = {c->rhs0, c->sol, c->ro, c->nxx, c->nxy, c->nxz, c->nyz, c->partials, c->dscal, c->e64, c->st12};
+	void *ps[] = {c->rhs0, c->sol, c->ro, c->nxx, c->nxy, c->nxz, c->nyz, c->partials, c->dscal, c->e64, c->st12,
+		      c->ws_feat, c->ws_y, c->ws_lith,
+		      c->ws_dP, c->ws_dv, c->cb_dirs, c->cb_ys};
 	for (void *p : ps)
 		if (p) hipFree(p);
 	if (c->hscal) hipHostFree(c->hscal);
@@ -914,12 +923,12 @@ extern "C" int i3g_set_coefficients(i3g_ctx *c, const float *nu, const float *nx
 	int viscmod = (int)phys[5];
 	if (viscmod < 0 || viscmod > 2) return I3G_EINVAL;
 	long N0 = c->L[0].N;
-	HCHK(hipMemcpyAsync(c->L[0].nu, nu, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
-	HCHK(hipMemcpyAsync(c->nxx, nxx, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
-	HCHK(hipMemcpyAsync(c->nxy, nxy, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
-	HCHK(hipMemcpyAsync(c->nxz, nxz, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
-	HCHK(hipMemcpyAsync(c->nyz, nyz, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
-	HCHK(hipMemcpyAsync(c->ro, ro, N0 * sizeof(float), hipMemcpyHostToDevice, c->s));
+	HCHK(hipMemcpyAsync(c->L[0].nu, nu, N0 * sizeof(float), hipMemcpyDefault, c->s));
+	HCHK(hipMemcpyAsync(c->nxx, nxx, N0 * sizeof(float), hipMemcpyDefault, c->s));
+	HCHK(hipMemcpyAsync(c->nxy, nxy, N0 * sizeof(float), hipMemcpyDefault, c->s));
+	HCHK(hipMemcpyAsync(c->nxz, nxz, N0 * sizeof(float), hipMemcpyDefault, c->s));
+	HCHK(hipMemcpyAsync(c->nyz, nyz, N0 * sizeof(float), hipMemcpyDefault, c->s));
+	HCHK(hipMemcpyAsync(c->ro, ro, N0 * sizeof(float), hipMemcpyDefault, c->s));
 	/* viscmod selects the arithmetic, geometric or harmonic averages of
 	 * the level-0 stress-point viscosities and of ronurestrict, as in the
 	 * reference; I3G_COARSE_VISC overrides the coarse levels' choice */
@@ -1312,7 +1321,6 @@ extern "C" int i3g_stokes_residual_noalloc(i3g_ctx *c, const double *d_x, double
 /* FGMRES                                                             */
 /* ------------------------------------------------------------------ */
 
-static inline dim3 lin_grid(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 template <typename T> static int ensure_krylov(i3g_ctx *c, int m)
 {
@@ -2029,13 +2037,16 @@ static int combine_core(i3g_ctx *c, int k, const double *const *planes, hipMemcp
 	Level &L = c->L[0];
 	long N = L.N, n4 = 4 * N;
 	int err = 0;
-	double *dirs = k ? dalloc<double>((size_t)k * n4, &err) : nullptr;
-	double *ys = dalloc<double>((size_t)(k + 1) * n4, &err);
-	if (err) {
-		if (dirs) hipFree(dirs);
-		if (ys) hipFree(ys);
-		return err;
+	if (c->cb_k < k) {
+		/* grown once to the largest k used (the learned start uses 2) */
+		if (c->cb_dirs) hipFree(c->cb_dirs);
+		if (c->cb_ys) hipFree(c->cb_ys);
+		c->cb_dirs = k ? dalloc<double>((size_t)k * n4, &err) : nullptr;
+		c->cb_ys = dalloc<double>((size_t)(k + 1) * n4, &err);
+		c->cb_k = err ? -1 : k;
+		if (err) return err;
 	}
+	double *dirs = c->cb_dirs, *ys = c->cb_ys;
 	int rc = upload_planes(c, planes, c->sol, 0, kind);
 	double nrm[2] = {0, 1}, a[4] = {0, 0, 0, 0};
 	if (!rc) rc = resid64(c, c->sol, L.r64, nrm);
@@ -2102,8 +2113,6 @@ static int combine_core(i3g_ctx *c, int k, const double *const *planes, hipMemcp
 		}
 	}
 	hipError_t e = hipStreamSynchronize(c->s);
-	if (dirs) hipFree(dirs);
-	hipFree(ys);
 	if (rc) return rc;
 	if (e != hipSuccess) return I3G_EDEVICE;
 	for (int i = 0; i < k; i++)
@@ -2176,26 +2185,39 @@ __global__ void k_ws_outputs(long N, const float *__restrict__ y, double ps, dou
 	dv[2 * N + t] = (double)y[4 * t + 2] * vs;
 }
 
-extern "C" int i3g_stokes_learned_start(i3g_ctx *c, i3g_net *net, double pscale, double vscale,
-					const long gauge[3], double *info)
+/* fallback 0: the zero start; 1: the resident iterate (the previous
+ * step's solution in a time loop), kept in e64 while the others are tried */
+extern "C" int i3g_stokes_learned_start_ex(i3g_ctx *c, i3g_net *net, double pscale, double vscale,
+					   const long gauge[3], int fallback, double *info)
 {
-	if (!c || !c->coef_ready || !net || !gauge || !info) return I3G_EINVAL;
+	if (!c || !c->coef_ready || !net || !gauge || !info || fallback < 0 || fallback > 1) return I3G_EINVAL;
 	Level &L = c->L[0];
 	long N = L.N;
 	long g3 = gauge[2] * L.nx * L.ny + gauge[0] * L.ny + gauge[1];
 	if (gauge[0] < 0 || gauge[0] >= L.nx || gauge[1] < 0 || gauge[1] >= L.ny || gauge[2] < 0 || gauge[2] >= L.nz)
 		return I3G_EINVAL;
 	int err = 0;
-	float *feat = dalloc<float>(2 * N, &err), *y = dalloc<float>(4 * N, &err);
-	double *lith = dalloc<double>(N + 1, &err), *dP = dalloc<double>(N, &err), *dv = dalloc<double>(3 * N, &err);
-	int rc = err;
+	if (!c->ws_feat) {
+		c->ws_feat = dalloc<float>(2 * N, &err);
+		c->ws_y = dalloc<float>(4 * N, &err);
+		c->ws_lith = dalloc<double>(N + 1, &err);
+		c->ws_dP = dalloc<double>(N, &err);
+		c->ws_dv = dalloc<double>(3 * N, &err);
+		if (err) return err;
+	}
+	float *feat = c->ws_feat, *y = c->ws_y;
+	double *lith = c->ws_lith, *dP = c->ws_dP, *dv = c->ws_dv;
+	int rc = I3G_OK;
 	double relf = 1, rel_lith = 1, rel = 1, a[2] = {0, 0};
 	int start = 0;
-	if (!rc) {
-		/* the zero start's residual: the fallback */
-		const double *zero[4] = {nullptr, nullptr, nullptr, nullptr};
-		rc = combine_core(c, 0, zero, hipMemcpyDeviceToDevice, nullptr, &relf);
+	const double *prev[4] = {c->e64, c->e64 + N, c->e64 + 2 * N, c->e64 + 3 * N};
+	const double *zero[4] = {nullptr, nullptr, nullptr, nullptr};
+	const double *const *fb = fallback ? prev : zero;
+	if (fallback) {
+		HCHK(hipMemcpyAsync(c->e64, c->sol, 4 * N * sizeof(double), hipMemcpyDeviceToDevice, c->s));
 	}
+	/* the fallback's residual */
+	rc = combine_core(c, 0, fb, hipMemcpyDeviceToDevice, nullptr, &relf);
 	if (!rc) {
 		dim3 lg = lin_grid(N);
 		hipLaunchKernelGGL(k_ws_features, lg, dim3(256), 0, c->s, N, (const float *)L.nu, (const float *)c->ro,
@@ -2219,15 +2241,11 @@ extern "C" int i3g_stokes_learned_start(i3g_ctx *c, i3g_net *net, double pscale,
 		rc = combine_core(c, 2, pl, hipMemcpyDeviceToDevice, a, &rel);
 		start = 1;
 		if (!rc && relf < rel) {
-			const double *zero[4] = {nullptr, nullptr, nullptr, nullptr};
-			rc = combine_core(c, 0, zero, hipMemcpyDeviceToDevice, nullptr, nullptr);
+			rc = combine_core(c, 0, fb, hipMemcpyDeviceToDevice, nullptr, nullptr);
 			start = 0;
 		}
 	}
 	hipStreamSynchronize(c->s);
-	void *ps[] = {feat, y, lith, dP, dv};
-	for (void *p : ps)
-		if (p) hipFree(p);
 	if (rc) return rc;
 	info[0] = a[0];
 	info[1] = a[1];
@@ -2235,6 +2253,80 @@ extern "C" int i3g_stokes_learned_start(i3g_ctx *c, i3g_net *net, double pscale,
 	info[3] = rel_lith;
 	info[4] = relf;
 	info[5] = start;
+	return I3G_OK;
+}
+
+extern "C" int i3g_stokes_learned_start(i3g_ctx *c, i3g_net *net, double pscale, double vscale,
+					const long gauge[3], double *info)
+{
+	return i3g_stokes_learned_start_ex(c, net, pscale, vscale, gauge, 0, info);
+}
+
+/* velocity planes rounded to float and the FP64 pressure of the resident
+ * solution into host or device arrays (any may be NULL), and the largest
+ * |v| of each component over its existing unknowns (vmax may be NULL) */
+__global__ void __launch_bounds__(256) k_fields(const double *__restrict__ sol, long N, const uint8_t *flags,
+						float *vf, double *part)
+{
+	__shared__ double sh[3][4];
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	double m[3] = {0, 0, 0};
+	if (t < N) {
+		unsigned fl = flags[t];
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+			double v = sol[(c + 1) * N + t];
+			vf[c * N + t] = (float)v;
+			if (fl & FL_EX(c + 1)) m[c] = fabs((double)(float)v);
+		}
+	}
+#pragma unroll
+	for (int c = 0; c < 3; c++) {
+		for (int o = 32; o > 0; o >>= 1) m[c] = fmax(m[c], __shfl_down(m[c], o, 64));
+		if ((threadIdx.x & 63) == 0) sh[c][threadIdx.x >> 6] = m[c];
+	}
+	__syncthreads();
+	if (threadIdx.x < 3) {
+		double a = 0;
+		for (int w = 0; w < 4; w++) a = fmax(a, sh[threadIdx.x][w]);
+		part[(long)blockIdx.x * 3 + threadIdx.x] = a;
+	}
+}
+
+__global__ void k_fields_max(const double *part, long nb, double *out)
+{
+	if (threadIdx.x < 3) {
+		double a = 0;
+		for (long b = 0; b < nb; b++) a = fmax(a, part[b * 3 + threadIdx.x]);
+		out[threadIdx.x] = a;
+	}
+}
+
+extern "C" int i3g_stokes_fields(i3g_ctx *c, float *vx, float *vy, float *vz, double *pr, double *vmax)
+{
+	if (!c) return I3G_EINVAL;
+	Level &L = c->L[0];
+	long N = L.N;
+	long nb = (N + 255) / 256;
+	if (3 * nb > 2 * c->npart) {
+		hipFree(c->partials);
+		int err = 0;
+		c->npart = (3 * nb + 1) / 2;
+		c->partials = dalloc<double>(2 * c->npart, &err);
+		if (err) return err;
+	}
+	hipLaunchKernelGGL(k_fields, dim3((unsigned)nb), dim3(256), 0, c->s, (const double *)c->sol, N,
+			   (const uint8_t *)L.flags, L.r32, c->partials);
+	hipLaunchKernelGGL(k_fields_max, dim3(1), dim3(64), 0, c->s, (const double *)c->partials, nb, c->dscal);
+	HCHK(hipGetLastError());
+	float *dst[3] = {vx, vy, vz};
+	for (int q = 0; q < 3; q++)
+		if (dst[q]) HCHK(hipMemcpyAsync(dst[q], L.r32 + q * N, N * sizeof(float), hipMemcpyDefault, c->s));
+	if (pr) HCHK(hipMemcpyAsync(pr, c->sol, N * sizeof(double), hipMemcpyDefault, c->s));
+	HCHK(hipMemcpyAsync(c->hscal, c->dscal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+	HCHK(hipStreamSynchronize(c->s));
+	if (vmax)
+		for (int q = 0; q < 3; q++) vmax[q] = c->hscal[q];
 	return I3G_OK;
 }
 

@@ -83,9 +83,10 @@ def label(n, seed, precision=64, rtol=1e-8):
 class WarmStart:
     """A trained network plus its scales."""
 
-    def __init__(self, n, seed=0):
+    def __init__(self, n, seed=0, shape=None):
+        """shape: (nz, nx, ny) node grid when not a cube of n"""
         self.n = n
-        self.net = WarmStartNet(n, n, n, seed=seed)
+        self.net = WarmStartNet(*(shape or (n, n, n)), seed=seed)
         self.vscale = 1.0
         self.pscale = 1.0
 
@@ -93,11 +94,11 @@ class WarmStart:
         np.savez_compressed(path, params=self.net.get_params(), n=self.n, vscale=self.vscale, pscale=self.pscale)
 
     @classmethod
-    def load(cls, path, n=None):
-        """weights do not depend on the grid size: n (8k+1) may differ from
-        the training grid"""
+    def load(cls, path, n=None, shape=None):
+        """weights do not depend on the grid size: n (8k+1) or shape
+        (nz, nx, ny; each 8k+1) may differ from the training grid"""
         d = np.load(path)
-        w = cls(int(d["n"]) if n is None else n)
+        w = cls(int(d["n"]) if n is None else n, shape=shape)
         w.net.set_params(d["params"])
         w.vscale, w.pscale = float(d["vscale"]), float(d["pscale"])
         return w

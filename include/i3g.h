@@ -95,6 +95,94 @@ int i3g_gen_config(int which, long nx, long ny, long nz, unsigned long long seed
 		   float *tk, float *nxx, float *nxy, float *nxz, float *nyz,
 		   double *params);
 
+/* Markers of the seeded configurations: per_axis^3 per cell at jittered
+ * sub-cell centres (seeded), rock type from the layout (configuration 2:
+ * 2 asthenosphere, 3 lithosphere, 4 slab, 5 weak channel), temperature from
+ * its profile.  With x NULL only the count is returned; else the count or
+ * a negative error. */
+long i3g_gen_markers(int which, long nx, long ny, long nz, unsigned long long seed, const double *gx,
+		     const double *gy, const double *gz, int per_axis, float *x, float *y, float *z, float *tk,
+		     signed char *type);
+
+/* ------------------------------------------------------------------ */
+/* Run files of the reference (host only)                              */
+/* ------------------------------------------------------------------ */
+
+/* The parameter deck: mode.t3c, and the run number in file.t3c; replaces
+ * loadconf (load3mg.c:2-180), same token rules (whitespace-separated,
+ * tokens starting with '/' are comments) and the same order and units
+ * (the output files' time step and timebeg/timeend/convend converted from
+ * years to seconds).  The caller provides the output-file table (capacity
+ * fl0cap); fl1in becomes the output file of the run number when file.t3c
+ * names one (fln3 >= 0).  Errors: I3G_EINVAL (missing file, early end,
+ * stp100 <= 0), I3G_ECAP (more output files than fl0cap, multinum or
+ * multinumt >= I3G_MAXLEV).  The reference exits instead; it also writes
+ * stop.yn, which is the caller's business here. */
+typedef struct {
+	/* output-file table, caller-owned, fl0cap rows */
+	int fl0cap;
+	char (*fl0out)[64];
+	int *fl0otp;
+	int (*fl0cyc)[4];
+	double (*fl0stp)[11];
+	/* results */
+	int fln3;
+	char fl1in[64];
+	int fl1itp, fl0num;
+	int printmod, movemod, tempmod, markmod, gridmod, outgrid, densimod, filesjob;
+	double stp100, eroslev, sedilev, waterlev;
+	int cyc1max, multinum, multicyc, multinnn[2][I3G_MAXLEV];
+	double DIVVMIN, STOKSMIN, DIVVMAX, STOKSMAX;
+	double p0koef, p1koef, p2koef, v0koef, v1koef, nubeg, nuend, nukoef;
+	int viscmod, spheryn;
+	int cyc2max, multinumt, multicyct, multittt[2][I3G_MAXLEV];
+	double HEATMIN, t0koef, t1koef, heatdif;
+	int frictyn, adiabyn;
+	double tkpor, zmpor, vyfluid, vymelt, dmwamin, tdeep, dtdeep, drdeep, zdeep, vdeep, nudeep;
+	double dxwater, dywater, dzwater, dserp, yserp, tserp, lambmlt, ydike, dydike, nusseltno;
+	double circdepth, circtemp, healrate, meltmin, maxwater, maxpmelt, teclmin, teclmax, xvolc, xtkplut;
+	double timebeg, timeend, tempbeg, convend;
+} i3g_mode;
+
+int i3g_mode_read(const char *mode_path, const char *file_path, i3g_mode *m);
+
+/* The binary state file (.prn) of loader / saver (load3mg.c:704-1096):
+ * byte sizes of int/long/float/double, the header scalars, the rock table
+ * (24 columns of rocknum doubles: markn0 markn1 marks0 marks1 marknu markdh
+ * markdv markss markmm markll marka0 marka1 markb0 markb1 marke0 marke1
+ * markro markbb markaa markcp markkt markkf markkp markht), 16 float fields
+ * per node (pr vx vy vz ro nu nxx nxy nxz nyz tk cp et kt ht rr), the grid
+ * lines as floats, bondnum BC records (bondm index, record, two floats,
+ * partner) and 12 float fields plus the type byte per marker (markx marky
+ * markz markk markw markd markv marke markex markft markrr markrn, markt).
+ * Arrays are planes: rock[c*rocknum + r], node[c*nodenum + n],
+ * mark[c*marknum + i], bond_v[2*q + 0/1].  i3g_prn_read_header reads the
+ * scalars (for sizing); i3g_prn_read needs the arrays and a header equal to
+ * the file's; i3g_prn_write writes the file byte for byte as saver does.
+ * timesum_years is the file's value (the reference converts it to seconds
+ * on loading). */
+typedef struct {
+	long xnumx, ynumy, znumz, mnumx, mnumy, mnumz;
+	double xsize, ysize, zsize;
+	long pxinit, pyinit, pzinit;
+	double pinit, gxkoef, gykoef, gzkoef;
+	int rocknum;
+	long bondnum, marknum;
+	int n0;
+	double timesum_years, gridcur, gridtot;
+	double *rock;
+	float *node;
+	float *gx, *gy, *gz;
+	long *bond_at, *bond_rec, *bond_partner;
+	float *bond_v;
+	float *mark;
+	signed char *markt;
+} i3g_prn;
+
+int i3g_prn_read_header(const char *path, i3g_prn *p);
+int i3g_prn_read(const char *path, i3g_prn *p);
+int i3g_prn_write(const char *path, const i3g_prn *p);
+
 /* ------------------------------------------------------------------ */
 /* Device context and the Stokes solve                                 */
 /* ------------------------------------------------------------------ */
@@ -238,6 +326,14 @@ int i3g_stokes_solve_resident(i3g_ctx *ctx, const i3g_stokes_params *p,
 			      i3g_stats *st);
 int i3g_stokes_get_solution(i3g_ctx *ctx, double *P, double *vx, double *vy,
 			    double *vz);
+/* The resident solution as the reference's node fields: vx, vy, vz rounded
+ * to float and the FP64 pressure (as vpiterate stores them,
+ * move3mg.c:2248-2308), into host or device arrays (any may be NULL), and
+ * vmax[3] (may be NULL): the largest |v| of each component over its
+ * existing unknowns, from which vpiterate limits the time step
+ * (move3mg.c:2609-2632). */
+int i3g_stokes_fields(i3g_ctx *ctx, float *vx, float *vy, float *vz, double *pr, double *vmax);
+
 /* Sets the resident iterate; unknowns that do not exist on the staggered
  * grid are zeroed and, once coefficients are set, the boundary rows are
  * applied (the guess as the solver starts from it). */
@@ -512,11 +608,75 @@ int i3g_warmstart_infer_device(i3g_net *net, long n, const float *d_in, float *d
  * the combination was taken (0: zero start).  No reference counterpart. */
 int i3g_stokes_learned_start(i3g_ctx *ctx, i3g_net *net, double pscale, double vscale, const long gauge[3],
 			     double *info);
+/* The same with the fallback chosen: 0 the zero start, 1 the resident
+ * iterate as it was before the call (in a time loop the previous step's
+ * solution, vpiterate's own start, move3mg.c:1973-2019); info[4] is then
+ * the resident iterate's residual and info[5] = 0 when it was kept. */
+int i3g_stokes_learned_start_ex(i3g_ctx *ctx, i3g_net *net, double pscale, double vscale,
+				const long gauge[3], int fallback, double *info);
 /* gradient of the mean squared error of one sample (grad may be NULL) */
 int i3g_warmstart_grad(i3g_net *net, const float *in, const float *target, double *loss, float *grad);
 /* one Adam step (beta 0.9/0.999, eps 1e-8) on the mean gradient of a batch */
 int i3g_warmstart_train_step(i3g_net *net, int batch, const float *in, const float *target, double lr,
 			     double *loss);
+
+/* ------------------------------------------------------------------ */
+/* The time-step loop                                                  */
+/* ------------------------------------------------------------------ */
+
+/* The per-step sequence of the reference's main loop (i3mg.c:80-151) on one
+ * GPU with every field resident on the device: the Stokes solve
+ * (vpiterate: the marker time step limited to maxxystep / |v|max), the heat
+ * step and the marker temperature update (titerate), RK4 advection
+ * (movemark), and the marker -> node transfer after a cell re-sort
+ * (ronurecalc with tkrecalc).  Node fields move between the stages as
+ * device arrays; the host sees only scalars.  No reference counterpart as
+ * one call: it is the caller i3mg.c restated over the C ABI above. */
+typedef struct i3g_model i3g_model;
+
+typedef struct {
+	i3g_mk_params mk;         /* ronurecalc (its timestep is ignored: the
+				   * marker viscosity is recomputed every step) */
+	i3g_stokes_params stokes; /* vpiterate's solve; the start is the
+				   * previous step's solution */
+	i3g_heat_params heat;     /* titerate; timestep set per step */
+	double maxtmstep;         /* the step before the displacement limit */
+	double maxxystep;         /* largest marker displacement per step */
+	double heatdif;           /* marker subgrid diffusion coefficient */
+	double pscale, vscale;    /* learned start output scales */
+	int learned;              /* 1: learned start (network given) with the
+				   * previous solution as its fallback */
+} i3g_model_params;
+
+typedef struct {
+	double seconds;           /* wall time of the step */
+	double t_stokes, t_heat, t_mktk, t_rk4, t_sort, t_transfer; /* per stage */
+	int vcycles, outer;
+	double rel_res, timestep, timesum;
+	int learned_taken;        /* 1: the learned start was used */
+	int heat_substeps, heat_sweeps;
+} i3g_model_stats;
+
+/* grid, BC tables (velocity and temperature) and marker capacity */
+int i3g_model_create(const i3g_setup *s, long marker_capacity, i3g_model **out);
+int i3g_model_destroy(i3g_model *m);
+/* markers (host arrays; ids 0..n-1 in the given order) and the rock table */
+int i3g_model_set_markers(i3g_model *m, long n, const float *x, const float *y, const float *z,
+			  const float *tk, const signed char *type, const i3g_rock *rocks, int nrock);
+/* the physics of the Stokes operator: gravity and viscosity limits,
+ * phys as i3g_set_coefficients */
+int i3g_model_set_physics(i3g_model *m, const double *phys);
+/* initial marker -> node transfer (needs the pressure for densities; NULL
+ * = zero), as the node fields a reference run starts from */
+int i3g_model_init(i3g_model *m, const i3g_model_params *p, const double *pr);
+int i3g_model_step(i3g_model *m, const i3g_model_params *p, i3g_net *net, i3g_model_stats *st);
+/* node fields (host or device arrays, any may be NULL): velocities and
+ * pressure of the last solve, temperature and viscosity of the last
+ * transfer */
+int i3g_model_get(i3g_model *m, float *vx, float *vy, float *vz, double *pr, float *tk, float *nu);
+/* the model's marker set (for downloads) and Stokes context */
+i3g_markers *i3g_model_markers(i3g_model *m);
+i3g_ctx *i3g_model_stokes(i3g_model *m);
 
 const char *i3g_error_string(int code);
 

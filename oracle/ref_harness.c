@@ -267,3 +267,19 @@ void refh_epscalc1(double x, double y, double z, double *out)
 	for (q = 0; q < 7; q++) out[q] = eps1[4 + q];
 	for (q = 0; q < 7; q++) out[7 + q] = eps1[54 + q];
 }
+
+/* The reference's run-file routines (load3mg.c): loadconf reads file.t3c and
+ * mode.t3c in the current directory (and writes stop.yn there); saver and
+ * loader write / read the .prn state file named by path (saver also writes
+ * file.t3c and reads stop.yn in the current directory). */
+int refh_loadconf(void) { return loadconf(); }
+void refh_saver(const char *path, int f0, int n0)
+{
+	strncpy(fl1out, path, sizeof(fl1out) - 1);
+	saver(f0, n0);
+}
+void refh_loader(const char *path)
+{
+	strncpy(fl1in, path, sizeof(fl1in) - 1);
+	loader();
+}

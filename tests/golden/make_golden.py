@@ -16,6 +16,11 @@ inputs and outputs only.
                     subduction layout) by an equilibrated sparse direct
                     solve, pressure gauge P(2,1,2) = 0 as vpiterate
                     (move3mg.c:2248-2269)
+  g13_*             the reference's run files: g13_mode.t3c / g13_file.t3c (a
+                    deck of this build's own values in the reference's
+                    format) and what the reference's loadconf reads from it;
+                    g13_state.prn written by the reference's saver from the
+                    state in g13_io.npz (load3mg.c:2-180, 704-1096)
   g12_smoother.npz  on the g1 inputs: the reference's own coefficient of each
                     assembled row's unknown (the first row entry, gausmat4's
                     divisor ival1, move3mg.c:3096-3124) and the state after
@@ -166,6 +171,161 @@ def g12(R):
     np.savez_compressed(os.path.join(HERE, "g12_smoother.npz"), diag0=diag, gs_out=gs_out, nd0=nd0,
                         v0koef=v0koef, p0koef=p0koef)
     print("g12_smoother.npz written")
+
+
+DECK = """/i3g_test_deck_in_the_reference_layout_(own_values)
+run/start.prn b
+
+/SAVEFILE_TYPE_cyc0max_maxxystep_maxtkstep_maxtmstep(yr)_seven_more_steps_three_cycle_counts
+out_a.prn b 4 2000.0 50.0 1e4  0.5 1.5 2.5 3.5 4.5 5.5 6.5 7.5 10 20 30
+out_b.prn t 2 1000.0 25.0 2e4  0.1 0.2 0.3 0.4 0.5 0.6 0.7 0.8 11 21 31
+/a_comment_between_entries
+out_c.prn b 1 500.0 10.0 3.5e3 9 8 7 6 5 4 3 2 12 22 32
+~
+/GENERAL
+0-printmod
+1-movemod
+1-tempmod
+1-markmod
+1-gridmod
+2-outgrid
+1-densimod
+1e3-stp100
+4-filestop
+/EROSION
+1.1e4-eroslev 1.2e4-sedilev 1.3e4-waterlev
+/V_P
+100-cyc1max
+1e-11-DIVVMIN
+2e-10-STOKSMIN
+3e-9-DIVVMAX
+4e-8-STOKSMAX
+3-multinum
+4-multicyc
+8 16 16 32
+8 16 16 64
+0.31-p0koef 0.12 0.0 0.71 0.69 1e18 1e25 0.001 1-viscmod 0-spheryn
+/T
+7-cyc2max 1e-5-HEATMIN 0-multinumt 3-multicyct
+30
+31
+0.9 0.8 1e-12-heatdif 1-frictyn 2.7-adiabyn
+/HYDRATION_AND_THE_REST
+373 0.1 -1e-9-vyfluid -2e-9 0.05 1573 0.5 0.2 6.6e5 6.0e5 1e20 1500 1600 1700 0.1 2e4 873 0.9 1.2e4 3e3
+2 1e4 8e2 3e-14 0.01 0.02 0.03 1073 1273 0.4 0.6
+/PLUME
+1e6-timebeg 2e6-timeend 1800-tempbeg 5e6-convend
+"""
+
+
+def g13(R):
+    import shutil
+    import tempfile
+    import ctypes as C
+    with open(os.path.join(HERE, "g13_mode.t3c"), "w") as f:
+        f.write(DECK)
+    with open(os.path.join(HERE, "g13_file.t3c"), "w") as f:
+        f.write("2 \n")
+    tmp = tempfile.mkdtemp()
+    cwd = os.getcwd()
+    try:
+        shutil.copy(os.path.join(HERE, "g13_mode.t3c"), os.path.join(tmp, "mode.t3c"))
+        shutil.copy(os.path.join(HERE, "g13_file.t3c"), os.path.join(tmp, "file.t3c"))
+        os.chdir(tmp)
+        fln3 = R.lib.refh_loadconf()
+        gi = lambda n: C.c_int.in_dll(R.lib, n).value
+        gd = lambda n: C.c_double.in_dll(R.lib, n).value
+        fl0num = gi("fl0num")
+        chars = lambda n, k: bytes((C.c_char * 60).in_dll(R.lib, n)).split(b"\0", 1)[0].decode()
+        fl0out = [bytes(np.ctypeslib.as_array((C.c_char * (60 * 5000)).in_dll(R.lib, "fl0out"))[60 * k:60 * k + 60])
+                  .split(b"\0", 1)[0].decode() for k in range(fl0num)]
+        fl0otp = np.ctypeslib.as_array((C.c_int * 5000).in_dll(R.lib, "fl0otp"))[:fl0num].copy()
+        fl0stp = np.ctypeslib.as_array((C.c_double * (5000 * 20)).in_dll(R.lib, "fl0stp")).reshape(5000, 20)[:fl0num, :11].copy()
+        fl0cyc = np.ctypeslib.as_array((C.c_int * (5000 * 5)).in_dll(R.lib, "fl0cyc")).reshape(5000, 5)[:fl0num, :4].copy()
+        mn, mnt = gi("multinum"), gi("multinumt")
+        mnnn = np.ctypeslib.as_array((C.c_int * (5 * 50)).in_dll(R.lib, "multinnn")).reshape(5, 50)[:2, :mn + 1].copy()
+        mttt = np.ctypeslib.as_array((C.c_int * (2 * 50)).in_dll(R.lib, "multittt")).reshape(2, 50)[:2, :mnt + 1].copy()
+        ints = {n: gi(n) for n in ("fl1itp", "printmod", "movemod", "tempmod", "markmod", "gridmod", "outgrid",
+                                    "densimod", "filesjob", "cyc1max", "multinum", "multicyc", "viscmod", "spheryn",
+                                    "cyc2max", "multinumt", "multicyct", "frictyn", "adiabyn")}
+        dbls = {n: gd(n) for n in ("stp100", "eroslev", "sedilev", "waterlev", "DIVVMIN", "STOKSMIN", "DIVVMAX",
+                                    "STOKSMAX", "p0koef", "p1koef", "p2koef", "v0koef", "v1koef", "nubeg", "nuend",
+                                    "nukoef", "HEATMIN", "t0koef", "t1koef", "heatdif", "tkpor", "zmpor", "vyfluid",
+                                    "vymelt", "dmwamin", "tdeep", "dtdeep", "drdeep", "zdeep", "vdeep", "nudeep",
+                                    "dxwater", "dywater", "dzwater", "dserp", "yserp", "tserp", "lambmlt", "ydike",
+                                    "dydike", "nusseltno", "circdepth", "circtemp", "healrate", "meltmin",
+                                    "maxwater", "maxpmelt", "teclmin", "teclmax", "xvolc", "xtkplut", "timebeg",
+                                    "timeend", "tempbeg", "convend")}
+        fl1in = chars("fl1in", 0)
+        # the .prn state: a small grid, rocks, BC records and markers
+        rng = np.random.default_rng(1313)
+        nx, ny, nz, nr, nm = 5, 4, 3, 3, 7
+        N = nx * ny * nz
+        gxv = np.cumsum(rng.uniform(1, 2, nx)) * 1e4
+        gyv = np.cumsum(rng.uniform(1, 2, ny)) * 1e4
+        gzv = np.cumsum(rng.uniform(1, 2, nz)) * 1e4
+        R.setup_grid(gxv, gyv, gzv, 0)  # gridcheck: nodenum and the level tables
+        st = {}
+        for nme, ct, v in (("xnumx", C.c_long, nx), ("ynumy", C.c_long, ny), ("znumz", C.c_long, nz),
+                           ("mnumx", C.c_long, 2), ("mnumy", C.c_long, 3), ("mnumz", C.c_long, 4),
+                           ("pxinit", C.c_long, 2), ("pyinit", C.c_long, 1), ("pzinit", C.c_long, 2),
+                           ("rocknum", C.c_int, nr), ("marknum", C.c_long, nm)):
+            ct.in_dll(R.lib, nme).value = v
+            st[nme] = v
+        for nme, v in (("xsize", 4e5), ("ysize", 3e5), ("zsize", 2e5), ("pinit", 1e5), ("GXKOEF", 0.1),
+                       ("GYKOEF", 9.81), ("GZKOEF", -0.2), ("timesum", 3.15576e7 * 1234.5), ("gridcur", 0.25),
+                       ("gridtot", 0.75)):
+            C.c_double.in_dll(R.lib, nme).value = v
+            st[nme] = v
+        rock_names = ("markn0", "markn1", "marks0", "marks1", "marknu", "markdh", "markdv", "markss", "markmm",
+                      "markll", "marka0", "marka1", "markb0", "markb1", "marke0", "marke1", "markro", "markbb",
+                      "markaa", "markcp", "markkt", "markkf", "markkp", "markht")
+        rock = rng.uniform(-1, 1, (24, nr)) * 10.0 ** rng.integers(-8, 20, (24, 1))
+        for c, nme in enumerate(rock_names):
+            R.arr(nme, C.c_double, nr)[:] = rock[c]
+        node_names = ("pr", "vx", "vy", "vz", "ro", "nu", "nxx", "nxy", "nxz", "nyz", "tk", "cp", "et", "kt", "ht",
+                      "rr")
+        node = (rng.standard_normal((16, N)) * 10.0 ** rng.integers(-9, 9, (16, 1))).astype(np.float32)
+        node[0] = rng.standard_normal(N) * 1e8  # pr: double in the program, float in the file
+        for c, nme in enumerate(node_names):
+            if nme == "pr":
+                R.f64("pr")[:N] = node[0].astype(np.float64) + 0.123  # rounded to float by saver
+            else:
+                R.f32(nme)[:N] = node[c]
+        node[0] = (node[0].astype(np.float64) + 0.123).astype(np.float32)
+        bm = R.i32("bondm", 10 * N)
+        bm[:] = 0
+        at = np.sort(rng.choice(10 * N, 9, replace=False))
+        bm[at] = np.arange(1, 10)
+        bv = R.arr("bondv1", C.c_float, 2 * 20)
+        bv[:] = rng.standard_normal(40).astype(np.float32)
+        bn = R.i32("bondn1", 20)
+        bn[:] = rng.integers(0, 4 * N, 20)
+        C.c_long.in_dll(R.lib, "bondnum").value = 9
+        mark_names = ("markx", "marky", "markz", "markk", "markw", "markd", "markv", "marke", "markex", "markft",
+                      "markrr", "markrn")
+        mark = rng.standard_normal((12, nm)).astype(np.float32) * 1e3
+        for c, nme in enumerate(mark_names):
+            R.f32(nme, R.MAXMRK)[:nm] = mark[c]
+        mt = rng.integers(0, 30, nm).astype(np.int8)
+        np.ctypeslib.as_array((C.c_char * R.MAXMRK).in_dll(R.lib, "markt")).view(np.int8)[:nm] = mt
+        with open("stop.yn", "w") as f:
+            f.write("n \n")
+        R.lib.refh_saver(b"state.prn", 0, 6)
+        shutil.copy("state.prn", os.path.join(HERE, "g13_state.prn"))
+    finally:
+        os.chdir(cwd)
+        shutil.rmtree(tmp)
+    np.savez_compressed(os.path.join(HERE, "g13_io.npz"), fln3=fln3, fl1in=np.array(fl1in), fl0num=fl0num,
+                        fl0out=np.array(fl0out), fl0otp=fl0otp, fl0stp=fl0stp, fl0cyc=fl0cyc, multinnn=mnnn,
+                        multittt=mttt, int_names=np.array(list(ints)), int_vals=np.array(list(ints.values())),
+                        dbl_names=np.array(list(dbls)), dbl_vals=np.array(list(dbls.values())),
+                        prn_rock=rock, prn_node=node, prn_gx=gxv.astype(np.float32), prn_gy=gyv.astype(np.float32),
+                        prn_gz=gzv.astype(np.float32), prn_bond_at=at, prn_bond_rec=np.arange(1, 10),
+                        prn_bond_v=np.array([[bv[2 * r], bv[2 * r + 1]] for r in range(1, 10)], np.float32),
+                        prn_bond_partner=np.array([bn[r] for r in range(1, 10)]), prn_mark=mark, prn_markt=mt,
+                        prn_n0=6, **{"prn_" + k: v for k, v in st.items()})
+    print("g13 written: fl0num", fl0num, "fln3", fln3, "fl1in", fl1in)
 
 
 ROCKS = [  # types 0,1 unused (air/water branches), 2..6 used
@@ -691,7 +851,7 @@ def g11(R):
 if __name__ == "__main__":
     R = Ref()
     R.lib.refh_set_threads(1)
-    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9", "g10", "g11", "g12", "g3c2"]
+    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9", "g10", "g11", "g12", "g3c2", "g13"]
     if "g1" in which:
         g1(R)
     if "g3" in which:
@@ -713,6 +873,8 @@ if __name__ == "__main__":
         g11(R)
     if "g12" in which:
         g12(R)
+    if "g13" in which:
+        g13(R)
     if "g3c2" in which:
         g3(R, 25, which=2, seed=2)
         g3(R, 33, which=2, seed=2)
