@@ -425,27 +425,35 @@ def main():
     roof = None
     vel = kst.get("smooth_vel_f32_L0", {})
     pl = kst.get("smooth_p_f32_L0", {})
+    slab = kst.get("sweep_slab_f32_L0", {})
     bcl = kst.get("bc_f32_L0", {})
-    if vel and pl:
+    if slab or (vel and pl):
         # the dominant kernel's unit of work: one level-0 FP32 sweep, i.e.
-        # six velocity colour launches, the pressure launch and the BC rows
-        sw_ms = 6 * vel["avg_ms"] + pl["avg_ms"] + bcl.get("avg_ms", 0.0)
+        # six velocity colour phases, the pressure phase and the BC rows
+        # (the phases as whole-level launches, or z-slab by z-slab)
+        phases_ms = slab["avg_ms"] if slab else 6 * vel["avg_ms"] + pl["avg_ms"]
+        sw_ms = phases_ms + bcl.get("avg_ms", 0.0)
         by = SWEEP_BYTES_PER_NODE * N
         ach = by / (sw_ms * 1e-3) / 1e9
         tr = committed_traffic(n)
+        launches = {"bc": dict(avg_ms=round(bcl.get("avg_ms", 0.0), 5), calls=bcl.get("calls", 0))}
+        if slab:
+            launches["phases"] = dict(avg_ms=round(slab["avg_ms"], 5), calls=slab["calls"])
+        else:
+            launches["vel"] = dict(avg_ms=round(vel["avg_ms"], 5), calls=vel["calls"])
+            launches["p"] = dict(avg_ms=round(pl["avg_ms"], 5), calls=pl["calls"])
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=tr,
                     kernel=("level-0 FP32 smoother sweep: k_smooth_vel_ilp<float, c> x 6 (three components, "
-                            "two colours) + k_smooth_p<float> + k_bc_chain<float>"),
+                            "two colours) + k_smooth_p<float> + k_bc_chain<float>"
+                            + (", phases launched z-slab by z-slab (phase q lagging q planes)" if slab else "")),
                     sweep_ms=round(sw_ms, 5), algorithmic_bytes=by, bytes_per_node=SWEEP_BYTES_PER_NODE,
-                    traffic_over_algorithmic=(round(tr / by, 3) if tr else None),
-                    launches={"vel": dict(avg_ms=round(vel["avg_ms"], 5), calls=vel["calls"]),
-                              "p": dict(avg_ms=round(pl["avg_ms"], 5), calls=pl["calls"]),
-                              "bc": dict(avg_ms=round(bcl.get("avg_ms", 0.0), 5), calls=bcl.get("calls", 0))},
-                    per_launch=dict(kernel="k_smooth_vel_ilp<float> (one colour of one component)",
-                                    bytes=smoother_bytes_per_launch(N, 4),
-                                    achieved=round(smoother_bytes_per_launch(N, 4) / (vel["avg_ms"] * 1e-3) / 1e9,
-                                                   1)))
+                    traffic_over_algorithmic=(round(tr / by, 3) if tr else None), launches=launches)
+        if vel:
+            roof["per_launch"] = dict(kernel="k_smooth_vel_ilp<float> (one colour of one component)",
+                                      bytes=smoother_bytes_per_launch(N, 4),
+                                      achieved=round(smoother_bytes_per_launch(N, 4) / (vel["avg_ms"] * 1e-3) / 1e9,
+                                                     1))
 
     extras = None if a.no_extras else other_paths(I, np, cfg)
     loop = None

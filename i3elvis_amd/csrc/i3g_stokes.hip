@@ -292,6 +292,11 @@ struct i3g_ctx {
 	/* velocity smoother with all row loads issued first (I3G_SMOOTH_ILP=0
 	 * selects the plain kernel) */
 	int smooth_ilp = 1;
+	/* smoother sweeps in z-slabs of this many planes on level 0 (on every
+	 * level with slab_all): -1 automatic (equal slabs of at most
+	 * SLAB_BYTES of sweep operands), 0 one slab (I3G_SLAB, I3G_SLAB_ALL) */
+	int slab = -1;
+	int slab_all = 0;
 	/* level transfer maps: 0 the reference's index rules, 1 geometric
 	 * bracketing (I3G_XFER_GEO=1) */
 	int xfer_geo = 0;
@@ -334,6 +339,11 @@ static hipEvent_t ev_get(i3g_ctx *c)
 	return c->evpool[c->evnext++];
 }
 
+/* bytes of sweep operands per z-slab of the automatic slab choice */
+#define SLAB_BYTES 300e6
+
+/* Event pair around the launches of one scope, kept under `name` when
+ * timing is on (a null name times nothing) */
 struct Timed {
 	i3g_ctx *c;
 	const char *name;
@@ -341,7 +351,7 @@ struct Timed {
 	hipEvent_t a, b;
 	Timed(i3g_ctx *c_, const char *n, double by) : c(c_), name(n), bytes(by)
 	{
-		if (c->timing) {
+		if (c->timing && name) {
 			a = ev_get(c);
 			b = ev_get(c);
 			hipEventRecord(a, c->s);
@@ -349,7 +359,7 @@ struct Timed {
 	}
 	~Timed()
 	{
-		if (c->timing) {
+		if (c->timing && name) {
 			hipEventRecord(b, c->s);
 			c->pending.push_back({name, {a, b}});
 			c->stats[name].bytes += bytes;
@@ -791,6 +801,10 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		if (e && e[0] >= '0' && e[0] <= '2') c->xcd_remap = e[0] - '0';
 		e = getenv("I3G_SMOOTH_ILP");
 		if (e && e[0] == '0') c->smooth_ilp = 0;
+		e = getenv("I3G_SLAB");
+		if (e && e[0]) c->slab = atoi(e) > 0 ? atoi(e) : 0;
+		e = getenv("I3G_SLAB_ALL");
+		if (e && e[0] == '1') c->slab_all = 1;
 		e = getenv("I3G_COARSE_VISC");
 		if (e && e[0] >= '0' && e[0] <= '2') c->coarse_visc = e[0] - '0';
 		e = getenv("I3G_COARSE_VISC_MAXLEV");
@@ -987,31 +1001,71 @@ static void sweep(i3g_ctx *c, int l, T *x, const T *b, const i3g_stokes_params *
 	LevelView<T> v = view<T>(L);
 	dim3 hg = half_grid(L.nx, L.ny, L.nz);
 	int chunks = (int)hg.x, remap = c->xcd_remap && (l == 0 || (c->xcd_remap == 2 && L.nz >= 64));
-	if (remap) hg = dim3((unsigned)((((long)chunks * L.nz + 7) / 8) * 8));
-	double by = (double)L.N * (sizeof(T) * 2 + 1 + sizeof(T) * 4 + sizeof(T) * 3) / 2;
-	for (int comp = 1; comp <= 3; comp++)
-		for (int col = 0; col < 2; col++) {
-			Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "smooth_vel_f64_L0" : "smooth_vel_f32_L0") : "smooth_vel_coarse", by);
-			if (!c->smooth_ilp)
-				hipLaunchKernelGGL(k_smooth_vel<T>, hg, dim3(256), 0, c->s, v, x, b, comp, col, (T)p->v0koef,
-						   remap, chunks);
-			else if (comp == 1)
-				hipLaunchKernelGGL((k_smooth_vel_ilp<T, 1>), hg, dim3(256), 0, c->s, v, x, b, col,
-						   (T)p->v0koef, remap, chunks);
-			else if (comp == 2)
-				hipLaunchKernelGGL((k_smooth_vel_ilp<T, 2>), hg, dim3(256), 0, c->s, v, x, b, col,
-						   (T)p->v0koef, remap, chunks);
-			else
-				hipLaunchKernelGGL((k_smooth_vel_ilp<T, 3>), hg, dim3(256), 0, c->s, v, x, b, col,
-						   (T)p->v0koef, remap, chunks);
-		}
+	const bool f64 = sizeof(T) == 8;
+	/* The seven phases (vx, vy, vz in two colours each, then P) in z-slabs:
+	 * in slab s (planes [b_s, b_s+1)) phase q relaxes planes
+	 * [b_s - q, b_s+1 - q), the last slab up to the top.  A row reaches
+	 * one plane up and down, so phase q at plane k finds phase q-1 done at
+	 * k+1 (and below), phase q+1 not yet begun at k-1, and the later
+	 * components and P untouched at k and k+1: every node is relaxed from
+	 * the same values as in the phase-by-phase sweep, and the result is
+	 * the same to the bit.  What it buys: the slab's planes, read from HBM
+	 * by the first phase, are still in the last-level cache for the other
+	 * six. */
+	const int nq = 7;
+	int slab = c->slab;
+	if (slab < 0) {
+		/* the operands of a sweep (four unknown and four rhs planes, four
+		 * viscosity planes, the flag byte) in slabs that stay in the 256 MB
+		 * last-level cache with the lagging phases' planes: three slabs at
+		 * 257^3 in FP32, one (no slabs) at 129^3 */
+		double bytes = (double)L.N * (12 * sizeof(T) + 1);
+		int ns = (int)ceil(bytes / SLAB_BYTES);
+		slab = ns > 1 ? (L.nz + ns - 1) / ns : 0;
+	}
+	if (!(slab > 0 && L.nz > slab + nq && (l == 0 || c->slab_all))) slab = L.nz;
+	double by = slab < L.nz ? 0 : (double)L.N * (sizeof(T) * 2 + 1 + sizeof(T) * 4 + sizeof(T) * 3) / 2;
 	{
-		Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "smooth_p_f64_L0" : "smooth_p_f32_L0") : "smooth_p_coarse", 0);
-		hipLaunchKernelGGL(k_smooth_p<T>, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, v, x, b,
-				   (T)p->p0koef);
+	Timed tsw(c, slab < L.nz ? (l == 0 ? (f64 ? "sweep_slab_f64_L0" : "sweep_slab_f32_L0") : "sweep_slab_coarse")
+				 : nullptr,
+		  0);
+	for (int b0 = 0; b0 < L.nz; b0 += slab) {
+		int b1 = b0 + slab >= L.nz ? L.nz : b0 + slab;
+		for (int q = 0; q < nq; q++) {
+			int k0 = b0 - q < 0 ? 0 : b0 - q;
+			int k1 = b1 == L.nz ? L.nz : b1 - q;
+			int np = k1 - k0;
+			if (np <= 0) continue;
+			if (q == 6) {
+				Timed tm(c, slab < L.nz ? nullptr : (l == 0 ? (f64 ? "smooth_p_f64_L0" : "smooth_p_f32_L0")
+									      : "smooth_p_coarse"),
+					 0);
+				dim3 pg = plane_grid(L.nx, L.ny, np);
+				hipLaunchKernelGGL(k_smooth_p<T>, pg, dim3(256), 0, c->s, v, x, b, (T)p->p0koef, k0);
+				continue;
+			}
+			int comp = 1 + q / 2, col = q & 1;
+			dim3 g = remap ? dim3((unsigned)((((long)chunks * np + 7) / 8) * 8)) : dim3((unsigned)chunks, (unsigned)np);
+			Timed tm(c, slab < L.nz ? nullptr : (l == 0 ? (f64 ? "smooth_vel_f64_L0" : "smooth_vel_f32_L0")
+								      : "smooth_vel_coarse"),
+				 by);
+			if (!c->smooth_ilp)
+				hipLaunchKernelGGL(k_smooth_vel<T>, g, dim3(256), 0, c->s, v, x, b, comp, col, (T)p->v0koef,
+						   remap, chunks, k0, np);
+			else if (comp == 1)
+				hipLaunchKernelGGL((k_smooth_vel_ilp<T, 1>), g, dim3(256), 0, c->s, v, x, b, col,
+						   (T)p->v0koef, remap, chunks, k0, np);
+			else if (comp == 2)
+				hipLaunchKernelGGL((k_smooth_vel_ilp<T, 2>), g, dim3(256), 0, c->s, v, x, b, col,
+						   (T)p->v0koef, remap, chunks, k0, np);
+			else
+				hipLaunchKernelGGL((k_smooth_vel_ilp<T, 3>), g, dim3(256), 0, c->s, v, x, b, col,
+						   (T)p->v0koef, remap, chunks, k0, np);
+		}
+	}
 	}
 	{
-		Timed tm(c, l == 0 ? (sizeof(T) == 8 ? "bc_f64_L0" : "bc_f32_L0") : "bc_coarse", 0);
+		Timed tm(c, l == 0 ? (f64 ? "bc_f64_L0" : "bc_f32_L0") : "bc_coarse", 0);
 		bc_enforce<T>(c, L, x, 1);
 	}
 }
@@ -1954,17 +2008,20 @@ static int smooth_phase_impl(i3g_ctx *c, int level, int phase, double *const *x,
 	const T *bb = lb<T>(L);
 	if (phase == 6) {
 		hipLaunchKernelGGL(k_smooth_p<T>, plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s, v, xx, bb,
-				   (T)p->p0koef);
+				   (T)p->p0koef, 0);
 	} else {
 		dim3 hg = half_grid(L.nx, L.ny, L.nz);
-		int chunks = (int)hg.x, col = phase & 1;
+		int chunks = (int)hg.x, col = phase & 1, nz = L.nz;
 		T v0 = (T)p->v0koef;
 		if (phase / 2 == 0)
-			hipLaunchKernelGGL((k_smooth_vel_ilp<T, 1>), hg, dim3(256), 0, c->s, v, xx, bb, col, v0, 0, chunks);
+			hipLaunchKernelGGL((k_smooth_vel_ilp<T, 1>), hg, dim3(256), 0, c->s, v, xx, bb, col, v0, 0, chunks, 0,
+					   nz);
 		else if (phase / 2 == 1)
-			hipLaunchKernelGGL((k_smooth_vel_ilp<T, 2>), hg, dim3(256), 0, c->s, v, xx, bb, col, v0, 0, chunks);
+			hipLaunchKernelGGL((k_smooth_vel_ilp<T, 2>), hg, dim3(256), 0, c->s, v, xx, bb, col, v0, 0, chunks, 0,
+					   nz);
 		else
-			hipLaunchKernelGGL((k_smooth_vel_ilp<T, 3>), hg, dim3(256), 0, c->s, v, xx, bb, col, v0, 0, chunks);
+			hipLaunchKernelGGL((k_smooth_vel_ilp<T, 3>), hg, dim3(256), 0, c->s, v, xx, bb, col, v0, 0, chunks, 0,
+					   nz);
 	}
 	return smooth_out<T>(c, L, x);
 }
@@ -1990,12 +2047,19 @@ static int smooth_impl(i3g_ctx *c, int level, int nsweeps, int variant, double *
 	Level &L = c->L[level];
 	int rc = smooth_in<T>(c, L, x, b);
 	if (rc) return rc;
-	int keep = c->fused, keepz = c->fused_minz;
-	if (variant == 1) c->fused = 0;
+	int keep = c->fused, keepz = c->fused_minz, keeps = c->slab, keepa = c->slab_all;
+	if (variant == 1) { c->fused = 0; c->slab = 0; }
 	if (variant == 2) { c->fused = 1; c->fused_minz = 0; }
+	if (variant == 3) {
+		c->fused = 0;
+		c->slab = keeps > 0 ? keeps : 4;
+		c->slab_all = 1;
+	}
 	smooth<T>(c, level, nsweeps, p);
 	c->fused = keep;
 	c->fused_minz = keepz;
+	c->slab = keeps;
+	c->slab_all = keepa;
 	return smooth_out<T>(c, L, x);
 }
 
@@ -2003,7 +2067,7 @@ extern "C" int i3g_stokes_smooth(i3g_ctx *c, int level, int precision, int nswee
 				 const double *const *b, double v0koef, double p0koef)
 {
 	if (!c || !c->coef_ready || !x || !b || level < 0 || level >= c->nlev || nsweeps < 0 || variant < 0 ||
-	    variant > 2 || (precision != 32 && precision != 64))
+	    variant > 3 || (precision != 32 && precision != 64))
 		return I3G_EINVAL;
 	i3g_stokes_params p;
 	i3g_stokes_default_params(&p);

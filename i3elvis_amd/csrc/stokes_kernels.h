@@ -229,28 +229,29 @@ k_residual(LevelView<T> L, const X *__restrict__ x, const T *__restrict__ b,
  * groups to the 8 XCDs round-robin, so group g is given work item
  * (g % 8) * per + g / 8: every XCD sweeps a contiguous slab of planes in
  * order and finds the neighbouring planes in its own L2. */
-__device__ __forceinline__ bool block_work(int remap, int chunks, int planes, int *chunk, int *plane)
+__device__ __forceinline__ bool block_work(int remap, int chunks, int planes, int k0, int *chunk, int *plane)
 {
 	if (!remap) {
 		*chunk = blockIdx.x;
-		*plane = blockIdx.y;
+		*plane = k0 + (int)blockIdx.y;
 		return true;
 	}
 	int per = gridDim.x >> 3;
 	int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 	if (w >= chunks * planes) return false;
-	*plane = w / chunks;
-	*chunk = w - *plane * chunks;
+	int q = w / chunks;
+	*chunk = w - q * chunks;
+	*plane = k0 + q;
 	return true;
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_smooth_vel(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int comp,
-	     int colour, T v0koef, int remap, int chunks)
+	     int colour, T v0koef, int remap, int chunks, int k0, int planes)
 {
 	int ch, k;
-	if (!block_work(remap, chunks, L.nz, &ch, &k)) return;
+	if (!block_work(remap, chunks, planes, k0, &ch, &k)) return;
 	int t = ch * blockDim.x + threadIdx.x;
 	int hy = (L.ny + 1) >> 1;
 	int i = t / hy, tj = t - i * hy;
@@ -379,10 +380,10 @@ __device__ __forceinline__ void vel_update(const LevelView<T> &L, T *__restrict_
 template <typename T, int COMP>
 __global__ void __launch_bounds__(256)
 k_smooth_vel_ilp(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int colour, T v0koef, int remap,
-		 int chunks)
+		 int chunks, int k0, int planes)
 {
 	int ch, k;
-	if (!block_work(remap, chunks, L.nz, &ch, &k)) return;
+	if (!block_work(remap, chunks, planes, k0, &ch, &k)) return;
 	int t = ch * blockDim.x + threadIdx.x;
 	int hy = (L.ny + 1) >> 1;
 	int i = t / hy, tj = t - i * hy;
@@ -421,10 +422,10 @@ __device__ __forceinline__ void p_update(const LevelView<T> &L, T *__restrict__ 
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_smooth_p(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, T p0koef)
+k_smooth_p(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, T p0koef, int k0)
 {
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	int k = blockIdx.y;
+	int k = k0 + (int)blockIdx.y;
 	int i = t / L.ny, j = t - i * L.ny;
 	if (i >= L.nx) return;
 	p_update<T>(L, x, b, i, j, k, p0koef);
@@ -479,7 +480,7 @@ k_sweep_comp(LevelView<T> L, const T *__restrict__ P, const T *__restrict__ vx, 
 	T *ring = (T *)lds_raw;
 	int w, dummy;
 	int total = G.ntx * G.nty * G.nzs;
-	if (!block_work(G.remap, total, 1, &w, &dummy)) return;
+	if (!block_work(G.remap, total, 1, 0, &w, &dummy)) return;
 	int tx = w % G.ntx, rest = w / G.ntx;
 	int ty = rest % G.nty, sl = rest / G.nty;
 	const int nx = L.nx, ny = L.ny, nz = L.nz;

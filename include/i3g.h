@@ -271,7 +271,9 @@ int i3g_stokes_prolong(i3g_ctx *ctx, int level, const double *const *coarse,
  * with the product's kernels: variant 0 the context's default, 1 the seven
  * phase launches, 2 one launch per velocity component (the pressure rows in
  * the vz launch; pairs of sweeps out of place, an odd last sweep as
- * phases).  precision 64 or 32 (the planes are rounded to float on the
+ * phases), 3 the seven phases in z-slabs (phase q of slab [b, b') on planes
+ * [b - q, b' - q); slab thickness from I3G_SLAB at create, else 4 planes).
+ * precision 64 or 32 (the planes are rounded to float on the
  * device, and back).  Both need i3g_set_coefficients first and overwrite
  * the level's multigrid work buffers. */
 int i3g_stokes_smooth_phase(i3g_ctx *ctx, int level, int phase, int precision,

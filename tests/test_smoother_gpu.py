@@ -8,7 +8,8 @@ P += p0koef nd r on every continuity row, everything else unchanged.  FP64
 to 1e-10 of the phase's largest increment (the rows are summed in another
 order than the reference's), FP32 to 1e-4 (the residual of a random state
 in float).  The single-pass sweep (one launch per velocity component, out
-of place) against the seven phase launches: equal to the bit.
+of place) and the z-slab schedule of the seven phases against the seven
+phase launches: equal to the bit.
 """
 import numpy as np
 import pytest
@@ -158,6 +159,57 @@ def test_single_pass_vcycles_match_phase_launch_vcycles(gpu, monkeypatch, prec):
         S.set_solution(z, z, z, z)
         st = S.solve(precision=prec, rtol=1e-8, max_outer=1, max_vcycles=5, check_conv=False)
         print(f"case: fused sweep={mode} prec={prec} vcycles={st.vcycles} rel={st.rel_res:.3e}", flush=True)
+        sols.append(S.get_solution())
+        S.close()
+    for c, (a, b) in enumerate(zip(*sols)):
+        assert np.array_equal(a, b), (c, np.abs(a - b).max())
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("case,slab", [("g1", 1), ("g1", 2), ("sub65", 1), ("sub65", 3), ("sub65", 16),
+                                       ("sub65", 50), ("sub65_level1", 2)])
+def test_slab_sweep_matches_phase_launches(gpu, monkeypatch, prec, case, slab):
+    """The seven phases in z-slabs (variant 3: phase q of slab [b, b') on
+    planes [b - q, b' - q)) equal the phase launches over whole levels to the
+    bit, for slabs thinner than the seven-plane lag, slabs that do not divide
+    the level, and one sweep and three."""
+    if case == "g1":
+        cfg = _cfg_from_g1(golden("g1_operator.npz"))
+        level = 0
+    else:
+        cfg = I.make_config(2, 65, 65, 65, seed=3)
+        level = 1 if case.endswith("level1") else 0
+    monkeypatch.setenv("I3G_SLAB", str(slab))
+    print(f"case: slab sweep {case} slab {slab} prec {prec} level {level}", flush=True)
+    S = I.Stokes(cfg)
+    S.set_coefficients()
+    Nl = S.level_nodes(level)
+    x = _random_state(Nl, 21)
+    b = _random_state(Nl, 22)
+    for ns in (1, 3):
+        a = S.smooth(level, prec, ns, x, b, 0.95, 0.7, variant=3)
+        r = S.smooth(level, prec, ns, x, b, 0.95, 0.7, variant=1)
+        for c in range(4):
+            assert np.array_equal(a[c], r[c]), (ns, c, np.abs(a[c] - r[c]).max())
+        assert any(not np.array_equal(a[c], x[c]) for c in range(4))
+    S.close()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_slab_vcycles_match_phase_launch_vcycles(gpu, monkeypatch, prec):
+    """Whole V-cycles with level-0 sweeps in z-slabs (I3G_SLAB=5) against the
+    phase launches: the fields after a few V-cycles are equal to the bit."""
+    n = 33
+    cfg = I.make_config(2, n, n, n, seed=7)
+    sols = []
+    for slab in ("0", "5"):
+        monkeypatch.setenv("I3G_SLAB", slab)
+        S = I.Stokes(cfg)
+        S.set_coefficients()
+        z = np.zeros(n ** 3)
+        S.set_solution(z, z, z, z)
+        st = S.solve(precision=prec, rtol=1e-8, max_outer=1, max_vcycles=5, check_conv=False)
+        print(f"case: slab={slab} prec={prec} vcycles={st.vcycles} rel={st.rel_res:.3e}", flush=True)
         sols.append(S.get_solution())
         S.close()
     for c, (a, b) in enumerate(zip(*sols)):
