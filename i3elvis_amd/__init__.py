@@ -321,7 +321,8 @@ class Stokes:
 
 # ---- markers ----------------------------------------------------------------
 
-ROCK_FIELDS = ("nu0", "dh", "dv", "n0", "n1", "ro", "bro", "aro", "cp", "kt", "kf", "kp", "ht")
+ROCK_FIELDS = ("nu0", "dh", "dv", "n0", "n1", "ro", "bro", "aro", "cp", "kt", "kf", "kp", "ht",
+               "ss", "mm", "ll", "s1", "a0", "a1", "b0", "b1", "e0", "e1")
 
 
 class Rock(C.Structure):
@@ -329,8 +330,27 @@ class Rock(C.Structure):
 
 
 class MkParams(C.Structure):
+    """i3g_mk_params; rheology 0 leaves the fields after it unused"""
     _fields_ = [("nubeg", C.c_double), ("nuend", C.c_double), ("nukoef", C.c_double), ("densimod", C.c_int),
-                ("timestep", C.c_double), ("vdeep", C.c_double), ("nudeep", C.c_double), ("viscmod", C.c_int)]
+                ("timestep", C.c_double), ("vdeep", C.c_double), ("nudeep", C.c_double), ("viscmod", C.c_int),
+                ("rheology", C.c_int), ("eroslev", C.c_double), ("sedilev", C.c_double), ("waterlev", C.c_double),
+                ("sedtype", C.c_int), ("mnumy", C.c_long), ("teclmin", C.c_double), ("teclmax", C.c_double),
+                ("timesum", C.c_double), ("convend", C.c_double), ("outgrid", C.c_int)]
+
+
+class MoveParams(C.Structure):
+    """i3g_move_params"""
+    _fields_ = [("outgrid", C.c_int), ("markmod", C.c_int), ("vyfluid", C.c_double), ("vymelt", C.c_double),
+                ("g", C.c_double * 3)]
+
+
+def mk_params(nubeg=1e18, nuend=1e25, nukoef=0.0, densimod=1, timestep=1.0, vdeep=1e30, nudeep=1e30, viscmod=0,
+              rheology=0, eroslev=-1e30, sedilev=1e30, waterlev=-1.0, sedtype=3, mnumy=1, teclmin=0.0,
+              teclmax=0.0, timesum=0.0, convend=0.0, outgrid=0):
+    return MkParams(float(nubeg), float(nuend), float(nukoef), int(densimod), float(timestep), float(vdeep),
+                    float(nudeep), int(viscmod), int(rheology), float(eroslev), float(sedilev), float(waterlev),
+                    int(sedtype), int(mnumy), float(teclmin), float(teclmax), float(timesum), float(convend),
+                    int(outgrid))
 
 
 def rocks_array(rocks):
@@ -402,10 +422,13 @@ class Markers:
         return order
 
     def to_nodes(self, rocks, pr, fld, nubeg, nuend, nukoef=0.0, densimod=1, timestep=1.0, vdeep=1e30,
-                 nudeep=1e30, viscmod=0):
+                 nudeep=1e30, viscmod=0, **rheo):
+        """marker -> node transfer; rheo: the i3g_mk_params fields of the
+        whole rheology (rheology=1, eroslev, sedilev, waterlev, sedtype,
+        mnumy, teclmin, teclmax, timesum, convend)"""
         fld = np.array(fld, np.float32, copy=True)
         pr = np.ascontiguousarray(pr, np.float64)
-        prm = MkParams(nubeg, nuend, nukoef, densimod, timestep, vdeep, nudeep, viscmod)
+        prm = mk_params(nubeg, nuend, nukoef, densimod, timestep, vdeep, nudeep, viscmod, **rheo)
         ra = rocks_array(rocks)
         check(lib().i3g_markers_to_nodes(self.h, ra, C.c_int(len(rocks)), C.byref(prm), _p(pr),
                                          *[_p(fld[q]) for q in range(11)]), "markers_to_nodes")
@@ -434,11 +457,13 @@ class Markers:
         check(lib().i3g_markers_epscalc(self.h, sp, _p(prd), _p(out)), "markers_epscalc")
         return out
 
-    def update_props(self, rocks, nubeg, nuend, timestep, healrate, st12, pr):
+    def update_props(self, rocks, nubeg, nuend, timestep, healrate, st12, pr, rheology=0, timesum=0.0, convend=0.0,
+                     outgrid=0):
         st = [np.ascontiguousarray(a, np.float32) for a in st12]
         sp = (C.c_void_p * 12)(*[a.ctypes.data for a in st])
         prd = np.ascontiguousarray(pr, np.float64)
-        prm = MkParams(nubeg, nuend, 0.0, 1, timestep, 1e30, 1e30, 0)
+        prm = mk_params(nubeg, nuend, 0.0, 1, timestep, rheology=rheology, timesum=timesum, convend=convend,
+                        outgrid=outgrid)
         ra = rocks_array(rocks)
         check(lib().i3g_markers_update_props(self.h, ra, C.c_int(len(rocks)), C.byref(prm), C.c_double(healrate), sp,
                                              _p(prd)), "markers_update_props")
@@ -464,6 +489,33 @@ class Markers:
     def advect_rk4(self, dt, vx, vy, vz):
         v = [np.ascontiguousarray(a, np.float32) for a in (vx, vy, vz)]
         check(lib().i3g_markers_advect_rk4(self.h, C.c_double(dt), *[_p(a) for a in v]), "markers_rk4")
+
+    def advect_ex(self, dt, vx, vy, vz, outgrid=0, markmod=0, vyfluid=0.0, vymelt=0.0, g=(0.0, 9.81, 0.0), pr=None,
+                  markd=None):
+        """movemark's motion (i3g_markers_advect_ex); markd by marker id"""
+        v = [np.ascontiguousarray(a, np.float32) for a in (vx, vy, vz)]
+        mp = MoveParams(int(outgrid), int(markmod), float(vyfluid), float(vymelt), (C.c_double * 3)(*g))
+        prd = None if pr is None else np.ascontiguousarray(pr, np.float64)
+        md = None if markd is None else np.ascontiguousarray(markd, np.float32)
+        check(lib().i3g_markers_advect_ex(self.h, C.c_double(dt), *[_p(a) for a in v], C.byref(mp), _p(prd), _p(md)),
+              "markers_advect_ex")
+
+    def set_grain(self, rr, rn):
+        a, b = (np.ascontiguousarray(q, np.float32) for q in (rr, rn))
+        check(lib().i3g_markers_set_grain(self.h, _p(a), _p(b)), "markers_set_grain")
+
+    def get_grain(self):
+        rr, rn = np.zeros(self.n, np.float32), np.zeros(self.n, np.float32)
+        check(lib().i3g_markers_get_grain(self.h, _p(rr), _p(rn)), "markers_get_grain")
+        return rr, rn
+
+    def set_stress(self, st12):
+        if st12 is None:
+            check(lib().i3g_markers_set_stress(self.h, None), "markers_set_stress")
+            return
+        self._st = [np.ascontiguousarray(a, np.float32) for a in st12]
+        sp = (C.c_void_p * 12)(*[a.ctypes.data for a in self._st])
+        check(lib().i3g_markers_set_stress(self.h, sp), "markers_set_stress")
 
 
 # ---- heat -------------------------------------------------------------------

@@ -18,6 +18,7 @@
 #pragma clang fp contract(off)
 
 #include "grid_interp.h"
+#include "rheology.h"
 
 #define MCHK(x)                                                                \
 	do {                                                                   \
@@ -40,9 +41,11 @@ struct i3g_markers {
 	/* SoA marker state (current order) and a second buffer for permuting */
 	float *x = nullptr, *y = nullptr, *z = nullptr, *tk = nullptr, *eta = nullptr;
 	float *e = nullptr;      /* accumulated strain (marke) */
+	float *rr = nullptr, *rn = nullptr; /* grain roughness (markrr), dislocation share (markrn) */
 	signed char *type = nullptr;
 	int *id = nullptr;
 	float *x2 = nullptr, *y2 = nullptr, *z2 = nullptr, *tk2 = nullptr, *eta2 = nullptr, *e2 = nullptr;
+	float *rr2 = nullptr, *rn2 = nullptr;
 	signed char *type2 = nullptr;
 	int *id2 = nullptr;
 	int *key = nullptr, *c1 = nullptr, *c2 = nullptr, *c3 = nullptr;
@@ -58,6 +61,9 @@ struct i3g_markers {
 	float *fld = nullptr;    /* 11 x N float in/out fields */
 	float *vel = nullptr;    /* 3 x N velocities */
 	float *st12 = nullptr;   /* 12 x N strain rates and stresses (epscalc1) */
+	int has_st = 0;          /* st12 holds the planes of i3g_markers_set_stress */
+	float *md = nullptr;     /* marker density by id (fluid velocity) */
+	long md_cap = 0;
 	/* movemark's depth rules (inactive by default) */
 	double zdeep = 1e30, vdeep = 1e30, tdeep = 0;
 	i3g_rock *rocks = nullptr;
@@ -209,10 +215,10 @@ __global__ void k_cellsort(long N, const int *start, const int *cnt, int *perm, 
 }
 
 /* The sorted order applied to every per-marker array in one pass: the
- * permutation is read once and all eight values of a marker are loaded
+ * permutation is read once and all ten values of a marker are loaded
  * before the first store. */
 struct MkArrays {
-	float *x, *y, *z, *tk, *eta, *e;
+	float *x, *y, *z, *tk, *eta, *e, *rr, *rn;
 	signed char *type;
 	int *id;
 };
@@ -221,10 +227,10 @@ __global__ void k_permute(long n, const int *__restrict__ perm, MkArrays s, MkAr
 	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= n) return;
 	int p = perm[t];
-	float x = s.x[p], y = s.y[p], z = s.z[p], tk = s.tk[p], eta = s.eta[p], e = s.e[p];
+	float x = s.x[p], y = s.y[p], z = s.z[p], tk = s.tk[p], eta = s.eta[p], e = s.e[p], rr = s.rr[p], rn = s.rn[p];
 	signed char ty = s.type[p];
 	int id = s.id[p];
-	d.x[t] = x; d.y[t] = y; d.z[t] = z; d.tk[t] = tk; d.eta[t] = eta; d.e[t] = e;
+	d.x[t] = x; d.y[t] = y; d.z[t] = z; d.tk[t] = tk; d.eta[t] = eta; d.e[t] = e; d.rr[t] = rr; d.rn[t] = rn;
 	d.type[t] = ty;
 	d.id[t] = id;
 }
@@ -311,12 +317,23 @@ struct __align__(16) MkRec {
 	double pad;
 };
 
+__device__ void epscalc(const float *st12, long N, const double *pr, const double *gx, const double *gy,
+			const double *gz, int nx, int ny, int nz, double x, double y, double z, double *o);
+
+/* ronurecalc's per-marker part (mark3mg.c:460-800): participation, with
+ * rheology the retyping, viscalc (from the stored viscosity when there is
+ * one and timestep > 0), dencalc1 with the phase-transition factor, the
+ * water/air proportions at the water level (the marker's own y cell sets
+ * the marker spacing, where the reference reads an index array shared
+ * between threads) and the viscosity caps; the record for the gather. */
 __global__ void k_marker_props(long n, const float *__restrict__ x, const float *__restrict__ y,
-			       const float *__restrict__ z, const float *__restrict__ tk, float *eta,
-			       const signed char *__restrict__ type, const int *c1, const int *c2, const int *c3,
-			       const double *pr, const double *gx, const double *gy, const double *gz, int nx, int ny,
-			       int nz, const i3g_rock *rocks, int nrock, i3g_mk_params prm, MkRec *rec, int *bad)
+			       const float *__restrict__ z, const float *__restrict__ tk, float *eta, signed char *type,
+			       float *e, float *rr, float *rn, const int *c1, const int *c2, const int *c3,
+			       const double *pr, const float *st12, long N, const double *gx, const double *gy,
+			       const double *gz, int nx, int ny, int nz, const i3g_rock *rocks, int nrock,
+			       i3g_mk_params prm, MkRec *rec, int *bad)
 {
+#pragma clang fp contract(off)
 	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= n) return;
 	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
@@ -327,31 +344,74 @@ __global__ void k_marker_props(long n, const float *__restrict__ x, const float 
 	MkRec *R = rec + t;
 	R->x = fx; R->y = fy; R->z = fz; R->in = in;
 	if (!in) return;
-	/* types with air/water, eclogite or melt branches in the reference
-	 * (mark3mg.c:463-474, 530-641, 660-686) are not handled here */
-	if (mt < 2 || mt == 7 || mt == 8 || (mt >= 16 && mt <= 18) || mt > 20) {
+	int mm2 = mt;
+	if (prm.rheology) {
+		/* erosion, sedimentation, water/air (mark3mg.c:465-472) */
+		if (mm2 > 1 && fy < prm.eroslev) mm2 = 0;
+		if (mm2 < 2 && fy > prm.sedilev) mm2 = prm.sedtype;
+		if (mm2 < 2 && fy > prm.waterlev) mm2 = 1;
+		if (mm2 < 2 && fy < prm.waterlev) mm2 = 0;
+		if (mm2 != mt) type[t] = (signed char)mm2;
+		if (mm2 > 20 || mm2 >= nrock || prm.densimod > 1) {
+			atomicOr(bad, 1);
+			R->in = 0;
+			return;
+		}
+	} else if (mm2 < 2 || mm2 == 7 || mm2 == 8 || (mm2 >= 16 && mm2 <= 18) || mm2 > 20) {
+		/* Newtonian subset: types with air/water, eclogite or melt branches
+		 * (mark3mg.c:463-474, 530-641, 660-686) are refused */
 		atomicOr(bad, 1);
 		R->in = 0;
 		return;
 	}
-	const i3g_rock r = rocks[mt];
+	const i3g_rock r = rocks[mm2];
 	double mpb = pr_interp(pr, gx, gy, gz, nx, ny, nz, (double)fx, (double)fy, (double)fz, c1[t], c2[t], c3[t]) * 1e-5;
 	double mtk = (double)tk[t];
 	double mnu = eta[t];
+	VisPrm vp{prm.nubeg, prm.nuend, prm.timestep, prm.timesum, prm.convend};
+	double eii = 0, sii = 0, dis = 0;
+	float me = e[t], mrr = rr[t], mrn = rn[t];
 	if (mnu <= 0 || prm.timestep <= 0) {
-		mnu = visc_ab(r, mtk, mpb, prm.nubeg, prm.nuend);
+		if (prm.rheology) {
+			double o[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+			if (st12) epscalc(st12, N, pr, gx, gy, gz, nx, ny, nz, (double)fx, (double)fy, (double)fz, o);
+			eii = inv2_dev(o, 0);
+			sii = inv2_dev(o, 7);
+			dis = o[13];
+		}
+		mnu = viscalc_dev(r, mm2, mtk, mpb, eii, sii, 1.0, dis, me, mrr, mrn, vp);
 		eta[t] = (float)mnu;
 	}
-	/* dencalc1 (mark3mg.c:3006-3024) */
-	double mro, mbb = 0;
-	if (prm.densimod == 0) mro = r.ro;
-	else {
-		mro = r.ro * (1.0 - r.bro * (mtk - 298.15)) * (1.0 + r.aro * (mpb - 1.0) * 1e-3);
-		mbb = r.bro / (1.0 - r.bro * (mtk - 298.15));
-	}
+	double mbb;
+	double mro = dencalc_dev(r, prm.densimod, mtk, mpb, mbb);
+	if (prm.rheology) mro *= phase_rokf_dev(mm2, mtk, mpb, prm.teclmin, prm.teclmax);
 	double mcp = r.cp;
 	double mkt = (r.kt + r.kf / (mtk + 77.0)) * exp(r.kp * mpb);
 	double mht = r.ht;
+	if (prm.rheology && prm.waterlev >= 0 && mm2 < 2) {
+		int wy = c2[t];
+		double dywa = (gy[wy + 1] - gy[wy]) / 2.0 / ((double)prm.mnumy);
+		double lev = ((double)fy - (prm.waterlev - dywa)) / (2.0 * dywa);
+		if (lev < 1.0 && lev > 0) {
+			const i3g_rock ra = rocks[0], rw = rocks[1];
+			double abb, wbb;
+			double anu = viscalc_dev(ra, 0, mtk, mpb, eii, sii, 1.0, dis, me, mrr, mrn, vp);
+			double aro = dencalc_dev(ra, prm.densimod, mtk, mpb, abb);
+			double wnu = viscalc_dev(rw, 1, mtk, mpb, eii, sii, 1.0, dis, me, mrr, mrn, vp);
+			double wro = dencalc_dev(rw, prm.densimod, mtk, mpb, wbb);
+			mnu = wnu * lev + anu * (1.0 - lev);
+			mro = wro * lev + aro * (1.0 - lev);
+			mbb = wbb * lev + abb * (1.0 - lev);
+			mcp = rw.cp * lev + ra.cp * (1.0 - lev);
+			mkt = rw.kt * lev + ra.kt * (1.0 - lev);
+			mht = rw.ht * lev + ra.ht * (1.0 - lev);
+		}
+	}
+	if (prm.rheology) {
+		e[t] = me;
+		rr[t] = mrr;
+		rn[t] = mrn;
+	}
 	/* viscosity caps of mark3mg.c:790-793 */
 	if ((double)fy > prm.vdeep && mnu > prm.nudeep) mnu = prm.nudeep;
 	if (fy < 5e+4 && fx > xs - 1e+4 && fz > 1.5e+5 && fz < 8.7e+5 && mnu > 1e+19) mnu = 1e+19;
@@ -801,11 +861,13 @@ __device__ void epscalc(const float *st12, long N, const double *pr, const doubl
 	}
 }
 
-/* markers that movemark moves (mark3mg.c:61, outgrid 0) */
-__device__ __forceinline__ bool mk_moves(float fx, float fy, float fz, int mt, double xs, double ys, double zs)
+/* markers that movemark moves (mark3mg.c:61): inside the grid, or outgrid
+ * != 1 and type < 50 or >= 100 */
+__device__ __forceinline__ bool mk_moves(float fx, float fy, float fz, int mt, double xs, double ys, double zs,
+					 int outgrid)
 {
-	return (fx >= 0 && fy >= 0 && fz >= 0 && (double)fx <= xs && (double)fy <= ys && (double)fz <= zs) || mt < 50 ||
-	       mt >= 100;
+	return (fx >= 0 && fy >= 0 && fz >= 0 && (double)fx <= xs && (double)fy <= ys && (double)fz <= zs) ||
+	       (outgrid != 1 && (mt < 50 || mt >= 100));
 }
 
 __global__ void k_epscalc(long n, const float *__restrict__ x, const float *__restrict__ y,
@@ -823,18 +885,20 @@ __global__ void k_epscalc(long n, const float *__restrict__ x, const float *__re
 }
 
 /* movemark's marker properties ahead of the position update
- * (mark3mg.c:61-145) for the Newtonian rock types: markers of type < 50
- * with T > 0 get eta = viscalc(T, P) with P from epscalc1 at the marker;
- * with no brittle strength viscalc marks them as not yielding (e = -|e|,
- * mark3mg.c:1179, 1648-1659), so the strain is not grown by dt * eii and
- * heals by dt * healrate toward zero. */
+ * (mark3mg.c:61-145): for the markers that move, epscalc1 at the marker
+ * (strain-rate and stress invariants, dissipation, P); markers of type < 50
+ * with T > 0 get the strain-rate correction of numerical diffusion
+ * (markers away from the three outer lines that have a viscosity,
+ * :97-105), eta = viscalc, the strain growth while yielding and the
+ * healing.  rheology 0 refuses the types whose transfer branches it does
+ * not take (as k_marker_props). */
 __global__ void k_mk_props(long n, const float *__restrict__ x, const float *__restrict__ y,
-			   const float *__restrict__ z, const float *__restrict__ tk, float *eta, float *e,
-			   const signed char *__restrict__ type, const float *st12, long N, const double *pr,
+			   const float *__restrict__ z, const float *__restrict__ tk, float *eta, float *e, float *rr,
+			   float *rn, const signed char *__restrict__ type, const float *st12, long N, const double *pr,
 			   const double *gx, const double *gy, const double *gz, int nx, int ny, int nz,
-			   const i3g_rock *rocks, int nrock, double nubeg, double nuend, double dt, double healrate,
-			   int *bad)
+			   const i3g_rock *rocks, int nrock, i3g_mk_params prm, double healrate, int *bad)
 {
+#pragma clang fp contract(off)
 	extern __shared__ double sg[];
 	stage_lines(sg, gx, gy, gz, nx, ny, nz);
 	gx = sg; gy = sg + nx; gz = sg + nx + ny;
@@ -842,21 +906,35 @@ __global__ void k_mk_props(long n, const float *__restrict__ x, const float *__r
 	if (t >= n) return;
 	float fx = x[t], fy = y[t], fz = z[t];
 	int mt = type[t];
-	if (!mk_moves(fx, fy, fz, mt, gx[nx - 1], gy[ny - 1], gz[nz - 1])) return;
+	if (!mk_moves(fx, fy, fz, mt, gx[nx - 1], gy[ny - 1], gz[nz - 1], prm.outgrid)) return;
 	double mtk = (double)tk[t];
 	if (!(mt < 50 && mtk > 0)) return;
-	if (mt < 2 || mt >= nrock || mt == 7 || mt == 8 || (mt >= 16 && mt <= 18) || mt > 20) {
+	if (mt >= nrock || (!prm.rheology && (mt < 2 || mt == 7 || mt == 8 || (mt >= 16 && mt <= 18) || mt > 20))) {
 		atomicOr(bad, 1);
 		return;
 	}
 	double o[14];
 	epscalc(st12, N, pr, gx, gy, gz, nx, ny, nz, (double)fx, (double)fy, (double)fz, o);
+	double eii = inv2_dev(o, 0), sii = inv2_dev(o, 7), dis = o[13];
 	double mpb = o[6] * 1e-5;
-	eta[t] = (float)visc_ab(rocks[mt], mtk, mpb, nubeg, nuend);
-	float ev = -fabsf(e[t]);
-	ev += dt * healrate;
-	if (ev > 0) ev = 0;
-	e[t] = ev;
+	float mv = eta[t];
+	double rii = 1.0;
+	if (mv > 0 && fx > gx[3] && fx < gx[nx - 4] && fy > gy[3] && fy < gy[ny - 4] && fz > gz[3] && fz < gz[nz - 4])
+		rii *= pow(0.5 * sii / eii / mv, 0.87);
+	float me = e[t], mrr = rr[t], mrn = rn[t];
+	VisPrm vp{prm.nubeg, prm.nuend, prm.timestep, prm.timesum, prm.convend};
+	eta[t] = (float)viscalc_dev(rocks[mt], mt, mtk, mpb, eii, sii, rii, dis, me, mrr, mrn, vp);
+	if (me > 0) me += prm.timestep * eii;
+	if (me > 0) {
+		me -= prm.timestep * healrate;
+		if (me < 0) me = 0;
+	} else {
+		me += prm.timestep * healrate;
+		if (me > 0) me = 0;
+	}
+	e[t] = me;
+	rr[t] = mrr;
+	rn[t] = mrn;
 }
 
 /* ------------------------------------------------------------------ */
@@ -871,10 +949,19 @@ struct DepthRules {
 	double zdeep, vdeep, tdeep;
 };
 
+/* movemark's motion of one marker (mark3mg.c:61, 146-284) */
+struct MoveArgs {
+	i3g_move_params mp;
+	const double *pr; /* P (level-0 nodes) for the fluid velocity, or null */
+	const float *md;  /* marker density by id, or null */
+	const int *id;
+};
+
 __global__ void k_rk4(long n, float *x, float *y, float *z, float *tk, float *eta, signed char *type,
 		      const float *vx, const float *vy, const float *vz, const double *gx, const double *gy,
-		      const double *gz, int nx, int ny, int nz, double dt, DepthRules D)
+		      const double *gz, int nx, int ny, int nz, double dt, DepthRules D, MoveArgs A)
 {
+#pragma clang fp contract(off)
 	extern __shared__ double sg[];
 	stage_lines(sg, gx, gy, gz, nx, ny, nz);
 	gx = sg; gy = sg + nx; gz = sg + nx + ny;
@@ -883,24 +970,94 @@ __global__ void k_rk4(long n, float *x, float *y, float *z, float *tk, float *et
 	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
 	float fx = x[t], fy = y[t], fz = z[t];
 	int mt = type[t];
-	/* moved markers (mark3mg.c:61): inside the grid, or outgrid != 1 with
-	 * markt < 50 or >= 100 (outgrid 0 here) */
+	/* moved markers (mark3mg.c:61) */
 	if (!((fx >= 0 && fy >= 0 && fz >= 0 && (double)fx <= xs && (double)fy <= ys && (double)fz <= zs) ||
-	      (mt < 50 || mt >= 100)))
+	      (A.mp.outgrid != 1 && (mt < 50 || mt >= 100))))
 		return;
-	double v1[3], v2[3], v3[3], v4[3];
 	double px = fx, py = fy, pz = fz;
-	/* stages 2-4 search their cell from the previous stage's (a step of
-	 * a fraction of a cell: one or two comparisons instead of a bisection) */
 	int cell[3];
+	double v1[3], v2[3], v3[3], v4[3], v[3], vw[3] = {0, 0, 0};
 	vxyz(vx, vy, vz, gx, gy, gz, nx, ny, nz, px, py, pz, v1, cell, false);
-	vxyz(vx, vy, vz, gx, gy, gz, nx, ny, nz, px + v1[0] * dt / 2.0, py + v1[1] * dt / 2.0, pz + v1[2] * dt / 2.0, v2,
-	     cell, true);
-	vxyz(vx, vy, vz, gx, gy, gz, nx, ny, nz, px + v2[0] * dt / 2.0, py + v2[1] * dt / 2.0, pz + v2[2] * dt / 2.0, v3,
-	     cell, true);
-	vxyz(vx, vy, vz, gx, gy, gz, nx, ny, nz, px + v3[0] * dt, py + v3[1] * dt, pz + v3[2] * dt, v4, cell, true);
-	double v[3];
-	for (int q = 0; q < 3; q++) v[q] = (v1[q] + 2.0 * v2[q] + 2.0 * v3[q] + v4[q]) / 6.0;
+	if (mt >= 50 && mt < 100) {
+		/* water velocity of fluid markers (mark3mg.c:146-207) */
+		float ftk = tk[t];
+		vw[1] = A.mp.vyfluid;
+		if (A.md && A.md[A.id[t]] > 1100.0f) vw[1] = A.mp.vymelt;
+		if (A.mp.vyfluid > 0 && (ftk == 0 || ftk > 298.0f)) {
+			int m1 = cell[0], m2 = cell[1], m3 = cell[2];
+			if ((double)fx > (gx[m1] + gx[m1 + 1]) / 2.0) m1 += 1;
+			if (m1 < 1) m1 = 1;
+			if (m1 > nx - 2) m1 = nx - 2;
+			if ((double)fy > (gy[m2] + gy[m2 + 1]) / 2.0) m2 += 1;
+			if (m2 < 1) m2 = 1;
+			if (m2 > ny - 2) m2 = ny - 2;
+			if ((double)fz > (gz[m3] + gz[m3 + 1]) / 2.0) m3 += 1;
+			if (m3 < 1) m3 = 1;
+			if (m3 > nz - 2) m3 = nz - 2;
+			double xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]);
+			double ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+			double zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+			/* the reference scales all three offsets by the x factor */
+			double ax = xkf * ((double)fx - (gx[m1 - 1] + gx[m1]) / 2.0);
+			double ay = xkf * ((double)fy - (gy[m2 - 1] + gy[m2]) / 2.0);
+			double az = xkf * ((double)fz - (gz[m3 - 1] + gz[m3]) / 2.0);
+			long N = (long)nx * ny * nz, sy = ny, sxy = (long)nx * ny;
+			long p[8];
+			p[0] = (long)m3 * sxy + (long)m1 * sy + m2;
+			p[1] = p[0] + 1;
+			p[2] = p[0] + sy;
+			p[3] = p[2] + 1;
+			p[4] = p[0] + sxy;
+			p[5] = p[4] + 1;
+			p[6] = p[5] + sy; /* one y line above p[4] + sy, as the reference */
+			p[7] = p[6] + 1;
+			double pv[8];
+			for (int q = 0; q < 8; q++) pv[q] = (A.pr && p[q] < N) ? A.pr[p[q]] : 0.0;
+			double d[3];
+			d[0] = xkf * ((1.0 - ay) * (1.0 - az) * (pv[2] - pv[0]) + ay * (1.0 - az) * (pv[3] - pv[1]) +
+				      (1.0 - ay) * az * (pv[6] - pv[4]) + ay * az * (pv[7] - pv[5]));
+			d[1] = ykf * ((1.0 - ax) * (1.0 - az) * (pv[1] - pv[0]) + ax * (1.0 - az) * (pv[3] - pv[2]) +
+				      (1.0 - ax) * az * (pv[5] - pv[4]) + ax * az * (pv[7] - pv[6]));
+			d[2] = zkf * ((1.0 - ax) * (1.0 - ay) * (pv[4] - pv[0]) + ax * (1.0 - ay) * (pv[6] - pv[2]) +
+				      (1.0 - ax) * ay * (pv[5] - pv[1]) + ax * ay * (pv[7] - pv[3]));
+			double k[3];
+			for (int q = 0; q < 3; q++) {
+				k[q] = (1000.0 * A.mp.g[q] - d[q]) / (2300.0 * 9.81);
+				if (k[q] > 2.0) k[q] = 2.0;
+				if (k[q] < -2.0) k[q] = -2.0;
+			}
+			vw[0] = vw[1] * k[0];
+			vw[2] = vw[1] * k[2];
+			vw[1] *= k[1];
+		} else {
+			vw[0] = 0;
+			vw[1] = -fabs(vw[1]);
+			vw[2] = 0;
+		}
+	}
+	if (A.mp.markmod == 1) {
+		for (int q = 0; q < 3; q++) v[q] = v1[q] + vw[q];
+	} else {
+		/* stages 2-4 search their cell from the previous stage's (a step of
+		 * a fraction of a cell: one or two comparisons instead of a
+		 * bisection) */
+		for (int q = 0; q < 3; q++) v1[q] += vw[q];
+		vxyz(vx, vy, vz, gx, gy, gz, nx, ny, nz, px + v1[0] * dt / 2.0, py + v1[1] * dt / 2.0, pz + v1[2] * dt / 2.0,
+		     v2, cell, true);
+		for (int q = 0; q < 3; q++) v2[q] += vw[q];
+		vxyz(vx, vy, vz, gx, gy, gz, nx, ny, nz, px + v2[0] * dt / 2.0, py + v2[1] * dt / 2.0, pz + v2[2] * dt / 2.0,
+		     v3, cell, true);
+		for (int q = 0; q < 3; q++) v3[q] += vw[q];
+		vxyz(vx, vy, vz, gx, gy, gz, nx, ny, nz, px + v3[0] * dt, py + v3[1] * dt, pz + v3[2] * dt, v4, cell, true);
+		for (int q = 0; q < 3; q++) v4[q] += vw[q];
+		for (int q = 0; q < 3; q++) v[q] = (v1[q] + 2.0 * v2[q] + 2.0 * v3[q] + v4[q]) / 6.0;
+	}
+	/* motion normal to the faces only (mark3mg.c:245-251) */
+	if (A.mp.outgrid == 2) {
+		if (fx < 0 || (double)fx > xs) v[1] = v[2] = 0;
+		if (fy < 0 || (double)fy > ys) v[0] = v[2] = 0;
+		if (fz < 0 || (double)fz > zs) v[0] = v[1] = 0;
+	}
 	if (mt < 100) {
 		float ftk = tk[t];
 		double yold = fy;
@@ -921,6 +1078,11 @@ __global__ void k_rk4(long n, float *x, float *y, float *z, float *tk, float *et
 			tk[t] = 0;
 			eta[t] = 0;
 		}
+	} else {
+		/* immobile markers only move; their reset stays with the caller */
+		x[t] = (float)((double)fx + (dt * v[0] - 0.0));
+		y[t] = (float)((double)fy + dt * v[1]);
+		z[t] = (float)((double)fz + dt * v[2]);
 	}
 }
 
@@ -960,6 +1122,8 @@ extern "C" int i3g_markers_create(long nx, long ny, long nz, const double *gx, c
 	m->x = mk_alloc<float>(c, &err); m->y = mk_alloc<float>(c, &err); m->z = mk_alloc<float>(c, &err);
 	m->tk = mk_alloc<float>(c, &err); m->eta = mk_alloc<float>(c, &err);
 	m->e = mk_alloc<float>(c, &err); m->e2 = mk_alloc<float>(c, &err);
+	m->rr = mk_alloc<float>(c, &err); m->rr2 = mk_alloc<float>(c, &err);
+	m->rn = mk_alloc<float>(c, &err); m->rn2 = mk_alloc<float>(c, &err);
 	m->type = mk_alloc<signed char>(c, &err); m->id = mk_alloc<int>(c, &err);
 	m->x2 = mk_alloc<float>(c, &err); m->y2 = mk_alloc<float>(c, &err); m->z2 = mk_alloc<float>(c, &err);
 	m->tk2 = mk_alloc<float>(c, &err); m->eta2 = mk_alloc<float>(c, &err);
@@ -989,7 +1153,7 @@ extern "C" int i3g_markers_destroy(i3g_markers *m)
 		      m->tk2, m->eta2, m->type2, m->id2, m->key, m->c1, m->c2, m->c3, m->perm, m->cnt, m->start,
 		      m->cur, m->bsum, m->prec, m->pr, m->fld,
 		      m->vel, m->rocks, m->bad, m->tbc_p, m->tbc_c, m->tbc_k, m->tk1, m->tnod, m->trec, m->sn, m->tin,
-		      m->e, m->e2, m->st12};
+		      m->e, m->e2, m->st12, m->rr, m->rr2, m->rn, m->rn2, m->md};
 	for (void *p : ps)
 		if (p) hipFree(p);
 	if (m->ev0) hipEventDestroy(m->ev0);
@@ -1016,6 +1180,8 @@ extern "C" int i3g_markers_upload(i3g_markers *m, long n, const float *x, const 
 	MCHK(hipMemcpyAsync(m->eta, eta, n * sizeof(float), hipMemcpyHostToDevice, m->s));
 	MCHK(hipMemcpyAsync(m->type, type, n * sizeof(signed char), hipMemcpyHostToDevice, m->s));
 	MCHK(hipMemsetAsync(m->e, 0, (n ? n : 1) * sizeof(float), m->s));
+	MCHK(hipMemsetAsync(m->rr, 0, (n ? n : 1) * sizeof(float), m->s));
+	MCHK(hipMemsetAsync(m->rn, 0, (n ? n : 1) * sizeof(float), m->s));
 	if (id) {
 		MCHK(hipMemcpyAsync(m->id, id, n * sizeof(int), hipMemcpyHostToDevice, m->s));
 	} else {
@@ -1076,11 +1242,12 @@ extern "C" int i3g_markers_cell_index(i3g_markers *m, int *m1, int *m2, int *m3,
 /* gather every per-marker array into the second buffers, then swap */
 static void permute_all(i3g_markers *m)
 {
-	MkArrays s{m->x, m->y, m->z, m->tk, m->eta, m->e, m->type, m->id};
-	MkArrays d{m->x2, m->y2, m->z2, m->tk2, m->eta2, m->e2, m->type2, m->id2};
+	MkArrays s{m->x, m->y, m->z, m->tk, m->eta, m->e, m->rr, m->rn, m->type, m->id};
+	MkArrays d{m->x2, m->y2, m->z2, m->tk2, m->eta2, m->e2, m->rr2, m->rn2, m->type2, m->id2};
 	hipLaunchKernelGGL(k_permute, g1(m->n), dim3(256), 0, m->s, m->n, (const int *)m->perm, s, d);
 	std::swap(m->x, m->x2); std::swap(m->y, m->y2); std::swap(m->z, m->z2);
 	std::swap(m->tk, m->tk2); std::swap(m->eta, m->eta2); std::swap(m->e, m->e2);
+	std::swap(m->rr, m->rr2); std::swap(m->rn, m->rn2);
 	std::swap(m->type, m->type2); std::swap(m->id, m->id2);
 }
 
@@ -1126,6 +1293,7 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 {
 	if (!m || !rocks || !prm || !pr || nrock < 1 || nrock > MAXROCK) return I3G_EINVAL;
 	if (prm->viscmod < 0 || prm->viscmod > 2 || prm->densimod < 0 || prm->densimod > 1) return I3G_EINVAL;
+	if (prm->rheology && (prm->mnumy < 1 || prm->sedtype < 0 || prm->sedtype >= nrock)) return I3G_EINVAL;
 	if (!m->sorted) return I3G_EINVAL;
 	float *host[11] = {nu, ro, cp, kt, ht, et, tk, nxx, nxy, nxz, nyz};
 	for (int q = 0; q < 11; q++)
@@ -1133,9 +1301,12 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 	int err = 0;
 	long n = m->n, N = m->N;
 	if (!m->prec) {
-		long c = m->cap;
-		m->prec = mk_alloc<MkRec>(c, &err);
-		m->pr = mk_alloc<double>(N, &err); m->fld = mk_alloc<float>(11 * N, &err);
+		m->prec = mk_alloc<MkRec>(m->cap, &err);
+		m->fld = mk_alloc<float>(11 * N, &err);
+		if (err) return err;
+	}
+	if (!m->pr) {
+		m->pr = mk_alloc<double>(N, &err);
 		if (err) return err;
 	}
 	MCHK(hipMemcpyAsync(m->rocks, rocks, nrock * sizeof(i3g_rock), hipMemcpyDefault, m->s));
@@ -1146,10 +1317,11 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 	mark_begin(m);
 	if (n)
 		hipLaunchKernelGGL(k_marker_props, g1(n), dim3(256), 0, m->s, n, (const float *)m->x, (const float *)m->y,
-				   (const float *)m->z, (const float *)m->tk, m->eta, (const signed char *)m->type,
+				   (const float *)m->z, (const float *)m->tk, m->eta, m->type, m->e, m->rr, m->rn,
 				   (const int *)m->c1, (const int *)m->c2, (const int *)m->c3, (const double *)m->pr,
-				   (const double *)m->gx, (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz,
-				   (const i3g_rock *)m->rocks, nrock, *prm, m->prec, m->bad);
+				   (const float *)(m->has_st ? m->st12 : nullptr), N, (const double *)m->gx,
+				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, (const i3g_rock *)m->rocks,
+				   nrock, *prm, m->prec, m->bad);
 	dim3 gg((unsigned)(((long)m->nx * m->ny + 255) / 256), (unsigned)m->nz);
 	hipLaunchKernelGGL(k_marker_gather, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz, (const double *)m->gx,
 			   (const double *)m->gy, (const double *)m->gz, (const int *)m->start, (const int *)m->cnt,
@@ -1250,9 +1422,10 @@ extern "C" int i3g_markers_update_tk(i3g_markers *m, const float *tk, const floa
 	return I3G_OK;
 }
 
-extern "C" int i3g_markers_advect_rk4(i3g_markers *m, double dt, const float *vx, const float *vy, const float *vz)
+extern "C" int i3g_markers_advect_ex(i3g_markers *m, double dt, const float *vx, const float *vy, const float *vz,
+				     const i3g_move_params *mp, const double *pr, const float *markd)
 {
-	if (!m || !vx || !vy || !vz) return I3G_EINVAL;
+	if (!m || !vx || !vy || !vz || !mp || mp->outgrid < 0 || mp->outgrid > 2) return I3G_EINVAL;
 	long N = m->N, n = m->n;
 	int err = 0;
 	if (!m->vel) {
@@ -1262,17 +1435,67 @@ extern "C" int i3g_markers_advect_rk4(i3g_markers *m, double dt, const float *vx
 	MCHK(hipMemcpyAsync(m->vel, vx, N * sizeof(float), hipMemcpyDefault, m->s));
 	MCHK(hipMemcpyAsync(m->vel + N, vy, N * sizeof(float), hipMemcpyDefault, m->s));
 	MCHK(hipMemcpyAsync(m->vel + 2 * N, vz, N * sizeof(float), hipMemcpyDefault, m->s));
+	MoveArgs A{*mp, nullptr, nullptr, (const int *)m->id};
+	if (pr) {
+		if (!m->pr) {
+			m->pr = mk_alloc<double>(N, &err);
+			if (err) return err;
+		}
+		MCHK(hipMemcpyAsync(m->pr, pr, N * sizeof(double), hipMemcpyDefault, m->s));
+		A.pr = m->pr;
+	}
+	if (markd && n) {
+		/* densities by marker id: as many as the largest id + 1 */
+		std::vector<int> ids(n);
+		MCHK(hipMemcpyAsync(ids.data(), m->id, n * sizeof(int), hipMemcpyDeviceToHost, m->s));
+		MCHK(hipStreamSynchronize(m->s));
+		long nid = 0;
+		for (long i = 0; i < n; i++) nid = ids[i] + 1L > nid ? ids[i] + 1L : nid;
+		if (nid > m->md_cap) {
+			if (m->md) hipFree(m->md);
+			m->md = mk_alloc<float>(nid, &err);
+			if (err) { m->md = nullptr; m->md_cap = 0; return err; }
+			m->md_cap = nid;
+		}
+		MCHK(hipMemcpyAsync(m->md, markd, nid * sizeof(float), hipMemcpyDefault, m->s));
+		A.md = m->md;
+	}
 	mark_begin(m);
 	if (n)
 		hipLaunchKernelGGL(k_rk4, g1(n), dim3(256), lines_bytes(m), m->s, n, m->x, m->y, m->z, m->tk, m->eta,
 				   m->type, (const float *)m->vel, (const float *)(m->vel + N),
 				   (const float *)(m->vel + 2 * N), (const double *)m->gx, (const double *)m->gy,
-				   (const double *)m->gz, m->nx, m->ny, m->nz, dt, DepthRules{m->zdeep, m->vdeep, m->tdeep});
+				   (const double *)m->gz, m->nx, m->ny, m->nz, dt, DepthRules{m->zdeep, m->vdeep, m->tdeep}, A);
 	MCHK(hipGetLastError());
 	mark_end(m);
 	MCHK(hipStreamSynchronize(m->s));
 	mark_read(m);
 	m->sorted = 0;
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_advect_rk4(i3g_markers *m, double dt, const float *vx, const float *vy, const float *vz)
+{
+	i3g_move_params mp;
+	memset(&mp, 0, sizeof(mp));
+	return i3g_markers_advect_ex(m, dt, vx, vy, vz, &mp, nullptr, nullptr);
+}
+
+extern "C" int i3g_markers_set_grain(i3g_markers *m, const float *rr, const float *rn)
+{
+	if (!m || (!rr && !rn)) return I3G_EINVAL;
+	if (m->n && rr) MCHK(hipMemcpyAsync(m->rr, rr, m->n * sizeof(float), hipMemcpyDefault, m->s));
+	if (m->n && rn) MCHK(hipMemcpyAsync(m->rn, rn, m->n * sizeof(float), hipMemcpyDefault, m->s));
+	MCHK(hipStreamSynchronize(m->s));
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_get_grain(i3g_markers *m, float *rr, float *rn)
+{
+	if (!m || (!rr && !rn)) return I3G_EINVAL;
+	if (m->n && rr) MCHK(hipMemcpyAsync(rr, m->rr, m->n * sizeof(float), hipMemcpyDefault, m->s));
+	if (m->n && rn) MCHK(hipMemcpyAsync(rn, m->rn, m->n * sizeof(float), hipMemcpyDefault, m->s));
+	MCHK(hipStreamSynchronize(m->s));
 	return I3G_OK;
 }
 
@@ -1307,9 +1530,23 @@ static int upload_st12(i3g_markers *m, const float *const *st12, const double *p
 	}
 	for (int q = 0; q < 12; q++) {
 		if (!st12[q]) return I3G_EINVAL;
-		MCHK(hipMemcpyAsync(m->st12 + q * N, st12[q], N * sizeof(float), hipMemcpyHostToDevice, m->s));
+		MCHK(hipMemcpyAsync(m->st12 + q * N, st12[q], N * sizeof(float), hipMemcpyDefault, m->s));
 	}
-	MCHK(hipMemcpyAsync(m->pr, pr, N * sizeof(double), hipMemcpyHostToDevice, m->s));
+	m->has_st = 1;
+	if (pr) MCHK(hipMemcpyAsync(m->pr, pr, N * sizeof(double), hipMemcpyDefault, m->s));
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_set_stress(i3g_markers *m, const float *const *st12)
+{
+	if (!m) return I3G_EINVAL;
+	if (!st12) {
+		m->has_st = 0;
+		return I3G_OK;
+	}
+	int rc = upload_st12(m, st12, nullptr);
+	if (rc) return rc;
+	MCHK(hipStreamSynchronize(m->s));
 	return I3G_OK;
 }
 
@@ -1343,18 +1580,19 @@ extern "C" int i3g_markers_update_props(i3g_markers *m, const i3g_rock *rocks, i
 					double healrate, const float *const *st12, const double *pr)
 {
 	if (!m || !rocks || !prm || !st12 || !pr || nrock < 1 || nrock > MAXROCK) return I3G_EINVAL;
+	if (prm->outgrid < 0 || prm->outgrid > 2) return I3G_EINVAL;
 	long n = m->n;
 	int rc = upload_st12(m, st12, pr);
 	if (rc) return rc;
-	MCHK(hipMemcpyAsync(m->rocks, rocks, nrock * sizeof(i3g_rock), hipMemcpyHostToDevice, m->s));
+	MCHK(hipMemcpyAsync(m->rocks, rocks, nrock * sizeof(i3g_rock), hipMemcpyDefault, m->s));
 	MCHK(hipMemsetAsync(m->bad, 0, sizeof(int), m->s));
 	mark_begin(m);
 	if (n)
 		hipLaunchKernelGGL(k_mk_props, g1(n), dim3(256), lines_bytes(m), m->s, n, (const float *)m->x,
-				   (const float *)m->y, (const float *)m->z, (const float *)m->tk, m->eta, m->e,
+				   (const float *)m->y, (const float *)m->z, (const float *)m->tk, m->eta, m->e, m->rr, m->rn,
 				   (const signed char *)m->type, (const float *)m->st12, m->N, (const double *)m->pr,
 				   (const double *)m->gx, (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz,
-				   (const i3g_rock *)m->rocks, nrock, prm->nubeg, prm->nuend, prm->timestep, healrate, m->bad);
+				   (const i3g_rock *)m->rocks, nrock, *prm, healrate, m->bad);
 	MCHK(hipGetLastError());
 	mark_end(m);
 	int bad = 0;

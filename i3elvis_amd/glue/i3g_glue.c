@@ -23,16 +23,22 @@
  *   i3g_titerate   the heat step of titerate (heat3mg.c:287-588) with the
  *                  adiabyn/frictyn heating terms, then the marker
  *                  temperature update (heat3mg.c:592-782).
- *   i3g_ronurecalc the node fields of ronurecalc (mark3mg.c:404-1015) for
- *                  Newtonian rock types, with tkrecalc.
- *   i3g_movemark   movemark (mark3mg.c:2-397) for the Newtonian rock types:
- *                  viscosity and accumulated strain at the old positions
- *                  (epscalc1 + viscalc), then the RK4 position update.
+ *   i3g_ronurecalc ronurecalc (mark3mg.c:404-1015) with the whole viscalc,
+ *                  retyping, water/air mixing and phase densities, with
+ *                  tkrecalc; melt types (21-49) and densimod >= 2 stop the
+ *                  program with a message.
+ *   i3g_movemark   movemark (mark3mg.c:2-397): topography of the markers,
+ *                  viscosity, strain and grain state at the old positions
+ *                  (epscalc1 + viscalc), the motion (outgrid, markmod,
+ *                  fluid and immobile markers), then the immobile-marker
+ *                  regeneration and the refill of markers that left.
  * Errors print a message and exit, as the reference does on fatal
  * conditions.
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "i3g.h"
 
@@ -109,29 +115,34 @@ static void glue_markers(void)
 	}
 	if ((rc = i3g_markers_upload(G.mk, marknum, markx, marky, markz, markk, markv, (const signed char *)markt, NULL)))
 		glue_fail("markers_upload", rc);
+	/* the rheology's per-marker state: strain, grain roughness,
+	 * dislocation share */
+	if (marknum && ((rc = i3g_markers_set_strain(G.mk, marke)) || (rc = i3g_markers_set_grain(G.mk, markrr, markrn))))
+		glue_fail("markers_set_state", rc);
 }
 
-/* device markers back into markx..markv (by id); with_strain (after
- * movemark) also marke and markt */
-static void glue_markers_back(int with_strain)
+/* device markers back into the program's arrays (by id): positions,
+ * temperature, viscosity, type (ronurecalc's retyping, movemark's depth
+ * rules), strain, grain roughness and dislocation share */
+static void glue_markers_back(void)
 {
 	int rc;
 	long n = marknum, i;
-	float *buf = malloc(sizeof(float) * 6 * (n ? n : 1));
+	float *buf = malloc(sizeof(float) * 8 * (n ? n : 1));
 	signed char *t = malloc(n ? n : 1);
 	int *id = malloc(sizeof(int) * (n ? n : 1));
 	if (!buf || !t || !id) glue_fail("host allocation", I3G_ENOMEM);
 	if ((rc = i3g_markers_download(G.mk, buf, buf + n, buf + 2 * n, buf + 3 * n, buf + 4 * n, t, id)))
 		glue_fail("markers_download", rc);
-	if (with_strain && (rc = i3g_markers_get_strain(G.mk, buf + 5 * n))) glue_fail("markers_get_strain", rc);
+	if (n && ((rc = i3g_markers_get_strain(G.mk, buf + 5 * n)) ||
+		  (rc = i3g_markers_get_grain(G.mk, buf + 6 * n, buf + 7 * n))))
+		glue_fail("markers_get_state", rc);
 	for (i = 0; i < n; i++) {
 		long j = id[i];
 		markx[j] = buf[i]; marky[j] = buf[n + i]; markz[j] = buf[2 * n + i];
 		markk[j] = buf[3 * n + i]; markv[j] = buf[4 * n + i];
-		if (with_strain) {
-			marke[j] = buf[5 * n + i];
-			markt[j] = t[i]; /* the depth rules of movemark change types */
-		}
+		marke[j] = buf[5 * n + i]; markrr[j] = buf[6 * n + i]; markrn[j] = buf[7 * n + i];
+		markt[j] = t[i];
 	}
 	free(buf); free(t); free(id);
 }
@@ -146,8 +157,25 @@ static int glue_rocks(i3g_rock *rocks)
 		rocks[t].ro = markro[t]; rocks[t].bro = markbb[t]; rocks[t].aro = markaa[t];
 		rocks[t].cp = markcp[t]; rocks[t].kt = markkt[t]; rocks[t].kf = markkf[t]; rocks[t].kp = markkp[t];
 		rocks[t].ht = markht[t];
+		rocks[t].ss = markss[t]; rocks[t].mm = markmm[t]; rocks[t].ll = markll[t]; rocks[t].s1 = marks1[t];
+		rocks[t].a0 = marka0[t]; rocks[t].a1 = marka1[t]; rocks[t].b0 = markb0[t]; rocks[t].b1 = markb1[t];
+		rocks[t].e0 = marke0[t]; rocks[t].e1 = marke1[t];
 	}
 	return nr;
+}
+
+/* i3g_mk_params from the program's globals, whole rheology */
+static void glue_mk_params(i3g_mk_params *mp, int sedtype)
+{
+	memset(mp, 0, sizeof(*mp));
+	mp->nubeg = nubeg; mp->nuend = nuend; mp->nukoef = nukoef; mp->densimod = densimod; mp->timestep = timestep;
+	mp->vdeep = vdeep; mp->nudeep = nudeep; mp->viscmod = viscmod;
+	mp->rheology = 1;
+	mp->eroslev = eroslev; mp->sedilev = sedilev; mp->waterlev = waterlev; mp->sedtype = sedtype;
+	mp->mnumy = mnumy > 0 ? mnumy : 1;
+	mp->teclmin = teclmin; mp->teclmax = teclmax;
+	mp->timesum = timesum; mp->convend = convend;
+	mp->outgrid = outgrid;
 }
 
 void i3g_vpiterate(int m0, int f0)
@@ -238,7 +266,7 @@ void i3g_titerate(int m0)
 	if ((rc = i3g_markers_sort(G.mk, NULL))) glue_fail("markers_sort", rc);
 	rc = i3g_markers_update_tk(G.mk, tk, G.tkstart, kt, ro, cp, heatdif, timestep, zdeep, tdeep);
 	if (rc) glue_fail("markers_update_tk", rc);
-	glue_markers_back(0);
+	glue_markers_back();
 }
 
 void i3g_ronurecalc(void)
@@ -252,30 +280,124 @@ void i3g_ronurecalc(void)
 	static i3g_rock rocks[64];
 	int nr = glue_rocks(rocks);
 	(void)t;
+	/* the sediment type of this call, as ronurecalc derives it
+	 * (mark3mg.c:421-424) */
+	sedimnum++;
+	long c1 = (long)(sedimnum / sedimcyc), c2 = (c1 / 2) * 2;
 	i3g_mk_params mp;
-	mp.nubeg = nubeg; mp.nuend = nuend; mp.nukoef = nukoef; mp.densimod = densimod; mp.timestep = timestep;
-	mp.vdeep = vdeep; mp.nudeep = nudeep; mp.viscmod = viscmod;
+	glue_mk_params(&mp, c2 == c1 ? 3 : 4);
+	/* viscalc's strain rates and stresses (epscalc1 at the markers) */
+	const float *st12[12] = {exx, eyy, ezz, exy, exz, eyz, sxx, syy, szz, sxy, sxz, syz};
+	if ((rc = i3g_markers_set_stress(G.mk, st12))) glue_fail("markers_set_stress", rc);
 	rc = i3g_markers_to_nodes(G.mk, rocks, nr, &mp, pr, nu, ro, cp, kt, ht, et, tk, nxx, nxy, nxz, nyz);
 	if (rc) glue_fail("markers_to_nodes", rc);
-	glue_markers_back(0);
+	glue_markers_back();
+}
+
+/* topography surface where the node density crosses topodens along y
+ * (mark3mg.c:27-41), and each marker of types 2-19 with T > 0 that moves
+ * gets its height (:75-88) */
+static void glue_topography(void)
+{
+	long m1, m2, m3, m4, mm1;
+	const double topodens = 2000.0;
+	double *top = bufv + 2 * xnumx * znumz;
+	for (m3 = 0; m3 < znumz; m3++)
+		for (m1 = 0; m1 < xnumx; m1++)
+			for (m2 = 1; m2 < ynumy; m2++) {
+				m4 = m3 * xnumx * ynumy + m1 * ynumy + m2;
+				if (ro[m4] > topodens && ro[m4 - 1] < topodens) {
+					top[m3 * xnumx + m1] = gy[m2 - 1] + (gy[m2] - gy[m2 - 1]) * (topodens - ro[m4 - 1]) / (ro[m4] - ro[m4 - 1]);
+					break;
+				}
+			}
+	for (mm1 = 0; mm1 < marknum; mm1++) {
+		int inside = markx[mm1] >= 0 && marky[mm1] >= 0 && markz[mm1] >= 0 && (double)markx[mm1] <= xsize &&
+			     (double)marky[mm1] <= ysize && (double)markz[mm1] <= zsize;
+		if (!(inside || (outgrid != 1 && (markt[mm1] < 50 || markt[mm1] >= 100)))) continue;
+		if (!(markt[mm1] > 1 && markt[mm1] < 20 && markk[mm1] > 0)) continue;
+		double x = markx[mm1], z = markz[mm1];
+		if (x < 0) x = 0; if (x > xsize) x = xsize;
+		if (z < 0) z = 0; if (z > zsize) z = zsize;
+		long w0 = m1serch(x), w2 = m3serch(z);
+		m4 = w2 * xnumx + w0;
+		double dx = (markx[mm1] - gx[w0]) / (gx[w0 + 1] - gx[w0]);
+		double dz = (markz[mm1] - gz[w2]) / (gz[w2 + 1] - gz[w2]);
+		markto[mm1] = (float)((1.0 - dx) * (1.0 - dz) * top[m4] + dx * (1.0 - dz) * top[m4 + 1] +
+				      (1.0 - dx) * dz * top[m4 + xnumx] + dx * dz * top[m4 + xnumx + 1]);
+	}
+}
+
+/* the end of movemark on the host (mark3mg.c:290-385): when an immobile
+ * marker (type >= 100) has moved farther than stp100 from its origin
+ * (markk, markd, markw), every immobile marker inside the grid spawns a
+ * normal marker of type - 100 and returns to its origin; spawned markers
+ * then refill the slots of normal markers that left the grid */
+static void glue_immobile(void)
+{
+	long mm1, marknum1 = marknum;
+	int ynreset = 0;
+	for (mm1 = 0; mm1 < marknum && !ynreset; mm1++)
+		if (markt[mm1] >= 100) {
+			double dx = markx[mm1] - markk[mm1], dy = marky[mm1] - markd[mm1], dz = markz[mm1] - markw[mm1];
+			if (pow(dx * dx + dy * dy + dz * dz, 0.5) > stp100) ynreset = 1;
+		}
+	if (!ynreset) return;
+	for (mm1 = 0; mm1 < marknum; mm1++)
+		if (markt[mm1] >= 100) {
+			if (markx[mm1] > 0 && marky[mm1] > 0 && markz[mm1] > 0 && (double)markx[mm1] < xsize &&
+			    (double)marky[mm1] < ysize && (double)markz[mm1] < zsize) {
+				if (marknum1 >= MAXMRK) glue_fail("marker arrays full", I3G_ECAP);
+				markt[marknum1] = markt[mm1] - 100;
+				markx[marknum1] = markx[mm1]; marky[marknum1] = marky[mm1]; markz[marknum1] = markz[mm1];
+				markk[marknum1] = 0; markv[marknum1] = 0; markft[marknum1] = 0;
+				markex[marknum1] = markex[mm1]; marke[marknum1] = marke[mm1];
+				markd[marknum1] = -1.0; markw[marknum1] = -1.0; markrr[marknum1] = 0;
+				marknum1++;
+			}
+			markx[mm1] = markk[mm1]; marky[mm1] = markd[mm1]; markz[mm1] = markw[mm1];
+		}
+	mm1 = 0;
+	while (marknum1 > marknum && mm1 < marknum) {
+		if (markt[mm1] < 100 && (markx[mm1] < 0 || marky[mm1] < 0 || markz[mm1] < 0 || (double)markx[mm1] > xsize ||
+					 (double)marky[mm1] > ysize || (double)markz[mm1] > zsize)) {
+			marknum1--;
+			markt[mm1] = markt[marknum1];
+			markk[mm1] = 0; markv[mm1] = 0; markft[mm1] = 0;
+			markex[mm1] = markex[marknum1]; marke[mm1] = marke[marknum1];
+			markd[mm1] = -1.0; markw[mm1] = -1.0; markrr[mm1] = 0;
+			markx[mm1] = markx[marknum1]; marky[mm1] = marky[marknum1]; markz[mm1] = markz[marknum1];
+		}
+		mm1++;
+	}
+	marknum = marknum1;
 }
 
 void i3g_movemark(void)
 {
 	int rc;
+	long i;
 	static i3g_rock rocks[64];
 	glue_grid();
+	glue_topography();
 	glue_markers();
-	if ((rc = i3g_markers_set_strain(G.mk, marke))) glue_fail("markers_set_strain", rc);
-	/* viscosity and accumulated strain at the old positions (epscalc1 +
-	 * viscalc), then the RK4 move */
+	/* viscosity, strain and grain state at the old positions (epscalc1 +
+	 * viscalc), then the motion */
 	int nr = glue_rocks(rocks);
 	i3g_mk_params mp;
-	mp.nubeg = nubeg; mp.nuend = nuend; mp.nukoef = nukoef; mp.densimod = densimod; mp.timestep = timestep;
-	mp.vdeep = vdeep; mp.nudeep = nudeep; mp.viscmod = viscmod;
+	glue_mk_params(&mp, 3);
 	const float *st12[12] = {exx, eyy, ezz, exy, exz, eyz, sxx, syy, szz, sxy, sxz, syz};
 	if ((rc = i3g_markers_update_props(G.mk, rocks, nr, &mp, healrate, st12, pr))) glue_fail("markers_update_props", rc);
 	if ((rc = i3g_markers_set_depth_rules(G.mk, zdeep, vdeep, tdeep))) glue_fail("markers_set_depth_rules", rc);
-	if ((rc = i3g_markers_advect_rk4(G.mk, timestep, vx, vy, vz))) glue_fail("markers_advect_rk4", rc);
-	glue_markers_back(1);
+	i3g_move_params mv;
+	mv.outgrid = outgrid; mv.markmod = markmod; mv.vyfluid = vyfluid; mv.vymelt = vymelt;
+	mv.g[0] = GXKOEF; mv.g[1] = GYKOEF; mv.g[2] = GZKOEF;
+	int fluids = 0;
+	for (i = 0; i < marknum && !fluids; i++) fluids = markt[i] >= 50 && markt[i] < 100;
+	if ((rc = i3g_markers_advect_ex(G.mk, timestep, vx, vy, vz, &mv, fluids ? pr : NULL, fluids ? markd : NULL)))
+		glue_fail("markers_advect_ex", rc);
+	glue_markers_back();
+	glue_immobile();
+	/* melt extraction regions cleared, as movemark's end (mark3mg.c:388-392) */
+	for (i = 0; i < xnumx * znumz * 5; i++) bufv[i] = 0;
 }

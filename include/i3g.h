@@ -448,29 +448,70 @@ int i3g_heat_solve(i3g_heat *h, float *tk, const float *ro, const float *cp, con
  * sorted by cell.  Marker ids give the reference's marker order. */
 typedef struct i3g_markers i3g_markers;
 
-/* Rock properties of one marker type (the per-type tables of the reference
- * deck, init.t3c ROCKS_DESCRIPTIONS; mark3mg.c:1114-1299, 3006-3024):
- * Newtonian rheology branch A (E = V = 0) or B (Arrhenius), density
- * ro*(1-bro*(T-298.15))*(1+aro*(P_bar-1)*1e-3), conductivity
- * (kt + kf/(T+77))*exp(kp*P_bar).  Brittle/power-law terms are zero. */
+/* Rock properties of one marker type: the per-type table of the reference
+ * deck (init.t3c ROCKS_DESCRIPTIONS, read by in3mg.c:204-231) as viscalc
+ * (mark3mg.c:1114-1681), dencalc1 (:3006-3024) and ronurecalc use it.
+ * Density ro*(1-bro*(T-298.15))*(1+aro*(P_bar-1)*1e-3), conductivity
+ * (kt + kf/(T+77))*exp(kp*P_bar).  Viscosity: with ss = 0 (or mm = 1)
+ * Newtonian, constant (dh = dv = 0) or exp((dh + dv P)/RT); with ss and
+ * mm power law (dh > 0: P-T dependent); nu0 < 0 Karato & Wu with ss, ll,
+ * s1; dh < 0 the grain-size model (nu0, -dh, dv dislocation; s1, ss, ll
+ * diffusion); a0/a1, b0/b1 the brittle strength a + b P at accumulated
+ * strains below e0 / above e1.  Zero in the fields after ht leaves only
+ * the Newtonian branches. */
 typedef struct {
 	double nu0, dh, dv;        /* marknu, markdh (J), markdv (J/bar) */
 	double n0, n1;             /* markn0, markn1 viscosity limits */
 	double ro, bro, aro;       /* markro, markbb, markaa */
 	double cp, kt, kf, kp, ht; /* markcp, markkt, markkf, markkp, markht */
+	double ss, mm, ll, s1;     /* markss, markmm, markll, marks1 */
+	double a0, a1, b0, b1;     /* marka0, marka1, markb0, markb1 */
+	double e0, e1;             /* marke0, marke1 */
 } i3g_rock;
 
-/* Parameters of the marker -> node transfer (mode.t3c values used by
- * ronurecalc): nubeg, nuend, nukoef, densimod (0|1), timestep (>0: the
- * stored marker viscosity is reused), vdeep/nudeep viscosity cap below
- * vdeep, viscmod (0 only). */
+/* Parameters of the marker -> node transfer (the mode.t3c values that
+ * ronurecalc uses) and of movemark's property update: nubeg, nuend,
+ * nukoef, densimod (0|1), timestep (> 0: the transfer reuses the stored
+ * marker viscosity; the strain step of the property update), vdeep/nudeep
+ * viscosity cap below vdeep, viscmod (0, 1, 2).
+ * rheology 0: marker types whose ronurecalc branches go beyond viscalc's
+ * Newtonian forms (< 2, 7, 8, 16-18, > 20) are refused (I3G_EINVAL), the
+ * fields after it are ignored.  rheology 1: the whole viscalc with the
+ * strain, grain roughness and dislocation share of the markers; in the
+ * transfer the retyping by eroslev / sedilev (to sedtype, the 3 or 4 that
+ * ronurecalc derives from sedimnum/sedimcyc) / waterlev, the water/air
+ * proportions of markers at the water level (mnumy markers per cell in y),
+ * viscalc from the strain rates and stresses of i3g_markers_set_stress,
+ * and the phase-transition densities damped between teclmin and teclmax;
+ * melt types (21-49) are refused.  timesum and convend (s) drive the grain
+ * growth and the dislocation share.  outgrid: movemark's, for the markers
+ * the property update covers (as i3g_move_params.outgrid). */
 typedef struct {
 	double nubeg, nuend, nukoef;
 	int densimod;
 	double timestep;
 	double vdeep, nudeep;
 	int viscmod;
+	int rheology;
+	double eroslev, sedilev, waterlev;
+	int sedtype;
+	long mnumy;
+	double teclmin, teclmax;
+	double timesum, convend;
+	int outgrid;
 } i3g_mk_params;
+
+/* movemark's motion parameters (mark3mg.c:61, 146-284): outgrid (0 markers
+ * outside the grid move too, 1 they stay, 2 they move only normal to the
+ * face they are outside of), markmod (1 one velocity evaluation, else
+ * RK4), vyfluid / vymelt the fluid velocities of types 50-99 (vymelt for
+ * markd above 1100) and g the gravity (GXKOEF..GZKOEF) that turns them
+ * along the pressure gradient. */
+typedef struct {
+	int outgrid, markmod;
+	double vyfluid, vymelt;
+	double g[3];
+} i3g_move_params;
 
 int i3g_markers_create(long nx, long ny, long nz, const double *gx, const double *gy,
 		       const double *gz, long capacity, i3g_markers **out);
@@ -497,12 +538,16 @@ int i3g_markers_cell_index(i3g_markers *m, int *m1, int *m2, int *m3, int *key);
  * NULL) receives the previous position of each marker. */
 int i3g_markers_sort(i3g_markers *m, long *order);
 
-/* Marker -> node transfer; replaces ronurecalc (mark3mg.c:404-1015) for
- * marker types without melt/phase-change/water branches.  pr: cell
- * pressures (level-0 node layout, host).  Node arrays are in/out (a node
- * with too little marker weight keeps its value, as the reference).
- * Markers must be sorted.  Deterministic: every node sums its markers in
- * ascending id, the order of the reference's serial loop. */
+/* Marker -> node transfer; replaces ronurecalc (mark3mg.c:404-1015)
+ * except its melt branch (types 21-49) and the thermodynamic tables
+ * (densimod >= 2), which return I3G_EINVAL; prm->rheology selects the
+ * Newtonian subset or the whole rheology (see i3g_mk_params).  pr: cell
+ * pressures (level-0 node layout, host or device).  Node arrays are
+ * in/out (a node with too little marker weight keeps its value, as the
+ * reference).  Markers must be sorted; their types, viscosities and
+ * strain/grain state are updated as ronurecalc updates them.
+ * Deterministic: every node sums its markers in ascending id, the order
+ * of the reference's serial loop. */
 int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int nrock, const i3g_mk_params *prm,
 			 const double *pr, float *nu, float *ro, float *cp, float *kt, float *ht,
 			 float *et, float *tk, float *nxx, float *nxy, float *nxz, float *nyz);
@@ -530,13 +575,15 @@ int i3g_markers_get_strain(i3g_markers *m, float *e);
 int i3g_markers_epscalc(i3g_markers *m, const float *const *st12, const double *pr, double *out);
 
 /* movemark's property update ahead of the position update
- * (mark3mg.c:61-145) for the Newtonian rock types: every marker of type
- * < 50 with T > 0 gets eta = viscalc (branches A/B and limits) at the
- * pressure epscalc1 gives at the marker; viscalc marks it as not yielding
- * (these rocks have no brittle strength), so its accumulated strain is
- * made non-positive and heals by timestep * healrate toward 0.  prm: nubeg,
- * nuend, timestep.  Types with other rheology branches return I3G_EINVAL
- * (as i3g_markers_to_nodes).  Then i3g_markers_advect_rk4 moves them. */
+ * (mark3mg.c:61-145): every marker that moves (prm->outgrid) of type < 50
+ * with T > 0 gets eta = viscalc at the strain-rate and stress invariants,
+ * dissipation and pressure epscalc1 gives at the marker (with the
+ * numerical-diffusion correction of the strain rate, :97-105); a yielding
+ * marker's strain grows by timestep * eii, every strain heals by
+ * timestep * healrate toward 0.  prm: nubeg, nuend, timestep, rheology
+ * (0: types < 2, 7, 8, 16-18, > 20 return I3G_EINVAL; 1: all, with
+ * timesum, convend and the grain state), outgrid.  Then
+ * i3g_markers_advect_ex moves them. */
 int i3g_markers_update_props(i3g_markers *m, const i3g_rock *rocks, int nrock, const i3g_mk_params *prm,
 			     double healrate, const float *const *st12, const double *pr);
 
@@ -566,9 +613,40 @@ int i3g_markers_update_tk(i3g_markers *m, const float *tk, const float *tk2, con
 
 /* RK4 advection of the resident markers by dt in the staggered velocity
  * field (vxyzcalc1 + movemark, mark3mg.c:215-284, 2347-2530); markers that
- * leave the grid get tk = eta = 0 (mark3mg.c:280-284). */
+ * leave the grid get tk = eta = 0 (mark3mg.c:280-284).  The motion of
+ * i3g_markers_advect_ex with outgrid 0, markmod 0 and no fluid velocity. */
 int i3g_markers_advect_rk4(i3g_markers *m, double dt, const float *vx, const float *vy,
 			   const float *vz);
+
+/* movemark's motion (mark3mg.c:61, 146-284) with its parameters: which
+ * markers move (outgrid), RK4 or one evaluation (markmod), the water
+ * velocity of fluid markers (types 50-99: vymelt where markd > 1100, else
+ * vyfluid; with vyfluid > 0 and T 0 or above 298 K turned along the
+ * pressure gradient of the 8 P nodes around the marker as the reference
+ * does it, including its use of the x cell factor for the y and z
+ * offsets and its y shift of the two upper nodes; a node index past the
+ * last node reads 0), outgrid 2's motion normal to the faces, the depth
+ * rules and out-of-grid reset of types < 100, and the motion of immobile
+ * markers (>= 100; their reset and the marker regeneration stay with the
+ * caller).  pr: P (level-0 nodes, double); markd: the markers' density by
+ * marker id (may be NULL when no fluid marker moves or vyfluid and vymelt
+ * are 0).  Host or device pointers. */
+int i3g_markers_advect_ex(i3g_markers *m, double dt, const float *vx, const float *vy,
+			  const float *vz, const i3g_move_params *mp, const double *pr,
+			  const float *markd);
+
+/* Grain roughness (markrr) and dislocation share (markrn) of the markers
+ * in the current order (as i3g_markers_download); upload sets both to 0,
+ * sorting keeps them with their marker.  The grain-size rheology reads and
+ * updates them. */
+int i3g_markers_set_grain(i3g_markers *m, const float *rr, const float *rn);
+int i3g_markers_get_grain(i3g_markers *m, float *rr, float *rn);
+
+/* Strain rates and stresses (12 float node planes exx eyy ezz exy exz eyz
+ * sxx syy szz sxy sxz syz, host or device) that the transfer's viscalc
+ * reads (rheology 1, markers without a viscosity or timestep <= 0); NULL
+ * clears them (zero strain rates). */
+int i3g_markers_set_stress(i3g_markers *m, const float *const *st12);
 
 /* ------------------------------------------------------------------ */
 /* Learned warm start                                                  */

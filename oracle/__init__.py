@@ -377,3 +377,119 @@ def maxvxyz(vx, vy, vz, nx, ny, nz, divvmax=0.0):
     if divvmax > 0 and mv > divvmax:
         mv = divvmax
     return mv
+
+
+# ---- full rheology -----------------------------------------------------------
+
+ROCKX_FIELDS = ROCK_FIELDS + ("ss", "mm", "ll", "s1", "a0", "a1", "b0", "b1", "e0", "e1")
+
+
+class RockX(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ROCKX_FIELDS]
+
+
+class VisPrm(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("nubeg", "nuend", "timestep", "timesum", "convend")]
+
+
+class MkX(C.Structure):
+    _fields_ = [("nukoef", C.c_double), ("densimod", C.c_int), ("vdeep", C.c_double), ("nudeep", C.c_double),
+                ("eroslev", C.c_double), ("sedilev", C.c_double), ("waterlev", C.c_double), ("sedtype", C.c_int),
+                ("mnumy", C.c_long), ("teclmin", C.c_double), ("teclmax", C.c_double)]
+
+
+class MoveX(C.Structure):
+    _fields_ = [("outgrid", C.c_int), ("markmod", C.c_int), ("vyfluid", C.c_double), ("vymelt", C.c_double),
+                ("g", C.c_double * 3)]
+
+
+def rocksx_array(rocks):
+    """rocks: list of dicts with ROCKX_FIELDS (missing ones 0)"""
+    arr = (RockX * len(rocks))()
+    for i, r in enumerate(rocks):
+        for f in ROCKX_FIELDS:
+            setattr(arr[i], f, float(r.get(f, 0.0)))
+    return arr
+
+
+def _visprm(nubeg, nuend, timestep, timesum, convend):
+    return VisPrm(float(nubeg), float(nuend), float(timestep), float(timesum), float(convend))
+
+
+def viscalc(rock, mm2, mtk, mpb, eii, sii, rii, dis, e, rr, rn, nubeg=1e18, nuend=1e25, timestep=0.0,
+            timesum=0.0, convend=0.0):
+    """viscalc for one marker: returns (eta, e, rr, rn)"""
+    ra = rocksx_array([rock])
+    fe, frr, frn = C.c_float(e), C.c_float(rr), C.c_float(rn)
+    lib().or_viscalc.restype = C.c_double
+    P = _visprm(nubeg, nuend, timestep, timesum, convend)
+    v = lib().or_viscalc(ra, C.c_int(mm2), C.c_double(mtk), C.c_double(mpb), C.c_double(eii), C.c_double(sii),
+                         C.c_double(rii), C.c_double(dis), C.byref(fe), C.byref(frr), C.byref(frn), C.byref(P))
+    return v, fe.value, frr.value, frn.value
+
+
+def mk_props_x(x, y, z, mtype, tk, eta, e, rr, rn, rocks, st12, pr, gx, gy, gz, nubeg, nuend, dt, timesum,
+               convend, healrate, outgrid=0):
+    """movemark's property update with the full viscalc; returns copies of
+    (eta, e, rr, rn)"""
+    eta, e, rr, rn = (np.array(a, np.float32, copy=True) for a in (eta, e, rr, rn))
+    st = [np.ascontiguousarray(a, np.float32) for a in st12]
+    sp = (C.c_void_p * 12)(*[a.ctypes.data for a in st])
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
+    f = [np.ascontiguousarray(a, np.float32) for a in (x, y, z)]
+    t = np.ascontiguousarray(mtype, np.int8)
+    tkf = np.ascontiguousarray(tk, np.float32)
+    prd = np.ascontiguousarray(pr, np.float64)
+    ra = rocksx_array(rocks)
+    P = _visprm(nubeg, nuend, dt, timesum, convend)
+    lib().or_mk_props_x(C.c_long(len(x)), *[_p(a) for a in f], _p(t), _p(tkf), _p(eta), _p(e), _p(rr), _p(rn), ra,
+                        sp, _p(prd), _p(gx), C.c_long(len(gx)), _p(gy), C.c_long(len(gy)), _p(gz),
+                        C.c_long(len(gz)), C.byref(P), C.c_double(healrate), C.c_int(outgrid))
+    return eta, e, rr, rn
+
+
+def ronurecalc_x(x, y, z, tk, eta, mtype, e, rr, rn, gx, gy, gz, pr, st12, rocks, fld, nubeg, nuend, timestep,
+                 timesum, convend, nukoef=0.0, densimod=1, vdeep=1e30, nudeep=1e30, eroslev=-1e30, sedilev=1e30,
+                 waterlev=-1.0, sedtype=3, mnumy=2, teclmin=0.0, teclmax=0.0):
+    """ronurecalc with the full rheology; returns (fld, eta, type, e, rr,
+    rn) copies; raises on a marker it does not restate (melt types,
+    densimod >= 2)"""
+    fld = np.array(fld, np.float32, copy=True)
+    eta, e, rr, rn = (np.array(a, np.float32, copy=True) for a in (eta, e, rr, rn))
+    t = np.array(mtype, np.int8, copy=True)
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
+    f = [np.ascontiguousarray(a, np.float32) for a in (x, y, z, tk)]
+    if st12 is not None:
+        st = [np.ascontiguousarray(a, np.float32) for a in st12]
+        sp = (C.c_void_p * 12)(*[a.ctypes.data for a in st])
+    else:
+        sp = None
+    prd = np.ascontiguousarray(pr, np.float64)
+    ra = rocksx_array(rocks)
+    P = _visprm(nubeg, nuend, timestep, timesum, convend)
+    M = MkX(float(nukoef), int(densimod), float(vdeep), float(nudeep), float(eroslev), float(sedilev),
+            float(waterlev), int(sedtype), int(mnumy), float(teclmin), float(teclmax))
+    rc = lib().or_ronurecalc_x(C.c_long(len(x)), *[_p(a) for a in f], _p(eta), _p(t), _p(e), _p(rr), _p(rn),
+                               _p(gx), C.c_long(len(gx)), _p(gy), C.c_long(len(gy)), _p(gz), C.c_long(len(gz)),
+                               _p(prd), sp, ra, C.byref(P), C.byref(M), _p(fld))
+    if rc:
+        raise ValueError("a participating marker needs the melt branch or densimod >= 2")
+    return fld, eta, t, e, rr, rn
+
+
+def rk4_x(x, y, z, tk, eta, mtype, md, vx, vy, vz, pr, gx, gy, gz, dt, depth=(1e30, 1e30, 0.0), outgrid=0,
+          markmod=0, vyfluid=0.0, vymelt=0.0, g=(0.0, 9.81, 0.0)):
+    """movemark's motion with outgrid, markmod, fluid and immobile markers;
+    returns (x, y, z, tk, eta, type) copies"""
+    x, y, z, tk, eta = (np.array(a, np.float32, copy=True) for a in (x, y, z, tk, eta))
+    t = np.array(mtype, np.int8, copy=True)
+    mdp = np.ascontiguousarray(md, np.float32) if md is not None else None
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz))
+    v = [np.ascontiguousarray(a, np.float32) for a in (vx, vy, vz)]
+    prd = np.ascontiguousarray(pr, np.float64)
+    M = MoveX(int(outgrid), int(markmod), float(vyfluid), float(vymelt), (C.c_double * 3)(*g))
+    lib().or_rk4_x(C.c_long(len(x)), _p(x), _p(y), _p(z), _p(tk), _p(eta), _p(t),
+                   _p(mdp) if mdp is not None else None, *[_p(a) for a in v], _p(prd), _p(gx), C.c_long(len(gx)),
+                   _p(gy), C.c_long(len(gy)), _p(gz), C.c_long(len(gz)), C.c_double(dt), C.c_double(depth[0]),
+                   C.c_double(depth[1]), C.c_double(depth[2]), C.byref(M))
+    return x, y, z, tk, eta, t

@@ -1389,3 +1389,521 @@ void or_mk_props(long n, const float *x, const float *y, const float *z, const s
 		}
 	}
 }
+
+/* ---- full rheology, air/water, phase densities, fluid and immobile
+ * markers (test infrastructure) ------------------------------------------ */
+
+/* viscalc (mark3mg.c:1114-1681): brittle Mohr-Coulomb strength weakened by
+ * the accumulated strain (:1148-1199), ductile branches A (:1266-1275),
+ * B (:1277-1299), C (:1301-1326), D (:1328-1358), Karato & Wu F
+ * (:1360-1378) and the grain-size model G (:1380-1623, the disabled
+ * stress bisection left out), the yield flag on the strain (:1648-1659),
+ * the dislocation share (:1661) and the limits (:1669-1681).  The
+ * Peierls branch (:1205) is disabled in the reference and the melt
+ * pathway (:1148) reads a buffer that only the disabled melt extraction
+ * fills, so both are left out.  sbrit is read by G without being set when
+ * the brittle branch did not run; it is taken as +inf there (no yield
+ * cap).  me, mrr, mrn: the marker's marke, markrr, markrn (in/out). */
+double or_viscalc(const or_rockx_t *R, int mm2, double mtk, double mpb, double eii, double sii, double rii,
+		  double dis, float *me, float *mrr, float *mrn, const or_visprm_t *P)
+{
+	double rt = 8.314 * mtk, epsin = eii, sigin = sii;
+	double strain = fabsf(*me);
+	double lamb = 1.0, abrit = R->a0, bbrit = R->b0;
+	if (strain > R->e1) {
+		abrit = R->a1;
+		bbrit = R->b1;
+	} else if (strain > R->e0 && R->e1 > R->e0) {
+		abrit = R->a0 + (R->a1 - R->a0) * (strain - R->e0) / (R->e1 - R->e0);
+		bbrit = R->b0 + (R->b1 - R->b0) * (strain - R->e0) / (R->e1 - R->e0);
+	}
+	double mpf = mpb * 1e+5, nubrit = 0, sbrit = INFINITY;
+	if (epsin > 0 && (bbrit > 0 || abrit * lamb > 0) && mm2 > 1) {
+		sbrit = abrit + bbrit * mpf * lamb;
+		if (mpf < 0) sbrit = abrit + mpf;
+		if (sbrit > 0) nubrit = 1.0 / (0.5 * sbrit / epsin / rii);
+		else { nubrit = 1.0 / R->n0; sbrit = 0; }
+	}
+	double nunewt = 0, nupowl = 0, e1, k1, sduct;
+	if (R->nu0 != 0) {
+		if (R->dh == 0 && R->dv == 0 && (R->ss == 0 || R->mm == 1.0)) nunewt = 1.0 / R->nu0;
+		if ((R->dh != 0 || R->dv != 0) && (R->ss == 0 || R->mm == 1.0)) {
+			e1 = (R->dh + R->dv * mpb) / rt;
+			if (e1 > 150.0) e1 = 150.0;
+			if (R->dh < 0 && R->dv > 0) { e1 = R->dh * (mtk - R->dv); if (e1 < -150.0) e1 = -150.0; }
+			if (R->dh < 0 && R->dv < 0) {
+				e1 = (-R->dh) * (1.0 - (mtk - (-R->dv)) / (-R->dv)) / 8.314 / (-R->dv);
+				if (e1 > 150.0) e1 = 150.0;
+			}
+			nunewt = 1.0 / (R->nu0 * exp(e1));
+		}
+		if ((R->dh == 0 && R->dv == 0) && R->ss != 0 && R->mm != 1.0) {
+			k1 = R->nu0 / pow(R->ss, R->mm - 1.0);
+			nunewt = 1.0 / (0.5 * k1);
+			sduct = pow(R->nu0 * epsin, 1.0 / R->mm);
+			if (epsin > 0) nupowl = 1.0 / (0.5 * sduct / epsin);
+		}
+		if (R->nu0 > 0 && R->dh > 0 && (R->dh != 0 || R->dv != 0) && R->ss != 0 && R->mm != 1.0) {
+			e1 = (R->dh + R->dv * mpb) / rt;
+			if (e1 > 150.0) e1 = 150.0;
+			e1 = exp(e1);
+			k1 = R->nu0 / pow(R->ss, R->mm - 1.0);
+			nunewt = 1.0 / (0.5 * k1 * e1);
+			sduct = pow(R->nu0 * e1 * epsin, 1.0 / R->mm);
+			if (epsin > 0) nupowl = 1.0 / (0.5 * sduct / epsin);
+		}
+		if (R->nu0 < 0) {
+			double mnu2 = 1.0;
+			e1 = (R->ss + mpb * R->ll) / rt;
+			if (e1 > 150.0) e1 = 150.0;
+			e1 = exp(e1);
+			nunewt = 1.0 / (0.5 * pow(mnu2 * 1e-3 / 5e-10, 2.5) * 8e+10 / R->s1 * e1);
+			nupowl = 0;
+			if (epsin > 0) {
+				e1 = (R->dh + R->dv * mpb) / rt / R->mm;
+				if (e1 > 150.0) e1 = 150.0;
+				e1 = exp(e1);
+				nupowl = 1.0 / (0.5 * pow(epsin, 1.0 / R->mm - 1.0) * 8e+10 / pow(-R->nu0, 1.0 / R->mm) * e1);
+			}
+		}
+		if (R->dh < 0) {
+			double dissip, r, r0, dtsum, drlim, dt, dt0, n, AA, BB, m, pival;
+			epsin = eii;
+			sigin = sii;
+			dissip = dis * (double)*mrn;
+			pival = 2.0 * asin(1.0);
+			r = r0 = *mrr;
+			if (r0 <= 0) {
+				*mrr = (float)(1e-3 * 2.0 / pival);
+				r = r0 = *mrr;
+			}
+			dtsum = 0;
+			drlim = 0.1;
+			dt0 = P->timestep;
+			n = R->mm;
+			e1 = (-R->dh + mpb * R->dv) / rt;
+			if (e1 > 150.0) e1 = 150.0;
+			e1 = exp(-e1);
+			AA = R->nu0 * e1;
+			m = 3.0;
+			e1 = (R->ss + mpb * R->ll) / rt;
+			if (e1 > 150.0) e1 = 150.0;
+			e1 = exp(-e1);
+			BB = R->s1 * e1;
+			BB = BB / pow(pival / 2.0, m);
+			if (P->timesum > P->convend) {
+				int j = 0;
+				do {
+					double q = 4.0, p = 2.0, phi1 = 0.4, phi2 = 1.0 - phi1, eta = 3.0 * phi1 * phi2, gammaI = 1.0;
+					double Eggrow = 300e+3, Aggrow = 2e+4 * pow(1e-6, p), Vggrow = R->ll, GG, Gfac, GI, f0, fG,
+					       fI, Psi, DlnrDt;
+					e1 = (Eggrow + mpb * Vggrow) / rt;
+					if (e1 > 150.0) e1 = 150.0;
+					e1 = exp(-e1);
+					GG = Aggrow * e1;
+					Gfac = 100.0;
+					GI = GG / Gfac * q / p * pow(r0, q - p);
+					f0 = 1e-3;
+					e1 = 2.0 * pow(mtk / 1000, 2.9) - 2.0;
+					if (e1 > 150.0) e1 = 150.0;
+					e1 = exp(-e1);
+					fG = f0 * e1;
+					fI = fG;
+					Psi = dissip;
+					(void)phi2;
+					DlnrDt = (eta * GI / q / pow(r0, q - 1.0) - fI * r0 * r0 / gammaI / eta * Psi) / r0;
+					dt = dt0 - dtsum;
+					if (fabs(DlnrDt * dt) > drlim) dt = dt * drlim / fabs(DlnrDt * dt);
+					r = r0 * exp(DlnrDt * dt);
+					if (r > 1e-2) r = 1e-2;
+					if (r < 1e-6) r = 1e-6;
+					j = j + 1;
+					dtsum = dtsum + dt;
+					r0 = r;
+				} while (dtsum < dt0 && j < 50);
+				*mrr = (float)r;
+			}
+			nunewt = pow(r, m) / BB / 2.0;
+			nupowl = 0;
+			if (epsin > 0) {
+				double epsinpowl = epsin, siginpowl = pow(epsinpowl / AA, 1.0 / n);
+				nupowl = siginpowl / 2.0 / epsinpowl;
+				if (sigin > sbrit) nupowl = 2.0 * AA * pow(sbrit, n - 1);
+				else nupowl = 1.0 / nupowl;
+			}
+			nunewt = 1.0 / nunewt;
+		}
+	}
+	double nueff = nunewt + nupowl;
+	if (0 > nueff) nueff = 0;
+	if (nubrit > nueff) {
+		nueff = nubrit;
+		if (*me == 0) *me = 1e-20f;
+		*me = fabsf(*me);
+	} else {
+		*me = -fabsf(*me);
+	}
+	if (R->dh < 0 && P->timesum > 1e+11) *mrn = (float)(nupowl / nueff);
+	if (nueff <= 0) nueff = 1.0 / R->n1;
+	nueff = 1.0 / nueff;
+	if (nueff < R->n0) nueff = R->n0;
+	if (nueff > R->n1) nueff = R->n1;
+	if (nueff < P->nubeg) nueff = P->nubeg;
+	if (nueff > P->nuend) nueff = P->nuend;
+	return nueff;
+}
+
+static double inv2(const double *o, int b)
+{
+	return pow(0.5 * (o[b] * o[b] + o[b + 1] * o[b + 1] + o[b + 2] * o[b + 2]) + o[b + 3] * o[b + 3] +
+			   o[b + 4] * o[b + 4] + o[b + 5] * o[b + 5],
+		   0.5);
+}
+
+/* movemark's property update (mark3mg.c:61-145) with the full viscalc:
+ * for the markers that move (inside the grid, or outgrid != 1 and type <
+ * 50 or >= 100), epscalc1 at the marker; for type < 50 with T > 0 the
+ * strain-rate correction of numerical diffusion (:97-105, markers away
+ * from the three outer grid lines with a viscosity), viscalc, the strain
+ * growth when yielding and the healing.  The topography of the marker
+ * (markto, :75-88) is left to the caller. */
+void or_mk_props_x(long n, const float *x, const float *y, const float *z, const signed char *type, const float *tk,
+		   float *eta, float *e, float *rr, float *rn, const or_rockx_t *rocks, const float *const *st12,
+		   const double *pr, const double *gx, long nx, const double *gy, long ny, const double *gz, long nz,
+		   const or_visprm_t *P, double healrate, int outgrid)
+{
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	long i;
+	for (i = 0; i < n; i++) {
+		int mt = type[i];
+		if (!((x[i] >= 0 && y[i] >= 0 && z[i] >= 0 && x[i] <= xs && y[i] <= ys && z[i] <= zs) ||
+		      (outgrid != 1 && (mt < 50 || mt >= 100))))
+			continue;
+		double o[14];
+		or_epscalc(st12, pr, gx, nx, gy, ny, gz, nz, x[i], y[i], z[i], o);
+		double eii = inv2(o, 0), sii = inv2(o, 7), dis = o[13];
+		double mpb = o[6] * 1e-5, mtk = tk[i];
+		if (mt < 50 && mtk > 0) {
+			double rii = 1.0;
+			if (eta[i] > 0 && x[i] > gx[3] && x[i] < gx[nx - 4] && y[i] > gy[3] && y[i] < gy[ny - 4] && z[i] > gz[3] &&
+			    z[i] < gz[nz - 4])
+				rii *= pow(0.5 * sii / eii / eta[i], 0.87);
+			double mnu = or_viscalc(&rocks[mt], mt, mtk, mpb, eii, sii, rii, dis, &e[i], &rr[i], &rn[i], P);
+			eta[i] = (float)mnu;
+			if (e[i] > 0) e[i] += P->timestep * eii;
+			if (e[i] > 0) {
+				e[i] -= P->timestep * healrate;
+				if (e[i] < 0) e[i] = 0;
+			} else {
+				e[i] += P->timestep * healrate;
+				if (e[i] > 0) e[i] = 0;
+			}
+		}
+	}
+}
+
+/* dencalc1 (mark3mg.c:3006-3024); *bb = eps1[20] */
+static double or_dencalc(const or_rockx_t *r, int densimod, double mtk, double mpb, double *bb)
+{
+	*bb = 0;
+	if (densimod == 0) return r->ro;
+	*bb = r->bro / (1.0 - r->bro * (mtk - 298.15));
+	return r->ro * (1.0 - r->bro * (mtk - 298.15)) * (1.0 + r->aro * (mpb - 1.0) * 1e-3);
+}
+
+/* density factor of the phase transitions of ronurecalc (mark3mg.c:
+ * 530-641): basalt -> garnet granulite -> eclogite, coesite -> stishovite
+ * and perovskite for types 7-8 and 16-18; olivine -> spinel and
+ * perovskite for 9-14; both damped below teclmax and off below teclmin */
+static double or_phase_rokf(int mm2, double mtk, double mpb, double teclmin, double teclmax)
+{
+	double rokf = 1.0;
+	int ecl = (mm2 >= 7 && mm2 <= 8) || (mm2 >= 16 && mm2 <= 18), man = mm2 >= 9 && mm2 <= 14;
+	if (!ecl && !man) return 1.0;
+	if (ecl) {
+		double p_ga_in = -9222.0 + mtk * 14.0, p_pl_out, p_st_in, p_sp_out, p_pv_in;
+		if (p_ga_in < 2000.0) p_ga_in = 2000.0;
+		p_pl_out = -1460.0 + mtk * 20.0;
+		if (p_pl_out < 12000.0) p_pl_out = 12000.0;
+		if (mpb > p_ga_in) {
+			if (mpb >= p_pl_out) rokf *= 1.16;
+			else rokf *= 1.0 + 0.16 * (mpb - p_ga_in) / (p_pl_out - p_ga_in);
+		}
+		p_st_in = 59100.0 + mtk * 22.6;
+		if (mpb > p_st_in) rokf *= 1.06;
+		p_sp_out = 354000.0 - mtk * 40.0;
+		p_pv_in = 352000.0 - mtk * 40.0;
+		if (mpb > p_pv_in) {
+			if (mpb >= p_sp_out) rokf *= 1.08;
+			else rokf *= 1.0 + 0.08 * (mpb - p_pv_in) / (p_sp_out - p_pv_in);
+		}
+	} else {
+		double p_ol_out = 91000.0 + mtk * 27.0, p_sp_in = 66000.0 + mtk * 39.0, p_sp_out, p_pv_in;
+		if (p_sp_in > p_ol_out - 2000.0) p_sp_in = p_ol_out - 2000.0;
+		if (mpb > p_sp_in) {
+			if (mpb >= p_ol_out) rokf *= 1.06;
+			else rokf *= 1.0 + 0.06 * (mpb - p_sp_in) / (p_ol_out - p_sp_in);
+		}
+		p_sp_out = 304000.0 - mtk * 40.0;
+		p_pv_in = 302000.0 - mtk * 40.0;
+		if (mpb > p_pv_in) {
+			if (mpb >= p_sp_out) rokf *= 1.11;
+			else rokf *= 1.0 + 0.11 * (mpb - p_pv_in) / (p_sp_out - p_pv_in);
+		}
+	}
+	if (mtk < teclmax) {
+		if (mtk < teclmin) rokf = 1.00;
+		else rokf = 1.0 + (rokf - 1.0) * (mtk - teclmin) / (teclmax - teclmin);
+	}
+	return rokf;
+}
+
+/* ronurecalc (mark3mg.c:404-1015) with the full rheology: erosion,
+ * sedimentation and water/air retyping (:465-472), viscalc from epscalc1
+ * strain rates when the marker has no viscosity or timestep <= 0
+ * (:506-520), dencalc1 with the phase-transition density factors
+ * (:521-641), the water/air proportions of markers at the water level
+ * (:723-763; the y-cell of the marker sets the marker spacing, where the
+ * reference reads a shared index array), the viscosity caps, then the
+ * scatter of or_ronurecalc.  Melt types (> 20) and densimod >= 2 are not
+ * restated: returns -1 if a participating marker needs them.  st12 may be
+ * NULL (zero strain rates and stresses). */
+int or_ronurecalc_x(long n, const float *x, const float *y, const float *z, const float *tk, float *eta,
+		    signed char *type, float *e, float *rr, float *rn, const double *gx, long nx, const double *gy,
+		    long ny, const double *gz, long nz, const double *pr, const float *const *st12,
+		    const or_rockx_t *rocks, const or_visprm_t *P, const or_mkx_t *M, float *fld)
+{
+	long N = nx * ny * nz, i, q;
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	float *acc = calloc((size_t)(11 * N), sizeof(float));
+	double *w = calloc((size_t)(6 * N), sizeof(double));
+	int rc = 0;
+	for (i = 0; i < n && !rc; i++) {
+		long cell[3], m1, m2, m3, m10, m20, m30;
+		double mpb, mtk, mnu, mro, mbb, mcp, mkt, mht, dx, dy, dz, eii = 0, sii = 0, dis = 0;
+		int mm2 = type[i];
+		if (!(x[i] > 0 && y[i] > 0 && z[i] > 0 && x[i] < xs && y[i] < ys && z[i] < zs && mm2 < 50 && tk[i] > 0))
+			continue;
+		if (mm2 > 1 && y[i] < M->eroslev) mm2 = type[i] = 0;
+		if (mm2 < 2 && y[i] > M->sedilev) mm2 = type[i] = (signed char)M->sedtype;
+		if (mm2 < 2 && y[i] > M->waterlev) mm2 = type[i] = 1;
+		if (mm2 < 2 && y[i] < M->waterlev) mm2 = type[i] = 0;
+		if (mm2 > 20 || M->densimod >= 2) { rc = -1; break; }
+		mpb = or_prcalc(pr, gx, nx, gy, ny, gz, nz, x[i], y[i], z[i], cell) * 1e-5;
+		mtk = tk[i];
+		mnu = eta[i];
+		if (mnu <= 0 || P->timestep <= 0) {
+			double o[14] = {0};
+			if (st12) or_epscalc(st12, pr, gx, nx, gy, ny, gz, nz, x[i], y[i], z[i], o);
+			eii = inv2(o, 0); sii = inv2(o, 7); dis = o[13];
+			mnu = or_viscalc(&rocks[mm2], mm2, mtk, mpb, eii, sii, 1.0, dis, &e[i], &rr[i], &rn[i], P);
+			eta[i] = (float)mnu;
+		}
+		mro = or_dencalc(&rocks[mm2], M->densimod, mtk, mpb, &mbb);
+		mro *= or_phase_rokf(mm2, mtk, mpb, M->teclmin, M->teclmax);
+		mcp = rocks[mm2].cp;
+		mkt = (rocks[mm2].kt + rocks[mm2].kf / (mtk + 77.0)) * exp(rocks[mm2].kp * mpb);
+		mht = rocks[mm2].ht;
+		if (M->waterlev >= 0 && mm2 < 2) {
+			double dywa = (gy[cell[1] + 1] - gy[cell[1]]) / 2.0 / ((double)M->mnumy);
+			double lev = ((double)y[i] - (M->waterlev - dywa)) / (2.0 * dywa);
+			if (lev < 1.0 && lev > 0) {
+				double abb, wbb, anu, aro, wnu, wro;
+				anu = or_viscalc(&rocks[0], 0, mtk, mpb, eii, sii, 1.0, dis, &e[i], &rr[i], &rn[i], P);
+				aro = or_dencalc(&rocks[0], M->densimod, mtk, mpb, &abb);
+				wnu = or_viscalc(&rocks[1], 1, mtk, mpb, eii, sii, 1.0, dis, &e[i], &rr[i], &rn[i], P);
+				wro = or_dencalc(&rocks[1], M->densimod, mtk, mpb, &wbb);
+				mnu = wnu * lev + anu * (1.0 - lev);
+				mro = wro * lev + aro * (1.0 - lev);
+				mbb = wbb * lev + abb * (1.0 - lev);
+				mcp = rocks[1].cp * lev + rocks[0].cp * (1.0 - lev);
+				mkt = rocks[1].kt * lev + rocks[0].kt * (1.0 - lev);
+				mht = rocks[1].ht * lev + rocks[0].ht * (1.0 - lev);
+			}
+		}
+		if (y[i] > M->vdeep && mnu > M->nudeep) mnu = M->nudeep;
+		if (y[i] < 5e+4 && x[i] > xs - 1e+4 && z[i] > 1.5e+5 && z[i] < 8.7e+5 && mnu > 1e+19) mnu = 1e+19;
+		m10 = cell[0]; m20 = cell[1]; m30 = cell[2];
+		dx = gx[m10 + 1] - gx[m10]; dy = gy[m20 + 1] - gy[m20]; dz = gz[m30 + 1] - gz[m30];
+		for (m3 = m30; m3 <= m30 + 1; m3++) {
+			double ddz = 1.0 - fabs((gz[m3] - z[i]) / dz);
+			for (m1 = m10; m1 <= m10 + 1; m1++) {
+				double ddx = 1.0 - fabs((gx[m1] - x[i]) / dx);
+				for (m2 = m20; m2 <= m20 + 1; m2++) {
+					double ddy = 1.0 - fabs((gy[m2] - y[i]) / dy);
+					double swt = ddx * ddy * ddz, s1;
+					long m4 = m3 * nx * ny + m1 * ny + m2;
+					if (ddx > M->nukoef && ddy > M->nukoef && ddz > M->nukoef) {
+						acc[m4] = (float)(acc[m4] + mnu * swt);
+						w[m4] += swt;
+					}
+					if (m1 > m10 && m2 > m20 && m3 > m30) {
+						s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (0.5 - fabs(ddy - 0.5)) * (0.5 - fabs(ddz - 0.5));
+						acc[7 * N + m4] = (float)(acc[7 * N + m4] + mnu * s1);
+						w[2 * N + m4] += s1;
+					}
+					if (m3 == m30 && ddx > 0.5 && ddy > 0.5) {
+						s1 = 8.0 * (ddx - 0.5) * (ddy - 0.5) * (0.5 - fabs(ddz - 0.5));
+						acc[8 * N + m4] = (float)(acc[8 * N + m4] + mnu * s1);
+						w[3 * N + m4] += s1;
+					}
+					if (m2 == m20 && ddx > 0.5 && ddz > 0.5) {
+						s1 = 8.0 * (ddx - 0.5) * (0.5 - fabs(ddy - 0.5)) * (ddz - 0.5);
+						acc[9 * N + m4] = (float)(acc[9 * N + m4] + mnu * s1);
+						w[4 * N + m4] += s1;
+					}
+					if (m1 == m10 && ddy > 0.5 && ddz > 0.5) {
+						s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (ddy - 0.5) * (ddz - 0.5);
+						acc[10 * N + m4] = (float)(acc[10 * N + m4] + mnu * s1);
+						w[5 * N + m4] += s1;
+					}
+					if (swt > 0) {
+						acc[1 * N + m4] = (float)(acc[1 * N + m4] + mro * swt);
+						acc[2 * N + m4] = (float)(acc[2 * N + m4] + mcp * mro * swt);
+						acc[3 * N + m4] = (float)(acc[3 * N + m4] + mkt * swt);
+						acc[4 * N + m4] = (float)(acc[4 * N + m4] + mht * swt);
+						acc[5 * N + m4] = (float)(acc[5 * N + m4] + mbb * swt);
+						acc[6 * N + m4] = (float)(acc[6 * N + m4] + mtk * swt);
+						w[N + m4] += swt;
+					}
+				}
+			}
+		}
+	}
+	if (!rc)
+		for (i = 0; i < N; i++) {
+			if (w[i] > 0.01) fld[i] = (float)(acc[i] / w[i]);
+			for (q = 0; q < 4; q++)
+				if (w[(2 + q) * N + i] > 0.01) fld[(7 + q) * N + i] = (float)(acc[(7 + q) * N + i] / w[(2 + q) * N + i]);
+			if (w[N + i] > 0.01) {
+				fld[N + i] = (float)(acc[N + i] / w[N + i]);
+				fld[2 * N + i] = (float)(acc[2 * N + i] / w[N + i] / fld[N + i]);
+				for (q = 3; q <= 6; q++) fld[q * N + i] = (float)(acc[q * N + i] / w[N + i]);
+			}
+			if (fld[i] < P->nubeg) fld[i] = (float)P->nubeg;
+			if (fld[i] > P->nuend) fld[i] = (float)P->nuend;
+		}
+	free(acc);
+	free(w);
+	return rc;
+}
+
+/* movemark's motion (mark3mg.c:61, 146-284): the markers that move (inside
+ * the grid, or outgrid != 1 and type < 50 or >= 100) take the velocity of
+ * vxyzcalc1 (markmod 1) or of the four RK4 stages, plus for fluid markers
+ * (types 50-99) the water velocity: vymelt above density 1100 (markd),
+ * else vyfluid; with vyfluid > 0 and T 0 or above 298 K it is turned along
+ * the pressure gradient of the eight P nodes around the marker (:157-201,
+ * the y and z offsets scaled by the x factor and the upper nodes 6, 7
+ * shifted by one y line as in the reference; a node index past the last
+ * node reads 0), else it is -|vy_water| (:203-207).  outgrid 2 keeps only
+ * the motion normal to a face the marker is outside of (:245-251).
+ * Markers below 100 take the depth rules and the out-of-grid reset;
+ * immobile markers (>= 100) only move. */
+void or_rk4_x(long n, float *x, float *y, float *z, float *tk, float *eta, signed char *type, const float *md,
+	      const float *vx, const float *vy, const float *vz, const double *pr, const double *gx, long nx,
+	      const double *gy, long ny, const double *gz, long nz, double dt, double zdeep, double vdeep,
+	      double tdeep, const or_movex_t *M)
+{
+	long i, N = nx * ny * nz;
+	int q;
+	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
+	for (i = 0; i < n; i++) {
+		double v1[3], v2[3], v3[3], v4[3], v[3], px = x[i], py = y[i], pz = z[i], vw[3] = {0, 0, 0};
+		int mt = type[i];
+		if (!((x[i] >= 0 && y[i] >= 0 && z[i] >= 0 && x[i] <= xs && y[i] <= ys && z[i] <= zs) ||
+		      (M->outgrid != 1 && (mt < 50 || mt >= 100))))
+			continue;
+		if (mt >= 50 && mt < 100) {
+			vw[1] = M->vyfluid;
+			if (md && md[i] > 1100.0) vw[1] = M->vymelt;
+			if (M->vyfluid > 0 && (tk[i] == 0 || tk[i] > 298.0)) {
+				long c0 = serch(gx, nx, clampx(px, xs)), c1 = serch(gy, ny, clampx(py, ys)),
+				     c2 = serch(gz, nz, clampx(pz, zs));
+				long m1 = c0, m2 = c1, m3 = c2, p[8];
+				double xkf, ykf, zkf, ax, ay, az, d[3], k[3], pv[8];
+				if (x[i] > (gx[m1] + gx[m1 + 1]) / 2.0) m1 += 1;
+				if (m1 < 1) m1 = 1;
+				if (m1 > nx - 2) m1 = nx - 2;
+				if (y[i] > (gy[m2] + gy[m2 + 1]) / 2.0) m2 += 1;
+				if (m2 < 1) m2 = 1;
+				if (m2 > ny - 2) m2 = ny - 2;
+				if (z[i] > (gz[m3] + gz[m3 + 1]) / 2.0) m3 += 1;
+				if (m3 < 1) m3 = 1;
+				if (m3 > nz - 2) m3 = nz - 2;
+				xkf = 2.0 / (gx[m1 + 1] - gx[m1 - 1]);
+				ykf = 2.0 / (gy[m2 + 1] - gy[m2 - 1]);
+				zkf = 2.0 / (gz[m3 + 1] - gz[m3 - 1]);
+				ax = xkf * (x[i] - (gx[m1 - 1] + gx[m1]) / 2.0);
+				ay = xkf * (y[i] - (gy[m2 - 1] + gy[m2]) / 2.0);
+				az = xkf * (z[i] - (gz[m3 - 1] + gz[m3]) / 2.0);
+				p[0] = m3 * nx * ny + m1 * ny + m2;
+				p[1] = p[0] + 1;
+				p[2] = p[0] + ny;
+				p[3] = p[2] + 1;
+				p[4] = p[0] + nx * ny;
+				p[5] = p[4] + 1;
+				p[6] = p[5] + ny;
+				p[7] = p[6] + 1;
+				for (q = 0; q < 8; q++) pv[q] = p[q] < N ? pr[p[q]] : 0.0;
+				d[0] = xkf * ((1.0 - ay) * (1.0 - az) * (pv[2] - pv[0]) + ay * (1.0 - az) * (pv[3] - pv[1]) +
+					      (1.0 - ay) * az * (pv[6] - pv[4]) + ay * az * (pv[7] - pv[5]));
+				d[1] = ykf * ((1.0 - ax) * (1.0 - az) * (pv[1] - pv[0]) + ax * (1.0 - az) * (pv[3] - pv[2]) +
+					      (1.0 - ax) * az * (pv[5] - pv[4]) + ax * az * (pv[7] - pv[6]));
+				d[2] = zkf * ((1.0 - ax) * (1.0 - ay) * (pv[4] - pv[0]) + ax * (1.0 - ay) * (pv[6] - pv[2]) +
+					      (1.0 - ax) * ay * (pv[5] - pv[1]) + ax * ay * (pv[7] - pv[3]));
+				for (q = 0; q < 3; q++) {
+					k[q] = (1000.0 * M->g[q] - d[q]) / (2300.0 * 9.81);
+					if (k[q] > 2.0) k[q] = 2.0;
+					if (k[q] < -2.0) k[q] = -2.0;
+				}
+				vw[0] = vw[1] * k[0];
+				vw[2] = vw[1] * k[2];
+				vw[1] *= k[1];
+			} else {
+				vw[0] = 0;
+				vw[1] = -fabs(vw[1]);
+				vw[2] = 0;
+			}
+		}
+		or_vxyz(vx, vy, vz, gx, nx, gy, ny, gz, nz, px, py, pz, v1);
+		if (M->markmod == 1) {
+			for (q = 0; q < 3; q++) v[q] = v1[q] + vw[q];
+		} else {
+			for (q = 0; q < 3; q++) v1[q] += vw[q];
+			or_vxyz(vx, vy, vz, gx, nx, gy, ny, gz, nz, px + v1[0] * dt / 2.0, py + v1[1] * dt / 2.0, pz + v1[2] * dt / 2.0, v2);
+			for (q = 0; q < 3; q++) v2[q] += vw[q];
+			or_vxyz(vx, vy, vz, gx, nx, gy, ny, gz, nz, px + v2[0] * dt / 2.0, py + v2[1] * dt / 2.0, pz + v2[2] * dt / 2.0, v3);
+			for (q = 0; q < 3; q++) v3[q] += vw[q];
+			or_vxyz(vx, vy, vz, gx, nx, gy, ny, gz, nz, px + v3[0] * dt, py + v3[1] * dt, pz + v3[2] * dt, v4);
+			for (q = 0; q < 3; q++) v4[q] += vw[q];
+			for (q = 0; q < 3; q++) v[q] = (v1[q] + 2.0 * v2[q] + 2.0 * v3[q] + v4[q]) / 6.0;
+		}
+		if (M->outgrid == 2) {
+			if (x[i] < 0 || (double)x[i] > xs) v[1] = v[2] = 0;
+			if (y[i] < 0 || (double)y[i] > ys) v[0] = v[2] = 0;
+			if (z[i] < 0 || (double)z[i] > zs) v[0] = v[1] = 0;
+		}
+		if (mt < 100) {
+			double yold = y[i];
+			if (y[i] > zdeep && v[1] < 0 && tk[i] < tdeep && type[i] == 10) tk[i] = (float)tdeep;
+			if (y[i] > zdeep && v[1] < 0 && type[i] != 10) type[i] = 10;
+			if (y[i] > zdeep && type[i] != 10 && type[i] != 9) type[i] = 9;
+			x[i] += dt * v[0] - 0.0;
+			y[i] += dt * v[1];
+			z[i] += dt * v[2];
+			if ((yold > vdeep && y[i] < vdeep) || (yold > zdeep && y[i] < zdeep)) {
+				if (tk[i] < tdeep) tk[i] = (float)tdeep;
+				type[i] = 10;
+			}
+			if (x[i] < 0 || y[i] < 0 || z[i] < 0 || x[i] > xs || y[i] > ys || z[i] > zs) {
+				tk[i] = 0;
+				eta[i] = 0;
+			}
+		} else {
+			x[i] += dt * v[0] - 0.0;
+			y[i] += dt * v[1];
+			z[i] += dt * v[2];
+		}
+	}
+}

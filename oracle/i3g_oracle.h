@@ -136,6 +136,54 @@ void or_mk_props(long n, const float *x, const float *y, const float *z, const s
 		 const double *gx, long nx, const double *gy, long ny, const double *gz, long nz, double nubeg,
 		 double nuend, double dt, double healrate);
 
+/* ---- full rheology (test infrastructure) ------------------------------- */
+
+/* every per-rock value viscalc and ronurecalc read (in3mg.c:204-231):
+ * the thirteen of or_rock_t, then markss, markmm, markll, marks1, marka0,
+ * marka1, markb0, markb1, marke0, marke1 */
+typedef struct {
+	double nu0, dh, dv, n0, n1, ro, bro, aro, cp, kt, kf, kp, ht;
+	double ss, mm, ll, s1, a0, a1, b0, b1, e0, e1;
+} or_rockx_t;
+
+/* viscalc's globals: nubeg, nuend, timestep, timesum, convend (s) */
+typedef struct {
+	double nubeg, nuend, timestep, timesum, convend;
+} or_visprm_t;
+
+/* ronurecalc's globals */
+typedef struct {
+	double nukoef;
+	int densimod;
+	double vdeep, nudeep;
+	double eroslev, sedilev, waterlev;
+	int sedtype; /* the sediment type of this call (3 or 4, from sedimnum/sedimcyc) */
+	long mnumy;  /* markers per cell along y */
+	double teclmin, teclmax;
+} or_mkx_t;
+
+/* movemark's globals of the motion */
+typedef struct {
+	int outgrid, markmod;
+	double vyfluid, vymelt;
+	double g[3]; /* GXKOEF, GYKOEF, GZKOEF */
+} or_movex_t;
+
+double or_viscalc(const or_rockx_t *R, int mm2, double mtk, double mpb, double eii, double sii, double rii,
+		  double dis, float *me, float *mrr, float *mrn, const or_visprm_t *P);
+void or_mk_props_x(long n, const float *x, const float *y, const float *z, const signed char *type, const float *tk,
+		   float *eta, float *e, float *rr, float *rn, const or_rockx_t *rocks, const float *const *st12,
+		   const double *pr, const double *gx, long nx, const double *gy, long ny, const double *gz, long nz,
+		   const or_visprm_t *P, double healrate, int outgrid);
+int or_ronurecalc_x(long n, const float *x, const float *y, const float *z, const float *tk, float *eta,
+		    signed char *type, float *e, float *rr, float *rn, const double *gx, long nx, const double *gy,
+		    long ny, const double *gz, long nz, const double *pr, const float *const *st12,
+		    const or_rockx_t *rocks, const or_visprm_t *P, const or_mkx_t *M, float *fld);
+void or_rk4_x(long n, float *x, float *y, float *z, float *tk, float *eta, signed char *type, const float *md,
+	      const float *vx, const float *vy, const float *vz, const double *pr, const double *gx, long nx,
+	      const double *gy, long ny, const double *gz, long nz, double dt, double zdeep, double vdeep,
+	      double tdeep, const or_movex_t *M);
+
 /* tkrecalc (heat3mg.c:846-899): temperature boundary rows on level 0 from
  * a copy of tk (reference bondm layout, T records at 8*nodenum). */
 void or_tkrecalc(long nodenum, const int *bondm, const float *bondv, const int *bondn, float *tk);

@@ -21,6 +21,13 @@ inputs and outputs only.
                     format) and what the reference's loadconf reads from it;
                     g13_state.prn written by the reference's saver from the
                     state in g13_io.npz (load3mg.c:2-180, 704-1096)
+  g14_movemark_rheology.npz  movemark with the deck's rock table (init.t3c):
+                    full viscalc (brittle, power law, Karato & Wu, grain
+                    size), outgrid 1/2, markmod 0/1, fluid and immobile
+                    markers (mark3mg.c:2-397, 1114-1681)
+  g15_ronurecalc_rheology.npz  ronurecalc with the deck's rocks: retyping,
+                    water/air mixing, viscalc from strain rates, phase
+                    densities (mark3mg.c:404-1015)
   g12_smoother.npz  on the g1 inputs: the reference's own coefficient of each
                     assembled row's unknown (the first row entry, gausmat4's
                     divisor ival1, move3mg.c:3096-3124) and the state after
@@ -848,10 +855,232 @@ def g11(R):
     print("g11_movemark_depth.npz written")
 
 
+# rock table fields in the order in3mg.c:204-231 reads them from a deck
+DECK_ROCK_ORDER = ("n0", "n1", "s0", "s1", "nu0", "dh", "dv", "ss", "mm", "ll", "a0", "a1", "b0", "b1", "e0",
+                   "e1", "ro", "bro", "aro", "cp", "kt", "kf", "kp", "ht")
+REF_ROCK_NAMES = dict(n0="markn0", n1="markn1", s0="marks0", s1="marks1", nu0="marknu", dh="markdh", dv="markdv",
+                      ss="markss", mm="markmm", ll="markll", a0="marka0", a1="marka1", b0="markb0", b1="markb1",
+                      e0="marke0", e1="marke1", ro="markro", bro="markbb", aro="markaa", cp="markcp", kt="markkt",
+                      kf="markkf", kp="markkp", ht="markht")
+
+
+def deck_rocks(path="/root/reference/init.t3c"):
+    """The /ROCKS_DESCRIPTIONS/ table of the reference's deck as a list of
+    dicts indexed by rock number: the numeric rows in the column order of
+    in3mg.c:204-231 (whitespace tokens, '/' tokens and the column header
+    skipped, up to '~')."""
+    toks = [t for t in open(path).read().split("/ROCKS_DESCRIPTIONS/", 1)[1].split() if not t.startswith("/")]
+    rocks = {}
+    i = 0
+    while not toks[i].isdigit():  # the column header line
+        i += 1
+    while toks[i] != "~":
+        num = int(toks[i])
+        vals = toks[i + 1:i + 1 + len(DECK_ROCK_ORDER)]
+        rocks[num] = {f: float(v[1:]) if v.startswith("k") else float(v) for f, v in zip(DECK_ROCK_ORDER, vals)}
+        i += 1 + len(DECK_ROCK_ORDER)
+    out = [dict() for _ in range(max(rocks) + 1)]
+    for k, r in rocks.items():
+        out[k] = r
+    return out
+
+
+def _set_ref_rocks(R, rocks):
+    import ctypes as C
+    for f, gname in REF_ROCK_NAMES.items():
+        a = R.arr(gname, C.c_double, R.MAXTMR)
+        a[:] = 0
+        for t, r in enumerate(rocks):
+            a[t] = r.get(f, 0.0)
+
+
+def _rock_matrix(rocks):
+    import oracle
+    return np.array([[r.get(f, 0.0) for f in oracle.ROCKX_FIELDS] for r in rocks])
+
+
+def g14(R):
+    """movemark (mark3mg.c:2-397) with the reference deck's rock table
+    (init.t3c: power law, brittle strength, Karato & Wu, the grain-size
+    model of the mantle rocks): marker viscosity, strain, grain roughness
+    and dislocation share, positions, for (a) outgrid 2, RK4, timesum past
+    convend and (b) outgrid 1, markmod 1, timesum 0; fluid markers (types
+    50-99, with markd) and immobile markers (>= 100) included.  Run with
+    stdin closed (see g10)."""
+    import ctypes as C
+    rng = np.random.default_rng(1414)
+    rocks = deck_rocks()
+    nx, ny, nz = 11, 9, 10
+    gx = np.cumsum(np.r_[0, rng.uniform(0.7, 1.3, nx - 1)]) * 5e3
+    gy = np.cumsum(np.r_[0, rng.uniform(0.7, 1.3, ny - 1)]) * 5e3
+    gz = np.cumsum(np.r_[0, rng.uniform(0.7, 1.3, nz - 1)]) * 5e3
+    N = nx * ny * nz
+    h = I.hierarchy(nx, ny, nz, 0)
+    rules, nr = I.freeslip_rules(273.0, 1600.0)
+    bondm, bondv, bondn = I.expand_bc(rules, nr, h)
+    n = 1500
+    x = rng.uniform(-3e3, gx[-1] + 3e3, n).astype(np.float32)
+    y = rng.uniform(-3e3, gy[-1] + 3e3, n).astype(np.float32)
+    z = rng.uniform(-3e3, gz[-1] + 3e3, n).astype(np.float32)
+    solid = np.array([0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 16, 17, 18])
+    mtype = solid[rng.integers(0, len(solid), n)].astype(np.int8)
+    mtype[rng.uniform(size=n) < 0.08] = rng.choice([52, 61, 75, 99], 1)[0]
+    fl = rng.uniform(size=n) < 0.08
+    mtype[fl] = rng.choice(np.array([52, 61, 75, 99], np.int8), fl.sum())
+    im = rng.uniform(size=n) < 0.05
+    mtype[im] = rng.choice(np.array([102, 109, 114, 127], np.int8), im.sum())
+    tk = rng.uniform(400, 1900, n).astype(np.float32)
+    tk[rng.uniform(size=n) < 0.04] = 0
+    tk[fl & (rng.uniform(size=n) < 0.3)] = 250.0  # cold fluid: -|vy_water|
+    eta = np.where(rng.uniform(size=n) < 0.3, 0.0, 10 ** rng.uniform(19, 24, n)).astype(np.float32)
+    marke = np.where(rng.uniform(size=n) < 0.3, 0.0, rng.uniform(-0.8, 1.5, n)).astype(np.float32)
+    markrr = np.where(rng.uniform(size=n) < 0.3, 0.0, 10 ** rng.uniform(-5.5, -2.5, n)).astype(np.float32)
+    markrr[rng.uniform(size=n) < 0.05] = -1e-4
+    markrn = rng.uniform(0, 1, n).astype(np.float32)
+    markd = np.where(rng.uniform(size=n) < 0.5, 1000.0, 1250.0).astype(np.float32)
+    names = ("exx", "eyy", "ezz", "exy", "exz", "eyz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    st = {k: (rng.standard_normal(N) * (10 ** rng.uniform(-15.5, -13, N) if k[0] == "e" else 10 ** rng.uniform(6, 8, N))
+              ).astype(np.float32) for k in names}
+    pr = rng.uniform(1e7, 3e9, N)
+    vel = {k: (rng.standard_normal(N) * 3e-9).astype(np.float32) for k in ("vx", "vy", "vz")}
+    ro = rng.uniform(1000, 3300, N).astype(np.float32)
+    cases = [dict(outgrid=2, markmod=0, timesum=5e13), dict(outgrid=1, markmod=1, timesum=0.0)]
+    prm = dict(timestep=3e11, healrate=1e-14, nubeg=1e18, nuend=1e25, convend=1e13, vyfluid=1e-9, vymelt=3e-9,
+               g=(0.0, 9.81, 0.0))
+    outs = {}
+    for ci, cs in enumerate(cases):
+        R.setup_grid(gx, gy, gz, 0)
+        R.set_bc(bondm, bondv, bondn)
+        for k, a in st.items():
+            R.f32(k)[:N] = a
+        for k, a in vel.items():
+            R.f32(k)[:N] = a
+        R.f32("ro")[:N] = ro
+        R.f64("pr")[:N] = pr
+        R.arr("bufv", C.c_double, R.MAXPPP)[:] = 0
+        R.set("marknum", n, C.c_long)
+        R.arr("markx", C.c_float, R.MAXMRK)[:n] = x
+        R.arr("marky", C.c_float, R.MAXMRK)[:n] = y
+        R.arr("markz", C.c_float, R.MAXMRK)[:n] = z
+        R.arr("markk", C.c_float, R.MAXMRK)[:n] = tk
+        R.arr("markv", C.c_float, R.MAXMRK)[:n] = eta
+        R.arr("markt", C.c_byte, R.MAXMRK)[:n] = mtype
+        R.arr("marke", C.c_float, R.MAXMRK)[:n] = marke
+        R.arr("markrr", C.c_float, R.MAXMRK)[:n] = markrr
+        R.arr("markrn", C.c_float, R.MAXMRK)[:n] = markrn
+        R.arr("markd", C.c_float, R.MAXMRK)[:n] = markd
+        for gname in ("markw", "markex", "markft", "markto"):
+            R.arr(gname, C.c_float, R.MAXMRK)[:n] = 0
+        _set_ref_rocks(R, rocks)
+        for k, v in dict(timestep=prm["timestep"], healrate=prm["healrate"], nubeg=prm["nubeg"], nuend=prm["nuend"],
+                         zdeep=1e30, vdeep=1e30, tdeep=0.0, stp100=1e30, vyfluid=prm["vyfluid"],
+                         vymelt=prm["vymelt"], timesum=cs["timesum"], convend=prm["convend"],
+                         GXKOEF=prm["g"][0], GYKOEF=prm["g"][1], GZKOEF=prm["g"][2]).items():
+            R.set(k, v)
+        R.set("outgrid", cs["outgrid"], C.c_int)
+        R.set("markmod", cs["markmod"], C.c_int)
+        R.lib.refh_movemark()
+        for k, gname, ct in (("x", "markx", C.c_float), ("y", "marky", C.c_float), ("z", "markz", C.c_float),
+                             ("tk", "markk", C.c_float), ("eta", "markv", C.c_float), ("e", "marke", C.c_float),
+                             ("rr", "markrr", C.c_float), ("rn", "markrn", C.c_float), ("type", "markt", C.c_byte)):
+            outs[f"{k}_out{ci}"] = R.arr(gname, ct, R.MAXMRK)[:n].copy()
+        assert R.get("marknum", C.c_long) == n
+    np.savez_compressed(os.path.join(HERE, "g14_movemark_rheology.npz"), gx=gx, gy=gy, gz=gz, x=x, y=y, z=z,
+                        mtype=mtype, tk=tk, eta=eta, marke=marke, markrr=markrr, markrn=markrn, markd=markd, pr=pr,
+                        ro=ro, **st, **vel, rocks=_rock_matrix(rocks),
+                        outgrid=np.array([c["outgrid"] for c in cases]),
+                        markmod=np.array([c["markmod"] for c in cases]),
+                        timesum=np.array([c["timesum"] for c in cases]), **{k: v for k, v in prm.items() if k != "g"},
+                        g=np.array(prm["g"]), **outs)
+    print("g14_movemark_rheology.npz written")
+
+
+def g15(R):
+    """ronurecalc (mark3mg.c:404-1015) with the reference deck's rocks:
+    erosion/sedimentation and water/air retyping, markers across the water
+    level (uniform y spacing), viscalc from epscalc1 strain rates
+    (timestep 0), eclogite and mantle phase-transition densities with the
+    teclmin/teclmax damping; densimod 1, no melt types."""
+    import ctypes as C
+    rng = np.random.default_rng(1515)
+    rocks = deck_rocks()
+    nx, ny, nz = 11, 9, 10
+    gx = np.cumsum(np.r_[0, rng.uniform(0.7, 1.3, nx - 1)]) * 5e3
+    gy = np.arange(ny) * 5e3
+    gz = np.cumsum(np.r_[0, rng.uniform(0.7, 1.3, nz - 1)]) * 5e3
+    N = nx * ny * nz
+    h = I.hierarchy(nx, ny, nz, 0)
+    rules, nr = I.freeslip_rules(273.0, 1600.0)
+    bondm, bondv, bondn = I.expand_bc(rules, nr, h)
+    R.setup_grid(gx, gy, gz, 0)
+    R.set_bc(bondm, bondv, bondn)
+    n = 2400
+    x = rng.uniform(1.0, gx[-1] - 1.0, n).astype(np.float32)
+    y = rng.uniform(1.0, gy[-1] - 1.0, n).astype(np.float32)
+    z = rng.uniform(1.0, gz[-1] - 1.0, n).astype(np.float32)
+    waterlev, eroslev, sedilev, mnumy = 11.3e3, 4.0e3, 9.0e3, 2
+    near = rng.uniform(size=n) < 0.15
+    y[near] = (waterlev + rng.uniform(-1.2, 1.2, near.sum()) * 1.25e3).astype(np.float32)
+    solid = np.array([0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 16, 17, 18])
+    mtype = solid[rng.integers(0, len(solid), n)].astype(np.int8)
+    mtype[near] = rng.choice(np.array([0, 1], np.int8), near.sum())
+    tk = rng.uniform(500, 2000, n).astype(np.float32)
+    tk[::113] = 0
+    eta = np.where(rng.uniform(size=n) < 0.5, 0.0, 10 ** rng.uniform(19, 24, n)).astype(np.float32)
+    marke = np.where(rng.uniform(size=n) < 0.3, 0.0, rng.uniform(-0.8, 1.5, n)).astype(np.float32)
+    markrr = np.where(rng.uniform(size=n) < 0.3, 0.0, 10 ** rng.uniform(-5.5, -2.5, n)).astype(np.float32)
+    markrn = rng.uniform(0, 1, n).astype(np.float32)
+    names = ("exx", "eyy", "ezz", "exy", "exz", "eyz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    st = {k: (rng.standard_normal(N) * (10 ** rng.uniform(-15.5, -13, N) if k[0] == "e" else 10 ** rng.uniform(6, 8, N))
+              ).astype(np.float32) for k in names}
+    pr = rng.uniform(1e8, 4e10, N)  # up to 400 kbar: every transition is crossed
+    for k, a in st.items():
+        R.f32(k)[:N] = a
+    R.f64("pr")[:N] = pr
+    R.set("marknum", n, C.c_long)
+    for gname, a in (("markx", x), ("marky", y), ("markz", z), ("markk", tk), ("markv", eta), ("marke", marke),
+                     ("markrr", markrr), ("markrn", markrn)):
+        R.arr(gname, C.c_float, R.MAXMRK)[:n] = a
+    R.arr("markt", C.c_byte, R.MAXMRK)[:n] = mtype
+    for gname in ("markd", "markw", "markex", "markft", "markto"):
+        R.arr(gname, C.c_float, R.MAXMRK)[:n] = 0
+    _set_ref_rocks(R, rocks)
+    R.set("densimod", 1, C.c_int)
+    R.set("viscmod", 0, C.c_int)
+    sedimnum, sedimcyc = 4, 3
+    R.set("sedimnum", sedimnum, C.c_int)
+    R.set("sedimcyc", sedimcyc, C.c_int)
+    R.set("mnumy", mnumy, C.c_long)
+    prm = dict(eroslev=eroslev, sedilev=sedilev, waterlev=waterlev, vdeep=gy[-1] * 0.8, nudeep=1e21, nukoef=0.0,
+               nubeg=1e18, nuend=1e25, timestep=0.0, timesum=5e13, convend=1e13, teclmin=700.0, teclmax=1100.0)
+    for k, v in prm.items():
+        R.set(k, v)
+    init = np.zeros((11, N), np.float32)
+    init[0] = 5e20; init[1] = 3111; init[2] = 999; init[3] = 2.2; init[4] = 1e-9; init[5] = 1e-5
+    init[6] = 1000; init[7:] = 7e20
+    gnames = ["nu", "ro", "cp", "kt", "ht", "et", "tk", "nxx", "nxy", "nxz", "nyz"]
+    for q, gname in enumerate(gnames):
+        R.f32(gname)[:N] = init[q]
+    R.lib.refh_ronurecalc()
+    m1 = (sedimnum + 1) // sedimcyc
+    sedtype = 3 if (m1 // 2) * 2 == m1 else 4
+    out = np.stack([R.f32(gname)[:N].copy() for gname in gnames])
+    res = {f"{k}_out": R.arr(g, C.c_float, R.MAXMRK)[:n].copy()
+           for k, g in (("eta", "markv"), ("e", "marke"), ("rr", "markrr"), ("rn", "markrn"))}
+    res["type_out"] = R.arr("markt", C.c_byte, R.MAXMRK)[:n].copy()
+    np.savez_compressed(os.path.join(HERE, "g15_ronurecalc_rheology.npz"), gx=gx, gy=gy, gz=gz, bondm=bondm,
+                        bondv=bondv, bondn=bondn, x=x, y=y, z=z,
+                        mtype=mtype, tk=tk, eta=eta, marke=marke, markrr=markrr, markrn=markrn, pr=pr, **st,
+                        rocks=_rock_matrix(rocks), init=init, out=out, sedtype=sedtype, mnumy=mnumy,
+                        densimod=1, **prm, **res)
+    print("g15_ronurecalc_rheology.npz written", "sedtype", sedtype)
+
+
 if __name__ == "__main__":
     R = Ref()
     R.lib.refh_set_threads(1)
-    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9", "g10", "g11", "g12", "g3c2", "g13"]
+    which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9", "g10", "g11", "g12", "g3c2", "g13", "g14",
+                             "g15"]
     if "g1" in which:
         g1(R)
     if "g3" in which:
@@ -878,3 +1107,7 @@ if __name__ == "__main__":
     if "g3c2" in which:
         g3(R, 25, which=2, seed=2)
         g3(R, 33, which=2, seed=2)
+    if "g14" in which:
+        g14(R)
+    if "g15" in which:
+        g15(R)

@@ -1,0 +1,138 @@
+"""GPU parity of the whole marker rheology and movemark's motion, through
+the C ABI, against the reference's own movemark and ronurecalc run with the
+rock table of its deck (goldens g14, g15: power law, brittle strength,
+Karato & Wu, the grain-size model; outgrid 1/2, markmod 0/1, fluid and
+immobile markers; retyping, water/air mixing, phase densities).  On these
+fixtures every output is equal to the reference's to the bit; the bounds
+allow for the device's own exp/pow/asin on other inputs: marker viscosities,
+grain roughness and dislocation shares within 2e-6 relative (the grain
+growth repeats up to 50 exponential steps), strain to 1e-6 absolute with
+the same sign, positions within 2 ulp, types exact; the node fields of
+the transfer within 2e-6 relative."""
+import numpy as np
+import pytest
+
+import i3elvis_amd as I
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+ST = ("exx", "eyy", "ezz", "exy", "exz", "eyz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+
+
+def _rel(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+
+
+def _ulp(a, b):
+    return np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_movemark_rheology_matches_reference(gpu, case):
+    g = golden("g14_movemark_rheology.npz")
+    n = len(g["x"])
+    print(f"case: movemark rheology g14 case {case} (outgrid {int(g['outgrid'][case])}, "
+          f"markmod {int(g['markmod'][case])}), {n} markers", flush=True)
+    M = I.Markers(g["gx"], g["gy"], g["gz"], n + 16)
+    M.upload(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"])
+    M.set_strain(g["marke"])
+    M.set_grain(g["markrr"], g["markrn"])
+    rocks = [list(r) for r in g["rocks"]]
+    dt = float(g["timestep"])
+    M.update_props(rocks, float(g["nubeg"]), float(g["nuend"]), dt, float(g["healrate"]), [g[k] for k in ST],
+                   g["pr"], rheology=1, timesum=float(g["timesum"][case]), convend=float(g["convend"]),
+                   outgrid=int(g["outgrid"][case]))
+    M.advect_ex(dt, g["vx"], g["vy"], g["vz"], outgrid=int(g["outgrid"][case]), markmod=int(g["markmod"][case]),
+                vyfluid=float(g["vyfluid"]), vymelt=float(g["vymelt"]), g=tuple(g["g"]), pr=g["pr"],
+                markd=g["markd"])
+    d = M.download()
+    e = M.get_strain()
+    rr, rn = M.get_grain()
+    assert np.array_equal(d["id"], np.arange(n))
+    assert np.array_equal(d["type"], g[f"type_out{case}"])
+    for name, a, tol in (("eta", d["eta"], 2e-6), ("rr", rr, 2e-6), ("rn", rn, 2e-6)):
+        rel = _rel(a, g[f"{name}_out{case}"])
+        print(f"  {name}: max rel {rel.max():.2e}, exact {int((a == g[f'{name}_out{case}']).sum())} of {n}", flush=True)
+        assert rel.max() <= tol, (name, int(rel.argmax()))
+    ref_e = g[f"e_out{case}"]
+    assert np.array_equal(np.sign(e), np.sign(ref_e))
+    assert np.abs(e - ref_e).max() <= 1e-6
+    for k in ("x", "y", "z"):
+        u = _ulp(d[k], g[f"{k}_out{case}"])
+        print(f"  {k}: max ulp {u.max()}", flush=True)
+        assert u.max() <= 2, (k, int(u.argmax()))
+    assert np.array_equal(d["tk"] == 0, g[f"tk_out{case}"] == 0)
+    M.close()
+
+
+def test_ronurecalc_rheology_matches_reference(gpu):
+    g = golden("g15_ronurecalc_rheology.npz")
+    n = len(g["x"])
+    print(f"case: ronurecalc rheology g15, {n} markers", flush=True)
+    M = I.Markers(g["gx"], g["gy"], g["gz"], n + 16)
+    M.upload(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"])
+    M.set_strain(g["marke"])
+    M.set_grain(g["markrr"], g["markrn"])
+    M.set_stress([g[k] for k in ST])
+    M.sort()
+    M.set_tbc(g["bondm"], g["bondv"], g["bondn"])
+    rocks = [list(r) for r in g["rocks"]]
+    fld = M.to_nodes(rocks, g["pr"], g["init"], float(g["nubeg"]), float(g["nuend"]), nukoef=float(g["nukoef"]),
+                     densimod=int(g["densimod"]), timestep=float(g["timestep"]), vdeep=float(g["vdeep"]),
+                     nudeep=float(g["nudeep"]), rheology=1, eroslev=float(g["eroslev"]), sedilev=float(g["sedilev"]),
+                     waterlev=float(g["waterlev"]), sedtype=int(g["sedtype"]), mnumy=int(g["mnumy"]),
+                     teclmin=float(g["teclmin"]), teclmax=float(g["teclmax"]), timesum=float(g["timesum"]),
+                     convend=float(g["convend"]))
+    d = M.download()
+    e = M.get_strain()
+    rr, rn = M.get_grain()
+    order = np.argsort(d["id"])
+    assert np.array_equal(d["type"][order], g["type_out"])
+    for name, a in (("eta", d["eta"]), ("rr", rr), ("rn", rn)):
+        rel = _rel(a[order], g[f"{name}_out"])
+        print(f"  {name}: max rel {rel.max():.2e}", flush=True)
+        assert rel.max() <= 2e-6, name
+    assert np.array_equal(np.sign(e[order]), np.sign(g["e_out"]))
+    for q in range(11):
+        rel = _rel(fld[q], g["out"][q])
+        print(f"  field {q}: max rel {rel.max():.2e}", flush=True)
+        assert rel.max() <= 2e-6, q
+    M.close()
+
+
+def test_transfer_refuses_melt_types(gpu):
+    """rheology 1 refuses the melt branch (types 21-49) and densimod 2"""
+    g = golden("g15_ronurecalc_rheology.npz")
+    n = len(g["x"])
+    print("case: melt types refused", flush=True)
+    M = I.Markers(g["gx"], g["gy"], g["gz"], n + 16)
+    mt = np.full(n, 25, np.int8)
+    M.upload(g["x"], g["y"], g["z"], g["tk"], g["eta"], mt)
+    M.sort()
+    rocks = [list(r) for r in g["rocks"]]
+    with pytest.raises(I.I3GError):
+        M.to_nodes(rocks, g["pr"], g["init"], 1e18, 1e25, rheology=1, mnumy=2)
+    M.close()
+
+
+def test_advect_rk4_equals_advect_ex_defaults(gpu):
+    """i3g_markers_advect_rk4 is advect_ex with outgrid 0, markmod 0 and no
+    fluid velocity: the same positions to the bit"""
+    g = golden("g14_movemark_rheology.npz")
+    n = len(g["x"])
+    print("case: advect_rk4 vs advect_ex defaults", flush=True)
+    out = []
+    for ex in (False, True):
+        M = I.Markers(g["gx"], g["gy"], g["gz"], n)
+        M.upload(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"])
+        if ex:
+            M.advect_ex(3e11, g["vx"], g["vy"], g["vz"])
+        else:
+            M.advect_rk4(3e11, g["vx"], g["vy"], g["vz"])
+        out.append(M.download())
+        M.close()
+    for k in ("x", "y", "z", "tk", "eta", "type"):
+        assert np.array_equal(out[0][k], out[1][k]), k
