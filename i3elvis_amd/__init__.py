@@ -879,3 +879,87 @@ def write_prn(path, d):
     for k in PRN_ARRAYS:
         setattr(h, k, keep[k].ctypes.data if keep[k].size else None)
     check(lib().i3g_prn_write(str(path).encode(), C.byref(h)), "prn_write")
+
+
+# ---- initial condition generator and ParaView output -----------------------
+
+
+class RockBox(C.Structure):
+    """i3g_rock_box"""
+    _fields_ = [("shape", C.c_int), ("type", C.c_int), ("ex", C.c_double), ("e", C.c_double),
+                ("x", C.c_double * 8), ("y", C.c_double * 8), ("z", C.c_double * 8)]
+
+
+class TempBox(C.Structure):
+    """i3g_temp_box"""
+    _fields_ = ([("type", C.c_int)] +
+                [(f, C.c_double) for f in ("x0", "x2", "y0", "y1", "y2", "y3", "z0", "x5", "x7", "y4", "y5", "y6",
+                                           "y7", "z7")] + [("t", C.c_double * 8)])
+
+
+def deck_boxes(path, xsize, ysize, zsize, maxbox=4096):
+    """rock and temperature boxes of a deck in the reference's format
+    (i3g_deck_boxes): (RockBox array, TempBox array)"""
+    rb = (RockBox * maxbox)()
+    tb = (TempBox * maxbox)()
+    nr, nt = C.c_int(), C.c_int()
+    check(lib().i3g_deck_boxes(str(path).encode(), C.c_double(xsize), C.c_double(ysize), C.c_double(zsize), rb,
+                               C.c_int(maxbox), C.byref(nr), tb, C.c_int(maxbox), C.byref(nt)), "deck_boxes")
+    return (RockBox * nr.value)(*rb[:nr.value]), (TempBox * nt.value)(*tb[:nt.value])
+
+
+def _paint(self, xsize, ysize, zsize, boxes, ex=None, od=None, ow=None):
+    """i3g_markers_paint; ex/od/ow (current marker order) updated copies"""
+    n = self.n
+    outs = [None if a is None else np.array(a, np.float32, copy=True) for a in (ex, od, ow)]
+    for a in outs:
+        if a is not None and a.size != n:
+            raise ValueError("marker arrays of the wrong length")
+    check(lib().i3g_markers_paint(self.h, C.c_double(xsize), C.c_double(ysize), C.c_double(zsize), boxes,
+                                  C.c_int(len(boxes)), *[_p(a) for a in outs]), "markers_paint")
+    return outs
+
+
+Markers.paint = _paint
+
+
+def temperature_boxes(gx, gy, gz, xsize, ysize, zsize, boxes, tk):
+    """i3g_temperature_boxes on a copy of the node temperatures"""
+    g = [np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz)]
+    out = np.array(tk, np.float32, copy=True)
+    check(lib().i3g_temperature_boxes(C.c_long(len(g[0])), C.c_long(len(g[1])), C.c_long(len(g[2])),
+                                      *[_p(a) for a in g], C.c_double(xsize), C.c_double(ysize), C.c_double(zsize),
+                                      boxes, C.c_int(len(boxes)), _p(out)), "temperature_boxes")
+    return out
+
+
+def write_vtr_nodal(path, gx, gy, gz, fields):
+    """fields: {name: node plane (m3, m1, m2 order)}; float64 planes are
+    written as Float64, others as Float32"""
+    g = [np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz)]
+    names = list(fields)
+    f64 = [np.asarray(fields[k]).dtype == np.float64 for k in names]
+    arrs = [np.ascontiguousarray(fields[k], np.float64 if d else np.float32) for k, d in zip(names, f64)]
+    cn = (C.c_char_p * len(names))(*[k.encode() for k in names])
+    cf = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    cd = (C.c_int * len(names))(*[int(d) for d in f64])
+    check(lib().i3g_write_vtr_nodal(str(path).encode(), C.c_long(len(g[0])), C.c_long(len(g[1])),
+                                    C.c_long(len(g[2])), *[_p(a) for a in g], C.c_int(len(names)), cn, cf, cd),
+          "write_vtr_nodal")
+
+
+def write_vtr_composition(path, xsize, ysize, zsize, mx, my, mz, x, y, z, mtype):
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (x, y, z)]
+    t = np.ascontiguousarray(mtype, np.int8)
+    check(lib().i3g_write_vtr_composition(str(path).encode(), C.c_double(xsize), C.c_double(ysize),
+                                          C.c_double(zsize), C.c_long(mx), C.c_long(my), C.c_long(mz),
+                                          C.c_long(len(t)), *[_p(a) for a in arrs], _p(t)), "write_vtr_composition")
+
+
+def pvd_open(path):
+    check(lib().i3g_pvd_open(str(path).encode()), "pvd_open")
+
+
+def pvd_append(path, time, datafile, directory=None):
+    check(lib().i3g_pvd_append(str(path).encode(), C.c_double(time), datafile.encode(),
+                               None if directory is None else directory.encode()), "pvd_append")

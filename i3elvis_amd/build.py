@@ -14,8 +14,8 @@ LIB = os.path.join(HERE, "libi3g.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HIP_SOURCES = ["i3g_stokes.hip", "i3g_markers.hip", "i3g_heat.hip", "i3g_net.hip"]
-C_SOURCES = ["i3g_host.c", "i3g_model.c", "i3g_io.c"]
+HIP_SOURCES = ["i3g_stokes.hip", "i3g_markers.hip", "i3g_heat.hip", "i3g_net.hip", "i3g_gen.hip"]
+C_SOURCES = ["i3g_host.c", "i3g_model.c", "i3g_io.c", "i3g_vtk.c"]
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]

@@ -274,3 +274,80 @@ int i3g_prn_write(const char *path, const i3g_prn *p)
 	if (fclose(f) != 0) ok = 0;
 	return ok ? I3G_OK : I3G_EINVAL;
 }
+
+/* ---- initial-condition boxes of a deck ------------------------------------- */
+
+/* a coordinate of a rock box: 'm' metres scaled to a fraction of the size,
+ * else a fraction (in3mg.c:573-575) */
+static double box_frac(const char *sa, double size) { return sa[0] == 'm' ? atof(sa + 1) / size : atof(sa); }
+/* a coordinate of a temperature box: 'm' metres, else a fraction of the
+ * size (in3mg.c:811-825) */
+static double box_metre(const char *sa, double size) { return sa[0] == 'm' ? atof(sa + 1) : atof(sa) * size; }
+
+int i3g_deck_boxes(const char *path, double xsize, double ysize, double zsize, i3g_rock_box *rb, int maxrb, int *nrb,
+		   i3g_temp_box *tb, int maxtb, int *ntb)
+{
+	if (!path || !rb || !tb || !nrb || !ntb || maxrb < 0 || maxtb < 0) return I3G_EINVAL;
+	FILE *f = fopen(path, "r");
+	if (!f) return I3G_EINVAL;
+	char sa[256];
+	*nrb = *ntb = 0;
+	/* the rock table, the boundary conditions: two sections ended by '~' */
+	int ends = 0;
+	while (ends < 2) {
+		TOK();
+		if (sa[0] == '~') ends++;
+	}
+	/* rock bodies (in3mg.c:550-576) */
+	TOK();
+	while (sa[0] != '~') {
+		i3g_rock_box b;
+		memset(&b, 0, sizeof(b));
+		if (sa[0] == 's' || sa[0] == 'c') {
+			b.type = atoi(sa + 1);
+			b.shape = sa[0] == 's' ? 1 : 2;
+		} else {
+			b.type = atoi(sa);
+			b.shape = 0;
+		}
+		TOK(); b.ex = atof(sa);
+		TOK(); b.e = atof(sa);
+		for (int q = 0; q < 8; q++) {
+			TOK(); b.x[q] = box_frac(sa, xsize);
+			TOK(); b.y[q] = box_frac(sa, ysize);
+			TOK(); b.z[q] = box_frac(sa, zsize);
+		}
+		if (*nrb >= maxrb) { fclose(f); return I3G_ECAP; }
+		rb[(*nrb)++] = b;
+		TOK();
+	}
+	/* temperature boxes (in3mg.c:806-829) */
+	TOK();
+	while (sa[0] != '~') {
+		i3g_temp_box b;
+		memset(&b, 0, sizeof(b));
+		b.type = atoi(sa);
+		TOK(); b.x0 = box_metre(sa, xsize);
+		TOK(); b.x2 = box_metre(sa, xsize);
+		TOK(); b.y0 = box_metre(sa, ysize);
+		TOK(); b.y1 = box_metre(sa, ysize);
+		TOK(); b.y2 = box_metre(sa, ysize);
+		TOK(); b.y3 = box_metre(sa, ysize);
+		TOK(); b.z0 = box_metre(sa, zsize);
+		TOK(); b.x5 = box_metre(sa, xsize);
+		TOK(); b.x7 = box_metre(sa, xsize);
+		TOK(); b.y4 = box_metre(sa, ysize);
+		TOK(); b.y5 = box_metre(sa, ysize);
+		TOK(); b.y6 = box_metre(sa, ysize);
+		TOK(); b.y7 = box_metre(sa, ysize);
+		TOK(); b.z7 = box_metre(sa, zsize);
+		for (int q = 0; q < 8; q++) {
+			TOK(); b.t[q] = atof(sa);
+		}
+		if (*ntb >= maxtb) { fclose(f); return I3G_ECAP; }
+		tb[(*ntb)++] = b;
+		TOK();
+	}
+	fclose(f);
+	return I3G_OK;
+}

@@ -1610,3 +1610,176 @@ extern "C" int i3g_markers_set_depth_rules(i3g_markers *m, double zdeep, double 
 	m->tdeep = tdeep;
 	return I3G_OK;
 }
+
+/* ------------------------------------------------------------------ */
+/* Rock bodies of the initial condition (in3mg.c:548-739)             */
+/* ------------------------------------------------------------------ */
+
+/* gausmat (gaus3mg.c:23-62): elimination without pivoting where each row
+ * below the pivot is divided by its own leading coefficient before the
+ * pivot row divided by the pivot is subtracted, then back substitution;
+ * rm x (rm+1) augmented matrix, rows of rm+1 */
+static void gauss_rows(int rm, double *a, double *sol)
+{
+	int rm1 = rm + 1;
+	for (int m1 = 0; m1 < rm - 1; m1++) {
+		double v1 = a[m1 * rm1 + m1];
+		if (v1 == 0) continue;
+		for (int m2 = m1 + 1; m2 < rm; m2++) {
+			double v2 = a[m2 * rm1 + m1];
+			if (v2 == 0) continue;
+			for (int m3 = m1 + 1; m3 <= rm; m3++) a[m2 * rm1 + m3] = a[m2 * rm1 + m3] / v2 - a[m1 * rm1 + m3] / v1;
+		}
+	}
+	for (int m1 = rm - 1; m1 >= 0; m1--) {
+		sol[m1] = 0;
+		double v1 = a[m1 * rm1 + m1];
+		if (v1 != 0) {
+			double v2 = a[m1 * rm1 + rm] / v1;
+			sol[m1] = v2;
+			for (int m2 = 0; m2 <= m1 - 1; m2++) {
+				double w = a[m2 * rm1 + m1];
+				if (w != 0) a[m2 * rm1 + rm] -= w * v2;
+			}
+		}
+	}
+}
+
+/* one bilinear face through the four corners c[]: value v = a0 + a1 p + a2
+ * q + a3 p q, with (p, q) the other two coordinates in the order the
+ * reference writes the matrix rows */
+static void face_coefs(const double *p, const double *q, const double *v, const int *c, double *out)
+{
+	double mat[20];
+	for (int r = 0; r < 4; r++) {
+		int i = c[r];
+		mat[r * 5 + 0] = 1.0;
+		mat[r * 5 + 1] = p[i];
+		mat[r * 5 + 2] = q[i];
+		mat[r * 5 + 3] = p[i] * q[i];
+		mat[r * 5 + 4] = v[i];
+	}
+	gauss_rows(4, mat, out);
+}
+
+struct PaintBox {
+	int shape, type;
+	float ex, e;
+	double k[24]; /* hexahedron: left, right (x), top, bottom (y), front, back (z) faces */
+	double c[8];  /* shells: centre x y z, inner, outer, scales x y z */
+};
+
+__global__ void k_paint(long n, const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+			float *tk, float *e, signed char *type, float *ex, float *od, float *ow, const PaintBox *B, int nbox,
+			double xs, double ys, double zs)
+{
+#pragma clang fp contract(off)
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	float fx = x[t], fy = y[t], fz = z[t];
+	double px = fx / xs, py = fy / ys, pz = fz / zs;
+	int hit = -1, hex = -1;
+	for (int b = 0; b < nbox; b++) {
+		const PaintBox &P = B[b];
+		bool in = false;
+		if (P.shape == 0) {
+			const double *k = P.k;
+			in = px >= k[0] + k[1] * py + k[2] * pz + k[3] * py * pz && px <= k[4] + k[5] * py + k[6] * pz + k[7] * py * pz &&
+			     py >= k[8] + k[9] * px + k[10] * pz + k[11] * px * pz &&
+			     py <= k[12] + k[13] * px + k[14] * pz + k[15] * px * pz &&
+			     pz >= k[16] + k[17] * py + k[18] * px + k[19] * py * px &&
+			     pz <= k[20] + k[21] * py + k[22] * px + k[23] * py * px;
+		} else {
+			double dx = (px - P.c[0]) / P.c[5];
+			double dy = (py - P.c[1]) / P.c[6];
+			double dz = (pz - P.c[2]) / P.c[7];
+			if (P.shape == 1) {
+				double ival = pow(dx * dx + dy * dy + dz * dz, 0.5);
+				in = ival >= P.c[3] && ival <= P.c[4];
+			} else {
+				double ival = pow(dx * dx + dz * dz, 0.5);
+				in = ival >= P.c[3] && ival <= P.c[4] && fabs(dy) <= 1.0;
+			}
+		}
+		if (in) {
+			hit = b;
+			if (P.shape == 0) hex = b;
+		}
+	}
+	if (hit < 0) return;
+	type[t] = (signed char)B[hit].type;
+	e[t] = B[hit].e;
+	if (ex) ex[t] = B[hit].ex;
+	if (hex >= 0) {
+		/* the last hexahedron sets markk/markd/markw; a later shell does not
+		 * touch them */
+		bool imm = B[hex].type >= 100;
+		tk[t] = imm ? fx : 0.0f;
+		if (od) od[t] = imm ? fy : 0.0f;
+		if (ow) ow[t] = imm ? fz : 0.0f;
+	}
+}
+
+extern "C" int i3g_markers_paint(i3g_markers *m, double xsize, double ysize, double zsize, const i3g_rock_box *boxes,
+				 int nbox, float *ex, float *od, float *ow)
+{
+	if (!m || !boxes || nbox < 1 || !(xsize > 0) || !(ysize > 0) || !(zsize > 0)) return I3G_EINVAL;
+	std::vector<PaintBox> pb(nbox);
+	for (int b = 0; b < nbox; b++) {
+		const i3g_rock_box &r = boxes[b];
+		if (r.shape < 0 || r.shape > 2 || r.type < -128 || r.type > 127) return I3G_EINVAL;
+		PaintBox &P = pb[b];
+		memset(&P, 0, sizeof(P));
+		P.shape = r.shape;
+		P.type = r.type;
+		P.ex = (float)r.ex;
+		P.e = (float)r.e;
+		if (r.shape == 0) {
+			/* the six faces in the order and with the corners of
+			 * in3mg.c:592-694 */
+			const int left[4] = {0, 1, 4, 5}, right[4] = {2, 3, 6, 7}, top[4] = {0, 2, 4, 6}, bot[4] = {1, 3, 5, 7};
+			const int front[4] = {0, 1, 2, 3}, back[4] = {4, 5, 6, 7};
+			face_coefs(r.y, r.z, r.x, left, P.k + 0);
+			face_coefs(r.y, r.z, r.x, right, P.k + 4);
+			face_coefs(r.x, r.z, r.y, top, P.k + 8);
+			face_coefs(r.x, r.z, r.y, bot, P.k + 12);
+			face_coefs(r.y, r.x, r.z, front, P.k + 16);
+			face_coefs(r.y, r.x, r.z, back, P.k + 20);
+		} else {
+			P.c[0] = r.x[0]; P.c[1] = r.y[0]; P.c[2] = r.z[0];
+			P.c[3] = r.x[2]; P.c[4] = r.x[5];
+			P.c[5] = r.x[7]; P.c[6] = r.y[7]; P.c[7] = r.z[7];
+		}
+	}
+	long n = m->n;
+	int err = 0;
+	PaintBox *dB = mk_alloc<PaintBox>(nbox, &err);
+	float *dex = ex ? mk_alloc<float>(n, &err) : nullptr;
+	float *dod = od ? mk_alloc<float>(n, &err) : nullptr;
+	float *dow = ow ? mk_alloc<float>(n, &err) : nullptr;
+	if (err) {
+		hipFree(dB); hipFree(dex); hipFree(dod); hipFree(dow);
+		return err;
+	}
+	int rc = I3G_OK;
+	hipError_t e = hipMemcpyAsync(dB, pb.data(), nbox * sizeof(PaintBox), hipMemcpyHostToDevice, m->s);
+	if (e == hipSuccess && ex && n) e = hipMemcpyAsync(dex, ex, n * sizeof(float), hipMemcpyDefault, m->s);
+	if (e == hipSuccess && od && n) e = hipMemcpyAsync(dod, od, n * sizeof(float), hipMemcpyDefault, m->s);
+	if (e == hipSuccess && ow && n) e = hipMemcpyAsync(dow, ow, n * sizeof(float), hipMemcpyDefault, m->s);
+	if (e == hipSuccess && n) {
+		mark_begin(m);
+		hipLaunchKernelGGL(k_paint, g1(n), dim3(256), 0, m->s, n, (const float *)m->x, (const float *)m->y,
+				   (const float *)m->z, m->tk, m->e, m->type, dex, dod, dow, (const PaintBox *)dB, nbox, xsize,
+				   ysize, zsize);
+		e = hipGetLastError();
+		mark_end(m);
+	}
+	if (e == hipSuccess && ex && n) e = hipMemcpyAsync(ex, dex, n * sizeof(float), hipMemcpyDefault, m->s);
+	if (e == hipSuccess && od && n) e = hipMemcpyAsync(od, dod, n * sizeof(float), hipMemcpyDefault, m->s);
+	if (e == hipSuccess && ow && n) e = hipMemcpyAsync(ow, dow, n * sizeof(float), hipMemcpyDefault, m->s);
+	if (e == hipSuccess) e = hipStreamSynchronize(m->s);
+	if (e != hipSuccess) rc = I3G_EDEVICE;
+	else mark_read(m);
+	hipFree(dB); hipFree(dex); hipFree(dod); hipFree(dow);
+	return rc;
+}

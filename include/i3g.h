@@ -758,6 +758,87 @@ int i3g_model_get(i3g_model *m, float *vx, float *vy, float *vz, double *pr, flo
 i3g_markers *i3g_model_markers(i3g_model *m);
 i3g_ctx *i3g_model_stokes(i3g_model *m);
 
+/* ------------------------------------------------------------------ */
+/* Initial condition generator and ParaView output                    */
+/* ------------------------------------------------------------------ */
+
+/* A rock body of the reference deck (init.t3c rock boxes, read by
+ * in3mg.c:550-575).  shape 0: a hexahedron whose faces are bilinear
+ * surfaces x = a0 + a1 y + a2 z + a3 y z (left: corners 0 1 4 5, right:
+ * 2 3 6 7), y = .. (top 0 2 4 6, bottom 1 3 5 7), z = .. (front 0-3, back
+ * 4-7); shape 1: a spherical shell around corner 0, the distance scaled by
+ * corner 7's x, y, z, between corner 2's x and corner 5's x; shape 2: the
+ * same for a cylinder along y (its half length scaled by corner 7's y).
+ * Coordinates are fractions of the domain size. */
+typedef struct {
+	int shape;
+	int type;       /* rock type given to the markers inside (>= 100: immobile) */
+	double ex, e;   /* melt extraction (markex) and strain (marke) given to them */
+	double x[8], y[8], z[8];
+} i3g_rock_box;
+
+/* Paint the resident markers with the rock bodies in order, as in3mg.c:
+ * 548-739 does (a later body wins): type and strain on the device, and for
+ * hexahedra the temperature slot markk, set to the marker's x (immobile
+ * types >= 100, whose origin it holds) or 0.  ex, od, ow: the markers'
+ * markex, markd, markw in the current order (in/out, host or device, any
+ * may be NULL); hexahedra set markd/markw to the immobile origin (y, z) or
+ * 0.  The face coefficients come from the reference's 4x4 elimination
+ * without pivoting (gausmat, gaus3mg.c:23-62), restated, so that markers on
+ * a face fall on the same side. */
+int i3g_markers_paint(i3g_markers *m, double xsize, double ysize, double zsize,
+		      const i3g_rock_box *boxes, int nbox, float *ex, float *od, float *ow);
+
+/* A temperature box of the reference deck (init.t3c, read by in3mg.c:
+ * 806-829): type 0 trilinear between the corner temperatures t0..t7; 1/4
+ * half-space cooling T = t1 - erfc(d / 2 sqrt(kappa age)) (t1 - t0) + t7 d
+ * with the age bilinear from t2..t5 and kappa t6; 5/6 a transition
+ * bilinear between the field's own columns at the box corners (x0, z0),
+ * (x2, z0), (x5, z7), (x7, z7); 2 a spherical shell around (x0, y0, z0),
+ * distances scaled by (x7, y7, z7), between x2/xsize and x5/xsize, T from
+ * t0 to t1; 3 the cylinder along y.  The corner coordinates in metres. */
+typedef struct {
+	int type;
+	double x0, x2, y0, y1, y2, y3, z0, x5, x7, y4, y5, y6, y7, z7;
+	double t[8];
+} i3g_temp_box;
+
+/* Node temperatures from the boxes in order (in3mg.c:830-951), on the
+ * device: tk (level-0 nodes, host or device, in/out).  xsize etc. are the
+ * deck's domain sizes (types 2, 3 scale by them).  The transition types
+ * read the field as the reference's serial loop leaves it: one thread per
+ * y line walks its nodes in the reference's order. */
+int i3g_temperature_boxes(long nx, long ny, long nz, const double *gx, const double *gy,
+			  const double *gz, double xsize, double ysize, double zsize,
+			  const i3g_temp_box *boxes, int nbox, float *tk);
+
+/* The rock and temperature boxes of a deck in the reference's format
+ * (init.t3c): the sections after its second and third '~' (ffscanf1's
+ * tokens: whitespace separated, tokens starting with '/' skipped; 'm'
+ * prefixes metres, other coordinates are fractions of the sizes; 's'/'c'
+ * type prefixes select the shell shapes).  At most maxrb / maxtb boxes. */
+int i3g_deck_boxes(const char *path, double xsize, double ysize, double zsize,
+		   i3g_rock_box *rb, int maxrb, int *nrb, i3g_temp_box *tb, int maxtb, int *ntb);
+
+/* ParaView output as the reference writes it (i3vis_output_paraview.c,
+ * binary appended, little endian): i3g_write_vtr_nodal writes the
+ * RectilinearGrid of the node fields (x fastest in the file; host arrays
+ * in the node order m3, m1, m2; f64[q] != 0 marks a double field, f64 may
+ * be NULL), as saver_paraview writes "viscosity", "temperature",
+ * "density", "vy", "vx", "vz" (float) and "pressure" (double);
+ * i3g_write_vtr_composition the cell
+ * composition map of the markers on an mx x my x mz uniform grid
+ * (i3vis_generate_compo_map, including its claim rule and neighbour fill,
+ * :1111-1314); i3g_pvd_open / i3g_pvd_append the time collection. */
+int i3g_write_vtr_nodal(const char *path, long nx, long ny, long nz, const double *gx,
+			const double *gy, const double *gz, int nfield, const char *const *names,
+			const void *const *fields, const int *f64);
+int i3g_write_vtr_composition(const char *path, double xsize, double ysize, double zsize, long mx,
+			      long my, long mz, long n, const float *x, const float *y,
+			      const float *z, const signed char *type);
+int i3g_pvd_open(const char *path);
+int i3g_pvd_append(const char *path, double time, const char *datafile, const char *dir);
+
 const char *i3g_error_string(int code);
 
 #ifdef __cplusplus

@@ -28,6 +28,11 @@ inputs and outputs only.
   g15_ronurecalc_rheology.npz  ronurecalc with the deck's rocks: retyping,
                     water/air mixing, viscalc from strain rates, phase
                     densities (mark3mg.c:404-1015)
+  g16_*             the reference's generator in3mg run on g16_init.t3c (a
+                    deck of this build's own values) with g16_mode.t3c: the
+                    state it saves (g16_generator.npz) and its ParaView
+                    nodal and composition files (in3mg.c:548-1049,
+                    i3vis_output_paraview.c)
   g12_smoother.npz  on the g1 inputs: the reference's own coefficient of each
                     assembled row's unknown (the first row entry, gausmat4's
                     divisor ival1, move3mg.c:3096-3124) and the state after
@@ -1076,11 +1081,161 @@ def g15(R):
     print("g15_ronurecalc_rheology.npz written", "sedtype", sedtype)
 
 
+G16_ROCKS = {
+    # type: n0 n1 s0 s1 nu dh dv ss mm ll a0 a1 b0 b1 e0 e1 ro bb aa cp kt kf kp ht (own values)
+    0: "1e18 1e18 0 5e29 1e18 0 0 0 1 0 0 0 0 0 0 0.5 1 0 0 3.3e6 200 0 0 0",
+    1: "1e18 1e18 0 5e29 1e18 0 0 0 1 0 0 0 0 0 0 0.5 1000 0 0 3.3e3 200 0 0 0",
+    2: "1e18 1e26 2 5e29 1.9e17 1.5e5 0 3e4 2.4 1 2e6 2e6 0 0 0 0.5 2650 2e-5 0.5e-3 1000 0.65 800 0 2e-6",
+    5: "1e18 1e26 0 5e29 1.9e17 1.5e5 0 3e4 2.4 1 3e6 3e6 0.3 0 0 0.5 2750 2e-5 0.6e-3 1000 0.65 800 0 2e-6",
+    6: "1e18 1e26 0 5e29 4.7e22 2.4e5 0 3e4 3.1 1 3e6 3e6 0.3 0 0 0.5 2950 2e-5 0.6e-3 1000 1.2 470 0 2e-7",
+    7: "1e18 1e26 0 5e29 4.7e22 2.4e5 0 3e4 3.1 1 3e6 3e6 0 0 0 0.5 3000 2e-5 0.6e-3 1000 1.2 470 0 2.2e-7",
+    8: "1e18 1e26 0 5e29 4.7e22 2.4e5 0 3e4 3.1 1 3e6 3e6 0.6 0 0 0.5 3000 2e-5 0.6e-3 1000 1.2 470 0 2.4e-7",
+    9: "1e18 1e26 0 1.4e-15 1.2e-16 -5.2e5 2.5 3.7e5 3.5 0.7 3e6 3e6 0.6 0 0 0.5 3300 2e-5 0.6e-3 1000 0.73 1290 4e-6 2.2e-8",
+    10: "1e18 1e26 0 1.4e-15 1.2e-16 -5.2e5 2.5 3.7e5 3.5 0.7 3e6 3e6 0.6 0 0 0.5 3300 2e-5 0.6e-3 1000 0.73 1290 4e-6 2.4e-8",
+    11: "1e18 1e26 0 5e15 -2e18 4.2e5 1 2.4e5 3 0.4 3e6 3e6 0 0 0 1 3250 2e-5 0.6e-3 1000 0.73 1290 4e-6 2.2e-8",
+    12: "1e18 1e26 0 1.4e-15 1.2e-16 -5.2e5 2.5 3.7e5 3.5 0.7 3e6 3e6 0 0 0 0.5 3300 2e-5 0.6e-3 1000 0.73 1290 4e-6 2.6e-8",
+    13: "1e18 1e26 0 5e29 3e36 9e3 0.3 3e6 3.8 1 3e6 3e6 0 0 0 1 3200 2e-5 0.6e-3 1000 0.73 1290 4e-6 2.2e-8",
+    14: "1e18 1e26 0 8e15 -3e22 5.3e5 2 3e5 3.5 0.4 3e6 3e6 0.6 0 0 0.5 3300 2e-5 0.6e-3 1000 0.73 1290 4e-6 2.2e-8",
+}
+
+# rock boxes: type, melt extraction, strain, 8 corners (x y z), fractions of
+# the size or 'm'-prefixed metres (in3mg.c:550-575)
+G16_RBOX = [
+    "10 0.00 0.00  -0.1 -0.1 -0.1  -0.1 1.1 -0.1  1.1 -0.1 -0.1  1.1 1.1 -0.1"
+    "  -0.1 -0.1 1.1  -0.1 1.1 1.1  1.1 -0.1 1.1  1.1 1.1 1.1",
+    "9 0.05 0.10  0 m18000 0  0 m62000 0  1 m22000 0  1 m58000 0  0 m18000 1  0 m62000 1  1 m22000 1  1 m58000 1",
+    "5 0.00 0.25  0 0 0  0 m18000 0  1 0 0  1 m22000 0  0 0 1  0 m18000 1  1 0 1  1 m22000 1",
+    "2 0.00 0.00  0 0 0  0 m7000 0  1 0 0  1 m7000 0  0 0 1  0 m7000 1  1 0 1  1 m7000 1",
+    "12 0.10 0.70  m90000 m18000 m40000  m130000 m80000 m40000  m110000 m18000 m40000  m150000 m80000 m40000"
+    "  m70000 m18000 m140000  m115000 m80000 m140000  m95000 m18000 m140000  m135000 m80000 m140000",
+    "7 0.00 0.00  m90000 m18000 m40000  m96000 m30000 m40000  m110000 m18000 m40000  m116000 m30000 m40000"
+    "  m70000 m18000 m140000  m76000 m30000 m140000  m95000 m18000 m140000  m101000 m30000 m140000",
+    "s11 0.20 0.05  0.62 0.70 0.45  0 0 0  0.0 0 0  0 0 0  0 0 0  0.09 0 0  0 0 0  1.0 1.2 0.9",
+    "c13 0.00 0.30  0.25 0.80 0.60  0 0 0  0.03 0 0  0 0 0  0 0 0  0.11 0 0  0 0 0  1.0 0.15 1.0",
+    "114 0.00 0.00  m160000 m70000 m20000  m160000 m100000 m20000  m200000 m70000 m20000  m200000 m100000 m20000"
+    "  m160000 m70000 m60000  m160000 m100000 m60000  m200000 m70000 m60000  m200000 m100000 m60000",
+    "8 0.00 0.00  0.05 0.40 0.10  0.05 0.47 0.10  0.20 0.40 0.12  0.20 0.47 0.12  0.05 0.40 0.30  0.05 0.47 0.30"
+    "  0.20 0.40 0.33  0.20 0.47 0.33",
+]
+
+# temperature boxes (in3mg.c:806-951): type x0 x2 y0 y1 y2 y3 z0 x5 x7 y4 y5 y6 y7 z7 t0..t7
+G16_TBOX = [
+    "0  -0.000001 1.000001  -0.000001 1.000001 -0.000001 1.000001 -0.000001  -0.000001 1.000001 -0.000001 1.000001"
+    " -0.000001 1.000001 1.000001  273 1700 273 1700 273 1700 273 1700",
+    "4  -0.000001 0.55  m7000 m80000 m7000 m80000 -0.000001  -0.000001 0.50 m7000 m80000 m7000 m80000 1.000001"
+    "  273 1580 3e7 5e7 3e7 5e7 1e-6 0.4e-3",
+    "4  0.75 1.000001  m7000 m80000 m7000 m80000 -0.000001  0.78 1.000001 m7000 m80000 m7000 m80000 1.000001"
+    "  273 1580 8e7 8e7 6e7 6e7 1e-6 0.4e-3",
+    "5  0.55 0.75  m7000 m80000 m7000 m80000 -0.000001  0.50 0.78 m7000 m80000 m7000 m80000 1.000001"
+    "  0 0 0 0 0 0 0 0",
+    "2  0.62 m6000  0.70 0 0 0 0.45  m21000 1.0 0 0 0 1.2 0.9  1750 1400 0 0 0 0 0 0",
+    "3  0.25 m3300  0.80 0 0 0 0.60  m12000 1.0 0 0 0 0.15 1.0  1300 900 0 0 0 0 0 0",
+]
+
+G16_BC = """ P     1   x     1   1     1   1     0       1.0 +0 +0 +1   0 0
+ P     1   1     1   1     2   z-1   0       1.0 +1 +0 +0   0 0
+ P     1   x     1   1     z   z     0       1.0 +0 +0 -1   0 0
+ P     x   x     1   1     2   z-1   0       1.0 -1 +0 +0   0 0
+ Vx    1   x-1   0   0     0   z-1   0       1.0 +0 +1 +0   0 0
+ Vy    0   x-1   0   0     0   z-1   0       0   0 0
+ Vz    0   x-1   0   0     1   z-1   0       1.0 +0 +1 +0   0 0
+ T     0   x     0   0     0   z     273     0   0 0
+ P     1   x     y   y     1   1     0       1.0 +0 +0 +1   0 0
+ P     1   1     y   y     2   z-1   0       1.0 +1 +0 +0   0 0
+ P     1   x     y   y     z   z     0       1.0 +0 +0 -1   0 0
+ P     x   x     y   y     2   z-1   0       1.0 -1 +0 +0   0 0
+ Vx    1   x-1   y-1 y-1   0   z-1   0       1.0 +0 -1 +0   0 0
+ Vy    0   x-1   y   y     0   z-1   0       0   0 0
+ Vz    0   x-1   y-1 y-1   1   z-1   0       1.0 +0 -1 +0   0 0
+ T     0   x     y   y     0   z     0       1.0 +0 -1 +0   0 0
+ P     1   1     2   y-1   1   1     0       1.0 +1 +0 +1   0 0
+ P     1   1     2   y-1   z   z     0       1.0 +1 +0 -1   0 0
+ Vx    0   0     0   y-1   0   z-1   0       0   0 0
+ Vy    0   0     1   y-1   0   z-1   0       1.0 +1 +0 +0   0 0
+ Vz    0   0     0   y-1   1   z-1   0       1.0 +1 +0 +0   0 0
+ T     0   0     1   y-1   0   z     0       1.0 +1 +0 +0   0 0
+ P     x   x     2   y-1   1   1     0       1.0 -1 +0 +1   0 0
+ P     x   x     2   y-1   z   z     0       1.0 -1 +0 -1   0 0
+ Vx    x   x     0   y-1   0   z-1   0       0   0 0
+ Vy    x-1 x-1   1   y-1   0   z-1   0       1.0 -1 +0 +0   0 0
+ Vz    x-1 x-1   0   y-1   1   z-1   0       1.0 -1 +0 +0   0 0
+ T     x   x     1   y-1   0   z     0       1.0 -1 +0 +0   0 0
+ Vx    1   x-1   0   y-1   0   0     0       1.0 +0 +0 +1   0 0
+ Vy    0   x-1   1   y-1   0   0     0       1.0 +0 +0 +1   0 0
+ Vz    0   x-1   0   y-1   0   0     0       0   0 0
+ T     1   x-1   1   y-1   0   0     0       1.0 +0 +0 +1   0 0
+ Vx    1   x-1   0   y-1   z-1 z-1   0       1.0 +0 +0 -1   0 0
+ Vy    0   x-1   1   y-1   z-1 z-1   0       1.0 +0 +0 -1   0 0
+ Vz    0   x-1   0   y-1   z   z     0       0   0 0
+ T     1   x-1   1   y-1   z   z     0       1.0 +0 +0 -1   0 0
+"""
+
+
+def g16_deck():
+    """init.t3c text of the g16 run (own values in the reference's layout)"""
+    lines = ["/g16_generator_deck_(own_values)",
+             "17-xnumx", "13-ynumy", "15-znumz", "2-mnumx", "2-mnumy", "2-mnumz",
+             "2.40000000e+05-xsize(m)", "1.20000000e+05-ysize(m)", "1.80000000e+05-zsize(m)",
+             "2-pxinit", "1-pyinit", "2-pzinit", "0.0-pinit(Pa)",
+             "0.0-GXKOEF", "9.80665-GYKOEF", "0.0-GZKOEF", "0.0-timesum(Years)",
+             "0-nonstab 0.5-xnonstab 0.5-ynonstab 0.5-znonstab",
+             "/input_file", "0", "/output_file", "g16.prn b", "/ROCKS"]
+    for t in sorted(G16_ROCKS):
+        lines.append(f"/rock_{t}")
+        lines.append(f"{t} {G16_ROCKS[t]}")
+    lines += ["~", "/BONDARIES", G16_BC.rstrip("\n"), "~", "/ROCK_BOXES"]
+    lines += G16_RBOX
+    lines += ["~", "/T_BOXES"] + G16_TBOX + ["~", ""]
+    return "\n".join(lines)
+
+
+def g16(R=None):
+    """The reference's initial-condition generator in3mg (its own program,
+    built by oracle/Makefile target in3mg) on the deck g16_init.t3c: rock
+    bodies painted on the regular marker lattice (hexahedra with bilinear
+    faces, ellipsoid and cylinder shells, immobile markers), temperature
+    boxes (trilinear, half-space cooling, transition, sphere, cylinder),
+    marker temperatures interpolated from them, two ronurecalc passes, then
+    its savers: the .prn state (parsed into g16_generator.npz) and the
+    ParaView nodal and composition files (kept as written)."""
+    import shutil
+    import subprocess
+    import tempfile
+    deck = g16_deck()
+    with open(os.path.join(HERE, "g16_init.t3c"), "w") as f:
+        f.write(deck)
+    mode = open(os.path.join(HERE, "g13_mode.t3c")).read()
+    mode = mode.replace("1.1e4-eroslev 1.2e4-sedilev 1.3e4-waterlev", "-1e30-eroslev 1e30-sedilev -1-waterlev")
+    assert "-1e30-eroslev" in mode
+    exe = os.path.join(ROOT, "oracle", "_ref", "in3mg")
+    with tempfile.TemporaryDirectory() as tmp:
+        with open(os.path.join(tmp, "init.t3c"), "w") as f:
+            f.write(deck)
+        with open(os.path.join(tmp, "mode.t3c"), "w") as f:
+            f.write(mode)
+        shutil.copy(os.path.join(HERE, "g13_file.t3c"), os.path.join(tmp, "file.t3c"))
+        r = subprocess.run([exe], cwd=tmp, stdin=subprocess.DEVNULL, capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        files = sorted(os.listdir(tmp))
+        print("in3mg wrote", files)
+        d = I.read_prn(os.path.join(tmp, "g16.prn"))
+        shutil.copy(os.path.join(tmp, "g16.vtr"), os.path.join(HERE, "g16_nodal.vtr"))
+        shutil.copy(os.path.join(tmp, "comp_g16.vtr"), os.path.join(HERE, "g16_comp.vtr"))
+        with open(os.path.join(HERE, "g16_mode.t3c"), "w") as f:
+            f.write(mode)
+    keep = {k: v for k, v in d.items() if isinstance(v, np.ndarray) and k not in ("bond_at", "bond_rec", "bond_partner",
+                                                                                  "bond_v")}
+    scal = {k: v for k, v in d.items() if not isinstance(v, np.ndarray)}
+    np.savez_compressed(os.path.join(HERE, "g16_generator.npz"), **keep,
+                        **{f"h_{k}": np.asarray(v) for k, v in scal.items()})
+    print("g16 written:", d["marknum"], "markers", np.bincount(d["markt"].astype(np.uint8)))
+
+
 if __name__ == "__main__":
     R = Ref()
     R.lib.refh_set_threads(1)
     which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9", "g10", "g11", "g12", "g3c2", "g13", "g14",
-                             "g15"]
+                             "g15", "g16"]
     if "g1" in which:
         g1(R)
     if "g3" in which:
@@ -1111,3 +1266,5 @@ if __name__ == "__main__":
         g14(R)
     if "g15" in which:
         g15(R)
+    if "g16" in which:
+        g16(R)
