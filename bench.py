@@ -201,7 +201,7 @@ def other_paths(I, np, cfg257):
     M.cell_index()
     res["cell_index"] = (M.last_ms(), m * 16.0)
     M.sort()
-    res["sort"] = (M.last_ms(), m * (8.0 + 2 * 29))
+    res["sort"] = (M.last_ms(), m * (8.0 + 2 * 37))
     rocks = [[1e21, 0, 0, 1e18, 1e25, 3300, 3e-5, 1e-3, 1000, 3, 0, 0, 1e-8]] * 2 + \
         [[2e16, 3e5, 4.0, 1e19, 1e24, 3300, 3e-5, 1e-3, 1000, 0.73, 1293, 4e-6, 2e-8]] * 5
     M.to_nodes(rocks, np.full(N, 1e8), np.zeros((11, N), np.float32), 1e18, 1e25, 0.0, 1, 0.0)
@@ -267,7 +267,7 @@ def time_loop(I, np, steps=3, cap_s=90.0, nx=257, ny=257, nz=129, per_axis=2, le
     sec = mean("seconds")
     # per-stage algorithmic HBM bytes: markers M * bytes, nodes N * bytes
     N = nx * ny * nz
-    stage_bytes = dict(t_rk4=nmk * 24.0 + N * 12.0, t_sort=nmk * (8.0 + 2 * 29), t_transfer=nmk * 40.0 + N * 44.0)
+    stage_bytes = dict(t_rk4=nmk * 24.0 + N * 12.0, t_sort=nmk * (8.0 + 2 * 37), t_transfer=nmk * 40.0 + N * 44.0)
     gbs = {k: round(b / mean(k) / 1e9, 1) for k, b in stage_bytes.items() if rows and mean(k) > 0}
     return dict(workload=f"time loop {nx}x{ny}x{nz} subduction, Arrhenius E=300 kJ/mol, {nmk} markers",
                 steps=len(rows), steps_per_s=(round(1.0 / sec, 4) if sec else None), mean_step_s=sec,
