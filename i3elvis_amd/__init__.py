@@ -335,7 +335,7 @@ class MkParams(C.Structure):
                 ("timestep", C.c_double), ("vdeep", C.c_double), ("nudeep", C.c_double), ("viscmod", C.c_int),
                 ("rheology", C.c_int), ("eroslev", C.c_double), ("sedilev", C.c_double), ("waterlev", C.c_double),
                 ("sedtype", C.c_int), ("mnumy", C.c_long), ("teclmin", C.c_double), ("teclmax", C.c_double),
-                ("timesum", C.c_double), ("convend", C.c_double), ("outgrid", C.c_int)]
+                ("timesum", C.c_double), ("convend", C.c_double), ("outgrid", C.c_int), ("maxpmelt", C.c_double)]
 
 
 class MoveParams(C.Structure):
@@ -346,11 +346,11 @@ class MoveParams(C.Structure):
 
 def mk_params(nubeg=1e18, nuend=1e25, nukoef=0.0, densimod=1, timestep=1.0, vdeep=1e30, nudeep=1e30, viscmod=0,
               rheology=0, eroslev=-1e30, sedilev=1e30, waterlev=-1.0, sedtype=3, mnumy=1, teclmin=0.0,
-              teclmax=0.0, timesum=0.0, convend=0.0, outgrid=0):
+              teclmax=0.0, timesum=0.0, convend=0.0, outgrid=0, maxpmelt=1e10):
     return MkParams(float(nubeg), float(nuend), float(nukoef), int(densimod), float(timestep), float(vdeep),
                     float(nudeep), int(viscmod), int(rheology), float(eroslev), float(sedilev), float(waterlev),
                     int(sedtype), int(mnumy), float(teclmin), float(teclmax), float(timesum), float(convend),
-                    int(outgrid))
+                    int(outgrid), float(maxpmelt))
 
 
 def rocks_array(rocks):
@@ -425,7 +425,7 @@ class Markers:
                  nudeep=1e30, viscmod=0, **rheo):
         """marker -> node transfer; rheo: the i3g_mk_params fields of the
         whole rheology (rheology=1, eroslev, sedilev, waterlev, sedtype,
-        mnumy, teclmin, teclmax, timesum, convend)"""
+        mnumy, teclmin, teclmax, timesum, convend, maxpmelt)"""
         fld = np.array(fld, np.float32, copy=True)
         pr = np.ascontiguousarray(pr, np.float64)
         prm = mk_params(nubeg, nuend, nukoef, densimod, timestep, vdeep, nudeep, viscmod, **rheo)
@@ -503,6 +503,11 @@ class Markers:
     def set_grain(self, rr, rn):
         a, b = (np.ascontiguousarray(q, np.float32) for q in (rr, rn))
         check(lib().i3g_markers_set_grain(self.h, _p(a), _p(b)), "markers_set_grain")
+
+    def set_melt(self, w, ex):
+        """water content and extracted melt by marker id"""
+        a, b = (np.ascontiguousarray(q, np.float32) for q in (w, ex))
+        check(lib().i3g_markers_set_melt(self.h, C.c_long(len(a)), _p(a), _p(b)), "markers_set_melt")
 
     def get_grain(self):
         rr, rn = np.zeros(self.n, np.float32), np.zeros(self.n, np.float32)

@@ -64,6 +64,10 @@ struct i3g_markers {
 	int has_st = 0;          /* st12 holds the planes of i3g_markers_set_stress */
 	float *md = nullptr;     /* marker density by id (fluid velocity) */
 	long md_cap = 0;
+	/* water content (markw) and extracted melt (markex) by id, of the melt
+	 * types' properties; mw_n ids, none until i3g_markers_set_melt */
+	float *mw = nullptr;
+	long mw_n = 0, mw_cap = 0;
 	/* movemark's depth rules (inactive by default) */
 	double zdeep = 1e30, vdeep = 1e30, tdeep = 0;
 	i3g_rock *rocks = nullptr;
@@ -331,7 +335,7 @@ __global__ void k_marker_props(long n, const float *__restrict__ x, const float 
 			       float *e, float *rr, float *rn, const int *c1, const int *c2, const int *c3,
 			       const double *pr, const float *st12, long N, const double *gx, const double *gy,
 			       const double *gz, int nx, int ny, int nz, const i3g_rock *rocks, int nrock,
-			       i3g_mk_params prm, MkRec *rec, int *bad)
+			       i3g_mk_params prm, const int *id, const float *mw, long mw_n, MkRec *rec, int *bad)
 {
 #pragma clang fp contract(off)
 	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -352,7 +356,7 @@ __global__ void k_marker_props(long n, const float *__restrict__ x, const float 
 		if (mm2 < 2 && fy > prm.waterlev) mm2 = 1;
 		if (mm2 < 2 && fy < prm.waterlev) mm2 = 0;
 		if (mm2 != mt) type[t] = (signed char)mm2;
-		if (mm2 > 20 || mm2 >= nrock || prm.densimod > 1) {
+		if (mm2 >= nrock || prm.densimod > 1) {
 			atomicOr(bad, 1);
 			R->in = 0;
 			return;
@@ -388,6 +392,26 @@ __global__ void k_marker_props(long n, const float *__restrict__ x, const float 
 	double mcp = r.cp;
 	double mkt = (r.kt + r.kf / (mtk + 77.0)) * exp(r.kp * mpb);
 	double mht = r.ht;
+	if (prm.rheology && mm2 > 20) {
+		/* partially molten rock (mark3mg.c:660-686) */
+		int mid = id[t];
+		bool has = mw && mid >= 0 && mid < mw_n;
+		double mwa = has ? (double)mw[mid] : 0.0, mex = has ? (double)mw[mw_n + mid] : 0.0, xm;
+		meltpart0_dev(r, rocks[mm2 == 34 ? 11 : mm2 - 20], mm2, mtk, mpb, mwa, mex, prm.densimod, prm.teclmin,
+			      prm.teclmax, prm.maxpmelt, mbb, mro, mcp, mkt, xm);
+		if ((mm2 == 27 || mm2 == 28) && xm > 0.001) {
+			/* basaltic melt viscosity (Schott and Schmeling 1995) */
+			double nm = 1e+15 * exp(2.5 + pow((1.0 - xm) / xm, 0.48) * (1.0 - xm));
+			if (nm < mnu) {
+				mnu = nm;
+				if (mnu < r.n0) mnu = r.n0;
+				if (mnu > r.n1) mnu = r.n1;
+				if (mnu < prm.nubeg) mnu = prm.nubeg;
+				if (mnu > prm.nuend) mnu = prm.nuend;
+				eta[t] = (float)mnu;
+			}
+		}
+	}
 	if (prm.rheology && prm.waterlev >= 0 && mm2 < 2) {
 		int wy = c2[t];
 		double dywa = (gy[wy + 1] - gy[wy]) / 2.0 / ((double)prm.mnumy);
@@ -1153,7 +1177,7 @@ extern "C" int i3g_markers_destroy(i3g_markers *m)
 		      m->tk2, m->eta2, m->type2, m->id2, m->key, m->c1, m->c2, m->c3, m->perm, m->cnt, m->start,
 		      m->cur, m->bsum, m->prec, m->pr, m->fld,
 		      m->vel, m->rocks, m->bad, m->tbc_p, m->tbc_c, m->tbc_k, m->tk1, m->tnod, m->trec, m->sn, m->tin,
-		      m->e, m->e2, m->st12, m->rr, m->rr2, m->rn, m->rn2, m->md};
+		      m->e, m->e2, m->st12, m->rr, m->rr2, m->rn, m->rn2, m->md, m->mw};
 	for (void *p : ps)
 		if (p) hipFree(p);
 	if (m->ev0) hipEventDestroy(m->ev0);
@@ -1321,7 +1345,7 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 				   (const int *)m->c1, (const int *)m->c2, (const int *)m->c3, (const double *)m->pr,
 				   (const float *)(m->has_st ? m->st12 : nullptr), N, (const double *)m->gx,
 				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, (const i3g_rock *)m->rocks,
-				   nrock, *prm, m->prec, m->bad);
+				   nrock, *prm, (const int *)m->id, (const float *)m->mw, m->mw_n, m->prec, m->bad);
 	dim3 gg((unsigned)(((long)m->nx * m->ny + 255) / 256), (unsigned)m->nz);
 	hipLaunchKernelGGL(k_marker_gather, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz, (const double *)m->gx,
 			   (const double *)m->gy, (const double *)m->gz, (const int *)m->start, (const int *)m->cnt,
@@ -1487,6 +1511,27 @@ extern "C" int i3g_markers_set_grain(i3g_markers *m, const float *rr, const floa
 	if (m->n && rr) MCHK(hipMemcpyAsync(m->rr, rr, m->n * sizeof(float), hipMemcpyDefault, m->s));
 	if (m->n && rn) MCHK(hipMemcpyAsync(m->rn, rn, m->n * sizeof(float), hipMemcpyDefault, m->s));
 	MCHK(hipStreamSynchronize(m->s));
+	return I3G_OK;
+}
+
+extern "C" int i3g_markers_set_melt(i3g_markers *m, long n, const float *w, const float *ex)
+{
+	if (!m || n < 0 || (n > 0 && (!w || !ex))) return I3G_EINVAL;
+	if (n > m->mw_cap) {
+		if (m->mw) hipFree(m->mw);
+		m->mw = nullptr;
+		m->mw_cap = m->mw_n = 0;
+		int err = 0;
+		m->mw = mk_alloc<float>(2 * n, &err);
+		if (err) return err;
+		m->mw_cap = n;
+	}
+	m->mw_n = n;
+	if (n) {
+		MCHK(hipMemcpyAsync(m->mw, w, n * sizeof(float), hipMemcpyDefault, m->s));
+		MCHK(hipMemcpyAsync(m->mw + n, ex, n * sizeof(float), hipMemcpyDefault, m->s));
+		MCHK(hipStreamSynchronize(m->s));
+	}
 	return I3G_OK;
 }
 

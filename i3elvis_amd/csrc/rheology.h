@@ -258,3 +258,144 @@ __device__ inline double phase_rokf_dev(int mm2, double mtk, double mpb, double 
 	}
 	return rokf;
 }
+
+/* ---- partially molten rocks (meltpart11, meltpart0) -------------------- */
+
+/* solidus depression by water (Katz et al. 2003, mark3mg.c:1950-1990):
+ * dT = 43 X^0.75, X = mwa / (0.01 + F (1 - 0.01)) up to the saturation
+ * 12 P^0.6 + P (P in GPa) */
+__device__ inline double melt_wet_dt(double mwa, double F, double pg)
+{
+#pragma clang fp contract(off)
+	double sat = 12.0 * pow(pg, 0.6) + 1.0 * pg;
+	double X = mwa / (0.01 + F * (1.0 - 0.01));
+	return 43.0 * pow(X > sat ? sat : X, 0.75);
+}
+
+/* rock family of the melting model (mark3mg.c:1889-1953): 0 granite
+ * (sediments, upper crust), 1 basalt, 2 gabbro, 3 peridotite, -1 none */
+__device__ inline int melt_family(int mm2)
+{
+	switch (mm2) {
+	case 3: case 4: case 5: case 15: case 17: case 23: case 24: case 25: case 35: case 37: return 0;
+	case 7: case 16: case 18: case 27: case 36: case 38: return 1;
+	case 6: case 8: case 26: case 28: return 2;
+	case 9: case 10: case 11: case 12: case 14: case 29: case 30: case 31: case 32: case 34: return 3;
+	default: return -1;
+	}
+}
+
+/* meltpart11 (mark3mg.c:1872-2070): melt fraction F and latent heat of
+ * rock type mm2 with water content mwa at mtk (K), mpb (bar); maxpmelt (Pa)
+ * stops dry peridotite melting.  Peridotite follows Katz et al. (2003),
+ * the water's solidus depression found by bisection over F (at most 100
+ * halvings, to 1e-3 K); F is finally linear between the solidus and the
+ * (effective) liquidus. */
+__device__ inline void meltpart11_dev(int mm2, double mtk, double mpb, double mwa, double maxpmelt, double &F,
+				      double &H)
+{
+#pragma clang fp contract(off)
+	double ykm = mpb * 3e-3, ts = 0, tl = 0, hl = 0;
+	F = 0;
+	H = 0;
+	int fam = ykm > 0 ? melt_family(mm2) : -1;
+	if (fam == 0) {
+		double a = ykm + 1.609;
+		ts = ykm < 36.0 ? 889.0 + 536.6 / a + 18.21 / a / a : 831.3 + 2.0 * ykm;
+		tl = 1262.0 + 3.0 * ykm;
+		hl = 300000.0;
+	} else if (fam == 1) {
+		double a = ykm + 10.63;
+		ts = ykm < 48.0 ? 972.6 - 2111.0 / a + 70033.0 / a / a : 935.4 + 0.1162 * ykm + 0.006937 * ykm * ykm;
+		tl = 1423.15 + 3.5 * ykm;
+		hl = 380000.0;
+	} else if (fam == 2) {
+		ts = 1327.15 + 3.02 * ykm;
+		tl = 1423.15 + 3.5 * ykm;
+		hl = 380000.0;
+	} else if (fam == 3 && (mwa > 0 || !(mpb * 1e+5 > maxpmelt))) {
+		double pg = mpb * 1e-4, dlo = 0, dhi = 0;
+		int steps = 1;
+		if (mwa > 0) {
+			dlo = dhi = melt_wet_dt(mwa, 0.0, pg);
+			double t0 = 273.15 + 1085.7 + 132.9 * pg - 5.1 * pg * pg - dlo;
+			if (mtk >= t0) {
+				dlo = melt_wet_dt(mwa, 1.0, pg);
+				steps = 100;
+			}
+		}
+		for (int it = 0; it < steps; it++) {
+			double dt = (dlo + dhi) / 2.0, f = 0;
+			ts = 273.15 + 1085.7 + 132.9 * pg - 5.1 * pg * pg - dt;
+			double tlz = 273.15 + 1475.0 + 80.0 * pg - 3.2 * pg * pg - dt;
+			tl = 273.15 + 1780.0 + 45.0 * pg - 2.0 * pg * pg - dt;
+			if (mtk >= ts) {
+				if (mtk >= tl) {
+					f = 1.0;
+				} else {
+					double fc = 0.15 / (0.5 + 0.08 * pg);
+					double tc = pow(fc, 1.0 / 1.5) * (tlz - ts) + ts;
+					if (mtk > tc) f = fc + (1.0 - fc) * pow((mtk - tc) / (tl - tc), 1.5);
+					else f = pow((mtk - ts) / (tlz - ts), 1.5);
+					tl = (mtk - ts) / f + ts;
+				}
+			}
+			if (mwa > 0) {
+				if (melt_wet_dt(mwa, f, pg) > dt) dlo = dt;
+				else dhi = dt;
+			}
+			if (fabs(dhi - dlo) < 0.001) break;
+		}
+		hl = 400000.0;
+	}
+	if (tl != 0) {
+		double f = (mtk - ts) / (tl - ts);
+		f = f < 0 ? 0 : (f > 1.0 ? 1.0 : f);
+		F = f;
+		H = hl * f;
+	}
+}
+
+/* meltpart0 (mark3mg.c:2152-2340) for a marker of melt type mm2 (> 20)
+ * whose solid is S (type mm2 - 20, 11 for 34): the expansion term bb,
+ * density ro, heat capacity cp and conductivity kt of the rock mixed by
+ * the melt fraction less the extracted melt mex; densimod 1 with the
+ * solid's eclogite (27-28, 36-38) or mantle (29-34) transition factor;
+ * inside the melting interval the latent heat's central differences in P
+ * (300 bar) and T (10 K) are added to bb (within +-1e-2) and cp
+ * (100..5e4).  xm is the melt fraction of the last evaluation (the
+ * reference's eps1[21]).  mm2 = 21: every output zero. */
+__device__ inline void meltpart0_dev(const i3g_rock &A, const i3g_rock &S, int mm2, double mtk, double mpb, double mwa,
+				     double mex, int densimod, double teclmin, double teclmax, double maxpmelt,
+				     double &bb, double &ro, double &cp, double &kt, double &xm)
+{
+#pragma clang fp contract(off)
+	bb = ro = cp = kt = xm = 0;
+	if (mm2 <= 21) return;
+	double H;
+	meltpart11_dev(mm2, mtk, mpb, mwa, maxpmelt, xm, H);
+	double f = xm - mex;
+	if (f < 0) f = 0;
+	bb = (A.bro * f + S.bro * (1.0 - f)) / (1.0 - (A.bro * f + S.bro * (1.0 - f)) * (mtk - 298.15));
+	if (densimod == 0) {
+		ro = A.ro * f + S.ro * (1.0 - f);
+	} else {
+		int fam = (mm2 >= 27 && mm2 <= 28) || (mm2 >= 36 && mm2 <= 38) ? 7 : (mm2 >= 29 && mm2 <= 34 ? 9 : 0);
+		double rokf = fam ? phase_rokf_dev(fam, mtk, mpb, teclmin, teclmax) : 1.0;
+		ro = f * A.ro * (1.0 - A.bro * (mtk - 298.15)) * (1.0 + A.aro * (mpb - 1.0) * 1e-3) +
+		     (1.0 - f) * rokf * S.ro * (1.0 - S.bro * (mtk - 298.15)) * (1.0 + S.aro * (mpb - 1.0) * 1e-3);
+	}
+	cp = A.cp * f + S.cp * (1.0 - f);
+	kt = ((A.kt + A.kf / (mtk + 77.0)) * exp(A.kp * mpb)) * f + ((S.kt + S.kf / (mtk + 77.0)) * exp(S.kp * mpb)) * (1.0 - f);
+	if (f > 0 && f < 1.0) {
+		double Hm, Hp, x;
+		meltpart11_dev(mm2, mtk, mpb - 300.0, mwa, maxpmelt, x, Hm);
+		meltpart11_dev(mm2, mtk, mpb + 300.0, mwa, maxpmelt, x, Hp);
+		bb += (Hm - Hp) * (ro / (mtk * 2.0 * 300.0 * 1e+5));
+		bb = bb < -1e-2 ? -1e-2 : (bb > 1e-2 ? 1e-2 : bb);
+		meltpart11_dev(mm2, mtk + 10.0, mpb, mwa, maxpmelt, x, Hp);
+		meltpart11_dev(mm2, mtk - 10.0, mpb, mwa, maxpmelt, xm, Hm);
+		cp += (Hp - Hm) / (2.0 * 10.0);
+		cp = cp < 1e+2 ? 1e+2 : (cp > 5e+4 ? 5e+4 : cp);
+	}
+}

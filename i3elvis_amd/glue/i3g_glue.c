@@ -24,9 +24,10 @@
  *                  adiabyn/frictyn heating terms, then the marker
  *                  temperature update (heat3mg.c:592-782).
  *   i3g_ronurecalc ronurecalc (mark3mg.c:404-1015) with the whole viscalc,
- *                  retyping, water/air mixing and phase densities, with
- *                  tkrecalc; melt types (21-49) and densimod >= 2 stop the
- *                  program with a message.
+ *                  retyping, water/air mixing, phase densities and the
+ *                  partially molten rocks of the melt types (21-49, from
+ *                  markw and markex), with tkrecalc; densimod >= 2 stops
+ *                  the program with a message.
  *   i3g_movemark   movemark (mark3mg.c:2-397): topography of the markers,
  *                  viscosity, strain and grain state at the old positions
  *                  (epscalc1 + viscalc), the motion (outgrid, markmod,
@@ -119,6 +120,8 @@ static void glue_markers(void)
 	 * dislocation share */
 	if (marknum && ((rc = i3g_markers_set_strain(G.mk, marke)) || (rc = i3g_markers_set_grain(G.mk, markrr, markrn))))
 		glue_fail("markers_set_state", rc);
+	/* water content and extracted melt, by id, for the melt types */
+	if ((rc = i3g_markers_set_melt(G.mk, marknum, markw, markex))) glue_fail("markers_set_melt", rc);
 }
 
 /* device markers back into the program's arrays (by id): positions,
@@ -176,6 +179,7 @@ static void glue_mk_params(i3g_mk_params *mp, int sedtype)
 	mp->teclmin = teclmin; mp->teclmax = teclmax;
 	mp->timesum = timesum; mp->convend = convend;
 	mp->outgrid = outgrid;
+	mp->maxpmelt = maxpmelt;
 }
 
 void i3g_vpiterate(int m0, int f0)

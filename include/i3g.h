@@ -483,9 +483,12 @@ typedef struct {
  * proportions of markers at the water level (mnumy markers per cell in y),
  * viscalc from the strain rates and stresses of i3g_markers_set_stress,
  * and the phase-transition densities damped between teclmin and teclmax;
- * melt types (21-49) are refused.  timesum and convend (s) drive the grain
- * growth and the dislocation share.  outgrid: movemark's, for the markers
- * the property update covers (as i3g_move_params.outgrid). */
+ * melt types (21-49) take the properties of the partially molten rock
+ * (meltpart0/meltpart11, mark3mg.c:660-686, 1872-2340) from the markers'
+ * water content and extracted melt (i3g_markers_set_melt), dry mantle
+ * melting stopping above maxpmelt (Pa).  timesum and convend (s) drive the
+ * grain growth and the dislocation share.  outgrid: movemark's, for the
+ * markers the property update covers (as i3g_move_params.outgrid). */
 typedef struct {
 	double nubeg, nuend, nukoef;
 	int densimod;
@@ -499,6 +502,7 @@ typedef struct {
 	double teclmin, teclmax;
 	double timesum, convend;
 	int outgrid;
+	double maxpmelt;
 } i3g_mk_params;
 
 /* movemark's motion parameters (mark3mg.c:61, 146-284): outgrid (0 markers
@@ -641,6 +645,13 @@ int i3g_markers_advect_ex(i3g_markers *m, double dt, const float *vx, const floa
  * updates them. */
 int i3g_markers_set_grain(i3g_markers *m, const float *rr, const float *rn);
 int i3g_markers_get_grain(i3g_markers *m, float *rr, float *rn);
+
+/* Water content (markw) and extracted melt fraction (markex) of markers
+ * 0..n-1 by marker id (the id given at upload): read by the melt types'
+ * properties in the node transfer (meltpart0, mark3mg.c:2152-2340, reads
+ * markw[mm1] and markex[mm1]); ids without an entry read 0.  n = 0 clears
+ * them.  Host or device pointers. */
+int i3g_markers_set_melt(i3g_markers *m, long n, const float *w, const float *ex);
 
 /* Strain rates and stresses (12 float node planes exx eyy ezz exy exz eyz
  * sxx syy szz sxy sxz syz, host or device) that the transfer's viscalc

@@ -395,7 +395,7 @@ class VisPrm(C.Structure):
 class MkX(C.Structure):
     _fields_ = [("nukoef", C.c_double), ("densimod", C.c_int), ("vdeep", C.c_double), ("nudeep", C.c_double),
                 ("eroslev", C.c_double), ("sedilev", C.c_double), ("waterlev", C.c_double), ("sedtype", C.c_int),
-                ("mnumy", C.c_long), ("teclmin", C.c_double), ("teclmax", C.c_double)]
+                ("mnumy", C.c_long), ("teclmin", C.c_double), ("teclmax", C.c_double), ("maxpmelt", C.c_double)]
 
 
 class MoveX(C.Structure):
@@ -450,10 +450,11 @@ def mk_props_x(x, y, z, mtype, tk, eta, e, rr, rn, rocks, st12, pr, gx, gy, gz, 
 
 def ronurecalc_x(x, y, z, tk, eta, mtype, e, rr, rn, gx, gy, gz, pr, st12, rocks, fld, nubeg, nuend, timestep,
                  timesum, convend, nukoef=0.0, densimod=1, vdeep=1e30, nudeep=1e30, eroslev=-1e30, sedilev=1e30,
-                 waterlev=-1.0, sedtype=3, mnumy=2, teclmin=0.0, teclmax=0.0):
-    """ronurecalc with the full rheology; returns (fld, eta, type, e, rr,
-    rn) copies; raises on a marker it does not restate (melt types,
-    densimod >= 2)"""
+                 waterlev=-1.0, sedtype=3, mnumy=2, teclmin=0.0, teclmax=0.0, maxpmelt=1e10, markw=None,
+                 markex=None):
+    """ronurecalc with the full rheology, melt types (21-49) from their
+    water content markw and extracted melt markex (None: zero); returns
+    (fld, eta, type, e, rr, rn) copies; raises on densimod >= 2"""
     fld = np.array(fld, np.float32, copy=True)
     eta, e, rr, rn = (np.array(a, np.float32, copy=True) for a in (eta, e, rr, rn))
     t = np.array(mtype, np.int8, copy=True)
@@ -468,12 +469,15 @@ def ronurecalc_x(x, y, z, tk, eta, mtype, e, rr, rn, gx, gy, gz, pr, st12, rocks
     ra = rocksx_array(rocks)
     P = _visprm(nubeg, nuend, timestep, timesum, convend)
     M = MkX(float(nukoef), int(densimod), float(vdeep), float(nudeep), float(eroslev), float(sedilev),
-            float(waterlev), int(sedtype), int(mnumy), float(teclmin), float(teclmax))
+            float(waterlev), int(sedtype), int(mnumy), float(teclmin), float(teclmax), float(maxpmelt))
+    mw = None if markw is None else np.ascontiguousarray(markw, np.float32)
+    mx = None if markex is None else np.ascontiguousarray(markex, np.float32)
     rc = lib().or_ronurecalc_x(C.c_long(len(x)), *[_p(a) for a in f], _p(eta), _p(t), _p(e), _p(rr), _p(rn),
                                _p(gx), C.c_long(len(gx)), _p(gy), C.c_long(len(gy)), _p(gz), C.c_long(len(gz)),
-                               _p(prd), sp, ra, C.byref(P), C.byref(M), _p(fld))
+                               _p(prd), sp, ra, C.byref(P), C.byref(M), _p(mw) if mw is not None else None,
+                               _p(mx) if mx is not None else None, _p(fld))
     if rc:
-        raise ValueError("a participating marker needs the melt branch or densimod >= 2")
+        raise ValueError("a participating marker needs densimod >= 2")
     return fld, eta, t, e, rr, rn
 
 
@@ -493,3 +497,21 @@ def rk4_x(x, y, z, tk, eta, mtype, md, vx, vy, vz, pr, gx, gy, gz, dt, depth=(1e
                    _p(gy), C.c_long(len(gy)), _p(gz), C.c_long(len(gz)), C.c_double(dt), C.c_double(depth[0]),
                    C.c_double(depth[1]), C.c_double(depth[2]), C.byref(M))
     return x, y, z, tk, eta, t
+
+
+def meltpart11(mm2, mtk, mpb, mwa=0.0, maxpmelt=1e10):
+    """melt fraction and latent heat (J/kg) of rock type mm2 at mtk (K),
+    mpb (bar) with water content mwa"""
+    xm, hl = C.c_double(), C.c_double()
+    lib().or_meltpart11(C.c_int(mm2), C.c_double(mtk), C.c_double(mpb), C.c_double(mwa), C.c_double(maxpmelt),
+                        C.byref(xm), C.byref(hl))
+    return xm.value, hl.value
+
+
+def meltpart0(rocks, mm2, mtk, mpb, mwa=0.0, mex=0.0, densimod=1, teclmin=0.0, teclmax=0.0, maxpmelt=1e10):
+    """eps1[20..26] of a marker of melt type mm2 and the return flag"""
+    e = (C.c_double * 7)()
+    rc = lib().or_meltpart0(rocksx_array(rocks), C.c_int(mm2), C.c_double(mtk), C.c_double(mpb), C.c_double(mwa),
+                            C.c_double(mex), C.c_int(densimod), C.c_double(teclmin), C.c_double(teclmax),
+                            C.c_double(maxpmelt), e)
+    return list(e), rc

@@ -1658,6 +1658,158 @@ static double or_phase_rokf(int mm2, double mtk, double mpb, double teclmin, dou
 	return rokf;
 }
 
+/* water dissolved in the melt and the solidus depression it gives (Katz
+ * et al. 2003 as mark3mg.c:1950-1990 uses it): xh2o = mwa / (0.01 + f
+ * (1 - 0.01)) capped by the saturation 12 P^0.6 + P (P in GPa);
+ * dT = 43 xh2o^0.75 */
+static double or_wet_dt(double mwa, double f, double mpg)
+{
+	double sat = 12.0 * pow(mpg, 0.6) + 1.0 * mpg;
+	double xh2o = mwa / (0.01 + f * (1.0 - 0.01));
+	if (xh2o > sat) xh2o = sat;
+	return 43.0 * pow(xh2o, 0.75);
+}
+
+/* meltpart11 (mark3mg.c:1872-2070).  Rock families by type: sediments and
+ * upper crust (wet granite solidus, dry granite liquidus, 300 kJ/kg),
+ * basalt (wet solidus of Schmidt & Poli, tholeiite liquidus, 380 kJ/kg),
+ * gabbro (dry tholeiite solidus and liquidus, 380 kJ/kg), peridotite
+ * (Katz et al. 2003 with cpx exhaustion; the water solidus depression by
+ * bisection on dT over the melt fraction, at most 100 steps; 400 kJ/kg;
+ * dry peridotite does not melt above maxpmelt).  The melt fraction is then
+ * linear between the family's solidus and (effective) liquidus. */
+void or_meltpart11(int mm2, double mtk, double mpb, double mwa, double maxpmelt, double *xmelt, double *hlat)
+{
+	double ykm = mpb * 3e-3, ts = 0, tl = 0, tll = 0, hl = 0;
+	*xmelt = 0;
+	*hlat = 0;
+	if (ykm > 0) {
+		int fam = -1;
+		switch (mm2) {
+		case 3: case 4: case 5: case 15: case 17: case 23: case 24: case 25: case 35: case 37: fam = 0; break;
+		case 7: case 16: case 18: case 27: case 36: case 38: fam = 1; break;
+		case 6: case 8: case 26: case 28: fam = 2; break;
+		case 9: case 10: case 11: case 12: case 14: case 29: case 30: case 31: case 32: case 34: fam = 3; break;
+		default: break;
+		}
+		if (fam == 0) {
+			ts = ykm < 36.0 ? 889.0 + 536.6 / (ykm + 1.609) + 18.21 / (ykm + 1.609) / (ykm + 1.609)
+					: 831.3 + 2.0 * ykm;
+			tl = 1262.0 + 3.0 * ykm;
+			hl = 300000.0;
+		} else if (fam == 1) {
+			ts = ykm < 48.0 ? 972.6 - 2111.0 / (ykm + 10.63) + 70033.0 / (ykm + 10.63) / (ykm + 10.63)
+					: 935.4 + 0.1162 * ykm + 0.006937 * ykm * ykm;
+			tl = 1423.15 + 3.5 * ykm;
+			hl = 380000.0;
+		} else if (fam == 2) {
+			ts = 1327.15 + 3.02 * ykm;
+			tl = 1423.15 + 3.5 * ykm;
+			hl = 380000.0;
+		} else if (fam == 3 && !(mwa <= 0 && mpb * 1e+5 > maxpmelt)) {
+			double mpg = mpb * 1e-4, lo = 0, hi = 0, dt;
+			int steps = 1, it;
+			if (mwa > 0) {
+				dt = or_wet_dt(mwa, 0.0, mpg);
+				ts = 273.15 + 1085.7 + 132.9 * mpg - 5.1 * mpg * mpg - dt;
+				lo = hi = dt;
+				if (mtk >= ts) {
+					lo = or_wet_dt(mwa, 1.0, mpg);
+					steps = 100;
+				}
+			}
+			for (it = 0; it < steps; it++) {
+				double f = 0;
+				dt = (lo + hi) / 2.0;
+				ts = 273.15 + 1085.7 + 132.9 * mpg - 5.1 * mpg * mpg - dt;
+				tll = 273.15 + 1475.0 + 80.0 * mpg - 3.2 * mpg * mpg - dt;
+				tl = 273.15 + 1780.0 + 45.0 * mpg - 2.0 * mpg * mpg - dt;
+				if (mtk >= ts) {
+					if (mtk >= tl) {
+						f = 1.0;
+					} else {
+						double fcpx = 0.15 / (0.5 + 0.08 * mpg);
+						double tcpx = pow(fcpx, 1.0 / 1.5) * (tll - ts) + ts;
+						f = mtk > tcpx ? fcpx + (1.0 - fcpx) * pow((mtk - tcpx) / (tl - tcpx), 1.5)
+							       : pow((mtk - ts) / (tll - ts), 1.5);
+						tl = (mtk - ts) / f + ts;
+					}
+				}
+				if (mwa > 0) {
+					if (or_wet_dt(mwa, f, mpg) > dt) lo = dt;
+					else hi = dt;
+				}
+				if (fabs(hi - lo) < 0.001) break;
+			}
+			hl = 400000.0;
+		}
+	}
+	if (tl != 0) {
+		double f = (mtk - ts) / (tl - ts);
+		if (f < 0) f = 0;
+		if (f > 1.0) f = 1.0;
+		*xmelt = f;
+		*hlat = hl * f;
+	}
+}
+
+/* meltpart0 (mark3mg.c:2152-2340).  e[0..6] = eps1[20..26]: thermal
+ * expansion term, melt fraction, latent heat, density, (unused), heat
+ * capacity, conductivity of the partially molten rock: melt (type mm2) and
+ * solid (mm2 - 20, type 11 for 34) mixed by the melt fraction less the
+ * extracted melt; densimod 1 adds the solid's eclogite (27-28, 36-38) and
+ * olivine-spinel-perovskite (29-34) factors; inside the melting interval
+ * the latent heat's pressure and temperature derivatives (central
+ * differences over 300 bar and 10 K) are added to the expansion term
+ * (capped at 1e-2) and the heat capacity (100..5e4).  e[1] and e[2] are
+ * those of the last melt-fraction evaluation, as in eps1. */
+int or_meltpart0(const or_rockx_t *rocks, int mm2, double mtk, double mpb, double mwa, double mex, int densimod,
+		 double teclmin, double teclmax, double maxpmelt, double *e)
+{
+	const or_rockx_t *A, *B;
+	double f, xm, hl, rokf = 1.0, a, b;
+	int q, solid;
+	for (q = 0; q < 7; q++) e[q] = 0;
+	if (mm2 <= 21) return 0;
+	or_meltpart11(mm2, mtk, mpb, mwa, maxpmelt, &xm, &hl);
+	e[1] = xm;
+	e[2] = hl;
+	f = xm - mex;
+	if (f < 0) f = 0;
+	solid = mm2 == 34 ? 11 : mm2 - 20;
+	A = &rocks[mm2];
+	B = &rocks[solid];
+	e[0] = (A->bro * f + B->bro * (1.0 - f)) / (1.0 - (A->bro * f + B->bro * (1.0 - f)) * (mtk - 298.15));
+	if (densimod == 0) {
+		e[3] = A->ro * f + B->ro * (1.0 - f);
+	} else {
+		if ((mm2 >= 27 && mm2 <= 28) || (mm2 >= 36 && mm2 <= 38)) rokf = or_phase_rokf(7, mtk, mpb, teclmin, teclmax);
+		if (mm2 >= 29 && mm2 <= 34) rokf = or_phase_rokf(9, mtk, mpb, teclmin, teclmax);
+		e[3] = f * A->ro * (1.0 - A->bro * (mtk - 298.15)) * (1.0 + A->aro * (mpb - 1.0) * 1e-3) +
+		       (1.0 - f) * rokf * B->ro * (1.0 - B->bro * (mtk - 298.15)) * (1.0 + B->aro * (mpb - 1.0) * 1e-3);
+	}
+	e[5] = A->cp * f + B->cp * (1.0 - f);
+	e[6] = ((A->kt + A->kf / (mtk + 77.0)) * exp(A->kp * mpb)) * f +
+	       ((B->kt + B->kf / (mtk + 77.0)) * exp(B->kp * mpb)) * (1.0 - f);
+	if (f > 0 && f < 1.0) {
+		double dp = 300.0, dT = 10.0, d;
+		or_meltpart11(mm2, mtk, mpb - dp, mwa, maxpmelt, &xm, &a);
+		or_meltpart11(mm2, mtk, mpb + dp, mwa, maxpmelt, &xm, &b);
+		d = (a - b) * (e[3] / (mtk * 2.0 * dp * 1e+5));
+		e[0] += d;
+		if (e[0] < -1e-2) e[0] = -1e-2;
+		else if (e[0] > 1e-2) e[0] = 1e-2;
+		or_meltpart11(mm2, mtk + dT, mpb, mwa, maxpmelt, &xm, &a);
+		or_meltpart11(mm2, mtk - dT, mpb, mwa, maxpmelt, &xm, &b);
+		e[1] = xm;
+		e[2] = b;
+		e[5] += (a - b) / (2.0 * dT);
+		if (e[5] < 1e+2) e[5] = 1e+2;
+		else if (e[5] > 5e+4) e[5] = 5e+4;
+	}
+	return 1;
+}
+
 /* ronurecalc (mark3mg.c:404-1015) with the full rheology: erosion,
  * sedimentation and water/air retyping (:465-472), viscalc from epscalc1
  * strain rates when the marker has no viscosity or timestep <= 0
@@ -1665,13 +1817,16 @@ static double or_phase_rokf(int mm2, double mtk, double mpb, double teclmin, dou
  * (:521-641), the water/air proportions of markers at the water level
  * (:723-763; the y-cell of the marker sets the marker spacing, where the
  * reference reads a shared index array), the viscosity caps, then the
- * scatter of or_ronurecalc.  Melt types (> 20) and densimod >= 2 are not
- * restated: returns -1 if a participating marker needs them.  st12 may be
- * NULL (zero strain rates and stresses). */
+ * scatter of or_ronurecalc.  Melt types (> 20) take the partially molten
+ * rock's properties (or_meltpart0) from their water content mw and
+ * extracted melt mex (NULL: zero).  densimod >= 2 is not restated: returns
+ * -1 if a participating marker needs it.  st12 may be NULL (zero strain
+ * rates and stresses). */
 int or_ronurecalc_x(long n, const float *x, const float *y, const float *z, const float *tk, float *eta,
 		    signed char *type, float *e, float *rr, float *rn, const double *gx, long nx, const double *gy,
 		    long ny, const double *gz, long nz, const double *pr, const float *const *st12,
-		    const or_rockx_t *rocks, const or_visprm_t *P, const or_mkx_t *M, float *fld)
+		    const or_rockx_t *rocks, const or_visprm_t *P, const or_mkx_t *M, const float *mw,
+		    const float *mex, float *fld)
 {
 	long N = nx * ny * nz, i, q;
 	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
@@ -1688,7 +1843,7 @@ int or_ronurecalc_x(long n, const float *x, const float *y, const float *z, cons
 		if (mm2 < 2 && y[i] > M->sedilev) mm2 = type[i] = (signed char)M->sedtype;
 		if (mm2 < 2 && y[i] > M->waterlev) mm2 = type[i] = 1;
 		if (mm2 < 2 && y[i] < M->waterlev) mm2 = type[i] = 0;
-		if (mm2 > 20 || M->densimod >= 2) { rc = -1; break; }
+		if (M->densimod >= 2) { rc = -1; break; }
 		mpb = or_prcalc(pr, gx, nx, gy, ny, gz, nz, x[i], y[i], z[i], cell) * 1e-5;
 		mtk = tk[i];
 		mnu = eta[i];
@@ -1704,6 +1859,28 @@ int or_ronurecalc_x(long n, const float *x, const float *y, const float *z, cons
 		mcp = rocks[mm2].cp;
 		mkt = (rocks[mm2].kt + rocks[mm2].kf / (mtk + 77.0)) * exp(rocks[mm2].kp * mpb);
 		mht = rocks[mm2].ht;
+		if (mm2 > 20) {
+			/* partially molten rock (mark3mg.c:660-686) */
+			double e7[7];
+			or_meltpart0(rocks, mm2, mtk, mpb, mw ? (double)mw[i] : 0.0, mex ? (double)mex[i] : 0.0,
+				     M->densimod, M->teclmin, M->teclmax, M->maxpmelt, e7);
+			mro = e7[3];
+			mbb = e7[0];
+			mcp = e7[5];
+			mkt = e7[6];
+			if ((mm2 == 27 || mm2 == 28) && e7[1] > 0.001) {
+				/* basaltic melt (Schott and Schmeling 1995) */
+				double nm = 1e+15 * exp(2.5 + pow((1.0 - e7[1]) / e7[1], 0.48) * (1.0 - e7[1]));
+				if (nm < mnu) {
+					mnu = nm;
+					if (mnu < rocks[mm2].n0) mnu = rocks[mm2].n0;
+					if (mnu > rocks[mm2].n1) mnu = rocks[mm2].n1;
+					if (mnu < P->nubeg) mnu = P->nubeg;
+					if (mnu > P->nuend) mnu = P->nuend;
+					eta[i] = (float)mnu;
+				}
+			}
+		}
 		if (M->waterlev >= 0 && mm2 < 2) {
 			double dywa = (gy[cell[1] + 1] - gy[cell[1]]) / 2.0 / ((double)M->mnumy);
 			double lev = ((double)y[i] - (M->waterlev - dywa)) / (2.0 * dywa);

@@ -160,6 +160,7 @@ typedef struct {
 	int sedtype; /* the sediment type of this call (3 or 4, from sedimnum/sedimcyc) */
 	long mnumy;  /* markers per cell along y */
 	double teclmin, teclmax;
+	double maxpmelt; /* Pa: no dry mantle melting above it */
 } or_mkx_t;
 
 /* movemark's globals of the motion */
@@ -178,7 +179,16 @@ void or_mk_props_x(long n, const float *x, const float *y, const float *z, const
 int or_ronurecalc_x(long n, const float *x, const float *y, const float *z, const float *tk, float *eta,
 		    signed char *type, float *e, float *rr, float *rn, const double *gx, long nx, const double *gy,
 		    long ny, const double *gz, long nz, const double *pr, const float *const *st12,
-		    const or_rockx_t *rocks, const or_visprm_t *P, const or_mkx_t *M, float *fld);
+		    const or_rockx_t *rocks, const or_visprm_t *P, const or_mkx_t *M, const float *mw,
+		    const float *mex, float *fld);
+/* meltpart11 (mark3mg.c:1872-2070): melt fraction and latent heat (J/kg)
+ * of rock type mm2 with water content mwa at mtk (K), mpb (bar) */
+void or_meltpart11(int mm2, double mtk, double mpb, double mwa, double maxpmelt, double *xmelt, double *hlat);
+/* meltpart0 (mark3mg.c:2152-2340): eps1[20..26] of a marker of melt type
+ * mm2 (> 20) with water content mwa and extracted melt mex; returns 1 for
+ * mm2 > 21, 0 (and zeros) otherwise */
+int or_meltpart0(const or_rockx_t *rocks, int mm2, double mtk, double mpb, double mwa, double mex, int densimod,
+		 double teclmin, double teclmax, double maxpmelt, double *e20_26);
 void or_rk4_x(long n, float *x, float *y, float *z, float *tk, float *eta, signed char *type, const float *md,
 	      const float *vx, const float *vy, const float *vz, const double *pr, const double *gx, long nx,
 	      const double *gy, long ny, const double *gz, long nz, double dt, double zdeep, double vdeep,

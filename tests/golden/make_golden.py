@@ -1081,6 +1081,87 @@ def g15(R):
     print("g15_ronurecalc_rheology.npz written", "sedtype", sedtype)
 
 
+def g17(R):
+    """ronurecalc (mark3mg.c:404-1015) on partially molten rocks: the deck's
+    melt types (23-30, 32, 34, 36-38) and their solid parents at
+    temperatures and pressures across every family's solidus and liquidus,
+    dry and wet (markw) markers, extracted melt (markex), the dry mantle
+    pressure limit maxpmelt; densimod 0 and 1."""
+    import ctypes as C
+    rng = np.random.default_rng(1717)
+    rocks = deck_rocks()
+    nx, ny, nz = 9, 11, 8
+    gx = np.cumsum(np.r_[0, rng.uniform(0.7, 1.3, nx - 1)]) * 5e3
+    gy = np.arange(ny) * 5e3
+    gz = np.cumsum(np.r_[0, rng.uniform(0.7, 1.3, nz - 1)]) * 5e3
+    N = nx * ny * nz
+    h = I.hierarchy(nx, ny, nz, 0)
+    rules, nr = I.freeslip_rules(273.0, 1600.0)
+    bondm, bondv, bondn = I.expand_bc(rules, nr, h)
+    R.setup_grid(gx, gy, gz, 0)
+    R.set_bc(bondm, bondv, bondn)
+    n = 3000
+    x = rng.uniform(1.0, gx[-1] - 1.0, n).astype(np.float32)
+    y = rng.uniform(1.0, gy[-1] - 1.0, n).astype(np.float32)
+    z = rng.uniform(1.0, gz[-1] - 1.0, n).astype(np.float32)
+    melt = np.array([23, 24, 25, 26, 27, 28, 29, 30, 32, 34, 36, 37, 38])
+    solid = np.array([2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14])
+    mtype = np.where(rng.uniform(size=n) < 0.8, melt[rng.integers(0, len(melt), n)],
+                     solid[rng.integers(0, len(solid), n)]).astype(np.int8)
+    tk = rng.uniform(800, 2200, n).astype(np.float32)
+    eta = np.where(rng.uniform(size=n) < 0.5, 0.0, 10 ** rng.uniform(17, 23, n)).astype(np.float32)
+    marke = np.where(rng.uniform(size=n) < 0.5, 0.0, rng.uniform(0, 1.5, n)).astype(np.float32)
+    markrr = np.where(rng.uniform(size=n) < 0.3, 0.0, 10 ** rng.uniform(-5.5, -2.5, n)).astype(np.float32)
+    markrn = rng.uniform(0, 1, n).astype(np.float32)
+    markw = np.where(rng.uniform(size=n) < 0.5, 0.0, rng.uniform(0.01, 3.0, n)).astype(np.float32)
+    markex = np.where(rng.uniform(size=n) < 0.6, 0.0, rng.uniform(0, 0.4, n)).astype(np.float32)
+    names = ("exx", "eyy", "ezz", "exy", "exz", "eyz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    st = {k: (rng.standard_normal(N) * (10 ** rng.uniform(-15.5, -13, N) if k[0] == "e" else 10 ** rng.uniform(6, 8, N))
+              ).astype(np.float32) for k in names}
+    # 0.1-60 kbar: melting intervals of every family, the dry mantle limit
+    # (maxpmelt 3 GPa) and the eclogite/spinel transitions of densimod 1
+    pr = 10 ** rng.uniform(7, np.log10(6e9), N)
+    maxpmelt = 3e9
+    out_all = {}
+    for densimod in (0, 1):
+        for k, a in st.items():
+            R.f32(k)[:N] = a
+        R.f64("pr")[:N] = pr
+        R.set("marknum", n, C.c_long)
+        for gname, a in (("markx", x), ("marky", y), ("markz", z), ("markk", tk), ("markv", eta),
+                         ("marke", marke), ("markrr", markrr), ("markrn", markrn), ("markw", markw),
+                         ("markex", markex)):
+            R.arr(gname, C.c_float, R.MAXMRK)[:n] = a
+        R.arr("markt", C.c_byte, R.MAXMRK)[:n] = mtype
+        for gname in ("markd", "markft", "markto"):
+            R.arr(gname, C.c_float, R.MAXMRK)[:n] = 0
+        _set_ref_rocks(R, rocks)
+        R.set("densimod", densimod, C.c_int)
+        R.set("viscmod", 0, C.c_int)
+        R.set("sedimnum", 4, C.c_int)
+        R.set("sedimcyc", 3, C.c_int)
+        R.set("mnumy", 2, C.c_long)
+        prm = dict(eroslev=-1e30, sedilev=1e30, waterlev=-1.0, vdeep=1e30, nudeep=1e30, nukoef=0.0,
+                   nubeg=1e17, nuend=1e25, timestep=0.0, timesum=5e13, convend=1e13, teclmin=700.0, teclmax=1100.0,
+                   maxpmelt=maxpmelt)
+        for k, v in prm.items():
+            R.set(k, v)
+        init = np.zeros((11, N), np.float32)
+        gnames = ["nu", "ro", "cp", "kt", "ht", "et", "tk", "nxx", "nxy", "nxz", "nyz"]
+        for q, gname in enumerate(gnames):
+            R.f32(gname)[:N] = init[q]
+        R.lib.refh_ronurecalc()
+        out_all[f"out{densimod}"] = np.stack([R.f32(gname)[:N].copy() for gname in gnames])
+        for k, g in (("eta", "markv"), ("e", "marke"), ("rr", "markrr"), ("rn", "markrn")):
+            out_all[f"{k}_out{densimod}"] = R.arr(g, C.c_float, R.MAXMRK)[:n].copy()
+        out_all[f"type_out{densimod}"] = R.arr("markt", C.c_byte, R.MAXMRK)[:n].copy()
+    np.savez_compressed(os.path.join(HERE, "g17_ronurecalc_melt.npz"), gx=gx, gy=gy, gz=gz, bondm=bondm,
+                        bondv=bondv, bondn=bondn, x=x, y=y, z=z, mtype=mtype, tk=tk, eta=eta, marke=marke,
+                        markrr=markrr, markrn=markrn, markw=markw, markex=markex, pr=pr, **st,
+                        rocks=_rock_matrix(rocks), sedtype=3, mnumy=2, **prm, **out_all)
+    print("g17_ronurecalc_melt.npz written")
+
+
 G16_ROCKS = {
     # type: n0 n1 s0 s1 nu dh dv ss mm ll a0 a1 b0 b1 e0 e1 ro bb aa cp kt kf kp ht (own values)
     0: "1e18 1e18 0 5e29 1e18 0 0 0 1 0 0 0 0 0 0 0.5 1 0 0 3.3e6 200 0 0 0",
@@ -1235,7 +1316,7 @@ if __name__ == "__main__":
     R = Ref()
     R.lib.refh_set_threads(1)
     which = sys.argv[1:] or ["g1", "g3", "g5", "g6", "g7", "g8", "g9", "g10", "g11", "g12", "g3c2", "g13", "g14",
-                             "g15", "g16"]
+                             "g15", "g16", "g17"]
     if "g1" in which:
         g1(R)
     if "g3" in which:
@@ -1268,3 +1349,5 @@ if __name__ == "__main__":
         g15(R)
     if "g16" in which:
         g16(R)
+    if "g17" in which:
+        g17(R)

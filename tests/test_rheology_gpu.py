@@ -2,7 +2,8 @@
 the C ABI, against the reference's own movemark and ronurecalc run with the
 rock table of its deck (goldens g14, g15: power law, brittle strength,
 Karato & Wu, the grain-size model; outgrid 1/2, markmod 0/1, fluid and
-immobile markers; retyping, water/air mixing, phase densities).  On these
+immobile markers; retyping, water/air mixing, phase densities; g17: the
+partially molten rocks of the melt types).  On these
 fixtures every output is equal to the reference's to the bit; the bounds
 allow for the device's own exp/pow/asin on other inputs: marker viscosities,
 grain roughness and dislocation shares within 2e-6 relative (the grain
@@ -103,18 +104,55 @@ def test_ronurecalc_rheology_matches_reference(gpu):
     M.close()
 
 
-def test_transfer_refuses_melt_types(gpu):
-    """rheology 1 refuses the melt branch (types 21-49) and densimod 2"""
+@pytest.mark.parametrize("densimod", [0, 1])
+def test_ronurecalc_melt_matches_reference(gpu, densimod):
+    """partially molten rocks (melt types with water content and extracted
+    melt, mark3mg.c:660-686): node fields, marker viscosities and types
+    against the reference's ronurecalc (golden g17)"""
+    g = golden("g17_ronurecalc_melt.npz")
+    n = len(g["x"])
+    print(f"case: ronurecalc melt g17 densimod {densimod}, {n} markers", flush=True)
+    M = I.Markers(g["gx"], g["gy"], g["gz"], n + 16)
+    M.upload(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"])
+    M.set_strain(g["marke"])
+    M.set_grain(g["markrr"], g["markrn"])
+    M.set_stress([g[k] for k in ST])
+    M.set_melt(g["markw"], g["markex"])
+    M.sort()
+    M.set_tbc(g["bondm"], g["bondv"], g["bondn"])
+    rocks = [list(r) for r in g["rocks"]]
+    fld = M.to_nodes(rocks, g["pr"], np.zeros((11, len(g["pr"])), np.float32), float(g["nubeg"]),
+                     float(g["nuend"]), nukoef=float(g["nukoef"]), densimod=densimod, timestep=float(g["timestep"]),
+                     vdeep=float(g["vdeep"]), nudeep=float(g["nudeep"]), rheology=1, eroslev=float(g["eroslev"]),
+                     sedilev=float(g["sedilev"]), waterlev=float(g["waterlev"]), sedtype=int(g["sedtype"]),
+                     mnumy=int(g["mnumy"]), teclmin=float(g["teclmin"]), teclmax=float(g["teclmax"]),
+                     timesum=float(g["timesum"]), convend=float(g["convend"]), maxpmelt=float(g["maxpmelt"]))
+    d = M.download()
+    order = np.argsort(d["id"])
+    assert np.array_equal(d["type"][order], g[f"type_out{densimod}"])
+    rel = _rel(d["eta"][order], g[f"eta_out{densimod}"])
+    print(f"  eta: max rel {rel.max():.2e}, exact {int((d['eta'][order] == g[f'eta_out{densimod}']).sum())} of {n}",
+          flush=True)
+    assert rel.max() <= 2e-6
+    for q in range(11):
+        ref = g[f"out{densimod}"][q]
+        rel = _rel(fld[q], ref)
+        print(f"  field {q}: max rel {rel.max():.2e}, exact {int((fld[q] == ref).sum())} of {ref.size}", flush=True)
+        assert rel.max() <= 2e-6, q
+    M.close()
+
+
+def test_transfer_refuses_densimod_2(gpu):
+    """the thermodynamic tables (densimod >= 2) are not restated"""
     g = golden("g15_ronurecalc_rheology.npz")
     n = len(g["x"])
-    print("case: melt types refused", flush=True)
+    print("case: densimod 2 refused", flush=True)
     M = I.Markers(g["gx"], g["gy"], g["gz"], n + 16)
-    mt = np.full(n, 25, np.int8)
-    M.upload(g["x"], g["y"], g["z"], g["tk"], g["eta"], mt)
+    M.upload(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"])
     M.sort()
     rocks = [list(r) for r in g["rocks"]]
     with pytest.raises(I.I3GError):
-        M.to_nodes(rocks, g["pr"], g["init"], 1e18, 1e25, rheology=1, mnumy=2)
+        M.to_nodes(rocks, g["pr"], g["init"], 1e18, 1e25, densimod=2, rheology=1, mnumy=2)
     M.close()
 
 

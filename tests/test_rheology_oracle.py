@@ -1,7 +1,8 @@
 """The oracle's full-rheology restatement against the reference's own
-output (goldens g14, g15, written by the reference's movemark and
+output (goldens g14, g15, g17, written by the reference's movemark and
 ronurecalc with the rock table of its deck init.t3c), plus closed forms of
-viscalc's branches that a wrong exponent, sign or clamp would fail."""
+viscalc's branches and of the melting parameterisations that a wrong
+exponent, sign or clamp would fail."""
 import numpy as np
 import pytest
 
@@ -68,14 +69,112 @@ def test_ronurecalc_rheology_matches_reference():
     assert (g["type_out"] != g["mtype"]).any()
 
 
-def test_ronurecalc_rejects_melt_types():
+def test_ronurecalc_rejects_densimod_2():
     g = golden("g15_ronurecalc_rheology.npz")
-    mt = np.array(g["mtype"], np.int8, copy=True)
-    mt[:] = 25
     with pytest.raises(ValueError):
-        oracle.ronurecalc_x(g["x"], g["y"], g["z"], g["tk"], g["eta"], mt, g["marke"], g["markrr"], g["markrn"],
-                            g["gx"], g["gy"], g["gz"], g["pr"], None, _rocks(g) + [{}] * 10, g["init"], 1e18, 1e25, 0.0,
-                            0.0, 0.0)
+        oracle.ronurecalc_x(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"], g["marke"], g["markrr"],
+                            g["markrn"], g["gx"], g["gy"], g["gz"], g["pr"], None, _rocks(g), g["init"], 1e18, 1e25,
+                            0.0, 0.0, 0.0, densimod=2)
+
+
+def _g17(g, densimod):
+    st12 = [g[k] for k in ST]
+    out = oracle.ronurecalc_x(
+        g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"], g["marke"], g["markrr"], g["markrn"], g["gx"], g["gy"],
+        g["gz"], g["pr"], st12, _rocks(g), np.zeros((11, len(g["pr"])), np.float32), float(g["nubeg"]),
+        float(g["nuend"]), float(g["timestep"]), float(g["timesum"]), float(g["convend"]), nukoef=float(g["nukoef"]),
+        densimod=densimod, vdeep=float(g["vdeep"]), nudeep=float(g["nudeep"]), eroslev=float(g["eroslev"]),
+        sedilev=float(g["sedilev"]), waterlev=float(g["waterlev"]), sedtype=int(g["sedtype"]), mnumy=int(g["mnumy"]),
+        teclmin=float(g["teclmin"]), teclmax=float(g["teclmax"]), maxpmelt=float(g["maxpmelt"]), markw=g["markw"],
+        markex=g["markex"])
+    return out
+
+
+@pytest.mark.parametrize("densimod", [0, 1])
+def test_ronurecalc_melt_matches_reference(densimod):
+    """partially molten rocks (meltpart0/meltpart11 in ronurecalc): node
+    fields and marker viscosities equal to the reference's to the bit"""
+    g = golden("g17_ronurecalc_melt.npz")
+    fld, eta, t, e, rr, rn = _g17(g, densimod)
+    fld[6] = oracle.tkrecalc(fld[6], g["bondm"], g["bondv"], g["bondn"])
+    assert np.array_equal(t, g[f"type_out{densimod}"])
+    for name, a in (("eta", eta), ("e", e), ("rr", rr), ("rn", rn)):
+        assert np.array_equal(a, g[f"{name}_out{densimod}"]), name
+    ref = g[f"out{densimod}"]
+    for q in range(11):
+        bad = np.flatnonzero(fld[q] != ref[q])
+        assert bad.size == 0, (q, bad[:5], fld[q][bad[:5]], ref[q][bad[:5]])
+    # the case covers what it is meant to: melt markers inside their melting
+    # interval, the basaltic melt viscosity taking over
+    melt = g["mtype"] > 20
+    xm = np.array([oracle.meltpart11(int(tt), float(T), 3e3, float(w))[0]
+                   for tt, T, w in zip(g["mtype"][melt], g["tk"][melt], g["markw"][melt])])
+    assert ((xm > 0) & (xm < 1)).sum() > 50
+    assert (eta != g["eta"])[np.isin(g["mtype"], (27, 28))].any()
+
+
+def test_meltpart11_closed_forms():
+    """melt fraction against the published parameterisations written out
+    here: linear between solidus and liquidus for crustal rocks (basalt:
+    Schmidt & Poli wet solidus above 48 km, Hess liquidus); dry peridotite
+    (Katz et al. 2003): F = ((T - Ts)/(Tlherz - Ts))^1.5 up to cpx
+    exhaustion F_cpx = 0.15/(0.5 + 0.08 P), then
+    F_cpx + (1 - F_cpx) ((T - T_cpx)/(Tliq - T_cpx))^1.5; no dry melting
+    above maxpmelt; latent heat 380 / 400 kJ/kg times F"""
+    mpb = 20000.0  # 2 GPa, 60 km
+    ykm = mpb * 3e-3
+    ts = 935.4 + 0.1162 * ykm + 0.006937 * ykm * ykm
+    tl = 1423.15 + 3.5 * ykm
+    for T in (900.0, ts + 1.0, 0.5 * (ts + tl), tl - 1.0, tl + 50):
+        xm, hl = oracle.meltpart11(27, T, mpb)
+        f = min(max((T - ts) / (tl - ts), 0.0), 1.0)
+        assert abs(xm - f) <= 1e-12 and abs(hl - 380000.0 * f) <= 1e-6
+    P = mpb * 1e-4
+    Ts = 273.15 + 1085.7 + 132.9 * P - 5.1 * P * P
+    Tlz = 273.15 + 1475.0 + 80.0 * P - 3.2 * P * P
+    Tlq = 273.15 + 1780.0 + 45.0 * P - 2.0 * P * P
+    fcpx = 0.15 / (0.5 + 0.08 * P)
+    tcpx = fcpx ** (1 / 1.5) * (Tlz - Ts) + Ts
+    for T in (Ts - 5, Ts + 20, tcpx - 1, tcpx + 1, 0.5 * (tcpx + Tlq), Tlq + 10):
+        if T < Ts:
+            f = 0.0
+        elif T >= Tlq:
+            f = 1.0
+        elif T <= tcpx:
+            f = ((T - Ts) / (Tlz - Ts)) ** 1.5
+        else:
+            f = fcpx + (1 - fcpx) * ((T - tcpx) / (Tlq - tcpx)) ** 1.5
+        xm, hl = oracle.meltpart11(29, T, mpb)
+        assert abs(xm - f) <= 1e-9, (T, xm, f)
+        assert abs(hl - 400000.0 * f) <= 1e-3
+    assert oracle.meltpart11(29, Tlq - 10, mpb, maxpmelt=1.9e9) == (0.0, 0.0)
+    assert oracle.meltpart11(29, Tlq - 10, mpb, mwa=0.1, maxpmelt=1.9e9)[0] > 0
+    # water lowers the mantle solidus
+    assert oracle.meltpart11(34, Ts - 40, mpb, mwa=0.5)[0] > 0 == oracle.meltpart11(29, Ts - 40, mpb)[0]
+
+
+def test_meltpart0_mixture():
+    """partially molten rock: properties mixed linearly by the melt fraction
+    less the extracted melt (densimod 0), zero below 22"""
+    rocks = [dict() for _ in range(40)]
+    rocks[7] = dict(ro=3000.0, bro=3e-5, aro=1e-3, cp=1000.0, kt=2.0, kf=300.0, kp=1e-5, n0=1e18, n1=1e25)
+    rocks[27] = dict(ro=2700.0, bro=5e-5, aro=2e-3, cp=1300.0, kt=1.0, kf=100.0, kp=2e-5, n0=1e17, n1=1e17)
+    mpb, mex = 20000.0, 0.1
+    ykm = mpb * 3e-3
+    ts = 935.4 + 0.1162 * ykm + 0.006937 * ykm * ykm
+    tl = 1423.15 + 3.5 * ykm
+    T = ts + 0.6 * (tl - ts)
+    e, rc = oracle.meltpart0(rocks, 27, T, mpb, mex=mex, densimod=0)
+    f = 0.6 - mex
+    assert rc == 1
+    assert abs(e[3] - (2700.0 * f + 3000.0 * (1 - f))) <= 1e-9
+    ktm = (1.0 + 100.0 / (T + 77.0)) * np.exp(2e-5 * mpb)
+    kts = (2.0 + 300.0 / (T + 77.0)) * np.exp(1e-5 * mpb)
+    assert abs(e[6] - (ktm * f + kts * (1 - f))) <= 1e-12
+    # latent heat: dH/dT = 380 kJ/kg / (tl - ts) inside the interval
+    assert abs(e[5] - (1300.0 * f + 1000.0 * (1 - f) + 380000.0 / (tl - ts))) <= 1e-6 * e[5]
+    e0, rc0 = oracle.meltpart0(rocks, 21, T, mpb)
+    assert rc0 == 0 and e0 == [0.0] * 7
 
 
 def _rock(**kw):
