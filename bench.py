@@ -175,24 +175,58 @@ def oracle_residual_check(cfg, sol):
     return float(w[0] / w[1]), n
 
 
+# warm-start network: (cin, cout, output level) of the seven 3x3x3
+# convolutions, channels padded to 16 as the MFMA tiles run them
+NET_LAYERS = ((2, 16, 0), (16, 32, 1), (32, 64, 2), (64, 64, 3), (64, 32, 2), (32, 16, 1), (16, 4, 0))
+
+
+def net_infer(I, np, k):
+    """k inferences of the trained warm-start network on one configuration-1
+    field at 129^3 (features in, four fields out, host copies included in
+    the wall time); the convolution flops one inference executes on the
+    matrix cores (2 x 27 x padded cin x padded cout per output voxel)"""
+    from i3elvis_amd import warmstart as W
+    n = 129
+    ws = W.WarmStart.load(os.path.join(os.path.dirname(I.__file__), "data", "warmstart_c1_129.npz"), n=n)
+    x = W.features(I.make_config(1, n, n, n, seed=2100))
+    ws.net.infer(x)
+    t = time.time()
+    for _ in range(k):
+        ws.net.infer(x)
+    wall = (time.time() - t) / k
+    pad = lambda c: (c + 15) // 16 * 16
+    vox = [((n - 1) // 2 ** lv + 1) ** 3 for lv in range(4)]
+    flops = sum(2 * 27 * pad(ci) * pad(co) * vox[lv] for ci, co, lv in NET_LAYERS)
+    useful = sum(2 * 27 * ci * co * vox[lv] for ci, co, lv in NET_LAYERS)
+    return dict(grid=n, inferences=k, wall_ms=round(wall * 1e3, 3), mfma_flops=flops, useful_flops=useful)
+
+
 def other_paths(I, np, cfg257):
     """Device throughput of the other hot-path kernels (HIP events around the
     kernels, host copies excluded): marker cell index, sort, marker -> node
-    transfer, RK4 and temperature update on a 129^3 grid with 8 markers per
-    cell (16.8M markers); one implicit heat step on the 257^3 grid."""
+    transfer, RK4 and temperature update on the 129^3 grid of configuration
+    1 with 4x4x4 markers per cell (seeded jitter inside each sub-cell,
+    134M markers in random id order); one implicit heat step on the 257^3
+    grid."""
     out = {}
     n = 129
     cfg = I.make_config(1, n, n, n, seed=12345)
     gx, gy, gz = cfg["gx"], cfg["gy"], cfg["gz"]
     rng = np.random.default_rng(4)
-    cells = np.stack(np.meshgrid(np.arange(n - 1), np.arange(n - 1), np.arange(n - 1), indexing="ij"), -1).reshape(-1, 3)
-    c = np.repeat(cells, 8, 0)
-    x = (gx[c[:, 0]] + rng.uniform(0.01, 0.99, len(c)) * (gx[1] - gx[0])).astype(np.float32)
-    y = (gy[c[:, 1]] + rng.uniform(0.01, 0.99, len(c)) * (gy[1] - gy[0])).astype(np.float32)
-    z = (gz[c[:, 2]] + rng.uniform(0.01, 0.99, len(c)) * (gz[1] - gz[0])).astype(np.float32)
-    perm = rng.permutation(len(x))
-    x, y, z = x[perm], y[perm], z[perm]
-    m = len(x)
+    k = 4
+    cells = np.stack(np.meshgrid(np.arange(n - 1), np.arange(n - 1), np.arange(n - 1), indexing="ij"),
+                     -1).reshape(-1, 3).astype(np.int16)
+    sub = np.stack(np.meshgrid(np.arange(k), np.arange(k), np.arange(k), indexing="ij"), -1).reshape(-1, 3)
+    m = len(cells) * k ** 3
+    perm = rng.permutation(m)
+    pos = []
+    for ax, g in enumerate((gx, gy, gz)):
+        c = np.repeat(cells[:, ax], k ** 3)
+        s = np.tile(sub[:, ax], len(cells))
+        f = (s + rng.uniform(0.05, 0.95, m)) / k
+        pos.append((g[c] + f * (g[c + 1] - g[c])).astype(np.float32)[perm])
+        del c, s, f
+    x, y, z = pos
     N = n ** 3
     M = I.Markers(gx, gy, gz, m)
     M.upload(x, y, z, rng.uniform(600, 1600, m).astype(np.float32), np.zeros(m, np.float32),
@@ -204,12 +238,17 @@ def other_paths(I, np, cfg257):
     res["sort"] = (M.last_ms(), m * (8.0 + 2 * 37))
     rocks = [[1e21, 0, 0, 1e18, 1e25, 3300, 3e-5, 1e-3, 1000, 3, 0, 0, 1e-8]] * 2 + \
         [[2e16, 3e5, 4.0, 1e19, 1e24, 3300, 3e-5, 1e-3, 1000, 0.73, 1293, 4e-6, 2e-8]] * 5
-    M.to_nodes(rocks, np.full(N, 1e8), np.zeros((11, N), np.float32), 1e18, 1e25, 0.0, 1, 0.0)
-    res["to_nodes"] = (M.last_ms(), m * 40.0 + N * 44.0)
     tk = rng.uniform(600, 1600, N).astype(np.float32)
     one = np.ones(N, np.float32)
-    M.update_tk(tk + 1, tk, 3 * one, 3300 * one, 1000 * one, 1.0, 1e12)
-    res["update_tk"] = (M.last_ms(), m * 48.0 + N * 28.0)
+    # cell-segmented node averages (the default), then the reference's
+    # serial marker order for comparison
+    for order, tag in ((1, ""), (0, "_serial_order")):
+        M.set_order(order)
+        M.to_nodes(rocks, np.full(N, 1e8), np.zeros((11, N), np.float32), 1e18, 1e25, 0.0, 1, 0.0)
+        res["to_nodes" + tag] = (M.last_ms(), m * 40.0 + N * 44.0)
+        M.update_tk(tk + 1, tk, 3 * one, 3300 * one, 1000 * one, 1.0, 1e12)
+        res["update_tk" + tag] = (M.last_ms(), m * 48.0 + N * 28.0)
+    M.set_order(1)
     v = [(rng.standard_normal(N) * 1e-9).astype(np.float32) for _ in range(3)]
     M.advect_rk4(1e11, *v)
     res["rk4"] = (M.last_ms(), m * 24.0 + N * 12.0)
@@ -298,6 +337,9 @@ def main():
                     help="profiling run: one mixed-precision solve capped at this many V-cycles, no JSON metric")
     ap.add_argument("--other-paths-only", action="store_true",
                     help="profiling run: only the marker and heat measurements, printed as JSON, no metric")
+    ap.add_argument("--net-infer", type=int, default=0,
+                    help="profiling run: this many warm-start network inferences on the 129^3 configuration-1 "
+                         "fields, printed as JSON (convolution MFMA flops per inference), no metric")
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cap", type=float, default=25.0, help=argparse.SUPPRESS)
     ap.add_argument("--threads", type=int, default=0, help=argparse.SUPPRESS)
@@ -323,6 +365,9 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print("# copied 1 GiB three times", file=sys.stderr)
+        return
+    if a.net_infer:
+        print(json.dumps({"net_infer": net_infer(I, np, a.net_infer)}))
         return
     n = a.n
     t = time.time()

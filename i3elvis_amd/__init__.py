@@ -365,7 +365,9 @@ def rocks_array(rocks):
 class Markers:
     """Device marker set (i3g_markers_create .. destroy)."""
 
-    def __init__(self, gx, gy, gz, capacity):
+    def __init__(self, gx, gy, gz, capacity, order=None):
+        """order: node-average order (i3g_markers_set_order), None keeps the
+        library's default (cell-segmented)"""
         self._g = [np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz)]
         self.nx, self.ny, self.nz = (len(a) for a in self._g)
         self.N = self.nx * self.ny * self.nz
@@ -374,6 +376,8 @@ class Markers:
                                        *[_p(a) for a in self._g], C.c_long(capacity), C.byref(self.h)),
               "markers_create")
         lib().i3g_markers_count.restype = C.c_long
+        if order is not None:
+            self.set_order(order)
 
     def close(self):
         if self.h:
@@ -503,6 +507,11 @@ class Markers:
     def set_grain(self, rr, rn):
         a, b = (np.ascontiguousarray(q, np.float32) for q in (rr, rn))
         check(lib().i3g_markers_set_grain(self.h, _p(a), _p(b)), "markers_set_grain")
+
+    def set_order(self, order):
+        """node averages: 1 cell-segmented (default), 0 the reference's
+        serial marker order"""
+        check(lib().i3g_markers_set_order(self.h, C.c_int(order)), "markers_set_order")
 
     def set_melt(self, w, ex):
         """water content and extracted melt by marker id"""

@@ -68,6 +68,10 @@ struct i3g_markers {
 	 * types' properties; mw_n ids, none until i3g_markers_set_melt */
 	float *mw = nullptr;
 	long mw_n = 0, mw_cap = 0;
+	/* node averages: 1 cell-segmented (k_cell_partials / k_node_sums), 0 the
+	 * reference's serial marker order (k_marker_gather, k_tk_gather) */
+	int order = 1;
+	double *part = nullptr; /* per-cell corner partials, PF_N x 8 x cells */
 	/* movemark's depth rules (inactive by default) */
 	double zdeep = 1e30, vdeep = 1e30, tdeep = 0;
 	i3g_rock *rocks = nullptr;
@@ -320,6 +324,7 @@ struct __align__(16) MkRec {
 	double nu, ro, cp, kt, ht, bb, tk;
 	double pad;
 };
+static_assert(sizeof(MkRec) == 80 && sizeof(MkRec) % 16 == 0, "MkRec is copied in 16-byte words");
 
 __device__ void epscalc(const float *st12, long N, const double *pr, const double *gx, const double *gy,
 			const double *gz, int nx, int ny, int nz, double x, double y, double z, double *o);
@@ -330,22 +335,19 @@ __device__ void epscalc(const float *st12, long N, const double *pr, const doubl
  * water/air proportions at the water level (the marker's own y cell sets
  * the marker spacing, where the reference reads an index array shared
  * between threads) and the viscosity caps; the record for the gather. */
-__global__ void k_marker_props(long n, const float *__restrict__ x, const float *__restrict__ y,
-			       const float *__restrict__ z, const float *__restrict__ tk, float *eta, signed char *type,
-			       float *e, float *rr, float *rn, const int *c1, const int *c2, const int *c3,
-			       const double *pr, const float *st12, long N, const double *gx, const double *gy,
-			       const double *gz, int nx, int ny, int nz, const i3g_rock *rocks, int nrock,
-			       i3g_mk_params prm, const int *id, const float *mw, long mw_n, MkRec *rec, int *bad)
+__device__ __forceinline__ void
+marker_props_one(long t, const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+		 const float *__restrict__ tk, float *eta, signed char *type, float *e, float *rr, float *rn,
+		 const int *c1, const int *c2, const int *c3, const double *pr, const float *st12, long N,
+		 const double *gx, const double *gy, const double *gz, int nx, int ny, int nz, const i3g_rock *rocks,
+		 int nrock, const i3g_mk_params &prm, const int *id, const float *mw, long mw_n, MkRec *R, int *bad)
 {
 #pragma clang fp contract(off)
-	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n) return;
 	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
 	float fx = x[t], fy = y[t], fz = z[t];
 	int mt = type[t];
 	/* marker participates (mark3mg.c:460-461) */
 	int in = fx > 0 && fy > 0 && fz > 0 && fx < xs && fy < ys && fz < zs && mt < 50 && tk[t] > 0 && mt < nrock;
-	MkRec *R = rec + t;
 	R->x = fx; R->y = fy; R->z = fz; R->in = in;
 	if (!in) return;
 	int mm2 = mt;
@@ -440,6 +442,35 @@ __global__ void k_marker_props(long n, const float *__restrict__ x, const float 
 	if ((double)fy > prm.vdeep && mnu > prm.nudeep) mnu = prm.nudeep;
 	if (fy < 5e+4 && fx > xs - 1e+4 && fz > 1.5e+5 && fz < 8.7e+5 && mnu > 1e+19) mnu = 1e+19;
 	R->nu = mnu; R->ro = mro; R->cp = mcp; R->kt = mkt; R->ht = mht; R->bb = mbb; R->tk = mtk;
+}
+
+/* one thread per marker (marker_props_one); the block's records are
+ * assembled in LDS and stored as one contiguous run with unit-stride
+ * 16-byte stores (each thread writing its own 80-byte record directly
+ * leaves partially written lines to the memory side) */
+__global__ void __launch_bounds__(256)
+k_marker_props(long n, const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+	       const float *__restrict__ tk, float *eta, signed char *type, float *e, float *rr, float *rn,
+	       const int *c1, const int *c2, const int *c3, const double *pr, const float *st12, long N,
+	       const double *gx, const double *gy, const double *gz, int nx, int ny, int nz, const i3g_rock *rocks,
+	       int nrock, i3g_mk_params prm, const int *id, const float *mw, long mw_n, MkRec *rec, int *bad)
+{
+	__shared__ MkRec srec[256];
+	long b0 = (long)blockIdx.x * blockDim.x, t = b0 + threadIdx.x;
+	MkRec R;
+	R.x = R.y = R.z = 0.f;
+	R.in = 0;
+	R.nu = R.ro = R.cp = R.kt = R.ht = R.bb = R.tk = R.pad = 0.0;
+	if (t < n)
+		marker_props_one(t, x, y, z, tk, eta, type, e, rr, rn, c1, c2, c3, pr, st12, N, gx, gy, gz, nx, ny, nz,
+				 rocks, nrock, prm, id, mw, mw_n, &R, bad);
+	srec[threadIdx.x] = R;
+	__syncthreads();
+	long cnt = n - b0 < (long)blockDim.x ? n - b0 : (long)blockDim.x;
+	const uint4 *src = (const uint4 *)srec;
+	uint4 *dst = (uint4 *)(rec + b0);
+	int words = (int)(cnt * (long)(sizeof(MkRec) / 16));
+	for (int w = threadIdx.x; w < words; w += blockDim.x) dst[w] = src[w];
 }
 
 /* ------------------------------------------------------------------ */
@@ -616,6 +647,165 @@ k_marker_gather(int nx, int ny, int nz, const double *gx, const double *gy, cons
 	f[nd] = nv;
 }
 
+/* ---- cell-segmented transfer ------------------------------------------- */
+
+/* The markers of a cell are one contiguous id-ordered segment of the sorted
+ * arrays.  k_cell_partials gives each cell eight lanes of a wavefront (lane
+ * q = 4 ec + 2 ea + eb is the cell's corner node (a+ea, b+eb, c+ec)); the
+ * eight lanes walk the segment together, each marker record loaded once for
+ * the eight (one address per group), and each lane sums the reference's
+ * weighted contributions (mark3mg.c:818-933) to its corner in FP64 in the
+ * segment's id order.  k_node_sums adds, per node, the partials of its
+ * (up to) eight cells in a fixed order and normalises as k_marker_gather.
+ * Every record is read once (instead of by each of its eight nodes) and the
+ * result does not depend on scheduling; it differs from the reference's
+ * serial float accumulation (k_marker_gather) by that accumulation's own
+ * rounding.  Partials: field F of corner q of cell s at
+ * part[(F * 8 + q) * NC + s], NC cells in (c, a, b) order; a field a corner
+ * never receives is not written. */
+enum { PF_NU, PF_RO, PF_CP, PF_KT, PF_HT, PF_ET, PF_TK, PF_XX, PF_XY, PF_XZ, PF_YZ,
+       PF_W1, PF_W0, PF_WXX, PF_WXY, PF_WXZ, PF_WYZ, PF_N };
+
+/* which partial fields corner q receives: 0-6, w1, w0 always; xx at the
+ * cell's far corner; xy with ec = 0; xz with eb = 0; yz with ea = 0 */
+__device__ __forceinline__ bool pf_used(int F, int q)
+{
+	switch (F) {
+	case PF_XX: case PF_WXX: return q == 7;
+	case PF_XY: case PF_WXY: return (q >> 2) == 0;
+	case PF_XZ: case PF_WXZ: return (q & 1) == 0;
+	case PF_YZ: case PF_WYZ: return ((q >> 1) & 1) == 0;
+	default: return true;
+	}
+}
+
+__device__ __forceinline__ bool cell_of(long s, int nx, int ny, int nz, int *a, int *b, int *c)
+{
+	long hy = ny - 1, hx = nx - 1;
+	if (s >= hx * hy * (nz - 1)) return false;
+	*b = (int)(s % hy);
+	long r = s / hy;
+	*a = (int)(r % hx);
+	*c = (int)(r / hx);
+	return true;
+}
+
+__global__ void __launch_bounds__(256)
+k_cell_partials(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz, const int *start,
+		const int *cnt, const MkRec *__restrict__ rec, double nukoef, int viscmod, double *part)
+{
+#pragma clang fp contract(off)
+	long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	long s = g >> 3;
+	int q = (int)(g & 7), a, b, c;
+	if (!cell_of(s, nx, ny, nz, &a, &b, &c)) return;
+	long NC = (long)(nx - 1) * (ny - 1) * (nz - 1);
+	int ea = (q >> 1) & 1, eb = q & 1, ec = q >> 2;
+	long key = (long)c * nx * ny + (long)a * ny + b;
+	int k0 = start[key], k1 = k0 + cnt[key];
+	double dx = gx[a + 1] - gx[a], dy = gy[b + 1] - gy[b], dz = gz[c + 1] - gz[c];
+	double gxm = gx[a + ea], gym = gy[b + eb], gzm = gz[c + ec];
+	double acc[PF_N];
+#pragma unroll
+	for (int F = 0; F < PF_N; F++) acc[F] = 0;
+	for (int k = k0; k < k1; k++) {
+		const MkRec r = rec[k];
+		if (!r.in) continue;
+		double ddz = 1.0 - fabs((gzm - (double)r.z) / dz);
+		double ddx = 1.0 - fabs((gxm - (double)r.x) / dx);
+		double ddy = 1.0 - fabs((gym - (double)r.y) / dy);
+		double swt = ddx * ddy * ddz;
+		double mnu = viscmod == 1 ? log(r.nu) : viscmod == 2 ? 1.0 / r.nu : r.nu;
+		if (ddx > nukoef && ddy > nukoef && ddz > nukoef) {
+			acc[PF_NU] += mnu * swt;
+			acc[PF_W1] += swt;
+		}
+		if (q == 7) {
+			double s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (0.5 - fabs(ddy - 0.5)) * (0.5 - fabs(ddz - 0.5));
+			acc[PF_XX] += mnu * s1;
+			acc[PF_WXX] += s1;
+		}
+		if (ec == 0 && ddx > 0.5 && ddy > 0.5) {
+			double s1 = 8.0 * (ddx - 0.5) * (ddy - 0.5) * (0.5 - fabs(ddz - 0.5));
+			acc[PF_XY] += mnu * s1;
+			acc[PF_WXY] += s1;
+		}
+		if (eb == 0 && ddx > 0.5 && ddz > 0.5) {
+			double s1 = 8.0 * (ddx - 0.5) * (0.5 - fabs(ddy - 0.5)) * (ddz - 0.5);
+			acc[PF_XZ] += mnu * s1;
+			acc[PF_WXZ] += s1;
+		}
+		if (ea == 0 && ddy > 0.5 && ddz > 0.5) {
+			double s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (ddy - 0.5) * (ddz - 0.5);
+			acc[PF_YZ] += mnu * s1;
+			acc[PF_WYZ] += s1;
+		}
+		if (swt > 0) {
+			acc[PF_RO] += r.ro * swt;
+			acc[PF_CP] += r.cp * r.ro * swt;
+			acc[PF_KT] += r.kt * swt;
+			acc[PF_HT] += r.ht * swt;
+			acc[PF_ET] += r.bb * swt;
+			acc[PF_TK] += r.tk * swt;
+			acc[PF_W0] += swt;
+		}
+	}
+#pragma unroll
+	for (int F = 0; F < PF_N; F++)
+		if (pf_used(F, q)) part[((long)F * 8 + q) * NC + s] = acc[F];
+}
+
+__global__ void __launch_bounds__(256)
+k_node_sums(int nx, int ny, int nz, const double *__restrict__ part, double nubeg, double nuend, int viscmod,
+	    float *fld)
+{
+#pragma clang fp contract(off)
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int m3 = blockIdx.y;
+	int m1 = t / ny, m2 = t - m1 * ny;
+	if (m1 >= nx) return;
+	long N = (long)nx * ny * nz, NC = (long)(nx - 1) * (ny - 1) * (nz - 1);
+	long nd = (long)m3 * nx * ny + (long)m1 * ny + m2;
+	double acc[PF_N];
+#pragma unroll
+	for (int F = 0; F < PF_N; F++) acc[F] = 0;
+#pragma unroll
+	for (int q = 0; q < 8; q++) {
+		/* slot q: cell (m1-1+da, m2-1+db, m3-1+dc), the node its corner 7-q */
+		int da = (q >> 1) & 1, db = q & 1, dc = q >> 2;
+		int a = m1 - 1 + da, b = m2 - 1 + db, c = m3 - 1 + dc;
+		if (a < 0 || b < 0 || c < 0 || a > nx - 2 || b > ny - 2 || c > nz - 2) continue;
+		long s = ((long)c * (nx - 1) + a) * (ny - 1) + b;
+		int qc = 7 - q;
+#pragma unroll
+		for (int F = 0; F < PF_N; F++)
+			if (pf_used(F, qc)) acc[F] += part[((long)F * 8 + qc) * NC + s];
+	}
+	float *f = fld;
+#define VMEAN(a, w) (viscmod == 1 ? (float)exp((double)(float)((a) / (w)))                    \
+		     : viscmod == 2 ? (float)(1.0 / (double)(float)((a) / (w))) : (float)((a) / (w)))
+	if (acc[PF_W1] > 0.01) f[nd] = VMEAN(acc[PF_NU], acc[PF_W1]);
+	if (acc[PF_WXX] > 0.01) f[7 * N + nd] = VMEAN(acc[PF_XX], acc[PF_WXX]);
+	if (acc[PF_WXY] > 0.01) f[8 * N + nd] = VMEAN(acc[PF_XY], acc[PF_WXY]);
+	if (acc[PF_WXZ] > 0.01) f[9 * N + nd] = VMEAN(acc[PF_XZ], acc[PF_WXZ]);
+	if (acc[PF_WYZ] > 0.01) f[10 * N + nd] = VMEAN(acc[PF_YZ], acc[PF_WYZ]);
+#undef VMEAN
+	double w0 = acc[PF_W0];
+	if (w0 > 0.01) {
+		float rn = (float)(acc[PF_RO] / w0);
+		f[N + nd] = rn;
+		f[2 * N + nd] = (float)(acc[PF_CP] / w0 / (double)rn);
+		f[3 * N + nd] = (float)(acc[PF_KT] / w0);
+		f[4 * N + nd] = (float)(acc[PF_HT] / w0);
+		f[5 * N + nd] = (float)(acc[PF_ET] / w0);
+		f[6 * N + nd] = (float)(acc[PF_TK] / w0);
+	}
+	float nv = f[nd];
+	if (nv < nubeg) nv = (float)nubeg;
+	if (nv > nuend) nv = (float)nuend;
+	f[nd] = nv;
+}
+
 /* tk = c (+ k * tk1[partner]) on temperature boundary nodes, in float as
  * tkrecalc (heat3mg.c:849-899); tk1 is the tk plane before the update */
 __global__ void k_tkbc(long N, const int *__restrict__ p, const float *__restrict__ c, const float *__restrict__ k,
@@ -772,6 +962,65 @@ k_tk_gather(int nx, int ny, int nz, const double *gx, const double *gy, const do
 			s0 += w;
 		}
 		cm.commit(slot, nid);
+	}
+	if (s0 != 0) s1 /= s0;
+	sn[nd] = s1;
+}
+
+/* The same nodal average cell-segmented (see k_cell_partials): per cell
+ * and corner the FP64 sums of dtk w and w over the cell's id-ordered
+ * segment, then per node the eight cells' partials in a fixed order.  The
+ * reference accumulates in double too (sol0/sol1), so only the order of
+ * the FP64 additions differs from k_tk_gather. */
+__global__ void __launch_bounds__(256)
+k_tk_cell_partials(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz, const int *start,
+		   const int *cnt, const TkRec *__restrict__ rec, double *part)
+{
+#pragma clang fp contract(off)
+	long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	long s = g >> 3;
+	int q = (int)(g & 7), a, b, c;
+	if (!cell_of(s, nx, ny, nz, &a, &b, &c)) return;
+	long NC = (long)(nx - 1) * (ny - 1) * (nz - 1);
+	int ea = (q >> 1) & 1, eb = q & 1, ec = q >> 2;
+	long key = (long)c * nx * ny + (long)a * ny + b;
+	int k0 = start[key], k1 = k0 + cnt[key];
+	double dx = gx[a + 1] - gx[a], dy = gy[b + 1] - gy[b], dz = gz[c + 1] - gz[c];
+	double gxm = gx[a + ea], gym = gy[b + eb], gzm = gz[c + ec];
+	double s1 = 0, s0 = 0;
+	for (int k = k0; k < k1; k++) {
+		const TkRec r = rec[k];
+		if (!r.in) continue;
+		double ddz = 1.0 - fabs((gzm - (double)r.z) / dz);
+		double ddx = 1.0 - fabs((gxm - (double)r.x) / dx);
+		double ddy = 1.0 - fabs((gym - (double)r.y) / dy);
+		double w = ddx * ddy * ddz;
+		s1 += r.dtk * w;
+		s0 += w;
+	}
+	part[(long)q * NC + s] = s1;
+	part[(long)(8 + q) * NC + s] = s0;
+}
+
+__global__ void __launch_bounds__(256)
+k_tk_node_sums(int nx, int ny, int nz, const double *__restrict__ part, double *sn)
+{
+#pragma clang fp contract(off)
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int m3 = blockIdx.y;
+	int m1 = t / ny, m2 = t - m1 * ny;
+	if (m1 >= nx) return;
+	long NC = (long)(nx - 1) * (ny - 1) * (nz - 1);
+	long nd = (long)m3 * nx * ny + (long)m1 * ny + m2;
+	double s1 = 0, s0 = 0;
+#pragma unroll
+	for (int q = 0; q < 8; q++) {
+		int da = (q >> 1) & 1, db = q & 1, dc = q >> 2;
+		int a = m1 - 1 + da, b = m2 - 1 + db, c = m3 - 1 + dc;
+		if (a < 0 || b < 0 || c < 0 || a > nx - 2 || b > ny - 2 || c > nz - 2) continue;
+		long s = ((long)c * (nx - 1) + a) * (ny - 1) + b;
+		s1 += part[(long)(7 - q) * NC + s];
+		s0 += part[(long)(15 - q) * NC + s];
 	}
 	if (s0 != 0) s1 /= s0;
 	sn[nd] = s1;
@@ -1177,7 +1426,7 @@ extern "C" int i3g_markers_destroy(i3g_markers *m)
 		      m->tk2, m->eta2, m->type2, m->id2, m->key, m->c1, m->c2, m->c3, m->perm, m->cnt, m->start,
 		      m->cur, m->bsum, m->prec, m->pr, m->fld,
 		      m->vel, m->rocks, m->bad, m->tbc_p, m->tbc_c, m->tbc_k, m->tk1, m->tnod, m->trec, m->sn, m->tin,
-		      m->e, m->e2, m->st12, m->rr, m->rr2, m->rn, m->rn2, m->md, m->mw};
+		      m->e, m->e2, m->st12, m->rr, m->rr2, m->rn, m->rn2, m->md, m->mw, m->part};
 	for (void *p : ps)
 		if (p) hipFree(p);
 	if (m->ev0) hipEventDestroy(m->ev0);
@@ -1311,6 +1560,23 @@ extern "C" int i3g_markers_sort(i3g_markers *m, long *order)
 	return I3G_OK;
 }
 
+/* the per-cell partials of the segmented node averages, allocated on first
+ * use (PF_N x 8 doubles per cell; the temperature average uses 16) */
+static int ensure_part(i3g_markers *m)
+{
+	if (m->order != 1 || m->part) return I3G_OK;
+	int err = 0;
+	m->part = mk_alloc<double>((size_t)PF_N * 8 * (m->nx - 1) * (m->ny - 1) * (m->nz - 1), &err);
+	return err;
+}
+
+extern "C" int i3g_markers_set_order(i3g_markers *m, int order)
+{
+	if (!m || order < 0 || order > 1) return I3G_EINVAL;
+	m->order = order;
+	return I3G_OK;
+}
+
 extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int nrock, const i3g_mk_params *prm,
 				    const double *pr, float *nu, float *ro, float *cp, float *kt, float *ht, float *et,
 				    float *tk, float *nxx, float *nxy, float *nxz, float *nyz)
@@ -1333,6 +1599,7 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 		m->pr = mk_alloc<double>(N, &err);
 		if (err) return err;
 	}
+	if ((err = ensure_part(m))) return err;
 	MCHK(hipMemcpyAsync(m->rocks, rocks, nrock * sizeof(i3g_rock), hipMemcpyDefault, m->s));
 	MCHK(hipMemsetAsync(m->bad, 0, sizeof(int), m->s));
 	MCHK(hipMemcpyAsync(m->pr, pr, N * sizeof(double), hipMemcpyDefault, m->s));
@@ -1347,10 +1614,20 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, (const i3g_rock *)m->rocks,
 				   nrock, *prm, (const int *)m->id, (const float *)m->mw, m->mw_n, m->prec, m->bad);
 	dim3 gg((unsigned)(((long)m->nx * m->ny + 255) / 256), (unsigned)m->nz);
-	hipLaunchKernelGGL(k_marker_gather, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz, (const double *)m->gx,
-			   (const double *)m->gy, (const double *)m->gz, (const int *)m->start, (const int *)m->cnt,
-			   (const int *)m->id, (const MkRec *)m->prec, prm->nukoef, prm->nubeg, prm->nuend, prm->viscmod,
-			   m->fld);
+	if (m->order == 1) {
+		long NC = (long)(m->nx - 1) * (m->ny - 1) * (m->nz - 1);
+		hipLaunchKernelGGL(k_cell_partials, dim3((unsigned)((NC * 8 + 255) / 256)), dim3(256), 0, m->s, m->nx, m->ny,
+				   m->nz, (const double *)m->gx, (const double *)m->gy, (const double *)m->gz,
+				   (const int *)m->start, (const int *)m->cnt, (const MkRec *)m->prec, prm->nukoef,
+				   prm->viscmod, m->part);
+		hipLaunchKernelGGL(k_node_sums, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz, (const double *)m->part,
+				   prm->nubeg, prm->nuend, prm->viscmod, m->fld);
+	} else {
+		hipLaunchKernelGGL(k_marker_gather, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz, (const double *)m->gx,
+				   (const double *)m->gy, (const double *)m->gz, (const int *)m->start, (const int *)m->cnt,
+				   (const int *)m->id, (const MkRec *)m->prec, prm->nukoef, prm->nubeg, prm->nuend,
+				   prm->viscmod, m->fld);
+	}
 	if (m->has_tbc) {
 		MCHK(hipMemcpyAsync(m->tk1, m->fld + 6 * N, N * sizeof(float), hipMemcpyDeviceToDevice, m->s));
 		hipLaunchKernelGGL(k_tkbc, g1(N), dim3(256), 0, m->s, N, (const int *)m->tbc_p, (const float *)m->tbc_c,
@@ -1420,6 +1697,7 @@ extern "C" int i3g_markers_update_tk(i3g_markers *m, const float *tk, const floa
 		m->tin = mk_alloc<uint8_t>(m->cap, &err);
 		if (err) return err;
 	}
+	if (heatdif != 0 && (err = ensure_part(m))) return err;
 	const float *src[5] = {tk, tk2, kt, ro, cp};
 	for (int q = 0; q < 5; q++)
 		MCHK(hipMemcpyAsync(m->tnod + q * N, src[q], N * sizeof(float), hipMemcpyDefault, m->s));
@@ -1432,9 +1710,19 @@ extern "C" int i3g_markers_update_tk(i3g_markers *m, const float *tk, const floa
 				   f + 3 * N, f + 4 * N, heatdif, timestep, zdeep, tdeep, m->trec, m->tin);
 	if (n && heatdif != 0) {
 		dim3 gg((unsigned)(((long)m->nx * m->ny + 255) / 256), (unsigned)m->nz);
-		hipLaunchKernelGGL(k_tk_gather, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz, (const double *)m->gx,
-				   (const double *)m->gy, (const double *)m->gz, (const int *)m->start, (const int *)m->cnt,
-				   (const int *)m->id, (const TkRec *)m->trec, m->sn);
+		if (m->order == 1) {
+			long NC = (long)(m->nx - 1) * (m->ny - 1) * (m->nz - 1);
+			hipLaunchKernelGGL(k_tk_cell_partials, dim3((unsigned)((NC * 8 + 255) / 256)), dim3(256), 0, m->s,
+					   m->nx, m->ny, m->nz, (const double *)m->gx, (const double *)m->gy,
+					   (const double *)m->gz, (const int *)m->start, (const int *)m->cnt,
+					   (const TkRec *)m->trec, m->part);
+			hipLaunchKernelGGL(k_tk_node_sums, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz,
+					   (const double *)m->part, m->sn);
+		} else {
+			hipLaunchKernelGGL(k_tk_gather, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz, (const double *)m->gx,
+					   (const double *)m->gy, (const double *)m->gz, (const int *)m->start,
+					   (const int *)m->cnt, (const int *)m->id, (const TkRec *)m->trec, m->sn);
+		}
 		hipLaunchKernelGGL(k_tk_anti, g1(n), dim3(256), lines_bytes(m), m->s, n, (const float *)m->x, (const float *)m->y,
 				   (const float *)m->z, (const uint8_t *)m->tin, m->tk, (const double *)m->gx,
 				   (const double *)m->gy, (const double *)m->gz, m->nx, m->ny, m->nz, (const double *)m->sn);

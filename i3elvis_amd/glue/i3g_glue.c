@@ -45,6 +45,10 @@
 
 double i3g_glue_rtol = 1e-8;
 int i3g_glue_verbose = 0;
+/* node averages of ronurecalc and titerate's marker update: 0 the
+ * reference's serial marker order (its float sums to the bit), 1 the
+ * library's cell-segmented FP64 sums (i3g_markers_set_order) */
+int i3g_glue_order = 0;
 
 static struct {
 	i3g_ctx *ctx;
@@ -113,6 +117,7 @@ static void glue_markers(void)
 		if (G.mk) i3g_markers_destroy(G.mk);
 		G.mcap = marknum + marknum / 4 + 1024;
 		if ((rc = i3g_markers_create(xnumx, ynumy, znumz, gx, gy, gz, G.mcap, &G.mk))) glue_fail("markers_create", rc);
+		if ((rc = i3g_markers_set_order(G.mk, i3g_glue_order))) glue_fail("markers_set_order", rc);
 	}
 	if ((rc = i3g_markers_upload(G.mk, marknum, markx, marky, markz, markk, markv, (const signed char *)markt, NULL)))
 		glue_fail("markers_upload", rc);

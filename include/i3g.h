@@ -646,6 +646,20 @@ int i3g_markers_advect_ex(i3g_markers *m, double dt, const float *vx, const floa
 int i3g_markers_set_grain(i3g_markers *m, const float *rr, const float *rn);
 int i3g_markers_get_grain(i3g_markers *m, float *rr, float *rn);
 
+/* How the node averages of i3g_markers_to_nodes and i3g_markers_update_tk
+ * (the scatters of ronurecalc, mark3mg.c:818-933, and of titerate's
+ * subgrid diffusion, heat3mg.c:687-728) sum the markers:
+ * 1 (default) cell-segmented: each cell's id-ordered marker segment is
+ *   reduced once to FP64 partials of its eight corner nodes, then each node
+ *   adds its eight cells' partials in a fixed order; every marker record is
+ *   read once and the result does not depend on scheduling;
+ * 0 the reference's serial order: each node accumulates the markers of its
+ *   eight cells in increasing id, node fields in float as the reference's
+ *   arrays (equal to the bit to the reference run on one thread).
+ * The two differ by the float accumulation's rounding (about 1e-7 of a
+ * field, at most a few 1e-6). */
+int i3g_markers_set_order(i3g_markers *m, int order);
+
 /* Water content (markw) and extracted melt fraction (markex) of markers
  * 0..n-1 by marker id (the id given at upload): read by the melt types'
  * properties in the node transfer (meltpart0, mark3mg.c:2152-2340, reads

@@ -515,3 +515,9 @@ def meltpart0(rocks, mm2, mtk, mpb, mwa=0.0, mex=0.0, densimod=1, teclmin=0.0, t
                             C.c_double(mex), C.c_int(densimod), C.c_double(teclmin), C.c_double(teclmax),
                             C.c_double(maxpmelt), e)
     return list(e), rc
+
+
+def set_sum64(on):
+    """marker -> node sums of ronurecalc / ronurecalc_x in double (True) or
+    in float marker by marker as the reference (False, the default)"""
+    lib().or_set_sum64(C.c_int(1 if on else 0))

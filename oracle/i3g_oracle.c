@@ -9,6 +9,25 @@
 
 #include "i3g_oracle.h"
 
+/* Node sums of the marker -> node scatters: 0 in float, marker by marker,
+ * as the reference's float node arrays (its serial order); 1 in double
+ * (the exact sum rounded once, the value a reordered FP64 reduction
+ * approaches). */
+static int or_sum64 = 0;
+void or_set_sum64(int on) { or_sum64 = on; }
+
+/* one scatter addition into node accumulator k (float or double) */
+static inline void or_acc_add(float *acc, double *accd, long k, double v)
+{
+	if (or_sum64) accd[k] += v;
+	else acc[k] = (float)(acc[k] + v);
+}
+
+static inline double or_acc_get(const float *acc, const double *accd, long k)
+{
+	return or_sum64 ? accd[k] : (double)acc[k];
+}
+
 #define NODE(L, m1, m2, m3) ((m3) * (L)->nx * (L)->ny + (m1) * (L)->ny + (m2))
 
 static double clampd(double v, double lo, double hi)
@@ -684,6 +703,7 @@ void or_ronurecalc(long n, const float *x, const float *y, const float *z, const
 	long N = nx * ny * nz, i, q;
 	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
 	float *acc = calloc((size_t)(11 * N), sizeof(float));
+	double *accd = calloc((size_t)(11 * N), sizeof(double));
 	double *w = calloc((size_t)(6 * N), sizeof(double)); /* w1 w0 wxx wxy wxz wyz */
 	for (i = 0; i < n; i++) {
 		long cell[3], m1, m2, m3, m10, m20, m30;
@@ -721,36 +741,36 @@ void or_ronurecalc(long n, const float *x, const float *y, const float *z, const
 					double swt = ddx * ddy * ddz, s1;
 					long m4 = m3 * nx * ny + m1 * ny + m2;
 					if (ddx > nukoef && ddy > nukoef && ddz > nukoef) {
-						acc[m4] = (float)(acc[m4] + mnu * swt);
+						or_acc_add(acc, accd, m4, mnu * swt);
 						w[m4] += swt;
 					}
 					if (m1 > m10 && m2 > m20 && m3 > m30) {
 						s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (0.5 - fabs(ddy - 0.5)) * (0.5 - fabs(ddz - 0.5));
-						acc[7 * N + m4] = (float)(acc[7 * N + m4] + mnu * s1);
+						or_acc_add(acc, accd, 7 * N + m4, mnu * s1);
 						w[2 * N + m4] += s1;
 					}
 					if (m3 == m30 && ddx > 0.5 && ddy > 0.5) {
 						s1 = 8.0 * (ddx - 0.5) * (ddy - 0.5) * (0.5 - fabs(ddz - 0.5));
-						acc[8 * N + m4] = (float)(acc[8 * N + m4] + mnu * s1);
+						or_acc_add(acc, accd, 8 * N + m4, mnu * s1);
 						w[3 * N + m4] += s1;
 					}
 					if (m2 == m20 && ddx > 0.5 && ddz > 0.5) {
 						s1 = 8.0 * (ddx - 0.5) * (0.5 - fabs(ddy - 0.5)) * (ddz - 0.5);
-						acc[9 * N + m4] = (float)(acc[9 * N + m4] + mnu * s1);
+						or_acc_add(acc, accd, 9 * N + m4, mnu * s1);
 						w[4 * N + m4] += s1;
 					}
 					if (m1 == m10 && ddy > 0.5 && ddz > 0.5) {
 						s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (ddy - 0.5) * (ddz - 0.5);
-						acc[10 * N + m4] = (float)(acc[10 * N + m4] + mnu * s1);
+						or_acc_add(acc, accd, 10 * N + m4, mnu * s1);
 						w[5 * N + m4] += s1;
 					}
 					if (swt > 0) {
-						acc[1 * N + m4] = (float)(acc[1 * N + m4] + mro * swt);
-						acc[2 * N + m4] = (float)(acc[2 * N + m4] + mcp * mro * swt);
-						acc[3 * N + m4] = (float)(acc[3 * N + m4] + mkt * swt);
-						acc[4 * N + m4] = (float)(acc[4 * N + m4] + mht * swt);
-						acc[5 * N + m4] = (float)(acc[5 * N + m4] + mbb * swt);
-						acc[6 * N + m4] = (float)(acc[6 * N + m4] + mtk * swt);
+						or_acc_add(acc, accd, 1 * N + m4, mro * swt);
+						or_acc_add(acc, accd, 2 * N + m4, mcp * mro * swt);
+						or_acc_add(acc, accd, 3 * N + m4, mkt * swt);
+						or_acc_add(acc, accd, 4 * N + m4, mht * swt);
+						or_acc_add(acc, accd, 5 * N + m4, mbb * swt);
+						or_acc_add(acc, accd, 6 * N + m4, mtk * swt);
 						w[N + m4] += swt;
 					}
 				}
@@ -758,18 +778,19 @@ void or_ronurecalc(long n, const float *x, const float *y, const float *z, const
 		}
 	}
 	for (i = 0; i < N; i++) {
-		if (w[i] > 0.01) fld[i] = (float)(acc[i] / w[i]);
+		if (w[i] > 0.01) fld[i] = (float)(or_acc_get(acc, accd, i) / w[i]);
 		for (q = 0; q < 4; q++)
-			if (w[(2 + q) * N + i] > 0.01) fld[(7 + q) * N + i] = (float)(acc[(7 + q) * N + i] / w[(2 + q) * N + i]);
+			if (w[(2 + q) * N + i] > 0.01) fld[(7 + q) * N + i] = (float)(or_acc_get(acc, accd, (7 + q) * N + i) / w[(2 + q) * N + i]);
 		if (w[N + i] > 0.01) {
-			fld[N + i] = (float)(acc[N + i] / w[N + i]);
-			fld[2 * N + i] = (float)(acc[2 * N + i] / w[N + i] / fld[N + i]);
-			for (q = 3; q <= 6; q++) fld[q * N + i] = (float)(acc[q * N + i] / w[N + i]);
+			fld[N + i] = (float)(or_acc_get(acc, accd, N + i) / w[N + i]);
+			fld[2 * N + i] = (float)(or_acc_get(acc, accd, 2 * N + i) / w[N + i] / fld[N + i]);
+			for (q = 3; q <= 6; q++) fld[q * N + i] = (float)(or_acc_get(acc, accd, q * N + i) / w[N + i]);
 		}
 		if (fld[i] < nubeg) fld[i] = (float)nubeg;
 		if (fld[i] > nuend) fld[i] = (float)nuend;
 	}
 	free(acc);
+	free(accd);
 	free(w);
 }
 
@@ -1831,6 +1852,7 @@ int or_ronurecalc_x(long n, const float *x, const float *y, const float *z, cons
 	long N = nx * ny * nz, i, q;
 	double xs = gx[nx - 1], ys = gy[ny - 1], zs = gz[nz - 1];
 	float *acc = calloc((size_t)(11 * N), sizeof(float));
+	double *accd = calloc((size_t)(11 * N), sizeof(double));
 	double *w = calloc((size_t)(6 * N), sizeof(double));
 	int rc = 0;
 	for (i = 0; i < n && !rc; i++) {
@@ -1911,36 +1933,36 @@ int or_ronurecalc_x(long n, const float *x, const float *y, const float *z, cons
 					double swt = ddx * ddy * ddz, s1;
 					long m4 = m3 * nx * ny + m1 * ny + m2;
 					if (ddx > M->nukoef && ddy > M->nukoef && ddz > M->nukoef) {
-						acc[m4] = (float)(acc[m4] + mnu * swt);
+						or_acc_add(acc, accd, m4, mnu * swt);
 						w[m4] += swt;
 					}
 					if (m1 > m10 && m2 > m20 && m3 > m30) {
 						s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (0.5 - fabs(ddy - 0.5)) * (0.5 - fabs(ddz - 0.5));
-						acc[7 * N + m4] = (float)(acc[7 * N + m4] + mnu * s1);
+						or_acc_add(acc, accd, 7 * N + m4, mnu * s1);
 						w[2 * N + m4] += s1;
 					}
 					if (m3 == m30 && ddx > 0.5 && ddy > 0.5) {
 						s1 = 8.0 * (ddx - 0.5) * (ddy - 0.5) * (0.5 - fabs(ddz - 0.5));
-						acc[8 * N + m4] = (float)(acc[8 * N + m4] + mnu * s1);
+						or_acc_add(acc, accd, 8 * N + m4, mnu * s1);
 						w[3 * N + m4] += s1;
 					}
 					if (m2 == m20 && ddx > 0.5 && ddz > 0.5) {
 						s1 = 8.0 * (ddx - 0.5) * (0.5 - fabs(ddy - 0.5)) * (ddz - 0.5);
-						acc[9 * N + m4] = (float)(acc[9 * N + m4] + mnu * s1);
+						or_acc_add(acc, accd, 9 * N + m4, mnu * s1);
 						w[4 * N + m4] += s1;
 					}
 					if (m1 == m10 && ddy > 0.5 && ddz > 0.5) {
 						s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (ddy - 0.5) * (ddz - 0.5);
-						acc[10 * N + m4] = (float)(acc[10 * N + m4] + mnu * s1);
+						or_acc_add(acc, accd, 10 * N + m4, mnu * s1);
 						w[5 * N + m4] += s1;
 					}
 					if (swt > 0) {
-						acc[1 * N + m4] = (float)(acc[1 * N + m4] + mro * swt);
-						acc[2 * N + m4] = (float)(acc[2 * N + m4] + mcp * mro * swt);
-						acc[3 * N + m4] = (float)(acc[3 * N + m4] + mkt * swt);
-						acc[4 * N + m4] = (float)(acc[4 * N + m4] + mht * swt);
-						acc[5 * N + m4] = (float)(acc[5 * N + m4] + mbb * swt);
-						acc[6 * N + m4] = (float)(acc[6 * N + m4] + mtk * swt);
+						or_acc_add(acc, accd, 1 * N + m4, mro * swt);
+						or_acc_add(acc, accd, 2 * N + m4, mcp * mro * swt);
+						or_acc_add(acc, accd, 3 * N + m4, mkt * swt);
+						or_acc_add(acc, accd, 4 * N + m4, mht * swt);
+						or_acc_add(acc, accd, 5 * N + m4, mbb * swt);
+						or_acc_add(acc, accd, 6 * N + m4, mtk * swt);
 						w[N + m4] += swt;
 					}
 				}
@@ -1949,18 +1971,19 @@ int or_ronurecalc_x(long n, const float *x, const float *y, const float *z, cons
 	}
 	if (!rc)
 		for (i = 0; i < N; i++) {
-			if (w[i] > 0.01) fld[i] = (float)(acc[i] / w[i]);
+			if (w[i] > 0.01) fld[i] = (float)(or_acc_get(acc, accd, i) / w[i]);
 			for (q = 0; q < 4; q++)
-				if (w[(2 + q) * N + i] > 0.01) fld[(7 + q) * N + i] = (float)(acc[(7 + q) * N + i] / w[(2 + q) * N + i]);
+				if (w[(2 + q) * N + i] > 0.01) fld[(7 + q) * N + i] = (float)(or_acc_get(acc, accd, (7 + q) * N + i) / w[(2 + q) * N + i]);
 			if (w[N + i] > 0.01) {
-				fld[N + i] = (float)(acc[N + i] / w[N + i]);
-				fld[2 * N + i] = (float)(acc[2 * N + i] / w[N + i] / fld[N + i]);
-				for (q = 3; q <= 6; q++) fld[q * N + i] = (float)(acc[q * N + i] / w[N + i]);
+				fld[N + i] = (float)(or_acc_get(acc, accd, N + i) / w[N + i]);
+				fld[2 * N + i] = (float)(or_acc_get(acc, accd, 2 * N + i) / w[N + i] / fld[N + i]);
+				for (q = 3; q <= 6; q++) fld[q * N + i] = (float)(or_acc_get(acc, accd, q * N + i) / w[N + i]);
 			}
 			if (fld[i] < P->nubeg) fld[i] = (float)P->nubeg;
 			if (fld[i] > P->nuend) fld[i] = (float)P->nuend;
 		}
 	free(acc);
+	free(accd);
 	free(w);
 	return rc;
 }

@@ -181,6 +181,9 @@ int or_ronurecalc_x(long n, const float *x, const float *y, const float *z, cons
 		    long ny, const double *gz, long nz, const double *pr, const float *const *st12,
 		    const or_rockx_t *rocks, const or_visprm_t *P, const or_mkx_t *M, const float *mw,
 		    const float *mex, float *fld);
+/* node sums of or_ronurecalc / or_ronurecalc_x: 0 float as the reference
+ * (default), 1 double */
+void or_set_sum64(int on);
 /* meltpart11 (mark3mg.c:1872-2070): melt fraction and latent heat (J/kg)
  * of rock type mm2 with water content mwa at mtk (K), mpb (bar) */
 void or_meltpart11(int mm2, double mtk, double mpb, double mwa, double maxpmelt, double *xmelt, double *hlat);

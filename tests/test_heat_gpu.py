@@ -82,11 +82,15 @@ def test_heat_41_matches_direct_solve(gpu):
     Hd.close()
 
 
-def test_marker_tk_update_matches_reference(gpu):
+@pytest.mark.parametrize("order", [0, 1])
+def test_marker_tk_update_matches_reference(gpu, order):
+    """subgrid diffusion and the nodal average of the changes: the
+    reference's serial order (0) and the cell-segmented average (1, FP64
+    sums reordered) against the reference's markers"""
     g = golden("g6_heat.npz")
     m = len(g["x"])
-    print("case: marker tk update", m, flush=True)
-    M = I.Markers(g["gx"], g["gy"], g["gz"], m)
+    print(f"case: marker tk update order {order}", m, flush=True)
+    M = I.Markers(g["gx"], g["gy"], g["gz"], m, order=order)
     ids = np.arange(m, dtype=np.int32)
     perm = np.random.default_rng(6).permutation(m)
     M.upload(g["x"][perm], g["y"][perm], g["z"][perm], g["markk"][perm], np.zeros(m, np.float32),

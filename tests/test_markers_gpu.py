@@ -1,10 +1,15 @@
 """GPU parity tests of the marker path, through the C ABI.
 
 Cell indices and the sort order must be bit-exact (integer work).  The
-marker -> node fields follow the reference's serial accumulation order, so
-they are expected to agree to the last float bit except where the device's
-exp differs from libm by an ulp (Arrhenius viscosities): tolerance 2e-7
-relative, with the bit-exact fraction reported.
+marker -> node fields in the reference's serial accumulation order (order 0)
+are expected to agree to the last float bit except where the device's exp
+differs from libm by an ulp (Arrhenius viscosities): tolerance 2e-7
+relative, with the bit-exact fraction reported.  The cell-segmented
+transfer (order 1, the default) sums in FP64: against the oracle's FP64
+node sums 2e-7; against the reference's float accumulation within that
+accumulation's own rounding, k 2^-24 for k weighted terms per node (4e-6
+here), and for the geometric mean (sums of ln(eta)) that bound times the
+largest |ln eta|.
 """
 import numpy as np
 import pytest
@@ -16,8 +21,8 @@ from conftest import golden
 pytestmark = pytest.mark.gpu
 
 
-def _markers(g):
-    M = I.Markers(g["gx"], g["gy"], g["gz"], len(g["x"]) + 16)
+def _markers(g, order=0):
+    M = I.Markers(g["gx"], g["gy"], g["gz"], len(g["x"]) + 16, order=order)
     M.upload(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"])
     return M
 
@@ -53,9 +58,11 @@ def test_sort_is_stable_by_cell_then_id(gpu):
     M.close()
 
 
-def test_to_nodes_matches_reference(gpu):
+@pytest.mark.parametrize("order", [0, 1])
+def test_to_nodes_matches_reference(gpu, order):
     g = golden("g5_markers.npz")
-    M = _markers(g)
+    print(f"case: g5 to_nodes order {order}", flush=True)
+    M = _markers(g, order)
     M.sort()
     M.set_tbc(g["bondm"], g["bondv"], g["bondn"])
     rocks = [list(r) for r in g["rocks"]]
@@ -67,8 +74,23 @@ def test_to_nodes_matches_reference(gpu):
         a, b = fld[q], ref[q]
         rel = np.abs(a.astype(np.float64) - b) / np.maximum(np.abs(b), 1e-30)
         exact += int(np.sum(a == b))
-        assert rel.max() <= 2e-7, (q, rel.max())
+        assert rel.max() <= (2e-7 if order == 0 else 4e-6), (q, rel.max())
     print("bit-exact node values:", exact, "of", ref.size, flush=True)
+    if order == 1:
+        # the oracle's FP64 node sums of the same markers (tkrecalc applied
+        # by the library is left out: BC nodes are compared with the golden)
+        oracle.set_sum64(True)
+        try:
+            o, _ = oracle.ronurecalc(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"], g["gx"], g["gy"],
+                                     g["gz"], g["pr"], [dict(zip(oracle.ROCK_FIELDS, r)) for r in g["rocks"]],
+                                     g["init"], float(g["nubeg"]), float(g["nuend"]), 0.0, 1, float(g["timestep"]),
+                                     float(g["vdeep"]), float(g["nudeep"]))
+        finally:
+            oracle.set_sum64(False)
+        o[6] = oracle.tkrecalc(o[6], g["bondm"], g["bondv"], g["bondn"])
+        for q in range(11):
+            rel = np.abs(fld[q].astype(np.float64) - o[q]) / np.maximum(np.abs(o[q]), 1e-30)
+            assert rel.max() <= 2e-7, (q, rel.max())
     d = M.download()
     eta_ref = g["eta_out"][d["id"]]
     assert np.allclose(d["eta"], eta_ref, rtol=2e-7, atol=0)
@@ -108,25 +130,32 @@ def _jittered(n, reps, seed):
     return gx, gy, gz, x[perm], y[perm], z[perm], tk, mtype
 
 
-def test_to_nodes_matches_oracle_65(gpu):
+@pytest.mark.parametrize("order", [0, 1])
+def test_to_nodes_matches_oracle_65(gpu, order):
     """65^3 grid, 8 shuffled markers per cell, Arrhenius rocks: GPU transfer
-    against the oracle's serial restatement."""
+    against the oracle's serial restatement (float node sums for order 0,
+    FP64 node sums for the cell-segmented order 1)."""
     n = 65
     gx, gy, gz, x, y, z, tk, mtype = _jittered(n, 8, 5)
     m = len(x)
-    print("case: to_nodes 65^3", m, flush=True)
+    print(f"case: to_nodes 65^3 order {order}", m, flush=True)
     g = golden("g5_markers.npz")
     rocks = [list(r) for r in g["rocks"]]
     eta = np.zeros(m, np.float32)
     pr = np.random.default_rng(2).uniform(1e7, 1e9, n ** 3)
     init = np.zeros((11, n ** 3), np.float32)
-    M = I.Markers(gx, gy, gz, m)
+    M = I.Markers(gx, gy, gz, m, order=order)
     M.upload(x, y, z, tk, eta, mtype)
     M.sort()
     fld = M.to_nodes(rocks, pr, init, 1e18, 1e25, 0.0, 1, 1e10, gy[-1] * 0.8, 1e21)
     d = M.download()
-    ref, eref = oracle.ronurecalc(x, y, z, tk, eta, mtype, gx, gy, gz, pr, [dict(zip(oracle.ROCK_FIELDS, r)) for r in rocks],
-                                  init, 1e18, 1e25, 0.0, 1, 1e10, gy[-1] * 0.8, 1e21)
+    oracle.set_sum64(order == 1)
+    try:
+        ref, eref = oracle.ronurecalc(x, y, z, tk, eta, mtype, gx, gy, gz, pr,
+                                      [dict(zip(oracle.ROCK_FIELDS, r)) for r in rocks],
+                                      init, 1e18, 1e25, 0.0, 1, 1e10, gy[-1] * 0.8, 1e21)
+    finally:
+        oracle.set_sum64(False)
     for q in range(11):
         rel = np.abs(fld[q].astype(np.float64) - ref[q]) / np.maximum(np.abs(ref[q]), 1e-30)
         assert rel.max() <= 2e-7, (q, rel.max())
@@ -150,31 +179,42 @@ def test_large_marker_set_deterministic(gpu):
     f1 = M.to_nodes(rocks, pr, init, 1e18, 1e25, densimod=0, timestep=1.0)
     f2 = M.to_nodes(rocks, pr, init, 1e18, 1e25, densimod=0, timestep=1.0)
     assert np.array_equal(f1, f2)
-    # float sums of ~64 weighted terms per node
-    assert np.allclose(f1[1], 3300, rtol=1e-5)
-    assert np.allclose(f1[3], 3, rtol=1e-5)
+    # FP64 sums of ~64 weighted terms per node (cell-segmented default)
+    assert np.allclose(f1[1], 3300, rtol=1e-7)
+    assert np.allclose(f1[3], 3, rtol=1e-7)
+    # the reference's serial order: float sums, within their rounding
+    M.set_order(0)
+    f0 = M.to_nodes(rocks, pr, init, 1e18, 1e25, densimod=0, timestep=1.0)
+    assert np.array_equal(f0, M.to_nodes(rocks, pr, init, 1e18, 1e25, densimod=0, timestep=1.0))
+    assert np.allclose(f0[1], 3300, rtol=1e-5)
+    assert np.allclose(f0, f1, rtol=64 * 2.0 ** -24, atol=0)
     M.close()
 
 
+@pytest.mark.parametrize("order", [0, 1])
 @pytest.mark.parametrize("vm", [1, 2])
-def test_to_nodes_viscmod_matches_reference(gpu, vm):
+def test_to_nodes_viscmod_matches_reference(gpu, vm, order):
     """ronurecalc with geometric (1) / harmonic (2) averaging of the marker
     viscosities (mark3mg.c:848-981) against the reference's node fields
-    (g9, the g5 markers): within 2 float ulps-level relative error."""
+    (g9, the g5 markers): within 2 float ulps-level relative error in the
+    serial order; the cell-segmented order within the float accumulation's
+    rounding (for the geometric mean scaled by ln eta ~ 55)."""
     g = golden("g5_markers.npz")
     r = golden("g9_viscmod.npz")
-    print(f"case: g9 to_nodes viscmod={vm}", flush=True)
-    M = _markers(g)
+    print(f"case: g9 to_nodes viscmod={vm} order {order}", flush=True)
+    M = _markers(g, order)
     M.sort()
     M.set_tbc(g["bondm"], g["bondv"], g["bondn"])
     rocks = [list(x) for x in g["rocks"]]
     fld = M.to_nodes(rocks, g["pr"], g["init"], float(g["nubeg"]), float(g["nuend"]), 0.0, 1,
                      float(g["timestep"]), float(g["vdeep"]), float(g["nudeep"]), viscmod=vm)
     ref = r[f"mk_out_{vm}"]
+    tol = 4e-7 if order == 0 else (4e-6 if vm == 2 else 4e-6 * 60)
     for q in range(11):
         a, b = fld[q], ref[q]
         rel = np.abs(a.astype(np.float64) - b) / np.maximum(np.abs(b), 1e-30)
-        assert rel.max() <= 4e-7, (vm, q, rel.max())
+        print(f"  field {q}: max rel {rel.max():.2e}", flush=True)
+        assert rel.max() <= tol, (vm, q, rel.max())
     assert not np.array_equal(ref[0], g["out"][0])
     M.close()
 
