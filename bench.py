@@ -22,12 +22,13 @@ METRIC = "Stokes time-to-1e-8 residual @257^3, learned+mixed vs FP64 MG; HBM GB/
 HBM_PEAK_GBS = 8000.0
 
 
-def committed_traffic(n):
-    """HBM bytes of one level-0 FP32 smoother sweep (six velocity colour
-    launches, the pressure launch and the BC rows) from the committed
-    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE runs scaled by the
-    copy calibration), or None"""
-    path = os.path.join(ROOT, "profiles", f"smoother_sweep_L0_{n}_subduction.md")
+def committed_traffic(n, summary="smoother_sweep_L0"):
+    """HBM bytes of one level-0 FP32 smoother sweep (the one-launch sweep
+    k_sweep7, or the six velocity colour launches and the pressure launch,
+    with the BC rows) from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by the copy calibration), or
+    None"""
+    path = os.path.join(ROOT, "profiles", f"{summary}_{n}_subduction.md")
     if not os.path.isfile(path):
         return None
     for line in open(path):
@@ -471,8 +472,24 @@ def main():
     vel = kst.get("smooth_vel_f32_L0", {})
     pl = kst.get("smooth_p_f32_L0", {})
     slab = kst.get("sweep_slab_f32_L0", {})
+    sw7 = kst.get("sweep7_f32_L0", {})
     bcl = kst.get("bc_f32_L0", {})
-    if slab or (vel and pl):
+    if sw7:
+        # the dominant kernel: one level-0 FP32 sweep in one launch
+        # (k_sweep7) and its BC rows
+        sw_ms = sw7["avg_ms"] + bcl.get("avg_ms", 0.0)
+        by = SWEEP_BYTES_PER_NODE * N
+        ach = by / (sw_ms * 1e-3) / 1e9
+        tr = committed_traffic(n, "k_sweep7_L0")
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=tr,
+                    kernel=("level-0 FP32 smoother sweep: k_sweep7 (the seven colour phases in one launch, "
+                            "(x, y) tiles marching through the planes in LDS) + k_bc_chain<float>"),
+                    sweep_ms=round(sw_ms, 5), algorithmic_bytes=by, bytes_per_node=SWEEP_BYTES_PER_NODE,
+                    traffic_over_algorithmic=(round(tr / by, 3) if tr else None),
+                    launches={"sweep7": dict(avg_ms=round(sw7["avg_ms"], 5), calls=sw7["calls"]),
+                              "bc": dict(avg_ms=round(bcl.get("avg_ms", 0.0), 5), calls=bcl.get("calls", 0))})
+    elif slab or (vel and pl):
         # the dominant kernel's unit of work: one level-0 FP32 sweep, i.e.
         # six velocity colour phases, the pressure phase and the BC rows
         # (the phases as whole-level launches, or z-slab by z-slab)

@@ -315,6 +315,15 @@ struct i3g_ctx {
 	 * Tile targets (x, y, z-slab) from I3G_FUSED_TILE=tx,ty,tz */
 	int fused = 0;
 	int ftile[3] = {16, 64, 16};
+	/* whole FP32 sweeps in one launch (k_sweep7) on levels whose planes
+	 * split into at least sweep7_min_tiles tiles, when I3G_SWEEP7=1
+	 * (I3G_SWEEP7_MIN_TILES sets the threshold).  Off by default: at 257^3
+	 * it measured 0.95 ms a sweep against 0.91 ms for the z-slabbed phase
+	 * launches (its eight barriers a step and the 1.6x recomputed halo rows
+	 * leave it bound by the phases' latency chains, not by memory) */
+	int sweep7 = 0;
+	int sweep7_min_tiles = 128;
+	size_t sweep7_lds = 0;       /* LDS opt-in granted to k_sweep7 */
 	int fused_minz = 64; /* levels with fewer planes keep the phase launches (I3G_FUSED_MINZ) */
 	/* learned start and start combination work buffers (kept across calls) */
 	float *ws_feat = nullptr, *ws_y = nullptr;
@@ -817,6 +826,10 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		if (e && e[0] == '0') c->bc_chain = 0;
 		e = getenv("I3G_FUSED_SWEEP");
 		if (e) c->fused = e[0] == '1';
+		e = getenv("I3G_SWEEP7");
+		if (e) c->sweep7 = e[0] == '1';
+		e = getenv("I3G_SWEEP7_MIN_TILES");
+		if (e) c->sweep7_min_tiles = atoi(e);
 		e = getenv("I3G_FUSED_MINZ");
 		if (e) c->fused_minz = atoi(e);
 		e = getenv("I3G_FUSED_TILE");
@@ -1125,6 +1138,54 @@ static void fused_sweep(i3g_ctx *c, int l, const T *xin, T *xout, const T *b, co
 	bc_enforce<T>(c, L, xout, 1);
 }
 
+/* k_sweep7 tiling of a level: tiles of at most SW7_T lines in x and y,
+ * enough of them to fill the device; false when the level keeps the
+ * phase launches */
+static bool sw7_geom(i3g_ctx *c, int l, Sw7Geom *G)
+{
+	const Level &L = c->L[l];
+	if (!c->sweep7 || L.nz > SW7_MAXZ || L.nx < 3 || L.ny < 3) return false;
+	int ntx = (L.nx + SW7_T - 1) / SW7_T, nty = (L.ny + SW7_T - 1) / SW7_T;
+	if (ntx * nty < c->sweep7_min_tiles) return false;
+	size_t lds = sw7_lds_bytes(L.nz);
+	if (lds > c->sweep7_lds) {
+		int dev = 0, lim = 0;
+		if (hipGetDevice(&dev) != hipSuccess ||
+		    hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+		    (size_t)lim < lds ||
+		    hipFuncSetAttribute((const void *)k_sweep7, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+			    hipSuccess) {
+			hipGetLastError();
+			c->sweep7 = 0;
+			return false;
+		}
+		c->sweep7_lds = lds;
+	}
+	G->ntx = ntx;
+	G->nty = nty;
+	G->remap = c->xcd_remap ? 1 : 0;
+	return true;
+}
+
+/* One FP32 sweep xin -> xout in one launch (k_sweep7), then the BC rows of
+ * xout.  Equals sweep() on xin to the bit. */
+static void sweep7(i3g_ctx *c, int l, const float *xin, float *xout, const float *b, const i3g_stokes_params *p,
+		   const Sw7Geom &G)
+{
+	Level &L = c->L[l];
+	int total = G.ntx * G.nty;
+	dim3 grid = G.remap ? dim3((unsigned)(((total + 7) / 8) * 8)) : dim3((unsigned)total);
+	{
+		Timed tm(c, l == 0 ? "sweep7_f32_L0" : "sweep7_coarse", 0);
+		hipLaunchKernelGGL(k_sweep7, grid, dim3(SW7_THREADS), sw7_lds_bytes(L.nz), c->s, view<float>(L), xin,
+				   xout, b, G, (float)p->v0koef, (float)p->p0koef);
+	}
+	{
+		Timed tm(c, l == 0 ? "bc_f32_L0" : "bc_coarse", 0);
+		bc_enforce<float>(c, L, xout, 1);
+	}
+}
+
 /* ns sweeps on the level's unknowns: pairs of out-of-place sweeps through
  * the second vector (so the result lands back in x), an odd one in place */
 template <typename T> static void smooth(i3g_ctx *c, int l, int ns, const i3g_stokes_params *p)
@@ -1132,6 +1193,14 @@ template <typename T> static void smooth(i3g_ctx *c, int l, int ns, const i3g_st
 	Level &L = c->L[l];
 	T *x = lx<T>(L), *xb = lxb<T>(L), *b = lb<T>(L);
 	int s = 0;
+	if constexpr (sizeof(T) == 4) {
+		Sw7Geom G;
+		if (sw7_geom(c, l, &G))
+			for (; s + 2 <= ns; s += 2) {
+				sweep7(c, l, x, xb, b, p, G);
+				sweep7(c, l, xb, x, b, p, G);
+			}
+	}
 	if (c->fused && L.nz >= c->fused_minz)
 		for (; s + 2 <= ns; s += 2) {
 			fused_sweep<T>(c, l, x, xb, b, p);
@@ -2048,6 +2117,8 @@ static int smooth_impl(i3g_ctx *c, int level, int nsweeps, int variant, double *
 	int rc = smooth_in<T>(c, L, x, b);
 	if (rc) return rc;
 	int keep = c->fused, keepz = c->fused_minz, keeps = c->slab, keepa = c->slab_all;
+	int keep7 = c->sweep7, keept = c->sweep7_min_tiles;
+	if (variant >= 1 && variant <= 3) c->sweep7 = 0;
 	if (variant == 1) { c->fused = 0; c->slab = 0; }
 	if (variant == 2) { c->fused = 1; c->fused_minz = 0; }
 	if (variant == 3) {
@@ -2055,11 +2126,19 @@ static int smooth_impl(i3g_ctx *c, int level, int nsweeps, int variant, double *
 		c->slab = keeps > 0 ? keeps : 4;
 		c->slab_all = 1;
 	}
+	if (variant == 4) {
+		c->sweep7 = 1;
+		c->sweep7_min_tiles = 1;
+		c->fused = 0;
+		c->slab = 0;
+	}
 	smooth<T>(c, level, nsweeps, p);
 	c->fused = keep;
 	c->fused_minz = keepz;
 	c->slab = keeps;
 	c->slab_all = keepa;
+	c->sweep7 = c->sweep7 ? keep7 : 0; /* a refused LDS opt-in stays off */
+	c->sweep7_min_tiles = keept;
 	return smooth_out<T>(c, L, x);
 }
 
@@ -2067,7 +2146,7 @@ extern "C" int i3g_stokes_smooth(i3g_ctx *c, int level, int precision, int nswee
 				 const double *const *b, double v0koef, double p0koef)
 {
 	if (!c || !c->coef_ready || !x || !b || level < 0 || level >= c->nlev || nsweeps < 0 || variant < 0 ||
-	    variant > 3 || (precision != 32 && precision != 64))
+	    variant > 4 || (precision != 32 && precision != 64))
 		return I3G_EINVAL;
 	i3g_stokes_params p;
 	i3g_stokes_default_params(&p);

@@ -277,36 +277,31 @@ k_smooth_vel(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int com
  * any is used.  Same arithmetic as row_ax; the scheduling barrier keeps the
  * compiler from sinking the loads back to their uses.  COMP = 1, 2, 3.
  *
- * vel_relax returns the relaxed value x + v0koef (b - A x)/diag of
- * component COMP at node n = (i,j,k) (gausmat4, move3mg.c:3121-3124).  The
- * component's own seven stencil values come through `own(di, dj, dk)`
- * (offsets from (i,j,k)); the cross-component unknowns, P and the
- * viscosities are read from global memory.  Every smoother kernel calls this
- * one function, so that they all round alike. */
-template <typename T> struct OwnGlobal {
-	const T *a;
-	long n, sx, sz;
-	__device__ __forceinline__ T operator()(int di, int dj, int dk) const { return a[n + di * sx + dj + dk * sz]; }
-};
-
-template <typename T, int COMP, class Own>
-__device__ __forceinline__ T vel_relax(const LevelView<T> &L, const T *__restrict__ P, const T *__restrict__ vx,
-				       const T *__restrict__ vy, const T *__restrict__ vz, T rhs, int i, int j, int k,
-				       long n, T v0koef, const Own &own)
+ * vel_relax_a returns the relaxed value x + v0koef (b - A x)/diag of
+ * component COMP at one node (gausmat4, move3mg.c:3121-3124).  Every operand
+ * comes through the accessor `a`, at offsets (di, dj, dk) from the node:
+ * a.u<c>() the unknowns (0 P, 1 vx, 2 vy, 3 vz), a.eC()/eXY()/eXZ()/eYZ()
+ * the stress-point viscosities, a.ihx(d)/icx(d) (and y, z) the line
+ * factors.  Every smoother kernel, whatever memory it reads from, calls
+ * this one function, so that they all round alike. */
+template <typename T, int COMP, class A>
+__device__ __forceinline__ T vel_relax_a(const A &a, T rhs, T v0koef)
 {
-	long sx = L.ny, sz = (long)L.nx * L.ny;
 	T ax, d, a0;
 	if (COMP == 1) {
-		T c0 = L.eC[n + sz + sx + 1], c1 = L.eC[n + sz + 1];
-		T xy1 = L.eXY[n + 1], xy0 = L.eXY[n], xz1 = L.eXZ[n + sz], xz0 = L.eXZ[n];
-		a0 = own(0, 0, 0);
-		T axp = own(1, 0, 0), axm = own(-1, 0, 0), ayp = own(0, 1, 0), aym = own(0, -1, 0);
-		T azp = own(0, 0, 1), azm = own(0, 0, -1);
-		T by1 = vy[n + 1], by1x = vy[n + 1 - sx], by0 = vy[n], by0x = vy[n - sx];
-		T cz1 = vz[n + sz], cz1x = vz[n + sz - sx], cz0 = vz[n], cz0x = vz[n - sx];
-		T p1 = P[n + sz + sx + 1], p0 = P[n + sz + 1];
-		T ihx1 = L.ihx[i], ihx0 = L.ihx[i - 1], icx = L.icx[i], icy1 = L.icy[j + 1], icy0 = L.icy[j];
-		T icz1 = L.icz[k + 1], icz0 = L.icz[k], ihy = L.ihy[j], ihz = L.ihz[k];
+		T c0 = a.eC(1, 1, 1), c1 = a.eC(0, 1, 1);
+		T xy1 = a.eXY(0, 1, 0), xy0 = a.eXY(0, 0, 0), xz1 = a.eXZ(0, 0, 1), xz0 = a.eXZ(0, 0, 0);
+		a0 = a.template u<1>(0, 0, 0);
+		T axp = a.template u<1>(1, 0, 0), axm = a.template u<1>(-1, 0, 0);
+		T ayp = a.template u<1>(0, 1, 0), aym = a.template u<1>(0, -1, 0);
+		T azp = a.template u<1>(0, 0, 1), azm = a.template u<1>(0, 0, -1);
+		T by1 = a.template u<2>(0, 1, 0), by1x = a.template u<2>(-1, 1, 0);
+		T by0 = a.template u<2>(0, 0, 0), by0x = a.template u<2>(-1, 0, 0);
+		T cz1 = a.template u<3>(0, 0, 1), cz1x = a.template u<3>(-1, 0, 1);
+		T cz0 = a.template u<3>(0, 0, 0), cz0x = a.template u<3>(-1, 0, 0);
+		T p1 = a.template u<0>(1, 1, 1), p0 = a.template u<0>(0, 1, 1);
+		T ihx1 = a.ihx(0), ihx0 = a.ihx(-1), icx = a.icx(0), icy1 = a.icy(1), icy0 = a.icy(0);
+		T icz1 = a.icz(1), icz0 = a.icz(0), ihy = a.ihy(0), ihz = a.ihz(0);
 		__builtin_amdgcn_sched_barrier(0);
 		T xkf = (T)2 * icx, ykf = ihy, zkf = ihz;
 		T sxx1 = (T)2 * c0 * (axp - a0) * ihx1, sxx0 = (T)2 * c1 * (a0 - axm) * ihx0;
@@ -318,16 +313,19 @@ __device__ __forceinline__ T vel_relax(const LevelView<T> &L, const T *__restric
 		d = -xkf * ((T)2 * c0 * ihx1 + (T)2 * c1 * ihx0) - ykf * ((T)2 * xy1 * icy1 + (T)2 * xy0 * icy0) -
 		    zkf * ((T)2 * xz1 * icz1 + (T)2 * xz0 * icz0);
 	} else if (COMP == 2) {
-		T c0 = L.eC[n + sz + sx + 1], c1 = L.eC[n + sz + sx];
-		T xy1 = L.eXY[n + sx], xy0 = L.eXY[n], yz1 = L.eYZ[n + sz], yz0 = L.eYZ[n];
-		a0 = own(0, 0, 0);
-		T ayp = own(0, 1, 0), aym = own(0, -1, 0), axp = own(1, 0, 0), axm = own(-1, 0, 0);
-		T azp = own(0, 0, 1), azm = own(0, 0, -1);
-		T bx1 = vx[n + sx], bx1y = vx[n + sx - 1], bx0 = vx[n], bx0y = vx[n - 1];
-		T cz1 = vz[n + sz], cz1y = vz[n + sz - 1], cz0 = vz[n], cz0y = vz[n - 1];
-		T p1 = P[n + sz + sx + 1], p0 = P[n + sz + sx];
-		T ihy1 = L.ihy[j], ihy0 = L.ihy[j - 1], icy = L.icy[j], icx1 = L.icx[i + 1], icx0 = L.icx[i];
-		T icz1 = L.icz[k + 1], icz0 = L.icz[k], ihx = L.ihx[i], ihz = L.ihz[k];
+		T c0 = a.eC(1, 1, 1), c1 = a.eC(1, 0, 1);
+		T xy1 = a.eXY(1, 0, 0), xy0 = a.eXY(0, 0, 0), yz1 = a.eYZ(0, 0, 1), yz0 = a.eYZ(0, 0, 0);
+		a0 = a.template u<2>(0, 0, 0);
+		T ayp = a.template u<2>(0, 1, 0), aym = a.template u<2>(0, -1, 0);
+		T axp = a.template u<2>(1, 0, 0), axm = a.template u<2>(-1, 0, 0);
+		T azp = a.template u<2>(0, 0, 1), azm = a.template u<2>(0, 0, -1);
+		T bx1 = a.template u<1>(1, 0, 0), bx1y = a.template u<1>(1, -1, 0);
+		T bx0 = a.template u<1>(0, 0, 0), bx0y = a.template u<1>(0, -1, 0);
+		T cz1 = a.template u<3>(0, 0, 1), cz1y = a.template u<3>(0, -1, 1);
+		T cz0 = a.template u<3>(0, 0, 0), cz0y = a.template u<3>(0, -1, 0);
+		T p1 = a.template u<0>(1, 1, 1), p0 = a.template u<0>(1, 0, 1);
+		T ihy1 = a.ihy(0), ihy0 = a.ihy(-1), icy = a.icy(0), icx1 = a.icx(1), icx0 = a.icx(0);
+		T icz1 = a.icz(1), icz0 = a.icz(0), ihx = a.ihx(0), ihz = a.ihz(0);
 		__builtin_amdgcn_sched_barrier(0);
 		T xkf = ihx, ykf = (T)2 * icy, zkf = ihz;
 		T syy1 = (T)2 * c0 * (ayp - a0) * ihy1, syy0 = (T)2 * c1 * (a0 - aym) * ihy0;
@@ -339,16 +337,19 @@ __device__ __forceinline__ T vel_relax(const LevelView<T> &L, const T *__restric
 		d = -ykf * ((T)2 * c0 * ihy1 + (T)2 * c1 * ihy0) - xkf * ((T)2 * xy1 * icx1 + (T)2 * xy0 * icx0) -
 		    zkf * ((T)2 * yz1 * icz1 + (T)2 * yz0 * icz0);
 	} else {
-		T c0 = L.eC[n + sz + sx + 1], c1 = L.eC[n + sx + 1];
-		T xz1 = L.eXZ[n + sx], xz0 = L.eXZ[n], yz1 = L.eYZ[n + 1], yz0 = L.eYZ[n];
-		a0 = own(0, 0, 0);
-		T azp = own(0, 0, 1), azm = own(0, 0, -1), axp = own(1, 0, 0), axm = own(-1, 0, 0);
-		T ayp = own(0, 1, 0), aym = own(0, -1, 0);
-		T bx1 = vx[n + sx], bx1z = vx[n + sx - sz], bx0 = vx[n], bx0z = vx[n - sz];
-		T cy1 = vy[n + 1], cy1z = vy[n + 1 - sz], cy0 = vy[n], cy0z = vy[n - sz];
-		T p1 = P[n + sz + sx + 1], p0 = P[n + sx + 1];
-		T ihz1 = L.ihz[k], ihz0 = L.ihz[k - 1], icz = L.icz[k], icx1 = L.icx[i + 1], icx0 = L.icx[i];
-		T icy1 = L.icy[j + 1], icy0 = L.icy[j], ihx = L.ihx[i], ihy = L.ihy[j];
+		T c0 = a.eC(1, 1, 1), c1 = a.eC(1, 1, 0);
+		T xz1 = a.eXZ(1, 0, 0), xz0 = a.eXZ(0, 0, 0), yz1 = a.eYZ(0, 1, 0), yz0 = a.eYZ(0, 0, 0);
+		a0 = a.template u<3>(0, 0, 0);
+		T azp = a.template u<3>(0, 0, 1), azm = a.template u<3>(0, 0, -1);
+		T axp = a.template u<3>(1, 0, 0), axm = a.template u<3>(-1, 0, 0);
+		T ayp = a.template u<3>(0, 1, 0), aym = a.template u<3>(0, -1, 0);
+		T bx1 = a.template u<1>(1, 0, 0), bx1z = a.template u<1>(1, 0, -1);
+		T bx0 = a.template u<1>(0, 0, 0), bx0z = a.template u<1>(0, 0, -1);
+		T cy1 = a.template u<2>(0, 1, 0), cy1z = a.template u<2>(0, 1, -1);
+		T cy0 = a.template u<2>(0, 0, 0), cy0z = a.template u<2>(0, 0, -1);
+		T p1 = a.template u<0>(1, 1, 1), p0 = a.template u<0>(1, 1, 0);
+		T ihz1 = a.ihz(0), ihz0 = a.ihz(-1), icz = a.icz(0), icx1 = a.icx(1), icx0 = a.icx(0);
+		T icy1 = a.icy(1), icy0 = a.icy(0), ihx = a.ihx(0), ihy = a.ihy(0);
 		__builtin_amdgcn_sched_barrier(0);
 		T xkf = ihx, ykf = ihy, zkf = (T)2 * icz;
 		T szz1 = (T)2 * c0 * (azp - a0) * ihz1, szz0 = (T)2 * c1 * (a0 - azm) * ihz0;
@@ -361,6 +362,51 @@ __device__ __forceinline__ T vel_relax(const LevelView<T> &L, const T *__restric
 		    ykf * ((T)2 * yz1 * icy1 + (T)2 * yz0 * icy0);
 	}
 	return a0 + v0koef * (rhs - ax) / d;
+}
+
+/* Operands of a row in global memory: the unknown planes P, vx, vy, vz and
+ * the level's viscosity and line-factor arrays at node n = (i, j, k), except
+ * unknown OC, which comes through the functor `f(di, dj, dk)`. */
+template <typename T> struct OwnGlobal {
+	const T *a;
+	long n, sx, sz;
+	__device__ __forceinline__ T operator()(int di, int dj, int dk) const { return a[n + di * sx + dj + dk * sz]; }
+};
+
+template <typename T, int OC, class F> struct AccGlobal {
+	const LevelView<T> &L;
+	const T *P, *vx, *vy, *vz;
+	long n, sx, sz;
+	int i, j, k;
+	const F &f;
+	__device__ __forceinline__ long o(int di, int dj, int dk) const { return n + di * sx + dj + dk * sz; }
+	template <int C> __device__ __forceinline__ T u(int di, int dj, int dk) const
+	{
+		if (C == OC) return f(di, dj, dk);
+		const T *p = C == 0 ? P : (C == 1 ? vx : (C == 2 ? vy : vz));
+		return p[o(di, dj, dk)];
+	}
+	__device__ __forceinline__ T eC(int di, int dj, int dk) const { return L.eC[o(di, dj, dk)]; }
+	__device__ __forceinline__ T eXY(int di, int dj, int dk) const { return L.eXY[o(di, dj, dk)]; }
+	__device__ __forceinline__ T eXZ(int di, int dj, int dk) const { return L.eXZ[o(di, dj, dk)]; }
+	__device__ __forceinline__ T eYZ(int di, int dj, int dk) const { return L.eYZ[o(di, dj, dk)]; }
+	__device__ __forceinline__ T ihx(int d) const { return L.ihx[i + d]; }
+	__device__ __forceinline__ T icx(int d) const { return L.icx[i + d]; }
+	__device__ __forceinline__ T ihy(int d) const { return L.ihy[j + d]; }
+	__device__ __forceinline__ T icy(int d) const { return L.icy[j + d]; }
+	__device__ __forceinline__ T ihz(int d) const { return L.ihz[k + d]; }
+	__device__ __forceinline__ T icz(int d) const { return L.icz[k + d]; }
+};
+
+/* vel_relax_a with the component's own seven stencil values through
+ * `own(di, dj, dk)` and everything else from global memory */
+template <typename T, int COMP, class Own>
+__device__ __forceinline__ T vel_relax(const LevelView<T> &L, const T *__restrict__ P, const T *__restrict__ vx,
+				       const T *__restrict__ vy, const T *__restrict__ vz, T rhs, int i, int j, int k,
+				       long n, T v0koef, const Own &own)
+{
+	AccGlobal<T, COMP, Own> a{L, P, vx, vy, vz, n, (long)L.ny, (long)L.nx * L.ny, i, j, k, own};
+	return vel_relax_a<T, COMP>(a, rhs, v0koef);
 }
 
 template <typename T, int COMP>
@@ -394,18 +440,25 @@ k_smooth_vel_ilp(LevelView<T> L, T *__restrict__ x, const T *__restrict__ b, int
 }
 
 /* Pressure relaxation P -= p0koef * eta_cell * (div v - b_P)
- * (move3mg.c:3105-3111); BC rows are left to k_bc.  p_relax returns the
- * relaxed value from the old p; vz comes through `vzf(di, dj, dk)`. */
+ * (move3mg.c:3105-3111); BC rows are left to k_bc.  p_relax_a returns the
+ * relaxed value from the old p, the operands through an accessor as in
+ * vel_relax_a; p_relax reads vz through `vzf(di, dj, dk)`. */
+template <typename T, class A>
+__device__ __forceinline__ T p_relax_a(const A &a, T p, T rhs, T p0koef)
+{
+	T ax = (a.template u<1>(0, -1, -1) - a.template u<1>(-1, -1, -1)) * a.ihx(-1) +
+	       (a.template u<2>(-1, 0, -1) - a.template u<2>(-1, -1, -1)) * a.ihy(-1) +
+	       (a.template u<3>(-1, -1, 0) - a.template u<3>(-1, -1, -1)) * a.ihz(-1);
+	T res = ax - rhs;
+	return p - res * a.eC(0, 0, 0) * p0koef;
+}
+
 template <typename T, class VZ>
 __device__ __forceinline__ T p_relax(const LevelView<T> &L, const T *__restrict__ vx, const T *__restrict__ vy,
 				     const VZ &vzf, T p, T rhs, int i, int j, int k, long n, T p0koef)
 {
-	long sx = L.ny, sz = (long)L.nx * L.ny;
-	long v0 = n - sz - sx - 1;
-	T ax = ((T)vx[v0 + sx] - (T)vx[v0]) * L.ihx[i - 1] + ((T)vy[v0 + 1] - (T)vy[v0]) * L.ihy[j - 1] +
-	       (vzf(-1, -1, 0) - vzf(-1, -1, -1)) * L.ihz[k - 1];
-	T res = ax - rhs;
-	return p - res * L.eC[n] * p0koef;
+	AccGlobal<T, 3, VZ> a{L, (const T *)nullptr, vx, vy, (const T *)nullptr, n, (long)L.ny, (long)L.nx * L.ny, i, j, k, vzf};
+	return p_relax_a<T>(a, p, rhs, p0koef);
 }
 
 template <typename T>
@@ -564,6 +617,270 @@ k_sweep_comp(LevelView<T> L, const T *__restrict__ P, const T *__restrict__ vx, 
 			}
 		}
 		__syncthreads();
+	}
+}
+
+/* ---- the whole sweep in one launch: a z-march through LDS ---------------- */
+
+/* One FP32 sweep (phases vx c0, vx c1, vy c0, vy c1, vz c0, vz c1, P) in one
+ * launch, out of place (xin -> xout), equal to the seven phase launches of
+ * the in-place sweep to the bit.
+ *
+ * How far behind the march front a phase can run.  A row reads one plane up
+ * and down, but a phase needs the previous phase's output one plane up only
+ * where it relaxes the second colour of a component (the first colour at
+ * k+1); the other phases read the earlier components at their own plane and
+ * below (vy rows read vx at k; vz rows vx, vy at k and k-1; P rows the
+ * velocities at k and k-1).  Conversely a phase must still find old values
+ * where it reads a later phase's unknowns: vx/vy rows read vz and P at k+1,
+ * vz rows P at k and k+1, and every first colour the component's second
+ * colour at k-1.  Phase q therefore runs SW7_LAG(q) = 0,1,1,2,2,3,3 planes
+ * behind phase 0 (phases of one step in order 0..6): every node is relaxed
+ * from the values the phase-by-phase sweep gives it.
+ *
+ * A work-group owns an (x, y) tile of at most SW7_T x SW7_T nodes and
+ * marches up through all planes, two per step: at step kk (even) phase q
+ * relaxes planes kk - lag and kk - lag + 1.  Phase q relaxes the tile plus
+ * the lines that the later phases of the tile read (its region R[q],
+ * clamped to the grid): the halo values are the neighbouring work-groups'
+ * own, computed by the same function from the same inputs.  The four
+ * unknowns and four viscosities of the staged region (tile + 5 lines on
+ * both sides in x, 6 below and 5 above in y) are held in LDS in one ring of
+ * planes per field, deep enough for the planes the step reads plus the two
+ * it loads for the next step; the next step's planes, right-hand sides and
+ * flags are loaded into registers while the current step computes.  Phase 6
+ * writes the tile's four unknowns of its two planes to xout (every node of
+ * the grid exactly once). */
+#define SW7_T 17
+#define SW7_RX (SW7_T + 10)
+#define SW7_RY (SW7_T + 11)
+#define SW7_PL (SW7_RX * SW7_RY)
+#define SW7_THREADS 1024
+#define SW7_SLOTS 47
+#define SW7_MAXZ 1024
+#ifndef SW7_LOAD_PHASE
+#define SW7_LOAD_PHASE 2 /* phase before which the next step's planes are loaded */
+#endif
+
+/* LDS fields: 0 P, 1 vx, 2 vy, 3 vz, 4 eC, 5 eXY, 6 eXZ, 7 eYZ.  Ring depth
+ * = planes read below the step's base plane kk (dn) + planes read above
+ * kk+1 (up) + 2; sw7_base: first slot of each ring. */
+__host__ __device__ constexpr int sw7_up(int f) { return (f == 0 || f == 1 || f == 3 || f == 4 || f == 6) ? 1 : 0; }
+__host__ __device__ constexpr int sw7_depth(int f)
+{
+	return f == 0 ? 6 : f == 1 ? 7 : f == 2 ? 6 : f == 3 ? 7 : f == 4 ? 6 : f == 5 ? 4 : f == 6 ? 6 : 5;
+}
+__host__ __device__ constexpr int sw7_base(int f)
+{
+	return f == 0 ? 0 : f == 1 ? 6 : f == 2 ? 13 : f == 3 ? 19 : f == 4 ? 26 : f == 5 ? 32 : f == 6 ? 36 : 42;
+}
+/* float offset of field f's slot holding plane pl (pl >= -8) */
+__device__ __forceinline__ int sw7_slot(int f, int pl) { return (sw7_base(f) + (pl + 8) % sw7_depth(f)) * SW7_PL; }
+
+/* phase q: lag, component, colour, region margins (x below, x above, y below, y above) */
+__host__ __device__ constexpr int sw7_lag(int q) { return q == 0 ? 0 : q <= 2 ? 1 : q <= 4 ? 2 : 3; }
+__host__ __device__ constexpr int sw7_comp(int q) { return q == 6 ? 0 : 1 + q / 2; }
+__host__ __device__ constexpr int sw7_mx0(int q) { return q == 0 ? 4 : q <= 2 ? 3 : q <= 4 ? 2 : q == 5 ? 1 : 0; }
+__host__ __device__ constexpr int sw7_mx1(int q) { return q == 0 ? 4 : q == 1 ? 3 : q == 2 ? 2 : q <= 4 ? 1 : 0; }
+__host__ __device__ constexpr int sw7_my0(int q) { return q == 0 ? 5 : q == 1 ? 4 : q == 2 ? 3 : q <= 4 ? 2 : q == 5 ? 1 : 0; }
+__host__ __device__ constexpr int sw7_my1(int q) { return q == 0 ? 4 : q <= 2 ? 3 : q == 3 ? 2 : q == 4 ? 1 : 0; }
+
+struct Sw7Acc {
+	const float *s;          /* LDS rings */
+	int o, li, lj, pl;       /* node: offset in a region plane, region line x, y; plane */
+	const float *gxh, *gxc, *gyh, *gyc, *gzh, *gzc;
+	__device__ __forceinline__ float at(int f, int di, int dj, int dk) const
+	{
+		return s[sw7_slot(f, pl + dk) + o + di * SW7_RY + dj];
+	}
+	template <int C> __device__ __forceinline__ float u(int di, int dj, int dk) const { return at(C, di, dj, dk); }
+	__device__ __forceinline__ float eC(int di, int dj, int dk) const { return at(4, di, dj, dk); }
+	__device__ __forceinline__ float eXY(int di, int dj, int dk) const { return at(5, di, dj, dk); }
+	__device__ __forceinline__ float eXZ(int di, int dj, int dk) const { return at(6, di, dj, dk); }
+	__device__ __forceinline__ float eYZ(int di, int dj, int dk) const { return at(7, di, dj, dk); }
+	__device__ __forceinline__ float ihx(int d) const { return gxh[li + d]; }
+	__device__ __forceinline__ float icx(int d) const { return gxc[li + d]; }
+	__device__ __forceinline__ float ihy(int d) const { return gyh[lj + d]; }
+	__device__ __forceinline__ float icy(int d) const { return gyc[lj + d]; }
+	__device__ __forceinline__ float ihz(int d) const { return gzh[pl + d]; }
+	__device__ __forceinline__ float icz(int d) const { return gzc[pl + d]; }
+};
+
+/* Work-group barrier for LDS only: waits for this wave's LDS operations,
+ * not for its global loads in flight (__syncthreads() also drains those,
+ * which would expose the next step's prefetch at every phase). */
+__device__ __forceinline__ void sw7_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+struct Sw7Geom {
+	int ntx, nty;   /* tiles in x and y */
+	int remap;      /* XCD-contiguous tile order */
+};
+
+/* dynamic LDS bytes of k_sweep7 for a level of nz planes */
+static inline size_t sw7_lds_bytes(int nz)
+{
+	return sizeof(float) * ((size_t)SW7_SLOTS * SW7_PL + 2 * SW7_RX + 2 * SW7_RY + 2 * (size_t)nz);
+}
+
+__global__ void __launch_bounds__(SW7_THREADS)
+k_sweep7(LevelView<float> L, const float *__restrict__ xin, float *__restrict__ xout, const float *__restrict__ b,
+	 Sw7Geom G, float v0koef, float p0koef)
+{
+	extern __shared__ __align__(16) unsigned char lds_raw[];
+	float *s = (float *)lds_raw;
+	float *gxh = s + SW7_SLOTS * SW7_PL, *gxc = gxh + SW7_RX, *gyh = gxc + SW7_RX, *gyc = gyh + SW7_RY;
+	float *gzh = gyc + SW7_RY, *gzc = gzh + L.nz;
+	int w, dummy;
+	if (!block_work(G.remap, G.ntx * G.nty, 1, 0, &w, &dummy)) return;
+	const int tx = w / G.nty, ty = w - tx * G.nty;
+	const int nx = L.nx, ny = L.ny, nz = L.nz;
+	const long N = L.N, sz = (long)nx * ny;
+	const int i0 = (int)((long)tx * nx / G.ntx), i1 = (int)((long)(tx + 1) * nx / G.ntx);
+	const int j0 = (int)((long)ty * ny / G.nty), j1 = (int)((long)(ty + 1) * ny / G.nty);
+	const int rx0 = i0 - 5, ry0 = j0 - 6; /* region origin (may lie outside the grid) */
+	const int t = threadIdx.x;
+	const int pidx = t >> 9, r = t & 511; /* half of the group: plane kk-lag+pidx */
+
+	for (int q = t; q < SW7_RX; q += SW7_THREADS) {
+		int i = rx0 + q;
+		bool in = i >= 0 && i < nx;
+		gxh[q] = in ? L.ihx[i] : 0.f;
+		gxc[q] = in ? L.icx[i] : 0.f;
+	}
+	for (int q = t; q < SW7_RY; q += SW7_THREADS) {
+		int j = ry0 + q;
+		bool in = j >= 0 && j < ny;
+		gyh[q] = in ? L.ihy[j] : 0.f;
+		gyc[q] = in ? L.icy[j] : 0.f;
+	}
+	for (int q = t; q < nz; q += SW7_THREADS) {
+		gzh[q] = L.ihz[q];
+		gzc[q] = L.icz[q];
+	}
+
+	/* this thread's node of each phase (same every step): region offsets
+	 * li << 8 | lj, or -1 */
+	int code[7];
+#pragma unroll
+	for (int q = 0; q < 7; q++) {
+		int xa = max(i0 - sw7_mx0(q), 0), xb = min(i1 + sw7_mx1(q), nx);
+		int ya = max(j0 - sw7_my0(q), 0), yb = min(j1 + sw7_my1(q), ny);
+		int rows = xb - xa;
+		int i, j;
+		bool ok;
+		if (q == 6) {
+			int wdt = yb - ya;
+			int ii = r / wdt;
+			i = xa + ii;
+			j = ya + (r - ii * wdt);
+			ok = ii < rows;
+		} else {
+			int hw = (yb - ya + 1) >> 1;
+			int ii = r / hw;
+			i = xa + ii;
+			int par = (sw7_lag(q) + pidx) & 1;
+			j = ya + ((ya + i + par + (q & 1)) & 1) + 2 * (r - ii * hw);
+			ok = ii < rows && j < yb;
+		}
+		code[q] = ok ? ((i - rx0) << 8 | (j - ry0)) : -1;
+	}
+
+	/* region node of this thread for the plane loads */
+	const int lrx = t / SW7_RY, lry = t - lrx * SW7_RY;
+	const int gi = rx0 + lrx, gj = ry0 + lry;
+	const bool lin = t < SW7_PL && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+	const long gn = (long)gi * ny + gj;
+	const float *src[8] = {xin, xin + N, xin + 2 * N, xin + 3 * N, L.eC, L.eXY, L.eXZ, L.eYZ};
+
+	/* planes of the first step */
+#pragma unroll
+	for (int f = 0; f < 8; f++)
+		for (int pl = 0; pl <= 1 + sw7_up(f) && pl < nz; pl++)
+			if (lin) s[sw7_slot(f, pl) + t] = src[f][pl * sz + gn];
+
+	/* right-hand sides and flags of a step's nodes (the flags are kept as
+	 * loaded: nothing may wait for the next step's loads before the step
+	 * ends) */
+	float rhs[7], nrhs[7];
+	unsigned flg[7], nflg[7];
+	auto fetch = [&](int kk, float *rr, unsigned *ff) {
+#pragma unroll
+		for (int q = 0; q < 7; q++) {
+			int pl = kk - sw7_lag(q) + pidx;
+			int cd = code[q];
+			asm volatile("" : "+v"(cd));
+			rr[q] = 0.f;
+			ff[q] = 0;
+			if (cd >= 0 && pl >= 0 && pl < nz) {
+				long n = pl * sz + (long)(rx0 + (cd >> 8)) * ny + (ry0 + (cd & 255));
+				ff[q] = L.flags[n];
+				if (b) rr[q] = b[sw7_comp(q) * N + n];
+			}
+		}
+	};
+	fetch(0, rhs, flg);
+	__syncthreads();
+
+	for (int kk = 0; kk <= nz + 2; kk += 2) {
+		fetch(kk + 2, nrhs, nflg);
+		float nv[8][2];
+#pragma unroll
+		for (int q = 0; q < 7; q++) {
+			if (q == SW7_LOAD_PHASE) {
+				/* the next step's planes, loaded while the last phases run */
+#pragma unroll
+				for (int f = 0; f < 8; f++)
+#pragma unroll
+					for (int e = 0; e < 2; e++) {
+						int pl = kk + 2 + sw7_up(f) + e;
+						nv[f][e] = (lin && pl < nz) ? src[f][pl * sz + gn] : 0.f;
+					}
+			}
+			const int C = sw7_comp(q);
+			int pl = __builtin_amdgcn_readfirstlane(kk - sw7_lag(q) + pidx);
+			int cd = code[q];
+			/* recompute what derives from the node code in every step
+			 * instead of holding it in registers across the march */
+			asm volatile("" : "+v"(cd));
+			if (cd >= 0 && pl >= 0 && pl < nz) {
+				int li = cd >> 8, lj = cd & 255;
+				Sw7Acc a{s, li * SW7_RY + lj, li, lj, pl, gxh, gxc, gyh, gyc, gzh, gzc};
+				unsigned fl = flg[q];
+				bool upd = (fl & FL_EX(C)) && !(fl & FL_BC(C));
+				if (q < 6) {
+					if (upd) {
+						float v;
+						if (C == 1) v = vel_relax_a<float, 1>(a, rhs[q], v0koef);
+						else if (C == 2) v = vel_relax_a<float, 2>(a, rhs[q], v0koef);
+						else v = vel_relax_a<float, 3>(a, rhs[q], v0koef);
+						s[sw7_slot(C, pl) + a.o] = v;
+					}
+				} else {
+					float pv = a.at(0, 0, 0, 0);
+					if (upd) pv = p_relax_a<float>(a, pv, rhs[q], p0koef);
+					long n = pl * sz + (long)(rx0 + li) * ny + (ry0 + lj);
+					xout[n] = pv;
+					xout[N + n] = a.at(1, 0, 0, 0);
+					xout[2 * N + n] = a.at(2, 0, 0, 0);
+					xout[3 * N + n] = a.at(3, 0, 0, 0);
+				}
+			}
+			sw7_sync();
+		}
+
+#pragma unroll
+		for (int f = 0; f < 8; f++)
+#pragma unroll
+			for (int e = 0; e < 2; e++) {
+				int pl = kk + 2 + sw7_up(f) + e;
+				if (lin && pl < nz) s[sw7_slot(f, pl) + t] = nv[f][e];
+			}
+#pragma unroll
+		for (int q = 0; q < 7; q++) {
+			rhs[q] = nrhs[q];
+			flg[q] = nflg[q];
+		}
+		sw7_sync();
 	}
 }
 

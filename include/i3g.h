@@ -272,7 +272,11 @@ int i3g_stokes_prolong(i3g_ctx *ctx, int level, const double *const *coarse,
  * phase launches, 2 one launch per velocity component (the pressure rows in
  * the vz launch; pairs of sweeps out of place, an odd last sweep as
  * phases), 3 the seven phases in z-slabs (phase q of slab [b, b') on planes
- * [b - q, b' - q); slab thickness from I3G_SLAB at create, else 4 planes).
+ * [b - q, b' - q); slab thickness from I3G_SLAB at create, else 4 planes),
+ * 4 the whole FP32 sweep in one launch (k_sweep7: (x, y) tiles marching
+ * up through the planes with the phases lagging 0,1,1,2,2,3,3 planes, out
+ * of place in pairs; on every level, however few its tiles; FP64 and an
+ * odd last sweep as phases).
  * precision 64 or 32 (the planes are rounded to float on the
  * device, and back).  Both need i3g_set_coefficients first and overwrite
  * the level's multigrid work buffers. */
