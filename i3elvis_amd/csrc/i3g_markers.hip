@@ -5,6 +5,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -198,49 +199,144 @@ __global__ void k_scan_add(int *start, long N, const int *bsum, long total_slot)
 	(void)total_slot;
 }
 
-__global__ void k_fill(long n, const int *key, const int *start, int *cur, int *perm)
-{
-	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n) return;
-	int k = key[t];
-	int slot = atomicAdd(&cur[k], 1);
-	perm[start[k] + slot] = (int)t;
-}
+/* A marker's whole state in the sort's staging buffer: 48 bytes, three
+ * 16-byte words (position and temperature; viscosity, strain, roughness,
+ * dislocation share; type, id, previous position). */
+struct SortRec {
+	float4 a, b;
+	int4 c;
+};
 
-/* within each cell, order the marker positions by marker id (insertion
- * sort; cells hold a few tens of markers) */
-__global__ void k_cellsort(long N, const int *start, const int *cnt, int *perm, const int *id)
-{
-	long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= N) return;
-	int s = start[c], e = s + cnt[c];
-	for (int i = s + 1; i < e; i++) {
-		int v = perm[i], vid = id[v];
-		int j = i - 1;
-		while (j >= s && id[perm[j]] > vid) { perm[j + 1] = perm[j]; j--; }
-		perm[j + 1] = v;
-	}
-}
-
-/* The sorted order applied to every per-marker array in one pass: the
- * permutation is read once and all ten values of a marker are loaded
- * before the first store. */
 struct MkArrays {
 	float *x, *y, *z, *tk, *eta, *e, *rr, *rn;
 	signed char *type;
 	int *id;
 };
-__global__ void k_permute(long n, const int *__restrict__ perm, MkArrays s, MkArrays d)
+
+/* Each marker's record into its cell's range (in arrival order).  Lanes of
+ * a wavefront with the same cell in a row (markers that are already sorted
+ * arrive that way) take their slots with one atomic per run. */
+__global__ void __launch_bounds__(256) k_fill(long n, const int *__restrict__ key, const int *__restrict__ start,
+					      int *cur, MkArrays src, SortRec *__restrict__ rec)
 {
 	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n) return;
-	int p = perm[t];
-	float x = s.x[p], y = s.y[p], z = s.z[p], tk = s.tk[p], eta = s.eta[p], e = s.e[p], rr = s.rr[p], rn = s.rn[p];
-	signed char ty = s.type[p];
-	int id = s.id[p];
-	d.x[t] = x; d.y[t] = y; d.z[t] = z; d.tk[t] = tk; d.eta[t] = eta; d.e[t] = e; d.rr[t] = rr; d.rn[t] = rn;
-	d.type[t] = ty;
-	d.id[t] = id;
+	const int lane = threadIdx.x & 63;
+	const bool act = t < n;
+	int k = act ? key[t] : -1;
+	int prev = __shfl_up(k, 1);
+	bool head = act && (lane == 0 || prev != k);
+	unsigned long long heads = __ballot(head), acts = __ballot(act);
+	unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+	int rs = 63 - __clzll(heads & upto); /* first lane of this lane's run */
+	unsigned long long after = heads & ~upto;
+	int re = after ? __ffsll(after) - 1 : 64 - __clzll(acts); /* one past its last lane */
+	int base = 0;
+	if (head) base = atomicAdd(&cur[k], re - lane);
+	base = __shfl(base, rs);
+	if (!act) return;
+	int pos = start[k] + base + (lane - rs);
+	SortRec r;
+	r.a = make_float4(src.x[t], src.y[t], src.z[t], src.tk[t]);
+	r.b = make_float4(src.eta[t], src.e[t], src.rr[t], src.rn[t]);
+	r.c = make_int4((int)src.type[t], src.id[t], (int)t, 0);
+	rec[pos] = r;
+}
+
+/* Within each cell, order the markers by id and write them out: one
+ * wavefront per cell sorts the cell's (id, slot) pairs with a bitonic
+ * network (in registers across the lanes for up to 64 markers, in LDS for
+ * up to CS_MAX, a serial insertion sort beyond), then writes every marker's
+ * arrays, its previous position and its cell indices in the sorted order.
+ * Ids are unique, so the order is fully determined (the same as any stable
+ * sort by id). */
+#define CS_MAX 1024
+__device__ __forceinline__ void put_sorted(const SortRec &r, long o, int c, int nx, int ny, MkArrays d, int *perm,
+					   int *key, int *c1, int *c2, int *c3)
+{
+	d.x[o] = r.a.x; d.y[o] = r.a.y; d.z[o] = r.a.z; d.tk[o] = r.a.w;
+	d.eta[o] = r.b.x; d.e[o] = r.b.y; d.rr[o] = r.b.z; d.rn[o] = r.b.w;
+	d.type[o] = (signed char)r.c.x;
+	d.id[o] = r.c.y;
+	perm[o] = r.c.z;
+	int pl = nx * ny, m3 = c / pl, rem = c - m3 * pl, m1 = rem / ny;
+	key[o] = c;
+	c1[o] = m1;
+	c2[o] = rem - m1 * ny;
+	c3[o] = m3;
+}
+
+__global__ void __launch_bounds__(256) k_cellsort(long N, int nx, int ny, const int *__restrict__ start,
+						  const int *__restrict__ cnt, const SortRec *__restrict__ rec, MkArrays d,
+						  int *perm, int *key, int *c1, int *c2, int *c3, int *scratch)
+{
+	__shared__ int sk[4][CS_MAX], sv[4][CS_MAX];
+	const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	long cl = (long)blockIdx.x * 4 + w;
+	if (cl >= N) return;
+	const int c = (int)cl, s = start[c], e = cnt[c];
+	if (e == 0) return;
+	if (e <= 64) {
+		int P = 1;
+		while (P < e) P <<= 1;
+		int k = lane < e ? rec[s + lane].c.y : INT_MAX, v = lane;
+		for (int kk = 2; kk <= P; kk <<= 1)
+			for (int j = kk >> 1; j > 0; j >>= 1) {
+				int ok = __shfl_xor(k, j), ov = __shfl_xor(v, j);
+				bool up = (lane & kk) == 0, lower = (lane & j) == 0;
+				if (lower == up ? ok < k : ok > k) {
+					k = ok;
+					v = ov;
+				}
+			}
+		if (lane < e) put_sorted(rec[s + v], s + lane, c, nx, ny, d, perm, key, c1, c2, c3);
+		return;
+	}
+	if (e <= CS_MAX) {
+		int P = 128;
+		while (P < e) P <<= 1;
+		int *K = sk[w], *V = sv[w];
+		for (int i = lane; i < P; i += 64) {
+			K[i] = i < e ? rec[s + i].c.y : INT_MAX;
+			V[i] = i;
+		}
+		/* one wavefront owns K, V: its LDS operations complete in program
+		 * order; the empty asm keeps the compiler from reordering them */
+		asm volatile("" ::: "memory");
+		for (int kk = 2; kk <= P; kk <<= 1)
+			for (int j = kk >> 1; j > 0; j >>= 1) {
+				for (int i = lane; i < P; i += 64) {
+					int ixj = i ^ j;
+					if (ixj > i) {
+						int a = K[i], b = K[ixj];
+						if ((a > b) == ((i & kk) == 0)) {
+							int va = V[i];
+							K[i] = b;
+							K[ixj] = a;
+							V[i] = V[ixj];
+							V[ixj] = va;
+						}
+					}
+				}
+				asm volatile("" ::: "memory");
+			}
+		for (int i = lane; i < e; i += 64) put_sorted(rec[s + V[i]], s + i, c, nx, ny, d, perm, key, c1, c2, c3);
+		return;
+	}
+	/* more than CS_MAX markers in one cell: slots ordered by insertion in
+	 * this cell's range of `scratch`, by one lane */
+	if (lane == 0) {
+		int *q = scratch + s;
+		for (int i = 0; i < e; i++) {
+			int vid = rec[s + i].c.y, j = i - 1;
+			while (j >= 0 && rec[s + q[j]].c.y > vid) {
+				q[j + 1] = q[j];
+				j--;
+			}
+			q[j + 1] = i;
+		}
+	}
+	__threadfence_block();
+	for (int i = lane; i < e; i += 64) put_sorted(rec[s + scratch[s + i]], s + i, c, nx, ny, d, perm, key, c1, c2, c3);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1512,12 +1608,9 @@ extern "C" int i3g_markers_cell_index(i3g_markers *m, int *m1, int *m2, int *m3,
 	return I3G_OK;
 }
 
-/* gather every per-marker array into the second buffers, then swap */
-static void permute_all(i3g_markers *m)
+/* the sorted arrays are in the second buffers: swap */
+static void swap_sorted(i3g_markers *m)
 {
-	MkArrays s{m->x, m->y, m->z, m->tk, m->eta, m->e, m->rr, m->rn, m->type, m->id};
-	MkArrays d{m->x2, m->y2, m->z2, m->tk2, m->eta2, m->e2, m->rr2, m->rn2, m->type2, m->id2};
-	hipLaunchKernelGGL(k_permute, g1(m->n), dim3(256), 0, m->s, m->n, (const int *)m->perm, s, d);
 	std::swap(m->x, m->x2); std::swap(m->y, m->y2); std::swap(m->z, m->z2);
 	std::swap(m->tk, m->tk2); std::swap(m->eta, m->eta2); std::swap(m->e, m->e2);
 	std::swap(m->rr, m->rr2); std::swap(m->rn, m->rn2);
@@ -1537,14 +1630,25 @@ extern "C" int i3g_markers_sort(i3g_markers *m, long *order)
 	hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SCAN_B), 0, m->s, m->start, N, (const int *)m->bsum, 0L);
 	MCHK(hipMemsetAsync(m->cur, 0, N * sizeof(int), m->s));
 	if (n) {
+		/* the records are staged in the node transfer's record buffer
+		 * (rewritten by every transfer after a sort) */
+		int err = 0;
+		if (!m->prec) m->prec = mk_alloc<MkRec>(m->cap, &err);
+		if (err) return err;
+		static_assert(sizeof(SortRec) <= sizeof(MkRec), "sort records fit the transfer records");
+		SortRec *rec = (SortRec *)m->prec;
+		MkArrays src{m->x, m->y, m->z, m->tk, m->eta, m->e, m->rr, m->rn, m->type, m->id};
+		MkArrays dst{m->x2, m->y2, m->z2, m->tk2, m->eta2, m->e2, m->rr2, m->rn2, m->type2, m->id2};
 		hipLaunchKernelGGL(k_fill, g1(n), dim3(256), 0, m->s, n, (const int *)m->key, (const int *)m->start, m->cur,
-				   m->perm);
-		hipLaunchKernelGGL(k_cellsort, g1(N), dim3(256), 0, m->s, N, (const int *)m->start, (const int *)m->cnt,
-				   m->perm, (const int *)m->id);
-		permute_all(m);
-		/* cell indices in the new order */
-		rc = cell_index(m, 0);
-		if (rc) return rc;
+				   src, rec);
+		/* the scratch of cells beyond CS_MAX markers (n ints, indexed by
+		 * sorted position): the record buffer's tail past cap SortRecs */
+		static_assert(sizeof(MkRec) - sizeof(SortRec) >= sizeof(int), "scratch fits behind the sort records");
+		int *scratch = (int *)(rec + m->cap);
+		hipLaunchKernelGGL(k_cellsort, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, m->s, N, m->nx, m->ny,
+				   (const int *)m->start, (const int *)m->cnt, (const SortRec *)rec, dst, m->perm, m->key,
+				   m->c1, m->c2, m->c3, scratch);
+		swap_sorted(m);
 	}
 	MCHK(hipGetLastError());
 	mark_end(m);
@@ -1590,11 +1694,9 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 		if (!host[q]) return I3G_EINVAL;
 	int err = 0;
 	long n = m->n, N = m->N;
-	if (!m->prec) {
-		m->prec = mk_alloc<MkRec>(m->cap, &err);
-		m->fld = mk_alloc<float>(11 * N, &err);
-		if (err) return err;
-	}
+	if (!m->prec) m->prec = mk_alloc<MkRec>(m->cap, &err);
+	if (!m->fld) m->fld = mk_alloc<float>(11 * N, &err);
+	if (err) return err;
 	if (!m->pr) {
 		m->pr = mk_alloc<double>(N, &err);
 		if (err) return err;

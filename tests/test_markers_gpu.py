@@ -58,6 +58,40 @@ def test_sort_is_stable_by_cell_then_id(gpu):
     M.close()
 
 
+def test_sort_cells_of_every_size(gpu):
+    """The in-cell ordering by id for cells of 1 to 64 markers (bitonic
+    network across the lanes), 65 to 1024 (in LDS) and more (serial), with
+    ids in random order: the order equals numpy's lexsort by (cell, id)."""
+    rng = np.random.default_rng(21)
+    n = 9
+    g = np.linspace(0.0, 8.0, n)
+    # cell (a, b, c) receives 0..1600 markers: a few crowded cells
+    sizes = {0: 1, 1: 7, 2: 64, 3: 65, 4: 200, 5: 1024, 6: 1025, 7: 1600, 100: 33, 300: 129}
+    rest = rng.integers(0, 40, 512)
+    pts = []
+    for cell in range(512):
+        k = sizes.get(cell, int(rest[cell]))
+        c3, r = divmod(cell, 64)
+        c1, c2 = divmod(r, 8)
+        pts.append(np.stack([c1 + rng.uniform(0.01, 0.99, k), c2 + rng.uniform(0.01, 0.99, k),
+                             c3 + rng.uniform(0.01, 0.99, k)], 1))
+    pts = np.concatenate(pts).astype(np.float32)
+    m = len(pts)
+    perm = rng.permutation(m)
+    ids = rng.permutation(m).astype(np.int32)
+    print(f"case: sort {m} markers, cells up to 1600", flush=True)
+    M = I.Markers(g, g, g, m)
+    z = np.zeros(m, np.float32)
+    M.upload(pts[perm, 0], pts[perm, 1], pts[perm, 2], z, z, np.full(m, 2, np.int8), ids[perm])
+    key0 = M.cell_index()[3]
+    order = M.sort()
+    expect = np.lexsort((ids[perm], key0))
+    assert np.array_equal(order, expect)
+    assert np.array_equal(M.download()["id"], ids[perm][expect])
+    assert np.array_equal(M.sort(), np.arange(m))
+    M.close()
+
+
 @pytest.mark.parametrize("order", [0, 1])
 def test_to_nodes_matches_reference(gpu, order):
     g = golden("g5_markers.npz")
