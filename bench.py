@@ -473,21 +473,26 @@ def main():
     pl = kst.get("smooth_p_f32_L0", {})
     slab = kst.get("sweep_slab_f32_L0", {})
     sw7 = kst.get("sweep7_f32_L0", {})
+    sw3 = kst.get("sweep3_f32_L0", {})
     bcl = kst.get("bc_f32_L0", {})
-    if sw7:
-        # the dominant kernel: one level-0 FP32 sweep in one launch
-        # (k_sweep7) and its BC rows
-        sw_ms = sw7["avg_ms"] + bcl.get("avg_ms", 0.0)
+    if sw3 or sw7:
+        # the dominant kernel: one level-0 FP32 sweep, three launches (one
+        # per velocity component, k_sweep3) or one (k_sweep7), and its BC
+        # rows
+        sw, tag = (sw3, "k_sweep3") if sw3 else (sw7, "k_sweep7")
+        sw_ms = sw["avg_ms"] + bcl.get("avg_ms", 0.0)
         by = SWEEP_BYTES_PER_NODE * N
         ach = by / (sw_ms * 1e-3) / 1e9
-        tr = committed_traffic(n, "k_sweep7_L0")
+        tr = committed_traffic(n, tag + "_L0")
+        what = ("k_sweep3<1..3> (one launch per velocity component, both colours through LDS, P with vz; "
+                "(x, y) tiles of z-slabs marching through the planes)" if sw3 else
+                "k_sweep7 (the seven colour phases in one launch, (x, y) tiles marching through the planes in LDS)")
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=tr,
-                    kernel=("level-0 FP32 smoother sweep: k_sweep7 (the seven colour phases in one launch, "
-                            "(x, y) tiles marching through the planes in LDS) + k_bc_chain<float>"),
+                    kernel="level-0 FP32 smoother sweep: " + what + " + k_bc_chain<float>",
                     sweep_ms=round(sw_ms, 5), algorithmic_bytes=by, bytes_per_node=SWEEP_BYTES_PER_NODE,
                     traffic_over_algorithmic=(round(tr / by, 3) if tr else None),
-                    launches={"sweep7": dict(avg_ms=round(sw7["avg_ms"], 5), calls=sw7["calls"]),
+                    launches={tag: dict(avg_ms=round(sw["avg_ms"], 5), calls=sw["calls"]),
                               "bc": dict(avg_ms=round(bcl.get("avg_ms", 0.0), 5), calls=bcl.get("calls", 0))})
     elif slab or (vel and pl):
         # the dominant kernel's unit of work: one level-0 FP32 sweep, i.e.

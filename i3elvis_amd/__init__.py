@@ -281,7 +281,8 @@ class Stokes:
         """nsweeps smoother sweeps (i3g_stokes_smooth; variant 0 the
         context's default, 1 the seven phase launches, 2 one launch per
         velocity component, 3 the seven phases in z-slabs, 4 the whole FP32
-        sweep in one launch on any level)"""
+        sweep in one launch on any level, 5 the FP32 sweep in three launches,
+        one per velocity component, on any level)"""
         out = [np.array(a, np.float64, copy=True) for a in x]
         keep = [np.ascontiguousarray(a, np.float64) for a in b]
         xp = (C.c_void_p * 4)(*[a.ctypes.data for a in out])

@@ -323,6 +323,19 @@ struct i3g_ctx {
 	 * leave it bound by the phases' latency chains, not by memory) */
 	int sweep7 = 0;
 	int sweep7_min_tiles = 128;
+	/* whole FP32 sweeps in three launches (k_sweep3<1..3>, one per velocity
+	 * component, both colours each, P with vz) on levels that give at least
+	 * sweep3_min_groups work-groups, when I3G_SWEEP3=1
+	 * (I3G_SWEEP3_MIN_GROUPS sets the threshold, I3G_SWEEP3_NZS the
+	 * z-slabs).  Off by default: 1.16 ms a sweep at 257^3 against 0.92 ms
+	 * for the z-slabbed phase launches; its counters show it bound by
+	 * vector-ALU issue (about 500 instructions per wave and plane step:
+	 * the ring and prefetch address arithmetic outweighs the relaxation
+	 * itself) */
+	int sweep3 = 0;
+	int sweep3_min_groups = 256;
+	int sweep3_nzs = 0;          /* 0: as many slabs as fill two groups per CU */
+	size_t sweep3_lds = 0;
 	size_t sweep7_lds = 0;       /* LDS opt-in granted to k_sweep7 */
 	int fused_minz = 64; /* levels with fewer planes keep the phase launches (I3G_FUSED_MINZ) */
 	/* learned start and start combination work buffers (kept across calls) */
@@ -828,6 +841,12 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		if (e) c->fused = e[0] == '1';
 		e = getenv("I3G_SWEEP7");
 		if (e) c->sweep7 = e[0] == '1';
+		e = getenv("I3G_SWEEP3");
+		if (e) c->sweep3 = e[0] == '1';
+		e = getenv("I3G_SWEEP3_MIN_GROUPS");
+		if (e) c->sweep3_min_groups = atoi(e);
+		e = getenv("I3G_SWEEP3_NZS");
+		if (e) c->sweep3_nzs = atoi(e);
 		e = getenv("I3G_SWEEP7_MIN_TILES");
 		if (e) c->sweep7_min_tiles = atoi(e);
 		e = getenv("I3G_FUSED_MINZ");
@@ -1186,6 +1205,70 @@ static void sweep7(i3g_ctx *c, int l, const float *xin, float *xout, const float
 	}
 }
 
+/* k_sweep3 tiling of a level: tiles of at most SW3_TX x SW3_TY lines and
+ * z-slabs of at least 8 planes, as many as fill two work-groups per
+ * compute unit; false when the level keeps the phase launches */
+static bool sw3_geom(i3g_ctx *c, int l, Sw3Geom *G)
+{
+	const Level &L = c->L[l];
+	if (!c->sweep3 || L.nz > 4096 || L.nx < 3 || L.ny < 3 || L.nz < 3) return false;
+	int ntx = (L.nx + SW3_TX - 1) / SW3_TX, nty = (L.ny + SW3_TY - 1) / SW3_TY;
+	int nzs = c->sweep3_nzs > 0 ? c->sweep3_nzs : 512 / (ntx * nty);
+	if (nzs > L.nz / 8) nzs = L.nz / 8;
+	if (nzs < 1) nzs = 1;
+	if (ntx * nty * nzs < c->sweep3_min_groups) return false;
+	size_t lds = sw3_lds_bytes(3, L.nz);
+	if (sw3_lds_bytes(1, L.nz) > lds) lds = sw3_lds_bytes(1, L.nz);
+	if (sw3_lds_bytes(2, L.nz) > lds) lds = sw3_lds_bytes(2, L.nz);
+	if (lds > c->sweep3_lds) {
+		int dev = 0, lim = 0;
+		if (hipGetDevice(&dev) != hipSuccess ||
+		    hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+		    (size_t)lim < lds ||
+		    hipFuncSetAttribute((const void *)k_sweep3<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+			    hipSuccess ||
+		    hipFuncSetAttribute((const void *)k_sweep3<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+			    hipSuccess ||
+		    hipFuncSetAttribute((const void *)k_sweep3<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+			    hipSuccess) {
+			hipGetLastError();
+			c->sweep3 = 0;
+			return false;
+		}
+		c->sweep3_lds = lds;
+	}
+	G->ntx = ntx;
+	G->nty = nty;
+	G->nzs = nzs;
+	G->remap = c->xcd_remap ? 1 : 0;
+	return true;
+}
+
+/* One FP32 sweep xin -> xout in three launches (k_sweep3<1..3>), then the
+ * BC rows of xout.  Equals sweep() on xin to the bit. */
+static void sweep3(i3g_ctx *c, int l, const float *xin, float *xout, const float *b, const i3g_stokes_params *p,
+		   const Sw3Geom &G)
+{
+	Level &L = c->L[l];
+	int total = G.ntx * G.nty * G.nzs;
+	dim3 grid = G.remap ? dim3((unsigned)(((total + 7) / 8) * 8)) : dim3((unsigned)total);
+	LevelView<float> v = view<float>(L);
+	float v0 = (float)p->v0koef, p0 = (float)p->p0koef;
+	{
+		Timed tm(c, l == 0 ? "sweep3_f32_L0" : "sweep3_coarse", 0);
+		hipLaunchKernelGGL(k_sweep3<1>, grid, dim3(SW3_THREADS), sw3_lds_bytes(1, L.nz), c->s, v, xin,
+				   (const float *)xout, xout, b, G, v0, p0);
+		hipLaunchKernelGGL(k_sweep3<2>, grid, dim3(SW3_THREADS), sw3_lds_bytes(2, L.nz), c->s, v, xin,
+				   (const float *)xout, xout, b, G, v0, p0);
+		hipLaunchKernelGGL(k_sweep3<3>, grid, dim3(SW3_THREADS), sw3_lds_bytes(3, L.nz), c->s, v, xin,
+				   (const float *)xout, xout, b, G, v0, p0);
+	}
+	{
+		Timed tm(c, l == 0 ? "bc_f32_L0" : "bc_coarse", 0);
+		bc_enforce<float>(c, L, xout, 1);
+	}
+}
+
 /* ns sweeps on the level's unknowns: pairs of out-of-place sweeps through
  * the second vector (so the result lands back in x), an odd one in place */
 template <typename T> static void smooth(i3g_ctx *c, int l, int ns, const i3g_stokes_params *p)
@@ -1195,10 +1278,16 @@ template <typename T> static void smooth(i3g_ctx *c, int l, int ns, const i3g_st
 	int s = 0;
 	if constexpr (sizeof(T) == 4) {
 		Sw7Geom G;
+		Sw3Geom G3;
 		if (sw7_geom(c, l, &G))
 			for (; s + 2 <= ns; s += 2) {
 				sweep7(c, l, x, xb, b, p, G);
 				sweep7(c, l, xb, x, b, p, G);
+			}
+		else if (sw3_geom(c, l, &G3))
+			for (; s + 2 <= ns; s += 2) {
+				sweep3(c, l, x, xb, b, p, G3);
+				sweep3(c, l, xb, x, b, p, G3);
 			}
 	}
 	if (c->fused && L.nz >= c->fused_minz)
@@ -2117,8 +2206,16 @@ static int smooth_impl(i3g_ctx *c, int level, int nsweeps, int variant, double *
 	int rc = smooth_in<T>(c, L, x, b);
 	if (rc) return rc;
 	int keep = c->fused, keepz = c->fused_minz, keeps = c->slab, keepa = c->slab_all;
-	int keep7 = c->sweep7, keept = c->sweep7_min_tiles;
-	if (variant >= 1 && variant <= 3) c->sweep7 = 0;
+	int keep7 = c->sweep7, keept = c->sweep7_min_tiles, keep3 = c->sweep3, keepg = c->sweep3_min_groups;
+	if (variant >= 1 && variant <= 3) c->sweep7 = c->sweep3 = 0;
+	if (variant == 4) c->sweep3 = 0;
+	if (variant == 5) {
+		c->sweep7 = 0;
+		c->sweep3 = 1;
+		c->sweep3_min_groups = 1;
+		c->fused = 0;
+		c->slab = 0;
+	}
 	if (variant == 1) { c->fused = 0; c->slab = 0; }
 	if (variant == 2) { c->fused = 1; c->fused_minz = 0; }
 	if (variant == 3) {
@@ -2139,6 +2236,8 @@ static int smooth_impl(i3g_ctx *c, int level, int nsweeps, int variant, double *
 	c->slab_all = keepa;
 	c->sweep7 = c->sweep7 ? keep7 : 0; /* a refused LDS opt-in stays off */
 	c->sweep7_min_tiles = keept;
+	c->sweep3 = c->sweep3 ? keep3 : 0;
+	c->sweep3_min_groups = keepg;
 	return smooth_out<T>(c, L, x);
 }
 
@@ -2146,7 +2245,7 @@ extern "C" int i3g_stokes_smooth(i3g_ctx *c, int level, int precision, int nswee
 				 const double *const *b, double v0koef, double p0koef)
 {
 	if (!c || !c->coef_ready || !x || !b || level < 0 || level >= c->nlev || nsweeps < 0 || variant < 0 ||
-	    variant > 4 || (precision != 32 && precision != 64))
+	    variant > 5 || (precision != 32 && precision != 64))
 		return I3G_EINVAL;
 	i3g_stokes_params p;
 	i3g_stokes_default_params(&p);
@@ -2783,6 +2882,7 @@ extern "C" int i3g_kernel_stats(i3g_ctx *c, char *names, int name_len, double *a
 				double *bytes, int cap)
 {
 	if (!c) return I3G_EINVAL;
+	flush_timing(c);
 	int q = 0;
 	for (auto &kv : c->stats) {
 		if (q >= cap) break;

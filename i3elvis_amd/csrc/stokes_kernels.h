@@ -884,6 +884,324 @@ k_sweep7(LevelView<float> L, const float *__restrict__ xin, float *__restrict__ 
 	}
 }
 
+/* ---- one launch per component: both colours through LDS ------------------ */
+
+/* One FP32 sweep in three launches: L = 1 relaxes vx (colour 0, then 1),
+ * L = 2 vy, L = 3 vz and then P; equal to the seven phase launches to the
+ * bit.  Launch L reads the sweep's input xa for its own component, P and
+ * the later components, and xb (where the earlier launches wrote) for the
+ * earlier components; it writes its component(s) to xb.
+ *
+ * A work-group owns an (x, y) tile of at most SW3_TX x SW3_TY nodes and a
+ * z-slab, and marches up one plane per step: colour 0 at plane kk, colour 1
+ * (and P for L = 3) at kk-1.  A second-colour row reads the first colour one
+ * plane up and down, so one plane of lag is enough; P reads vz at its plane
+ * and the one below.  Colour 0 runs on the tile plus the ring that colour 1
+ * reads (one line on every side; for L = 3 two below and one above, as P
+ * reads vz at (i-1, j-1)), and on one plane below and above the slab (two
+ * below for L = 3); those values are the neighbouring work-groups' own,
+ * computed by the same function from the same inputs.  The operands of the
+ * staged region (tile + 2 lines around, 3 below for L = 3) live in LDS in
+ * one ring of planes per field read by the launch; the next plane, the
+ * right-hand sides and the flags of the next step are loaded into
+ * registers while the current step computes, and the barriers wait for LDS
+ * only. */
+#define SW3_TX 23
+#define SW3_TY 23
+#define SW3_RX (SW3_TX + 5)
+#define SW3_RY (SW3_TY + 5)
+#define SW3_PL (SW3_RX * SW3_RY)
+#define SW3_THREADS 512
+#define SW3_IT 2 /* nodes a thread relaxes per phase at most */
+
+/* ring depth of field f (0 P, 1 vx, 2 vy, 3 vz, 4 eC, 5 eXY, 6 eXZ, 7 eYZ;
+ * 0: not read) in launch L: at least the planes read below the step's plane
+ * kk, the planes read above (sw3_up) and kk itself, rounded up to a power of
+ * two so that a plane's slot is a mask, not a division */
+__host__ __device__ constexpr int sw3_depth(int L, int f)
+{
+	return L == 1 ? (f == 0 ? 2 : f == 1 ? 4 : f == 2 ? 2 : f == 3 ? 4 : f == 4 ? 2 : f == 5 ? 2 : f == 6 ? 4 : 0)
+	     : L == 2 ? (f == 0 ? 2 : f == 1 ? 2 : f == 2 ? 4 : f == 3 ? 4 : f == 4 ? 2 : f == 5 ? 2 : f == 6 ? 0 : 4)
+		      : (f == 0 ? 4 : f == 1 ? 4 : f == 2 ? 4 : f == 3 ? 4 : f == 4 ? 4 : f == 5 ? 0 : f == 6 ? 2 : 2);
+}
+/* planes read below kk: depth - up - 1 before the rounding */
+__host__ __device__ constexpr int sw3_dn(int L, int f)
+{
+	return L == 1 ? (f == 1 ? 2 : f == 2 || f == 3 || f == 5 || f == 6 ? 1 : 0)
+	     : L == 2 ? (f == 2 ? 2 : f == 1 || f == 3 || f == 5 || f == 7 ? 1 : 0)
+		      : (f == 1 || f == 2 || f == 3 ? 2 : f == 0 || f == 4 || f == 6 || f == 7 ? 1 : 0);
+}
+__host__ __device__ constexpr int sw3_up(int L, int f)
+{
+	return L == 1 ? (f == 0 || f == 1 || f == 3 || f == 4 || f == 6)
+	     : L == 2 ? (f == 0 || f == 2 || f == 3 || f == 4 || f == 7)
+		      : (f == 0 || f == 3 || f == 4);
+}
+__host__ __device__ constexpr int sw3_base(int L, int f)
+{
+	return (f > 0 ? sw3_depth(L, 0) : 0) + (f > 1 ? sw3_depth(L, 1) : 0) + (f > 2 ? sw3_depth(L, 2) : 0) +
+	       (f > 3 ? sw3_depth(L, 3) : 0) + (f > 4 ? sw3_depth(L, 4) : 0) + (f > 5 ? sw3_depth(L, 5) : 0) +
+	       (f > 6 ? sw3_depth(L, 6) : 0) + (f > 7 ? sw3_depth(L, 7) : 0);
+}
+__host__ __device__ constexpr int sw3_slots(int L) { return sw3_base(L, 8); }
+__device__ __forceinline__ int sw3_slot(int L, int f, int pl)
+{
+	return (sw3_base(L, f) + (int)((unsigned)pl & (unsigned)(sw3_depth(L, f) - 1))) * SW3_PL;
+}
+/* region margins of phase q (x below, x above, y below, y above), lags and
+ * relaxed component; the staged region starts sw3_h(L) lines below */
+__host__ __device__ constexpr int sw3_h(int L) { return L == 3 ? 3 : 2; }
+__host__ __device__ constexpr int sw3_nq(int L) { return L == 3 ? 3 : 2; }
+__host__ __device__ constexpr int sw3_lag(int q) { return q == 0 ? 0 : 1; }
+__host__ __device__ constexpr int sw3_m0(int L, int q) { return q == 0 ? (L == 3 ? 2 : 1) : q == 1 ? (L == 3 ? 1 : 0) : 0; }
+__host__ __device__ constexpr int sw3_m1(int L, int q) { return q == 0 ? 1 : 0; }
+
+struct Sw3Acc {
+	const float *s;
+	int L, o, li, lj, pl;
+	const float *gxh, *gxc, *gyh, *gyc, *gzh, *gzc;
+	/* o is the node's offset less one line and one row, so that every
+	 * stencil offset below is a non-negative constant (an LDS immediate) */
+	__device__ __forceinline__ float at(int f, int di, int dj, int dk) const
+	{
+		const float *base = s + sw3_slot(L, f, pl + dk) + o;
+		return base[(di + 1) * SW3_RY + (dj + 1)];
+	}
+	template <int C> __device__ __forceinline__ float u(int di, int dj, int dk) const { return at(C, di, dj, dk); }
+	__device__ __forceinline__ float eC(int di, int dj, int dk) const { return at(4, di, dj, dk); }
+	__device__ __forceinline__ float eXY(int di, int dj, int dk) const { return at(5, di, dj, dk); }
+	__device__ __forceinline__ float eXZ(int di, int dj, int dk) const { return at(6, di, dj, dk); }
+	__device__ __forceinline__ float eYZ(int di, int dj, int dk) const { return at(7, di, dj, dk); }
+	__device__ __forceinline__ float ihx(int d) const { return gxh[li + d]; }
+	__device__ __forceinline__ float icx(int d) const { return gxc[li + d]; }
+	__device__ __forceinline__ float ihy(int d) const { return gyh[lj + d]; }
+	__device__ __forceinline__ float icy(int d) const { return gyc[lj + d]; }
+	__device__ __forceinline__ float ihz(int d) const { return gzh[pl + d]; }
+	__device__ __forceinline__ float icz(int d) const { return gzc[pl + d]; }
+};
+
+struct Sw3Geom {
+	int ntx, nty, nzs; /* tiles in x, y; slabs in z */
+	int remap;         /* XCD-contiguous work order */
+};
+
+static inline size_t sw3_lds_bytes(int L, int nz)
+{
+	return sizeof(float) * ((size_t)sw3_slots(L) * SW3_PL + 2 * SW3_RX + 2 * SW3_RY + 2 * (size_t)nz);
+}
+
+template <int L>
+__global__ void __launch_bounds__(SW3_THREADS, 4)
+k_sweep3(LevelView<float> V, const float *__restrict__ xa, const float *__restrict__ xb_in, float *__restrict__ xout,
+	 const float *__restrict__ b, Sw3Geom G, float v0koef, float p0koef)
+{
+	extern __shared__ __align__(16) unsigned char lds_raw[];
+	float *s = (float *)lds_raw;
+	float *gxh = s + sw3_slots(L) * SW3_PL, *gxc = gxh + SW3_RX, *gyh = gxc + SW3_RX, *gyc = gyh + SW3_RY;
+	float *gzh = gyc + SW3_RY, *gzc = gzh + V.nz;
+	constexpr int NQ = sw3_nq(L);
+	int w, dummy;
+	if (!block_work(G.remap, G.ntx * G.nty * G.nzs, 1, 0, &w, &dummy)) return;
+	const int tx = w % G.ntx, rest = w / G.ntx;
+	const int ty = rest % G.nty, sl = rest / G.nty;
+	const int nx = V.nx, ny = V.ny, nz = V.nz;
+	const long N = V.N, sz = (long)nx * ny;
+	const int i0 = (int)((long)tx * nx / G.ntx), i1 = (int)((long)(tx + 1) * nx / G.ntx);
+	const int j0 = (int)((long)ty * ny / G.nty), j1 = (int)((long)(ty + 1) * ny / G.nty);
+	const int k0 = (int)((long)sl * nz / G.nzs), k1 = (int)((long)(sl + 1) * nz / G.nzs);
+	const int rx0 = i0 - sw3_h(L), ry0 = j0 - sw3_h(L);
+	const int t = threadIdx.x;
+
+	for (int q = t; q < SW3_RX; q += SW3_THREADS) {
+		int i = rx0 + q;
+		bool in = i >= 0 && i < nx;
+		gxh[q] = in ? V.ihx[i] : 0.f;
+		gxc[q] = in ? V.icx[i] : 0.f;
+	}
+	for (int q = t; q < SW3_RY; q += SW3_THREADS) {
+		int j = ry0 + q;
+		bool in = j >= 0 && j < ny;
+		gyh[q] = in ? V.ihy[j] : 0.f;
+		gyc[q] = in ? V.icy[j] : 0.f;
+	}
+	for (int q = t; q < nz; q += SW3_THREADS) {
+		gzh[q] = V.ihz[q];
+		gzc[q] = V.icz[q];
+	}
+
+	/* this thread's nodes of each phase: region offsets li << 8 | lj (or
+	 * 0xffff: none) for even planes in the low and odd planes in the high
+	 * 16 bits */
+	unsigned code[NQ][SW3_IT];
+#pragma unroll
+	for (int q = 0; q < NQ; q++) {
+		int xa0 = max(i0 - sw3_m0(L, q), 0), xb0 = min(i1 + sw3_m1(L, q), nx);
+		int ya0 = max(j0 - sw3_m0(L, q), 0), yb0 = min(j1 + sw3_m1(L, q), ny);
+		int rows = xb0 - xa0;
+#pragma unroll
+		for (int it = 0; it < SW3_IT; it++) code[q][it] = 0;
+#pragma unroll
+		for (int par = 0; par < 2; par++)
+#pragma unroll
+			for (int it = 0; it < SW3_IT; it++) {
+				int r = t + it * SW3_THREADS;
+				int i, j;
+				bool ok;
+				if (q == 2) {
+					int wdt = yb0 - ya0, ii = r / wdt;
+					i = xa0 + ii;
+					j = ya0 + (r - ii * wdt);
+					ok = ii < rows;
+				} else {
+					int hw = (yb0 - ya0 + 1) >> 1, ii = r / hw;
+					i = xa0 + ii;
+					j = ya0 + ((ya0 + i + par + q) & 1) + 2 * (r - ii * hw);
+					ok = ii < rows && j < yb0;
+				}
+				code[q][it] |= (ok ? (unsigned)((i - rx0) << 8 | (j - ry0)) : 0xffffu) << (16 * par);
+			}
+	}
+
+	/* region node of this thread for the plane loads (up to two) */
+	auto src = [&](int f) -> const float * {
+		switch (f) {
+		case 0: return xa;
+		case 1: return L >= 2 ? xb_in + N : xa + N;
+		case 2: return L >= 3 ? xb_in + 2 * N : xa + 2 * N;
+		case 3: return xa + 3 * N;
+		case 4: return V.eC;
+		case 5: return V.eXY;
+		case 6: return V.eXZ;
+		default: return V.eYZ;
+		}
+	};
+	long gn[2];
+	bool lin[2];
+#pragma unroll
+	for (int e = 0; e < 2; e++) {
+		int r = t + e * SW3_THREADS;
+		int lrx = r / SW3_RY, lry = r - lrx * SW3_RY;
+		int gi = rx0 + lrx, gj = ry0 + lry;
+		lin[e] = r < SW3_PL && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+		gn[e] = (long)gi * ny + gj;
+	}
+
+	const int kb = k0 - (L == 3 ? 2 : 1); /* first step: colour 0 on the plane(s) below the slab */
+	const int ke = k1;                     /* last step: colour 1 / P on the slab's top plane */
+	/* planes of the first step */
+#pragma unroll
+	for (int f = 0; f < 8; f++) {
+		if (!sw3_depth(L, f)) continue;
+		for (int pl = max(kb - sw3_dn(L, f), 0); pl <= kb + sw3_up(L, f) && pl < nz; pl++)
+#pragma unroll
+			for (int e = 0; e < 2; e++)
+				if (lin[e]) s[sw3_slot(L, f, pl) + t + e * SW3_THREADS] = src(f)[pl * sz + gn[e]];
+	}
+
+	/* right-hand sides and flags of a step's nodes, loaded a step ahead */
+	float rhs[NQ][SW3_IT], nrhs[NQ][SW3_IT];
+	unsigned flg[NQ][SW3_IT], nflg[NQ][SW3_IT];
+	auto fetch = [&](int kk, float (*rr)[SW3_IT], unsigned (*ff)[SW3_IT]) {
+#pragma unroll
+		for (int q = 0; q < NQ; q++) {
+			int pl = kk - sw3_lag(q);
+			int par = pl & 1;
+#pragma unroll
+			for (int it = 0; it < SW3_IT; it++) {
+				unsigned cd = (code[q][it] >> (16 * par)) & 0xffffu;
+				asm volatile("" : "+v"(cd));
+				rr[q][it] = 0.f;
+				ff[q][it] = 0;
+				if (cd != 0xffffu && pl >= 0 && pl < nz) {
+					long n = pl * sz + (long)(rx0 + (cd >> 8)) * ny + (ry0 + (cd & 255));
+					ff[q][it] = V.flags[n];
+					if (b) rr[q][it] = b[(q == 2 ? 0 : L) * N + n];
+				}
+			}
+		}
+	};
+	fetch(kb, rhs, flg);
+	__syncthreads();
+
+	for (int kk = kb; kk <= ke; kk++) {
+		fetch(kk + 1, nrhs, nflg);
+		float nv[8][2];
+#pragma unroll
+		for (int f = 0; f < 8; f++)
+#pragma unroll
+			for (int e = 0; e < 2; e++) {
+				int pl = kk + 1 + sw3_up(L, f);
+				nv[f][e] = (sw3_depth(L, f) && lin[e] && pl < nz) ? src(f)[pl * sz + gn[e]] : 0.f;
+			}
+#pragma unroll
+		for (int q = 0; q < NQ; q++) {
+			const int C = q == 2 ? 0 : L;
+			const int pl = kk - sw3_lag(q);
+			/* colour 0 on the slab's planes and the halo planes around
+			 * it; colour 1 and P on the slab's own planes, and for L = 3
+			 * colour 1 also on the plane below (P reads vz there) */
+			const int lo = q == 0 ? kb : (q == 1 && L == 3 ? k0 - 1 : k0);
+			const int hi = q == 0 ? ke : k1 - 1;
+			if (pl >= lo && pl <= hi && pl >= 0 && pl < nz) {
+				const int par = pl & 1;
+#pragma unroll
+				for (int it = 0; it < SW3_IT; it++) {
+					unsigned cd = (code[q][it] >> (16 * par)) & 0xffffu;
+					asm volatile("" : "+v"(cd));
+					if (cd == 0xffffu) continue;
+					int li = cd >> 8, lj = cd & 255;
+					Sw3Acc a{s, L, (li - 1) * SW3_RY + lj - 1, li, lj, pl, gxh, gxc, gyh, gyc, gzh, gzc};
+					unsigned fl = flg[q][it];
+					bool upd = (fl & FL_EX(C)) && !(fl & FL_BC(C));
+					if (q == 2) {
+						/* P: every node of the tile; the new value goes
+						 * straight out (no later phase reads it) */
+						float v = a.at(0, 0, 0, 0);
+						if (upd) v = p_relax_a<float>(a, v, rhs[q][it], p0koef);
+						xout[pl * sz + (long)(rx0 + li) * ny + (ry0 + lj)] = v;
+					} else if (upd) {
+						float v;
+						if (L == 1) v = vel_relax_a<float, 1>(a, rhs[q][it], v0koef);
+						else if (L == 2) v = vel_relax_a<float, 2>(a, rhs[q][it], v0koef);
+						else v = vel_relax_a<float, 3>(a, rhs[q][it], v0koef);
+						s[sw3_slot(L, C, pl) + li * SW3_RY + lj] = v;
+					}
+				}
+			}
+			sw7_sync();
+		}
+		/* the tile's component at plane kk-1 is final: out (its ring slot is
+		 * not the one the loads below overwrite) */
+		if (kk - 1 >= k0 && kk - 1 < k1) {
+			const int pl = kk - 1, wdt = j1 - j0, cnt = (i1 - i0) * wdt;
+			for (int r = t; r < cnt; r += SW3_THREADS) {
+				int ii = r / wdt, i = i0 + ii, j = j0 + (r - ii * wdt);
+				xout[L * N + pl * sz + (long)i * ny + j] =
+					s[sw3_slot(L, L, pl) + (i - rx0) * SW3_RY + (j - ry0)];
+			}
+		}
+#pragma unroll
+		for (int f = 0; f < 8; f++)
+#pragma unroll
+			for (int e = 0; e < 2; e++) {
+				int pl = kk + 1 + sw3_up(L, f);
+				if (sw3_depth(L, f) && lin[e] && pl < nz) s[sw3_slot(L, f, pl) + t + e * SW3_THREADS] = nv[f][e];
+			}
+#pragma unroll
+		for (int q = 0; q < NQ; q++)
+#pragma unroll
+			for (int it = 0; it < SW3_IT; it++) {
+				rhs[q][it] = nrhs[q][it];
+				flg[q][it] = nflg[q][it];
+				/* wait for them here, where they have long arrived: their
+				 * loads are conditional, so a wait placed at their use in
+				 * the next step would also drain that step's prefetch */
+				asm volatile("" : "+v"(rhs[q][it]), "+v"(flg[q][it]));
+			}
+		sw7_sync();
+	}
+}
+
 /* Boundary rows x_u = c + k x_partner for the records [start, end) of one
  * dependency depth (partners of depth-d rows are interior or depth < d). */
 template <typename T>

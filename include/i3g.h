@@ -276,7 +276,9 @@ int i3g_stokes_prolong(i3g_ctx *ctx, int level, const double *const *coarse,
  * 4 the whole FP32 sweep in one launch (k_sweep7: (x, y) tiles marching
  * up through the planes with the phases lagging 0,1,1,2,2,3,3 planes, out
  * of place in pairs; on every level, however few its tiles; FP64 and an
- * odd last sweep as phases).
+ * odd last sweep as phases), 5 the FP32 sweep in three launches
+ * (k_sweep3: one per velocity component, both colours through LDS, P with
+ * vz; on every level; pairs out of place).
  * precision 64 or 32 (the planes are rounded to float on the
  * device, and back).  Both need i3g_set_coefficients first and overwrite
  * the level's multigrid work buffers. */

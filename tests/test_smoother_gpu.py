@@ -10,8 +10,9 @@ order than the reference's), FP32 to 1e-4 (the residual of a random state
 in float).  The single-pass sweep (one launch per velocity component, out
 of place), the z-slab schedule of the seven phases and the one-launch
 sweep (k_sweep7: (x, y) tiles marching through the planes, phases lagging
-0,1,1,2,2,3,3 planes, halo lines recomputed) against the seven phase
-launches: equal to the bit.
+0,1,1,2,2,3,3 planes, halo lines recomputed) and the three-launch sweep
+(k_sweep3: one launch per velocity component, both colours through LDS)
+against the seven phase launches: equal to the bit.
 """
 import numpy as np
 import pytest
@@ -268,6 +269,64 @@ def test_sweep7_vcycles_match_phase_launch_vcycles(gpu, monkeypatch):
         S.set_solution(z, z, z, z)
         st = S.solve(precision=32, rtol=1e-8, max_outer=1, max_vcycles=5, check_conv=False)
         print(f"case: sweep7={mode} vcycles={st.vcycles} rel={st.rel_res:.3e}", flush=True)
+        sols.append(S.get_solution())
+        S.close()
+    for c, (a, b) in enumerate(zip(*sols)):
+        assert np.array_equal(a, b), (c, np.abs(a - b).max())
+
+
+@pytest.mark.parametrize("case,nzs", [("g1", 1), ("g1", 3), ("sub65", 1), ("sub65", 2), ("sub65_level1", 2),
+                                      ("sub_2x3", 1), ("c4_49x45x21", 2), ("c4_49x45x21", 21)])
+def test_sweep3_matches_phase_launches(gpu, monkeypatch, case, nzs):
+    """k_sweep3 (variant 5: three launches a FP32 sweep, out of place in
+    pairs) equals the seven phase launches plus BC rows to the bit: one tile
+    over the whole plane (g1, the 35^3 level 1), uneven tiles of 24 x 28
+    lines (65^3, 49x45x21), z-slabs of 1, 3, 12 or more planes, and 2, 4
+    and 3 sweeps (an odd last sweep runs as phases)."""
+    if case == "g1":
+        cfg = _cfg_from_g1(golden("g1_operator.npz"))
+        level = 0
+    elif case.startswith("sub65"):
+        cfg = I.make_config(2, 65, 65, 65, seed=3)
+        level = 1 if case.endswith("level1") else 0
+    elif case == "sub_2x3":
+        cfg = I.make_config(2, 33, 51, 17, seed=5)
+        level = 0
+    else:
+        cfg = I.make_config(2, 49, 45, 21, seed=6)
+        level = 0
+    monkeypatch.setenv("I3G_SWEEP3_NZS", str(nzs))
+    print(f"case: sweep3 {case} level {level} nzs {nzs}", flush=True)
+    S = I.Stokes(cfg)
+    S.set_coefficients()
+    Nl = S.level_nodes(level)
+    x = _random_state(Nl, 41)
+    b = _random_state(Nl, 42)
+    for ns in (2, 4, 3):
+        a = S.smooth(level, 32, ns, x, b, 0.95, 0.7, variant=5)
+        r = S.smooth(level, 32, ns, x, b, 0.95, 0.7, variant=1)
+        for c in range(4):
+            assert np.array_equal(a[c], r[c]), (ns, c, np.abs(a[c] - r[c]).max())
+        assert any(not np.array_equal(a[c], x[c]) for c in range(4))
+    S.close()
+
+
+def test_sweep3_vcycles_match_phase_launch_vcycles(gpu, monkeypatch):
+    """Whole mixed-precision V-cycles with k_sweep3 on every level
+    (I3G_SWEEP3_MIN_GROUPS=1) against the phase launches (I3G_SWEEP3=0): the
+    fields after a few V-cycles are equal to the bit."""
+    n = 41
+    cfg = I.make_config(2, n, n, n, seed=7)
+    sols = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("I3G_SWEEP3", mode)
+        monkeypatch.setenv("I3G_SWEEP3_MIN_GROUPS", "1")
+        S = I.Stokes(cfg)
+        S.set_coefficients()
+        z = np.zeros(n ** 3)
+        S.set_solution(z, z, z, z)
+        st = S.solve(precision=32, rtol=1e-8, max_outer=1, max_vcycles=5, check_conv=False)
+        print(f"case: sweep3={mode} vcycles={st.vcycles} rel={st.rel_res:.3e}", flush=True)
         sols.append(S.get_solution())
         S.close()
     for c, (a, b) in enumerate(zip(*sols)):
