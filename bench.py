@@ -313,6 +313,10 @@ def time_loop(I, np, steps=3, cap_s=90.0, nx=257, ny=257, nz=129, per_axis=2, le
                 steps=len(rows), steps_per_s=(round(1.0 / sec, 4) if sec else None), mean_step_s=sec,
                 stage_mean_s={k: mean(k) for k in ("t_stokes", "t_heat", "t_mktk", "t_rk4", "t_sort", "t_transfer")},
                 stage_gbs_algorithmic=gbs, vcycles=[r["vcycles"] for r in rows],
+                learned_start=("network weights " + os.path.relpath(learned.path, ROOT) +
+                               "; per step 1 = the learned start won over the previous step's solution"
+                               if learned is not None else None),
+                learned_taken=[r["learned_taken"] for r in rows] if learned is not None else None,
                 rel_res=[r["rel_res"] for r in rows], timestep=[r["timestep"] for r in rows],
                 setup_s=round(setup_s, 2), init_transfer_s=round(t1 - t0, 3))
 
@@ -527,7 +531,13 @@ def main():
     if a.loop_steps > 0 and not a.no_extras:
         S.close()
         S = None
-        loop = time_loop(I, np, steps=a.loop_steps)
+        # the learned start (previous step's solution as the fallback), as
+        # the time loop of configuration 4 asks
+        wl = None
+        if ws is not None:
+            wl = W.WarmStart.load(wpath, shape=(129, 257, 257))
+            wl.path = wpath
+        loop = time_loop(I, np, steps=a.loop_steps, learned=wl)
 
     cpu = None
     if not a.no_cpu:
