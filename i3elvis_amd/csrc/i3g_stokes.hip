@@ -1863,6 +1863,7 @@ static int gcr(i3g_ctx *c, GcrHost &G, R *r, R *e, double target, int max_new, c
 		/* classical Gram-Schmidt against q_0..q_{k-1}; a second pass only
 		 * when the first removed more than half of ||q||^2 */
 		int again = 0;
+		Timed tgs(c, "gcr_orthogonalise", 0);
 		if (k > 0) {
 			multidot_dev<T, T>(c, (const T *const *)(dQ + k), 1, (const T *)qk, n, sc + 3);
 			multidot_dev<T, T>(c, (const T *const *)dQ, k, (const T *)qk, n, h);
