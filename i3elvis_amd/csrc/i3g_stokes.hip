@@ -323,6 +323,9 @@ struct i3g_ctx {
 	 * leave it bound by the phases' latency chains, not by memory) */
 	int sweep7 = 0;
 	int sweep7_min_tiles = 128;
+	/* the mixed V-cycle's coarsest level solved in FP64 (I3G_COARSEST64=0:
+	 * in the V-cycle's FP32) */
+	int coarsest64 = 1;
 	/* whole FP32 sweeps in three launches (k_sweep3<1..3>, one per velocity
 	 * component, both colours each, P with vz) on levels that give at least
 	 * sweep3_min_groups work-groups, when I3G_SWEEP3=1
@@ -841,6 +844,8 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		if (e) c->fused = e[0] == '1';
 		e = getenv("I3G_SWEEP7");
 		if (e) c->sweep7 = e[0] == '1';
+		e = getenv("I3G_COARSEST64");
+		if (e) c->coarsest64 = e[0] != '0';
 		e = getenv("I3G_SWEEP3");
 		if (e) c->sweep3 = e[0] == '1';
 		e = getenv("I3G_SWEEP3_MIN_GROUPS");
@@ -1403,6 +1408,21 @@ template <typename T> static void vcycle(i3g_ctx *c, int l, const i3g_stokes_par
 	T *x = lx<T>(L), *b = lb<T>(L);
 	if (l == c->nlev - 1) {
 		int ns = p->nu_coarse > 0 ? p->nu_coarse : 64;
+		if (sizeof(T) == 4 && c->coarsest64) {
+			/* the mixed V-cycle's coarsest solve in FP64: its right-hand
+			 * side and start widened, the sweeps in double, the result
+			 * rounded back */
+			long n4 = 4 * L.N;
+			Timed tm(c, "coarsest_f64", 0);
+			hipLaunchKernelGGL((k_copy_scale<double, T>), lin_grid(n4), dim3(256), 0, c->s, L.b64, (const T *)b, 1.0,
+					   n4);
+			hipLaunchKernelGGL((k_copy_scale<double, T>), lin_grid(n4), dim3(256), 0, c->s, L.x64, (const T *)x, 1.0,
+					   n4);
+			if (!coarsest_launch<double>(c, p, ns)) smooth<double>(c, l, ns, p);
+			hipLaunchKernelGGL((k_copy_scale<T, double>), lin_grid(n4), dim3(256), 0, c->s, x, (const double *)L.x64,
+					   1.0, n4);
+			return;
+		}
 		if (coarsest_launch<T>(c, p, ns)) return;
 		smooth<T>(c, l, ns, p);
 		return;
