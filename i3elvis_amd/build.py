@@ -43,7 +43,8 @@ def build(verbose=False):
         objs.append(o)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs +
-         ["-fopenmp", "-lm"])
+         ["-fopenmp", "-lm", "-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas",
+          "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -16,6 +16,9 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "../../include/i3g.h"
 #include "stokes_kernels.h"
 
@@ -326,6 +329,18 @@ struct i3g_ctx {
 	/* the mixed V-cycle's coarsest level solved in FP64 (I3G_COARSEST64=0:
 	 * in the V-cycle's FP32) */
 	int coarsest64 = 1;
+	/* the coarsest level solved directly in FP64 (I3G_COARSE_DIRECT=1): the
+	 * truncated pseudo-inverse of its assembled operator, singular values
+	 * below cd_tol times the largest dropped (I3G_COARSE_DIRECT_TOL) */
+	int coarse_direct = 0;
+	double cd_tol = 1e-10;
+	int cd_n = 0;                /* unknowns (4 N of the coarsest level) */
+	double *cd_u = nullptr;      /* U, column-major n x n */
+	double *cd_vt = nullptr;     /* V^T, column-major n x n */
+	double *cd_s = nullptr;      /* reciprocal kept singular values (n) */
+	double *cd_y = nullptr;      /* work (n) */
+	int cd_rank = 0;             /* singular values kept */
+	double cd_setup_s = 0;       /* time of the last factorisation */
 	/* whole FP32 sweeps in three launches (k_sweep3<1..3>, one per velocity
 	 * component, both colours each, P with vz) on levels that give at least
 	 * sweep3_min_groups work-groups, when I3G_SWEEP3=1
@@ -844,6 +859,10 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		if (e) c->fused = e[0] == '1';
 		e = getenv("I3G_SWEEP7");
 		if (e) c->sweep7 = e[0] == '1';
+		e = getenv("I3G_COARSE_DIRECT");
+		if (e) c->coarse_direct = e[0] == '1';
+		e = getenv("I3G_COARSE_DIRECT_TOL");
+		if (e && atof(e) > 0) c->cd_tol = atof(e);
 		e = getenv("I3G_COARSEST64");
 		if (e) c->coarsest64 = e[0] != '0';
 		e = getenv("I3G_SWEEP3");
@@ -956,13 +975,127 @@ extern "C" int i3g_destroy(i3g_ctx *c)
 			}
 	void *ps[] = {c->rhs0, c->sol, c->ro, c->nxx, c->nxy, c->nxz, c->nyz, c->partials, c->dscal, c->e64, c->st12,
 		      c->ws_feat, c->ws_y, c->ws_lith,
-		      c->ws_dP, c->ws_dv, c->cb_dirs, c->cb_ys};
+		      c->ws_dP, c->ws_dv, c->cb_dirs, c->cb_ys, c->cd_u, c->cd_vt, c->cd_s, c->cd_y};
 	for (void *p : ps)
 		if (p) hipFree(p);
 	if (c->hscal) hipHostFree(c->hscal);
 	for (auto e : c->evpool) hipEventDestroy(e);
 	if (c->s) hipStreamDestroy(c->s);
 	delete c;
+	return I3G_OK;
+}
+
+/* ---- the coarsest level's direct FP64 solve ---------------------------- */
+
+__global__ void k_unit(double *x, long j, long jprev)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0) {
+		if (jprev >= 0) x[jprev] = 0;
+		x[j] = 1;
+	}
+}
+/* column j of the column-major operator from the residual of e_j (r = -A e_j) */
+__global__ void k_dense_col(double *A, const double *r, long n, long j)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) A[j * n + i] = -r[i];
+}
+/* unknowns that do not exist on the staggered grid: identity rows */
+__global__ void k_dense_ident(double *A, const uint8_t *flags, long N)
+{
+	long u = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (u >= 4 * N) return;
+	int c = (int)(u / N);
+	if (!(flags[u - c * N] & FL_EX(c))) A[u * 4 * N + u] = 1.0;
+}
+/* out[k] = scale[k] * sum_i M[k n + i] v[i] (M column-major: column k of M
+ * against v); one wavefront per k, fixed summation order */
+__global__ void __launch_bounds__(256) k_gemvt(const double *__restrict__ M, const double *__restrict__ v,
+					       const double *__restrict__ scale, double *__restrict__ out, int n)
+{
+	int k = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+	if (k >= n) return;
+	const double *col = M + (long)k * n;
+	double acc = 0;
+	for (int i = lane; i < n; i += 64) acc += col[i] * v[i];
+	for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+	if (lane == 0) out[k] = scale ? scale[k] * acc : acc;
+}
+
+/* Assemble the coarsest level's operator (BC rows x_u - k x_partner,
+ * identity rows for unknowns that do not exist), factor it by SVD
+ * (rocSOLVER, FP64) and keep U, V^T and the reciprocal singular values
+ * above cd_tol times the largest: the V-cycle's coarsest solve is then
+ * x = V S^+ U^T b, the minimum-norm least-squares solution, which leaves
+ * the operator's null and near-null modes (constant pressure, pressure
+ * cells the coarse viscosities decouple) out of the correction. */
+static int coarse_direct_setup(i3g_ctx *c)
+{
+	Level &L = c->L[c->nlev - 1];
+	long n = 4 * L.N;
+	if (n > 16384) return I3G_ECAP;
+	double t0 = (double)clock() / CLOCKS_PER_SEC;
+	int err = 0;
+	if (c->cd_n != (int)n) {
+		for (void *p : {(void *)c->cd_u, (void *)c->cd_vt, (void *)c->cd_s, (void *)c->cd_y})
+			if (p) hipFree(p);
+		c->cd_u = dalloc<double>(n * n, &err);
+		c->cd_vt = dalloc<double>(n * n, &err);
+		c->cd_s = dalloc<double>(n, &err);
+		c->cd_y = dalloc<double>(n, &err);
+		if (err) return err;
+		c->cd_n = (int)n;
+	}
+	double *A = dalloc<double>(n * n, &err);
+	double *E = dalloc<double>(n, &err);
+	int *info = dalloc<int>(1, &err);
+	if (err) return err;
+	HCHK(hipMemsetAsync(L.x64, 0, n * sizeof(double), c->s));
+	for (long j = 0; j < n; j++) {
+		hipLaunchKernelGGL(k_unit, dim3(1), dim3(64), 0, c->s, L.x64, j, j - 1);
+		hipLaunchKernelGGL((k_residual<double, double, 0>), plane_grid(L.nx, L.ny, L.nz), dim3(256), 0, c->s,
+				   view<double>(L), (const double *)L.x64, (const double *)nullptr, L.r64, L.bcid,
+				   bcview<double>(L), (double *)nullptr);
+		hipLaunchKernelGGL(k_dense_col, lin_grid(n), dim3(256), 0, c->s, A, (const double *)L.r64, n, j);
+	}
+	hipLaunchKernelGGL(k_dense_ident, lin_grid(n), dim3(256), 0, c->s, A, (const uint8_t *)L.flags, L.N);
+	HCHK(hipGetLastError());
+	rocblas_handle h = nullptr;
+	int rc = I3G_OK;
+	if (rocblas_create_handle(&h) != rocblas_status_success || rocblas_set_stream(h, c->s) != rocblas_status_success ||
+	    rocsolver_dgesvd(h, rocblas_svect_all, rocblas_svect_all, (rocblas_int)n, (rocblas_int)n, A, (rocblas_int)n,
+			     c->cd_s, c->cd_u, (rocblas_int)n, c->cd_vt, (rocblas_int)n, E, rocblas_outofplace,
+			     info) != rocblas_status_success)
+		rc = I3G_EDEVICE;
+	int hinfo = 0;
+	std::vector<double> sv(n);
+	if (!rc) {
+		HCHK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c->s));
+		HCHK(hipMemcpyAsync(sv.data(), c->cd_s, n * sizeof(double), hipMemcpyDeviceToHost, c->s));
+	}
+	HCHK(hipStreamSynchronize(c->s));
+	if (h) rocblas_destroy_handle(h);
+	hipFree(A);
+	hipFree(E);
+	hipFree(info);
+	if (rc) return rc;
+	if (hinfo != 0) return I3G_ENOCONV;
+	int rank = 0;
+	const double smax = sv[0];
+	for (long k = 0; k < n; k++) {
+		bool keep = sv[k] > c->cd_tol * smax;
+		sv[k] = keep ? 1.0 / sv[k] : 0.0;
+		rank += keep;
+	}
+	if (upload(c->cd_s, sv.data(), n)) return I3G_EDEVICE;
+	c->cd_rank = rank;
+	c->cd_setup_s = (double)clock() / CLOCKS_PER_SEC - t0;
+	fprintf(stderr, "i3g coarse direct: %ld unknowns, rank %d (tol %.1e), setup %.2f s CPU\n", n, rank, c->cd_tol,
+		c->cd_setup_s);
+	if (getenv("I3G_COARSE_DIRECT_DIAG")) {
+		fprintf(stderr, "  largest singular value %.3e, smallest kept %.3e\n", smax,
+			rank > 0 ? 1.0 / sv[rank - 1] : 0.0);
+	}
 	return I3G_OK;
 }
 
@@ -1005,6 +1138,11 @@ extern "C" int i3g_set_coefficients(i3g_ctx *c, const float *nu, const float *nx
 			   c->ro, phys[0], phys[1], phys[2], L0.flags, L0.bcid, L0.bc_c64, c->rhs0);
 	HCHK(hipGetLastError());
 	HCHK(hipStreamSynchronize(c->s));
+	c->cd_rank = 0;
+	if (c->coarse_direct && c->nlev > 1) {
+		int rc = coarse_direct_setup(c);
+		if (rc) return rc;
+	}
 	c->coef_ready = 1;
 	return I3G_OK;
 }
@@ -1408,6 +1546,29 @@ template <typename T> static void vcycle(i3g_ctx *c, int l, const i3g_stokes_par
 	T *x = lx<T>(L), *b = lb<T>(L);
 	if (l == c->nlev - 1) {
 		int ns = p->nu_coarse > 0 ? p->nu_coarse : 64;
+		if (c->cd_rank > 0) {
+			/* direct FP64 solve: x = V S^+ U^T b */
+			long n4 = 4 * L.N;
+			const double *bb = (const double *)b;
+			if (sizeof(T) == 4) {
+				hipLaunchKernelGGL((k_copy_scale<double, T>), lin_grid(n4), dim3(256), 0, c->s, L.b64,
+						   (const T *)b, 1.0, n4);
+				bb = L.b64;
+			}
+			dim3 g((unsigned)((n4 + 3) / 4));
+			hipLaunchKernelGGL(k_gemvt, g, dim3(256), 0, c->s, (const double *)c->cd_u, bb, (const double *)c->cd_s,
+					   c->cd_y, (int)n4);
+			if (sizeof(T) == 4) {
+				hipLaunchKernelGGL(k_gemvt, g, dim3(256), 0, c->s, (const double *)c->cd_vt,
+						   (const double *)c->cd_y, (const double *)nullptr, L.x64, (int)n4);
+				hipLaunchKernelGGL((k_copy_scale<T, double>), lin_grid(n4), dim3(256), 0, c->s, x,
+						   (const double *)L.x64, 1.0, n4);
+			} else {
+				hipLaunchKernelGGL(k_gemvt, g, dim3(256), 0, c->s, (const double *)c->cd_vt,
+						   (const double *)c->cd_y, (const double *)nullptr, (double *)x, (int)n4);
+			}
+			return;
+		}
 		if (sizeof(T) == 4 && c->coarsest64) {
 			/* the mixed V-cycle's coarsest solve in FP64: its right-hand
 			 * side and start widened, the sweeps in double, the result

@@ -208,7 +208,11 @@ int i3g_destroy(i3g_ctx *ctx);
  * reads, move3mg.c:1585). nu/ro at level-0 nodes; nxx at cell centres;
  * nxy/nxz/nyz at edges (index of the edge's own node, as in sxycalc etc.).
  * phys: [0]=GXKOEF [1]=GYKOEF [2]=GZKOEF [3]=nubeg [4]=nuend [5]=viscmod.
- * Coarse-level viscosities are restricted as ronurestrict (move3mg.c:2642). */
+ * Coarse-level viscosities are restricted as ronurestrict (move3mg.c:2642).
+ * With I3G_COARSE_DIRECT=1 in the environment at i3g_create, this call also
+ * assembles the coarsest level's operator and factors it by SVD in FP64
+ * (the V-cycle then solves that level directly; grids whose coarsest level
+ * has more than 4096 nodes return I3G_ECAP). */
 int i3g_set_coefficients(i3g_ctx *ctx, const float *nu, const float *nxx,
 			 const float *nxy, const float *nxz, const float *nyz,
 			 const float *ro, const double *phys);

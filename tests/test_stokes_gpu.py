@@ -84,6 +84,34 @@ def test_solve_matches_reference_converged(gpu, which, n, prec):
     S.close()
 
 
+def test_solve_coarse_direct_matches_reference_converged(gpu, monkeypatch):
+    """The coarsest level solved directly in FP64 (I3G_COARSE_DIRECT=1: the
+    SVD of its assembled operator, singular values below 1e-17 of the
+    largest dropped) inside the mixed V-cycle: the 33^3 subduction solve
+    converges to the reference's direct solve (g3) within 1e-6 per field."""
+    monkeypatch.setenv("I3G_COARSE_DIRECT", "1")
+    monkeypatch.setenv("I3G_COARSE_DIRECT_TOL", "1e-17")
+    g = golden("g3_converged_c2_33.npz")
+    n = 33
+    cfg = I.make_config(2, n, n, n, seed=int(g["seed"]))
+    print(f"case: coarse direct solve n={n} multinum={cfg['multinum']}", flush=True)
+    S = I.Stokes(cfg)
+    S.set_coefficients()
+    z = np.zeros(n ** 3)
+    S.set_solution(z, z, z, z)
+    st = S.solve(precision=32, rtol=1e-12, max_outer=40, max_vcycles=6000)
+    print("  outer", st.outer_iters, "vcycles", st.vcycles, "rel", st.rel_res, flush=True)
+    assert st.converged and st.rel_res <= 1e-12
+    sol, ex = g["sol"], g["exists"]
+    for c, f in enumerate(S.get_solution()):
+        e = ex[c::4]
+        ref = sol[c::4][e]
+        err = np.linalg.norm(f[e] - ref) / np.linalg.norm(ref)
+        print(f"  field {c}: rel L2 err {err:.3e}", flush=True)
+        assert err <= 1e-6, (c, err)
+    S.close()
+
+
 def test_combine_start_recovers_solution(gpu):
     """i3g_stokes_combine_start: with the converged solution and a random
     vector as directions from a zero base, the residual-optimal coefficients
