@@ -245,10 +245,13 @@ __global__ void __launch_bounds__(256) k_fill(long n, const int *__restrict__ ke
 /* Within each cell, order the markers by id and write them out: one
  * wavefront per cell sorts the cell's (id, slot) pairs with a bitonic
  * network (in registers across the lanes for up to 64 markers, in LDS for
- * up to CS_MAX, a serial insertion sort beyond), then writes every marker's
- * arrays, its previous position and its cell indices in the sorted order.
- * Ids are unique, so the order is fully determined (the same as any stable
- * sort by id). */
+ * up to CS_MAX), then writes every marker's arrays, its previous position
+ * and its cell indices in the sorted order.  A cell of more than CS_MAX
+ * markers is sorted in chunks of CS_MAX in LDS and the sorted chunks are
+ * merged pairwise in `scratch` (merge path: each lane merges one 64th of
+ * the output after a binary search on its diagonal), so its cost grows as
+ * e log e, not e^2.  Ids are unique, so the order is fully determined (the
+ * same as any stable sort by id). */
 #define CS_MAX 1024
 __device__ __forceinline__ void put_sorted(const SortRec &r, long o, int c, int nx, int ny, MkArrays d, int *perm,
 					   int *key, int *c1, int *c2, int *c3)
@@ -265,7 +268,71 @@ __device__ __forceinline__ void put_sorted(const SortRec &r, long o, int c, int 
 	c3[o] = m3;
 }
 
-__global__ void __launch_bounds__(256) k_cellsort(long N, int nx, int ny, const int *__restrict__ start,
+/* Bitonic sort of the e <= CS_MAX records rec[0..e) by id into K (ids) and
+ * V (indices into rec) in one wavefront's LDS. */
+__device__ __forceinline__ void lds_bitonic(const SortRec *__restrict__ rec, int e, int *K, int *V, int lane)
+{
+	int P = 128;
+	while (P < e) P <<= 1;
+	for (int i = lane; i < P; i += 64) {
+		K[i] = i < e ? rec[i].c.y : INT_MAX;
+		V[i] = i;
+	}
+	/* one wavefront owns K, V: its LDS operations complete in program
+	 * order; the empty asm keeps the compiler from reordering them */
+	asm volatile("" ::: "memory");
+	for (int kk = 2; kk <= P; kk <<= 1)
+		for (int j = kk >> 1; j > 0; j >>= 1) {
+			for (int i = lane; i < P; i += 64) {
+				int ixj = i ^ j;
+				if (ixj > i) {
+					int a = K[i], b = K[ixj];
+					if ((a > b) == ((i & kk) == 0)) {
+						int va = V[i];
+						K[i] = b;
+						K[ixj] = a;
+						V[i] = V[ixj];
+						V[ixj] = va;
+					}
+				}
+			}
+			asm volatile("" ::: "memory");
+		}
+}
+
+/* Merge the sorted runs (ka, va)[0..na) and (kb, vb)[0..nb) (distinct
+ * keys) into (ko, vo)[0..na+nb): lane l writes outputs [l*per, (l+1)*per),
+ * starting where the merge path crosses that diagonal. */
+__device__ __forceinline__ void wave_merge(const int *ka, const int *va, int na, const int *kb, const int *vb, int nb,
+					   int *ko, int *vo, int lane)
+{
+	const int tot = na + nb, per = (tot + 63) / 64;
+	const int d0 = min(lane * per, tot), d1 = min(d0 + per, tot);
+	if (d0 >= d1) return;
+	int lo = max(0, d0 - nb), hi = min(d0, na);
+	while (lo < hi) {
+		int mid = (lo + hi) >> 1;
+		if (ka[mid] < kb[d0 - 1 - mid]) lo = mid + 1;
+		else hi = mid;
+	}
+	int i = lo, j = d0 - lo;
+	int a = i < na ? ka[i] : INT_MAX, b = j < nb ? kb[j] : INT_MAX;
+	for (int o = d0; o < d1; o++) {
+		if (i < na && (j >= nb || a < b)) {
+			ko[o] = a;
+			vo[o] = va[i];
+			i++;
+			a = i < na ? ka[i] : INT_MAX;
+		} else {
+			ko[o] = b;
+			vo[o] = vb[j];
+			j++;
+			b = j < nb ? kb[j] : INT_MAX;
+		}
+	}
+}
+
+__global__ void __launch_bounds__(256) k_cellsort(long N, long cap, int nx, int ny, const int *__restrict__ start,
 						  const int *__restrict__ cnt, const SortRec *__restrict__ rec, MkArrays d,
 						  int *perm, int *key, int *c1, int *c2, int *c3, int *scratch)
 {
@@ -291,52 +358,37 @@ __global__ void __launch_bounds__(256) k_cellsort(long N, int nx, int ny, const 
 		if (lane < e) put_sorted(rec[s + v], s + lane, c, nx, ny, d, perm, key, c1, c2, c3);
 		return;
 	}
+	int *K = sk[w], *V = sv[w];
 	if (e <= CS_MAX) {
-		int P = 128;
-		while (P < e) P <<= 1;
-		int *K = sk[w], *V = sv[w];
-		for (int i = lane; i < P; i += 64) {
-			K[i] = i < e ? rec[s + i].c.y : INT_MAX;
-			V[i] = i;
-		}
-		/* one wavefront owns K, V: its LDS operations complete in program
-		 * order; the empty asm keeps the compiler from reordering them */
-		asm volatile("" ::: "memory");
-		for (int kk = 2; kk <= P; kk <<= 1)
-			for (int j = kk >> 1; j > 0; j >>= 1) {
-				for (int i = lane; i < P; i += 64) {
-					int ixj = i ^ j;
-					if (ixj > i) {
-						int a = K[i], b = K[ixj];
-						if ((a > b) == ((i & kk) == 0)) {
-							int va = V[i];
-							K[i] = b;
-							K[ixj] = a;
-							V[i] = V[ixj];
-							V[ixj] = va;
-						}
-					}
-				}
-				asm volatile("" ::: "memory");
-			}
+		lds_bitonic(rec + s, e, K, V, lane);
 		for (int i = lane; i < e; i += 64) put_sorted(rec[s + V[i]], s + i, c, nx, ny, d, perm, key, c1, c2, c3);
 		return;
 	}
-	/* more than CS_MAX markers in one cell: slots ordered by insertion in
-	 * this cell's range of `scratch`, by one lane */
-	if (lane == 0) {
-		int *q = scratch + s;
-		for (int i = 0; i < e; i++) {
-			int vid = rec[s + i].c.y, j = i - 1;
-			while (j >= 0 && rec[s + q[j]].c.y > vid) {
-				q[j + 1] = q[j];
-				j--;
-			}
-			q[j + 1] = i;
+	/* more than CS_MAX markers: four planes of cap ints in scratch (ids and
+	 * slots twice), this cell's range at offset s in each */
+	int *ka = scratch + s, *va = ka + cap, *kb = va + cap, *vb = kb + cap;
+	for (int b = 0; b < e; b += CS_MAX) {
+		const int len = min(CS_MAX, e - b);
+		lds_bitonic(rec + s + b, len, K, V, lane);
+		for (int i = lane; i < len; i += 64) {
+			ka[b + i] = K[i];
+			va[b + i] = b + V[i];
 		}
+		asm volatile("" ::: "memory");
 	}
-	__threadfence_block();
-	for (int i = lane; i < e; i += 64) put_sorted(rec[s + scratch[s + i]], s + i, c, nx, ny, d, perm, key, c1, c2, c3);
+	for (int wd = CS_MAX; wd < e; wd <<= 1) {
+		/* the previous pass's stores, by other lanes, before this pass's
+		 * loads (agent scope also invalidates the vector L1) */
+		__threadfence();
+		for (int lo = 0; lo < e; lo += 2 * wd) {
+			const int na = min(wd, e - lo), nb = max(0, min(wd, e - lo - wd));
+			wave_merge(ka + lo, va + lo, na, ka + lo + na, va + lo + na, nb, kb + lo, vb + lo, lane);
+		}
+		int *t = ka; ka = kb; kb = t;
+		t = va; va = vb; vb = t;
+	}
+	__threadfence();
+	for (int i = lane; i < e; i += 64) put_sorted(rec[s + va[i]], s + i, c, nx, ny, d, perm, key, c1, c2, c3);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1641,11 +1693,12 @@ extern "C" int i3g_markers_sort(i3g_markers *m, long *order)
 		MkArrays dst{m->x2, m->y2, m->z2, m->tk2, m->eta2, m->e2, m->rr2, m->rn2, m->type2, m->id2};
 		hipLaunchKernelGGL(k_fill, g1(n), dim3(256), 0, m->s, n, (const int *)m->key, (const int *)m->start, m->cur,
 				   src, rec);
-		/* the scratch of cells beyond CS_MAX markers (n ints, indexed by
-		 * sorted position): the record buffer's tail past cap SortRecs */
-		static_assert(sizeof(MkRec) - sizeof(SortRec) >= sizeof(int), "scratch fits behind the sort records");
+		/* the scratch of cells beyond CS_MAX markers (four planes of cap
+		 * ints, indexed by sorted position): the record buffer's tail past
+		 * cap SortRecs */
+		static_assert(sizeof(MkRec) - sizeof(SortRec) >= 4 * sizeof(int), "scratch fits behind the sort records");
 		int *scratch = (int *)(rec + m->cap);
-		hipLaunchKernelGGL(k_cellsort, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, m->s, N, m->nx, m->ny,
+		hipLaunchKernelGGL(k_cellsort, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, m->s, N, m->cap, m->nx, m->ny,
 				   (const int *)m->start, (const int *)m->cnt, (const SortRec *)rec, dst, m->perm, m->key,
 				   m->c1, m->c2, m->c3, scratch);
 		swap_sorted(m);

@@ -60,13 +60,16 @@ def test_sort_is_stable_by_cell_then_id(gpu):
 
 def test_sort_cells_of_every_size(gpu):
     """The in-cell ordering by id for cells of 1 to 64 markers (bitonic
-    network across the lanes), 65 to 1024 (in LDS) and more (serial), with
-    ids in random order: the order equals numpy's lexsort by (cell, id)."""
+    network across the lanes), 65 to 1024 (in LDS) and more (sorted chunks
+    of 1024 merged pairwise: 2, 3, 5 and 9 chunks, so that some passes have
+    an odd run left over), with ids in random order: the order equals
+    numpy's lexsort by (cell, id)."""
     rng = np.random.default_rng(21)
     n = 9
     g = np.linspace(0.0, 8.0, n)
-    # cell (a, b, c) receives 0..1600 markers: a few crowded cells
-    sizes = {0: 1, 1: 7, 2: 64, 3: 65, 4: 200, 5: 1024, 6: 1025, 7: 1600, 100: 33, 300: 129}
+    # cell (a, b, c) receives 0..9000 markers: a few crowded cells
+    sizes = {0: 1, 1: 7, 2: 64, 3: 65, 4: 200, 5: 1024, 6: 1025, 7: 1600, 8: 2048, 9: 3100, 10: 5000,
+             11: 9000, 100: 33, 300: 129, 511: 4097}
     rest = rng.integers(0, 40, 512)
     pts = []
     for cell in range(512):
@@ -79,7 +82,7 @@ def test_sort_cells_of_every_size(gpu):
     m = len(pts)
     perm = rng.permutation(m)
     ids = rng.permutation(m).astype(np.int32)
-    print(f"case: sort {m} markers, cells up to 1600", flush=True)
+    print(f"case: sort {m} markers, cells up to 9000", flush=True)
     M = I.Markers(g, g, g, m)
     z = np.zeros(m, np.float32)
     M.upload(pts[perm, 0], pts[perm, 1], pts[perm, 2], z, z, np.full(m, 2, np.int8), ids[perm])
