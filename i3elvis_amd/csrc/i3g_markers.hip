@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -69,9 +70,13 @@ struct i3g_markers {
 	 * types' properties; mw_n ids, none until i3g_markers_set_melt */
 	float *mw = nullptr;
 	long mw_n = 0, mw_cap = 0;
-	/* node averages: 1 cell-segmented (k_cell_partials / k_node_sums), 0 the
+	/* node averages: 1 cell-segmented (k_cell_transfer / k_node_sums), 0 the
 	 * reference's serial marker order (k_marker_gather, k_tk_gather) */
 	int order = 1;
+	/* 1: the cell-segmented transfer stages its records in prec
+	 * (k_marker_props + k_cell_partials) instead of computing them in the
+	 * reduction (I3G_MK_STAGED=1 at create); the same sums to the bit */
+	int staged = 0;
 	double *part = nullptr; /* per-cell corner partials, PF_N x 8 x cells */
 	/* movemark's depth rules (inactive by default) */
 	double zdeep = 1e30, vdeep = 1e30, tdeep = 0;
@@ -838,69 +843,162 @@ __device__ __forceinline__ bool cell_of(long s, int nx, int ny, int nz, int *a, 
 	return true;
 }
 
+/* One cell corner's share of a cell's markers: lane q = 4 ec + 2 ea + eb
+ * of the cell's eight is corner node (a+ea, b+eb, c+ec). */
+struct CornerGeom {
+	double dx, dy, dz, gxm, gym, gzm;
+	int q, ea, eb, ec;
+	__device__ __forceinline__ void init(const double *gx, const double *gy, const double *gz, int a, int b, int c,
+					     int lane)
+	{
+		q = lane;
+		ea = (q >> 1) & 1;
+		eb = q & 1;
+		ec = q >> 2;
+		dx = gx[a + 1] - gx[a];
+		dy = gy[b + 1] - gy[b];
+		dz = gz[c + 1] - gz[c];
+		gxm = gx[a + ea];
+		gym = gy[b + eb];
+		gzm = gz[c + ec];
+	}
+};
+
+/* the reference's weighted contributions of one marker record to corner
+ * G.q (mark3mg.c:818-933), added to acc in FP64 */
+__device__ __forceinline__ void corner_accum(const MkRec &r, const CornerGeom &G, double nukoef, int viscmod,
+					     double *acc)
+{
+#pragma clang fp contract(off)
+	if (!r.in) return;
+	double ddz = 1.0 - fabs((G.gzm - (double)r.z) / G.dz);
+	double ddx = 1.0 - fabs((G.gxm - (double)r.x) / G.dx);
+	double ddy = 1.0 - fabs((G.gym - (double)r.y) / G.dy);
+	double swt = ddx * ddy * ddz;
+	double mnu = viscmod == 1 ? log(r.nu) : viscmod == 2 ? 1.0 / r.nu : r.nu;
+	if (ddx > nukoef && ddy > nukoef && ddz > nukoef) {
+		acc[PF_NU] += mnu * swt;
+		acc[PF_W1] += swt;
+	}
+	if (G.q == 7) {
+		double s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (0.5 - fabs(ddy - 0.5)) * (0.5 - fabs(ddz - 0.5));
+		acc[PF_XX] += mnu * s1;
+		acc[PF_WXX] += s1;
+	}
+	if (G.ec == 0 && ddx > 0.5 && ddy > 0.5) {
+		double s1 = 8.0 * (ddx - 0.5) * (ddy - 0.5) * (0.5 - fabs(ddz - 0.5));
+		acc[PF_XY] += mnu * s1;
+		acc[PF_WXY] += s1;
+	}
+	if (G.eb == 0 && ddx > 0.5 && ddz > 0.5) {
+		double s1 = 8.0 * (ddx - 0.5) * (0.5 - fabs(ddy - 0.5)) * (ddz - 0.5);
+		acc[PF_XZ] += mnu * s1;
+		acc[PF_WXZ] += s1;
+	}
+	if (G.ea == 0 && ddy > 0.5 && ddz > 0.5) {
+		double s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (ddy - 0.5) * (ddz - 0.5);
+		acc[PF_YZ] += mnu * s1;
+		acc[PF_WYZ] += s1;
+	}
+	if (swt > 0) {
+		acc[PF_RO] += r.ro * swt;
+		acc[PF_CP] += r.cp * r.ro * swt;
+		acc[PF_KT] += r.kt * swt;
+		acc[PF_HT] += r.ht * swt;
+		acc[PF_ET] += r.bb * swt;
+		acc[PF_TK] += r.tk * swt;
+		acc[PF_W0] += swt;
+	}
+}
+
+__device__ __forceinline__ void store_partials(const double *acc, int q, long NC, long s, double *part)
+{
+#pragma unroll
+	for (int F = 0; F < PF_N; F++)
+		if (pf_used(F, q)) part[((long)F * 8 + q) * NC + s] = acc[F];
+}
+
+/* the partials from the records k_marker_props staged (I3G_MK_STAGED=1) */
 __global__ void __launch_bounds__(256)
 k_cell_partials(int nx, int ny, int nz, const double *gx, const double *gy, const double *gz, const int *start,
 		const int *cnt, const MkRec *__restrict__ rec, double nukoef, int viscmod, double *part)
 {
-#pragma clang fp contract(off)
 	long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	long s = g >> 3;
-	int q = (int)(g & 7), a, b, c;
+	int a, b, c;
 	if (!cell_of(s, nx, ny, nz, &a, &b, &c)) return;
 	long NC = (long)(nx - 1) * (ny - 1) * (nz - 1);
-	int ea = (q >> 1) & 1, eb = q & 1, ec = q >> 2;
+	CornerGeom G;
+	G.init(gx, gy, gz, a, b, c, (int)(g & 7));
 	long key = (long)c * nx * ny + (long)a * ny + b;
 	int k0 = start[key], k1 = k0 + cnt[key];
-	double dx = gx[a + 1] - gx[a], dy = gy[b + 1] - gy[b], dz = gz[c + 1] - gz[c];
-	double gxm = gx[a + ea], gym = gy[b + eb], gzm = gz[c + ec];
 	double acc[PF_N];
 #pragma unroll
 	for (int F = 0; F < PF_N; F++) acc[F] = 0;
-	for (int k = k0; k < k1; k++) {
-		const MkRec r = rec[k];
-		if (!r.in) continue;
-		double ddz = 1.0 - fabs((gzm - (double)r.z) / dz);
-		double ddx = 1.0 - fabs((gxm - (double)r.x) / dx);
-		double ddy = 1.0 - fabs((gym - (double)r.y) / dy);
-		double swt = ddx * ddy * ddz;
-		double mnu = viscmod == 1 ? log(r.nu) : viscmod == 2 ? 1.0 / r.nu : r.nu;
-		if (ddx > nukoef && ddy > nukoef && ddz > nukoef) {
-			acc[PF_NU] += mnu * swt;
-			acc[PF_W1] += swt;
-		}
-		if (q == 7) {
-			double s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (0.5 - fabs(ddy - 0.5)) * (0.5 - fabs(ddz - 0.5));
-			acc[PF_XX] += mnu * s1;
-			acc[PF_WXX] += s1;
-		}
-		if (ec == 0 && ddx > 0.5 && ddy > 0.5) {
-			double s1 = 8.0 * (ddx - 0.5) * (ddy - 0.5) * (0.5 - fabs(ddz - 0.5));
-			acc[PF_XY] += mnu * s1;
-			acc[PF_WXY] += s1;
-		}
-		if (eb == 0 && ddx > 0.5 && ddz > 0.5) {
-			double s1 = 8.0 * (ddx - 0.5) * (0.5 - fabs(ddy - 0.5)) * (ddz - 0.5);
-			acc[PF_XZ] += mnu * s1;
-			acc[PF_WXZ] += s1;
-		}
-		if (ea == 0 && ddy > 0.5 && ddz > 0.5) {
-			double s1 = 8.0 * (0.5 - fabs(ddx - 0.5)) * (ddy - 0.5) * (ddz - 0.5);
-			acc[PF_YZ] += mnu * s1;
-			acc[PF_WYZ] += s1;
-		}
-		if (swt > 0) {
-			acc[PF_RO] += r.ro * swt;
-			acc[PF_CP] += r.cp * r.ro * swt;
-			acc[PF_KT] += r.kt * swt;
-			acc[PF_HT] += r.ht * swt;
-			acc[PF_ET] += r.bb * swt;
-			acc[PF_TK] += r.tk * swt;
-			acc[PF_W0] += swt;
-		}
+	for (int k = k0; k < k1; k++) corner_accum(rec[k], G, nukoef, viscmod, acc);
+	store_partials(acc, G.q, NC, s, part);
+}
+
+/* The default transfer in one pass: the cell's eight lanes take its
+ * id-ordered segment in chunks of eight markers; lane j computes marker
+ * k + j's properties (marker_props_one) into the group's LDS slots, then
+ * every lane adds the chunk's eight records to its corner in id order.
+ * No record goes through memory, and the sums are those of
+ * k_marker_props + k_cell_partials to the bit (same records, same order).
+ * The lane's FP64 sums wait in LDS while the properties are computed.
+ * The property code (FP64 rheology, pressure interpolation) needs more
+ * than 256 registers a lane; bounded to two waves per SIMD it spills 88
+ * bytes a lane and still runs faster than one wave without spills (134M
+ * markers at 129^3: 12.8 against 17.4 ms; the staged two-pass transfer
+ * 15.2 ms). */
+#define CT_CELLS 32 /* cells per 256-thread block */
+__global__ void __launch_bounds__(256, 2)
+k_cell_transfer(int nx, int ny, int nz, const int *start, const int *cnt, const float *__restrict__ x,
+		const float *__restrict__ y, const float *__restrict__ z, const float *__restrict__ tk, float *eta,
+		signed char *type, float *e, float *rr, float *rn, const int *c1, const int *c2, const int *c3,
+		const double *pr, const float *st12, long N, const double *gx, const double *gy, const double *gz,
+		const i3g_rock *rocks, int nrock, i3g_mk_params prm, const int *id, const float *mw, long mw_n,
+		int *bad, double *part)
+{
+	__shared__ MkRec srec[CT_CELLS][8];
+	__shared__ double sacc[PF_N][256];
+	const int grp = threadIdx.x >> 3, lane = threadIdx.x & 7;
+	long s = (long)blockIdx.x * CT_CELLS + grp;
+	int a, b, c;
+	if (!cell_of(s, nx, ny, nz, &a, &b, &c)) return;
+	long NC = (long)(nx - 1) * (ny - 1) * (nz - 1);
+	long key = (long)c * nx * ny + (long)a * ny + b;
+	int k0 = start[key], k1 = k0 + cnt[key];
+	double acc[PF_N];
+#pragma unroll
+	for (int F = 0; F < PF_N; F++) sacc[F][threadIdx.x] = 0;
+	MkRec *slot = srec[grp];
+	for (int k = k0; k < k1; k += 8) {
+		MkRec R;
+		R.x = R.y = R.z = 0.f;
+		R.in = 0;
+		R.nu = R.ro = R.cp = R.kt = R.ht = R.bb = R.tk = R.pad = 0.0;
+		if (k + lane < k1)
+			marker_props_one(k + lane, x, y, z, tk, eta, type, e, rr, rn, c1, c2, c3, pr, st12, N, gx, gy, gz, nx,
+					 ny, nz, rocks, nrock, prm, id, mw, mw_n, &R, bad);
+		/* the group's eight lanes are in one wavefront, whose LDS
+		 * operations complete in program order; the empty asm keeps the
+		 * compiler from moving them across each other */
+		slot[lane] = R;
+		asm volatile("" ::: "memory");
+		CornerGeom G;
+		G.init(gx, gy, gz, a, b, c, lane);
+#pragma unroll
+		for (int F = 0; F < PF_N; F++) acc[F] = sacc[F][threadIdx.x];
+		const int e8 = k1 - k < 8 ? k1 - k : 8;
+		for (int j = 0; j < e8; j++) corner_accum(slot[j], G, prm.nukoef, prm.viscmod, acc);
+#pragma unroll
+		for (int F = 0; F < PF_N; F++) sacc[F][threadIdx.x] = acc[F];
+		asm volatile("" ::: "memory");
 	}
 #pragma unroll
-	for (int F = 0; F < PF_N; F++)
-		if (pf_used(F, q)) part[((long)F * 8 + q) * NC + s] = acc[F];
+	for (int F = 0; F < PF_N; F++) acc[F] = sacc[F][threadIdx.x];
+	store_partials(acc, lane, NC, s, part);
 }
 
 __global__ void __launch_bounds__(256)
@@ -1532,6 +1630,7 @@ extern "C" int i3g_markers_create(long nx, long ny, long nz, const double *gx, c
 	m->nx = (int)nx; m->ny = (int)ny; m->nz = (int)nz;
 	m->N = nx * ny * nz;
 	m->cap = capacity;
+	if (const char *ev = getenv("I3G_MK_STAGED")) m->staged = ev[0] == '1';
 	int err = 0;
 	if (hipStreamCreate(&m->s) != hipSuccess) { delete m; return I3G_EDEVICE; }
 	if (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess) {
@@ -1747,7 +1846,8 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 		if (!host[q]) return I3G_EINVAL;
 	int err = 0;
 	long n = m->n, N = m->N;
-	if (!m->prec) m->prec = mk_alloc<MkRec>(m->cap, &err);
+	const bool fused = m->order == 1 && !m->staged;
+	if (!fused && !m->prec) m->prec = mk_alloc<MkRec>(m->cap, &err);
 	if (!m->fld) m->fld = mk_alloc<float>(11 * N, &err);
 	if (err) return err;
 	if (!m->pr) {
@@ -1761,7 +1861,7 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 	for (int q = 0; q < 11; q++)
 		MCHK(hipMemcpyAsync(m->fld + q * N, host[q], N * sizeof(float), hipMemcpyDefault, m->s));
 	mark_begin(m);
-	if (n)
+	if (n && !fused)
 		hipLaunchKernelGGL(k_marker_props, g1(n), dim3(256), 0, m->s, n, (const float *)m->x, (const float *)m->y,
 				   (const float *)m->z, (const float *)m->tk, m->eta, m->type, m->e, m->rr, m->rn,
 				   (const int *)m->c1, (const int *)m->c2, (const int *)m->c3, (const double *)m->pr,
@@ -1771,10 +1871,20 @@ extern "C" int i3g_markers_to_nodes(i3g_markers *m, const i3g_rock *rocks, int n
 	dim3 gg((unsigned)(((long)m->nx * m->ny + 255) / 256), (unsigned)m->nz);
 	if (m->order == 1) {
 		long NC = (long)(m->nx - 1) * (m->ny - 1) * (m->nz - 1);
-		hipLaunchKernelGGL(k_cell_partials, dim3((unsigned)((NC * 8 + 255) / 256)), dim3(256), 0, m->s, m->nx, m->ny,
-				   m->nz, (const double *)m->gx, (const double *)m->gy, (const double *)m->gz,
-				   (const int *)m->start, (const int *)m->cnt, (const MkRec *)m->prec, prm->nukoef,
-				   prm->viscmod, m->part);
+		if (fused)
+			hipLaunchKernelGGL(k_cell_transfer, dim3((unsigned)((NC + CT_CELLS - 1) / CT_CELLS)), dim3(256), 0, m->s,
+					   m->nx, m->ny, m->nz, (const int *)m->start, (const int *)m->cnt, (const float *)m->x,
+					   (const float *)m->y, (const float *)m->z, (const float *)m->tk, m->eta, m->type, m->e,
+					   m->rr, m->rn, (const int *)m->c1, (const int *)m->c2, (const int *)m->c3,
+					   (const double *)m->pr, (const float *)(m->has_st ? m->st12 : nullptr), N,
+					   (const double *)m->gx, (const double *)m->gy, (const double *)m->gz,
+					   (const i3g_rock *)m->rocks, nrock, *prm, (const int *)m->id, (const float *)m->mw,
+					   m->mw_n, m->bad, m->part);
+		else
+			hipLaunchKernelGGL(k_cell_partials, dim3((unsigned)((NC * 8 + 255) / 256)), dim3(256), 0, m->s, m->nx,
+					   m->ny, m->nz, (const double *)m->gx, (const double *)m->gy, (const double *)m->gz,
+					   (const int *)m->start, (const int *)m->cnt, (const MkRec *)m->prec, prm->nukoef,
+					   prm->viscmod, m->part);
 		hipLaunchKernelGGL(k_node_sums, gg, dim3(256), 0, m->s, m->nx, m->ny, m->nz, (const double *)m->part,
 				   prm->nubeg, prm->nuend, prm->viscmod, m->fld);
 	} else {

@@ -200,6 +200,33 @@ def test_to_nodes_matches_oracle_65(gpu, order):
     M.close()
 
 
+def test_fused_transfer_equals_staged_records_33(gpu, monkeypatch):
+    """33^3 grid, 12 shuffled markers per cell (one full and one half chunk
+    of eight per cell), Arrhenius rocks, viscosities computed by the
+    transfer (eta 0): the one-pass transfer equals the two-pass one that
+    stages the records in memory (I3G_MK_STAGED=1) to the bit."""
+    n = 33
+    gx, gy, gz, x, y, z, tk, mtype = _jittered(n, 12, 7)
+    m = len(x)
+    print(f"case: fused vs staged transfer 33^3, {m} markers", flush=True)
+    rocks = [list(r) for r in golden("g5_markers.npz")["rocks"]]
+    pr = np.random.default_rng(4).uniform(1e7, 1e9, n ** 3)
+    init = np.zeros((11, n ** 3), np.float32)
+    out = []
+    for staged in ("0", "1"):
+        monkeypatch.setenv("I3G_MK_STAGED", staged)
+        M = I.Markers(gx, gy, gz, m)
+        M.upload(x, y, z, tk, np.zeros(m, np.float32), mtype)
+        M.sort()
+        f = M.to_nodes(rocks, pr, init, 1e18, 1e25, 0.0, 1, 1e10, gy[-1] * 0.8, 1e21)
+        out.append((f, M.download()))
+        M.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    for k in ("eta", "type", "id"):
+        assert np.array_equal(out[0][1][k], out[1][1][k]), k
+    assert np.isfinite(out[0][0]).all()
+
+
 def test_large_marker_set_deterministic(gpu):
     """129^3 grid, 8 markers per cell (16.8M): two transfers give identical
     node fields; constant density maps to a constant."""

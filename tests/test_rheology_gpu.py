@@ -10,6 +10,8 @@ grain roughness and dislocation shares within 2e-6 relative (the grain
 growth repeats up to 50 exponential steps), strain to 1e-6 absolute with
 the same sign, positions within 2 ulp, types exact; the node fields of
 the transfer within 2e-6 relative."""
+import os
+
 import numpy as np
 import pytest
 
@@ -137,6 +139,50 @@ def test_ronurecalc_rheology_matches_reference(gpu, mode):
         for q in range(11):
             assert _scaled(fld[q], o[q]).max() <= 2e-7, q
     M.close()
+
+
+def _g15_cell_segmented(staged):
+    """one cell-segmented ronurecalc of the g15 markers (whole rheology),
+    the records staged in memory or not (I3G_MK_STAGED, read at create)"""
+    g = golden("g15_ronurecalc_rheology.npz")
+    n = len(g["x"])
+    os.environ["I3G_MK_STAGED"] = "1" if staged else "0"
+    try:
+        M = I.Markers(g["gx"], g["gy"], g["gz"], n + 16, order=1)
+    finally:
+        del os.environ["I3G_MK_STAGED"]
+    M.upload(g["x"], g["y"], g["z"], g["tk"], g["eta"], g["mtype"])
+    M.set_strain(g["marke"])
+    M.set_grain(g["markrr"], g["markrn"])
+    M.set_stress([g[k] for k in ST])
+    M.sort()
+    M.set_tbc(g["bondm"], g["bondv"], g["bondn"])
+    rocks = [list(r) for r in g["rocks"]]
+    fld = M.to_nodes(rocks, g["pr"], g["init"], float(g["nubeg"]), float(g["nuend"]), nukoef=float(g["nukoef"]),
+                     densimod=int(g["densimod"]), timestep=float(g["timestep"]), vdeep=float(g["vdeep"]),
+                     nudeep=float(g["nudeep"]), rheology=1, eroslev=float(g["eroslev"]), sedilev=float(g["sedilev"]),
+                     waterlev=float(g["waterlev"]), sedtype=int(g["sedtype"]), mnumy=int(g["mnumy"]),
+                     teclmin=float(g["teclmin"]), teclmax=float(g["teclmax"]), timesum=float(g["timesum"]),
+                     convend=float(g["convend"]))
+    out = dict(fld=fld, e=M.get_strain(), grain=M.get_grain(), **M.download())
+    M.close()
+    return out
+
+
+def test_fused_transfer_equals_staged_records(gpu):
+    """The default one-pass transfer (each cell's eight corner lanes compute
+    the properties of eight markers at a time and sum them from LDS) and the
+    two-pass one that stages every marker's record in memory
+    (I3G_MK_STAGED=1) give the same node fields and marker updates (type,
+    viscosity, strain, grain) to the bit: the same records summed in the
+    same order."""
+    print("case: fused vs staged transfer, g15 whole rheology", flush=True)
+    a, b = _g15_cell_segmented(False), _g15_cell_segmented(True)
+    assert np.array_equal(a["fld"], b["fld"])
+    for k in ("x", "y", "z", "tk", "eta", "type", "id", "e"):
+        assert np.array_equal(a[k], b[k]), k
+    for q in range(2):
+        assert np.array_equal(a["grain"][q], b["grain"][q])
 
 
 @pytest.mark.parametrize("mode", [0, 1])
