@@ -1432,6 +1432,99 @@ k_prolong(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__ 
 #define COARSEST_THREADS 1024
 #define COARSEST_MAXD 8
 
+/* ---- coupled cell smoother (SCGS, opt-in) --------------------------------
+ *
+ * Symmetric coupled Gauss-Seidel (Vanka's box smoother with the velocity
+ * block reduced to its diagonal): cell P(i,j,k) and its six faces
+ * vx(i,j-1,k-1) (+x), vx(i-1,j-1,k-1) (-x), vy(i-1,j,k-1) (+y),
+ * vy(i-1,j-1,k-1) (-y), vz(i-1,j-1,k) (+z), vz(i-1,j-1,k-1) (-z) are
+ * corrected together: with r_f, A_f the face rows' residuals and diagonals,
+ * G_f the rows' coefficient of this cell's P and D_f the continuity row's
+ * coefficient of the face,
+ *   dp = (sum_f D_f r_f / A_f - r_c) / sum_f D_f G_f / A_f,
+ *   du_f = (r_f - G_f dp) / A_f.
+ * BC and missing unknowns take no part.  Cells in two colours (i+j+k
+ * parity; two cells of one colour share no face): k_scgs_cell computes the
+ * seven corrections of every cell of the colour from the current iterate
+ * into D (7 planes: dp, +x, -x, +y, -y, +z, -z), k_scgs_apply adds them,
+ * damped, each face from its one cell of that colour. */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_scgs_cell(LevelView<T> L, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ D, int colour)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int k = blockIdx.y;
+	int i = t / L.ny, j = t - i * L.ny;
+	if (i >= L.nx || ((i + j + k) & 1) != colour) return;
+	long N = L.N, sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = (long)k * sz + (long)i * sx + j;
+	T dp = 0, du[6] = {0, 0, 0, 0, 0, 0};
+	unsigned fc = L.flags[n];
+	if ((fc & FL_EX(0)) && !(fc & FL_BC(0)) && i >= 1 && j >= 1 && k >= 1) {
+		const T *P = x, *vx = x + N, *vy = x + 2 * N, *vz = x + 3 * N;
+		T rc = (b ? b[n] : (T)0) - row_ax<T, T>(L, P, vx, vy, vz, 0, i, j, k, (T *)0);
+		const T Dc[6] = {L.ihx[i - 1], -L.ihx[i - 1], L.ihy[j - 1], -L.ihy[j - 1], L.ihz[k - 1], -L.ihz[k - 1]};
+		const T Gc[6] = {(T)2 * L.icx[i], -(T)2 * L.icx[i - 1], (T)2 * L.icy[j], -(T)2 * L.icy[j - 1],
+				 (T)2 * L.icz[k], -(T)2 * L.icz[k - 1]};
+		T rf[6], af[6], S = 0, R = 0;
+#pragma unroll
+		for (int f = 0; f < 6; f++) {
+			const int comp = 1 + f / 2;
+			const int fi = i - (f == 0 ? 0 : 1), fj = j - (f == 2 ? 0 : 1), fk = k - (f == 4 ? 0 : 1);
+			long nf = (long)fk * sz + (long)fi * sx + fj;
+			unsigned ff = L.flags[nf];
+			rf[f] = 0;
+			af[f] = 0;
+			if ((ff & FL_EX(comp)) && !(ff & FL_BC(comp))) {
+				T d;
+				T ax = row_ax<T, T>(L, P, vx, vy, vz, comp, fi, fj, fk, &d);
+				rf[f] = (b ? b[comp * N + nf] : (T)0) - ax;
+				af[f] = d;
+				S += Dc[f] * Gc[f] / d;
+				R += Dc[f] * rf[f] / d;
+			}
+		}
+		if (S != 0) dp = (R - rc) / S;
+#pragma unroll
+		for (int f = 0; f < 6; f++)
+			if (af[f] != 0) du[f] = (rf[f] - Gc[f] * dp) / af[f];
+	}
+	D[n] = dp;
+#pragma unroll
+	for (int f = 0; f < 6; f++) D[(1 + f) * N + n] = du[f];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_scgs_apply(LevelView<T> L, T *__restrict__ x, const T *__restrict__ D, int colour, T wv, T wp)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	int k = blockIdx.y;
+	int i = t / L.ny, j = t - i * L.ny;
+	if (i >= L.nx) return;
+	long N = L.N, sx = L.ny, sz = (long)L.nx * L.ny;
+	long n = (long)k * sz + (long)i * sx + j;
+	unsigned fl = L.flags[n];
+	const bool same = ((i + j + k) & 1) == colour;
+	if (same && (fl & FL_EX(0)) && !(fl & FL_BC(0))) x[n] += wp * D[n];
+#pragma unroll
+	for (int comp = 1; comp <= 3; comp++) {
+		if (!(fl & FL_EX(comp)) || (fl & FL_BC(comp))) continue;
+		/* the face's cell of this colour: the one on its + side (this
+		 * face is its - face) or on its - side (its + face) */
+		int ci = i + 1, cj = j + 1, ck = k + 1, slot = 2 * comp;
+		if (same) {
+			if (comp == 1) ci = i;
+			else if (comp == 2) cj = j;
+			else ck = k;
+			slot = 2 * comp - 1;
+		}
+		if (ci >= L.nx || cj >= L.ny || ck >= L.nz) continue;
+		long nc = (long)ck * sz + (long)ci * sx + cj;
+		x[comp * N + n] += wv * D[slot * N + nc];
+	}
+}
+
 template <typename T> struct CoarsestArgs {
 	LevelView<T> v;
 	T *x;

@@ -112,6 +112,41 @@ def test_solve_coarse_direct_matches_reference_converged(gpu, monkeypatch):
     S.close()
 
 
+def test_scgs_smoother_converges_to_reference(gpu, monkeypatch):
+    """The coupled cell smoother (I3G_SCGS=1, on every level, damping 0.5)
+    inside the mixed V-cycle: the 33^3 subduction solve converges to the
+    reference's direct solve (g3) within 1e-6 per field, and reaches 1e-8 in
+    fewer V-cycles than the colour phases (97 against 654 measured)."""
+    g = golden("g3_converged_c2_33.npz")
+    n = 33
+    cfg = I.make_config(2, n, n, n, seed=int(g["seed"]))
+    z = np.zeros(n ** 3)
+    counts = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("I3G_SCGS", mode)
+        monkeypatch.setenv("I3G_SCGS_LEVELS", "64")
+        monkeypatch.setenv("I3G_SCGS_W", "0.5,0.5")
+        S = I.Stokes(cfg)
+        S.set_coefficients()
+        S.set_solution(z, z, z, z)
+        st = S.solve(precision=32, rtol=1e-8, max_outer=40, max_vcycles=6000)
+        counts[mode] = st.vcycles
+        print(f"case: SCGS={mode} n={n}: {st.vcycles} V-cycles to rel {st.rel_res:.2e}", flush=True)
+        if mode == "1":
+            S.set_solution(z, z, z, z)
+            st = S.solve(precision=32, rtol=1e-12, max_outer=40, max_vcycles=6000)
+            assert st.converged and st.rel_res <= 1e-12
+            sol, ex = g["sol"], g["exists"]
+            for c, f in enumerate(S.get_solution()):
+                e = ex[c::4]
+                ref = sol[c::4][e]
+                err = np.linalg.norm(f[e] - ref) / np.linalg.norm(ref)
+                print(f"  field {c}: rel L2 err {err:.3e}", flush=True)
+                assert err <= 1e-6, (c, err)
+        S.close()
+    assert counts["1"] < counts["0"], counts
+
+
 def test_combine_start_recovers_solution(gpu):
     """i3g_stokes_combine_start: with the converged solution and a random
     vector as directions from a zero base, the residual-optimal coefficients
