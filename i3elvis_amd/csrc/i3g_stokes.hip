@@ -335,13 +335,21 @@ struct i3g_ctx {
 	 * below cd_tol times the largest dropped (I3G_COARSE_DIRECT_TOL) */
 	int coarse_direct = 0;
 	double cd_tol = 1e-10;
-	/* the coupled cell smoother (k_scgs_cell / k_scgs_apply) instead of the
-	 * colour phases on levels scgs_minlev..scgs_maxlev (I3G_SCGS=1,
-	 * I3G_SCGS_LEVELS "last" or "first,last", damping I3G_SCGS_W
-	 * "velocity,pressure"); the defaults are the best measured on the 65^3
-	 * subduction layout */
-	int scgs = 0, scgs_minlev = 0, scgs_maxlev = 1;
-	double scgs_wv = 0.5, scgs_wp = 0.5;
+	/* The coupled cell smoother (k_scgs_cell / k_scgs_apply) on levels
+	 * scgs_minlev..scgs_maxlev (I3G_SCGS=0 off; I3G_SCGS_LEVELS "last" or
+	 * "first,last"; damping I3G_SCGS_W "velocity,pressure").  scgs_mix > 0
+	 * (I3G_SCGS_MIX): at most scgs_phases colour-phase sweeps
+	 * (I3G_SCGS_PHASES) and then scgs_mix coupled sweeps per smoothing;
+	 * 0: coupled sweeps only.  Guard (I3G_SCGS_GUARD, V-cycles; 0 off):
+	 * when a GCR pass's residual falls by less than half over that many
+	 * V-cycles, the coupled sweeps are switched off for the rest of the
+	 * context's life (scgs_off) and the levels get their full colour-phase
+	 * sweeps.  The defaults are the fastest measured on the 257^3
+	 * subduction layout. */
+	int scgs = 1, scgs_minlev = 0, scgs_maxlev = 0;
+	double scgs_wv = 0.4, scgs_wp = 0.5;
+	int scgs_mix = 1, scgs_phases = 6, scgs_guard = 100;
+	int scgs_off = 0;
 	int cd_n = 0;                /* unknowns (4 N of the coarsest level) */
 	double *cd_u = nullptr;      /* U, column-major n x n */
 	double *cd_vt = nullptr;     /* V^T, column-major n x n */
@@ -877,6 +885,12 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 			if (sscanf(e, "%d,%d", &a, &b) == 2) { c->scgs_minlev = a; c->scgs_maxlev = b; }
 			else c->scgs_maxlev = atoi(e);
 		}
+		e = getenv("I3G_SCGS_MIX");
+		if (e) c->scgs_mix = atoi(e);
+		e = getenv("I3G_SCGS_PHASES");
+		if (e) c->scgs_phases = atoi(e);
+		e = getenv("I3G_SCGS_GUARD");
+		if (e) c->scgs_guard = atoi(e);
 		e = getenv("I3G_SCGS_W");
 		if (e) {
 			double a = 0, b = 0;
@@ -1455,15 +1469,29 @@ template <typename T> static void scgs_sweep(i3g_ctx *c, int l, T *x, const T *b
 
 /* ns sweeps on the level's unknowns: pairs of out-of-place sweeps through
  * the second vector (so the result lands back in x), an odd one in place */
+template <typename T> static void smooth_phases(i3g_ctx *c, int l, int ns, const i3g_stokes_params *p);
+
 template <typename T> static void smooth(i3g_ctx *c, int l, int ns, const i3g_stokes_params *p)
+{
+	Level &L = c->L[l];
+	if (!L.scgs || c->scgs_off) {
+		smooth_phases<T>(c, l, ns, p);
+		return;
+	}
+	T *x = lx<T>(L), *b = lb<T>(L);
+	if (c->scgs_mix > 0) {
+		smooth_phases<T>(c, l, c->scgs_phases > 0 && c->scgs_phases < ns ? c->scgs_phases : ns, p);
+		for (int s = 0; s < c->scgs_mix; s++) scgs_sweep<T>(c, l, x, b);
+		return;
+	}
+	for (int s = 0; s < ns; s++) scgs_sweep<T>(c, l, x, b);
+}
+
+template <typename T> static void smooth_phases(i3g_ctx *c, int l, int ns, const i3g_stokes_params *p)
 {
 	Level &L = c->L[l];
 	T *x = lx<T>(L), *xb = lxb<T>(L), *b = lb<T>(L);
 	int s = 0;
-	if (L.scgs) {
-		for (; s < ns; s++) scgs_sweep<T>(c, l, x, b);
-		return;
-	}
 	if constexpr (sizeof(T) == 4) {
 		Sw7Geom G;
 		Sw3Geom G3;
@@ -1557,7 +1585,7 @@ template <typename T> static bool coarsest_launch(i3g_ctx *c, const i3g_stokes_p
 template <typename T> static void coarse_cycle_graph(i3g_ctx *c, const i3g_stokes_params *p)
 {
 	int q = sizeof(T) == 8 ? 1 : 0;
-	std::vector<double> key = {p->p0koef, p->p1koef, p->v0koef, p->v1koef, (double)p->nu_coarse};
+	std::vector<double> key = {p->p0koef, p->p1koef, p->v0koef, p->v1koef, (double)p->nu_coarse, (double)c->scgs_off};
 	for (int l = 1; l < c->nlev; l++) { key.push_back(p->nu_pre[l]); key.push_back(p->nu_post[l]); }
 	if (!c->cg_exec[q] || c->cg_key[q] != key) {
 		if (c->cg_exec[q]) { hipGraphExecDestroy(c->cg_exec[q]); c->cg_exec[q] = nullptr; }
@@ -2059,6 +2087,11 @@ static int gcr(i3g_ctx *c, GcrHost &G, R *r, R *e, double target, int max_new, c
 	}
 	int used = 0;
 	double rn = sqrt(dot(c, r, r, n));
+	/* the coupled smoother's guard: residual and V-cycles at the start of
+	 * the current window */
+	bool guard = c->L[0].scgs && !c->scgs_off && c->scgs_guard > 0;
+	double rn_w = rn;
+	int used_w = 0;
 	while (rn > target * r0 && used < max_new) {
 		if (k == m) {
 			/* storage full: fold the directions into e and start afresh */
@@ -2126,6 +2159,20 @@ static int gcr(i3g_ctx *c, GcrHost &G, R *r, R *e, double target, int max_new, c
 		if (p->verbose > 1 || (p->verbose && used % 20 == 0))
 			fprintf(stderr, "  gcr %d (v-cycles %d): rel %.3e\n", k, used, rn / r0);
 		if (!std::isfinite(rn)) break;
+		if (guard && used - used_w >= c->scgs_guard) {
+			if (rn > 0.5 * rn_w) {
+				/* the coupled sweeps stall this operator: the colour
+				 * phases from here on (GCR is flexible, the kept
+				 * directions stay valid) */
+				c->scgs_off = 1;
+				guard = false;
+				if (p->verbose)
+					fprintf(stderr, "  gcr: coupled smoother off after %d V-cycles (rel %.3e over the last %d)\n",
+						used, rn / rn_w, used - used_w);
+			}
+			rn_w = rn;
+			used_w = used;
+		}
 	}
 	gcr_flush<T, R>(c, G, k, e);
 	c->kr_k = k;
@@ -2456,7 +2503,7 @@ static int smooth_impl(i3g_ctx *c, int level, int nsweeps, int variant, double *
 		c->fused = 0;
 		c->slab = 0;
 	}
-	smooth<T>(c, level, nsweeps, p);
+	smooth_phases<T>(c, level, nsweeps, p);
 	c->fused = keep;
 	c->fused_minz = keepz;
 	c->slab = keeps;

@@ -113,10 +113,11 @@ def test_solve_coarse_direct_matches_reference_converged(gpu, monkeypatch):
 
 
 def test_scgs_smoother_converges_to_reference(gpu, monkeypatch):
-    """The coupled cell smoother (I3G_SCGS=1, on every level, damping 0.5)
-    inside the mixed V-cycle: the 33^3 subduction solve converges to the
-    reference's direct solve (g3) within 1e-6 per field, and reaches 1e-8 in
-    fewer V-cycles than the colour phases (97 against 654 measured)."""
+    """The coupled cell smoother alone (I3G_SCGS=1 on every level, damping
+    0.5, no colour phases, no guard) inside the mixed V-cycle: the 33^3
+    subduction solve converges to the reference's direct solve (g3) within
+    1e-6 per field, and reaches 1e-8 in fewer V-cycles than the colour
+    phases alone (I3G_SCGS=0; 97 against 654 measured)."""
     g = golden("g3_converged_c2_33.npz")
     n = 33
     cfg = I.make_config(2, n, n, n, seed=int(g["seed"]))
@@ -126,6 +127,8 @@ def test_scgs_smoother_converges_to_reference(gpu, monkeypatch):
         monkeypatch.setenv("I3G_SCGS", mode)
         monkeypatch.setenv("I3G_SCGS_LEVELS", "64")
         monkeypatch.setenv("I3G_SCGS_W", "0.5,0.5")
+        monkeypatch.setenv("I3G_SCGS_MIX", "0")
+        monkeypatch.setenv("I3G_SCGS_GUARD", "0")
         S = I.Stokes(cfg)
         S.set_coefficients()
         S.set_solution(z, z, z, z)
@@ -145,6 +148,42 @@ def test_scgs_smoother_converges_to_reference(gpu, monkeypatch):
                 assert err <= 1e-6, (c, err)
         S.close()
     assert counts["1"] < counts["0"], counts
+
+
+@pytest.mark.parametrize("guard", ["100", "1"])
+def test_default_smoother_with_guard_converges_to_reference(gpu, monkeypatch, guard):
+    """The default V-cycle smoother (level 0: at most six colour-phase
+    sweeps, then one coupled cell sweep; damping 0.4/0.5) with the stall
+    guard at its default window (100 V-cycles) and at its tightest (1: the
+    coupled sweeps are switched off at the first GCR direction that does
+    not halve the residual, and the flexible GCR carries on with the colour
+    phases): the 33^3 subduction solve converges to the reference's direct
+    solve (g3) within 1e-6 per field either way."""
+    for k in ("I3G_SCGS", "I3G_SCGS_LEVELS", "I3G_SCGS_W", "I3G_SCGS_MIX", "I3G_SCGS_PHASES"):
+        monkeypatch.delenv(k, raising=False)
+    monkeypatch.setenv("I3G_SCGS_GUARD", guard)
+    g = golden("g3_converged_c2_33.npz")
+    n = 33
+    cfg = I.make_config(2, n, n, n, seed=int(g["seed"]))
+    S = I.Stokes(cfg)
+    S.set_coefficients()
+    z = np.zeros(n ** 3)
+    S.set_solution(z, z, z, z)
+    S.set_timing(True)
+    st = S.solve(precision=32, rtol=1e-12, max_outer=40, max_vcycles=6000)
+    calls = S.kernel_stats().get("scgs_L0", {}).get("calls", 0)
+    print(f"case: default smoother, guard {guard}: {st.vcycles} V-cycles, {calls} coupled sweeps, "
+          f"rel {st.rel_res:.2e}", flush=True)
+    assert st.converged and st.rel_res <= 1e-12
+    assert 0 < calls <= 2 * st.vcycles
+    sol, ex = g["sol"], g["exists"]
+    for c, f in enumerate(S.get_solution()):
+        e = ex[c::4]
+        ref = sol[c::4][e]
+        err = np.linalg.norm(f[e] - ref) / np.linalg.norm(ref)
+        print(f"  field {c}: rel L2 err {err:.3e}", flush=True)
+        assert err <= 1e-6, (c, err)
+    S.close()
 
 
 def test_combine_start_recovers_solution(gpu):
