@@ -341,14 +341,15 @@ struct i3g_ctx {
 	 * (I3G_SCGS_MIX): at most scgs_phases colour-phase sweeps
 	 * (I3G_SCGS_PHASES) and then scgs_mix coupled sweeps per smoothing;
 	 * 0: coupled sweeps only.  Guard (I3G_SCGS_GUARD, V-cycles; 0 off):
-	 * when a GCR pass's residual falls by less than half over that many
-	 * V-cycles, the coupled sweeps are switched off for the rest of the
-	 * context's life (scgs_off) and the levels get their full colour-phase
-	 * sweeps.  The defaults are the fastest measured on the 257^3
+	 * when a GCR pass's residual falls by less than 10% (to more than
+	 * scgs_guard_ratio of where it was) over that many V-cycles, the
+	 * coupled sweeps are switched off for the rest of the context's life
+	 * (scgs_off) and the levels get their full colour-phase sweeps.  The defaults are the fastest measured on the 257^3
 	 * subduction layout. */
 	int scgs = 1, scgs_minlev = 0, scgs_maxlev = 0;
 	double scgs_wv = 0.4, scgs_wp = 0.5;
 	int scgs_mix = 1, scgs_phases = 6, scgs_guard = 100;
+	double scgs_guard_ratio = 0.9; /* I3G_SCGS_GUARD_RATIO */
 	int scgs_off = 0;
 	int cd_n = 0;                /* unknowns (4 N of the coarsest level) */
 	double *cd_u = nullptr;      /* U, column-major n x n */
@@ -891,6 +892,8 @@ extern "C" int i3g_create(const i3g_setup *s, i3g_ctx **out)
 		if (e) c->scgs_phases = atoi(e);
 		e = getenv("I3G_SCGS_GUARD");
 		if (e) c->scgs_guard = atoi(e);
+		e = getenv("I3G_SCGS_GUARD_RATIO");
+		if (e && atof(e) > 0) c->scgs_guard_ratio = atof(e);
 		e = getenv("I3G_SCGS_W");
 		if (e) {
 			double a = 0, b = 0;
@@ -1456,10 +1459,11 @@ template <typename T> static void scgs_sweep(i3g_ctx *c, int l, T *x, const T *b
 	Level &L = c->L[l];
 	LevelView<T> v = view<T>(L);
 	dim3 g = plane_grid(L.nx, L.ny, L.nz);
+	dim3 gh((unsigned)(((long)L.nx * ((L.ny + 1) / 2) + 255) / 256), (unsigned)L.nz);
 	{
 		Timed tm(c, l == 0 ? "scgs_L0" : "scgs_coarse", 0);
 		for (int col = 0; col < 2; col++) {
-			hipLaunchKernelGGL(k_scgs_cell<T>, g, dim3(256), 0, c->s, v, (const T *)x, b, (T *)L.scgs, col);
+			hipLaunchKernelGGL(k_scgs_cell<T>, gh, dim3(256), 0, c->s, v, (const T *)x, b, (T *)L.scgs, col);
 			hipLaunchKernelGGL(k_scgs_apply<T>, g, dim3(256), 0, c->s, v, x, (const T *)L.scgs, col,
 					   (T)c->scgs_wv, (T)c->scgs_wp);
 		}
@@ -2160,7 +2164,7 @@ static int gcr(i3g_ctx *c, GcrHost &G, R *r, R *e, double target, int max_new, c
 			fprintf(stderr, "  gcr %d (v-cycles %d): rel %.3e\n", k, used, rn / r0);
 		if (!std::isfinite(rn)) break;
 		if (guard && used - used_w >= c->scgs_guard) {
-			if (rn > 0.5 * rn_w) {
+			if (rn > c->scgs_guard_ratio * rn_w) {
 				/* the coupled sweeps stall this operator: the colour
 				 * phases from here on (GCR is flexible, the kept
 				 * directions stay valid) */

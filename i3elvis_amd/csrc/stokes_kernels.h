@@ -1452,10 +1452,14 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_scgs_cell(LevelView<T> L, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ D, int colour)
 {
+	/* one thread per cell of the colour: (i, k) and every other j */
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	int k = blockIdx.y;
-	int i = t / L.ny, j = t - i * L.ny;
-	if (i >= L.nx || ((i + j + k) & 1) != colour) return;
+	int hy = (L.ny + 1) >> 1;
+	int i = t / hy, tj = t - i * hy;
+	if (i >= L.nx) return;
+	int j = 2 * tj + ((i + k + colour) & 1);
+	if (j >= L.ny) return;
 	long N = L.N, sx = L.ny, sz = (long)L.nx * L.ny;
 	long n = (long)k * sz + (long)i * sx + j;
 	T dp = 0, du[6] = {0, 0, 0, 0, 0, 0};

@@ -211,7 +211,8 @@ typedef struct {
  * 0).  A stall guard (I3G_SCGS_GUARD V-cycles, default 100, 0 off) switches
  * the coupled sweeps off for the rest of the context's life, and gives the
  * levels their full colour-phase sweeps, when a GCR pass's residual falls
- * by less than half over that window.  I3G_SCGS=0: colour phases only;
+ * to more than I3G_SCGS_GUARD_RATIO (default 0.9) of where it was over
+ * that window.  I3G_SCGS=0: colour phases only;
  * I3G_SCGS_MIX=0: coupled sweeps only.  257^3 subduction: 10.7 s (358
  * V-cycles) from a zero start against 16.1 s (366) with the phases only. */
 int i3g_create(const i3g_setup *s, i3g_ctx **out);

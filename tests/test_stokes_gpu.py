@@ -150,18 +150,23 @@ def test_scgs_smoother_converges_to_reference(gpu, monkeypatch):
     assert counts["1"] < counts["0"], counts
 
 
-@pytest.mark.parametrize("guard", ["100", "1"])
-def test_default_smoother_with_guard_converges_to_reference(gpu, monkeypatch, guard):
+@pytest.mark.parametrize("forced", [False, True])
+def test_default_smoother_with_guard_converges_to_reference(gpu, monkeypatch, forced):
     """The default V-cycle smoother (level 0: at most six colour-phase
     sweeps, then one coupled cell sweep; damping 0.4/0.5) with the stall
-    guard at its default window (100 V-cycles) and at its tightest (1: the
-    coupled sweeps are switched off at the first GCR direction that does
-    not halve the residual, and the flexible GCR carries on with the colour
-    phases): the 33^3 subduction solve converges to the reference's direct
-    solve (g3) within 1e-6 per field either way."""
-    for k in ("I3G_SCGS", "I3G_SCGS_LEVELS", "I3G_SCGS_W", "I3G_SCGS_MIX", "I3G_SCGS_PHASES"):
+    guard as it is (window 100 V-cycles, ratio 0.9: it does not fire here)
+    and forced to fire (window 1, ratio 1e-3: the coupled sweeps are
+    switched off after the first GCR direction, and the flexible GCR
+    carries on with the full colour phases): the 33^3 subduction solve
+    converges to the reference's direct solve (g3) within 1e-6 per field
+    either way."""
+    for k in ("I3G_SCGS", "I3G_SCGS_LEVELS", "I3G_SCGS_W", "I3G_SCGS_MIX", "I3G_SCGS_PHASES", "I3G_SCGS_GUARD",
+              "I3G_SCGS_GUARD_RATIO"):
         monkeypatch.delenv(k, raising=False)
-    monkeypatch.setenv("I3G_SCGS_GUARD", guard)
+    if forced:
+        monkeypatch.setenv("I3G_SCGS_GUARD", "1")
+        monkeypatch.setenv("I3G_SCGS_GUARD_RATIO", "0.001")
+    guard = "forced" if forced else "default"
     g = golden("g3_converged_c2_33.npz")
     n = 33
     cfg = I.make_config(2, n, n, n, seed=int(g["seed"]))
@@ -175,7 +180,10 @@ def test_default_smoother_with_guard_converges_to_reference(gpu, monkeypatch, gu
     print(f"case: default smoother, guard {guard}: {st.vcycles} V-cycles, {calls} coupled sweeps, "
           f"rel {st.rel_res:.2e}", flush=True)
     assert st.converged and st.rel_res <= 1e-12
-    assert 0 < calls <= 2 * st.vcycles
+    if forced:
+        assert 0 < calls < 2 * st.vcycles  # switched off part-way
+    else:
+        assert calls == 2 * st.vcycles  # one coupled sweep before and after the coarse correction
     sol, ex = g["sol"], g["exists"]
     for c, f in enumerate(S.get_solution()):
         e = ex[c::4]
