@@ -561,7 +561,11 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic (seeded subduction layout, 10^6 viscosity contrast)",
         "config": {"workload": f"stokes_{n}^3_subduction", "grid": [n, n, n], "levels": cfg["multinum"] + 1,
-                   "rtol": 1e-8, "krylov": a.krylov},
+                   "rtol": 1e-8, "krylov": a.krylov,
+                   "smoother": ("level 0: at most 6 colour-phase sweeps, then 1 coupled cell sweep (damping 0.4/0.5, "
+                                "stall guard); levels 1-4: colour phases 8/8/32/32 before and after; coarsest: 128 "
+                                "FP64 sweeps" if os.environ.get("I3G_SCGS", "1") != "0" else
+                                "colour phases 16/8/8/32/32 before and after; coarsest: 128 FP64 sweeps")},
         "start": chosen, "warmup_paths": warm,
         "vcycles": int(np.mean(vcyc)), "vcycles_per_s": round(float(np.mean(vcyc)) / mixed_t, 2),
         "solves_per_s": round(1.0 / mixed_t, 4),
