@@ -1458,14 +1458,21 @@ template <typename T> static void scgs_sweep(i3g_ctx *c, int l, T *x, const T *b
 {
 	Level &L = c->L[l];
 	LevelView<T> v = view<T>(L);
-	dim3 g = plane_grid(L.nx, L.ny, L.nz);
-	dim3 gh((unsigned)(((long)L.nx * ((L.ny + 1) / 2) + 255) / 256), (unsigned)L.nz);
+	/* blocks over (chunk, plane), remapped so that each XCD takes a
+	 * contiguous run of planes (block_work) on level 0 */
+	int remap = c->xcd_remap && l == 0;
+	int ch_c = (int)(((long)L.nx * ((L.ny + 1) / 2) + 255) / 256), ch_a = (int)(((long)L.nx * L.ny + 255) / 256);
+	auto grid_of = [&](int chunks) {
+		return remap ? dim3((unsigned)((((long)chunks * L.nz + 7) / 8) * 8)) : dim3((unsigned)chunks, (unsigned)L.nz);
+	};
+	dim3 gc = grid_of(ch_c), ga = grid_of(ch_a);
 	{
 		Timed tm(c, l == 0 ? "scgs_L0" : "scgs_coarse", 0);
 		for (int col = 0; col < 2; col++) {
-			hipLaunchKernelGGL(k_scgs_cell<T>, gh, dim3(256), 0, c->s, v, (const T *)x, b, (T *)L.scgs, col);
-			hipLaunchKernelGGL(k_scgs_apply<T>, g, dim3(256), 0, c->s, v, x, (const T *)L.scgs, col,
-					   (T)c->scgs_wv, (T)c->scgs_wp);
+			hipLaunchKernelGGL(k_scgs_cell<T>, gc, dim3(256), 0, c->s, v, (const T *)x, b, (T *)L.scgs, col, remap,
+					   ch_c);
+			hipLaunchKernelGGL(k_scgs_apply<T>, ga, dim3(256), 0, c->s, v, x, (const T *)L.scgs, col,
+					   (T)c->scgs_wv, (T)c->scgs_wp, remap, ch_a);
 		}
 	}
 	bc_enforce<T>(c, L, x, 1);

@@ -1450,11 +1450,14 @@ k_prolong(LevelView<T> F, LevelView<T> Cl, XferView<T> M, const T *__restrict__ 
  * damped, each face from its one cell of that colour. */
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_scgs_cell(LevelView<T> L, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ D, int colour)
+k_scgs_cell(LevelView<T> L, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ D, int colour,
+	    int remap, int chunks)
 {
-	/* one thread per cell of the colour: (i, k) and every other j */
-	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	int k = blockIdx.y;
+	/* one thread per cell of the colour: (i, k) and every other j; blocks
+	 * over (chunk, plane) as block_work maps them */
+	int ch, k;
+	if (!block_work(remap, chunks, L.nz, 0, &ch, &k)) return;
+	int t = ch * blockDim.x + threadIdx.x;
 	int hy = (L.ny + 1) >> 1;
 	int i = t / hy, tj = t - i * hy;
 	if (i >= L.nx) return;
@@ -1500,10 +1503,12 @@ k_scgs_cell(LevelView<T> L, const T *__restrict__ x, const T *__restrict__ b, T 
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_scgs_apply(LevelView<T> L, T *__restrict__ x, const T *__restrict__ D, int colour, T wv, T wp)
+k_scgs_apply(LevelView<T> L, T *__restrict__ x, const T *__restrict__ D, int colour, T wv, T wp, int remap,
+	     int chunks)
 {
-	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	int k = blockIdx.y;
+	int ch, k;
+	if (!block_work(remap, chunks, L.nz, 0, &ch, &k)) return;
+	int t = ch * blockDim.x + threadIdx.x;
 	int i = t / L.ny, j = t - i * L.ny;
 	if (i >= L.nx) return;
 	long N = L.N, sx = L.ny, sz = (long)L.nx * L.ny;
